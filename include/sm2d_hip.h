@@ -1,0 +1,218 @@
+/* sm2d_hip.h — C ABI of the MI355X transport-step library (libsm2d_hip.so).
+ *
+ * The library replaces the K1–K6 transport step of SM_2D behind its boundary
+ * `sm_2d::run_k1k6_pipeline_transport` (src/include/cuda/gpu_transport_wrapper.hpp:78-90,
+ * implementation src/cuda/gpu_transport_wrapper.cu:207-641) and the LUT upload
+ * `init_device_lut` (gpu_transport_wrapper.hpp:42, .cu:199-205).  Signatures use plain
+ * pointers, sizes and versioned POD structs; no C++ or torch types cross the ABI.
+ *
+ * Conventions
+ *  - Every entry point returns an int status (SM2D_OK = 0, negative on error) and never
+ *    throws; sm2d_last_error() returns a thread-local message for the last failure.
+ *  - Every struct starts with `struct_size`; callers set it to sizeof(the struct).
+ *  - A context owns one HIP stream and all device memory it allocates; it is reusable across
+ *    spots (no per-spot GB-scale allocation) and re-entrant (no device globals).
+ *  - Pointers named d_* are device pointers; all others are host pointers.
+ */
+#ifndef SM2D_HIP_H
+#define SM2D_HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SM2D_ABI_VERSION 1
+
+#define SM2D_OK 0
+#define SM2D_ERR_ARG -1        /* invalid argument / struct_size mismatch */
+#define SM2D_ERR_HIP -2        /* HIP runtime error */
+#define SM2D_ERR_CAPACITY -3   /* a bounded device pool overflowed */
+#define SM2D_ERR_AUDIT -4      /* K5 failed and fail_fast_on_audit was set */
+#define SM2D_ERR_UNSUPPORTED -5 /* input needs a feature the GPU path does not implement */
+
+#define SM2D_KB 32                 /* slots per cell, DEVICE_Kb (src/cuda/device/device_psic.cuh:30) */
+#define SM2D_EMPTY 0xFFFFFFFFu     /* EMPTY_BLOCK_ID (src/include/core/block_encoding.hpp:30) */
+
+/* Spatial grid, phase-space grid and local-bin layout of one transport problem.
+ * The reference fixes the layout at compile time (src/include/core/local_bins.hpp:19-23:
+ * 4 theta x 2 E x 8 x_sub x 4 z_sub = 256); here it is a runtime choice among the
+ * layouts the library is built for: (4,2,8,4) and (8,4,4,1). */
+typedef struct sm2d_grid {
+    uint32_t struct_size;
+    int Nx, Nz;                 /* cells in x (lateral) and z (depth) */
+    float dx, dz;               /* cell size [mm] */
+    float x_min, z_min;         /* grid origin [mm]; the runner uses -Nx*dx/2 and 0 */
+    int N_theta;                /* global angular bins, uniform over [theta_min, theta_max] */
+    float theta_min, theta_max; /* [rad]; the reference runner hard-codes +-0.35 */
+    int N_E;                    /* global energy bins; edges are supplied separately */
+    int n_theta_local, n_e_local, n_x_sub, n_z_sub;  /* local-bin layout of one block */
+} sm2d_grid;
+
+/* TransportConfig fields the step reads (src/include/core/incident_particle_config.hpp:174-200). */
+typedef struct sm2d_transport {
+    uint32_t struct_size;
+    float E_fine_on, E_fine_off;  /* K1 thresholds [MeV] (defaults 10, 11) */
+    float weight_active_min;      /* K1 weight threshold (1e-12) */
+    float step_coarse;            /* K2 step [mm] (5) */
+    int max_iterations;
+    int fail_fast_on_audit;
+    int step_rule;                /* 0: device rule of device_lut.cuh:168-214 (default);
+                                     1: 2 % of range rule of step_control.hpp:18-66 */
+    int enable_straggling;        /* K3 Vavilov straggling (pipeline default 1) */
+    int enable_nuclear;           /* K3 nuclear attenuation (pipeline default 1) */
+    float sigma_x_initial;        /* K1K6PipelineConfig::sigma_x_initial [mm] used by the coarse
+                                     moments; sm2d_run_spot replaces it with the spot's sigma_x */
+} sm2d_transport;
+
+/* One pencil-beam spot: the beam arguments of run_k1k6_pipeline_transport. */
+typedef struct sm2d_spot {
+    uint32_t struct_size;
+    float x0, z0, theta0, E0, W_total;
+    float sigma_x, sigma_theta, sigma_E;
+    int n_samples;
+    uint32_t random_seed;
+} sm2d_spot;
+
+/* Energy and weight bookkeeping of one spot (whole run).  Mirrors the quantities of the
+ * reference's "Energy Conservation Report" (src/cuda/k1k6_pipeline.cu:2101-2183) plus the
+ * per-source-cell closure and the work counter this library adds. */
+typedef struct sm2d_audit {
+    uint32_t struct_size;
+    int iterations;        /* iterations executed (the terminating empty one included) */
+    int terminated;        /* 1 when both lists were empty before max_iterations */
+    int W_fail_iters, E_fail_iters;   /* iterations whose K5 W / E check failed */
+    double max_W_error, max_E_error;  /* largest K5 relative errors over iterations */
+    double W_source_injected, W_source_out_of_grid, W_source_slot_drop;
+    double E_source_injected, E_source_out_of_grid, E_source_slot_drop, E_representation_loss;
+    double E_dep, W_cutoff, E_cutoff, W_nuclear, E_nuclear, W_boundary, E_boundary;
+    double W_transport_drop, E_transport_drop, E_audit_residual;
+    double W_leftover, E_leftover;    /* phase space left when max_iterations was reached */
+    double max_cell_W_closure;        /* max over (iteration, processed cell) of
+                                         |w_in - w_out_total| / w_in, FP64 */
+    double max_cell_E_closure;        /* same for energy (K3 cells only; K2 has no closure) */
+    uint64_t cell_bin_updates;        /* sum over iterations and processed cells of
+                                         occupied slots x local bins */
+    uint64_t processed_cells, processed_components;
+    uint64_t unsupported_events;      /* lateral emissions into another listed cell's bucket */
+} sm2d_audit;
+
+/* Per-iteration K5 report: AuditReport of src/cuda/kernels/k5_audit.cuh:5-28, with the weight
+ * sums kept in FP64 and reduced in a fixed order. */
+typedef struct sm2d_k5_report {
+    uint32_t struct_size;
+    int W_pass, E_pass;
+    double W_error, E_error;
+    double W_in, W_out, W_cutoff, W_nuclear, W_boundary, W_source_out_of_grid,
+           W_source_slot_drop, W_transport_drop;
+    double E_in, E_out, E_dep, E_cutoff, E_nuclear, E_boundary, E_source_out_of_grid,
+           E_source_slot_drop, E_transport_drop;
+} sm2d_k5_report;
+
+typedef struct sm2d_ctx sm2d_ctx;
+
+const char* sm2d_last_error(void);
+int sm2d_abi_version(void);
+
+/* --- host-side table builders (run on the CPU, identical to the reference's host code) --- */
+
+/* EnergyGrid(E_min, E_max, N_E) log-spaced constructor (src/core/grids.cpp:6-23).
+ * edges[N_E+1], rep[N_E]. */
+int sm2d_energy_grid_log(float E_min, float E_max, int N_E, float* edges, float* rep);
+/* EnergyGrid::CreatePiecewise (src/core/grids.cpp:33-96).  groups[3*n_groups] = (start, end,
+ * width) triples.  Returns N_E in *n_bins; with edges/rep NULL only counts. */
+int sm2d_energy_grid_piecewise(const float* groups, int n_groups, float* edges, float* rep,
+                               int* n_bins);
+/* AngularGrid(theta_min, theta_max, N_theta) edges (src/core/grids.cpp:121-135). */
+int sm2d_angular_grid(float theta_min, float theta_max, int N_theta, float* edges);
+/* GenerateRLUT(const EnergyGrid&) (src/lut/r_lut.cpp:69-129) from the NIST PSTAR water table.
+ * Outputs R, S, log_E, log_R, log_S, each [N_E]. */
+int sm2d_build_rlut(const float* rep, int N_E, float* R, float* S, float* log_E, float* log_R,
+                    float* log_S);
+/* Fermi–Eyges precompute of init_pipeline_state (src/cuda/k1k6_pipeline.cu:1061-1212):
+ * A[Nz], B[Nz], sigma_total[Nz]. */
+int sm2d_fe_precompute(float E0, float sigma_x0, int Nz, float dz, float* A, float* B,
+                       float* sigma_total);
+/* MCS post-convolution of edep rows (src/cuda/gpu_transport_wrapper.cu:551-616), in place. */
+int sm2d_post_convolve(double* edep, int Nx, int Nz, float dx, const float* sigma_total,
+                       float sigma_x0);
+
+/* --- context ---------------------------------------------------------------------------- */
+
+/* Creates a context on HIP device `device` for problems of the given grid.
+ * Replaces device_psic_init x2 + init_pipeline_state (gpu_transport_wrapper.cu:282-312);
+ * the context persists across spots. */
+int sm2d_ctx_create(int device, const sm2d_grid* grid, sm2d_ctx** out);
+int sm2d_ctx_destroy(sm2d_ctx* ctx);
+
+/* Uploads the phase-space edges and the R-LUT.  Replaces init_device_lut
+ * (gpu_transport_wrapper.cu:199-205 -> DeviceLUTWrapper::init, k3_finetransport.cuh:115-149)
+ * and the edge copies of init_pipeline_state (k1k6_pipeline.cu:1012-1048). */
+int sm2d_lut_upload(sm2d_ctx* ctx, const float* theta_edges, const float* E_edges,
+                    const float* R, const float* S, const float* log_E, const float* log_R,
+                    const float* log_S);
+
+/* Uploads per-depth-row Fermi–Eyges moments A[Nz], B[Nz] used by the fine transport
+ * (k1k6_pipeline.cu:1195-1206).  Passing NULL for both sets them to zero. */
+int sm2d_fe_upload(sm2d_ctx* ctx, const float* A, const float* B);
+
+/* Per-row material (density rho [g/cm^3] and radiation-length ratio to water) for slab
+ * phantoms.  NULL restores water (rho = 1, ratio = 1), which reproduces the reference. */
+int sm2d_material_upload(sm2d_ctx* ctx, const float* rho, const float* x0_ratio);
+
+/* Full spot: injection, K1–K6 loop, EdepC extraction.  This is the body of
+ * run_k1k6_pipeline_transport (gpu_transport_wrapper.cu:207-641) without the host-side MCS
+ * post-convolution, which the caller applies with sm2d_post_convolve.
+ * edep_out[Nz*Nx] (row-major, FP64, MeV per cell); audit may be NULL. */
+int sm2d_run_spot(sm2d_ctx* ctx, const sm2d_spot* spot, const sm2d_transport* tp,
+                  double* edep_out, sm2d_audit* audit);
+
+/* --- stage-level entry points (used by the parity tests) -------------------------------- */
+
+/* Clears both phase-space buffers and all tallies. */
+int sm2d_reset(sm2d_ctx* ctx);
+/* Source injection only (inject_gaussian_source_kernel / inject_source_kernel,
+ * k1k6_pipeline.cu:300-551) into psi_in; accounting goes to *audit (may be NULL). */
+int sm2d_inject(sm2d_ctx* ctx, const sm2d_spot* spot, sm2d_audit* audit);
+/* Loads psi_in from rows (cell, slot, bid, lidx, weight); rows of one (cell, slot) must
+ * carry the same bid, slots of one cell must form a prefix 0..n-1. */
+int sm2d_psi_load(sm2d_ctx* ctx, int n_rows, const int32_t* cell, const int32_t* slot,
+                  const uint32_t* bid, const int32_t* lidx, const float* w);
+/* Exports psi_in: bids[Nx*Nz*32] (may be NULL) and the rows with weight != 0 in
+ * (cell, slot, lidx) order; *n_rows receives the row count (rows beyond capacity are
+ * counted but not written). */
+int sm2d_psi_dump(sm2d_ctx* ctx, uint32_t* bids, int capacity, int32_t* cell, int32_t* slot,
+                  uint32_t* bid, int32_t* lidx, float* w, int* n_rows);
+/* K1 ActiveMask + K2 CompactActive on psi_in (k1_activemask.cu:6-48,
+ * k1k6_pipeline.cu:589-665).  mask_prev[N] (may be NULL = all zero), mask_out[N],
+ * active/coarse lists in ascending cell order with their counts. */
+int sm2d_k1k2_mask_compact(sm2d_ctx* ctx, const sm2d_transport* tp, const uint8_t* mask_prev,
+                           uint8_t* mask_out, int32_t* active, int* n_active, int32_t* coarse,
+                           int* n_coarse);
+/* Runs `n_iter` full iterations (K1, compaction, coarse K3+K4, fine K3+K4, K5, K6) from the
+ * current psi_in; stops early when both lists are empty.  tallies_out (may be NULL) receives
+ * the cumulative per-cell tallies [7][N]: EdepC, W_cutoff, E_cutoff, W_nuclear, E_nuclear,
+ * W_boundary, E_boundary (weights widened to FP64).  k5_last receives the last iteration's
+ * K5 report. */
+int sm2d_run_iterations(sm2d_ctx* ctx, const sm2d_transport* tp, int n_iter,
+                        int include_source_terms, double* tallies_out, sm2d_k5_report* k5_last,
+                        sm2d_audit* audit);
+/* K5 on caller data (the k5_audit known-answer test, tests/gpu/test_k5_drop_channels_gpu.cu):
+ * audits psi_in against psi_out with per-cell tally deltas d_tallies[7][N] (host) and the
+ * drop/source scalars. */
+int sm2d_k5_audit(sm2d_ctx* ctx, const double* tallies_delta, double W_src_out, double W_src_slot,
+                  double E_src_out, double E_src_slot, double W_drop, double E_drop,
+                  int include_source_terms, sm2d_k5_report* out);
+/* Copies psi_in <-> psi_out (load the second buffer for sm2d_k5_audit). */
+int sm2d_swap(sm2d_ctx* ctx);
+
+/* --- self-test of the shared elementary functions on the device -------------------------- */
+/* Evaluates f(x[i]) for i < n on the GPU: which = 0 exp, 1 log, 2 sin, 3 cos, 4 erf. */
+int sm2d_math_eval(int which, const float* x, float* y, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SM2D_HIP_H */

@@ -1,0 +1,51 @@
+"""Python handle on the CPU oracle (oracle/_build/liboracle_sm2d.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "_build", "liboracle_sm2d.so")
+LIB_OMP = os.path.join(HERE, "_build", "liboracle_sm2d_omp.so")
+
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from sm_2d_amd import abi  # noqa: E402
+from sm_2d_amd.engine import Engine, Library  # noqa: E402
+
+
+def build(quiet: bool = True) -> None:
+    """Compiles the oracle (and oracle/_ref when the reference checkout is present)."""
+    out = subprocess.run(["make", "-C", HERE, "-j4"], capture_output=quiet, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + (out.stdout or "") + (out.stderr or ""))
+
+
+def load(omp: bool = False) -> Library:
+    path = LIB_OMP if omp else LIB
+    if not os.path.exists(path):
+        build()
+    return Library(path, "orc_", {
+        "eval": [C.c_void_p, C.c_int, abi.fp, abi.fp, abi.fp, C.c_int],
+        "set_threads": [C.c_void_p, C.c_int],
+    })
+
+
+class OracleEngine(Engine):
+    def eval(self, which: int, x, y=None):
+        import numpy as np
+        x = np.ascontiguousarray(x, np.float32)
+        yy = None if y is None else np.ascontiguousarray(y, np.float32)
+        out = np.zeros_like(x)
+        self.L.call("eval", self.h, which, x.ctypes.data_as(abi.fp),
+                    None if yy is None else yy.ctypes.data_as(abi.fp),
+                    out.ctypes.data_as(abi.fp), len(x))
+        return out

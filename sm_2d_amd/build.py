@@ -1,0 +1,52 @@
+"""Builds libsm2d_hip.so in-tree (hipcc for gfx950 + g++ for the host table builders).
+
+Run as `python -m sm_2d_amd.build`.  The library lands next to this file so it travels with
+the source tree; nothing is installed.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libsm2d_hip.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("SM2D_ARCH", "gfx950")
+
+# -ffp-contract=off on both compilers: the shared elementary functions and the per-bin physics
+# must round identically on the CPU oracle and on the GPU.  Packed FP32 instructions are
+# disabled for device code.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={ARCH}",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-result"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed: " + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    host_o = os.path.join(BUILD, "sm2d_host.o")
+    dev_o = os.path.join(BUILD, "sm2d_runtime.o")
+    steps = [
+        ["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, "sm2d_host.cpp"), "-o", host_o],
+        [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "sm2d_runtime.hip"), "-o", dev_o],
+        [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", host_o, dev_o, "-o", LIB + ".tmp"],
+    ]
+    for cmd in steps:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        _run(cmd)
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

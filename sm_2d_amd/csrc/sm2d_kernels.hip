@@ -1,0 +1,1164 @@
+// sm2d_kernels.hip — gfx950 kernels of the K1–K6 transport step.
+//
+// K1  k_mask         ActiveMask per occupied cell (src/cuda/kernels/k1_activemask.cu:6-48)
+// K2  k_scan/k_emit  CompactActive: per-block counts, one-workgroup exclusive scan, ordered
+//                    scatter with wave ballots -> ascending active / coarse lists
+//                    (replaces the atomic compaction of k1k6_pipeline.cu:589-665)
+// K3  k_transport    coarse (K2_CoarseTransport, k2_coarsetransport.cu:76-704) and fine
+//                    (K3_FineTransport, k3_finetransport.cu:94-760) per-bin transport: one
+//                    wavefront per listed cell; lanes evaluate the physics of up to 64 non-zero
+//                    bins at once, then the side effects are applied in bin order so slot
+//                    claims and FP32 accumulation order equal the reference's serial loop
+// K4  k_gather       BucketTransfer (k4_transfer.cu:42-201) as a per-destination gather in
+//                    fixed (face, bucket slot) order: one writer per destination cell
+// K5  k_audit_*      ConservationAudit (k5_audit.cu:22-167) as fixed-order FP64 reductions
+// K6  host           buffer swap; the next iteration resets only the cells that were occupied
+#include "sm2d_device.h"
+
+namespace sm2d {
+
+// ------------------------------------------------------------------------------------------
+// Utility kernels
+// ------------------------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Resets the slot tables of every occupied cell of a buffer and its payload pool.
+__global__ void k_clear(Psi p, int N) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+        uint32_t n = p.ns[c];
+        if (n) {
+            for (uint32_t s = 0; s < n; ++s) p.bid[(size_t)c * KB + s] = EMPTY;
+            p.ns[c] = 0;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p.pcount = 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// K1: mask + per-cell FP64 totals of psi_in.  One thread per cell, serial sum in slot->lidx
+// order so that the threshold test W > w_min sees the reference's exact FP32 sum.
+// flag: 1 = active (fine), 2 = coarse (mask 0 with weight), 0 = nothing.
+// ------------------------------------------------------------------------------------------
+template <class LY>
+__global__ __launch_bounds__(256) void k_mask(Tabs T, Tp P, Psi in, uint32_t* maskv, uint32_t ep,
+                                              uint8_t* flag, uint32_t* bcount, double* win, double* ein) {
+    __shared__ uint32_t s_cnt[2];
+    if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    uint8_t f = 0;
+    if (c < T.N) {
+        uint32_t n = in.ns[c];
+        double W64 = 0.0, E64 = 0.0;
+        if (n) {
+            float W = 0.0f;
+            bool on = false, off = false, has = false;
+            for (uint32_t s = 0; s < n; ++s) {
+                uint32_t bid = in.bid[(size_t)c * KB + s];
+                if (bid == EMPTY) continue;
+                uint32_t bE = (bid >> 12) & 0xFFF;
+                if (bE <= (uint32_t)P.b_on) on = true;
+                if (bE <= (uint32_t)P.b_off) off = true;
+                const float* v = in.pool + (size_t)in.sref[(size_t)c * KB + s] * LY::LB;
+                for (int l = 0; l < LY::LB; ++l) {
+                    float w = v[l];
+                    W += w;
+                    if (w > 0.0f) {
+                        has = true;
+                        W64 += (double)w;
+                        int eb = min(max((int)bE * LY::EL + LY::le(l), 0), T.NE - 1);
+                        E64 += (double)w * (double)ecenter(T.ee, eb);
+                    }
+                }
+            }
+            bool prev = maskv[c] == (((ep - 1u) << 1) | 1u);
+            bool m = (on || (prev && off)) && W > P.wmin;
+            maskv[c] = (ep << 1) | (m ? 1u : 0u);
+            f = m ? 1 : (has ? 2 : 0);
+        }
+        flag[c] = f;
+        win[c] = W64;
+        ein[c] = E64;
+    }
+    if (f) atomicAdd(&s_cnt[f - 1], 1u);  // block-local count; the order is irrelevant
+    __syncthreads();
+    if (threadIdx.x < 2) bcount[blockIdx.x * 2 + threadIdx.x] = s_cnt[threadIdx.x];
+}
+
+// Exclusive scan of the per-block counts by one workgroup (1024 threads), fixed order.
+__global__ __launch_bounds__(1024) void k_scan(uint32_t* bcount, int nblk, uint32_t* totals) {
+    __shared__ uint32_t s[2][1024];
+    __shared__ uint32_t carry[2];
+    if (threadIdx.x < 2) carry[threadIdx.x] = 0;
+    __syncthreads();
+    for (int base = 0; base < nblk; base += 1024) {
+        int i = base + threadIdx.x;
+        for (int k = 0; k < 2; ++k) s[k][threadIdx.x] = (i < nblk) ? bcount[i * 2 + k] : 0;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            uint32_t a0 = threadIdx.x >= o ? s[0][threadIdx.x - o] : 0;
+            uint32_t a1 = threadIdx.x >= o ? s[1][threadIdx.x - o] : 0;
+            __syncthreads();
+            s[0][threadIdx.x] += a0;
+            s[1][threadIdx.x] += a1;
+            __syncthreads();
+        }
+        for (int k = 0; k < 2; ++k) {
+            uint32_t incl = s[k][threadIdx.x];
+            uint32_t own = (i < nblk) ? bcount[i * 2 + k] : 0;
+            if (i < nblk) bcount[i * 2 + k] = carry[k] + incl - own;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            carry[0] += s[0][1023];
+            carry[1] += s[1][1023];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        totals[0] = carry[0];  // active (fine)
+        totals[1] = carry[1];  // coarse
+    }
+}
+
+// Ordered scatter: cells of a block in ascending order; ranks from wave ballots.
+__global__ __launch_bounds__(256) void k_emit(int N, const uint8_t* flag, const uint32_t* boff,
+                                              uint32_t* act, uint32_t* crs, uint32_t* ltag,
+                                              uint32_t* lpos, uint32_t tag_c, uint32_t tag_f) {
+    __shared__ uint32_t wcnt[2][4];
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    uint8_t f = c < N ? flag[c] : 0;
+    int w = threadIdx.x >> 6, l = lane_id();
+    uint64_t ba = __ballot(f == 1), bc = __ballot(f == 2);
+    uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
+    if (l == 0) {
+        wcnt[0][w] = __popcll(ba);
+        wcnt[1][w] = __popcll(bc);
+    }
+    __syncthreads();
+    if (f) {
+        int k = f - 1;
+        uint32_t pos = boff[blockIdx.x * 2 + k];
+        for (int j = 0; j < w; ++j) pos += wcnt[k][j];
+        pos += __popcll((k == 0 ? ba : bc) & below);
+        if (k == 0) {
+            act[pos] = (uint32_t)c;
+            ltag[c] = tag_f;
+        } else {
+            crs[pos] = (uint32_t)c;
+            ltag[c] = tag_c;
+        }
+        lpos[c] = pos;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// K3 transport
+// ------------------------------------------------------------------------------------------
+struct WaveCell {
+    // psi_out of the own cell: lane s < 32 holds slot s
+    uint32_t obid, oref, n_out;
+    // the cell's four outflow buckets
+    uint32_t bbid[4], bref[4], bn[4];
+};
+
+// Finds desired in a 32-slot table held one slot per lane, else claims the next slot, else
+// falls back to the closest occupied block (device_psic.cuh:192-260, device_bucket.cuh:61-128).
+// Returns the slot (-1 when saturated and empty) and the selected block id.
+__device__ inline int table_find_or_claim(uint32_t& mybid, uint32_t& myref, uint32_t& n, uint32_t want,
+                                          uint32_t& sel, float* pool, uint32_t* pcount, uint32_t pcap,
+                                          uint32_t* gbid, uint32_t* gsref, int LB, uint32_t* flags,
+                                          int flag_full) {
+    const int l = lane_id();
+    sel = want;
+    uint64_t hit = __ballot(l < 32 && mybid == want);
+    if (hit) return __ffsll((unsigned long long)hit) - 1;
+    if (n < (uint32_t)KB) {
+        int s = (int)n;
+        uint32_t ref = 0;
+        if (l == 0) ref = atomicAdd(pcount, 1u);
+        ref = rlu(ref, 0);
+        if (ref >= pcap) {
+            if (l == 0) atomicOr(&flags[flag_full], 1u);
+            return -1;
+        }
+        for (int i = l; i < LB; i += 64) pool[(size_t)ref * LB + i] = 0.0f;
+        if (l == 0) {
+            gbid[s] = want;
+            gsref[s] = ref;
+        }
+        if (l == s) {
+            mybid = want;
+            myref = ref;
+        }
+        n = n + 1;
+        __threadfence_block();
+        return s;
+    }
+    int key = 0x7fffffff;
+    if (l < 32 && mybid != EMPTY) {
+        int d = abs((int)((mybid >> 12) & 0xFFF) - (int)((want >> 12) & 0xFFF)) * 8 +
+                abs((int)(mybid & 0xFFF) - (int)(want & 0xFFF));
+        key = (d << 5) | l;
+    }
+    key = wave_min(key);
+    if (key == 0x7fffffff) return -1;
+    int s = key & 31;
+    sel = rlu(mybid, s);
+    return s;
+}
+
+template <class LY>
+__device__ inline int remap(int l, uint32_t src, uint32_t dst) {  // device_psic.cuh:265-292
+    if (src == dst) return l;
+    int tg = (int)(src & 0xFFF) * LY::TL + LY::lt(l);
+    int eg = (int)((src >> 12) & 0xFFF) * LY::EL + LY::le(l);
+    int td = min(max(tg - (int)(dst & 0xFFF) * LY::TL, 0), LY::TL - 1);
+    int ed = min(max(eg - (int)((dst >> 12) & 0xFFF) * LY::EL, 0), LY::EL - 1);
+    return LY::enc(td, ed, LY::lx(l), LY::lz(l));
+}
+
+struct CellAccum {
+    double edep, E_cut, E_nuc, E_bnd, drop_w, drop_E, cl_win, cl_wout, cl_Ein, cl_Eout;
+    float w_cut, w_nuc, w_bnd;
+    uint32_t comps;
+};
+
+template <class LY>
+struct Ctx {
+    const Tabs& T;
+    Bkt& bk;
+    uint32_t li;
+    uint32_t* flags;
+    WaveCell& wc;
+    CellAccum& a;
+
+    // device_emit_component_to_bucket_4d (device_bucket.cuh:431-533); wave-uniform arguments.
+    __device__ float emit4d(int face, float theta, float E, float weight, int xs, int zs) {
+        if (weight <= 0.0f) return 0.0f;
+        const int NT = T.NT, NE = T.NE;
+        const float tmin = T.te[0], tmax = T.te[NT];
+        const float dth = (tmax - tmin) / NT;
+        float tc = fmaxf(tmin, fminf(theta, tmax));
+        float c0 = tmin + 0.5f * dth;
+        int bc = (int)floorf((tc - c0) / dth + 0.5f);
+        bc = max(0, min(bc, NT - 1));
+        float delta = tc - (c0 + (float)bc * dth);
+        int tb1 = bc;
+        float tw0 = 1.0f, tw1 = 0.0f;
+        if (delta > 0.0f && bc < NT - 1) tb1 = bc + 1;
+        else if (delta < 0.0f && bc > 0) tb1 = bc - 1;
+        if (tb1 != bc) {
+            float fr = fminf(1.0f, fabsf(delta) / dth);
+            tw1 = fr * fr;
+            tw0 = 1.0f - tw1;
+        }
+        float Ec = fmaxf(T.ee[0], fminf(E, T.ee[NE]));
+        int eb = find_bin(T.ee, NE, Ec);
+        eb = max(0, min(eb, NE - 1));
+        const int el = eb % LY::EL;
+        const uint32_t bE = (uint32_t)(eb / LY::EL);
+        float dropped = 0.0f;
+        uint32_t base = (li * 4 + face) * KB;
+        for (int ti = 0; ti < 2; ++ti) {
+            float tw = ti == 0 ? tw0 : tw1;
+            if (tw <= 0.0f) continue;
+            float we = weight * tw;
+            int t = ti == 0 ? bc : tb1;
+            uint32_t bid = ((uint32_t)(t / LY::TL) & 0xFFF) | ((bE & 0xFFF) << 12);
+            int lidx = LY::enc(t % LY::TL, el, xs, zs);
+            uint32_t sel;
+            int s = table_find_or_claim(wc.bbid[face], wc.bref[face], wc.bn[face], bid, sel, bk.pool,
+                                        bk.pcount, bk.pcap, bk.bid + base, bk.sref + base, LY::LB,
+                                        flags, F_BKT_FULL);
+            if (s < 0) {
+                dropped += we;
+                continue;
+            }
+            int l2 = (sel == bid) ? lidx : remap<LY>(lidx, bid, sel);
+            uint32_t ref = rlu(wc.bref[face], s);
+            if (lane_id() == 0) bk.pool[(size_t)ref * LY::LB + l2] += we;
+            a.cl_wout += we;
+            int ebw = min(max((int)((sel >> 12) & 0xFFF) * LY::EL + LY::le(l2), 0), NE - 1);
+            a.cl_Eout += (double)we * (double)ecenter(T.ee, ebw);
+        }
+        return dropped;
+    }
+};
+
+// Wave-uniform broadcast of one component's results (lane j -> all lanes).
+struct Comp {
+    int kind;     // 0 prune, 1 cutoff before the step, 2 transported
+    float w, E;
+    float edep, w_rem, E_rem, w_new, E_new;
+    int face;     // -1 stays in the cell
+    float th_new, x_new, z_new;
+    float sw[11]; // lateral weights (+z exit) or sub-bin fractions (remain)
+    float wcf, wl, wr;
+    int lidx;
+};
+
+template <class LY, bool FINE>
+__global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out, Bkt bk,
+                                                   const uint32_t* __restrict__ list,
+                                                   const uint32_t* __restrict__ nlist,
+                                                   const uint32_t* __restrict__ ltag, uint32_t tag,
+                                                   Tally tal, double* acc, uint32_t* flags) {
+    const int l = lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t n = *nlist;
+    const int Nx = T.Nx, NT = T.NT, NE = T.NE;
+    const float dx = T.dx, dz = T.dz, hdx = dx * 0.5f, hdz = dz * 0.5f;
+    const float tmin = T.te[0], tmax = T.te[NT];
+    const float dth = (tmax - tmin) / NT;
+    for (uint32_t li = wave; li < n; li += nw) {
+        const int cell = (int)list[li];
+        const int iz = cell / Nx, ix = cell % Nx;
+        const float rh = T.rho[iz];
+        const float X0 = X0W * T.x0r[iz];
+        WaveCell wc;
+        wc.n_out = out.ns[cell];
+        wc.obid = (l < 32 && (uint32_t)l < wc.n_out) ? out.bid[(size_t)cell * KB + l] : EMPTY;
+        wc.oref = (l < 32 && (uint32_t)l < wc.n_out) ? out.sref[(size_t)cell * KB + l] : 0u;
+        for (int f = 0; f < 4; ++f) {
+            wc.bbid[f] = EMPTY;
+            wc.bref[f] = 0;
+            wc.bn[f] = 0;
+        }
+        CellAccum a = {};
+        Ctx<LY> cx{T, bk, li, flags, wc, a};
+        const uint32_t ns = in.ns[cell];
+        const float A = T.feA[iz], B = T.feB[iz];
+        for (uint32_t slot = 0; slot < ns; ++slot) {
+            const uint32_t bid = in.bid[(size_t)cell * KB + slot];
+            if (bid == EMPTY) continue;
+            const uint32_t bT = bid & 0xFFF, bEb = (bid >> 12) & 0xFFF;
+            const float* vin = in.pool + (size_t)in.sref[(size_t)cell * KB + slot] * LY::LB;
+            float v[LY::PL];
+#pragma unroll
+            for (int k = 0; k < LY::PL; ++k) v[k] = vin[l + 64 * k];
+#pragma unroll
+            for (int k = 0; k < LY::PL; ++k) {
+                uint64_t m = __ballot(v[k] > 0.0f);
+                if (!m) continue;
+                // ---------------- physics, one bin per lane ----------------
+                Comp r;
+                r.kind = 2;
+                r.face = -1;
+                r.lidx = l + 64 * k;
+                r.w = v[k];
+                r.edep = r.w_rem = r.E_rem = r.w_new = r.E_new = 0.0f;
+                r.th_new = r.x_new = r.z_new = 0.0f;
+                r.wcf = r.wl = r.wr = 0.0f;
+#pragma unroll
+                for (int q = 0; q < 11; ++q) r.sw[q] = 0.0f;
+                if (v[k] > 0.0f) {
+                    const int lidx = r.lidx;
+                    const float weight = v[k];
+                    int tbin = (int)bT * LY::TL + LY::lt(lidx);
+                    int ebin = (int)bEb * LY::EL + LY::le(lidx);
+                    if (FINE) {
+                        tbin = max(0, min(tbin, NT - 1));
+                        ebin = max(0, min(ebin, NE - 1));
+                    }
+                    const float E = ecenter(T.ee, ebin);
+                    r.E = E;
+                    const bool pruned = FINE ? (weight < 1e-15f) : (weight < 1e-12f);
+                    if (pruned) {
+                        r.kind = 0;
+                    } else if (E <= 0.1f) {
+                        r.kind = 1;
+                    } else {
+                        const float theta = FINE ? T.te[tbin] + 0.5f * dth : tmin + (tbin + 0.5f) * dth;
+                        const float xc = dx * ((float)(2 * LY::lx(lidx) + 1 - LY::XS) / (float)(2 * LY::XS));
+                        const float zc = dz * ((float)(2 * LY::lz(lidx) + 1 - LY::ZS) / (float)(2 * LY::ZS));
+                        const float mu = cosf_(theta), eta = sinf_(theta);
+                        float ds, dE, E_new, edep, w_new, w_rem, E_rem, sx, sxt, sth;
+                        unsigned pseed;
+                        if (FINE) {
+                            float s1 = (mu > 0) ? (hdz - zc) / mu : 1e30f;
+                            float s2 = (mu < 0) ? (-hdz - zc) / mu : 1e30f;
+                            float s3 = (eta > 0) ? (hdx - xc) / eta : 1e30f;
+                            float s4 = (eta < 0) ? (-hdx - xc) / eta : 1e30f;
+                            float sb = fmaxf(fminf(fminf(s1, s2), fminf(s3, s4)), 0.0f);
+                            ds = fminf(max_step(T, E, rh, P.rule), sb * 1.001f);
+                            float mdE = deposition(T, E, ds, rh);
+                            if (P.strag) {
+                                unsigned sd = (unsigned)((cell * 7 + (int)slot * 13 + lidx * 17) ^ 0x5DEECE66DL);
+                                float sE = straggling(E, ds, rh);
+                                dE = (sE <= 0.0f) ? mdE : fmaxf(mdE + gauss(sd) * sE, 0.0f);
+                            } else {
+                                dE = mdE;
+                            }
+                            dE = fminf(dE, E);
+                            E_new = E - dE;
+                            edep = dE * weight;
+                            if (P.nuc) {
+                                float surv = expf_(-(nuc_sigma(E) * rh) * ds);
+                                w_new = weight * surv;
+                                w_rem = weight - w_new;
+                                E_rem = w_rem * E;
+                            } else {
+                                w_new = weight;
+                                w_rem = 0.0f;
+                                E_rem = 0.0f;
+                            }
+                            float smm = fmaxf(ds, 0.0f);
+                            float Em = fmaxf(E - 0.5f * dE, T.ee[0]);
+                            float Tp_ = scat_power(Em, smm, X0);
+                            sth = sqrtf_(fmaxf(Tp_ * smm, 0.0f));
+                            float dC = 2.0f * B * smm + A * smm * smm + Tp_ * smm * smm * smm / 3.0f;
+                            dC = fmaxf(dC, 0.0f);
+                            sx = fmaxf(sqrtf_(dC), 0.01f);
+                            sxt = 0.001f;
+                            pseed = (unsigned)fminf((float)iz * dz * 1000.0f, 4294967295.0f);
+                        } else {
+                            float cstep = fminf(P.step_coarse, fminf(dx, dz));
+                            float crs = cstep;
+                            bool split = false;
+                            if (E > P.E_on + 1e-6f) {
+                                float Rc = lookup_R(T, E);
+                                float Rf = lookup_R(T, fmaxf(P.E_on, T.ee[0]));
+                                float rtf = (Rc - Rf) / rh;
+                                if (rtf > 0.0f && rtf < crs) {
+                                    crs = rtf;
+                                    cstep = crs;
+                                    split = true;
+                                }
+                            }
+                            dE = fminf(deposition(T, E, crs, rh), E);
+                            E_new = E - dE;
+                            edep = dE * weight;
+                            float surv = expf_(-(nuc_sigma(E) * rh) * crs);
+                            w_new = weight * surv;
+                            w_rem = weight - w_new;
+                            E_rem = w_rem * E;
+                            if (split && E_new > P.E_on) {
+                                float ex = E_new - P.E_on;
+                                if (ex > 0.0f && w_new > 0.0f) edep += ex * w_new;
+                                E_new = P.E_on;
+                            }
+                            float ps = fmaxf(0.0f, (float)iz * dz + (zc + 0.5f * dz));
+                            float smm = fmaxf(crs, 0.0f);
+                            float st0 = (ps > 0.0f) ? highland(E, ps, X0) : 0.0f;
+                            float Ao = st0 * st0;
+                            float Bo = 0.5f * Ao * ps;
+                            float Co = P.sx0 * P.sx0 + (Ao * ps * ps) / 3.0f;
+                            float Em = fmaxf(E - 0.5f * dE, T.ee[0]);
+                            float Tp_ = scat_power(Em, smm, X0);
+                            float An = fmaxf(Ao + Tp_ * smm, 0.0f);
+                            float Cn = fmaxf(Co + 2.0f * Bo * smm + Ao * smm * smm +
+                                                 (1.0f / 3.0f) * Tp_ * smm * smm * smm,
+                                             0.0f);
+                            sth = sqrtf_(fmaxf(An - Ao, 0.0f));
+                            float dC = fmaxf(Cn - Co, 0.0f);
+                            float unres = fminf(0.35f, 0.04f + 0.003f * ps);
+                            sx = fmaxf(sqrtf_(fmaxf(dC + unres * Co, 0.0f)), 0.01f);
+                            sxt = fmaxf(sinf_(fminf(sth, 1.57f)) * smm / 1.7320508f, 0.01f);
+                            pseed = (unsigned)fminf(ps * 1000.0f, 4294967295.0f);
+                            ds = cstep;
+                        }
+                        unsigned ts = ((unsigned)(cell + 1) * 2654435761u) ^
+                                      ((unsigned)(slot + 1) * 2246822519u) ^
+                                      ((unsigned)(lidx + 1) * 3266489917u) ^ (unsigned)(ebin + 17) ^
+                                      (pseed * 747796405u);
+                        float tsc = (sth <= 0.0f) ? 0.0f : gauss(ts) * sth;
+                        float thn = fmaxf(T.te[0], fminf(theta + tsc, T.te[NT]));
+                        float mun = cosf_(thn), etn = sinf_(thn);
+                        float xn = xc + etn * ds, zn = zc + mun * ds;
+                        int face = -1;
+                        if (zn >= hdz - 0.001f) face = 0;
+                        else if (zn <= -hdz + 0.001f) face = 1;
+                        else if (xn >= hdx - 0.001f) face = 2;
+                        else if (xn <= -hdx + 0.001f) face = 3;
+                        xn = fmaxf(-hdx, fminf(xn, hdx));
+                        zn = fmaxf(-hdz, fminf(zn, hdz));
+                        r.edep = edep;
+                        r.w_rem = w_rem;
+                        r.E_rem = E_rem;
+                        r.w_new = w_new;
+                        r.E_new = E_new;
+                        r.face = face;
+                        r.th_new = thn;
+                        r.x_new = xn;
+                        r.z_new = zn;
+                        if (E_new > 0.1f) {
+                            if (face == 0) {
+                                // device_emit_lateral_spread weights (device_bucket.cuh:880-916)
+                                float s = fmaxf(sxt, 1e-6f);
+                                float xmin = xn - 0.5f * 11.0f * dx;
+                                float prev = gcdf(xmin, xn, s), sum = 0.0f;
+#pragma unroll
+                                for (int q = 0; q < 11; ++q) {
+                                    float cur = gcdf(xmin + (q + 1) * dx, xn, s);
+                                    r.sw[q] = cur - prev;
+                                    prev = cur;
+                                }
+#pragma unroll
+                                for (int q = 0; q < 11; ++q) sum += r.sw[q];
+                                if (sum > 1e-10f) {
+#pragma unroll
+                                    for (int q = 0; q < 11; ++q) r.sw[q] /= sum;
+                                } else {
+#pragma unroll
+                                    for (int q = 0; q < 11; ++q) r.sw[q] = (q == 5) ? 1.0f : 0.0f;
+                                }
+                                sum = 0.0f;
+#pragma unroll
+                                for (int q = 0; q < 11; ++q) {
+                                    r.sw[q] = fmaxf(0.0f, r.sw[q]);
+                                    sum += r.sw[q];
+                                }
+                                if (sum < 1e-12f) {
+#pragma unroll
+                                    for (int q = 0; q < 11; ++q) r.sw[q] = (q == 5) ? 1.0f : 0.0f;
+                                } else {
+#pragma unroll
+                                    for (int q = 0; q < 11; ++q) r.sw[q] /= sum;
+                                }
+                            } else if (face < 0) {
+                                // device_gaussian_spread_weights_subcell + in-cell fraction and
+                                // tails (k3_finetransport.cu:585-653)
+                                float s = fmaxf(FINE ? sxt : sx, 1e-6f);
+                                float dxs = dx / LY::XS;
+                                float xmin = -dx * 0.5f;
+                                float prev = gcdf(xmin, xn, s), sum = 0.0f;
+                                float w8[LY::XS];
+#pragma unroll
+                                for (int q = 0; q < LY::XS; ++q) {
+                                    float cur = gcdf(xmin + (q + 1) * dxs, xn, s);
+                                    w8[q] = cur - prev;
+                                    prev = cur;
+                                }
+#pragma unroll
+                                for (int q = 0; q < LY::XS; ++q) sum += w8[q];
+                                if (sum > 1e-10f) {
+#pragma unroll
+                                    for (int q = 0; q < LY::XS; ++q) w8[q] /= sum;
+                                } else {
+#pragma unroll
+                                    for (int q = 0; q < LY::XS; ++q) w8[q] = (q == LY::XS / 2) ? 1.0f : 0.0f;
+                                }
+                                float win = gcdf(hdx, xn, sx) - gcdf(-hdx, xn, sx);
+                                float wcf = fminf(1.0f, fmaxf(0.0f, win));
+                                float wss = 0.0f;
+#pragma unroll
+                                for (int q = 0; q < LY::XS; ++q) {
+                                    w8[q] = fmaxf(0.0f, w8[q]);
+                                    wss += w8[q];
+                                }
+                                if (wss < 1e-12f) {
+#pragma unroll
+                                    for (int q = 0; q < LY::XS; ++q) w8[q] = (q == LY::XS / 2) ? 1.0f : 0.0f;
+                                    wss = 1.0f;
+                                }
+#pragma unroll
+                                for (int q = 0; q < LY::XS; ++q) r.sw[q] = (w8[q] / wss) * wcf;
+                                float wl = 0.0f, wr = 0.0f;
+                                if (wcf < 1.0f - 1e-6f) {
+                                    float wlr = fmaxf(0.0f, gcdf(-hdx, xn, sx));
+                                    float wrr = fmaxf(0.0f, 1.0f - gcdf(hdx, xn, sx));
+                                    float wtt = fmaxf(0.0f, 1.0f - wcf);
+                                    float wts = wlr + wrr;
+                                    if (wtt > 0.0f && wts > 1e-12f) {
+                                        float sc = wtt / wts;
+                                        wl = wlr * sc;
+                                        wr = wrr * sc;
+                                    }
+                                }
+                                r.wcf = wcf;
+                                r.wl = wl;
+                                r.wr = wr;
+                            }
+                        }
+                    }
+                }
+                // ---------------- ordered apply, bin order ----------------
+                while (m) {
+                    const int j = __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    const int kind = rli(r.kind, j);
+                    const float w = rlf(r.w, j), E = rlf(r.E, j);
+                    a.comps++;
+                    a.cl_win += (double)w;
+                    a.cl_Ein += (double)w * (double)E;
+                    if (kind == 0) {
+                        a.drop_w += (double)w;
+                        a.drop_E += (double)(E * w);
+                        a.cl_wout += (double)w;
+                        a.cl_Eout += (double)(E * w);
+                        continue;
+                    }
+                    if (kind == 1) {
+                        a.w_cut += w;
+                        a.E_cut += E * w;
+                        a.cl_wout += (double)w;
+                        a.cl_Eout += (double)(E * w);
+                        continue;
+                    }
+                    const float edep = rlf(r.edep, j), w_rem = rlf(r.w_rem, j), E_rem = rlf(r.E_rem, j);
+                    const float w_new = rlf(r.w_new, j), E_new = rlf(r.E_new, j);
+                    const int face = rli(r.face, j);
+                    const float thn = rlf(r.th_new, j), xn = rlf(r.x_new, j), zn = rlf(r.z_new, j);
+                    a.edep += edep;
+                    a.w_nuc += w_rem;
+                    a.E_nuc += E_rem;
+                    a.cl_wout += (double)w_rem;
+                    a.cl_Eout += (double)edep + (double)E_rem;
+                    if (E_new <= 0.1f) {
+                        a.w_cut += w_new;
+                        a.E_cut += E_new * w_new;
+                        a.cl_wout += (double)w_new;
+                        a.cl_Eout += (double)(E_new * w_new);
+                        continue;
+                    }
+                    if (face >= 0) {
+                        bool nb = (face == 0) ? (iz + 1 < T.Nz) : (face == 1) ? (iz > 0)
+                                : (face == 2) ? (ix + 1 < Nx) : (ix > 0);
+                        if (!nb) {
+                            a.w_bnd += w_new;
+                            a.E_bnd += E_new * w_new;
+                            a.cl_wout += (double)w_new;
+                            a.cl_Eout += (double)(E_new * w_new);
+                            continue;
+                        }
+                        const float E_emit = ecenter(T.ee, find_bin(T.ee, NE, E_new));
+                        float drop = 0.0f;
+                        float lat_bnd = 0.0f;
+                        if (face == 0) {
+#pragma unroll
+                            for (int q = 0; q < 11; ++q) {
+                                float wf = rlf(r.sw[q], j);
+                                if (wf <= 0.0f) continue;
+                                int off = q - 5;
+                                int it = ix + off;
+                                if (it < 0 || it >= Nx) {
+                                    lat_bnd += w_new * wf;
+                                    continue;
+                                }
+                                if (off == 0) {
+                                    drop += cx.emit4d(0, thn, E_new, w_new * wf, LY::XS / 2, 0);
+                                } else {
+                                    // Emission into another cell's +z bucket: a drop when that
+                                    // cell is not listed in this phase (device_bucket.cuh:953-963);
+                                    // a listed target would need a cross-wave ordered write,
+                                    // which this path rejects.
+                                    if (ltag[cell + off] == tag && l == 0) atomicAdd(&flags[F_UNSUPPORTED], 1u);
+                                    drop += w_new * wf;
+                                }
+                            }
+                            if (lat_bnd > 0.0f) {
+                                a.w_bnd += lat_bnd;
+                                a.E_bnd += E_new * lat_bnd;
+                                a.cl_wout += (double)lat_bnd;
+                                a.cl_Eout += (double)(E_new * lat_bnd);
+                            }
+                            if (drop > 0.0f) {
+                                a.drop_w += (double)drop;
+                                a.drop_E += (double)(E_new * drop);
+                                a.cl_wout += (double)drop;
+                                a.cl_Eout += (double)(E_new * drop);
+                            }
+                            if (FINE) {
+                                float tw = fmaxf(0.0f, w_new - lat_bnd - drop);
+                                if (tw > 0.0f) {
+                                    a.edep += (double)((E_new - E_emit) * tw);
+                                    a.cl_Eout += (double)((E_new - E_emit) * tw);
+                                }
+                            }
+                        } else {
+                            float xon = (face == 2) ? -dx * 0.5f : (face == 3 ? dx * 0.5f : xn);
+                            int xsn = min(max((int)(((xon / dx) + 0.5f) * LY::XS), 0), LY::XS - 1);
+                            int zsn;
+                            if (face == 1) zsn = LY::ZS - 1;
+                            else zsn = min(max((int)(((zn / dz) + 0.5f) * LY::ZS), 0), LY::ZS - 1);
+                            drop = cx.emit4d(face, thn, E_new, w_new, xsn, zsn);
+                            if (drop > 0.0f) {
+                                a.drop_w += (double)drop;
+                                a.drop_E += (double)(E_new * drop);
+                                a.cl_wout += (double)drop;
+                                a.cl_Eout += (double)(E_new * drop);
+                            }
+                            if (FINE) {
+                                float tw = fmaxf(0.0f, w_new - drop);
+                                if (tw > 0.0f) {
+                                    a.edep += (double)((E_new - E_emit) * tw);
+                                    a.cl_Eout += (double)((E_new - E_emit) * tw);
+                                }
+                            }
+                        }
+                        continue;
+                    }
+                    // ---- remain in the cell ----
+                    int tbn = (int)((thn - tmin) / dth);
+                    tbn = (int)fmaxf(0, fminf(tbn, NT - 1));
+                    int ebn = find_bin(T.ee, NE, E_new);
+                    const uint32_t bidn = ((uint32_t)(ebn / LY::EL) << 12) | ((uint32_t)(tbn / LY::TL) & 0xFFF);
+                    uint32_t sel;
+                    int os = table_find_or_claim(wc.obid, wc.oref, wc.n_out, bidn, sel, out.pool, out.pcount,
+                                                 out.pcap, out.bid + (size_t)cell * KB,
+                                                 out.sref + (size_t)cell * KB, LY::LB, flags, F_PSI_FULL);
+                    if (os < 0) {
+                        a.drop_w += (double)w_new;
+                        a.drop_E += (double)(E_new * w_new);
+                        a.cl_wout += (double)w_new;
+                        a.cl_Eout += (double)(E_new * w_new);
+                        continue;
+                    }
+                    int tbe = tbn, ebe = ebn;
+                    if (sel != bidn) {
+                        int bte = (int)(sel & 0xFFF), bee = (int)((sel >> 12) & 0xFFF);
+                        tbe = max(bte * LY::TL, min(tbe, bte * LY::TL + LY::TL - 1));
+                        ebe = max(bee * LY::EL, min(ebe, bee * LY::EL + LY::EL - 1));
+                    }
+                    const float E_emit = T.ee[ebe] + 0.5f * (T.ee[ebe + 1] - T.ee[ebe]);
+                    const int zst = min(max((int)(((zn / dz) + 0.5f) * LY::ZS), 0), LY::ZS - 1);
+                    const uint32_t oref = rlu(wc.oref, os);
+                    float mywf = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < LY::XS; ++q) {
+                        float wf = rlf(r.sw[q], j);
+                        if (l == q) mywf = wf;
+                        if (wf > 0.0f) {
+                            float wsp = w_new * wf;
+                            a.cl_wout += (double)wsp;
+                            a.cl_Eout += (double)wsp * (double)ecenter(T.ee, ebe);
+                        }
+                    }
+                    if (l < LY::XS && mywf > 0.0f) {
+                        size_t addr = (size_t)oref * LY::LB + LY::enc(tbe % LY::TL, ebe % LY::EL, l, zst);
+                        out.pool[addr] += w_new * mywf;
+                    }
+                    const float wcf = rlf(r.wcf, j), wl = rlf(r.wl, j), wr = rlf(r.wr, j);
+                    if (wcf < 1.0f - 1e-6f) {
+                        const int zsb = zst;
+                        if (wl > 0.0f) {
+                            float wsl = w_new * wl;
+                            if (ix > 0) {
+                                float d = cx.emit4d(3, thn, E_new, wsl, LY::XS - 1, zsb);
+                                if (d > 0.0f) {
+                                    a.drop_w += (double)d;
+                                    a.drop_E += (double)(E_new * d);
+                                    a.cl_wout += (double)d;
+                                    a.cl_Eout += (double)(E_new * d);
+                                }
+                            } else {
+                                a.w_bnd += wsl;
+                                a.E_bnd += E_new * wsl;
+                                a.cl_wout += (double)wsl;
+                                a.cl_Eout += (double)(E_new * wsl);
+                            }
+                        }
+                        if (wr > 0.0f) {
+                            float wsr = w_new * wr;
+                            if (ix < Nx - 1) {
+                                float d = cx.emit4d(2, thn, E_new, wsr, 0, zsb);
+                                if (d > 0.0f) {
+                                    a.drop_w += (double)d;
+                                    a.drop_E += (double)(E_new * d);
+                                    a.cl_wout += (double)d;
+                                    a.cl_Eout += (double)(E_new * d);
+                                }
+                            } else {
+                                a.w_bnd += wsr;
+                                a.E_bnd += E_new * wsr;
+                                a.cl_wout += (double)wsr;
+                                a.cl_Eout += (double)(E_new * wsr);
+                            }
+                        }
+                    }
+                    if (FINE) {
+                        float rf = wcf;
+                        if (ix > 0) rf += wl;
+                        if (ix < Nx - 1) rf += wr;
+                        rf = fminf(1.0f, fmaxf(0.0f, rf));
+                        float rw = w_new * rf;
+                        if (rw > 0.0f) {
+                            a.edep += (double)((E_new - E_emit) * rw);
+                            a.cl_Eout += (double)((E_new - E_emit) * rw);
+                        }
+                    }
+                }
+            }
+        }
+        // ---- end of cell: tallies, bucket counts, psi_out slot count ----
+        if (l == 0) {
+            tal.edep[cell] += a.edep;
+            tal.w_cut[cell] += a.w_cut;
+            tal.E_cut[cell] += a.E_cut;
+            tal.w_nuc[cell] += a.w_nuc;
+            tal.E_nuc[cell] += a.E_nuc;
+            tal.w_bnd[cell] += a.w_bnd;
+            tal.E_bnd[cell] += a.E_bnd;
+            out.ns[cell] = wc.n_out;
+            for (int f = 0; f < 4; ++f) bk.n[li * 4 + f] = wc.bn[f];
+            double* o = acc + (size_t)li * NACC;
+            o[A_EDEP] = a.edep;
+            o[A_WCUT] = a.w_cut;
+            o[A_ECUT] = a.E_cut;
+            o[A_WNUC] = a.w_nuc;
+            o[A_ENUC] = a.E_nuc;
+            o[A_WBND] = a.w_bnd;
+            o[A_EBND] = a.E_bnd;
+            o[A_DROPW] = a.drop_w;
+            o[A_DROPE] = a.drop_E;
+            o[A_CLWIN] = a.cl_win;
+            o[A_CLWOUT] = a.cl_wout;
+            o[A_CLEIN] = a.cl_Ein;
+            o[A_CLEOUT] = a.cl_Eout;
+            o[A_COMPS] = (double)a.comps;
+            o[A_UPD] = (double)((size_t)ns * LY::LB);
+        }
+    }
+}
+
+// Destination set of a phase: the four neighbours of every listed cell, deduplicated with a
+// per-cell tag (the order of the resulting list does not affect any result).
+__global__ void k_dest(const uint32_t* list, const uint32_t* nlist, int Nx, int Nz, uint32_t* dtag,
+                       uint32_t tag, uint32_t* dlist, uint32_t* ndest, uint32_t cap, uint32_t* flags) {
+    const uint32_t n = *nlist;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int c = (int)list[i];
+        const int ix = c % Nx, iz = c / Nx;
+        int nb[4] = {iz + 1 < Nz ? c + Nx : -1, iz > 0 ? c - Nx : -1, ix + 1 < Nx ? c + 1 : -1,
+                     ix > 0 ? c - 1 : -1};
+        for (int f = 0; f < 4; ++f) {
+            int d = nb[f];
+            if (d < 0) continue;
+            uint32_t old = dtag[d];
+            if (old == tag) continue;
+            if (atomicCAS(&dtag[d], old, tag) == old) {
+                uint32_t p = atomicAdd(ndest, 1u);
+                if (p < cap) dlist[p] = (uint32_t)d;
+                else atomicOr(&flags[F_DEST_FULL], 1u);
+            }
+        }
+    }
+}
+
+// K4: gather the neighbours' buckets into psi_out of one destination (faces 0..3, bucket slots
+// in order), one wavefront per destination.
+template <class LY>
+__global__ __launch_bounds__(256) void k_gather(Tabs T, Psi out, Bkt bk, const uint32_t* __restrict__ dlist,
+                                                const uint32_t* __restrict__ ndest,
+                                                const uint32_t* __restrict__ ltag,
+                                                const uint32_t* __restrict__ lpos, uint32_t tag,
+                                                double* k4drop_w, double* k4drop_E, uint32_t* flags) {
+    const int l = lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t n = *ndest;
+    const int Nx = T.Nx, Nz = T.Nz;
+    for (uint32_t i = wave; i < n; i += nw) {
+        const int d = (int)dlist[i];
+        const int ix = d % Nx, iz = d / Nx;
+        uint32_t n_out = out.ns[d];
+        uint32_t obid = (l < 32 && (uint32_t)l < n_out) ? out.bid[(size_t)d * KB + l] : EMPTY;
+        uint32_t oref = (l < 32 && (uint32_t)l < n_out) ? out.sref[(size_t)d * KB + l] : 0u;
+        double dw = 0.0, de = 0.0;
+        for (int face = 0; face < 4; ++face) {
+            int src = -1;
+            if (face == 0 && iz > 0) src = d - Nx;
+            if (face == 1 && iz + 1 < Nz) src = d + Nx;
+            if (face == 2 && ix > 0) src = d - 1;
+            if (face == 3 && ix + 1 < Nx) src = d + 1;
+            if (src < 0 || ltag[src] != tag) continue;
+            const uint32_t pos = lpos[src];
+            const uint32_t base = (pos * 4 + face) * KB;  // source face == receiving face index
+            const uint32_t nb = bk.n[pos * 4 + face];
+            for (uint32_t s = 0; s < nb; ++s) {
+                const uint32_t bid = bk.bid[base + s];
+                if (bid == EMPTY) continue;
+                const float* bv = bk.pool + (size_t)bk.sref[base + s] * LY::LB;
+                float v[LY::PL];
+#pragma unroll
+                for (int k = 0; k < LY::PL; ++k) v[k] = bv[l + 64 * k];
+                uint32_t sel;
+                int os = table_find_or_claim(obid, oref, n_out, bid, sel, out.pool, out.pcount, out.pcap,
+                                             out.bid + (size_t)d * KB, out.sref + (size_t)d * KB, LY::LB,
+                                             flags, F_PSI_FULL);
+                if (os < 0) {
+                    double pw = 0.0, pe = 0.0;
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k) {
+                        if (v[k] > 0.0f) {
+                            int lidx = l + 64 * k;
+                            int eb = min(max((int)((bid >> 12) & 0xFFF) * LY::EL + LY::le(lidx), 0), T.NE - 1);
+                            pw += (double)v[k];
+                            pe += (double)ecenter(T.ee, eb) * (double)v[k];
+                        }
+                    }
+                    dw += wave_sum(pw);
+                    de += wave_sum(pe);
+                    continue;
+                }
+                const uint32_t ref = rlu(oref, os);
+                float* dst = out.pool + (size_t)ref * LY::LB;
+                if (sel == bid) {
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k)
+                        if (v[k] > 0.0f) dst[l + 64 * k] += v[k];
+                } else {
+                    // closest-block fallback: remapped bins may coincide, add in lidx order
+                    for (int k = 0; k < LY::PL; ++k) {
+                        for (int j = 0; j < 64; ++j) {
+                            float w = rlf(v[k], j);
+                            if (w > 0.0f && l == 0) dst[remap<LY>(j + 64 * k, bid, sel)] += w;
+                        }
+                    }
+                }
+            }
+        }
+        if (l == 0) {
+            out.ns[d] = n_out;
+            k4drop_w[d] += dw;
+            k4drop_E[d] += de;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// K5: fixed-order FP64 reductions.
+// Stage 1 (grid): per block sums over cells of W_in, E_in (from K1), W_out, E_out (psi_out)
+// and the K4 drops (consumed and zeroed).  Stage 2 (one block): block partials in order, then
+// the per-listed-cell accumulators of both phases in list order.
+// ------------------------------------------------------------------------------------------
+enum { R_WIN, R_EIN, R_WOUT, R_EOUT, R_K4W, R_K4E, NR1 };
+
+template <class LY>
+__global__ __launch_bounds__(256) void k_audit_cells(Tabs T, Psi out, const double* win, const double* ein,
+                                                     double* k4w, double* k4e, double* part) {
+    __shared__ double sh[NR1][256];
+    double r[NR1] = {0, 0, 0, 0, 0, 0};
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < T.N) {
+        r[R_WIN] = win[c];
+        r[R_EIN] = ein[c];
+        r[R_K4W] = k4w[c];
+        r[R_K4E] = k4e[c];
+        if (r[R_K4W] != 0.0 || r[R_K4E] != 0.0) {
+            k4w[c] = 0.0;
+            k4e[c] = 0.0;
+        }
+        uint32_t n = out.ns[c];
+        for (uint32_t s = 0; s < n; ++s) {
+            uint32_t bid = out.bid[(size_t)c * KB + s];
+            if (bid == EMPTY) continue;
+            const float* v = out.pool + (size_t)out.sref[(size_t)c * KB + s] * LY::LB;
+            int bE = (int)((bid >> 12) & 0xFFF);
+            for (int l = 0; l < LY::LB; ++l) {
+                float w = v[l];
+                if (w <= 0.0f) continue;
+                r[R_WOUT] += (double)w;
+                int eb = min(max(bE * LY::EL + LY::le(l), 0), T.NE - 1);
+                r[R_EOUT] += (double)w * (double)ecenter(T.ee, eb);
+            }
+        }
+    }
+    for (int k = 0; k < NR1; ++k) sh[k][threadIdx.x] = r[k];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int k = 0; k < NR1; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < NR1) part[blockIdx.x * NR1 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// out[0..NR1) cell sums, out[NR1..NR1+NACC) phase sums, out[NR1+NACC] max cell W closure,
+// out[NR1+NACC+1] max fine-cell E closure.
+__global__ __launch_bounds__(256) void k_audit_final(const double* part, int nblk, const double* acc_c,
+                                                     const uint32_t* nc, const double* acc_f,
+                                                     const uint32_t* nf, double* outv) {
+    __shared__ double sh[NR1 + NACC + 2][256];
+    double r[NR1 + NACC + 2];
+    for (int k = 0; k < NR1 + NACC + 2; ++k) r[k] = 0.0;
+    // contiguous chunks per thread keep the order fixed
+    for (int b = threadIdx.x; b < nblk; b += 256)
+        for (int k = 0; k < NR1; ++k) r[k] += part[b * NR1 + k];
+    for (int ph = 0; ph < 2; ++ph) {
+        const double* acc = ph ? acc_f : acc_c;
+        const uint32_t n = ph ? *nf : *nc;
+        for (uint32_t i = threadIdx.x; i < n; i += 256) {
+            const double* o = acc + (size_t)i * NACC;
+            for (int k = 0; k < NACC; ++k) r[NR1 + k] += o[k];
+            if (o[A_CLWIN] > 0) r[NR1 + NACC] = fmax(r[NR1 + NACC], fabs(o[A_CLWIN] - o[A_CLWOUT]) / o[A_CLWIN]);
+            if (ph && o[A_CLEIN] > 0)
+                r[NR1 + NACC + 1] = fmax(r[NR1 + NACC + 1], fabs(o[A_CLEIN] - o[A_CLEOUT]) / o[A_CLEIN]);
+        }
+    }
+    for (int k = 0; k < NR1 + NACC + 2; ++k) sh[k][threadIdx.x] = r[k];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            for (int k = 0; k < NR1 + NACC; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
+            for (int k = NR1 + NACC; k < NR1 + NACC + 2; ++k)
+                sh[k][threadIdx.x] = fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < NR1 + NACC + 2) outv[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// ------------------------------------------------------------------------------------------
+// Source injection (k1k6_pipeline.cu:300-551, device_psic.cuh:347-450).  Samples are drawn in
+// parallel; their phase-space bins are then applied in sample order by one wavefront, so slot
+// claims are deterministic (the reference claims them in thread-race order).
+// ------------------------------------------------------------------------------------------
+struct InjSample {
+    int cell;      // -1 outside the grid
+    uint32_t bid;
+    int lidx;
+    float E;
+};
+
+__device__ inline float inj_gauss(unsigned& s) {  // gaussian_sample, k1k6_pipeline.cu:433-447
+    s = 1664525u * s + 1013904223u;
+    float u1 = (s >> 16) / 65536.0f;
+    s = 1664525u * s + 1013904223u;
+    float u2 = (s >> 16) / 65536.0f;
+    float r = sqrtf_(-2.0f * logf_(fmaxf(u1, 1e-10f)));
+    float th = (float)(2.0 * 3.14159265358979323846 * (double)u2);
+    return r * cosf_(th);
+}
+
+template <class LY>
+__device__ inline InjSample inj_bin(const Tabs& T, int cell, float theta, float E, float x, float z) {
+    InjSample o;
+    o.cell = cell;
+    o.E = E;
+    x = fmaxf(0.0f, fminf(x, T.dx));
+    z = fmaxf(0.0f, fminf(z, T.dz));
+    int xs = min(max((int)((((x - T.dx * 0.5f) / T.dx) + 0.5f) * LY::XS), 0), LY::XS - 1);
+    int zs = min(max((int)((((z - T.dz * 0.5f) / T.dz) + 0.5f) * LY::ZS), 0), LY::ZS - 1);
+    const int NT = T.NT, NE = T.NE;
+    float tmin = T.te[0], tmax = T.te[NT], Emin = T.ee[0], Emax = T.ee[NE];
+    theta = fmaxf(tmin, fminf(theta, tmax));
+    E = fmaxf(Emin, fminf(E, Emax));
+    float dth = (tmax - tmin) / NT;
+    float tc = (theta - tmin) / dth;
+    int tb = (int)tc;
+    float ft = tc - tb;
+    int eb = find_bin(T.ee, NE, E);
+    float lo = T.ee[eb], hi = T.ee[eb + 1];
+    float fe = (hi - lo) > 1e-10f ? (E - lo) / (hi - lo) : 0.0f;
+    if (tb >= NT - 1) { tb = NT - 1; ft = 0.0f; }
+    if (eb >= NE - 1) { eb = NE - 1; fe = 0.0f; }
+    if (ft >= 0.5f && tb < NT - 1) tb++;
+    if (fe >= 0.5f && eb < NE - 1) eb++;
+    o.bid = ((uint32_t)(tb / LY::TL) & 0xFFF) | (((uint32_t)(eb / LY::EL) & 0xFFF) << 12);
+    o.lidx = LY::enc(tb % LY::TL, eb % LY::EL, xs, zs);
+    return o;
+}
+
+template <class LY>
+__global__ void k_inject_samples(Tabs T, sm2d_spot sp, InjSample* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= sp.n_samples) return;
+    unsigned seed = sp.random_seed + (unsigned)i;
+    float xs = sp.x0 + sp.sigma_x * inj_gauss(seed);
+    float ts = sp.theta0 + sp.sigma_theta * inj_gauss(seed);
+    float Es = sp.E0 + sp.sigma_E * inj_gauss(seed);
+    Es = fmaxf(T.ee[0], fminf(Es, T.ee[T.NE]));
+    ts = fmaxf(T.te[0], fminf(ts, T.te[T.NT]));
+    float xr = xs - T.x_min, zr = sp.z0 - T.z_min;
+    int ix = (int)(xr / T.dx), iz = (int)(zr / T.dz);
+    if (ix < 0 || ix >= T.Nx || iz < 0 || iz >= T.Nz) {
+        InjSample o;
+        o.cell = -1;
+        o.bid = EMPTY;
+        o.lidx = 0;
+        o.E = Es;
+        out[i] = o;
+        return;
+    }
+    out[i] = inj_bin<LY>(T, iz * T.Nx + ix, ts, Es, xr - ix * T.dx, zr - iz * T.dz);
+}
+
+template <class LY>
+__global__ void k_inject_pencil(Tabs T, sm2d_spot sp, InjSample* out) {
+    float xr = sp.x0 - T.x_min, zr = sp.z0 - T.z_min;
+    int sx = (int)(xr / T.dx), sz = (int)(zr / T.dz);
+    sx = sx < 0 ? 0 : (sx >= T.Nx ? T.Nx - 1 : sx);
+    sz = sz < 0 ? 0 : (sz >= T.Nz ? T.Nz - 1 : sz);
+    out[0] = inj_bin<LY>(T, sz * T.Nx + sx, sp.theta0, sp.E0, xr - sx * T.dx, zr - sz * T.dz);
+}
+
+// One wavefront applies the samples in order.  acc[6]: W in / out / slot-drop, E in / out / drop.
+template <class LY>
+__global__ __launch_bounds__(64) void k_inject_apply(Psi in, const InjSample* s, int n, float w, double* acc,
+                                                     int pencil, float E0, uint32_t* flags) {
+    const int l = lane_id();
+    double Wi = 0, Wo = 0, Ws = 0, Ei = 0, Eo = 0, Es = 0;
+    for (int i = 0; i < n; ++i) {
+        const InjSample q = s[i];
+        if (q.cell < 0) {
+            Wo += (double)w;
+            Eo += (double)(q.E * w);
+            continue;
+        }
+        if (!(w > 0.0f)) {  // device_psic_inject_source rejects non-positive weights
+            Ws += (double)w;
+            Es += (double)(q.E * w);
+            continue;
+        }
+        uint32_t nn = in.ns[q.cell];
+        uint32_t mb = (l < 32 && (uint32_t)l < nn) ? in.bid[(size_t)q.cell * KB + l] : EMPTY;
+        uint32_t mr = (l < 32 && (uint32_t)l < nn) ? in.sref[(size_t)q.cell * KB + l] : 0u;
+        int slot = -1;
+        uint64_t hit = __ballot(l < 32 && mb == q.bid);
+        if (hit) {
+            slot = __ffsll((unsigned long long)hit) - 1;
+        } else if (nn < (uint32_t)KB) {
+            uint32_t sel;
+            slot = table_find_or_claim(mb, mr, nn, q.bid, sel, in.pool, in.pcount, in.pcap,
+                                       in.bid + (size_t)q.cell * KB, in.sref + (size_t)q.cell * KB, LY::LB,
+                                       flags, F_PSI_FULL);
+            if (l == 0 && slot >= 0) in.ns[q.cell] = nn;
+        }
+        if (slot >= 0) {
+            uint32_t ref = rlu(mr, slot);
+            if (l == 0) in.pool[(size_t)ref * LY::LB + q.lidx] += w;
+            Wi += (double)w;
+            Ei += (double)(q.E * w);
+        } else {
+            Ws += (double)w;
+            Es += (double)(q.E * w);
+        }
+        __threadfence_block();  // the next sample's slot search reads what lane 0 wrote
+    }
+    if (l == 0) {
+        if (pencil) {  // the wrapper books the pencil source as E0 * W (gpu_transport_wrapper.cu:328-333)
+            Wi = (double)w;
+            Ei = (double)E0 * (double)w;
+            Wo = Ws = Eo = Es = 0.0;
+        }
+        acc[0] = Wi;
+        acc[1] = Wo;
+        acc[2] = Ws;
+        acc[3] = Ei;
+        acc[4] = Eo;
+        acc[5] = Es;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Stage helpers for tests: load / dump of psi_in rows
+// ------------------------------------------------------------------------------------------
+__global__ void k_math_eval(int which, const float* x, float* y, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v = x[i];
+    switch (which) {
+        case 0: y[i] = expf_(v); break;
+        case 1: y[i] = logf_(v); break;
+        case 2: y[i] = sinf_(v); break;
+        case 3: y[i] = cosf_(v); break;
+        default: y[i] = erff_(v); break;
+    }
+}
+
+}  // namespace sm2d

@@ -1,0 +1,253 @@
+"""Host-side driver of a transport-step library through its C ABI.
+
+`Engine` wraps one context of a library that exports the entry points of
+include/sm2d_hip.h (the HIP library, prefix ``sm2d_``).  It mirrors the reference's
+boundary `sm_2d::run_k1k6_pipeline_transport` (src/cuda/gpu_transport_wrapper.cu:207-641):
+grids and the R-LUT are built on the host exactly as the reference runner does
+(src/gpu/gpu_transport_runner.cpp:63-98,145), uploaded once, and spots are run against the
+persistent context.  The same class drives the CPU oracle in the tests (with another
+prefix), so parity tests call identical methods on both.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import abi
+
+DEFAULT_ENERGY_GROUPS = ((0.1, 2.0, 0.1), (2.0, 20.0, 0.2), (20.0, 100.0, 0.25),
+                         (100.0, 250.0, 0.25))
+
+
+def _f(a):
+    return a.ctypes.data_as(abi.fp)
+
+
+def _d(a):
+    return a.ctypes.data_as(abi.dp)
+
+
+class LibError(RuntimeError):
+    pass
+
+
+@dataclass
+class Tables:
+    """Phase-space grids and R-LUT of one problem (all float32 host arrays)."""
+    theta_edges: np.ndarray
+    E_edges: np.ndarray
+    E_rep: np.ndarray
+    R: np.ndarray
+    S: np.ndarray
+    log_E: np.ndarray
+    log_R: np.ndarray
+    log_S: np.ndarray
+
+
+class Library:
+    """A loaded C-ABI library (HIP product or oracle) with bound prototypes."""
+
+    def __init__(self, path: str, prefix: str, extra_protos: dict | None = None):
+        self.path = path
+        self.prefix = prefix
+        self.lib = C.CDLL(path)
+        protos = dict(abi.COMMON_PROTOS)
+        protos["ctx_create"] = ([C.c_int, C.POINTER(abi.Grid), C.POINTER(C.c_void_p)]
+                                if prefix == "sm2d_" else
+                                [C.POINTER(abi.Grid), C.POINTER(C.c_void_p)])
+        protos["math_eval"] = [C.c_int, abi.fp, abi.fp, C.c_int]
+        if extra_protos:
+            protos.update(extra_protos)
+        abi.bind(self.lib, prefix, protos)
+        self.lib[prefix + "last_error"].restype = C.c_char_p
+        self.lib[prefix + "last_error"].argtypes = []
+
+    def call(self, name: str, *args) -> int:
+        rc = getattr(self.lib, self.prefix + name)(*args)
+        if rc != abi.OK:
+            msg = self.lib[self.prefix + "last_error"]().decode(errors="replace")
+            raise LibError(f"{self.prefix}{name} failed ({rc}): {msg}")
+        return rc
+
+    # ---- host table builders ---------------------------------------------------------------
+    def energy_grid_log(self, E_min: float, E_max: float, N_E: int):
+        edges = np.zeros(N_E + 1, np.float32)
+        rep = np.zeros(N_E, np.float32)
+        self.call("energy_grid_log", E_min, E_max, N_E, _f(edges), _f(rep))
+        return edges, rep
+
+    def energy_grid_piecewise(self, groups=DEFAULT_ENERGY_GROUPS):
+        g = np.asarray(groups, np.float32).reshape(-1)
+        n = C.c_int(0)
+        self.call("energy_grid_piecewise", _f(g), len(groups), None, None, C.byref(n))
+        edges = np.zeros(n.value + 1, np.float32)
+        rep = np.zeros(n.value, np.float32)
+        self.call("energy_grid_piecewise", _f(g), len(groups), _f(edges), _f(rep), C.byref(n))
+        return edges, rep
+
+    def angular_grid(self, theta_min: float, theta_max: float, N_theta: int):
+        e = np.zeros(N_theta + 1, np.float32)
+        self.call("angular_grid", theta_min, theta_max, N_theta, _f(e))
+        return e
+
+    def build_rlut(self, rep: np.ndarray):
+        n = len(rep)
+        out = [np.zeros(n, np.float32) for _ in range(5)]
+        rep = np.ascontiguousarray(rep, np.float32)
+        self.call("build_rlut", _f(rep), n, *[_f(o) for o in out])
+        return out
+
+    def fe_precompute(self, E0: float, sigma_x0: float, Nz: int, dz: float):
+        A, B, st = (np.zeros(Nz, np.float32) for _ in range(3))
+        self.call("fe_precompute", E0, sigma_x0, Nz, dz, _f(A), _f(B), _f(st))
+        return A, B, st
+
+    def post_convolve(self, edep: np.ndarray, dx: float, sigma_total: np.ndarray,
+                      sigma_x0: float) -> np.ndarray:
+        e = np.ascontiguousarray(edep, np.float64).copy()
+        Nz, Nx = e.shape
+        st = np.ascontiguousarray(sigma_total, np.float32)
+        self.call("post_convolve", _d(e), Nx, Nz, dx, _f(st), sigma_x0)
+        return e
+
+    def math_eval(self, which: int, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros_like(x)
+        self.call("math_eval", which, _f(x), _f(y), len(x))
+        return y
+
+    def make_tables(self, N_theta: int, theta_min: float, theta_max: float,
+                    energy: str | tuple = "piecewise", groups=DEFAULT_ENERGY_GROUPS) -> Tables:
+        """Grids + R-LUT as the reference runner builds them (gpu_transport_runner.cpp:89-98,145).
+
+        energy="piecewise" uses `groups`; energy=("log", E_min, E_max, N_E) a log grid."""
+        th = self.angular_grid(theta_min, theta_max, N_theta)
+        if energy == "piecewise":
+            E_edges, rep = self.energy_grid_piecewise(groups)
+        else:
+            _, emin, emax, ne = energy
+            E_edges, rep = self.energy_grid_log(emin, emax, ne)
+        R, S, lE, lR, lS = self.build_rlut(rep)
+        return Tables(th, E_edges, rep, R, S, lE, lR, lS)
+
+
+class Engine:
+    """One context (grid + tables) of a Library."""
+
+    def __init__(self, lib: Library, grid: abi.Grid, tables: Tables, device: int = 0):
+        self.L = lib
+        self.grid = grid
+        self.tables = tables
+        self.N = grid.Nx * grid.Nz
+        self.LB = grid.local_bins
+        h = C.c_void_p()
+        if lib.prefix == "sm2d_":
+            lib.call("ctx_create", device, C.byref(grid), C.byref(h))
+        else:
+            lib.call("ctx_create", C.byref(grid), C.byref(h))
+        self.h = h
+        t = tables
+        self._keep = [np.ascontiguousarray(a, np.float32) for a in
+                      (t.theta_edges, t.E_edges, t.R, t.S, t.log_E, t.log_R, t.log_S)]
+        lib.call("lut_upload", self.h, *[_f(a) for a in self._keep])
+        lib.call("fe_upload", self.h, None, None)
+
+    def close(self):
+        if self.h:
+            self.L.call("ctx_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- configuration -------------------------------------------------------------------
+    def upload_fe(self, A, B):
+        if A is None:
+            self.L.call("fe_upload", self.h, None, None)
+            return
+        self._fe = (np.ascontiguousarray(A, np.float32), np.ascontiguousarray(B, np.float32))
+        self.L.call("fe_upload", self.h, _f(self._fe[0]), _f(self._fe[1]))
+
+    def upload_material(self, rho=None, x0_ratio=None):
+        if rho is None:
+            self.L.call("material_upload", self.h, None, None)
+            return
+        self._mat = (np.ascontiguousarray(rho, np.float32),
+                     np.ascontiguousarray(x0_ratio, np.float32))
+        self.L.call("material_upload", self.h, _f(self._mat[0]), _f(self._mat[1]))
+
+    # ---- stages ------------------------------------------------------------------------------
+    def reset(self):
+        self.L.call("reset", self.h)
+
+    def inject(self, spot: abi.Spot) -> abi.Audit:
+        a = abi.Audit()
+        self.L.call("inject", self.h, C.byref(spot), C.byref(a))
+        return a
+
+    def psi_load(self, rows: dict):
+        n = len(rows["cell"])
+        cols = dict(cell=np.int32, slot=np.int32, bid=np.uint32, lidx=np.int32, w=np.float32)
+        arr = {k: np.ascontiguousarray(rows[k], t) for k, t in cols.items()}
+        self.L.call("psi_load", self.h, n, arr["cell"].ctypes.data_as(abi.i32p),
+                    arr["slot"].ctypes.data_as(abi.i32p), arr["bid"].ctypes.data_as(abi.u32p),
+                    arr["lidx"].ctypes.data_as(abi.i32p), _f(arr["w"]))
+
+    def psi_dump(self, capacity: int = 1 << 20):
+        bids = np.zeros(self.N * abi.KB, np.uint32)
+        out = dict(cell=np.zeros(capacity, np.int32), slot=np.zeros(capacity, np.int32),
+                   bid=np.zeros(capacity, np.uint32), lidx=np.zeros(capacity, np.int32),
+                   w=np.zeros(capacity, np.float32))
+        n = C.c_int(0)
+        self.L.call("psi_dump", self.h, bids.ctypes.data_as(abi.u32p), capacity,
+                    out["cell"].ctypes.data_as(abi.i32p), out["slot"].ctypes.data_as(abi.i32p),
+                    out["bid"].ctypes.data_as(abi.u32p), out["lidx"].ctypes.data_as(abi.i32p),
+                    _f(out["w"]), C.byref(n))
+        if n.value > capacity:
+            return self.psi_dump(n.value)
+        rows = {k: v[: n.value].copy() for k, v in out.items()}
+        return bids.reshape(self.N, abi.KB), rows
+
+    def mask_compact(self, tp: abi.Transport, mask_prev=None):
+        mask = np.zeros(self.N, np.uint8)
+        act = np.zeros(self.N, np.int32)
+        crs = np.zeros(self.N, np.int32)
+        na, nc = C.c_int(0), C.c_int(0)
+        mp = None
+        if mask_prev is not None:
+            self._mp = np.ascontiguousarray(mask_prev, np.uint8)
+            mp = self._mp.ctypes.data_as(abi.u8p)
+        self.L.call("k1k2_mask_compact", self.h, C.byref(tp), mp, mask.ctypes.data_as(abi.u8p),
+                    act.ctypes.data_as(abi.i32p), C.byref(na), crs.ctypes.data_as(abi.i32p),
+                    C.byref(nc))
+        return mask, act[: na.value].copy(), crs[: nc.value].copy()
+
+    def run_iterations(self, tp: abi.Transport, n_iter: int, include_source_terms: bool = False):
+        t = np.zeros((7, self.N), np.float64)
+        k5 = abi.K5Report()
+        a = abi.Audit()
+        self.L.call("run_iterations", self.h, C.byref(tp), n_iter, int(include_source_terms),
+                    _d(t), C.byref(k5), C.byref(a))
+        return t, k5, a
+
+    def k5_audit(self, tallies_delta, W_src_out=0.0, W_src_slot=0.0, E_src_out=0.0,
+                 E_src_slot=0.0, W_drop=0.0, E_drop=0.0, include_source_terms=False):
+        td = np.ascontiguousarray(tallies_delta, np.float64)
+        r = abi.K5Report()
+        self.L.call("k5_audit", self.h, _d(td), W_src_out, W_src_slot, E_src_out, E_src_slot,
+                    W_drop, E_drop, int(include_source_terms), C.byref(r))
+        return r
+
+    def swap(self):
+        self.L.call("swap", self.h)
+
+    def run_spot(self, spot: abi.Spot, tp: abi.Transport):
+        edep = np.zeros((self.grid.Nz, self.grid.Nx), np.float64)
+        a = abi.Audit()
+        self.L.call("run_spot", self.h, C.byref(spot), C.byref(tp), _d(edep), C.byref(a))
+        return edep, a

@@ -1,0 +1,26 @@
+"""Loader of the HIP library (libsm2d_hip.so) with a loud failure when it is unusable."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from . import abi
+from .engine import Library
+
+_LIB = None
+
+
+def load_library() -> Library:
+    """Loads the in-tree HIP library.  There is no CPU fallback: a missing library or a
+    missing GPU raises."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(abi.LIB_PATH):
+        raise RuntimeError(f"{abi.LIB_PATH} is missing; build it with `python -m sm_2d_amd.build`")
+    lib = Library(abi.LIB_PATH, "sm2d_")
+    lib.lib.sm2d_abi_version.restype = C.c_int
+    if lib.lib.sm2d_abi_version() != 1:
+        raise RuntimeError("libsm2d_hip.so ABI version mismatch")
+    _LIB = lib
+    return lib

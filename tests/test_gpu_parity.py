@@ -1,0 +1,163 @@
+"""GPU parity: the HIP library against the CPU oracle, through the C ABI.
+
+The bar is bit-exactness for everything the transport step computes (block ids, masks,
+compacted lists, phase-space weights, per-cell FP64 tallies): both sides evaluate the same
+binary32 expressions in the same order with the same elementary functions.  K5 report sums
+are FP64 reductions whose order differs (the oracle sums serially, the GPU with fixed trees),
+so they are compared to 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+from sm_2d_amd import abi, problems
+from sm_2d_amd.engine import Engine
+
+import fixtures as fx
+
+pytestmark = pytest.mark.gpu
+
+
+def _rows_key(rows):
+    return sorted(zip(rows["cell"].tolist(), rows["slot"].tolist(), rows["bid"].tolist(),
+                      rows["lidx"].tolist(), rows["w"].view(np.uint32).tolist()))
+
+
+def _pair(hip, orc, prob):
+    from oracle.pyoracle import OracleEngine
+    g = problems.setup_engine(hip, prob, Engine)
+    o = problems.setup_engine(orc, prob, OracleEngine)
+    return g, o
+
+
+def test_math_functions_bit_exact(hip, orc):
+    rng = np.random.default_rng(1)
+    ranges = {0: (-80, 80), 1: (1e-12, 1e6), 2: (-7, 7), 3: (-7, 7), 4: (-7, 7)}
+    for which, (a, b) in ranges.items():
+        if which == 1:
+            x = np.exp(rng.uniform(np.log(a), np.log(b), 200_000)).astype(np.float32)
+        else:
+            x = rng.uniform(a, b, 200_000).astype(np.float32)
+        y_gpu = hip.math_eval(which, x)
+        y_cpu = orc.math_eval(which, x)
+        assert np.array_equal(y_gpu.view(np.uint32), y_cpu.view(np.uint32)), which
+
+
+@pytest.mark.parametrize("case", ["A", "B", "C", "D"])
+def test_injection_and_one_iteration_bit_exact(hip, orc, case):
+    g_, tp, spot = fx.problem(case)
+    prob = problems.Problem(f"case{case}", g_, "piecewise",
+                            dict(E_fine_on=tp.E_fine_on, E_fine_off=tp.E_fine_off, max_iterations=1),
+                            [dict(x0=spot.x0, z0=spot.z0, theta0=spot.theta0, E0=spot.E0,
+                                  W_total=spot.W_total, sigma_x=spot.sigma_x,
+                                  sigma_theta=spot.sigma_theta, sigma_E=spot.sigma_E,
+                                  n_samples=spot.n_samples, random_seed=spot.random_seed)])
+    g, o = _pair(hip, orc, prob)
+    for e in (g, o):
+        e.reset()
+    ag, ao = g.inject(spot), o.inject(spot)
+    bg, rg = g.psi_dump()
+    bo, ro = o.psi_dump()
+    assert np.array_equal(bg, bo)
+    assert _rows_key(rg) == _rows_key(ro)
+    assert ag.W_source_injected == ao.W_source_injected
+    assert abs(ag.E_representation_loss - ao.E_representation_loss) <= 1e-9
+    tpx = prob.make_transport(sigma_x_initial=spot.sigma_x)
+    tg, kg, aug = g.run_iterations(tpx, 1, include_source_terms=True)
+    to, ko, auo = o.run_iterations(tpx, 1, include_source_terms=True)
+    bg, rg = g.psi_dump()
+    bo, ro = o.psi_dump()
+    assert np.array_equal(bg, bo)
+    assert _rows_key(rg) == _rows_key(ro)
+    assert np.array_equal(tg, to)
+    for k in ("W_in", "W_out", "E_in", "E_out", "E_dep", "W_nuclear", "E_nuclear"):
+        a, b = getattr(kg, k), getattr(ko, k)
+        assert abs(a - b) <= 1e-12 * max(1.0, abs(b)), (k, a, b)
+    assert kg.W_pass == ko.W_pass == 1
+    assert aug.cell_bin_updates == auo.cell_bin_updates
+
+
+def _random_psi(rng, N, LB, NE_blocks, NT_blocks, n_cells=200, max_slots=6):
+    cells = rng.choice(N, size=n_cells, replace=False)
+    rows = {k: [] for k in ("cell", "slot", "bid", "lidx", "w")}
+    for c in sorted(cells.tolist()):
+        ns = int(rng.integers(1, max_slots + 1))
+        bids = set()
+        while len(bids) < ns:
+            bids.add(int(rng.integers(0, NT_blocks)) | (int(rng.integers(0, NE_blocks)) << 12))
+        for s, b in enumerate(sorted(bids)):
+            nl = int(rng.integers(1, 12))
+            for l in rng.choice(LB, size=nl, replace=False).tolist():
+                w = float(10 ** rng.uniform(-14, -2))
+                rows["cell"].append(c)
+                rows["slot"].append(s)
+                rows["bid"].append(b)
+                rows["lidx"].append(l)
+                rows["w"].append(w)
+    return {k: np.asarray(v) for k, v in rows.items()}
+
+
+@pytest.mark.parametrize("layout", [problems.LAYOUT_REF, problems.LAYOUT_CFG])
+def test_mask_compact_random_bit_exact(hip, orc, layout):
+    prob = problems.reference_grid_case(Nx=60, Nz=50, layout=layout)
+    g, o = _pair(hip, orc, prob)
+    LB = g.LB
+    rng = np.random.default_rng(3)
+    NEb = (g.grid.N_E + g.grid.n_e_local - 1) // g.grid.n_e_local
+    NTb = g.grid.N_theta // g.grid.n_theta_local
+    rows = _random_psi(rng, g.N, LB, NEb, NTb)
+    prev = (rng.random(g.N) < 0.3).astype(np.uint8)
+    for wmin in (1e-12, 1e-6):
+        tp = prob.make_transport(weight_active_min=wmin)
+        for e in (g, o):
+            e.psi_load(rows)
+        mg, ag, cg = g.mask_compact(tp, prev)
+        mo, ao, co = o.mask_compact(tp, prev)
+        assert np.array_equal(mg, mo)
+        assert np.array_equal(ag, ao)
+        assert np.array_equal(cg, co)
+        assert np.all(np.diff(ag) > 0) and np.all(np.diff(cg) > 0)
+
+
+@pytest.mark.parametrize("layout", [problems.LAYOUT_REF, problems.LAYOUT_CFG])
+def test_random_psi_iterations_bit_exact(hip, orc, layout):
+    """Random phase space (several blocks per cell, all energies) through 3 iterations: exercises
+    both transports, slot claims, bucket emission to all faces and the K4 gather."""
+    prob = problems.reference_grid_case(Nx=40, Nz=40, layout=layout)
+    g, o = _pair(hip, orc, prob)
+    rng = np.random.default_rng(11)
+    NEb = (g.grid.N_E + g.grid.n_e_local - 1) // g.grid.n_e_local
+    NTb = g.grid.N_theta // g.grid.n_theta_local
+    rows = _random_psi(rng, g.N, g.LB, NEb, NTb, n_cells=120, max_slots=8)
+    tp = prob.make_transport(sigma_x_initial=2.0)
+    for e in (g, o):
+        e.psi_load(rows)
+    tg, kg, aug = g.run_iterations(tp, 3)
+    to, ko, auo = o.run_iterations(tp, 3)
+    bg, rg = g.psi_dump()
+    bo, ro = o.psi_dump()
+    assert np.array_equal(bg, bo)
+    assert _rows_key(rg) == _rows_key(ro)
+    assert np.array_equal(tg, to)
+    assert aug.cell_bin_updates == auo.cell_bin_updates
+    assert aug.unsupported_events == 0
+
+
+@pytest.mark.parametrize("prob", [
+    problems.reference_grid_case(Nx=60, Nz=100, E0=70.0, sigma_x=3.0, n_samples=500),
+    problems.config1(n_samples=500),
+], ids=["refgrid70_LB256", "config1_LB128"])
+def test_full_spot_bit_exact(hip, orc, prob):
+    g, o = _pair(hip, orc, prob)
+    sp = prob.make_spot()
+    tp = prob.make_transport()
+    eg, ag = g.run_spot(sp, tp)
+    eo, ao = o.run_spot(sp, tp)
+    assert ag.terminated == ao.terminated == 1
+    assert ag.iterations == ao.iterations
+    assert np.array_equal(eg, eo)
+    assert ag.cell_bin_updates == ao.cell_bin_updates
+    assert ag.processed_components == ao.processed_components
+    for k in ("E_dep", "W_nuclear", "E_nuclear", "W_boundary", "E_boundary", "W_cutoff", "E_cutoff"):
+        assert getattr(ag, k) == pytest.approx(getattr(ao, k), rel=1e-12, abs=1e-15), k
+    assert ag.max_cell_W_closure < 1e-6
+    assert ag.W_fail_iters == 0

@@ -1,0 +1,42 @@
+"""The project's elementary functions (sm_2d_amd/csrc/sm2d_math.h) against numpy's binary64
+functions rounded once to binary32: they must agree on (practically) every input — a wrong
+polynomial coefficient, reduction constant or quadrant would show as mass disagreement."""
+import numpy as np
+import pytest
+
+
+@pytest.mark.parametrize("which,fn,lo,hi,log", [
+    (0, np.exp, -85.0, 85.0, False),
+    (1, np.log, 1e-30, 1e30, True),
+    (2, np.sin, -7.0, 7.0, False),
+    (3, np.cos, -7.0, 7.0, False),
+    (4, None, -7.0, 7.0, False),
+])
+def test_elementary_functions_correctly_rounded(orc, which, fn, lo, hi, log):
+    rng = np.random.default_rng(which)
+    if log:
+        x = np.exp(rng.uniform(np.log(lo), np.log(hi), 300_000)).astype(np.float32)
+    else:
+        x = rng.uniform(lo, hi, 300_000).astype(np.float32)
+    y = orc.math_eval(which, x)
+    if which == 4:
+        from scipy.special import erf
+        ref = erf(x.astype(np.float64)).astype(np.float32)
+    else:
+        ref = fn(x.astype(np.float64)).astype(np.float32)
+    mism = np.count_nonzero(y.view(np.uint32) != ref.view(np.uint32))
+    assert mism <= 3, mism  # only inputs within ~1e-16 of a rounding boundary may differ
+    ulp = np.abs(y.view(np.int32).astype(np.int64) - ref.view(np.int32).astype(np.int64))
+    assert ulp.max() <= 1
+
+
+def test_special_values(orc):
+    x = np.array([0.0, -0.0, 1.0, np.float32(np.pi / 2), 1e-20], np.float32)
+    assert orc.math_eval(0, np.array([0.0], np.float32))[0] == 1.0
+    assert orc.math_eval(1, np.array([1.0], np.float32))[0] == 0.0
+    assert orc.math_eval(2, np.array([0.0], np.float32))[0] == 0.0
+    assert orc.math_eval(3, np.array([0.0], np.float32))[0] == 1.0
+    assert orc.math_eval(4, np.array([0.0, 6.5, -9.0], np.float32)).tolist() == [0.0, 1.0, -1.0]
+    assert np.isneginf(orc.math_eval(1, np.array([0.0], np.float32))[0])
+    assert orc.math_eval(0, np.array([-200.0], np.float32))[0] == 0.0
+    del x
