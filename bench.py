@@ -1,0 +1,159 @@
+"""Throughput of the K1-K6 transport step on one MI355X.
+
+A step is one complete pencil-beam spot of the benchmark workload (injection, every K1..K6
+iteration until the phase space is empty, EdepC read-back) through the HIP library.  The
+workload is the 150 MeV water spot on 200 x 400 cells of 0.5 mm with 256 log-spaced energy
+bins (0.1-250 MeV), 512 angular bins over [-pi/2, pi/2] and 128 local bins per block;
+inputs are synthetic (the seeded Gaussian source of the reference injector).
+
+Reported: cell-bin updates/s (value), spots/s, ms per spot, the roofline of the dominant
+kernel (transport, HBM-bound: 8 B per cell-bin update = one FP32 read of psi_in + one FP32
+write of psi_out, divided by the kernel time from HIP events on the library's stream) and the
+CPU oracle timed on the same host on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _profiles_traffic(name: str):
+    """HBM bytes per transport launch measured by rocprofv3 counters, when a summary exists."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].strip())
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(prob, budget_s: float = 20.0) -> dict:
+    """The oracle (a serial C++ port of the reference step) on the same spot.  The spot is run
+    with a bounded iteration count chosen from a short probe so the sample takes ~budget_s."""
+    from oracle import pyoracle
+    from sm_2d_amd import problems
+
+    orc = pyoracle.load()
+    eng = problems.setup_engine(orc, prob, pyoracle.OracleEngine)
+    sp = prob.make_spot()
+    probe_iters = 20
+    t0 = time.perf_counter()
+    _, a = eng.run_spot(sp, prob.make_transport(max_iterations=probe_iters))
+    dt = time.perf_counter() - t0
+    full = prob.transport["max_iterations"]
+    iters = full if dt <= 0 else int(min(full, max(probe_iters, probe_iters * budget_s / dt)))
+    t0 = time.perf_counter()
+    _, a = eng.run_spot(sp, prob.make_transport(max_iterations=iters))
+    dt = time.perf_counter() - t0
+    eng.close()
+    return {"value": a.cell_bin_updates / dt, "unit": "cell-bin updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"first {a.iterations} iterations of the same spot ({a.cell_bin_updates} "
+                      f"cell-bin updates in {dt:.2f} s, serial C++ oracle, -O2)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (spots shard "
+              f"across GPUs by a longest-first schedule, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.zeros(1, device="cuda:0")
+    from sm_2d_amd import problems, runtime
+    from sm_2d_amd.engine import Engine
+
+    prob = problems.config2()
+    hip = runtime.load_library()
+    eng = problems.setup_engine(hip, prob, Engine)
+    sp, tp = prob.make_spot(), prob.make_transport()
+    for _ in range(args.warmup):
+        eng.run_spot(sp, tp)
+    eng.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    updates = 0
+    audits = []
+    for _ in range(args.steps):
+        _, a = eng.run_spot(sp, tp)
+        updates += a.cell_bin_updates
+        audits.append(a)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    tm = eng.timing()
+    a = audits[-1]
+    achieved = 8.0 * tm.cell_bin_updates / (tm.transport_ms * 1e-3) / 1e9 if tm.transport_ms > 0 else 0.0
+    traffic = _profiles_traffic("transport_config2.txt")
+    out = {
+        "metric": "cell-bin updates/s + spots/s (batched+surrogate vs plain) at <=1% dose diff",
+        "value": updates / dt,
+        "unit": "cell-bin updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Gaussian source, water phantom)",
+        "config": {
+            "workload": "plain K1-K6, one 150 MeV spot (sigma_x 3.8 mm, 1% sigma_E, 2 mrad), "
+                        "water 200x400 cells @ 0.5 mm, 256 log E bins x 512 theta bins, "
+                        "128 local bins (8 theta x 4 E x 4 x_sub)",
+            "path": "plain",
+            "spots_per_step": 1,
+            "iterations_per_spot": a.iterations,
+            "cell_bin_updates_per_spot": a.cell_bin_updates,
+        },
+        "spots_per_s": args.steps / dt,
+        "ms_per_spot": dt / args.steps * 1e3,
+        "stage_ms_per_spot": {k: getattr(tm, k) / args.steps for k in
+                              ("mask_ms", "transport_ms", "gather_ms", "audit_ms")},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_transport",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_update": 8,
+            "launches": tm.transport_launches,
+            "avg_launch_us": tm.transport_ms * 1e3 / max(1, tm.transport_launches),
+        },
+        "audit": {"max_cell_W_closure": a.max_cell_W_closure, "W_fail_iters": a.W_fail_iters,
+                  "E_fail_iters": a.E_fail_iters, "terminated": a.terminated,
+                  "E_boundary_MeV": a.E_boundary, "E_dep_MeV": a.E_dep},
+    }
+    eng.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(prob, args.cpu_budget)
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -111,6 +111,16 @@ typedef struct sm2d_k5_report {
            E_source_slot_drop, E_transport_drop;
 } sm2d_k5_report;
 
+/* Kernel timing of a context, from HIP events recorded on the context's stream around each
+ * stage (no counterpart in the reference, which has no kernel timing).  Times in ms. */
+typedef struct sm2d_timing {
+    uint32_t struct_size;
+    int enabled;
+    uint64_t iterations, transport_launches, gather_launches;
+    double mask_ms, transport_ms, gather_ms, audit_ms;
+    uint64_t cell_bin_updates;   /* updates done by the timed transport launches */
+} sm2d_timing;
+
 typedef struct sm2d_ctx sm2d_ctx;
 
 const char* sm2d_last_error(void);
@@ -169,6 +179,22 @@ int sm2d_material_upload(sm2d_ctx* ctx, const float* rho, const float* x0_ratio)
 int sm2d_run_spot(sm2d_ctx* ctx, const sm2d_spot* spot, const sm2d_transport* tp,
                   double* edep_out, sm2d_audit* audit);
 
+/* --- batched spots ------------------------------------------------------------------------ */
+
+/* Creates a context holding `n_spots` independent copies of the grid (spot s owns cells
+ * [s*Nx*Nz, (s+1)*Nx*Nz)).  Spots of one batch are transported together: one launch per
+ * stage covers every spot's listed cells.  No counterpart in the reference, whose runner
+ * calls run_k1k6_pipeline_transport once per spot (src/gpu/gpu_transport_runner.cpp:28-213);
+ * each spot's result is bit-identical to sm2d_run_spot on the same spot. */
+int sm2d_ctx_create_batch(int device, const sm2d_grid* grid, int n_spots, sm2d_ctx** out);
+
+/* Runs spots[0..n) (n <= the context's spot count) to completion in one batch.
+ * edep_sum[Nz*Nx] (FP64, may be NULL) receives the sum over spots in spot order;
+ * edep_per_spot[n*Nz*Nx] (FP32, may be NULL) each spot's EdepC; audit (may be NULL) the
+ * batch totals (iterations = the batch's iteration count). */
+int sm2d_run_batch(sm2d_ctx* ctx, const sm2d_spot* spots, int n, const sm2d_transport* tp,
+                   double* edep_sum, float* edep_per_spot, sm2d_audit* audit);
+
 /* --- stage-level entry points (used by the parity tests) -------------------------------- */
 
 /* Clears both phase-space buffers and all tallies. */
@@ -207,6 +233,10 @@ int sm2d_k5_audit(sm2d_ctx* ctx, const double* tallies_delta, double W_src_out, 
                   int include_source_terms, sm2d_k5_report* out);
 /* Copies psi_in <-> psi_out (load the second buffer for sm2d_k5_audit). */
 int sm2d_swap(sm2d_ctx* ctx);
+
+/* Turns stage timing on (1) or off (0) and zeroes the counters; sm2d_get_timing reads them. */
+int sm2d_set_profiling(sm2d_ctx* ctx, int on);
+int sm2d_get_timing(sm2d_ctx* ctx, sm2d_timing* out);
 
 /* --- self-test of the shared elementary functions on the device -------------------------- */
 /* Evaluates f(x[i]) for i < n on the GPU: which = 0 exp, 1 log, 2 sin, 3 cos, 4 erf. */

@@ -1720,6 +1720,14 @@ int orc_run_spot(orc_ctx* c, const sm2d_spot* sp, const sm2d_transport* tp, doub
     std::memset(au, 0, sizeof(*au));
     au->struct_size = sizeof(*au);
     orc_reset(c);
+    // FE moments of this spot (init_pipeline_state, k1k6_pipeline.cu:1061-1212)
+    {
+        std::vector<float> st(c->g.Nz);
+        c->feA.assign(c->g.Nz, 0.0f);
+        c->feB.assign(c->g.Nz, 0.0f);
+        int r0 = orc_fe_precompute(sp->E0, sp->sigma_x, c->g.Nz, c->g.dz, c->feA.data(), c->feB.data(), st.data());
+        if (r0) return r0;
+    }
     int rc = orc_inject(c, sp, au);
     if (rc) return rc;
     rc = run_iters(c, tp, sp->sigma_x, tp->max_iterations, 1, nullptr, au);

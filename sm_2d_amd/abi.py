@@ -133,6 +133,24 @@ class K5Report(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "struct_size"}
 
 
+class Timing(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32),
+        ("enabled", C.c_int),
+        ("iterations", C.c_uint64), ("transport_launches", C.c_uint64),
+        ("gather_launches", C.c_uint64),
+        ("mask_ms", C.c_double), ("transport_ms", C.c_double), ("gather_ms", C.c_double),
+        ("audit_ms", C.c_double),
+        ("cell_bin_updates", C.c_uint64),
+    ]
+
+    def __init__(self):
+        super().__init__(struct_size=C.sizeof(Timing))
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "struct_size"}
+
+
 P = C.c_void_p
 fp = C.POINTER(C.c_float)
 dp = C.POINTER(C.c_double)

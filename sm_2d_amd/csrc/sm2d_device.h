@@ -39,22 +39,32 @@ struct Layout {
 using LayoutRef = Layout<4, 2, 8, 4>;   // the reference build (LOCAL_BINS = 256)
 using LayoutCfg = Layout<8, 4, 4, 1>;   // 128 local bins: 8 theta x 4 E x 4 x_sub
 
+// Cells of all spots of a batch are numbered cell = spot*Ns + iz*Nx + ix (Ns = Nx*Nz); the
+// neighbour tests below never cross a spot.
 struct Tabs {
     int Nx, Nz, N, NT, NE;
+    int Ns, S;         // cells per spot, spots
     float dx, dz, x_min, z_min;
     const float* te;   // theta edges [NT+1]
     const float* ee;   // energy edges [NE+1]
     const float* R;    // CSDA range [NE] (mm of water)
     const float* lR;
     const float* lE;
-    const float* feA;  // Fermi-Eyges A per row [Nz]
+    const float* feA;  // Fermi-Eyges A per spot and row [S*Nz]
     const float* feB;
+    const float* sx0;  // initial beam sigma_x per spot [S] (coarse moments)
     const float* rho;  // density per row [Nz]
     const float* x0r;  // radiation-length ratio per row [Nz]
 };
 
+// Device control block (u32 words).
+enum Ctl {
+    C_NACT = 0, C_NCRS = 1, C_NDEST = 2, C_DONE = 3, C_EP = 4, C_IT = 5, C_DONE_IT = 6,
+    C_NOCC0 = 8, C_NOCC1 = 9, NCTL = 16
+};
+
 struct Tp {
-    float E_on, E_off, wmin, step_coarse, sx0;
+    float E_on, E_off, wmin, step_coarse;
     int b_on, b_off, rule, strag, nuc;
 };
 

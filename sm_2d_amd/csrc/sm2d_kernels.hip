@@ -24,38 +24,56 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
-// Resets the slot tables of every occupied cell of a buffer and its payload pool.
-__global__ void k_clear(Psi p, int N) {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
-        uint32_t n = p.ns[c];
-        if (n) {
-            for (uint32_t s = 0; s < n; ++s) p.bid[(size_t)c * KB + s] = EMPTY;
-            p.ns[c] = 0;
-        }
+// Opens an iteration: epoch and iteration counters live on the device so that a chunk of
+// iterations can be queued (or replayed from a graph) without host round trips.
+__global__ void k_begin(uint32_t* ctl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !ctl[C_DONE]) {
+        ctl[C_EP] += 1;
+        ctl[C_IT] += 1;
+        ctl[C_NDEST] = 0;
+    }
+}
+
+// Resets the slot tables of the cells occupied in a buffer (its occupied-cell list) and the
+// buffer's payload pool.
+__global__ void k_clear(Psi p, const uint32_t* __restrict__ occ, const uint32_t* nocc, const uint32_t* ctl) {
+    if (ctl[C_DONE]) return;
+    const uint32_t n = *nocc;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t c = occ[i];
+        const uint32_t k = p.ns[c];
+        for (uint32_t s = 0; s < k; ++s) p.bid[(size_t)c * KB + s] = EMPTY;
+        p.ns[c] = 0;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *p.pcount = 0;
 }
 
 // ------------------------------------------------------------------------------------------
-// K1: mask + per-cell FP64 totals of psi_in.  One thread per cell, serial sum in slot->lidx
-// order so that the threshold test W > w_min sees the reference's exact FP32 sum.
-// flag: 1 = active (fine), 2 = coarse (mask 0 with weight), 0 = nothing.
+// K1: mask + per-cell FP64 totals of psi_in over its occupied-cell list (ascending).  One
+// thread per occupied cell, serial sum in slot->lidx order so that the threshold test
+// W > w_min sees the reference's exact FP32 sum.  Tiles of 256 list entries; per-entry flag:
+// 1 = active (fine), 2 = coarse (mask 0 with weight), 0 = nothing.
 // ------------------------------------------------------------------------------------------
 template <class LY>
-__global__ __launch_bounds__(256) void k_mask(Tabs T, Tp P, Psi in, uint32_t* maskv, uint32_t ep,
-                                              uint8_t* flag, uint32_t* bcount, double* win, double* ein) {
+__global__ __launch_bounds__(256) void k_mask(Tabs T, Tp P, Psi in, const uint32_t* __restrict__ occ,
+                                              const uint32_t* nocc, uint32_t* maskv, const uint32_t* ctl,
+                                              uint8_t* flag, uint32_t* tcount, double* win, double* ein) {
+    if (ctl[C_DONE]) return;
     __shared__ uint32_t s_cnt[2];
-    if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
-    __syncthreads();
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    uint8_t f = 0;
-    if (c < T.N) {
-        uint32_t n = in.ns[c];
-        double W64 = 0.0, E64 = 0.0;
-        if (n) {
+    const uint32_t n = *nocc, ep = ctl[C_EP];
+    const uint32_t ntile = (n + 255) / 256;
+    for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t i = tile * 256 + threadIdx.x;
+        uint8_t f = 0;
+        if (i < n) {
+            const uint32_t c = occ[i];
+            const uint32_t ns = in.ns[c];
+            double W64 = 0.0, E64 = 0.0;
             float W = 0.0f;
             bool on = false, off = false, has = false;
-            for (uint32_t s = 0; s < n; ++s) {
+            for (uint32_t s = 0; s < ns; ++s) {
                 uint32_t bid = in.bid[(size_t)c * KB + s];
                 if (bid == EMPTY) continue;
                 uint32_t bE = (bid >> 12) & 0xFFF;
@@ -73,84 +91,173 @@ __global__ __launch_bounds__(256) void k_mask(Tabs T, Tp P, Psi in, uint32_t* ma
                     }
                 }
             }
-            bool prev = maskv[c] == (((ep - 1u) << 1) | 1u);
-            bool m = (on || (prev && off)) && W > P.wmin;
-            maskv[c] = (ep << 1) | (m ? 1u : 0u);
-            f = m ? 1 : (has ? 2 : 0);
+            if (ns) {
+                bool prev = maskv[c] == (((ep - 1u) << 1) | 1u);
+                bool m = (on || (prev && off)) && W > P.wmin;
+                maskv[c] = (ep << 1) | (m ? 1u : 0u);
+                f = m ? 1 : (has ? 2 : 0);
+            }
+            flag[i] = f;
+            win[i] = W64;
+            ein[i] = E64;
         }
-        flag[c] = f;
-        win[c] = W64;
-        ein[c] = E64;
+        if (f) atomicAdd(&s_cnt[f - 1], 1u);  // block-local count; the order is irrelevant
+        __syncthreads();
+        if (threadIdx.x < 2) tcount[tile * 2 + threadIdx.x] = s_cnt[threadIdx.x];
+        __syncthreads();
     }
-    if (f) atomicAdd(&s_cnt[f - 1], 1u);  // block-local count; the order is irrelevant
-    __syncthreads();
-    if (threadIdx.x < 2) bcount[blockIdx.x * 2 + threadIdx.x] = s_cnt[threadIdx.x];
 }
 
-// Exclusive scan of the per-block counts by one workgroup (1024 threads), fixed order.
-__global__ __launch_bounds__(1024) void k_scan(uint32_t* bcount, int nblk, uint32_t* totals) {
-    __shared__ uint32_t s[2][1024];
-    __shared__ uint32_t carry[2];
-    if (threadIdx.x < 2) carry[threadIdx.x] = 0;
+// Exclusive scan of per-tile counts (K counters interleaved) by one workgroup, fixed order.
+// With K = 2 (mask compaction) the totals go to ctl[C_NACT], ctl[C_NCRS] and an empty result
+// ends the transport (C_DONE, C_DONE_IT); with K = 1 the total goes to *total1.
+template <int K>
+__global__ __launch_bounds__(1024) void k_scan(uint32_t* tcount, const uint32_t* n_items, uint32_t per_tile,
+                                               uint32_t* ctl, uint32_t* total1, uint32_t cap, uint32_t* flags) {
+    if (ctl[C_DONE]) {
+        if (K == 2 && threadIdx.x == 0) {
+            ctl[C_NACT] = 0;
+            ctl[C_NCRS] = 0;
+        }
+        return;
+    }
+    __shared__ uint32_t s[K][1024];
+    __shared__ uint32_t carry[K];
+    const int ntile = (int)((*n_items + per_tile - 1) / per_tile);
+    if (threadIdx.x < K) carry[threadIdx.x] = 0;
     __syncthreads();
-    for (int base = 0; base < nblk; base += 1024) {
-        int i = base + threadIdx.x;
-        for (int k = 0; k < 2; ++k) s[k][threadIdx.x] = (i < nblk) ? bcount[i * 2 + k] : 0;
+    for (int base = 0; base < ntile; base += 1024) {
+        const int i = base + threadIdx.x;
+        uint32_t own[K];
+        for (int k = 0; k < K; ++k) {
+            own[k] = (i < ntile) ? tcount[i * K + k] : 0;
+            s[k][threadIdx.x] = own[k];
+        }
         __syncthreads();
         for (int o = 1; o < 1024; o <<= 1) {
-            uint32_t a0 = threadIdx.x >= o ? s[0][threadIdx.x - o] : 0;
-            uint32_t a1 = threadIdx.x >= o ? s[1][threadIdx.x - o] : 0;
+            uint32_t a[K];
+            for (int k = 0; k < K; ++k) a[k] = threadIdx.x >= o ? s[k][threadIdx.x - o] : 0;
             __syncthreads();
-            s[0][threadIdx.x] += a0;
-            s[1][threadIdx.x] += a1;
+            for (int k = 0; k < K; ++k) s[k][threadIdx.x] += a[k];
             __syncthreads();
         }
-        for (int k = 0; k < 2; ++k) {
-            uint32_t incl = s[k][threadIdx.x];
-            uint32_t own = (i < nblk) ? bcount[i * 2 + k] : 0;
-            if (i < nblk) bcount[i * 2 + k] = carry[k] + incl - own;
-        }
+        for (int k = 0; k < K; ++k)
+            if (i < ntile) tcount[i * K + k] = carry[k] + s[k][threadIdx.x] - own[k];
         __syncthreads();
-        if (threadIdx.x == 0) {
-            carry[0] += s[0][1023];
-            carry[1] += s[1][1023];
-        }
+        if (threadIdx.x < K) carry[threadIdx.x] += s[threadIdx.x][1023];
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        totals[0] = carry[0];  // active (fine)
-        totals[1] = carry[1];  // coarse
+        // A list longer than its buffers stops the transport with a capacity flag, so no later
+        // kernel of the queued iterations reads past a buffer.
+        bool over = false;
+        for (int k = 0; k < K; ++k) over = over || carry[k] > cap;
+        if (over) atomicOr(&flags[F_DEST_FULL], 1u);
+        if (K == 2) {
+            ctl[C_NACT] = over ? 0u : carry[0];
+            ctl[C_NCRS] = over ? 0u : carry[1 % K];
+            if (over || (carry[0] == 0 && carry[1 % K] == 0)) {
+                ctl[C_DONE] = 1;
+                ctl[C_DONE_IT] = ctl[C_IT];
+            }
+        } else {
+            *total1 = over ? 0u : carry[0];
+            if (over) {
+                ctl[C_DONE] = 1;
+                ctl[C_DONE_IT] = ctl[C_IT];
+            }
+        }
     }
 }
 
-// Ordered scatter: cells of a block in ascending order; ranks from wave ballots.
-__global__ __launch_bounds__(256) void k_emit(int N, const uint8_t* flag, const uint32_t* boff,
-                                              uint32_t* act, uint32_t* crs, uint32_t* ltag,
-                                              uint32_t* lpos, uint32_t tag_c, uint32_t tag_f) {
+// Ordered scatter of the mask compaction: list entries of a tile in ascending order, ranks
+// from wave ballots.  Listed cells are marked occupied for the next iteration's list.
+__global__ __launch_bounds__(256) void k_emit(const uint32_t* __restrict__ occ, const uint32_t* nocc,
+                                              const uint8_t* flag, const uint32_t* toff, uint32_t* act,
+                                              uint32_t* crs, uint32_t* ltag, uint32_t* lpos, uint32_t* bitmap,
+                                              const uint32_t* ctl) {
+    if (ctl[C_DONE]) return;
     __shared__ uint32_t wcnt[2][4];
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    uint8_t f = c < N ? flag[c] : 0;
-    int w = threadIdx.x >> 6, l = lane_id();
-    uint64_t ba = __ballot(f == 1), bc = __ballot(f == 2);
-    uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
-    if (l == 0) {
-        wcnt[0][w] = __popcll(ba);
-        wcnt[1][w] = __popcll(bc);
-    }
-    __syncthreads();
-    if (f) {
-        int k = f - 1;
-        uint32_t pos = boff[blockIdx.x * 2 + k];
-        for (int j = 0; j < w; ++j) pos += wcnt[k][j];
-        pos += __popcll((k == 0 ? ba : bc) & below);
-        if (k == 0) {
-            act[pos] = (uint32_t)c;
-            ltag[c] = tag_f;
-        } else {
-            crs[pos] = (uint32_t)c;
-            ltag[c] = tag_c;
+    const uint32_t n = *nocc, ep = ctl[C_EP];
+    const uint32_t ntile = (n + 255) / 256;
+    const int w = threadIdx.x >> 6, l = lane_id();
+    const uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
+    for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const uint32_t i = tile * 256 + threadIdx.x;
+        const uint8_t f = i < n ? flag[i] : 0;
+        const uint64_t ba = __ballot(f == 1), bc = __ballot(f == 2);
+        if (l == 0) {
+            wcnt[0][w] = __popcll(ba);
+            wcnt[1][w] = __popcll(bc);
         }
-        lpos[c] = pos;
+        __syncthreads();
+        if (f) {
+            const int k = f - 1;
+            const uint32_t c = occ[i];
+            uint32_t pos = toff[tile * 2 + k];
+            for (int j = 0; j < w; ++j) pos += wcnt[k][j];
+            pos += __popcll((k == 0 ? ba : bc) & below);
+            if (k == 0) {
+                act[pos] = c;
+                ltag[c] = 2 * ep + 1;
+            } else {
+                crs[pos] = c;
+                ltag[c] = 2 * ep;
+            }
+            lpos[c] = pos;
+            atomicOr(&bitmap[c >> 5], 1u << (c & 31));
+        }
+        __syncthreads();
+    }
+}
+
+// Occupied-cell list of the buffer just written: per-tile popcounts of the occupancy bitmap
+// (256 words = 8192 cells per tile) ...
+__global__ __launch_bounds__(256) void k_occ_count(const uint32_t* __restrict__ bitmap, uint32_t nwords,
+                                                   uint32_t* tcount, const uint32_t* ctl) {
+    if (ctl[C_DONE]) return;
+    __shared__ uint32_t s;
+    const uint32_t ntile = (nwords + 255) / 256;
+    for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        if (threadIdx.x == 0) s = 0;
+        __syncthreads();
+        const uint32_t i = tile * 256 + threadIdx.x;
+        const uint32_t b = i < nwords ? bitmap[i] : 0u;
+        if (b) atomicAdd(&s, (uint32_t)__popc(b));
+        __syncthreads();
+        if (threadIdx.x == 0) tcount[tile] = s;
+        __syncthreads();
+    }
+}
+
+// ... and the ordered emission (ascending cells), clearing the bitmap as it goes.
+__global__ __launch_bounds__(256) void k_occ_emit(uint32_t* bitmap, uint32_t nwords, const uint32_t* toff,
+                                                  uint32_t* occ, uint32_t cap, const uint32_t* ctl) {
+    if (ctl[C_DONE]) return;
+    __shared__ uint32_t sc[256];
+    const uint32_t ntile = (nwords + 255) / 256;
+    for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const uint32_t i = tile * 256 + threadIdx.x;
+        uint32_t b = i < nwords ? bitmap[i] : 0u;
+        sc[threadIdx.x] = __popc(b);
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            uint32_t a = threadIdx.x >= (uint32_t)o ? sc[threadIdx.x - o] : 0;
+            __syncthreads();
+            sc[threadIdx.x] += a;
+            __syncthreads();
+        }
+        uint32_t pos = toff[tile] + sc[threadIdx.x] - __popc(b);
+        if (b) {
+            bitmap[i] = 0u;
+            while (b) {
+                const int k = __ffs(b) - 1;
+                b &= b - 1;
+                if (pos < cap) occ[pos] = i * 32 + (uint32_t)k;
+                ++pos;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -304,8 +411,10 @@ template <class LY, bool FINE>
 __global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out, Bkt bk,
                                                    const uint32_t* __restrict__ list,
                                                    const uint32_t* __restrict__ nlist,
-                                                   const uint32_t* __restrict__ ltag, uint32_t tag,
+                                                   const uint32_t* __restrict__ ltag, const uint32_t* ctl,
                                                    Tally tal, double* acc, uint32_t* flags) {
+    if (ctl[C_DONE]) return;
+    const uint32_t tag = 2 * ctl[C_EP] + (FINE ? 1u : 0u);
     const int l = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
@@ -316,7 +425,8 @@ __global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out
     const float dth = (tmax - tmin) / NT;
     for (uint32_t li = wave; li < n; li += nw) {
         const int cell = (int)list[li];
-        const int iz = cell / Nx, ix = cell % Nx;
+        const int sp = cell / T.Ns, lc = cell - sp * T.Ns;  // spot, cell within the spot
+        const int iz = lc / Nx, ix = lc - iz * Nx;
         const float rh = T.rho[iz];
         const float X0 = X0W * T.x0r[iz];
         WaveCell wc;
@@ -331,7 +441,8 @@ __global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out
         CellAccum a = {};
         Ctx<LY> cx{T, bk, li, flags, wc, a};
         const uint32_t ns = in.ns[cell];
-        const float A = T.feA[iz], B = T.feB[iz];
+        const float A = T.feA[sp * T.Nz + iz], B = T.feB[sp * T.Nz + iz];
+        const float sx0 = T.sx0[sp];
         for (uint32_t slot = 0; slot < ns; ++slot) {
             const uint32_t bid = in.bid[(size_t)cell * KB + slot];
             if (bid == EMPTY) continue;
@@ -387,7 +498,7 @@ __global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out
                             ds = fminf(max_step(T, E, rh, P.rule), sb * 1.001f);
                             float mdE = deposition(T, E, ds, rh);
                             if (P.strag) {
-                                unsigned sd = (unsigned)((cell * 7 + (int)slot * 13 + lidx * 17) ^ 0x5DEECE66DL);
+                                unsigned sd = (unsigned)((lc * 7 + (int)slot * 13 + lidx * 17) ^ 0x5DEECE66DL);
                                 float sE = straggling(E, ds, rh);
                                 dE = (sE <= 0.0f) ? mdE : fmaxf(mdE + gauss(sd) * sE, 0.0f);
                             } else {
@@ -446,7 +557,7 @@ __global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out
                             float st0 = (ps > 0.0f) ? highland(E, ps, X0) : 0.0f;
                             float Ao = st0 * st0;
                             float Bo = 0.5f * Ao * ps;
-                            float Co = P.sx0 * P.sx0 + (Ao * ps * ps) / 3.0f;
+                            float Co = sx0 * sx0 + (Ao * ps * ps) / 3.0f;
                             float Em = fmaxf(E - 0.5f * dE, T.ee[0]);
                             float Tp_ = scat_power(Em, smm, X0);
                             float An = fmaxf(Ao + Tp_ * smm, 0.0f);
@@ -461,7 +572,7 @@ __global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out
                             pseed = (unsigned)fminf(ps * 1000.0f, 4294967295.0f);
                             ds = cstep;
                         }
-                        unsigned ts = ((unsigned)(cell + 1) * 2654435761u) ^
+                        unsigned ts = ((unsigned)(lc + 1) * 2654435761u) ^
                                       ((unsigned)(slot + 1) * 2246822519u) ^
                                       ((unsigned)(lidx + 1) * 3266489917u) ^ (unsigned)(ebin + 17) ^
                                       (pseed * 747796405u);
@@ -817,12 +928,16 @@ __global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out
 
 // Destination set of a phase: the four neighbours of every listed cell, deduplicated with a
 // per-cell tag (the order of the resulting list does not affect any result).
-__global__ void k_dest(const uint32_t* list, const uint32_t* nlist, int Nx, int Nz, uint32_t* dtag,
-                       uint32_t tag, uint32_t* dlist, uint32_t* ndest, uint32_t cap, uint32_t* flags) {
+__global__ void k_dest(const uint32_t* list, const uint32_t* nlist, int Nx, int Nz, int Ns, uint32_t* dtag,
+                       const uint32_t* ctl, int fine, uint32_t* dlist, uint32_t* ndest, uint32_t cap,
+                       uint32_t* bitmap, uint32_t* flags) {
+    if (ctl[C_DONE]) return;
     const uint32_t n = *nlist;
+    const uint32_t tag = 2 * ctl[C_EP] + (uint32_t)fine;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int c = (int)list[i];
-        const int ix = c % Nx, iz = c / Nx;
+        const int lc = c % Ns;
+        const int ix = lc % Nx, iz = lc / Nx;
         int nb[4] = {iz + 1 < Nz ? c + Nx : -1, iz > 0 ? c - Nx : -1, ix + 1 < Nx ? c + 1 : -1,
                      ix > 0 ? c - 1 : -1};
         for (int f = 0; f < 4; ++f) {
@@ -834,6 +949,7 @@ __global__ void k_dest(const uint32_t* list, const uint32_t* nlist, int Nx, int 
                 uint32_t p = atomicAdd(ndest, 1u);
                 if (p < cap) dlist[p] = (uint32_t)d;
                 else atomicOr(&flags[F_DEST_FULL], 1u);
+                atomicOr(&bitmap[(uint32_t)d >> 5], 1u << ((uint32_t)d & 31));
             }
         }
     }
@@ -845,16 +961,19 @@ template <class LY>
 __global__ __launch_bounds__(256) void k_gather(Tabs T, Psi out, Bkt bk, const uint32_t* __restrict__ dlist,
                                                 const uint32_t* __restrict__ ndest,
                                                 const uint32_t* __restrict__ ltag,
-                                                const uint32_t* __restrict__ lpos, uint32_t tag,
-                                                double* k4drop_w, double* k4drop_E, uint32_t* flags) {
+                                                const uint32_t* __restrict__ lpos, const uint32_t* ctl, int fine,
+                                                uint32_t dcap, double* k4drop_w, double* k4drop_E, uint32_t* flags) {
+    if (ctl[C_DONE]) return;
+    const uint32_t tag = 2 * ctl[C_EP] + (uint32_t)fine;
     const int l = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
-    const uint32_t n = *ndest;
+    const uint32_t n = min(*ndest, dcap);  // k_dest flags an overflow; the host reports it
     const int Nx = T.Nx, Nz = T.Nz;
     for (uint32_t i = wave; i < n; i += nw) {
         const int d = (int)dlist[i];
-        const int ix = d % Nx, iz = d / Nx;
+        const int lc = d % T.Ns;
+        const int ix = lc % Nx, iz = lc / Nx;
         uint32_t n_out = out.ns[d];
         uint32_t obid = (l < 32 && (uint32_t)l < n_out) ? out.bid[(size_t)d * KB + l] : EMPTY;
         uint32_t oref = (l < 32 && (uint32_t)l < n_out) ? out.sref[(size_t)d * KB + l] : 0u;
@@ -927,61 +1046,86 @@ __global__ __launch_bounds__(256) void k_gather(Tabs T, Psi out, Bkt bk, const u
 // the per-listed-cell accumulators of both phases in list order.
 // ------------------------------------------------------------------------------------------
 enum { R_WIN, R_EIN, R_WOUT, R_EOUT, R_K4W, R_K4E, NR1 };
+// Audit vector: [0,NR1) cell sums, [NR1,NR1+NACC) phase sums, then max cell W closure, max
+// fine-cell E closure and the number of listed cells of the iteration.
+constexpr int NAUD = NR1 + NACC + 3;
 
+// Stage 1: per tile of 256 list entries: W_in / E_in from K1's per-entry sums of psi_in, and
+// W_out / E_out of psi_out plus the K4 drops (consumed) over psi_out's occupied-cell list.
+// `force` runs the pass after the transport ended (totals of a buffer for the host).
 template <class LY>
-__global__ __launch_bounds__(256) void k_audit_cells(Tabs T, Psi out, const double* win, const double* ein,
-                                                     double* k4w, double* k4e, double* part) {
+__global__ __launch_bounds__(256) void k_audit_lists(Tabs T, Psi out, const uint32_t* nin, const double* win,
+                                                     const double* ein, const uint32_t* __restrict__ occ_out,
+                                                     const uint32_t* nout, double* k4w, double* k4e,
+                                                     double* part, const uint32_t* ctl, int force) {
+    if (ctl[C_DONE] && !force) return;
     __shared__ double sh[NR1][256];
-    double r[NR1] = {0, 0, 0, 0, 0, 0};
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < T.N) {
-        r[R_WIN] = win[c];
-        r[R_EIN] = ein[c];
-        r[R_K4W] = k4w[c];
-        r[R_K4E] = k4e[c];
-        if (r[R_K4W] != 0.0 || r[R_K4E] != 0.0) {
-            k4w[c] = 0.0;
-            k4e[c] = 0.0;
+    const uint32_t n_in = nin ? *nin : 0u, n_out = *nout;
+    const uint32_t ntile = (max(n_in, n_out) + 255) / 256;
+    for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        double r[NR1] = {0, 0, 0, 0, 0, 0};
+        const uint32_t i = tile * 256 + threadIdx.x;
+        if (i < n_in) {
+            r[R_WIN] = win[i];
+            r[R_EIN] = ein[i];
         }
-        uint32_t n = out.ns[c];
-        for (uint32_t s = 0; s < n; ++s) {
-            uint32_t bid = out.bid[(size_t)c * KB + s];
-            if (bid == EMPTY) continue;
-            const float* v = out.pool + (size_t)out.sref[(size_t)c * KB + s] * LY::LB;
-            int bE = (int)((bid >> 12) & 0xFFF);
-            for (int l = 0; l < LY::LB; ++l) {
-                float w = v[l];
-                if (w <= 0.0f) continue;
-                r[R_WOUT] += (double)w;
-                int eb = min(max(bE * LY::EL + LY::le(l), 0), T.NE - 1);
-                r[R_EOUT] += (double)w * (double)ecenter(T.ee, eb);
+        if (i < n_out) {
+            const uint32_t c = occ_out[i];
+            if (k4w) {
+                r[R_K4W] = k4w[c];
+                r[R_K4E] = k4e[c];
+                if (r[R_K4W] != 0.0 || r[R_K4E] != 0.0) {
+                    k4w[c] = 0.0;
+                    k4e[c] = 0.0;
+                }
+            }
+            const uint32_t n = out.ns[c];
+            for (uint32_t s = 0; s < n; ++s) {
+                uint32_t bid = out.bid[(size_t)c * KB + s];
+                if (bid == EMPTY) continue;
+                const float* v = out.pool + (size_t)out.sref[(size_t)c * KB + s] * LY::LB;
+                int bE = (int)((bid >> 12) & 0xFFF);
+                for (int l = 0; l < LY::LB; ++l) {
+                    float w = v[l];
+                    if (w <= 0.0f) continue;
+                    r[R_WOUT] += (double)w;
+                    int eb = min(max(bE * LY::EL + LY::le(l), 0), T.NE - 1);
+                    r[R_EOUT] += (double)w * (double)ecenter(T.ee, eb);
+                }
             }
         }
-    }
-    for (int k = 0; k < NR1; ++k) sh[k][threadIdx.x] = r[k];
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o)
-            for (int k = 0; k < NR1; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
+        for (int k = 0; k < NR1; ++k) sh[k][threadIdx.x] = r[k];
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o)
+                for (int k = 0; k < NR1; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x < NR1) part[tile * NR1 + threadIdx.x] = sh[threadIdx.x][0];
         __syncthreads();
     }
-    if (threadIdx.x < NR1) part[blockIdx.x * NR1 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-// out[0..NR1) cell sums, out[NR1..NR1+NACC) phase sums, out[NR1+NACC] max cell W closure,
-// out[NR1+NACC+1] max fine-cell E closure.
-__global__ __launch_bounds__(256) void k_audit_final(const double* part, int nblk, const double* acc_c,
-                                                     const uint32_t* nc, const double* acc_f,
-                                                     const uint32_t* nf, double* outv) {
-    __shared__ double sh[NR1 + NACC + 2][256];
-    double r[NR1 + NACC + 2];
-    for (int k = 0; k < NR1 + NACC + 2; ++k) r[k] = 0.0;
-    // contiguous chunks per thread keep the order fixed
-    for (int b = threadIdx.x; b < nblk; b += 256)
+// Stage 2 (one workgroup): the tiles in a fixed order, then the per-listed-cell accumulators of
+// both phases in list order.  Output v: [0,NR1) cell sums, [NR1,NR1+NACC) phase sums,
+// [NR1+NACC] max cell W closure, [NR1+NACC+1] max fine-cell E closure.  With hist != null the
+// vector goes to row (iteration-1) % hist_rows of hist, else to outv.
+__global__ __launch_bounds__(256) void k_audit_final(const double* part, const uint32_t* nin, const uint32_t* nout,
+                                                     const double* acc_c, const uint32_t* nc, const double* acc_f,
+                                                     const uint32_t* nf, double* outv, double* hist,
+                                                     uint32_t hist_rows, const uint32_t* ctl, int force) {
+    if (ctl[C_DONE] && !force) return;
+    constexpr int NV = NAUD - 1;  // reduced entries; the list size is appended
+    __shared__ double sh[NV][256];
+    double r[NV];
+    for (int k = 0; k < NV; ++k) r[k] = 0.0;
+    const uint32_t n_in = nin ? *nin : 0u, n_out = *nout;
+    const uint32_t ntile = (max(n_in, n_out) + 255) / 256;
+    for (uint32_t b = threadIdx.x; b < ntile; b += 256)
         for (int k = 0; k < NR1; ++k) r[k] += part[b * NR1 + k];
     for (int ph = 0; ph < 2; ++ph) {
         const double* acc = ph ? acc_f : acc_c;
-        const uint32_t n = ph ? *nf : *nc;
+        const uint32_t n = acc ? (ph ? *nf : *nc) : 0u;
         for (uint32_t i = threadIdx.x; i < n; i += 256) {
             const double* o = acc + (size_t)i * NACC;
             for (int k = 0; k < NACC; ++k) r[NR1 + k] += o[k];
@@ -990,17 +1134,18 @@ __global__ __launch_bounds__(256) void k_audit_final(const double* part, int nbl
                 r[NR1 + NACC + 1] = fmax(r[NR1 + NACC + 1], fabs(o[A_CLEIN] - o[A_CLEOUT]) / o[A_CLEIN]);
         }
     }
-    for (int k = 0; k < NR1 + NACC + 2; ++k) sh[k][threadIdx.x] = r[k];
+    for (int k = 0; k < NV; ++k) sh[k][threadIdx.x] = r[k];
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) {
             for (int k = 0; k < NR1 + NACC; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
-            for (int k = NR1 + NACC; k < NR1 + NACC + 2; ++k)
-                sh[k][threadIdx.x] = fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + o]);
+            for (int k = NR1 + NACC; k < NV; ++k) sh[k][threadIdx.x] = fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + o]);
         }
         __syncthreads();
     }
-    if (threadIdx.x < NR1 + NACC + 2) outv[threadIdx.x] = sh[threadIdx.x][0];
+    double* dst = hist ? hist + (size_t)((ctl[C_IT] - 1u) % hist_rows) * NAUD : outv;
+    if (threadIdx.x < NV) dst[threadIdx.x] = sh[threadIdx.x][0];
+    if (threadIdx.x == NV) dst[NV] = (double)((acc_c ? *nc : 0u) + (acc_f ? *nf : 0u));
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1054,44 +1199,67 @@ __device__ inline InjSample inj_bin(const Tabs& T, int cell, float theta, float 
     return o;
 }
 
+__device__ inline bool spot_is_gaussian(const sm2d_spot& sp) {  // gpu_transport_wrapper.cu:240
+    return (sp.n_samples > 1) || (sp.sigma_x > 0) || (sp.sigma_theta > 0);
+}
+
+// Samples of all spots: out[spot * nmax + i]; a Gaussian spot draws n_samples
+// (inject_gaussian_source_kernel, k1k6_pipeline.cu:451-551), a pencil spot one entry
+// (inject_source_kernel pencil branch, k1k6_pipeline.cu:325-343).
 template <class LY>
-__global__ void k_inject_samples(Tabs T, sm2d_spot sp, InjSample* out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= sp.n_samples) return;
-    unsigned seed = sp.random_seed + (unsigned)i;
-    float xs = sp.x0 + sp.sigma_x * inj_gauss(seed);
-    float ts = sp.theta0 + sp.sigma_theta * inj_gauss(seed);
-    float Es = sp.E0 + sp.sigma_E * inj_gauss(seed);
-    Es = fmaxf(T.ee[0], fminf(Es, T.ee[T.NE]));
-    ts = fmaxf(T.te[0], fminf(ts, T.te[T.NT]));
-    float xr = xs - T.x_min, zr = sp.z0 - T.z_min;
-    int ix = (int)(xr / T.dx), iz = (int)(zr / T.dz);
-    if (ix < 0 || ix >= T.Nx || iz < 0 || iz >= T.Nz) {
+__global__ void k_inject_samples(Tabs T, const sm2d_spot* __restrict__ spots, int S, int nmax, InjSample* out) {
+    const int total = S * nmax;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x) {
+        const int s = idx / nmax, i = idx - s * nmax;
+        const sm2d_spot sp = spots[s];
         InjSample o;
-        o.cell = -1;
+        o.cell = -2;
         o.bid = EMPTY;
         o.lidx = 0;
-        o.E = Es;
-        out[i] = o;
-        return;
+        o.E = 0.0f;
+        const int base = s * T.Ns;
+        if (spot_is_gaussian(sp)) {
+            if (i < sp.n_samples) {
+                unsigned seed = sp.random_seed + (unsigned)i;
+                float xs = sp.x0 + sp.sigma_x * inj_gauss(seed);
+                float ts = sp.theta0 + sp.sigma_theta * inj_gauss(seed);
+                float Es = sp.E0 + sp.sigma_E * inj_gauss(seed);
+                Es = fmaxf(T.ee[0], fminf(Es, T.ee[T.NE]));
+                ts = fmaxf(T.te[0], fminf(ts, T.te[T.NT]));
+                float xr = xs - T.x_min, zr = sp.z0 - T.z_min;
+                int ix = (int)(xr / T.dx), iz = (int)(zr / T.dz);
+                if (ix < 0 || ix >= T.Nx || iz < 0 || iz >= T.Nz) {
+                    o.cell = -1;
+                    o.E = Es;
+                } else {
+                    o = inj_bin<LY>(T, base + iz * T.Nx + ix, ts, Es, xr - ix * T.dx, zr - iz * T.dz);
+                }
+            }
+        } else if (i == 0) {
+            float xr = sp.x0 - T.x_min, zr = sp.z0 - T.z_min;
+            int sx = (int)(xr / T.dx), sz = (int)(zr / T.dz);
+            sx = sx < 0 ? 0 : (sx >= T.Nx ? T.Nx - 1 : sx);
+            sz = sz < 0 ? 0 : (sz >= T.Nz ? T.Nz - 1 : sz);
+            o = inj_bin<LY>(T, base + sz * T.Nx + sx, sp.theta0, sp.E0, xr - sx * T.dx, zr - sz * T.dz);
+        }
+        out[idx] = o;
     }
-    out[i] = inj_bin<LY>(T, iz * T.Nx + ix, ts, Es, xr - ix * T.dx, zr - iz * T.dz);
 }
 
+// One wavefront per spot applies its samples in order (deterministic slot claims; the
+// reference claims them in thread-race order).  acc[spot*6 + k]: W in / out-of-grid / slot
+// drop, E in / out-of-grid / slot drop.  Touched cells are marked in the occupancy bitmap.
 template <class LY>
-__global__ void k_inject_pencil(Tabs T, sm2d_spot sp, InjSample* out) {
-    float xr = sp.x0 - T.x_min, zr = sp.z0 - T.z_min;
-    int sx = (int)(xr / T.dx), sz = (int)(zr / T.dz);
-    sx = sx < 0 ? 0 : (sx >= T.Nx ? T.Nx - 1 : sx);
-    sz = sz < 0 ? 0 : (sz >= T.Nz ? T.Nz - 1 : sz);
-    out[0] = inj_bin<LY>(T, sz * T.Nx + sx, sp.theta0, sp.E0, xr - sx * T.dx, zr - sz * T.dz);
-}
-
-// One wavefront applies the samples in order.  acc[6]: W in / out / slot-drop, E in / out / drop.
-template <class LY>
-__global__ __launch_bounds__(64) void k_inject_apply(Psi in, const InjSample* s, int n, float w, double* acc,
-                                                     int pencil, float E0, uint32_t* flags) {
+__global__ __launch_bounds__(64) void k_inject_apply(Psi in, const sm2d_spot* __restrict__ spots,
+                                                     const InjSample* smp, int nmax, double* acc,
+                                                     uint32_t* bitmap, uint32_t* flags) {
     const int l = lane_id();
+    const int spi = blockIdx.x;
+    const sm2d_spot sp = spots[spi];
+    const bool gauss = spot_is_gaussian(sp);
+    const int n = gauss ? sp.n_samples : 1;
+    const float w = gauss ? sp.W_total / sp.n_samples : sp.W_total;
+    const InjSample* s = smp + (size_t)spi * nmax;
     double Wi = 0, Wo = 0, Ws = 0, Ei = 0, Eo = 0, Es = 0;
     for (int i = 0; i < n; ++i) {
         const InjSample q = s[i];
@@ -1112,12 +1280,15 @@ __global__ __launch_bounds__(64) void k_inject_apply(Psi in, const InjSample* s,
         uint64_t hit = __ballot(l < 32 && mb == q.bid);
         if (hit) {
             slot = __ffsll((unsigned long long)hit) - 1;
-        } else if (nn < (uint32_t)KB) {
+        } else if (nn < (uint32_t)KB) {  // device_psic_find_or_allocate_slot: no fallback
             uint32_t sel;
             slot = table_find_or_claim(mb, mr, nn, q.bid, sel, in.pool, in.pcount, in.pcap,
                                        in.bid + (size_t)q.cell * KB, in.sref + (size_t)q.cell * KB, LY::LB,
                                        flags, F_PSI_FULL);
-            if (l == 0 && slot >= 0) in.ns[q.cell] = nn;
+            if (l == 0 && slot >= 0) {
+                in.ns[q.cell] = nn;
+                atomicOr(&bitmap[(uint32_t)q.cell >> 5], 1u << ((uint32_t)q.cell & 31));
+            }
         }
         if (slot >= 0) {
             uint32_t ref = rlu(mr, slot);
@@ -1131,17 +1302,18 @@ __global__ __launch_bounds__(64) void k_inject_apply(Psi in, const InjSample* s,
         __threadfence_block();  // the next sample's slot search reads what lane 0 wrote
     }
     if (l == 0) {
-        if (pencil) {  // the wrapper books the pencil source as E0 * W (gpu_transport_wrapper.cu:328-333)
+        if (!gauss) {  // the wrapper books the pencil source as E0 * W (gpu_transport_wrapper.cu:328-333)
             Wi = (double)w;
-            Ei = (double)E0 * (double)w;
+            Ei = (double)sp.E0 * (double)w;
             Wo = Ws = Eo = Es = 0.0;
         }
-        acc[0] = Wi;
-        acc[1] = Wo;
-        acc[2] = Ws;
-        acc[3] = Ei;
-        acc[4] = Eo;
-        acc[5] = Es;
+        double* a = acc + (size_t)spi * 6;
+        a[0] = Wi;
+        a[1] = Wo;
+        a[2] = Ws;
+        a[3] = Ei;
+        a[4] = Eo;
+        a[5] = Es;
     }
 }
 

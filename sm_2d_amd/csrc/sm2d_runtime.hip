@@ -1,11 +1,14 @@
-// sm2d_runtime.hip — context, per-iteration orchestration and the C ABI of libsm2d_hip.so.
+// sm2d_runtime.hip — context, iteration orchestration and the C ABI of libsm2d_hip.so.
 //
-// One context = one problem grid on one device, with its own HIP stream and all device memory
-// (persistent across spots).  The iteration mirrors run_k1k6_pipeline_transport
-// (src/cuda/k1k6_pipeline.cu:1540-2038):
+// One context = one problem grid times S spot slots on one device, with its own HIP stream and
+// all device memory (persistent across spots and batches).  An iteration mirrors
+// run_k1k6_pipeline_transport (src/cuda/k1k6_pipeline.cu:1540-2038):
 //   reset psi_out -> K1 mask -> K2 compaction -> [coarse K3 -> K4] -> [fine K3 -> K4] -> K5 -> K6
-// without the reference's 60 host-synchronous calls: one host read-back per iteration (list
-// counts + K5 report) in the same stream.
+// Every pass walks the occupied-cell lists, never the grid (except one bitmap word per 32
+// cells).  Epoch, counts and the end-of-transport test live on the device, so iterations are
+// queued in chunks and the host reads back once per chunk instead of the reference's ~60
+// synchronous calls per iteration.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -41,6 +44,8 @@ struct DBuf {
     T* p = nullptr;
     size_t n = 0;
     hipError_t alloc(size_t count) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
         n = count;
         return hipMalloc((void**)&p, sizeof(T) * (count ? count : 1));
     }
@@ -50,6 +55,9 @@ struct DBuf {
 };
 
 enum LayoutId { L_REF = 0, L_CFG = 1 };
+constexpr int NV = NAUD;  // audit vector
+constexpr int HROWS = 64;           // audit rows per chunk
+constexpr int CHUNK = 32;           // iterations queued per host read-back
 
 }  // namespace
 
@@ -58,40 +66,45 @@ struct sm2d_ctx {
     hipStream_t stream = nullptr;
     sm2d_grid g{};
     LayoutId layout = L_REF;
-    int LB = 256, N = 0;
-    uint32_t pcap = 0, lcap = 0;
+    int LB = 256, S = 1, Ns = 0, N = 0;
+    uint32_t pcap = 0, lcap = 0, dcap = 0, ocap = 0, nwords = 0;
     bool tables = false;
-    // tables
-    DBuf<float> te, ee, R, S, lE, lR, lS, feA, feB, rho, x0r;
+    DBuf<float> te, ee, R, Sp, lE, lR, lS, feA, feB, sx0, rho, x0r;
     std::vector<float> h_ee;
-    // phase space (ping-pong)
-    DBuf<uint32_t> bid[2], ns[2], sref[2], pcount[2];
+    DBuf<uint32_t> bid[2], ns[2], sref[2], pcount[2], occ[2];
     DBuf<float> pool[2];
     int in = 0;
-    // buckets
+    DBuf<uint32_t> bitmap;
     DBuf<uint32_t> bbid, bn, bsref, bpcount;
     DBuf<float> bpool;
-    // lists and tags
-    DBuf<uint32_t> maskv, ltag, lpos, dtag, act, crs, dlist, counts /*0 nact 1 ncrs 2 ndest*/;
+    DBuf<uint32_t> maskv, ltag, lpos, dtag, act, crs, dlist, ctl, tcount, tcount2, flags;
     DBuf<uint8_t> flag;
-    DBuf<uint32_t> bcount;
-    uint32_t epoch = 1;
-    // tallies
-    DBuf<double> edep, E_cut, E_nuc, E_bnd, win, ein, k4w, k4e, acc_c, acc_f, part, k5v;
-    DBuf<float> w_cut, w_nuc, w_bnd;
-    DBuf<uint32_t> flags;
-    // injection
+    DBuf<double> edep, E_cut, E_nuc, E_bnd, win, ein, k4w, k4e, acc_c, acc_f, part, k5v, hist, edep_sum;
+    DBuf<float> w_cut, w_nuc, w_bnd, edep_f;
+    DBuf<sm2d_spot> spots;
     DBuf<InjSample> samples;
     DBuf<double> injacc;
-    // host source accounting of the current spot
-    double src[7] = {0, 0, 0, 0, 0, 0, 0};  // W in/out/slot, E in/out/slot, E repr loss
+    std::vector<double> src;  // per spot: W in/out/slot, E in/out/slot, E representation loss
+    uint32_t hint = 64;       // expected listed cells per phase (launch sizing only)
+    // stage timing (eager launches with events on `stream`)
+    bool prof = false;
+    hipEvent_t ev[12] = {};
+    bool ev_used[12] = {};
+    sm2d_timing timing{};
+    void mark(int k) {
+        if (prof) {
+            (void)hipEventRecord(ev[k], stream);
+            ev_used[k] = true;
+        }
+    }
 
     Tabs tabs() const {
         Tabs t;
         t.Nx = g.Nx; t.Nz = g.Nz; t.N = N; t.NT = g.N_theta; t.NE = g.N_E;
+        t.Ns = Ns; t.S = S;
         t.dx = g.dx; t.dz = g.dz; t.x_min = g.x_min; t.z_min = g.z_min;
         t.te = te.p; t.ee = ee.p; t.R = R.p; t.lR = lR.p; t.lE = lE.p;
-        t.feA = feA.p; t.feB = feB.p; t.rho = rho.p; t.x0r = x0r.p;
+        t.feA = feA.p; t.feB = feB.p; t.sx0 = sx0.p; t.rho = rho.p; t.x0r = x0r.p;
         return t;
     }
     Psi psi(int k) const {
@@ -111,14 +124,14 @@ struct sm2d_ctx {
         t.w_bnd = w_bnd.p; t.E_bnd = E_bnd.p;
         return t;
     }
-    int blocks256() const { return (N + 255) / 256; }
+    uint32_t* nocc(int k) const { return ctl.p + (k ? C_NOCC1 : C_NOCC0); }
 };
 
 namespace {
 
-int grid_blocks(uint32_t n, int per_block, int cap = 4096) {
-    int b = (int)((n + per_block - 1) / per_block);
-    return b < 1 ? 1 : (b > cap ? cap : b);
+int blocks_for(uint64_t items, int per_block, int cap) {
+    uint64_t b = (items + per_block - 1) / per_block;
+    return (int)std::max<uint64_t>(1, std::min<uint64_t>(b, (uint64_t)cap));
 }
 
 int b_threshold(const sm2d_ctx* c, float E) {  // compute_b_E_threshold (k1k6_pipeline.cuh:419-422)
@@ -143,7 +156,6 @@ Tp make_tp(const sm2d_ctx* c, const sm2d_transport* t) {
     p.E_off = t->E_fine_off;
     p.wmin = t->weight_active_min;
     p.step_coarse = t->step_coarse;
-    p.sx0 = t->sigma_x_initial;
     p.b_on = b_threshold(c, t->E_fine_on);
     p.b_off = b_threshold(c, t->E_fine_off);
     p.rule = t->step_rule;
@@ -152,87 +164,127 @@ Tp make_tp(const sm2d_ctx* c, const sm2d_transport* t) {
     return p;
 }
 
-template <class LY>
-void launch_clear_out(sm2d_ctx* c) {
-    k_clear<<<grid_blocks(c->N, 256, 2048), 256, 0, c->stream>>>(c->psi(1 - c->in), c->N);
+__global__ void k_ctl_reset(uint32_t* ctl) {
+    ctl[C_DONE] = 0;
+    ctl[C_IT] = 0;
+    ctl[C_DONE_IT] = 0;
 }
 
-// K1 + K2: lists of the current psi_in, counts copied to host.
-template <class LY>
-int mask_compact(sm2d_ctx* c, const Tp& P, uint32_t ep, uint32_t* h_counts) {
-    const int nb = c->blocks256();
-    k_mask<LY><<<nb, 256, 0, c->stream>>>(c->tabs(), P, c->psi(c->in), c->maskv.p, ep, c->flag.p,
-                                           c->bcount.p, c->win.p, c->ein.p);
-    k_scan<<<1, 1024, 0, c->stream>>>(c->bcount.p, nb, c->counts.p);
-    k_emit<<<nb, 256, 0, c->stream>>>(c->N, c->flag.p, c->bcount.p, c->act.p, c->crs.p, c->ltag.p, c->lpos.p,
-                                      2 * ep, 2 * ep + 1);
-    HIPCHK(hipGetLastError());
-    if (h_counts) {
-        HIPCHK(hipMemcpyAsync(h_counts, c->counts.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+// Resets the per-phase bucket pool and destination count (skipped once the transport ended).
+__global__ void k_phase_reset(uint32_t* ctl, uint32_t* bpcount) {
+    if (!ctl[C_DONE]) {
+        ctl[C_NDEST] = 0;
+        *bpcount = 0;
     }
+}
+
+// Occupancy bitmap -> ascending occupied-cell list of buffer k.
+void build_occ(sm2d_ctx* c, int k) {
+    const int nt = blocks_for(c->nwords, 256, 1 << 20);
+    k_occ_count<<<nt, 256, 0, c->stream>>>(c->bitmap.p, c->nwords, c->tcount2.p, c->ctl.p);
+    // n_items = nwords is held in ctl[15]
+    k_scan<1><<<1, 1024, 0, c->stream>>>(c->tcount2.p, c->ctl.p + 15, 256, c->ctl.p, c->nocc(k), c->ocap,
+                                         c->flags.p);
+    k_occ_emit<<<nt, 256, 0, c->stream>>>(c->bitmap.p, c->nwords, c->tcount2.p, c->occ[k].p, c->ocap, c->ctl.p);
+}
+
+template <class LY>
+void launch_iteration(sm2d_ctx* c, const Tp& P) {
+    const int in = c->in, out = 1 - c->in;
+    Tabs T = c->tabs();
+    const uint32_t h = std::max<uint32_t>(c->hint, 64);
+    const int gm = blocks_for(4ull * h, 256, 4096);  // occupied-list tiles
+    k_begin<<<1, 64, 0, c->stream>>>(c->ctl.p);
+    k_clear<<<gm, 256, 0, c->stream>>>(c->psi(out), c->occ[out].p, c->nocc(out), c->ctl.p);
+    c->mark(0);
+    k_mask<LY><<<gm, 256, 0, c->stream>>>(T, P, c->psi(in), c->occ[in].p, c->nocc(in), c->maskv.p, c->ctl.p,
+                                          c->flag.p, c->tcount.p, c->win.p, c->ein.p);
+    k_scan<2><<<1, 1024, 0, c->stream>>>(c->tcount.p, c->nocc(in), 256, c->ctl.p, nullptr, c->lcap, c->flags.p);
+    k_emit<<<gm, 256, 0, c->stream>>>(c->occ[in].p, c->nocc(in), c->flag.p, c->tcount.p, c->act.p, c->crs.p,
+                                      c->ltag.p, c->lpos.p, c->bitmap.p, c->ctl.p);
+    c->mark(1);
+    for (int fine = 0; fine < 2; ++fine) {
+        const uint32_t* list = fine ? c->act.p : c->crs.p;
+        const uint32_t* nl = c->ctl.p + (fine ? C_NACT : C_NCRS);
+        double* acc = fine ? c->acc_f.p : c->acc_c.p;
+        const int e0 = fine ? 6 : 2;
+        k_phase_reset<<<1, 64, 0, c->stream>>>(c->ctl.p, c->bpcount.p);
+        c->mark(e0);
+        if (fine)
+            k_transport<LY, true><<<blocks_for(h, 4, 8192), 256, 0, c->stream>>>(
+                T, P, c->psi(in), c->psi(out), c->bkt(), list, nl, c->ltag.p, c->ctl.p, c->tally(), acc, c->flags.p);
+        else
+            k_transport<LY, false><<<blocks_for(h, 4, 8192), 256, 0, c->stream>>>(
+                T, P, c->psi(in), c->psi(out), c->bkt(), list, nl, c->ltag.p, c->ctl.p, c->tally(), acc, c->flags.p);
+        c->mark(e0 + 1);
+        k_dest<<<blocks_for(h, 256, 1024), 256, 0, c->stream>>>(list, nl, c->g.Nx, c->g.Nz, c->Ns, c->dtag.p, c->ctl.p,
+                                                                fine, c->dlist.p, c->ctl.p + C_NDEST, c->dcap,
+                                                                c->bitmap.p, c->flags.p);
+        c->mark(e0 + 2);
+        k_gather<LY><<<blocks_for(4ull * h, 4, 8192), 256, 0, c->stream>>>(
+            T, c->psi(out), c->bkt(), c->dlist.p, c->ctl.p + C_NDEST, c->ltag.p, c->lpos.p, c->ctl.p, fine, c->dcap,
+            c->k4w.p, c->k4e.p, c->flags.p);
+        c->mark(e0 + 3);
+    }
+    build_occ(c, out);
+    c->mark(10);
+    k_audit_lists<LY><<<gm, 256, 0, c->stream>>>(T, c->psi(out), c->nocc(in), c->win.p, c->ein.p, c->occ[out].p,
+                                                 c->nocc(out), c->k4w.p, c->k4e.p, c->part.p, c->ctl.p, 0);
+    k_audit_final<<<1, 256, 0, c->stream>>>(c->part.p, c->nocc(in), c->nocc(out), c->acc_c.p, c->ctl.p + C_NCRS,
+                                            c->acc_f.p, c->ctl.p + C_NACT, nullptr, c->hist.p, HROWS, c->ctl.p, 0);
+    c->mark(11);
+    c->in = out;  // K6
+}
+
+// Totals (W, E) of buffer k over its occupied list, after the transport (forced pass).
+template <class LY>
+int buffer_totals(sm2d_ctx* c, int k, double& W, double& E) {
+    const int gm = blocks_for(c->ocap, 256, 4096);
+    k_audit_lists<LY><<<gm, 256, 0, c->stream>>>(c->tabs(), c->psi(k), nullptr, c->win.p, c->ein.p, c->occ[k].p,
+                                                 c->nocc(k), nullptr, nullptr, c->part.p, c->ctl.p, 1);
+    k_audit_final<<<1, 256, 0, c->stream>>>(c->part.p, nullptr, c->nocc(k), nullptr, nullptr, nullptr, nullptr,
+                                            c->k5v.p, nullptr, 1, c->ctl.p, 1);
+    HIPCHK(hipGetLastError());
+    double v[NV];
+    HIPCHK(hipMemcpyAsync(v, c->k5v.p, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    W = v[R_WOUT];
+    E = v[R_EOUT];
     return SM2D_OK;
 }
 
-template <class LY>
-void run_phase(sm2d_ctx* c, const Tp& P, bool fine, uint32_t ep, uint32_t n_host) {
-    if (n_host == 0) return;
-    const uint32_t tag = fine ? 2 * ep + 1 : 2 * ep;
-    const uint32_t* list = fine ? c->act.p : c->crs.p;
-    const uint32_t* nl = c->counts.p + (fine ? 0 : 1);
-    double* acc = fine ? c->acc_f.p : c->acc_c.p;
-    Psi in = c->psi(c->in), out = c->psi(1 - c->in);
-    k_fill_u32<<<1, 64, 0, c->stream>>>(c->bpcount.p, 0u, 1);
-    k_fill_u32<<<1, 64, 0, c->stream>>>(c->counts.p + 2, 0u, 1);
-    const int tb = grid_blocks(n_host, 4, 8192);  // 4 waves (cells) per block
-    if (fine)
-        k_transport<LY, true><<<tb, 256, 0, c->stream>>>(c->tabs(), P, in, out, c->bkt(), list, nl, c->ltag.p,
-                                                          tag, c->tally(), acc, c->flags.p);
-    else
-        k_transport<LY, false><<<tb, 256, 0, c->stream>>>(c->tabs(), P, in, out, c->bkt(), list, nl, c->ltag.p,
-                                                           tag, c->tally(), acc, c->flags.p);
-    k_dest<<<grid_blocks(n_host, 256, 1024), 256, 0, c->stream>>>(list, nl, c->g.Nx, c->g.Nz, c->dtag.p, tag,
-                                                                   c->dlist.p, c->counts.p + 2, c->lcap, c->flags.p);
-    k_gather<LY><<<grid_blocks(4 * n_host, 4, 8192), 256, 0, c->stream>>>(
-        c->tabs(), out, c->bkt(), c->dlist.p, c->counts.p + 2, c->ltag.p, c->lpos.p, tag, c->k4w.p, c->k4e.p,
-        c->flags.p);
+void collect_timing(sm2d_ctx* c) {
+    if (!c->prof) return;
+    auto span = [&](int a, int b) -> double {
+        if (!c->ev_used[a] || !c->ev_used[b]) return 0.0;
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, c->ev[a], c->ev[b]);
+        return (double)ms;
+    };
+    c->timing.mask_ms += span(0, 1);
+    c->timing.transport_ms += span(2, 3) + span(6, 7);
+    c->timing.gather_ms += span(3, 5) + span(7, 9);
+    c->timing.audit_ms += span(10, 11);
+    c->timing.transport_launches += (c->ev_used[2] ? 1 : 0) + (c->ev_used[6] ? 1 : 0);
+    c->timing.gather_launches += (c->ev_used[4] ? 1 : 0) + (c->ev_used[8] ? 1 : 0);
+    for (bool& u : c->ev_used) u = false;
 }
 
-int check_flags(sm2d_ctx* c) {
-    uint32_t f[NFLAG];
-    HIPCHK(hipMemcpyAsync(f, c->flags.p, sizeof(f), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+int check_flags(sm2d_ctx* c, const uint32_t* f) {
     if (f[F_PSI_FULL]) return err("phase-space payload pool full (raise pool capacity)", SM2D_ERR_CAPACITY);
     if (f[F_BKT_FULL]) return err("bucket payload pool full (raise pool capacity)", SM2D_ERR_CAPACITY);
-    if (f[F_DEST_FULL]) return err("destination list full", SM2D_ERR_CAPACITY);
+    if (f[F_DEST_FULL]) return err("destination / list capacity exceeded", SM2D_ERR_CAPACITY);
     return SM2D_OK;
 }
 
-// K5 of the iteration just transported (before the swap).  v: NR1 cell sums, NACC phase sums,
-// 2 closure maxima.
-template <class LY>
-int audit(sm2d_ctx* c, std::vector<double>& v) {
-    const int nb = c->blocks256();
-    k_audit_cells<LY><<<nb, 256, 0, c->stream>>>(c->tabs(), c->psi(1 - c->in), c->win.p, c->ein.p, c->k4w.p,
-                                                  c->k4e.p, c->part.p);
-    k_audit_final<<<1, 256, 0, c->stream>>>(c->part.p, nb, c->acc_c.p, c->counts.p + 1, c->acc_f.p,
-                                            c->counts.p + 0, c->k5v.p);
-    HIPCHK(hipGetLastError());
-    v.assign(NR1 + NACC + 2, 0.0);
-    HIPCHK(hipMemcpyAsync(v.data(), c->k5v.p, sizeof(double) * v.size(), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    return SM2D_OK;
-}
-
-void fill_report(const std::vector<double>& v, double Wso, double Wss, double Eso, double Ess, int src,
-                 sm2d_k5_report* r) {
+void fill_report(const double* v, double Wso, double Wss, double Eso, double Ess, int src, sm2d_k5_report* r) {
     std::memset(r, 0, sizeof(*r));
     r->struct_size = sizeof(*r);
     r->W_in = v[R_WIN];
     r->E_in = v[R_EIN];
     r->W_out = v[R_WOUT];
     r->E_out = v[R_EOUT];
-    const double* a = v.data() + NR1;
+    const double* a = v + NR1;
     r->W_cutoff = a[A_WCUT];
     r->W_nuclear = a[A_WNUC];
     r->W_boundary = a[A_WBND];
@@ -265,94 +317,117 @@ template <class LY>
 int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, sm2d_k5_report* last,
               sm2d_audit* au) {
     Tp P = make_tp(c, tp);
-    std::vector<double> v;
-    for (int it = 0; it < n_iter; ++it) {
-        const uint32_t ep = ++c->epoch;
-        launch_clear_out<LY>(c);
-        uint32_t cnt[2] = {0, 0};
-        int rc = mask_compact<LY>(c, P, ep, cnt);
-        if (rc) return rc;
-        if (au) au->iterations++;
-        if (cnt[0] == 0 && cnt[1] == 0) {
-            if (au) au->terminated = 1;
-            return SM2D_OK;
-        }
-        if (cnt[0] > c->lcap || cnt[1] > c->lcap) return err("list capacity", SM2D_ERR_CAPACITY);
-        run_phase<LY>(c, P, false, ep, cnt[1]);
-        run_phase<LY>(c, P, true, ep, cnt[0]);
-        HIPCHK(hipGetLastError());
-        rc = audit<LY>(c, v);
-        if (rc) return rc;
-        rc = check_flags(c);
-        if (rc) return rc;
-        sm2d_k5_report r;
-        const bool first = src_terms && it == 0;
-        fill_report(v, c->src[1], c->src[2], c->src[4], c->src[5], first ? 1 : 0, &r);
-        if (last) *last = r;
-        if (au) {
-            const double* a = v.data() + NR1;
-            au->W_transport_drop += a[A_DROPW] + v[R_K4W];
-            au->E_transport_drop += a[A_DROPE] + v[R_K4E];
-            double Er = r.E_out + r.E_dep + r.E_cutoff + r.E_nuclear + r.E_boundary + r.E_transport_drop +
-                        r.E_source_out_of_grid + r.E_source_slot_drop;
-            double res = r.E_in - Er;
-            if (std::isfinite(res)) au->E_audit_residual += res;
-            au->max_W_error = std::max(au->max_W_error, r.W_error);
-            au->max_E_error = std::max(au->max_E_error, r.E_error);
-            if (!r.W_pass) au->W_fail_iters++;
-            if (!r.E_pass) au->E_fail_iters++;
-            au->max_cell_W_closure = std::max(au->max_cell_W_closure, v[NR1 + NACC]);
-            au->max_cell_E_closure = std::max(au->max_cell_E_closure, v[NR1 + NACC + 1]);
-            au->cell_bin_updates += (uint64_t)a[A_UPD];
-            au->processed_components += (uint64_t)a[A_COMPS];
-            au->processed_cells += (uint64_t)cnt[0] + cnt[1];
-        }
-        if (tp->fail_fast_on_audit && (!r.W_pass || !r.E_pass))
-            return err("K5 audit fail-fast", SM2D_ERR_AUDIT);
-        c->in = 1 - c->in;  // K6
+    k_ctl_reset<<<1, 1, 0, c->stream>>>(c->ctl.p);
+    HIPCHK(hipGetLastError());
+    const int in0 = c->in;
+    const int chunk = (tp->fail_fast_on_audit || c->prof) ? 1 : CHUNK;
+    int queued = 0, audited = 0;
+    double Wso = 0, Wss = 0, Eso = 0, Ess = 0;
+    for (int s = 0; s < c->S; ++s) {
+        Wso += c->src[s * 7 + 1];
+        Wss += c->src[s * 7 + 2];
+        Eso += c->src[s * 7 + 4];
+        Ess += c->src[s * 7 + 5];
     }
+    std::vector<double> rows((size_t)HROWS * NV);
+    while (queued < n_iter) {
+        const int R = std::min(chunk, n_iter - queued);
+        for (int r = 0; r < R; ++r) launch_iteration<LY>(c, P);
+        queued += R;
+        HIPCHK(hipGetLastError());
+        uint32_t ctl[NCTL], f[NFLAG];
+        HIPCHK(hipMemcpyAsync(ctl, c->ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(f, c->flags.p, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(rows.data(), c->hist.p, sizeof(double) * rows.size(), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        collect_timing(c);
+        int rc = check_flags(c, f);
+        if (rc) return rc;
+        const bool done = ctl[C_DONE] != 0;
+        const int upto = done ? (int)ctl[C_DONE_IT] - 1 : (int)ctl[C_IT];  // iterations with an audit row
+        uint32_t maxlist = 0;
+        for (int it = audited; it < upto; ++it) {
+            const double* v = rows.data() + (size_t)(it % HROWS) * NV;
+            sm2d_k5_report r;
+            fill_report(v, Wso, Wss, Eso, Ess, (src_terms && it == 0) ? 1 : 0, &r);
+            if (last) *last = r;
+            if (au) {
+                const double* a = v + NR1;
+                au->W_transport_drop += a[A_DROPW] + v[R_K4W];
+                au->E_transport_drop += a[A_DROPE] + v[R_K4E];
+                double Er = r.E_out + r.E_dep + r.E_cutoff + r.E_nuclear + r.E_boundary + r.E_transport_drop +
+                            r.E_source_out_of_grid + r.E_source_slot_drop;
+                double res = r.E_in - Er;
+                if (std::isfinite(res)) au->E_audit_residual += res;
+                au->max_W_error = std::max(au->max_W_error, r.W_error);
+                au->max_E_error = std::max(au->max_E_error, r.E_error);
+                if (!r.W_pass) au->W_fail_iters++;
+                if (!r.E_pass) au->E_fail_iters++;
+                au->max_cell_W_closure = std::max(au->max_cell_W_closure, v[NR1 + NACC]);
+                au->max_cell_E_closure = std::max(au->max_cell_E_closure, v[NR1 + NACC + 1]);
+                au->cell_bin_updates += (uint64_t)a[A_UPD];
+                au->processed_cells += (uint64_t)v[NV - 1];
+                au->processed_components += (uint64_t)a[A_COMPS];
+            }
+            if (c->prof) c->timing.cell_bin_updates += (uint64_t)v[NR1 + A_UPD];
+            if (tp->fail_fast_on_audit && (!r.W_pass || !r.E_pass)) {
+                c->in = in0 ^ (it & 1);  // psi_in of the failing iteration
+                return err("K5 audit fail-fast", SM2D_ERR_AUDIT);
+            }
+        }
+        audited = upto;
+        maxlist = std::max(ctl[C_NACT], ctl[C_NCRS]);
+        c->hint = std::max<uint32_t>(64, 2 * maxlist);
+        if (au) au->iterations = done ? (int)ctl[C_DONE_IT] : (int)ctl[C_IT];
+        if (done) {
+            if (au) au->terminated = 1;
+            c->in = in0 ^ (int)((ctl[C_DONE_IT] - 1u) & 1u);  // the buffer of the terminating iteration
+            break;
+        }
+    }
+    if (c->prof) c->timing.iterations += (uint64_t)audited;
     return SM2D_OK;
 }
 
 template <class LY>
-int inject(sm2d_ctx* c, const sm2d_spot* sp) {
-    const bool gauss = (sp->n_samples > 1) || (sp->sigma_x > 0) || (sp->sigma_theta > 0);
-    int n = gauss ? sp->n_samples : 1;
-    if (n <= 0) return err("n_samples must be positive");
-    if (c->samples.n < (size_t)n) {
-        if (c->samples.p) (void)hipFree(c->samples.p);
-        c->samples.p = nullptr;
-        HIPCHK(c->samples.alloc(n));
+int inject_all(sm2d_ctx* c, const sm2d_spot* sp, int n) {
+    int nmax = 1;
+    for (int s = 0; s < n; ++s) {
+        const bool gauss = (sp[s].n_samples > 1) || (sp[s].sigma_x > 0) || (sp[s].sigma_theta > 0);
+        if (gauss && sp[s].n_samples <= 0) return err("n_samples must be positive");
+        nmax = std::max(nmax, gauss ? sp[s].n_samples : 1);
     }
-    if (gauss)
-        k_inject_samples<LY><<<(n + 255) / 256, 256, 0, c->stream>>>(c->tabs(), *sp, c->samples.p);
-    else
-        k_inject_pencil<LY><<<1, 1, 0, c->stream>>>(c->tabs(), *sp, c->samples.p);
-    float w = gauss ? sp->W_total / sp->n_samples : sp->W_total;
-    k_inject_apply<LY><<<1, 64, 0, c->stream>>>(c->psi(c->in), c->samples.p, n, w, c->injacc.p, gauss ? 0 : 1,
-                                                sp->E0, c->flags.p);
+    if (c->samples.n < (size_t)n * nmax) HIPCHK(c->samples.alloc((size_t)n * nmax));
+    HIPCHK(hipMemcpyAsync(c->spots.p, sp, sizeof(sm2d_spot) * n, hipMemcpyHostToDevice, c->stream));
+    Tabs T = c->tabs();
+    k_inject_samples<LY><<<blocks_for((uint64_t)n * nmax, 256, 8192), 256, 0, c->stream>>>(T, c->spots.p, n, nmax,
+                                                                                           c->samples.p);
+    k_inject_apply<LY><<<n, 64, 0, c->stream>>>(c->psi(c->in), c->spots.p, c->samples.p, nmax, c->injacc.p,
+                                                c->bitmap.p, c->flags.p);
+    k_ctl_reset<<<1, 1, 0, c->stream>>>(c->ctl.p);  // a finished transport must not gate the list build
+    build_occ(c, c->in);
     HIPCHK(hipGetLastError());
-    double acc[6];
-    HIPCHK(hipMemcpyAsync(acc, c->injacc.p, sizeof(acc), hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> acc((size_t)n * 6);
+    uint32_t f[NFLAG];
+    HIPCHK(hipMemcpyAsync(acc.data(), c->injacc.p, sizeof(double) * acc.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(f, c->flags.p, sizeof(f), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    int rc = check_flags(c);
+    int rc = check_flags(c, f);
     if (rc) return rc;
-    // represented energy of psi_in (gpu_transport_wrapper.cu:125-180,443-457)
-    std::vector<double> v;
-    Psi pin = c->psi(c->in);
-    const int nb = c->blocks256();
-    k_audit_cells<LY><<<nb, 256, 0, c->stream>>>(c->tabs(), pin, c->win.p, c->ein.p, c->k4w.p, c->k4e.p,
-                                                  c->part.p);
-    k_fill_u32<<<1, 64, 0, c->stream>>>(c->counts.p, 0u, 2);
-    k_audit_final<<<1, 256, 0, c->stream>>>(c->part.p, nb, c->acc_c.p, c->counts.p + 1, c->acc_f.p,
-                                            c->counts.p + 0, c->k5v.p);
-    v.assign(NR1 + NACC + 2, 0.0);
-    HIPCHK(hipMemcpyAsync(v.data(), c->k5v.p, sizeof(double) * v.size(), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    double loss = acc[3] - v[R_EOUT];
+    // represented energy of psi_in (gpu_transport_wrapper.cu:125-180,443-457); per batch
+    double W, E;
+    rc = buffer_totals<LY>(c, c->in, W, E);
+    if (rc) return rc;
+    double Ein = 0.0;
+    for (int s = 0; s < n; ++s) {
+        for (int k = 0; k < 6; ++k) c->src[s * 7 + k] = acc[s * 6 + k];
+        Ein += acc[s * 6 + 3];
+    }
+    double loss = Ein - E;
     if (std::fabs(loss) < 1e-12) loss = 0.0;
-    for (int k = 0; k < 6; ++k) c->src[k] = acc[k];
-    c->src[6] = loss;
+    for (int s = 0; s < c->S; ++s) c->src[s * 7 + 6] = 0.0;
+    c->src[6] = loss;  // the batch's representation loss is booked on spot 0
     return SM2D_OK;
 }
 
@@ -362,19 +437,23 @@ int zero_state(sm2d_ctx* c) {
         HIPCHK(hipMemsetAsync(c->ns[k].p, 0, sizeof(uint32_t) * c->ns[k].n, c->stream));
         HIPCHK(hipMemsetAsync(c->pcount[k].p, 0, sizeof(uint32_t), c->stream));
     }
-    HIPCHK(hipMemsetAsync(c->edep.p, 0, sizeof(double) * c->N, c->stream));
-    HIPCHK(hipMemsetAsync(c->E_cut.p, 0, sizeof(double) * c->N, c->stream));
-    HIPCHK(hipMemsetAsync(c->E_nuc.p, 0, sizeof(double) * c->N, c->stream));
-    HIPCHK(hipMemsetAsync(c->E_bnd.p, 0, sizeof(double) * c->N, c->stream));
-    HIPCHK(hipMemsetAsync(c->w_cut.p, 0, sizeof(float) * c->N, c->stream));
-    HIPCHK(hipMemsetAsync(c->w_nuc.p, 0, sizeof(float) * c->N, c->stream));
-    HIPCHK(hipMemsetAsync(c->w_bnd.p, 0, sizeof(float) * c->N, c->stream));
-    HIPCHK(hipMemsetAsync(c->k4w.p, 0, sizeof(double) * c->N, c->stream));
-    HIPCHK(hipMemsetAsync(c->k4e.p, 0, sizeof(double) * c->N, c->stream));
+    const size_t N = c->N;
+    for (DBuf<double>* b : {&c->edep, &c->E_cut, &c->E_nuc, &c->E_bnd, &c->k4w, &c->k4e})
+        HIPCHK(hipMemsetAsync(b->p, 0, sizeof(double) * N, c->stream));
+    for (DBuf<float>* b : {&c->w_cut, &c->w_nuc, &c->w_bnd}) HIPCHK(hipMemsetAsync(b->p, 0, sizeof(float) * N, c->stream));
     HIPCHK(hipMemsetAsync(c->flags.p, 0, sizeof(uint32_t) * NFLAG, c->stream));
-    c->epoch += 4;  // stale mask / list / destination tags can no longer match
+    HIPCHK(hipMemsetAsync(c->bitmap.p, 0, sizeof(uint32_t) * c->nwords, c->stream));
+    // control block: counts zero, epoch advanced past every stored tag, word 15 = bitmap words
+    uint32_t ctl[NCTL];
+    HIPCHK(hipMemcpyAsync(ctl, c->ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const uint32_t ep = ctl[C_EP] + 4;
+    std::memset(ctl, 0, sizeof(ctl));
+    ctl[C_EP] = ep;
+    ctl[15] = c->nwords;
+    HIPCHK(hipMemcpyAsync(c->ctl.p, ctl, sizeof(ctl), hipMemcpyHostToDevice, c->stream));
     c->in = 0;
-    for (double& s : c->src) s = 0.0;
+    std::fill(c->src.begin(), c->src.end(), 0.0);
     HIPCHK(hipStreamSynchronize(c->stream));
     return SM2D_OK;
 }
@@ -391,6 +470,142 @@ int check_ctx(sm2d_ctx* c) {
     return SM2D_OK;
 }
 
+int download_tallies(sm2d_ctx* c, double* t, int spot) {
+    const size_t Ns = c->Ns, off = (size_t)spot * Ns;
+    std::vector<float> f(Ns);
+    HIPCHK(hipMemcpyAsync(t + 0 * Ns, c->edep.p + off, sizeof(double) * Ns, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(t + 2 * Ns, c->E_cut.p + off, sizeof(double) * Ns, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(t + 4 * Ns, c->E_nuc.p + off, sizeof(double) * Ns, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(t + 6 * Ns, c->E_bnd.p + off, sizeof(double) * Ns, hipMemcpyDeviceToHost, c->stream));
+    const float* src[3] = {c->w_cut.p + off, c->w_nuc.p + off, c->w_bnd.p + off};
+    const int dst[3] = {1, 3, 5};
+    for (int k = 0; k < 3; ++k) {
+        HIPCHK(hipMemcpyAsync(f.data(), src[k], sizeof(float) * Ns, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < Ns; ++i) t[dst[k] * Ns + i] = f[i];
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SM2D_OK;
+}
+
+// Sum over spots (FP64, spot order) and optional FP32 copies of every spot's EdepC.
+__global__ void k_edep_reduce(const double* edep, int n_spots, int Ns, double* sum, float* per) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < Ns; i += gridDim.x * blockDim.x) {
+        double a = 0.0;
+        for (int s = 0; s < n_spots; ++s) {
+            double v = edep[(size_t)s * Ns + i];
+            a += v;
+            if (per) per[(size_t)s * Ns + i] = (float)v;
+        }
+        sum[i] = a;
+    }
+}
+
+int create(int device, const sm2d_grid* g, int n_spots, sm2d_ctx** out) {
+    if (!g || !out || g->struct_size != sizeof(sm2d_grid)) return err("sm2d_ctx_create: grid struct_size");
+    if (g->Nx <= 0 || g->Nz <= 0 || g->N_theta <= 0 || g->N_E <= 0 || n_spots <= 0 ||
+        g->N_theta > 4095 * g->n_theta_local)
+        return err("sm2d_ctx_create: bad grid");
+    LayoutId lid;
+    if (g->n_theta_local == 4 && g->n_e_local == 2 && g->n_x_sub == 8 && g->n_z_sub == 4) lid = L_REF;
+    else if (g->n_theta_local == 8 && g->n_e_local == 4 && g->n_x_sub == 4 && g->n_z_sub == 1) lid = L_CFG;
+    else return err("sm2d_ctx_create: local-bin layout must be (4,2,8,4) or (8,4,4,1)");
+    const size_t Ns = (size_t)g->Nx * g->Nz, N = Ns * (size_t)n_spots;
+    if (N >= (1ull << 31) / 1) return err("sm2d_ctx_create: too many cells");
+    HIPCHK(hipSetDevice(device));
+    std::unique_ptr<sm2d_ctx> c(new sm2d_ctx());
+    c->device = device;
+    c->g = *g;
+    c->layout = lid;
+    c->LB = g->n_theta_local * g->n_e_local * g->n_x_sub * g->n_z_sub;
+    c->S = n_spots;
+    c->Ns = (int)Ns;
+    c->N = (int)N;
+    // Capacities: listed cells per phase <= 48 rows of every spot; occupied cells and
+    // destinations <= 5x that; payload slots <= 16 per listed cell (at least 64 Ki).  An
+    // overflow stops the transport and is reported as SM2D_ERR_CAPACITY.
+    c->lcap = (uint32_t)std::min<size_t>(N, (size_t)n_spots * 48 * g->Nx + 1024);
+    c->ocap = (uint32_t)std::min<size_t>(N, 5ull * c->lcap);
+    c->dcap = c->ocap;
+    c->pcap = (uint32_t)std::min<size_t>(N * KB, std::max<size_t>(65536, 16ull * c->lcap));
+    c->nwords = (uint32_t)((N + 31) / 32);
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (int k = 0; k < 2; ++k) {
+        HIPCHK(c->bid[k].alloc(N * KB));
+        HIPCHK(c->ns[k].alloc(N));
+        HIPCHK(c->sref[k].alloc(N * KB));
+        HIPCHK(c->pcount[k].alloc(1));
+        HIPCHK(c->pool[k].alloc((size_t)c->pcap * c->LB));
+        HIPCHK(c->occ[k].alloc(c->ocap));
+    }
+    HIPCHK(c->bitmap.alloc(c->nwords));
+    HIPCHK(c->bbid.alloc((size_t)c->lcap * 4 * KB));
+    HIPCHK(c->bn.alloc((size_t)c->lcap * 4));
+    HIPCHK(c->bsref.alloc((size_t)c->lcap * 4 * KB));
+    HIPCHK(c->bpcount.alloc(1));
+    HIPCHK(c->bpool.alloc((size_t)c->pcap * c->LB));
+    for (DBuf<uint32_t>* b : {&c->maskv, &c->ltag, &c->lpos, &c->dtag}) HIPCHK(b->alloc(N));
+    HIPCHK(c->act.alloc(c->ocap));
+    HIPCHK(c->crs.alloc(c->ocap));
+    HIPCHK(c->dlist.alloc(c->dcap));
+    HIPCHK(c->ctl.alloc(NCTL));
+    HIPCHK(c->tcount.alloc(2 * ((size_t)c->ocap / 256 + 2)));
+    HIPCHK(c->tcount2.alloc((size_t)c->nwords / 256 + 2));
+    HIPCHK(c->flags.alloc(NFLAG));
+    HIPCHK(c->flag.alloc(c->ocap));
+    for (DBuf<double>* b : {&c->edep, &c->E_cut, &c->E_nuc, &c->E_bnd, &c->k4w, &c->k4e}) HIPCHK(b->alloc(N));
+    HIPCHK(c->win.alloc(c->ocap));
+    HIPCHK(c->ein.alloc(c->ocap));
+    HIPCHK(c->acc_c.alloc((size_t)c->ocap * NACC));
+    HIPCHK(c->acc_f.alloc((size_t)c->ocap * NACC));
+    HIPCHK(c->part.alloc(((size_t)c->ocap / 256 + 2) * NR1));
+    HIPCHK(c->k5v.alloc(NV));
+    HIPCHK(c->hist.alloc((size_t)HROWS * NV));
+    HIPCHK(c->edep_sum.alloc(Ns));
+    for (DBuf<float>* b : {&c->w_cut, &c->w_nuc, &c->w_bnd}) HIPCHK(b->alloc(N));
+    HIPCHK(c->spots.alloc(n_spots));
+    HIPCHK(c->injacc.alloc((size_t)n_spots * 6));
+    HIPCHK(c->te.alloc(g->N_theta + 1));
+    HIPCHK(c->ee.alloc(g->N_E + 1));
+    for (DBuf<float>* b : {&c->R, &c->Sp, &c->lE, &c->lR, &c->lS}) HIPCHK(b->alloc(g->N_E));
+    HIPCHK(c->feA.alloc((size_t)n_spots * g->Nz));
+    HIPCHK(c->feB.alloc((size_t)n_spots * g->Nz));
+    HIPCHK(c->sx0.alloc(n_spots));
+    HIPCHK(c->rho.alloc(g->Nz));
+    HIPCHK(c->x0r.alloc(g->Nz));
+    c->src.assign((size_t)n_spots * 7, 0.0);
+    HIPCHK(hipMemsetAsync(c->maskv.p, 0, sizeof(uint32_t) * N, c->stream));
+    HIPCHK(hipMemsetAsync(c->ltag.p, 0, sizeof(uint32_t) * N, c->stream));
+    HIPCHK(hipMemsetAsync(c->dtag.p, 0, sizeof(uint32_t) * N, c->stream));
+    HIPCHK(hipMemsetAsync(c->ctl.p, 0, sizeof(uint32_t) * NCTL, c->stream));
+    HIPCHK(hipMemsetAsync(c->feA.p, 0, sizeof(float) * n_spots * g->Nz, c->stream));
+    HIPCHK(hipMemsetAsync(c->feB.p, 0, sizeof(float) * n_spots * g->Nz, c->stream));
+    HIPCHK(hipMemsetAsync(c->sx0.p, 0, sizeof(float) * n_spots, c->stream));
+    std::vector<float> ones(g->Nz, 1.0f);
+    HIPCHK(hipMemcpyAsync(c->rho.p, ones.data(), sizeof(float) * g->Nz, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->x0r.p, ones.data(), sizeof(float) * g->Nz, hipMemcpyHostToDevice, c->stream));
+    c->hint = (uint32_t)std::min<size_t>(c->lcap, (size_t)n_spots * g->Nx);
+    int rc = zero_state(c.get());
+    if (rc) return rc;
+    *out = c.release();
+    return SM2D_OK;
+}
+
+// FE moments + beam width of spot slot s from the host precompute (init_pipeline_state,
+// k1k6_pipeline.cu:1061-1212).
+int upload_fe_for(sm2d_ctx* c, int s, const sm2d_spot& sp, std::vector<float>* sigma_total) {
+    const int Nz = c->g.Nz;
+    std::vector<float> A(Nz), B(Nz), st(Nz);
+    int rc = sm2d_fe_precompute(sp.E0, sp.sigma_x, Nz, c->g.dz, A.data(), B.data(), st.data());
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->feA.p + (size_t)s * Nz, A.data(), sizeof(float) * Nz, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->feB.p + (size_t)s * Nz, B.data(), sizeof(float) * Nz, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->sx0.p + s, &sp.sigma_x, sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (sigma_total) *sigma_total = st;
+    return SM2D_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -398,82 +613,18 @@ extern "C" {
 const char* sm2d_last_error(void) { return sm2d_host::error(); }
 int sm2d_abi_version(void) { return SM2D_ABI_VERSION; }
 
-int sm2d_ctx_create(int device, const sm2d_grid* g, sm2d_ctx** out) {
-    if (!g || !out || g->struct_size != sizeof(sm2d_grid)) return err("sm2d_ctx_create: grid struct_size");
-    if (g->Nx <= 0 || g->Nz <= 0 || g->N_theta <= 0 || g->N_E <= 0 || g->N_theta > 4095 * g->n_theta_local)
-        return err("sm2d_ctx_create: bad grid");
-    LayoutId lid;
-    if (g->n_theta_local == 4 && g->n_e_local == 2 && g->n_x_sub == 8 && g->n_z_sub == 4) lid = L_REF;
-    else if (g->n_theta_local == 8 && g->n_e_local == 4 && g->n_x_sub == 4 && g->n_z_sub == 1) lid = L_CFG;
-    else return err("sm2d_ctx_create: local-bin layout must be (4,2,8,4) or (8,4,4,1)");
-    HIPCHK(hipSetDevice(device));
-    std::unique_ptr<sm2d_ctx> c(new sm2d_ctx());
-    c->device = device;
-    c->g = *g;
-    c->layout = lid;
-    c->LB = g->n_theta_local * g->n_e_local * g->n_x_sub * g->n_z_sub;
-    c->N = g->Nx * g->Nz;
-    const size_t N = c->N;
-    // Payload pool: enough for every cell of a 2-row beam front band with all 32 slots, at
-    // least 64 Ki slots, at most one slot table's worth.
-    size_t pc = std::max<size_t>(65536, (size_t)g->Nx * 2 * KB);
-    pc = std::min(pc, N * KB);
-    c->pcap = (uint32_t)pc;
-    c->lcap = (uint32_t)N;
-    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    for (int k = 0; k < 2; ++k) {
-        HIPCHK(c->bid[k].alloc(N * KB));
-        HIPCHK(c->ns[k].alloc(N));
-        HIPCHK(c->sref[k].alloc(N * KB));
-        HIPCHK(c->pcount[k].alloc(1));
-        HIPCHK(c->pool[k].alloc(pc * c->LB));
-    }
-    HIPCHK(c->bbid.alloc(N * 4 * KB));
-    HIPCHK(c->bn.alloc(N * 4));
-    HIPCHK(c->bsref.alloc(N * 4 * KB));
-    HIPCHK(c->bpcount.alloc(1));
-    HIPCHK(c->bpool.alloc(pc * c->LB));
-    HIPCHK(c->maskv.alloc(N));
-    HIPCHK(c->ltag.alloc(N));
-    HIPCHK(c->lpos.alloc(N));
-    HIPCHK(c->dtag.alloc(N));
-    HIPCHK(c->act.alloc(N));
-    HIPCHK(c->crs.alloc(N));
-    HIPCHK(c->dlist.alloc(N));
-    HIPCHK(c->counts.alloc(4));
-    HIPCHK(c->flag.alloc(N));
-    HIPCHK(c->bcount.alloc(2 * c->blocks256()));
-    for (DBuf<double>* b : {&c->edep, &c->E_cut, &c->E_nuc, &c->E_bnd, &c->win, &c->ein, &c->k4w, &c->k4e})
-        HIPCHK(b->alloc(N));
-    HIPCHK(c->acc_c.alloc(N * NACC));
-    HIPCHK(c->acc_f.alloc(N * NACC));
-    HIPCHK(c->part.alloc((size_t)c->blocks256() * NR1));
-    HIPCHK(c->k5v.alloc(NR1 + NACC + 2));
-    for (DBuf<float>* b : {&c->w_cut, &c->w_nuc, &c->w_bnd}) HIPCHK(b->alloc(N));
-    HIPCHK(c->flags.alloc(NFLAG));
-    HIPCHK(c->injacc.alloc(6));
-    HIPCHK(c->te.alloc(g->N_theta + 1));
-    HIPCHK(c->ee.alloc(g->N_E + 1));
-    for (DBuf<float>* b : {&c->R, &c->S, &c->lE, &c->lR, &c->lS}) HIPCHK(b->alloc(g->N_E));
-    for (DBuf<float>* b : {&c->feA, &c->feB, &c->rho, &c->x0r}) HIPCHK(b->alloc(g->Nz));
-    HIPCHK(hipMemsetAsync(c->maskv.p, 0, sizeof(uint32_t) * N, c->stream));
-    HIPCHK(hipMemsetAsync(c->ltag.p, 0, sizeof(uint32_t) * N, c->stream));
-    HIPCHK(hipMemsetAsync(c->dtag.p, 0, sizeof(uint32_t) * N, c->stream));
-    HIPCHK(hipMemsetAsync(c->feA.p, 0, sizeof(float) * g->Nz, c->stream));
-    HIPCHK(hipMemsetAsync(c->feB.p, 0, sizeof(float) * g->Nz, c->stream));
-    std::vector<float> ones(g->Nz, 1.0f);
-    HIPCHK(hipMemcpyAsync(c->rho.p, ones.data(), sizeof(float) * g->Nz, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->x0r.p, ones.data(), sizeof(float) * g->Nz, hipMemcpyHostToDevice, c->stream));
-    int rc = zero_state(c.get());
-    if (rc) return rc;
-    *out = c.release();
-    return SM2D_OK;
+int sm2d_ctx_create(int device, const sm2d_grid* g, sm2d_ctx** out) { return create(device, g, 1, out); }
+
+int sm2d_ctx_create_batch(int device, const sm2d_grid* g, int n_spots, sm2d_ctx** out) {
+    return create(device, g, n_spots, out);
 }
 
 int sm2d_ctx_destroy(sm2d_ctx* c) {
     if (!c) return SM2D_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    for (hipEvent_t& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
     hipStream_t s = c->stream;
     delete c;
     if (s) (void)hipStreamDestroy(s);
@@ -488,7 +639,7 @@ int sm2d_lut_upload(sm2d_ctx* c, const float* te, const float* ee, const float* 
     HIPCHK(hipMemcpyAsync(c->te.p, te, sizeof(float) * (NT + 1), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->ee.p, ee, sizeof(float) * (NE + 1), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->R.p, R, sizeof(float) * NE, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->S.p, S, sizeof(float) * NE, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->Sp.p, S, sizeof(float) * NE, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->lE.p, lE, sizeof(float) * NE, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->lR.p, lR, sizeof(float) * NE, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->lS.p, lS, sizeof(float) * NE, hipMemcpyHostToDevice, c->stream));
@@ -536,7 +687,8 @@ int sm2d_inject(sm2d_ctx* c, const sm2d_spot* sp, sm2d_audit* au) {
     if (rc) return rc;
     if (!sp || sp->struct_size != sizeof(sm2d_spot)) return err("spot struct_size");
     HIPCHK(hipSetDevice(c->device));
-    rc = dispatch(c, [&](auto ly) { return inject<decltype(ly)>(c, sp); });
+    HIPCHK(hipMemcpyAsync(c->sx0.p, &sp->sigma_x, sizeof(float), hipMemcpyHostToDevice, c->stream));
+    rc = dispatch(c, [&](auto ly) { return inject_all<decltype(ly)>(c, sp, 1); });
     if (rc) return rc;
     if (au) {
         au->W_source_injected = c->src[0];
@@ -574,14 +726,18 @@ int sm2d_psi_load(sm2d_ctx* c, int n, const int32_t* cell, const int32_t* slot, 
         hn[cl] = std::max<uint32_t>(hn[cl], (uint32_t)s + 1);
         pool[(size_t)where[k] * LB + lidx[i]] = w[i];
     }
-    for (size_t cl = 0; cl < N; ++cl)
+    std::vector<uint32_t> occ;
+    for (size_t cl = 0; cl < N; ++cl) {
         for (uint32_t s = 0; s < hn[cl]; ++s) {
             size_t k = cl * KB + s;
             if (where[k] < 0) return err("psi_load: slots of a cell must be a prefix");
             hs[k] = (uint32_t)where[k];
         }
+        if (hn[cl]) occ.push_back((uint32_t)cl);
+    }
     uint32_t used = (uint32_t)(pool.size() / LB);
     if (used > c->pcap) return err("psi_load: pool capacity", SM2D_ERR_CAPACITY);
+    if (occ.size() > c->ocap) return err("psi_load: occupied-cell capacity", SM2D_ERR_CAPACITY);
     rc = zero_state(c);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(c->bid[0].p, hb.data(), sizeof(uint32_t) * hb.size(), hipMemcpyHostToDevice, c->stream));
@@ -590,6 +746,10 @@ int sm2d_psi_load(sm2d_ctx* c, int n, const int32_t* cell, const int32_t* slot, 
     if (!pool.empty())
         HIPCHK(hipMemcpyAsync(c->pool[0].p, pool.data(), sizeof(float) * pool.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->pcount[0].p, &used, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    if (!occ.empty())
+        HIPCHK(hipMemcpyAsync(c->occ[0].p, occ.data(), sizeof(uint32_t) * occ.size(), hipMemcpyHostToDevice, c->stream));
+    uint32_t no = (uint32_t)occ.size();
+    HIPCHK(hipMemcpyAsync(c->nocc(0), &no, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return SM2D_OK;
 }
@@ -639,44 +799,44 @@ int sm2d_k1k2_mask_compact(sm2d_ctx* c, const sm2d_transport* tp, const uint8_t*
     if (rc) return rc;
     if (!tp || tp->struct_size != sizeof(sm2d_transport)) return err("transport struct_size");
     HIPCHK(hipSetDevice(c->device));
-    const uint32_t ep = ++c->epoch;
+    uint32_t ctl[NCTL];
+    HIPCHK(hipMemcpyAsync(ctl, c->ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const uint32_t ep = ctl[C_EP] + 1;  // the epoch k_begin is about to open
     std::vector<uint32_t> mv(c->N, 0);
     if (mask_prev)
         for (int i = 0; i < c->N; ++i) mv[i] = mask_prev[i] ? (((ep - 1u) << 1) | 1u) : 0u;
     HIPCHK(hipMemcpyAsync(c->maskv.p, mv.data(), sizeof(uint32_t) * c->N, hipMemcpyHostToDevice, c->stream));
     Tp P = make_tp(c, tp);
-    uint32_t cnt[2];
-    rc = dispatch(c, [&](auto ly) { return mask_compact<decltype(ly)>(c, P, ep, cnt); });
-    if (rc) return rc;
-    std::vector<uint32_t> a(cnt[0] ? cnt[0] : 1), k(cnt[1] ? cnt[1] : 1);
+    k_ctl_reset<<<1, 1, 0, c->stream>>>(c->ctl.p);
+    k_begin<<<1, 64, 0, c->stream>>>(c->ctl.p);
+    const int in = c->in;
+    const int gm = blocks_for(c->ocap, 256, 4096);
+    dispatch(c, [&](auto ly) {
+        using LY = decltype(ly);
+        k_mask<LY><<<gm, 256, 0, c->stream>>>(c->tabs(), P, c->psi(in), c->occ[in].p, c->nocc(in), c->maskv.p,
+                                              c->ctl.p, c->flag.p, c->tcount.p, c->win.p, c->ein.p);
+        return 0;
+    });
+    k_scan<2><<<1, 1024, 0, c->stream>>>(c->tcount.p, c->nocc(in), 256, c->ctl.p, nullptr, c->lcap, c->flags.p);
+    k_emit<<<gm, 256, 0, c->stream>>>(c->occ[in].p, c->nocc(in), c->flag.p, c->tcount.p, c->act.p, c->crs.p,
+                                      c->ltag.p, c->lpos.p, c->bitmap.p, c->ctl.p);
+    k_fill_u32<<<blocks_for(c->nwords, 256, 4096), 256, 0, c->stream>>>(c->bitmap.p, 0u, c->nwords);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ctl, c->ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const uint32_t na = ctl[C_NACT], nc = ctl[C_NCRS];
+    std::vector<uint32_t> a(na ? na : 1), k(nc ? nc : 1);
     HIPCHK(hipMemcpyAsync(mv.data(), c->maskv.p, sizeof(uint32_t) * c->N, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(a.data(), c->act.p, sizeof(uint32_t) * cnt[0], hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(k.data(), c->crs.p, sizeof(uint32_t) * cnt[1], hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(a.data(), c->act.p, sizeof(uint32_t) * na, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(k.data(), c->crs.p, sizeof(uint32_t) * nc, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (mask_out)
         for (int i = 0; i < c->N; ++i) mask_out[i] = (mv[i] == ((ep << 1) | 1u)) ? 1 : 0;
-    for (uint32_t i = 0; i < cnt[0]; ++i) active[i] = (int32_t)a[i];
-    for (uint32_t i = 0; i < cnt[1]; ++i) coarse[i] = (int32_t)k[i];
-    *n_active = (int)cnt[0];
-    *n_coarse = (int)cnt[1];
-    return SM2D_OK;
-}
-
-static int download_tallies(sm2d_ctx* c, double* t) {
-    const size_t N = c->N;
-    std::vector<float> f(N);
-    HIPCHK(hipMemcpyAsync(t + 0 * N, c->edep.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(t + 2 * N, c->E_cut.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(t + 4 * N, c->E_nuc.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(t + 6 * N, c->E_bnd.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
-    const float* src[3] = {c->w_cut.p, c->w_nuc.p, c->w_bnd.p};
-    const int dst[3] = {1, 3, 5};
-    for (int k = 0; k < 3; ++k) {
-        HIPCHK(hipMemcpyAsync(f.data(), src[k], sizeof(float) * N, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        for (size_t i = 0; i < N; ++i) t[dst[k] * N + i] = f[i];
-    }
-    HIPCHK(hipStreamSynchronize(c->stream));
+    for (uint32_t i = 0; i < na; ++i) active[i] = (int32_t)a[i];
+    for (uint32_t i = 0; i < nc; ++i) coarse[i] = (int32_t)k[i];
+    *n_active = (int)na;
+    *n_coarse = (int)nc;
     return SM2D_OK;
 }
 
@@ -690,6 +850,7 @@ int sm2d_run_iterations(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int s
         std::memset(au, 0, sizeof(*au));
         au->struct_size = sizeof(*au);
     }
+    HIPCHK(hipMemcpyAsync(c->sx0.p, &tp->sigma_x_initial, sizeof(float), hipMemcpyHostToDevice, c->stream));
     rc = dispatch(c, [&](auto ly) { return run_iters<decltype(ly)>(c, tp, n_iter, src_terms, last, au); });
     if (rc) return rc;
     if (au) {
@@ -697,7 +858,7 @@ int sm2d_run_iterations(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int s
         HIPCHK(hipMemcpy(f, c->flags.p, sizeof(f), hipMemcpyDeviceToHost));
         au->unsupported_events = f[F_UNSUPPORTED];
     }
-    if (tallies) return download_tallies(c, tallies);
+    if (tallies) return download_tallies(c, tallies, 0);
     return SM2D_OK;
 }
 
@@ -707,36 +868,14 @@ int sm2d_k5_audit(sm2d_ctx* c, const double* td, double Wso, double Wss, double 
     if (rc) return rc;
     if (!td || !out) return err("sm2d_k5_audit: null argument");
     HIPCHK(hipSetDevice(c->device));
-    // psi_in / psi_out totals on the device, tally deltas summed on the host in cell order
-    auto totals = [&](int k, double& W, double& E) -> int {
-        const int nb = c->blocks256();
-        Psi p = c->psi(k);
-        int r2 = dispatch(c, [&](auto ly) {
-            k_audit_cells<decltype(ly)><<<nb, 256, 0, c->stream>>>(c->tabs(), p, c->win.p, c->ein.p, c->k4w.p,
-                                                                   c->k4e.p, c->part.p);
-            return 0;
-        });
-        (void)r2;
-        k_fill_u32<<<1, 64, 0, c->stream>>>(c->counts.p, 0u, 2);
-        k_audit_final<<<1, 256, 0, c->stream>>>(c->part.p, nb, c->acc_c.p, c->counts.p + 1, c->acc_f.p,
-                                                c->counts.p, c->k5v.p);
-        std::vector<double> v(NR1 + NACC + 2);
-        HIPCHK(hipMemcpyAsync(v.data(), c->k5v.p, sizeof(double) * v.size(), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        W = v[R_WOUT];
-        E = v[R_EOUT];
-        return SM2D_OK;
-    };
-    HIPCHK(hipMemsetAsync(c->win.p, 0, sizeof(double) * c->N, c->stream));
-    HIPCHK(hipMemsetAsync(c->ein.p, 0, sizeof(double) * c->N, c->stream));
     double Wi, Ei, Wo, Eo;
-    rc = totals(c->in, Wi, Ei);
+    rc = dispatch(c, [&](auto ly) { return buffer_totals<decltype(ly)>(c, c->in, Wi, Ei); });
     if (rc) return rc;
-    rc = totals(1 - c->in, Wo, Eo);
+    rc = dispatch(c, [&](auto ly) { return buffer_totals<decltype(ly)>(c, 1 - c->in, Wo, Eo); });
     if (rc) return rc;
-    const size_t N = c->N;
-    std::vector<double> v(NR1 + NACC + 2, 0.0);
-    double* a = v.data() + NR1;
+    const size_t N = c->Ns;
+    double v[NV] = {};
+    double* a = v + NR1;
     for (size_t i = 0; i < N; ++i) {
         a[A_EDEP] += std::max(0.0, td[0 * N + i]);
         a[A_WCUT] += std::max(0.0, td[1 * N + i]);
@@ -762,6 +901,56 @@ int sm2d_swap(sm2d_ctx* c) {
     return SM2D_OK;
 }
 
+static int finish_audit(sm2d_ctx* c, sm2d_audit* au, const sm2d_audit& it, int n_spots) {
+    au->iterations = it.iterations;
+    au->terminated = it.terminated;
+    au->W_fail_iters = it.W_fail_iters;
+    au->E_fail_iters = it.E_fail_iters;
+    au->max_W_error = it.max_W_error;
+    au->max_E_error = it.max_E_error;
+    au->W_transport_drop = it.W_transport_drop;
+    au->E_transport_drop = it.E_transport_drop;
+    au->E_audit_residual = it.E_audit_residual;
+    au->max_cell_W_closure = it.max_cell_W_closure;
+    au->max_cell_E_closure = it.max_cell_E_closure;
+    au->cell_bin_updates = it.cell_bin_updates;
+    au->processed_components = it.processed_components;
+    au->processed_cells = it.processed_cells;
+    for (int s = 0; s < n_spots; ++s) {
+        au->W_source_injected += c->src[s * 7 + 0];
+        au->W_source_out_of_grid += c->src[s * 7 + 1];
+        au->W_source_slot_drop += c->src[s * 7 + 2];
+        au->E_source_injected += c->src[s * 7 + 3];
+        au->E_source_out_of_grid += c->src[s * 7 + 4];
+        au->E_source_slot_drop += c->src[s * 7 + 5];
+        au->E_representation_loss += c->src[s * 7 + 6];
+    }
+    if (!au->terminated) {
+        int rc = dispatch(c, [&](auto ly) { return buffer_totals<decltype(ly)>(c, c->in, au->W_leftover, au->E_leftover); });
+        if (rc) return rc;
+    }
+    uint32_t f[NFLAG];
+    HIPCHK(hipMemcpy(f, c->flags.p, sizeof(f), hipMemcpyDeviceToHost));
+    au->unsupported_events = f[F_UNSUPPORTED];
+    if (f[F_UNSUPPORTED])
+        return err("lateral emission into another listed cell's bucket is not supported on the GPU path",
+                   SM2D_ERR_UNSUPPORTED);
+    return SM2D_OK;
+}
+
+// Totals of the per-cell tallies of spots [0, n) (FP64, on the host from a device reduction).
+static int tally_totals(sm2d_ctx* c, int n, double* s7) {
+    for (int k = 0; k < 7; ++k) s7[k] = 0.0;
+    std::vector<double> t(7 * (size_t)c->Ns);
+    for (int s = 0; s < n; ++s) {
+        int rc = download_tallies(c, t.data(), s);
+        if (rc) return rc;
+        for (int k = 0; k < 7; ++k)
+            for (int i = 0; i < c->Ns; ++i) s7[k] += t[(size_t)k * c->Ns + i];
+    }
+    return SM2D_OK;
+}
+
 int sm2d_run_spot(sm2d_ctx* c, const sm2d_spot* sp, const sm2d_transport* tp, double* edep_out, sm2d_audit* au) {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -775,69 +964,84 @@ int sm2d_run_spot(sm2d_ctx* c, const sm2d_spot* sp, const sm2d_transport* tp, do
     HIPCHK(hipSetDevice(c->device));
     rc = zero_state(c);
     if (rc) return rc;
-    rc = sm2d_inject(c, sp, au);
+    rc = upload_fe_for(c, 0, *sp, nullptr);
     if (rc) return rc;
-    sm2d_transport t2 = *tp;
-    t2.sigma_x_initial = sp->sigma_x;
+    rc = dispatch(c, [&](auto ly) { return inject_all<decltype(ly)>(c, sp, 1); });
+    if (rc) return rc;
     sm2d_audit it;
     std::memset(&it, 0, sizeof(it));
-    rc = dispatch(c, [&](auto ly) { return run_iters<decltype(ly)>(c, &t2, tp->max_iterations, 1, nullptr, &it); });
+    rc = dispatch(c, [&](auto ly) { return run_iters<decltype(ly)>(c, tp, tp->max_iterations, 1, nullptr, &it); });
     if (rc) return rc;
-    // keep the source block, merge the transport block
-    au->iterations = it.iterations;
-    au->terminated = it.terminated;
-    au->W_fail_iters = it.W_fail_iters;
-    au->E_fail_iters = it.E_fail_iters;
-    au->max_W_error = it.max_W_error;
-    au->max_E_error = it.max_E_error;
-    au->W_transport_drop = it.W_transport_drop;
-    au->E_transport_drop = it.E_transport_drop;
-    au->E_audit_residual = it.E_audit_residual;
-    au->max_cell_W_closure = it.max_cell_W_closure;
-    au->max_cell_E_closure = it.max_cell_E_closure;
-    au->cell_bin_updates = it.cell_bin_updates;
-    au->processed_cells = it.processed_cells;
-    au->processed_components = it.processed_components;
-    std::vector<double> t(7 * (size_t)c->N);
-    rc = download_tallies(c, t.data());
+    double s7[7];
+    rc = tally_totals(c, 1, s7);
     if (rc) return rc;
-    const size_t N = c->N;
-    double s[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int k = 0; k < 7; ++k)
-        for (size_t i = 0; i < N; ++i) s[k] += t[k * N + i];
-    au->E_dep = s[0];
-    au->W_cutoff = s[1];
-    au->E_cutoff = s[2];
-    au->W_nuclear = s[3];
-    au->E_nuclear = s[4];
-    au->W_boundary = s[5];
-    au->E_boundary = s[6];
-    if (!au->terminated) {
-        // phase space left when max_iterations stopped the loop (not accounted by the reference)
-        std::vector<double> v;
-        const int nb = c->blocks256();
-        int r2 = dispatch(c, [&](auto ly) {
-            k_audit_cells<decltype(ly)><<<nb, 256, 0, c->stream>>>(c->tabs(), c->psi(c->in), c->win.p, c->ein.p,
-                                                                   c->k4w.p, c->k4e.p, c->part.p);
-            return 0;
-        });
-        (void)r2;
-        k_fill_u32<<<1, 64, 0, c->stream>>>(c->counts.p, 0u, 2);
-        k_audit_final<<<1, 256, 0, c->stream>>>(c->part.p, nb, c->acc_c.p, c->counts.p + 1, c->acc_f.p,
-                                                c->counts.p, c->k5v.p);
-        v.assign(NR1 + NACC + 2, 0.0);
-        HIPCHK(hipMemcpyAsync(v.data(), c->k5v.p, sizeof(double) * v.size(), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        au->W_leftover = v[R_WOUT];
-        au->E_leftover = v[R_EOUT];
+    au->E_dep = s7[0]; au->W_cutoff = s7[1]; au->E_cutoff = s7[2]; au->W_nuclear = s7[3];
+    au->E_nuclear = s7[4]; au->W_boundary = s7[5]; au->E_boundary = s7[6];
+    if (edep_out)
+        HIPCHK(hipMemcpy(edep_out, c->edep.p, sizeof(double) * c->Ns, hipMemcpyDeviceToHost));
+    return finish_audit(c, au, it, 1);
+}
+
+int sm2d_run_batch(sm2d_ctx* c, const sm2d_spot* spots, int n, const sm2d_transport* tp, double* edep_sum,
+                   float* edep_per_spot, sm2d_audit* au) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!tp || tp->struct_size != sizeof(sm2d_transport)) return err("transport struct_size");
+    if (!spots || n <= 0 || n > c->S) return err("sm2d_run_batch: 1 <= n_spots <= the context's spot slots");
+    for (int s = 0; s < n; ++s)
+        if (spots[s].struct_size != sizeof(sm2d_spot)) return err("spot struct_size");
+    if (tp->max_iterations <= 0) return err("max_iterations must be positive");
+    sm2d_audit local;
+    if (!au) au = &local;
+    std::memset(au, 0, sizeof(*au));
+    au->struct_size = sizeof(*au);
+    HIPCHK(hipSetDevice(c->device));
+    rc = zero_state(c);
+    if (rc) return rc;
+    for (int s = 0; s < n; ++s) {
+        rc = upload_fe_for(c, s, spots[s], nullptr);
+        if (rc) return rc;
     }
-    uint32_t f[NFLAG];
-    HIPCHK(hipMemcpy(f, c->flags.p, sizeof(f), hipMemcpyDeviceToHost));
-    au->unsupported_events = f[F_UNSUPPORTED];
-    if (edep_out) std::memcpy(edep_out, t.data(), sizeof(double) * N);
-    if (f[F_UNSUPPORTED])
-        return err("lateral emission into another listed cell's bucket is not supported on the GPU path",
-                   SM2D_ERR_UNSUPPORTED);
+    rc = dispatch(c, [&](auto ly) { return inject_all<decltype(ly)>(c, spots, n); });
+    if (rc) return rc;
+    sm2d_audit it;
+    std::memset(&it, 0, sizeof(it));
+    rc = dispatch(c, [&](auto ly) { return run_iters<decltype(ly)>(c, tp, tp->max_iterations, 1, nullptr, &it); });
+    if (rc) return rc;
+    DBuf<float> per;
+    if (edep_per_spot) HIPCHK(per.alloc((size_t)n * c->Ns));
+    k_edep_reduce<<<blocks_for(c->Ns, 256, 4096), 256, 0, c->stream>>>(c->edep.p, n, c->Ns, c->edep_sum.p,
+                                                                       edep_per_spot ? per.p : nullptr);
+    HIPCHK(hipGetLastError());
+    if (edep_sum)
+        HIPCHK(hipMemcpyAsync(edep_sum, c->edep_sum.p, sizeof(double) * c->Ns, hipMemcpyDeviceToHost, c->stream));
+    if (edep_per_spot)
+        HIPCHK(hipMemcpyAsync(edep_per_spot, per.p, sizeof(float) * n * c->Ns, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double s7[7];
+    rc = tally_totals(c, n, s7);
+    if (rc) return rc;
+    au->E_dep = s7[0]; au->W_cutoff = s7[1]; au->E_cutoff = s7[2]; au->W_nuclear = s7[3];
+    au->E_nuclear = s7[4]; au->W_boundary = s7[5]; au->E_boundary = s7[6];
+    return finish_audit(c, au, it, n);
+}
+
+int sm2d_set_profiling(sm2d_ctx* c, int on) {
+    if (!c) return err("null context");
+    HIPCHK(hipSetDevice(c->device));
+    for (hipEvent_t& e : c->ev)
+        if (!e) HIPCHK(hipEventCreate(&e));
+    c->prof = on != 0;
+    std::memset(&c->timing, 0, sizeof(c->timing));
+    c->timing.struct_size = sizeof(c->timing);
+    c->timing.enabled = c->prof ? 1 : 0;
+    for (bool& u : c->ev_used) u = false;
+    return SM2D_OK;
+}
+
+int sm2d_get_timing(sm2d_ctx* c, sm2d_timing* out) {
+    if (!c || !out || out->struct_size != sizeof(sm2d_timing)) return err("sm2d_get_timing: struct_size");
+    *out = c->timing;
     return SM2D_OK;
 }
 

@@ -136,14 +136,18 @@ class Library:
 class Engine:
     """One context (grid + tables) of a Library."""
 
-    def __init__(self, lib: Library, grid: abi.Grid, tables: Tables, device: int = 0):
+    def __init__(self, lib: Library, grid: abi.Grid, tables: Tables, device: int = 0,
+                 n_spots: int = 1):
         self.L = lib
         self.grid = grid
         self.tables = tables
         self.N = grid.Nx * grid.Nz
         self.LB = grid.local_bins
         h = C.c_void_p()
-        if lib.prefix == "sm2d_":
+        self.n_spots = n_spots
+        if n_spots > 1:
+            lib.call("ctx_create_batch", device, C.byref(grid), n_spots, C.byref(h))
+        elif lib.prefix == "sm2d_":
             lib.call("ctx_create", device, C.byref(grid), C.byref(h))
         else:
             lib.call("ctx_create", C.byref(grid), C.byref(h))
@@ -246,8 +250,28 @@ class Engine:
     def swap(self):
         self.L.call("swap", self.h)
 
+    def set_profiling(self, on: bool = True):
+        self.L.call("set_profiling", self.h, int(on))
+
+    def timing(self) -> abi.Timing:
+        t = abi.Timing()
+        self.L.call("get_timing", self.h, C.byref(t))
+        return t
+
     def run_spot(self, spot: abi.Spot, tp: abi.Transport):
         edep = np.zeros((self.grid.Nz, self.grid.Nx), np.float64)
         a = abi.Audit()
         self.L.call("run_spot", self.h, C.byref(spot), C.byref(tp), _d(edep), C.byref(a))
         return edep, a
+
+    def run_batch(self, spots, tp: abi.Transport, per_spot: bool = False):
+        """Transports `spots` together; returns (summed EdepC [Nz, Nx] FP64, per-spot EdepC
+        [n, Nz, Nx] FP32 or None, batch audit)."""
+        n = len(spots)
+        arr = (abi.Spot * n)(*spots)
+        edep = np.zeros((self.grid.Nz, self.grid.Nx), np.float64)
+        per = np.zeros((n, self.grid.Nz, self.grid.Nx), np.float32) if per_spot else None
+        a = abi.Audit()
+        self.L.call("run_batch", self.h, arr, n, C.byref(tp), _d(edep),
+                    _f(per) if per is not None else None, C.byref(a))
+        return edep, per, a

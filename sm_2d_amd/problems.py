@@ -106,13 +106,15 @@ def slab_materials(Nz: int, dz: float, seed: int = 7):
     return rho, x0r
 
 
-def setup_engine(lib: Library, prob: Problem, engine_cls=None, spot_index: int = 0):
-    """Builds tables, creates a context and uploads FE moments for spot `spot_index`."""
+def setup_engine(lib: Library, prob: Problem, engine_cls=None, spot_index: int = 0,
+                 n_spots: int = 1):
+    """Builds tables, creates a context (of `n_spots` spot slots) and uploads FE moments for
+    spot `spot_index` (the stage entry points use them; full-spot calls compute their own)."""
     from .engine import Engine
     cls = engine_cls or Engine
     tab = prob.tables(lib)
     grid = prob.make_grid(len(tab.E_rep))
-    eng = cls(lib, grid, tab)
+    eng = cls(lib, grid, tab, n_spots=n_spots) if n_spots > 1 else cls(lib, grid, tab)
     sp = prob.make_spot(spot_index)
     A, B, st = lib.fe_precompute(sp.E0, sp.sigma_x, grid.Nz, grid.dz)
     eng.upload_fe(A, B)

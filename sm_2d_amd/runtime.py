@@ -18,7 +18,13 @@ def load_library() -> Library:
         return _LIB
     if not os.path.exists(abi.LIB_PATH):
         raise RuntimeError(f"{abi.LIB_PATH} is missing; build it with `python -m sm_2d_amd.build`")
-    lib = Library(abi.LIB_PATH, "sm2d_")
+    lib = Library(abi.LIB_PATH, "sm2d_", {
+        "set_profiling": [C.c_void_p, C.c_int],
+        "get_timing": [C.c_void_p, C.POINTER(abi.Timing)],
+        "ctx_create_batch": [C.c_int, C.POINTER(abi.Grid), C.c_int, C.POINTER(C.c_void_p)],
+        "run_batch": [C.c_void_p, C.POINTER(abi.Spot), C.c_int, C.POINTER(abi.Transport),
+                      abi.dp, abi.fp, C.POINTER(abi.Audit)],
+    })
     lib.lib.sm2d_abi_version.restype = C.c_int
     if lib.lib.sm2d_abi_version() != 1:
         raise RuntimeError("libsm2d_hip.so ABI version mismatch")

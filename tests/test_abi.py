@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
 
 
 STRUCTS = {"sm2d_grid": abi.Grid, "sm2d_transport": abi.Transport, "sm2d_spot": abi.Spot,
-           "sm2d_audit": abi.Audit, "sm2d_k5_report": abi.K5Report}
+           "sm2d_audit": abi.Audit, "sm2d_k5_report": abi.K5Report, "sm2d_timing": abi.Timing}
 
 
 def test_ctypes_structs_match_c_layout(tmp_path):

@@ -161,3 +161,38 @@ def test_full_spot_bit_exact(hip, orc, prob):
         assert getattr(ag, k) == pytest.approx(getattr(ao, k), rel=1e-12, abs=1e-15), k
     assert ag.max_cell_W_closure < 1e-6
     assert ag.W_fail_iters == 0
+
+
+def test_batched_spots_match_single_spots(hip, orc):
+    """Spots transported together in one batch context give, spot by spot, the same EdepC as
+    the single-spot path (and hence the oracle), and the FP64 sum is taken in spot order."""
+    from oracle.pyoracle import OracleEngine
+    base = problems.reference_grid_case(Nx=40, Nz=60, E0=60.0, sigma_x=2.0, n_samples=300)
+    spots = []
+    for i, (E0, x0, sx) in enumerate([(60.0, 0.0, 2.0), (45.0, -3.0, 1.5), (70.0, 4.0, 3.0)]):
+        sp = base.make_spot()
+        sp.E0, sp.x0, sp.sigma_x, sp.random_seed = E0, x0, sx, 1234 + 7 * i
+        spots.append(sp)
+    tp = base.make_transport()
+    single = problems.setup_engine(hip, base, Engine)
+    oracle = problems.setup_engine(orc, base, OracleEngine)
+    ref, auds = [], []
+    for sp in spots:
+        e, a = single.run_spot(sp, tp)
+        eo, _ = oracle.run_spot(sp, tp)
+        assert np.array_equal(e, eo)
+        ref.append(e)
+        auds.append(a)
+    batch = problems.setup_engine(hip, base, Engine, n_spots=4)
+    esum, per, ab = batch.run_batch(spots, tp, per_spot=True)
+    expect = ref[0] + ref[1] + ref[2]
+    assert np.array_equal(esum, expect)
+    for k in range(3):
+        assert np.array_equal(per[k], ref[k].astype(np.float32))
+    assert ab.terminated == 1
+    assert ab.iterations == max(a.iterations for a in auds)
+    assert ab.cell_bin_updates == sum(a.cell_bin_updates for a in auds)
+    assert ab.W_fail_iters == 0
+    # the context is reusable: a second, smaller batch on the same context
+    esum2, _, _ = batch.run_batch(spots[1:2], tp)
+    assert np.array_equal(esum2, ref[1])
