@@ -106,6 +106,38 @@ def slab_materials(Nz: int, dz: float, seed: int = 7):
     return rho, x0r
 
 
+def config3(layout=LAYOUT_CFG, n_samples=1000, slab_seed: int = 4) -> Problem:
+    """230 MeV, sigma_x 5 mm, 1 % energy spread, 100 x 300 mm at 0.25 mm (400 x 1200) through
+    3-6 seeded water / bone / lung slabs, 256 log E bins, 512 theta bins over [-pi/2, pi/2]."""
+    p = Problem(
+        "config3_230MeV_slabs_400x1200",
+        _grid(400, 1200, 0.25, 0.25, 512, -math.pi / 2, math.pi / 2, layout),
+        ("log", 0.1, 250.0, 256),
+        dict(E_fine_on=10.0, E_fine_off=11.0, max_iterations=3000),
+        [dict(E0=230.0, sigma_x=5.0, sigma_E=2.3, sigma_theta=0.0, n_samples=n_samples,
+              random_seed=42)])
+    p.materials = slab_materials(1200, 0.25, slab_seed)
+    return p
+
+
+def config4(n_spots: int = 256, layout=LAYOUT_CFG, n_samples=1000, seed: int = 2024,
+            Nz: int = 600) -> Problem:
+    """Batch of pencil beams on water 100 x 300 mm at 0.5 mm (200 x 600): E0 ~ U(70, 230) MeV,
+    x0 ~ U(-40, 40) mm, sigma_x ~ U(3, 6) mm, 1 % energy spread, seeded."""
+    rng = np.random.default_rng(seed)
+    E0 = rng.uniform(70.0, 230.0, n_spots)
+    x0 = rng.uniform(-40.0, 40.0, n_spots)
+    sx = rng.uniform(3.0, 6.0, n_spots)
+    spots = [dict(E0=float(E0[i]), x0=float(x0[i]), sigma_x=float(sx[i]), sigma_E=0.01 * float(E0[i]),
+                  sigma_theta=0.0, n_samples=n_samples, random_seed=int(1000 + i))
+             for i in range(n_spots)]
+    return Problem(
+        f"config4_{n_spots}spots_200x{Nz}",
+        _grid(200, Nz, 0.5, 0.5, 512, -math.pi / 2, math.pi / 2, layout),
+        ("log", 0.1, 250.0, 256),
+        dict(E_fine_on=10.0, E_fine_off=11.0, max_iterations=2000), spots)
+
+
 def setup_engine(lib: Library, prob: Problem, engine_cls=None, spot_index: int = 0,
                  n_spots: int = 1):
     """Builds tables, creates a context (of `n_spots` spot slots) and uploads FE moments for
