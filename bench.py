@@ -90,6 +90,8 @@ def main() -> int:
     sp, tp = prob.make_spot(), prob.make_transport()
     for _ in range(args.warmup):
         eng.run_spot(sp, tp)
+    # HIP events around every stage of every iteration, on the library's stream; they are read
+    # once per chunk of queued iterations, so they do not change the launch pattern.
     eng.set_profiling(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

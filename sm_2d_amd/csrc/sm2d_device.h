@@ -29,6 +29,7 @@ struct Layout {
     static constexpr int TL = TL_, EL = EL_, XS = XS_, ZS = ZS_;
     static constexpr int LB = TL * EL * XS * ZS;
     static constexpr int PL = LB / 64;  // values per lane when a wave holds one slot
+    static constexpr int NL = 6144 / LB;  // LDS payload rows per transport wave (24 KiB)
     static_assert(LB % 64 == 0, "a slot must split evenly over a wavefront");
     __host__ __device__ static int enc(int t, int e, int x, int z) { return t + TL * (e + EL * (x + XS * z)); }
     __host__ __device__ static int lt(int l) { return l % TL; }

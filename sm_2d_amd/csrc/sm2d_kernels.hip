@@ -50,60 +50,101 @@ __global__ void k_clear(Psi p, const uint32_t* __restrict__ occ, const uint32_t*
 
 // ------------------------------------------------------------------------------------------
 // K1: mask + per-cell FP64 totals of psi_in over its occupied-cell list (ascending).  One
-// thread per occupied cell, serial sum in slot->lidx order so that the threshold test
-// W > w_min sees the reference's exact FP32 sum.  Tiles of 256 list entries; per-entry flag:
-// 1 = active (fine), 2 = coarse (mask 0 with weight), 0 = nothing.
+// wavefront per occupied cell: the slots' payloads are read coalesced and summed in FP64 with a
+// fixed butterfly.  The threshold test W > w_min is defined on the reference's serial FP32 sum
+// (slot -> lidx order); that sum differs from the exact sum of n <= 32*LB non-negative terms by
+// less than n * 2^-24 relative (< 5e-4), so the FP64 sum decides unless it lies within 1e-3 of
+// w_min (or a weight is negative), where lane 0 forms the serial FP32 sum itself.
+// Per-entry flag: 1 = active (fine), 2 = coarse (mask 0 with weight), 0 = nothing.
 // ------------------------------------------------------------------------------------------
 template <class LY>
-__global__ __launch_bounds__(256) void k_mask(Tabs T, Tp P, Psi in, const uint32_t* __restrict__ occ,
-                                              const uint32_t* nocc, uint32_t* maskv, const uint32_t* ctl,
-                                              uint8_t* flag, uint32_t* tcount, double* win, double* ein) {
+__global__ __launch_bounds__(64) void k_mask(Tabs T, Tp P, Psi in, const uint32_t* __restrict__ occ,
+                                             const uint32_t* nocc, uint32_t* maskv, const uint32_t* ctl,
+                                             uint8_t* flag, double* win, double* ein) {
     if (ctl[C_DONE]) return;
-    __shared__ uint32_t s_cnt[2];
     const uint32_t n = *nocc, ep = ctl[C_EP];
-    const uint32_t ntile = (n + 255) / 256;
-    for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
-        if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
-        __syncthreads();
-        const uint32_t i = tile * 256 + threadIdx.x;
-        uint8_t f = 0;
-        if (i < n) {
-            const uint32_t c = occ[i];
-            const uint32_t ns = in.ns[c];
-            double W64 = 0.0, E64 = 0.0;
-            float W = 0.0f;
-            bool on = false, off = false, has = false;
-            for (uint32_t s = 0; s < ns; ++s) {
-                uint32_t bid = in.bid[(size_t)c * KB + s];
-                if (bid == EMPTY) continue;
-                uint32_t bE = (bid >> 12) & 0xFFF;
-                if (bE <= (uint32_t)P.b_on) on = true;
-                if (bE <= (uint32_t)P.b_off) off = true;
-                const float* v = in.pool + (size_t)in.sref[(size_t)c * KB + s] * LY::LB;
-                for (int l = 0; l < LY::LB; ++l) {
-                    float w = v[l];
-                    W += w;
-                    if (w > 0.0f) {
-                        has = true;
-                        W64 += (double)w;
-                        int eb = min(max((int)bE * LY::EL + LY::le(l), 0), T.NE - 1);
-                        E64 += (double)w * (double)ecenter(T.ee, eb);
-                    }
+    const int l = lane_id();
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const uint32_t c = occ[i];
+        const uint32_t ns = in.ns[c];
+        const uint32_t mb = (l < (int)ns) ? in.bid[(size_t)c * KB + l] : EMPTY;
+        const uint32_t mr = (l < (int)ns) ? in.sref[(size_t)c * KB + l] : 0u;
+        const uint32_t bE = (mb >> 12) & 0xFFF;
+        const bool on = __ballot(mb != EMPTY && bE <= (uint32_t)P.b_on) != 0;
+        const bool off = __ballot(mb != EMPTY && bE <= (uint32_t)P.b_off) != 0;
+        double W64 = 0.0, E64 = 0.0;
+        bool neg = false;
+        for (uint32_t s = 0; s < ns; ++s) {
+            const uint32_t b = rlu(mb, (int)s);
+            if (b == EMPTY) continue;
+            const float* v = in.pool + (size_t)rlu(mr, (int)s) * LY::LB;
+            const int bEs = (int)((b >> 12) & 0xFFF);
+#pragma unroll
+            for (int k = 0; k < LY::PL; ++k) {
+                const int lidx = l + 64 * k;
+                const float w = v[lidx];
+                if (w > 0.0f) {
+                    W64 += (double)w;
+                    const int eb = min(max(bEs * LY::EL + LY::le(lidx), 0), T.NE - 1);
+                    E64 += (double)w * (double)ecenter(T.ee, eb);
+                } else if (w < 0.0f || w != w) {
+                    neg = true;
                 }
             }
+        }
+        W64 = wave_sum(W64);
+        E64 = wave_sum(E64);
+        const bool anyneg = __ballot(neg) != 0;
+        const double wm = (double)P.wmin;
+        bool pass;
+        if (!anyneg && W64 > wm * (1.0 + 1e-3)) pass = true;
+        else if (!anyneg && W64 < wm * (1.0 - 1e-3)) pass = false;
+        else {
+            float W = 0.0f;  // the reference's serial FP32 sum (k1_activemask.cu:24-34)
+            if (l == 0)
+                for (uint32_t s = 0; s < ns; ++s) {
+                    const uint32_t b = in.bid[(size_t)c * KB + s];
+                    if (b == EMPTY) continue;
+                    const float* v = in.pool + (size_t)in.sref[(size_t)c * KB + s] * LY::LB;
+                    for (int q = 0; q < LY::LB; ++q) W += v[q];
+                }
+            pass = rlf(W, 0) > P.wmin;
+        }
+        if (l == 0) {
+            uint8_t f = 0;
             if (ns) {
-                bool prev = maskv[c] == (((ep - 1u) << 1) | 1u);
-                bool m = (on || (prev && off)) && W > P.wmin;
+                const bool prev = maskv[c] == (((ep - 1u) << 1) | 1u);
+                const bool m = (on || (prev && off)) && pass;
                 maskv[c] = (ep << 1) | (m ? 1u : 0u);
-                f = m ? 1 : (has ? 2 : 0);
+                f = m ? 1 : (W64 > 0.0 ? 2 : 0);
             }
             flag[i] = f;
             win[i] = W64;
             ein[i] = E64;
         }
-        if (f) atomicAdd(&s_cnt[f - 1], 1u);  // block-local count; the order is irrelevant
+    }
+}
+
+// Per-tile (256 list entries) counts of the K1 flags, for the ordered compaction.
+__global__ __launch_bounds__(256) void k_flag_count(const uint8_t* __restrict__ flag, const uint32_t* nocc,
+                                                    uint32_t* tcount, const uint32_t* ctl) {
+    if (ctl[C_DONE]) return;
+    __shared__ uint32_t s_cnt[2][4];
+    const uint32_t n = *nocc;
+    const uint32_t ntile = (n + 255) / 256;
+    const int w = threadIdx.x >> 6, l = lane_id();
+    for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const uint32_t i = tile * 256 + threadIdx.x;
+        const uint8_t f = i < n ? flag[i] : 0;
+        const uint64_t ba = __ballot(f == 1), bc = __ballot(f == 2);
+        if (l == 0) {
+            s_cnt[0][w] = __popcll(ba);
+            s_cnt[1][w] = __popcll(bc);
+        }
         __syncthreads();
-        if (threadIdx.x < 2) tcount[tile * 2 + threadIdx.x] = s_cnt[threadIdx.x];
+        if (threadIdx.x < 2)
+            tcount[tile * 2 + threadIdx.x] =
+                s_cnt[threadIdx.x][0] + s_cnt[threadIdx.x][1] + s_cnt[threadIdx.x][2] + s_cnt[threadIdx.x][3];
         __syncthreads();
     }
 }
@@ -264,13 +305,6 @@ __global__ __launch_bounds__(256) void k_occ_emit(uint32_t* bitmap, uint32_t nwo
 // ------------------------------------------------------------------------------------------
 // K3 transport
 // ------------------------------------------------------------------------------------------
-struct WaveCell {
-    // psi_out of the own cell: lane s < 32 holds slot s
-    uint32_t obid, oref, n_out;
-    // the cell's four outflow buckets
-    uint32_t bbid[4], bref[4], bn[4];
-};
-
 // Finds desired in a 32-slot table held one slot per lane, else claims the next slot, else
 // falls back to the closest occupied block (device_psic.cuh:192-260, device_bucket.cuh:61-128).
 // Returns the slot (-1 when saturated and empty) and the selected block id.
@@ -333,67 +367,135 @@ struct CellAccum {
     uint32_t comps;
 };
 
-template <class LY>
-struct Ctx {
-    const Tabs& T;
-    Bkt& bk;
-    uint32_t li;
-    uint32_t* flags;
-    WaveCell& wc;
-    CellAccum& a;
+// Phase-space tables of one listed cell, held by its wavefront: table 0 is the cell's own
+// psi_out, tables 1..4 its outflow buckets of faces 0..3.  Lane s < 32 holds slot s (block id,
+// payload reference).  New payloads live in a per-wave LDS row pool (reference CACHED | row)
+// and are written to the global pools once, when the cell ends; slots beyond the LDS rows and
+// the psi_out slots that existed before the cell started use the global pools directly.
+constexpr uint32_t CACHED = 0x80000000u;
 
-    // device_emit_component_to_bucket_4d (device_bucket.cuh:431-533); wave-uniform arguments.
-    __device__ float emit4d(int face, float theta, float E, float weight, int xs, int zs) {
-        if (weight <= 0.0f) return 0.0f;
-        const int NT = T.NT, NE = T.NE;
-        const float tmin = T.te[0], tmax = T.te[NT];
-        const float dth = (tmax - tmin) / NT;
-        float tc = fmaxf(tmin, fminf(theta, tmax));
-        float c0 = tmin + 0.5f * dth;
-        int bc = (int)floorf((tc - c0) / dth + 0.5f);
-        bc = max(0, min(bc, NT - 1));
-        float delta = tc - (c0 + (float)bc * dth);
-        int tb1 = bc;
-        float tw0 = 1.0f, tw1 = 0.0f;
-        if (delta > 0.0f && bc < NT - 1) tb1 = bc + 1;
-        else if (delta < 0.0f && bc > 0) tb1 = bc - 1;
-        if (tb1 != bc) {
-            float fr = fminf(1.0f, fabsf(delta) / dth);
-            tw1 = fr * fr;
-            tw0 = 1.0f - tw1;
-        }
-        float Ec = fmaxf(T.ee[0], fminf(E, T.ee[NE]));
-        int eb = find_bin(T.ee, NE, Ec);
-        eb = max(0, min(eb, NE - 1));
-        const int el = eb % LY::EL;
-        const uint32_t bE = (uint32_t)(eb / LY::EL);
-        float dropped = 0.0f;
-        uint32_t base = (li * 4 + face) * KB;
-        for (int ti = 0; ti < 2; ++ti) {
-            float tw = ti == 0 ? tw0 : tw1;
-            if (tw <= 0.0f) continue;
-            float we = weight * tw;
-            int t = ti == 0 ? bc : tb1;
-            uint32_t bid = ((uint32_t)(t / LY::TL) & 0xFFF) | ((bE & 0xFFF) << 12);
-            int lidx = LY::enc(t % LY::TL, el, xs, zs);
-            uint32_t sel;
-            int s = table_find_or_claim(wc.bbid[face], wc.bref[face], wc.bn[face], bid, sel, bk.pool,
-                                        bk.pcount, bk.pcap, bk.bid + base, bk.sref + base, LY::LB,
-                                        flags, F_BKT_FULL);
-            if (s < 0) {
-                dropped += we;
-                continue;
+template <class LY>
+struct WTabs {
+    uint32_t b0, b1, b2, b3, b4;  // block id of slot (lane)
+    uint32_t r0, r1, r2, r3, r4;  // payload reference of slot (lane)
+    uint32_t n0, n1, n2, n3, n4;  // slots in use (wave-uniform)
+    uint32_t own0;                // own psi_out slots present when the cell started
+    uint32_t used;                // LDS rows in use (wave-uniform)
+    float* rows;                  // this wave's LDS rows [NL][LB]
+
+    __device__ uint32_t gb(int t) const { return t == 0 ? b0 : t == 1 ? b1 : t == 2 ? b2 : t == 3 ? b3 : b4; }
+    __device__ uint32_t gr(int t) const { return t == 0 ? r0 : t == 1 ? r1 : t == 2 ? r2 : t == 3 ? r3 : r4; }
+    __device__ uint32_t gn(int t) const { return t == 0 ? n0 : t == 1 ? n1 : t == 2 ? n2 : t == 3 ? n3 : n4; }
+    __device__ void sb(int t, uint32_t v) {
+        if (t == 0) b0 = v; else if (t == 1) b1 = v; else if (t == 2) b2 = v; else if (t == 3) b3 = v; else b4 = v;
+    }
+    __device__ void sr(int t, uint32_t v) {
+        if (t == 0) r0 = v; else if (t == 1) r1 = v; else if (t == 2) r2 = v; else if (t == 3) r3 = v; else r4 = v;
+    }
+    __device__ void sn(int t, uint32_t v) {
+        if (t == 0) n0 = v; else if (t == 1) n1 = v; else if (t == 2) n2 = v; else if (t == 3) n3 = v; else n4 = v;
+    }
+
+    // Same search / claim / closest-block fallback as table_find_or_claim; a claimed slot gets a
+    // zeroed LDS row while rows remain, else a zeroed global payload.
+    __device__ int claim(int t, uint32_t want, uint32_t& sel, float* gpool, uint32_t* gpcount, uint32_t gpcap,
+                         uint32_t* flags, int flag_full) {
+        const int l = lane_id();
+        sel = want;
+        const uint32_t mb = gb(t);
+        const uint64_t hit = __ballot(l < 32 && mb == want);
+        if (hit) return __ffsll((unsigned long long)hit) - 1;
+        const uint32_t n = gn(t);
+        if (n < (uint32_t)KB) {
+            const int s = (int)n;
+            uint32_t ref;
+            if (used < (uint32_t)LY::NL) {
+                ref = CACHED | used;
+                float* row = rows + (size_t)used * LY::LB;
+#pragma unroll
+                for (int k = 0; k < LY::PL; ++k) row[l + 64 * k] = 0.0f;
+                used = used + 1;
+            } else {
+                ref = 0;
+                if (l == 0) ref = atomicAdd(gpcount, 1u);
+                ref = rlu(ref, 0);
+                if (ref >= gpcap) {
+                    if (l == 0) atomicOr(&flags[flag_full], 1u);
+                    return -1;
+                }
+#pragma unroll
+                for (int k = 0; k < LY::PL; ++k) gpool[(size_t)ref * LY::LB + l + 64 * k] = 0.0f;
+                __threadfence_block();
             }
-            int l2 = (sel == bid) ? lidx : remap<LY>(lidx, bid, sel);
-            uint32_t ref = rlu(wc.bref[face], s);
-            if (lane_id() == 0) bk.pool[(size_t)ref * LY::LB + l2] += we;
-            a.cl_wout += we;
-            int ebw = min(max((int)((sel >> 12) & 0xFFF) * LY::EL + LY::le(l2), 0), NE - 1);
-            a.cl_Eout += (double)we * (double)ecenter(T.ee, ebw);
+            if (l == s) {
+                sb(t, want);
+                sr(t, ref);
+            }
+            sn(t, n + 1);
+            return s;
         }
-        return dropped;
+        int key = 0x7fffffff;
+        if (l < 32 && mb != EMPTY) {
+            int d = abs((int)((mb >> 12) & 0xFFF) - (int)((want >> 12) & 0xFFF)) * 8 +
+                    abs((int)(mb & 0xFFF) - (int)(want & 0xFFF));
+            key = (d << 5) | l;
+        }
+        key = wave_min(key);
+        if (key == 0x7fffffff) return -1;
+        const int s = key & 31;
+        sel = rlu(mb, s);
+        return s;
+    }
+
+    // payload[ref][lidx] += w (called by the lanes that add)
+    __device__ void add(uint32_t ref, int lidx, float w, float* gpool) const {
+        if (ref & CACHED) {
+            float* p = rows + (size_t)(ref & ~CACHED) * LY::LB + lidx;
+            *p += w;
+        } else {
+            gpool[(size_t)ref * LY::LB + lidx] += w;
+        }
     }
 };
+
+// Angular split and energy bin of an emission (device_emit_component_to_bucket_4d,
+// device_bucket.cuh:431-533): depends only on the emitted (theta, E), so it is evaluated in the
+// lane-parallel phase.
+struct Bins {
+    int bc, tb1, eb;
+    float tw0, tw1, Ecen;
+};
+
+__device__ inline Bins emission_bins(const Tabs& T, float theta, float E) {
+    const int NT = T.NT, NE = T.NE;
+    const float tmin = T.te[0], tmax = T.te[NT];
+    const float dth = (tmax - tmin) / NT;
+    Bins b;
+    float tc = fmaxf(tmin, fminf(theta, tmax));
+    float c0 = tmin + 0.5f * dth;
+    int bc = (int)floorf((tc - c0) / dth + 0.5f);
+    bc = max(0, min(bc, NT - 1));
+    float delta = tc - (c0 + (float)bc * dth);
+    int tb1 = bc;
+    float tw0 = 1.0f, tw1 = 0.0f;
+    if (delta > 0.0f && bc < NT - 1) tb1 = bc + 1;
+    else if (delta < 0.0f && bc > 0) tb1 = bc - 1;
+    if (tb1 != bc) {
+        float fr = fminf(1.0f, fabsf(delta) / dth);
+        tw1 = fr * fr;
+        tw0 = 1.0f - tw1;
+    }
+    float Ec = fmaxf(T.ee[0], fminf(E, T.ee[NE]));
+    int eb = find_bin(T.ee, NE, Ec);
+    eb = max(0, min(eb, NE - 1));
+    b.bc = bc;
+    b.tb1 = tb1;
+    b.tw0 = tw0;
+    b.tw1 = tw1;
+    b.eb = eb;
+    b.Ecen = ecenter(T.ee, eb);
+    return b;
+}
 
 // Wave-uniform broadcast of one component's results (lane j -> all lanes).
 struct Comp {
@@ -405,41 +507,158 @@ struct Comp {
     float sw[11]; // lateral weights (+z exit) or sub-bin fractions (remain)
     float wcf, wl, wr;
     int lidx;
+    Bins bn;      // emission bins of (th_new, E_new)
+    // face >= 0: g0 = E_emit, g1 = lateral drop with a zero centre drop, g2 = lateral boundary
+    //            weight (face 0); i0, i1 = x_sub, z_sub of the emission (faces 1..3)
+    // remain:    i0 = theta bin, i1 = E bin, i2 = z_sub, g0 = E bin centre
+    int i0, i1, i2;
+    float g0, g1, g2;
 };
 
+template <class LY>
+__device__ inline float emit(WTabs<LY>& W, CellAccum& a, const Tabs& T, Bkt& bk, uint32_t* flags, int face,
+                             const Bins& b, float weight, int xs, int zs) {
+    if (weight <= 0.0f) return 0.0f;
+    const int el = b.eb % LY::EL;
+    const uint32_t bE = (uint32_t)(b.eb / LY::EL);
+    float dropped = 0.0f;
+    for (int ti = 0; ti < 2; ++ti) {
+        const float tw = ti == 0 ? b.tw0 : b.tw1;
+        if (tw <= 0.0f) continue;
+        const float we = weight * tw;
+        const int t = ti == 0 ? b.bc : b.tb1;
+        const uint32_t bid = ((uint32_t)(t / LY::TL) & 0xFFF) | ((bE & 0xFFF) << 12);
+        const int lidx = LY::enc(t % LY::TL, el, xs, zs);
+        uint32_t sel;
+        const int s = W.claim(1 + face, bid, sel, bk.pool, bk.pcount, bk.pcap, flags, F_BKT_FULL);
+        if (s < 0) {
+            dropped += we;
+            continue;
+        }
+        const int l2 = (sel == bid) ? lidx : remap<LY>(lidx, bid, sel);
+        const uint32_t ref = rlu(W.gr(1 + face), s);
+        if (lane_id() == 0) W.add(ref, l2, we, bk.pool);
+        a.cl_wout += we;
+        float Ew = b.Ecen;
+        if (sel != bid) {
+            const int ebw = min(max((int)((sel >> 12) & 0xFFF) * LY::EL + LY::le(l2), 0), T.NE - 1);
+            Ew = ecenter(T.ee, ebw);
+        }
+        a.cl_Eout += (double)we * (double)Ew;
+    }
+    return dropped;
+}
+
+// End of a cell: one pool reservation per pool for the cell's LDS-cached slots, payload copies
+// LDS -> global (coalesced) and the table writes.  A pool overflow raises its flag and leaves
+// the affected tables without the new slots, so no later kernel reads past a pool.
+template <class LY>
+__device__ inline void flush_tables(WTabs<LY>& W, Psi& out, Bkt& bk, int cell, uint32_t li, uint32_t* flags) {
+    const int l = lane_id();
+    const uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
+    // own psi_out
+    const uint64_t mo = __ballot(l < 32 && (uint32_t)l >= W.own0 && (uint32_t)l < W.n0 && (W.r0 & CACHED));
+    uint32_t base = 0;
+    const uint32_t co = (uint32_t)__popcll(mo);
+    if (co) {
+        if (l == 0) base = atomicAdd(out.pcount, co);
+        base = rlu(base, 0);
+    }
+    bool ok = !co || base + co <= out.pcap;
+    if (!ok && l == 0) atomicOr(&flags[F_PSI_FULL], 1u);
+    if (ok) {
+        uint32_t ref = W.r0;
+        if ((mo >> l) & 1ull) ref = base + (uint32_t)__popcll(mo & below);
+        for (uint64_t m = mo; m; m &= m - 1) {
+            const int s = __ffsll((unsigned long long)m) - 1;
+            const float* src = W.rows + (size_t)(rlu(W.r0, s) & ~CACHED) * LY::LB;
+            float* dst = out.pool + (size_t)rlu(ref, s) * LY::LB;
+#pragma unroll
+            for (int k = 0; k < LY::PL; ++k) dst[l + 64 * k] = src[l + 64 * k];
+        }
+        if (l < 32 && (uint32_t)l >= W.own0 && (uint32_t)l < W.n0) {
+            out.bid[(size_t)cell * KB + l] = W.b0;
+            out.sref[(size_t)cell * KB + l] = ref;
+        }
+        if (l == 0) out.ns[cell] = W.n0;
+    }
+    // buckets of faces 0..3
+    uint64_t mf[4];
+    uint32_t cb = 0;
+    for (int f = 0; f < 4; ++f) {
+        mf[f] = __ballot(l < 32 && (uint32_t)l < W.gn(1 + f) && (W.gr(1 + f) & CACHED));
+        cb += (uint32_t)__popcll(mf[f]);
+    }
+    base = 0;
+    if (cb) {
+        if (l == 0) base = atomicAdd(bk.pcount, cb);
+        base = rlu(base, 0);
+    }
+    ok = !cb || base + cb <= bk.pcap;
+    if (!ok && l == 0) atomicOr(&flags[F_BKT_FULL], 1u);
+    uint32_t k0 = 0;
+    for (int f = 0; f < 4; ++f) {
+        const uint32_t nf = ok ? W.gn(1 + f) : 0u;
+        if (ok) {
+            uint32_t ref = W.gr(1 + f);
+            if ((mf[f] >> l) & 1ull) ref = base + k0 + (uint32_t)__popcll(mf[f] & below);
+            for (uint64_t m = mf[f]; m; m &= m - 1) {
+                const int s = __ffsll((unsigned long long)m) - 1;
+                const float* src = W.rows + (size_t)(rlu(W.gr(1 + f), s) & ~CACHED) * LY::LB;
+                float* dst = bk.pool + (size_t)rlu(ref, s) * LY::LB;
+#pragma unroll
+                for (int k = 0; k < LY::PL; ++k) dst[l + 64 * k] = src[l + 64 * k];
+            }
+            const uint32_t base_t = (li * 4 + f) * KB;
+            if (l < 32 && (uint32_t)l < nf) {
+                bk.bid[base_t + l] = W.gb(1 + f);
+                bk.sref[base_t + l] = ref;
+            }
+            k0 += (uint32_t)__popcll(mf[f]);
+        }
+        if (l == 0) bk.n[li * 4 + f] = nf;
+    }
+}
+
 template <class LY, bool FINE>
-__global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out, Bkt bk,
-                                                   const uint32_t* __restrict__ list,
-                                                   const uint32_t* __restrict__ nlist,
-                                                   const uint32_t* __restrict__ ltag, const uint32_t* ctl,
-                                                   Tally tal, double* acc, uint32_t* flags) {
+__global__ __launch_bounds__(64) void k_transport(Tabs T, Tp P, Psi in, Psi out, Bkt bk,
+                                                  const uint32_t* __restrict__ list,
+                                                  const uint32_t* __restrict__ nlist,
+                                                  const uint32_t* __restrict__ ltag, const uint32_t* ctl,
+                                                  Tally tal, double* acc, uint32_t* flags) {
     if (ctl[C_DONE]) return;
+    __shared__ float s_rows[LY::NL * LY::LB];
     const uint32_t tag = 2 * ctl[C_EP] + (FINE ? 1u : 0u);
     const int l = lane_id();
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
     const uint32_t n = *nlist;
     const int Nx = T.Nx, NT = T.NT, NE = T.NE;
     const float dx = T.dx, dz = T.dz, hdx = dx * 0.5f, hdz = dz * 0.5f;
     const float tmin = T.te[0], tmax = T.te[NT];
     const float dth = (tmax - tmin) / NT;
-    for (uint32_t li = wave; li < n; li += nw) {
+    for (uint32_t li = blockIdx.x; li < n; li += gridDim.x) {
         const int cell = (int)list[li];
         const int sp = cell / T.Ns, lc = cell - sp * T.Ns;  // spot, cell within the spot
         const int iz = lc / Nx, ix = lc - iz * Nx;
         const float rh = T.rho[iz];
         const float X0 = X0W * T.x0r[iz];
-        WaveCell wc;
-        wc.n_out = out.ns[cell];
-        wc.obid = (l < 32 && (uint32_t)l < wc.n_out) ? out.bid[(size_t)cell * KB + l] : EMPTY;
-        wc.oref = (l < 32 && (uint32_t)l < wc.n_out) ? out.sref[(size_t)cell * KB + l] : 0u;
-        for (int f = 0; f < 4; ++f) {
-            wc.bbid[f] = EMPTY;
-            wc.bref[f] = 0;
-            wc.bn[f] = 0;
+        WTabs<LY> W;
+        W.n0 = out.ns[cell];
+        W.own0 = W.n0;
+        W.b0 = (l < 32 && (uint32_t)l < W.n0) ? out.bid[(size_t)cell * KB + l] : EMPTY;
+        W.r0 = (l < 32 && (uint32_t)l < W.n0) ? out.sref[(size_t)cell * KB + l] : 0u;
+        W.b1 = W.b2 = W.b3 = W.b4 = EMPTY;
+        W.r1 = W.r2 = W.r3 = W.r4 = 0u;
+        W.n1 = W.n2 = W.n3 = W.n4 = 0u;
+        W.used = 0;
+        W.rows = s_rows;
+        // lateral +z neighbours listed in this phase (their buckets cannot take foreign emissions)
+        uint32_t latlisted;
+        {
+            const int off = l - 5, it = ix + off;
+            const bool q = l < 11 && off != 0 && it >= 0 && it < Nx && ltag[cell + off] == tag;
+            latlisted = (uint32_t)__ballot(q);
         }
         CellAccum a = {};
-        Ctx<LY> cx{T, bk, li, flags, wc, a};
         const uint32_t ns = in.ns[cell];
         const float A = T.feA[sp * T.Nz + iz], B = T.feB[sp * T.Nz + iz];
         const float sx0 = T.sx0[sp];
@@ -687,6 +906,57 @@ __global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out
                         }
                     }
                 }
+                // ------- emission binning and the table-independent parts of the apply -------
+                r.bn = Bins{0, 0, 0, 0.0f, 0.0f, 0.0f};
+                r.i0 = r.i1 = r.i2 = 0;
+                r.g0 = r.g1 = r.g2 = 0.0f;
+                if (v[k] > 0.0f && r.kind == 2 && r.E_new > 0.1f) {
+                    r.bn = emission_bins(T, r.th_new, r.E_new);
+                    const float w_new = r.w_new;
+                    if (r.face >= 0) {
+                        r.g0 = ecenter(T.ee, find_bin(T.ee, NE, r.E_new));
+                        if (r.face == 0) {
+                            float drop = 0.0f, lat_bnd = 0.0f;
+                            uint32_t unsup = 0;
+#pragma unroll
+                            for (int q = 0; q < 11; ++q) {
+                                const float wf = r.sw[q];
+                                if (wf <= 0.0f) continue;
+                                const int it = ix + q - 5;
+                                if (it < 0 || it >= Nx) {
+                                    lat_bnd += w_new * wf;
+                                    continue;
+                                }
+                                if (q == 5) {
+                                    drop += 0.0f;  // the centre emission's drop, zero unless a pool fills
+                                } else {
+                                    // another cell's +z bucket: a drop when that cell is not listed
+                                    // in this phase (device_bucket.cuh:953-963); a listed target
+                                    // would need a cross-wave ordered write, which this path rejects
+                                    if ((latlisted >> q) & 1u) ++unsup;
+                                    drop += w_new * wf;
+                                }
+                            }
+                            if (unsup) atomicAdd(&flags[F_UNSUPPORTED], unsup);
+                            r.g1 = drop;
+                            r.g2 = lat_bnd;
+                        } else {
+                            const int face = r.face;
+                            const float xon = (face == 2) ? -dx * 0.5f : (face == 3 ? dx * 0.5f : r.x_new);
+                            r.i0 = min(max((int)(((xon / dx) + 0.5f) * LY::XS), 0), LY::XS - 1);
+                            if (face == 1) r.i1 = LY::ZS - 1;
+                            else r.i1 = min(max((int)(((r.z_new / dz) + 0.5f) * LY::ZS), 0), LY::ZS - 1);
+                        }
+                    } else {
+                        int tbn = (int)((r.th_new - tmin) / dth);
+                        tbn = (int)fmaxf(0, fminf(tbn, NT - 1));
+                        const int ebn = find_bin(T.ee, NE, r.E_new);
+                        r.i0 = tbn;
+                        r.i1 = ebn;
+                        r.i2 = min(max((int)(((r.z_new / dz) + 0.5f) * LY::ZS), 0), LY::ZS - 1);
+                        r.g0 = ecenter(T.ee, ebn);
+                    }
+                }
                 // ---------------- ordered apply, bin order ----------------
                 while (m) {
                     const int j = __ffsll((unsigned long long)m) - 1;
@@ -713,7 +983,6 @@ __global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out
                     const float edep = rlf(r.edep, j), w_rem = rlf(r.w_rem, j), E_rem = rlf(r.E_rem, j);
                     const float w_new = rlf(r.w_new, j), E_new = rlf(r.E_new, j);
                     const int face = rli(r.face, j);
-                    const float thn = rlf(r.th_new, j), xn = rlf(r.x_new, j), zn = rlf(r.z_new, j);
                     a.edep += edep;
                     a.w_nuc += w_rem;
                     a.E_nuc += E_rem;
@@ -726,6 +995,13 @@ __global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out
                         a.cl_Eout += (double)(E_new * w_new);
                         continue;
                     }
+                    Bins bn;
+                    bn.bc = rli(r.bn.bc, j);
+                    bn.tb1 = rli(r.bn.tb1, j);
+                    bn.eb = rli(r.bn.eb, j);
+                    bn.tw0 = rlf(r.bn.tw0, j);
+                    bn.tw1 = rlf(r.bn.tw1, j);
+                    bn.Ecen = rlf(r.bn.Ecen, j);
                     if (face >= 0) {
                         bool nb = (face == 0) ? (iz + 1 < T.Nz) : (face == 1) ? (iz > 0)
                                 : (face == 2) ? (ix + 1 < Nx) : (ix > 0);
@@ -736,29 +1012,22 @@ __global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out
                             a.cl_Eout += (double)(E_new * w_new);
                             continue;
                         }
-                        const float E_emit = ecenter(T.ee, find_bin(T.ee, NE, E_new));
+                        const float E_emit = rlf(r.g0, j);
                         float drop = 0.0f;
-                        float lat_bnd = 0.0f;
                         if (face == 0) {
-#pragma unroll
-                            for (int q = 0; q < 11; ++q) {
-                                float wf = rlf(r.sw[q], j);
-                                if (wf <= 0.0f) continue;
-                                int off = q - 5;
-                                int it = ix + off;
-                                if (it < 0 || it >= Nx) {
-                                    lat_bnd += w_new * wf;
-                                    continue;
-                                }
-                                if (off == 0) {
-                                    drop += cx.emit4d(0, thn, E_new, w_new * wf, LY::XS / 2, 0);
-                                } else {
-                                    // Emission into another cell's +z bucket: a drop when that
-                                    // cell is not listed in this phase (device_bucket.cuh:953-963);
-                                    // a listed target would need a cross-wave ordered write,
-                                    // which this path rejects.
-                                    if (ltag[cell + off] == tag && l == 0) atomicAdd(&flags[F_UNSUPPORTED], 1u);
-                                    drop += w_new * wf;
+                            const float wf5 = rlf(r.sw[5], j);
+                            const float lat_bnd = rlf(r.g2, j);
+                            float d5 = 0.0f;
+                            if (wf5 > 0.0f) d5 = emit<LY>(W, a, T, bk, flags, 0, bn, w_new * wf5, LY::XS / 2, 0);
+                            if (d5 == 0.0f) {
+                                drop = rlf(r.g1, j);
+                            } else {  // the lateral sum with the centre's drop in its place (q = 5)
+                                for (int q = 0; q < 11; ++q) {
+                                    const float wf = rlf(r.sw[q], j);
+                                    if (wf <= 0.0f) continue;
+                                    const int it = ix + q - 5;
+                                    if (it < 0 || it >= Nx) continue;
+                                    drop += (q == 5) ? d5 : w_new * wf;
                                 }
                             }
                             if (lat_bnd > 0.0f) {
@@ -781,12 +1050,8 @@ __global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out
                                 }
                             }
                         } else {
-                            float xon = (face == 2) ? -dx * 0.5f : (face == 3 ? dx * 0.5f : xn);
-                            int xsn = min(max((int)(((xon / dx) + 0.5f) * LY::XS), 0), LY::XS - 1);
-                            int zsn;
-                            if (face == 1) zsn = LY::ZS - 1;
-                            else zsn = min(max((int)(((zn / dz) + 0.5f) * LY::ZS), 0), LY::ZS - 1);
-                            drop = cx.emit4d(face, thn, E_new, w_new, xsn, zsn);
+                            const int xsn = rli(r.i0, j), zsn = rli(r.i1, j);
+                            drop = emit<LY>(W, a, T, bk, flags, face, bn, w_new, xsn, zsn);
                             if (drop > 0.0f) {
                                 a.drop_w += (double)drop;
                                 a.drop_E += (double)(E_new * drop);
@@ -804,14 +1069,10 @@ __global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out
                         continue;
                     }
                     // ---- remain in the cell ----
-                    int tbn = (int)((thn - tmin) / dth);
-                    tbn = (int)fmaxf(0, fminf(tbn, NT - 1));
-                    int ebn = find_bin(T.ee, NE, E_new);
+                    const int tbn = rli(r.i0, j), ebn = rli(r.i1, j), zst = rli(r.i2, j);
                     const uint32_t bidn = ((uint32_t)(ebn / LY::EL) << 12) | ((uint32_t)(tbn / LY::TL) & 0xFFF);
                     uint32_t sel;
-                    int os = table_find_or_claim(wc.obid, wc.oref, wc.n_out, bidn, sel, out.pool, out.pcount,
-                                                 out.pcap, out.bid + (size_t)cell * KB,
-                                                 out.sref + (size_t)cell * KB, LY::LB, flags, F_PSI_FULL);
+                    const int os = W.claim(0, bidn, sel, out.pool, out.pcount, out.pcap, flags, F_PSI_FULL);
                     if (os < 0) {
                         a.drop_w += (double)w_new;
                         a.drop_E += (double)(E_new * w_new);
@@ -820,14 +1081,14 @@ __global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out
                         continue;
                     }
                     int tbe = tbn, ebe = ebn;
+                    float E_emit = rlf(r.g0, j);
                     if (sel != bidn) {
                         int bte = (int)(sel & 0xFFF), bee = (int)((sel >> 12) & 0xFFF);
                         tbe = max(bte * LY::TL, min(tbe, bte * LY::TL + LY::TL - 1));
                         ebe = max(bee * LY::EL, min(ebe, bee * LY::EL + LY::EL - 1));
+                        E_emit = T.ee[ebe] + 0.5f * (T.ee[ebe + 1] - T.ee[ebe]);
                     }
-                    const float E_emit = T.ee[ebe] + 0.5f * (T.ee[ebe + 1] - T.ee[ebe]);
-                    const int zst = min(max((int)(((zn / dz) + 0.5f) * LY::ZS), 0), LY::ZS - 1);
-                    const uint32_t oref = rlu(wc.oref, os);
+                    const uint32_t oref = rlu(W.r0, os);
                     float mywf = 0.0f;
 #pragma unroll
                     for (int q = 0; q < LY::XS; ++q) {
@@ -836,20 +1097,18 @@ __global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out
                         if (wf > 0.0f) {
                             float wsp = w_new * wf;
                             a.cl_wout += (double)wsp;
-                            a.cl_Eout += (double)wsp * (double)ecenter(T.ee, ebe);
+                            a.cl_Eout += (double)wsp * (double)E_emit;
                         }
                     }
-                    if (l < LY::XS && mywf > 0.0f) {
-                        size_t addr = (size_t)oref * LY::LB + LY::enc(tbe % LY::TL, ebe % LY::EL, l, zst);
-                        out.pool[addr] += w_new * mywf;
-                    }
+                    if (l < LY::XS && mywf > 0.0f)
+                        W.add(oref, LY::enc(tbe % LY::TL, ebe % LY::EL, l, zst), w_new * mywf, out.pool);
                     const float wcf = rlf(r.wcf, j), wl = rlf(r.wl, j), wr = rlf(r.wr, j);
                     if (wcf < 1.0f - 1e-6f) {
                         const int zsb = zst;
                         if (wl > 0.0f) {
                             float wsl = w_new * wl;
                             if (ix > 0) {
-                                float d = cx.emit4d(3, thn, E_new, wsl, LY::XS - 1, zsb);
+                                float d = emit<LY>(W, a, T, bk, flags, 3, bn, wsl, LY::XS - 1, zsb);
                                 if (d > 0.0f) {
                                     a.drop_w += (double)d;
                                     a.drop_E += (double)(E_new * d);
@@ -866,7 +1125,7 @@ __global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out
                         if (wr > 0.0f) {
                             float wsr = w_new * wr;
                             if (ix < Nx - 1) {
-                                float d = cx.emit4d(2, thn, E_new, wsr, 0, zsb);
+                                float d = emit<LY>(W, a, T, bk, flags, 2, bn, wsr, 0, zsb);
                                 if (d > 0.0f) {
                                     a.drop_w += (double)d;
                                     a.drop_E += (double)(E_new * d);
@@ -895,7 +1154,8 @@ __global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out
                 }
             }
         }
-        // ---- end of cell: tallies, bucket counts, psi_out slot count ----
+        // ---- end of cell: tables and payloads, tallies, per-cell accumulators ----
+        flush_tables<LY>(W, out, bk, cell, li, flags);
         if (l == 0) {
             tal.edep[cell] += a.edep;
             tal.w_cut[cell] += a.w_cut;
@@ -904,8 +1164,6 @@ __global__ __launch_bounds__(256) void k_transport(Tabs T, Tp P, Psi in, Psi out
             tal.E_nuc[cell] += a.E_nuc;
             tal.w_bnd[cell] += a.w_bnd;
             tal.E_bnd[cell] += a.E_bnd;
-            out.ns[cell] = wc.n_out;
-            for (int f = 0; f < 4; ++f) bk.n[li * 4 + f] = wc.bn[f];
             double* o = acc + (size_t)li * NACC;
             o[A_EDEP] = a.edep;
             o[A_WCUT] = a.w_cut;
@@ -1050,28 +1308,49 @@ enum { R_WIN, R_EIN, R_WOUT, R_EOUT, R_K4W, R_K4E, NR1 };
 // fine-cell E closure and the number of listed cells of the iteration.
 constexpr int NAUD = NR1 + NACC + 3;
 
-// Stage 1: per tile of 256 list entries: W_in / E_in from K1's per-entry sums of psi_in, and
-// W_out / E_out of psi_out plus the K4 drops (consumed) over psi_out's occupied-cell list.
+// Stage 1a: one wavefront per list entry i: W_in / E_in from K1's per-entry sums of psi_in,
+// W_out / E_out of psi_out's i-th occupied cell (coalesced payload reads, FP64 butterfly) and
+// the K4 drops booked on that cell (consumed).  Per-entry values go to pe[i * NR1 + k].
 // `force` runs the pass after the transport ended (totals of a buffer for the host).
 template <class LY>
-__global__ __launch_bounds__(256) void k_audit_lists(Tabs T, Psi out, const uint32_t* nin, const double* win,
-                                                     const double* ein, const uint32_t* __restrict__ occ_out,
-                                                     const uint32_t* nout, double* k4w, double* k4e,
-                                                     double* part, const uint32_t* ctl, int force) {
+__global__ __launch_bounds__(64) void k_audit_cells(Tabs T, Psi out, const uint32_t* nin, const double* win,
+                                                    const double* ein, const uint32_t* __restrict__ occ_out,
+                                                    const uint32_t* nout, double* k4w, double* k4e, double* pe,
+                                                    const uint32_t* ctl, int force) {
     if (ctl[C_DONE] && !force) return;
-    __shared__ double sh[NR1][256];
     const uint32_t n_in = nin ? *nin : 0u, n_out = *nout;
-    const uint32_t ntile = (max(n_in, n_out) + 255) / 256;
-    for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+    const uint32_t n = max(n_in, n_out);
+    const int l = lane_id();
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
         double r[NR1] = {0, 0, 0, 0, 0, 0};
-        const uint32_t i = tile * 256 + threadIdx.x;
         if (i < n_in) {
             r[R_WIN] = win[i];
             r[R_EIN] = ein[i];
         }
         if (i < n_out) {
             const uint32_t c = occ_out[i];
-            if (k4w) {
+            const uint32_t ns = out.ns[c];
+            const uint32_t mb = (l < (int)ns) ? out.bid[(size_t)c * KB + l] : EMPTY;
+            const uint32_t mr = (l < (int)ns) ? out.sref[(size_t)c * KB + l] : 0u;
+            double W = 0.0, E = 0.0;
+            for (uint32_t s = 0; s < ns; ++s) {
+                const uint32_t b = rlu(mb, (int)s);
+                if (b == EMPTY) continue;
+                const float* v = out.pool + (size_t)rlu(mr, (int)s) * LY::LB;
+                const int bE = (int)((b >> 12) & 0xFFF);
+#pragma unroll
+                for (int k = 0; k < LY::PL; ++k) {
+                    const int lidx = l + 64 * k;
+                    const float w = v[lidx];
+                    if (w <= 0.0f) continue;
+                    W += (double)w;
+                    const int eb = min(max(bE * LY::EL + LY::le(lidx), 0), T.NE - 1);
+                    E += (double)w * (double)ecenter(T.ee, eb);
+                }
+            }
+            r[R_WOUT] = wave_sum(W);
+            r[R_EOUT] = wave_sum(E);
+            if (k4w && l == 0) {
                 r[R_K4W] = k4w[c];
                 r[R_K4E] = k4e[c];
                 if (r[R_K4W] != 0.0 || r[R_K4E] != 0.0) {
@@ -1079,22 +1358,24 @@ __global__ __launch_bounds__(256) void k_audit_lists(Tabs T, Psi out, const uint
                     k4e[c] = 0.0;
                 }
             }
-            const uint32_t n = out.ns[c];
-            for (uint32_t s = 0; s < n; ++s) {
-                uint32_t bid = out.bid[(size_t)c * KB + s];
-                if (bid == EMPTY) continue;
-                const float* v = out.pool + (size_t)out.sref[(size_t)c * KB + s] * LY::LB;
-                int bE = (int)((bid >> 12) & 0xFFF);
-                for (int l = 0; l < LY::LB; ++l) {
-                    float w = v[l];
-                    if (w <= 0.0f) continue;
-                    r[R_WOUT] += (double)w;
-                    int eb = min(max(bE * LY::EL + LY::le(l), 0), T.NE - 1);
-                    r[R_EOUT] += (double)w * (double)ecenter(T.ee, eb);
-                }
-            }
         }
-        for (int k = 0; k < NR1; ++k) sh[k][threadIdx.x] = r[k];
+        if (l == 0)
+            for (int k = 0; k < NR1; ++k) pe[(size_t)i * NR1 + k] = r[k];
+    }
+}
+
+// Stage 1b: per tile of 256 entries, a fixed tree over the entries' values.
+__global__ __launch_bounds__(256) void k_audit_tiles(const double* __restrict__ pe, const uint32_t* nin,
+                                                     const uint32_t* nout, double* part, const uint32_t* ctl,
+                                                     int force) {
+    if (ctl[C_DONE] && !force) return;
+    __shared__ double sh[NR1][256];
+    const uint32_t n_in = nin ? *nin : 0u, n_out = *nout;
+    const uint32_t n = max(n_in, n_out);
+    const uint32_t ntile = (n + 255) / 256;
+    for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const uint32_t i = tile * 256 + threadIdx.x;
+        for (int k = 0; k < NR1; ++k) sh[k][threadIdx.x] = i < n ? pe[(size_t)i * NR1 + k] : 0.0;
         __syncthreads();
         for (int o = 128; o > 0; o >>= 1) {
             if ((int)threadIdx.x < o)

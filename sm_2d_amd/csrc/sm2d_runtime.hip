@@ -79,23 +79,24 @@ struct sm2d_ctx {
     DBuf<float> bpool;
     DBuf<uint32_t> maskv, ltag, lpos, dtag, act, crs, dlist, ctl, tcount, tcount2, flags;
     DBuf<uint8_t> flag;
-    DBuf<double> edep, E_cut, E_nuc, E_bnd, win, ein, k4w, k4e, acc_c, acc_f, part, k5v, hist, edep_sum;
+    DBuf<double> edep, E_cut, E_nuc, E_bnd, win, ein, k4w, k4e, acc_c, acc_f, pe, part, k5v, hist, edep_sum;
     DBuf<float> w_cut, w_nuc, w_bnd, edep_f;
     DBuf<sm2d_spot> spots;
     DBuf<InjSample> samples;
     DBuf<double> injacc;
     std::vector<double> src;  // per spot: W in/out/slot, E in/out/slot, E representation loss
     uint32_t hint = 64;       // expected listed cells per phase (launch sizing only)
+    uint32_t hint_occ = 64;   // expected occupied cells per buffer (launch sizing only)
     // stage timing (eager launches with events on `stream`)
+    // Events are recorded for every queued iteration of a chunk (12 per iteration) and read
+    // after the chunk's host synchronisation, so profiling does not change the queueing.
+    static constexpr int EPI = 12;
     bool prof = false;
-    hipEvent_t ev[12] = {};
-    bool ev_used[12] = {};
+    std::vector<hipEvent_t> ev;
+    int ev_it = 0;  // iteration slot within the current chunk
     sm2d_timing timing{};
     void mark(int k) {
-        if (prof) {
-            (void)hipEventRecord(ev[k], stream);
-            ev_used[k] = true;
-        }
+        if (prof) (void)hipEventRecord(ev[(size_t)ev_it * EPI + k], stream);
     }
 
     Tabs tabs() const {
@@ -188,6 +189,30 @@ void build_occ(sm2d_ctx* c, int k) {
     k_occ_emit<<<nt, 256, 0, c->stream>>>(c->bitmap.p, c->nwords, c->tcount2.p, c->occ[k].p, c->ocap, c->ctl.p);
 }
 
+// K1 + K2 on psi_in (buffer `in`): mask, flag counts, ordered compaction.
+template <class LY>
+void launch_mask_compact(sm2d_ctx* c, const Tp& P, int in, int grid_entries) {
+    const int ge = blocks_for(grid_entries, 1, 1 << 16);
+    const int gt = blocks_for(grid_entries, 256, 4096);
+    k_mask<LY><<<ge, 64, 0, c->stream>>>(c->tabs(), P, c->psi(in), c->occ[in].p, c->nocc(in), c->maskv.p, c->ctl.p,
+                                         c->flag.p, c->win.p, c->ein.p);
+    k_flag_count<<<gt, 256, 0, c->stream>>>(c->flag.p, c->nocc(in), c->tcount.p, c->ctl.p);
+    k_scan<2><<<1, 1024, 0, c->stream>>>(c->tcount.p, c->nocc(in), 256, c->ctl.p, nullptr, c->lcap, c->flags.p);
+    k_emit<<<gt, 256, 0, c->stream>>>(c->occ[in].p, c->nocc(in), c->flag.p, c->tcount.p, c->act.p, c->crs.p,
+                                      c->ltag.p, c->lpos.p, c->bitmap.p, c->ctl.p);
+}
+
+// K5 stage 1 over (nin entries of K1 sums, nout occupied cells of buffer k).
+template <class LY>
+void launch_audit_cells(sm2d_ctx* c, int k, const uint32_t* nin, bool k4, int force, int grid_entries) {
+    const int ge = blocks_for(grid_entries, 1, 1 << 16);
+    const int gt = blocks_for(grid_entries, 256, 4096);
+    k_audit_cells<LY><<<ge, 64, 0, c->stream>>>(c->tabs(), c->psi(k), nin, c->win.p, c->ein.p, c->occ[k].p,
+                                                c->nocc(k), k4 ? c->k4w.p : nullptr, k4 ? c->k4e.p : nullptr,
+                                                c->pe.p, c->ctl.p, force);
+    k_audit_tiles<<<gt, 256, 0, c->stream>>>(c->pe.p, nin, c->nocc(k), c->part.p, c->ctl.p, force);
+}
+
 template <class LY>
 void launch_iteration(sm2d_ctx* c, const Tp& P) {
     const int in = c->in, out = 1 - c->in;
@@ -197,11 +222,7 @@ void launch_iteration(sm2d_ctx* c, const Tp& P) {
     k_begin<<<1, 64, 0, c->stream>>>(c->ctl.p);
     k_clear<<<gm, 256, 0, c->stream>>>(c->psi(out), c->occ[out].p, c->nocc(out), c->ctl.p);
     c->mark(0);
-    k_mask<LY><<<gm, 256, 0, c->stream>>>(T, P, c->psi(in), c->occ[in].p, c->nocc(in), c->maskv.p, c->ctl.p,
-                                          c->flag.p, c->tcount.p, c->win.p, c->ein.p);
-    k_scan<2><<<1, 1024, 0, c->stream>>>(c->tcount.p, c->nocc(in), 256, c->ctl.p, nullptr, c->lcap, c->flags.p);
-    k_emit<<<gm, 256, 0, c->stream>>>(c->occ[in].p, c->nocc(in), c->flag.p, c->tcount.p, c->act.p, c->crs.p,
-                                      c->ltag.p, c->lpos.p, c->bitmap.p, c->ctl.p);
+    launch_mask_compact<LY>(c, P, in, (int)c->hint_occ);
     c->mark(1);
     for (int fine = 0; fine < 2; ++fine) {
         const uint32_t* list = fine ? c->act.p : c->crs.p;
@@ -211,10 +232,10 @@ void launch_iteration(sm2d_ctx* c, const Tp& P) {
         k_phase_reset<<<1, 64, 0, c->stream>>>(c->ctl.p, c->bpcount.p);
         c->mark(e0);
         if (fine)
-            k_transport<LY, true><<<blocks_for(h, 4, 8192), 256, 0, c->stream>>>(
+            k_transport<LY, true><<<blocks_for(h, 1, 65536), 64, 0, c->stream>>>(
                 T, P, c->psi(in), c->psi(out), c->bkt(), list, nl, c->ltag.p, c->ctl.p, c->tally(), acc, c->flags.p);
         else
-            k_transport<LY, false><<<blocks_for(h, 4, 8192), 256, 0, c->stream>>>(
+            k_transport<LY, false><<<blocks_for(h, 1, 65536), 64, 0, c->stream>>>(
                 T, P, c->psi(in), c->psi(out), c->bkt(), list, nl, c->ltag.p, c->ctl.p, c->tally(), acc, c->flags.p);
         c->mark(e0 + 1);
         k_dest<<<blocks_for(h, 256, 1024), 256, 0, c->stream>>>(list, nl, c->g.Nx, c->g.Nz, c->Ns, c->dtag.p, c->ctl.p,
@@ -228,8 +249,7 @@ void launch_iteration(sm2d_ctx* c, const Tp& P) {
     }
     build_occ(c, out);
     c->mark(10);
-    k_audit_lists<LY><<<gm, 256, 0, c->stream>>>(T, c->psi(out), c->nocc(in), c->win.p, c->ein.p, c->occ[out].p,
-                                                 c->nocc(out), c->k4w.p, c->k4e.p, c->part.p, c->ctl.p, 0);
+    launch_audit_cells<LY>(c, out, c->nocc(in), true, 0, (int)c->hint_occ);
     k_audit_final<<<1, 256, 0, c->stream>>>(c->part.p, c->nocc(in), c->nocc(out), c->acc_c.p, c->ctl.p + C_NCRS,
                                             c->acc_f.p, c->ctl.p + C_NACT, nullptr, c->hist.p, HROWS, c->ctl.p, 0);
     c->mark(11);
@@ -239,9 +259,7 @@ void launch_iteration(sm2d_ctx* c, const Tp& P) {
 // Totals (W, E) of buffer k over its occupied list, after the transport (forced pass).
 template <class LY>
 int buffer_totals(sm2d_ctx* c, int k, double& W, double& E) {
-    const int gm = blocks_for(c->ocap, 256, 4096);
-    k_audit_lists<LY><<<gm, 256, 0, c->stream>>>(c->tabs(), c->psi(k), nullptr, c->win.p, c->ein.p, c->occ[k].p,
-                                                 c->nocc(k), nullptr, nullptr, c->part.p, c->ctl.p, 1);
+    launch_audit_cells<LY>(c, k, nullptr, false, 1, (int)std::max<uint32_t>(c->hint_occ, 1024));
     k_audit_final<<<1, 256, 0, c->stream>>>(c->part.p, nullptr, c->nocc(k), nullptr, nullptr, nullptr, nullptr,
                                             c->k5v.p, nullptr, 1, c->ctl.p, 1);
     HIPCHK(hipGetLastError());
@@ -253,21 +271,24 @@ int buffer_totals(sm2d_ctx* c, int k, double& W, double& E) {
     return SM2D_OK;
 }
 
-void collect_timing(sm2d_ctx* c) {
+// Stage times of the first n_valid iterations of the chunk just synchronised (iterations queued
+// after the transport ended return at once and are not counted).
+void collect_timing(sm2d_ctx* c, int n_valid) {
     if (!c->prof) return;
-    auto span = [&](int a, int b) -> double {
-        if (!c->ev_used[a] || !c->ev_used[b]) return 0.0;
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, c->ev[a], c->ev[b]);
-        return (double)ms;
-    };
-    c->timing.mask_ms += span(0, 1);
-    c->timing.transport_ms += span(2, 3) + span(6, 7);
-    c->timing.gather_ms += span(3, 5) + span(7, 9);
-    c->timing.audit_ms += span(10, 11);
-    c->timing.transport_launches += (c->ev_used[2] ? 1 : 0) + (c->ev_used[6] ? 1 : 0);
-    c->timing.gather_launches += (c->ev_used[4] ? 1 : 0) + (c->ev_used[8] ? 1 : 0);
-    for (bool& u : c->ev_used) u = false;
+    for (int r = 0; r < n_valid; ++r) {
+        hipEvent_t* e = c->ev.data() + (size_t)r * sm2d_ctx::EPI;
+        auto span = [&](int a, int b) -> double {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e[a], e[b]);
+            return (double)ms;
+        };
+        c->timing.mask_ms += span(0, 1);
+        c->timing.transport_ms += span(2, 3) + span(6, 7);
+        c->timing.gather_ms += span(3, 5) + span(7, 9);
+        c->timing.audit_ms += span(9, 11);
+        c->timing.transport_launches += 2;
+        c->timing.gather_launches += 2;
+    }
 }
 
 int check_flags(sm2d_ctx* c, const uint32_t* f) {
@@ -320,7 +341,7 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
     k_ctl_reset<<<1, 1, 0, c->stream>>>(c->ctl.p);
     HIPCHK(hipGetLastError());
     const int in0 = c->in;
-    const int chunk = (tp->fail_fast_on_audit || c->prof) ? 1 : CHUNK;
+    const int chunk = tp->fail_fast_on_audit ? 1 : CHUNK;
     int queued = 0, audited = 0;
     double Wso = 0, Wss = 0, Eso = 0, Ess = 0;
     for (int s = 0; s < c->S; ++s) {
@@ -332,7 +353,10 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
     std::vector<double> rows((size_t)HROWS * NV);
     while (queued < n_iter) {
         const int R = std::min(chunk, n_iter - queued);
-        for (int r = 0; r < R; ++r) launch_iteration<LY>(c, P);
+        for (int r = 0; r < R; ++r) {
+            c->ev_it = r;
+            launch_iteration<LY>(c, P);
+        }
         queued += R;
         HIPCHK(hipGetLastError());
         uint32_t ctl[NCTL], f[NFLAG];
@@ -341,7 +365,6 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
         HIPCHK(hipMemcpyAsync(rows.data(), c->hist.p, sizeof(double) * rows.size(), hipMemcpyDeviceToHost,
                               c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        collect_timing(c);
         int rc = check_flags(c, f);
         if (rc) return rc;
         const bool done = ctl[C_DONE] != 0;
@@ -376,9 +399,11 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
                 return err("K5 audit fail-fast", SM2D_ERR_AUDIT);
             }
         }
+        collect_timing(c, upto - audited);
         audited = upto;
         maxlist = std::max(ctl[C_NACT], ctl[C_NCRS]);
         c->hint = std::max<uint32_t>(64, 2 * maxlist);
+        c->hint_occ = std::max<uint32_t>(64, 2 * std::max(ctl[C_NOCC0], ctl[C_NOCC1]));
         if (au) au->iterations = done ? (int)ctl[C_DONE_IT] : (int)ctl[C_IT];
         if (done) {
             if (au) au->terminated = 1;
@@ -409,10 +434,12 @@ int inject_all(sm2d_ctx* c, const sm2d_spot* sp, int n) {
     build_occ(c, c->in);
     HIPCHK(hipGetLastError());
     std::vector<double> acc((size_t)n * 6);
-    uint32_t f[NFLAG];
+    uint32_t f[NFLAG], no = 0;
     HIPCHK(hipMemcpyAsync(acc.data(), c->injacc.p, sizeof(double) * acc.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(f, c->flags.p, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&no, c->nocc(c->in), sizeof(no), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->hint_occ = std::max<uint32_t>(64, 2 * no);
     int rc = check_flags(c, f);
     if (rc) return rc;
     // represented energy of psi_in (gpu_transport_wrapper.cu:125-180,443-457); per batch
@@ -558,6 +585,7 @@ int create(int device, const sm2d_grid* g, int n_spots, sm2d_ctx** out) {
     HIPCHK(c->ein.alloc(c->ocap));
     HIPCHK(c->acc_c.alloc((size_t)c->ocap * NACC));
     HIPCHK(c->acc_f.alloc((size_t)c->ocap * NACC));
+    HIPCHK(c->pe.alloc((size_t)c->ocap * NR1));
     HIPCHK(c->part.alloc(((size_t)c->ocap / 256 + 2) * NR1));
     HIPCHK(c->k5v.alloc(NV));
     HIPCHK(c->hist.alloc((size_t)HROWS * NV));
@@ -750,6 +778,7 @@ int sm2d_psi_load(sm2d_ctx* c, int n, const int32_t* cell, const int32_t* slot, 
         HIPCHK(hipMemcpyAsync(c->occ[0].p, occ.data(), sizeof(uint32_t) * occ.size(), hipMemcpyHostToDevice, c->stream));
     uint32_t no = (uint32_t)occ.size();
     HIPCHK(hipMemcpyAsync(c->nocc(0), &no, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    c->hint_occ = std::max<uint32_t>(64, 2 * no);
     HIPCHK(hipStreamSynchronize(c->stream));
     return SM2D_OK;
 }
@@ -811,16 +840,10 @@ int sm2d_k1k2_mask_compact(sm2d_ctx* c, const sm2d_transport* tp, const uint8_t*
     k_ctl_reset<<<1, 1, 0, c->stream>>>(c->ctl.p);
     k_begin<<<1, 64, 0, c->stream>>>(c->ctl.p);
     const int in = c->in;
-    const int gm = blocks_for(c->ocap, 256, 4096);
     dispatch(c, [&](auto ly) {
-        using LY = decltype(ly);
-        k_mask<LY><<<gm, 256, 0, c->stream>>>(c->tabs(), P, c->psi(in), c->occ[in].p, c->nocc(in), c->maskv.p,
-                                              c->ctl.p, c->flag.p, c->tcount.p, c->win.p, c->ein.p);
+        launch_mask_compact<decltype(ly)>(c, P, in, (int)std::max<uint32_t>(c->hint_occ, 1024));
         return 0;
     });
-    k_scan<2><<<1, 1024, 0, c->stream>>>(c->tcount.p, c->nocc(in), 256, c->ctl.p, nullptr, c->lcap, c->flags.p);
-    k_emit<<<gm, 256, 0, c->stream>>>(c->occ[in].p, c->nocc(in), c->flag.p, c->tcount.p, c->act.p, c->crs.p,
-                                      c->ltag.p, c->lpos.p, c->bitmap.p, c->ctl.p);
     k_fill_u32<<<blocks_for(c->nwords, 256, 4096), 256, 0, c->stream>>>(c->bitmap.p, 0u, c->nwords);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(ctl, c->ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, c->stream));
@@ -1029,13 +1052,14 @@ int sm2d_run_batch(sm2d_ctx* c, const sm2d_spot* spots, int n, const sm2d_transp
 int sm2d_set_profiling(sm2d_ctx* c, int on) {
     if (!c) return err("null context");
     HIPCHK(hipSetDevice(c->device));
-    for (hipEvent_t& e : c->ev)
-        if (!e) HIPCHK(hipEventCreate(&e));
+    if (c->ev.empty()) {
+        c->ev.resize((size_t)CHUNK * sm2d_ctx::EPI);
+        for (hipEvent_t& e : c->ev) HIPCHK(hipEventCreate(&e));
+    }
     c->prof = on != 0;
     std::memset(&c->timing, 0, sizeof(c->timing));
     c->timing.struct_size = sizeof(c->timing);
     c->timing.enabled = c->prof ? 1 : 0;
-    for (bool& u : c->ev_used) u = false;
     return SM2D_OK;
 }
 
