@@ -380,6 +380,7 @@ struct WTabs {
     uint32_t r0, r1, r2, r3, r4;  // payload reference of slot (lane)
     uint32_t n0, n1, n2, n3, n4;  // slots in use (wave-uniform)
     uint32_t own0;                // own psi_out slots present when the cell started
+    uint32_t orig;                // global reference of a preloaded own slot (lane < own0)
     uint32_t used;                // LDS rows in use (wave-uniform)
     float* rows;                  // this wave's LDS rows [NL][LB]
 
@@ -398,8 +399,9 @@ struct WTabs {
 
     // Same search / claim / closest-block fallback as table_find_or_claim; a claimed slot gets a
     // zeroed LDS row while rows remain, else a zeroed global payload.
+    // fallback = false: a full table without `want` returns -2 instead of the closest block.
     __device__ int claim(int t, uint32_t want, uint32_t& sel, float* gpool, uint32_t* gpcount, uint32_t gpcap,
-                         uint32_t* flags, int flag_full) {
+                         uint32_t* flags, int flag_full, bool fallback = true) {
         const int l = lane_id();
         sel = want;
         const uint32_t mb = gb(t);
@@ -434,6 +436,7 @@ struct WTabs {
             sn(t, n + 1);
             return s;
         }
+        if (!fallback) return -2;
         int key = 0x7fffffff;
         if (l < 32 && mb != EMPTY) {
             int d = abs((int)((mb >> 12) & 0xFFF) - (int)((want >> 12) & 0xFFF)) * 8 +
@@ -556,7 +559,15 @@ template <class LY>
 __device__ inline void flush_tables(WTabs<LY>& W, Psi& out, Bkt& bk, int cell, uint32_t li, uint32_t* flags) {
     const int l = lane_id();
     const uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
-    // own psi_out
+    // own psi_out: preloaded slots go back to their payloads, new cached slots get new ones
+    const uint64_t mp = __ballot(l < 32 && (uint32_t)l < W.own0 && (W.r0 & CACHED));
+    for (uint64_t m = mp; m; m &= m - 1) {
+        const int s = __ffsll((unsigned long long)m) - 1;
+        const float* src = W.rows + (size_t)(rlu(W.r0, s) & ~CACHED) * LY::LB;
+        float* dst = out.pool + (size_t)rlu(W.orig, s) * LY::LB;
+#pragma unroll
+        for (int k = 0; k < LY::PL; ++k) dst[l + 64 * k] = src[l + 64 * k];
+    }
     const uint64_t mo = __ballot(l < 32 && (uint32_t)l >= W.own0 && (uint32_t)l < W.n0 && (W.r0 & CACHED));
     uint32_t base = 0;
     const uint32_t co = (uint32_t)__popcll(mo);
@@ -620,14 +631,328 @@ __device__ inline void flush_tables(WTabs<LY>& W, Psi& out, Bkt& bk, int cell, u
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// Lane-parallel form of the ordered apply.  Each lane turns its component into
+//  - claim keys (table << 24 | block id) in the serial claim order,
+//  - payload records (key, local bin, weight) whose destinations are distinct within the
+//    component, and
+//  - per-accumulator contributions in the serial order (two per accumulator, the second 0 when
+//    unused; adding +0.0 leaves every accumulator unchanged).
+// The wave then claims the distinct keys in order of first appearance (= the serial claim
+// order) and walks the components in bin order: per component one LDS read-modify-write round
+// applies all its records, and lanes 0..12 each add the component's contributions to the
+// accumulator they hold.  Every FP32 payload sum and every tally is therefore formed in the
+// serial order.  A chunk whose claims need the closest-block fallback, a global payload or an
+// exhausted pool is applied by the serial code instead.
+// ------------------------------------------------------------------------------------------
+enum {
+    K_EDEP, K_ECUT, K_ENUC, K_EBND, K_WCUT, K_WNUC, K_WBND,
+    K_DROPW, K_DROPE, K_CLWIN, K_CLWOUT, K_CLEIN, K_CLEOUT, NKACC
+};
+constexpr int MAXREC = 12;  // x_sub spread (<= 8) + two two-bin side emissions
+constexpr int MAXKEY = 5;   // own block + two two-bin side emissions
+constexpr int NCON = 2 * NKACC;
+
+__device__ inline bool kacc_is_float(int a) { return a == K_WCUT || a == K_WNUC || a == K_WBND; }
+
+// LDS scratch of one transport wave for one chunk of 64 components.
+struct FastLds {
+    double con[64 * NCON];
+    uint32_t rec[64 * MAXREC];   // (key index << 16 | lidx), then the LDS payload address
+    float recw[64 * MAXREC];
+    uint32_t key[64 * MAXKEY];
+    uint32_t krow[64 * MAXKEY];
+};
+
+struct LaneOut {
+    int nkey, nrec;
+    int nbnd;          // boundary contributions written so far (w_bnd / E_bnd slots)
+    double clwout, cleout;
+};
+
+template <class LY>
+__device__ inline void rec_emit(FastLds& F, LaneOut& o, int t, const Bins& b, float weight, int xs, int zs) {
+    if (weight <= 0.0f) return;
+    const int l = lane_id();
+    const int el = b.eb % LY::EL;
+    const uint32_t bE = (uint32_t)(b.eb / LY::EL);
+    for (int ti = 0; ti < 2; ++ti) {
+        const float tw = ti == 0 ? b.tw0 : b.tw1;
+        if (tw <= 0.0f) continue;
+        const float we = weight * tw;
+        const int tb = ti == 0 ? b.bc : b.tb1;
+        const uint32_t bid = ((uint32_t)(tb / LY::TL) & 0xFFF) | ((bE & 0xFFF) << 12);
+        const int lidx = LY::enc(tb % LY::TL, el, xs, zs);
+        const int k = o.nkey++;
+        F.key[l * MAXKEY + k] = ((uint32_t)t << 24) | bid;
+        const int r = o.nrec++;
+        F.rec[l * MAXREC + r] = ((uint32_t)k << 16) | (uint32_t)lidx;
+        F.recw[l * MAXREC + r] = we;
+        o.clwout += we;
+        o.cleout += (double)we * (double)b.Ecen;
+    }
+}
+
+__device__ inline void con_set(FastLds& F, int a, int second, double v) {
+    F.con[lane_id() * NCON + 2 * a + second] = v;
+}
+
+// The component of this lane (see the serial apply for the meaning of each quantity).
 template <class LY, bool FINE>
-__global__ __launch_bounds__(64) void k_transport(Tabs T, Tp P, Psi in, Psi out, Bkt bk,
+__device__ inline LaneOut fast_prepare(FastLds& F, const Comp& r, bool live, int ix, int iz, int Nx, int Nz) {
+    const int l = lane_id();
+    LaneOut o;
+    o.nkey = o.nrec = o.nbnd = 0;
+    o.clwout = o.cleout = 0.0;
+    for (int q = 0; q < NCON; ++q) F.con[l * NCON + q] = 0.0;
+    if (!live) return o;
+    const float w = r.w, E = r.E;
+    con_set(F, K_CLWIN, 0, (double)w);
+    con_set(F, K_CLEIN, 0, (double)w * (double)E);
+    if (r.kind == 0) {
+        con_set(F, K_DROPW, 0, (double)w);
+        con_set(F, K_DROPE, 0, (double)(E * w));
+        o.clwout = (double)w;
+        o.cleout = (double)(E * w);
+    } else if (r.kind == 1) {
+        con_set(F, K_WCUT, 0, (double)w);
+        con_set(F, K_ECUT, 0, (double)(E * w));
+        o.clwout = (double)w;
+        o.cleout = (double)(E * w);
+    } else {
+        const float w_new = r.w_new, E_new = r.E_new;
+        con_set(F, K_EDEP, 0, (double)r.edep);
+        con_set(F, K_WNUC, 0, (double)r.w_rem);
+        con_set(F, K_ENUC, 0, (double)r.E_rem);
+        o.clwout += (double)r.w_rem;
+        o.cleout += (double)r.edep + (double)r.E_rem;
+        auto bnd = [&](float wb) {
+            con_set(F, K_WBND, o.nbnd, (double)wb);
+            con_set(F, K_EBND, o.nbnd, (double)(E_new * wb));
+            o.nbnd++;
+            o.clwout += (double)wb;
+            o.cleout += (double)(E_new * wb);
+        };
+        if (E_new <= 0.1f) {
+            con_set(F, K_WCUT, 0, (double)w_new);
+            con_set(F, K_ECUT, 0, (double)(E_new * w_new));
+            o.clwout += (double)w_new;
+            o.cleout += (double)(E_new * w_new);
+        } else if (r.face >= 0) {
+            const int face = r.face;
+            const bool nb = (face == 0) ? (iz + 1 < Nz) : (face == 1) ? (iz > 0) : (face == 2) ? (ix + 1 < Nx) : (ix > 0);
+            if (!nb) {
+                bnd(w_new);
+            } else if (face == 0) {
+                const float E_emit = r.g0, drop = r.g1, lat_bnd = r.g2;
+                const float wf5 = r.sw[5];
+                if (wf5 > 0.0f) rec_emit<LY>(F, o, 1, r.bn, w_new * wf5, LY::XS / 2, 0);
+                if (lat_bnd > 0.0f) bnd(lat_bnd);
+                if (drop > 0.0f) {
+                    con_set(F, K_DROPW, 0, (double)drop);
+                    con_set(F, K_DROPE, 0, (double)(E_new * drop));
+                    o.clwout += (double)drop;
+                    o.cleout += (double)(E_new * drop);
+                }
+                if (FINE) {
+                    const float tw = fmaxf(0.0f, w_new - lat_bnd - drop);
+                    if (tw > 0.0f) {
+                        con_set(F, K_EDEP, 1, (double)((E_new - E_emit) * tw));
+                        o.cleout += (double)((E_new - E_emit) * tw);
+                    }
+                }
+            } else {
+                const float E_emit = r.g0;
+                rec_emit<LY>(F, o, 1 + face, r.bn, w_new, r.i0, r.i1);
+                if (FINE) {
+                    const float tw = fmaxf(0.0f, w_new - 0.0f);
+                    if (tw > 0.0f) {
+                        con_set(F, K_EDEP, 1, (double)((E_new - E_emit) * tw));
+                        o.cleout += (double)((E_new - E_emit) * tw);
+                    }
+                }
+            }
+        } else {
+            const int tbn = r.i0, ebn = r.i1, zst = r.i2;
+            const float E_emit = r.g0;
+            const uint32_t bidn = ((uint32_t)(ebn / LY::EL) << 12) | ((uint32_t)(tbn / LY::TL) & 0xFFF);
+            F.key[l * MAXKEY + 0] = bidn;  // table 0
+            o.nkey = 1;
+#pragma unroll
+            for (int q = 0; q < LY::XS; ++q) {
+                const float wf = r.sw[q];
+                if (wf > 0.0f) {
+                    const float wsp = w_new * wf;
+                    o.clwout += (double)wsp;
+                    o.cleout += (double)wsp * (double)E_emit;
+                    const int rr = o.nrec++;
+                    F.rec[l * MAXREC + rr] = (uint32_t)LY::enc(tbn % LY::TL, ebn % LY::EL, q, zst);
+                    F.recw[l * MAXREC + rr] = wsp;
+                }
+            }
+            const float wcf = r.wcf, wl = r.wl, wr = r.wr;
+            if (wcf < 1.0f - 1e-6f) {
+                if (wl > 0.0f) {
+                    const float wsl = w_new * wl;
+                    if (ix > 0) rec_emit<LY>(F, o, 4, r.bn, wsl, LY::XS - 1, zst);
+                    else bnd(wsl);
+                }
+                if (wr > 0.0f) {
+                    const float wsr = w_new * wr;
+                    if (ix < Nx - 1) rec_emit<LY>(F, o, 3, r.bn, wsr, 0, zst);
+                    else bnd(wsr);
+                }
+            }
+            if (FINE) {
+                float rf = wcf;
+                if (ix > 0) rf += wl;
+                if (ix < Nx - 1) rf += wr;
+                rf = fminf(1.0f, fmaxf(0.0f, rf));
+                const float rw = w_new * rf;
+                if (rw > 0.0f) {
+                    con_set(F, K_EDEP, 1, (double)((E_new - E_emit) * rw));
+                    o.cleout += (double)((E_new - E_emit) * rw);
+                }
+            }
+        }
+    }
+    con_set(F, K_CLWOUT, 0, o.clwout);
+    con_set(F, K_CLEOUT, 0, o.cleout);
+    return o;
+}
+
+// Claims of the chunk's distinct keys in order of first appearance; false when a claim needs
+// the serial path (full table, global payload, pool overflow).  Resolves record addresses.
+template <class LY>
+__device__ inline bool fast_claims(FastLds& F, WTabs<LY>& W, const LaneOut& o, Psi& out, Bkt& bk,
+                                   uint32_t* flags) {
+    const int l = lane_id();
+    uint32_t pend = (o.nkey > 0) ? ((1u << o.nkey) - 1u) : 0u;
+    while (true) {
+        const uint64_t hp = __ballot(pend != 0u);
+        if (!hp) break;
+        const int j = __ffsll((unsigned long long)hp) - 1;
+        const int kk = __ffs(rlu(pend, j)) - 1;
+        const uint32_t K = F.key[j * MAXKEY + kk];
+        uint32_t hit = 0;
+        for (int q = 0; q < MAXKEY; ++q)
+            if ((pend >> q) & 1u)
+                if (F.key[l * MAXKEY + q] == K) hit |= 1u << q;
+        pend &= ~hit;
+        const int t = (int)(K >> 24);
+        const uint32_t bid = K & 0xFFFFFFu;
+        uint32_t sel;
+        const int s = (t == 0) ? W.claim(0, bid, sel, out.pool, out.pcount, out.pcap, flags, F_PSI_FULL, false)
+                               : W.claim(t, bid, sel, bk.pool, bk.pcount, bk.pcap, flags, F_BKT_FULL, false);
+        if (s < 0) return false;
+        const uint32_t ref = rlu(W.gr(t), s);
+        if (!(ref & CACHED)) return false;
+        for (int q = 0; q < MAXKEY; ++q)
+            if ((hit >> q) & 1u) F.krow[l * MAXKEY + q] = ref & ~CACHED;
+    }
+    for (int rr = 0; rr < o.nrec; ++rr) {
+        const uint32_t p = F.rec[l * MAXREC + rr];
+        F.rec[l * MAXREC + rr] = F.krow[l * MAXKEY + (p >> 16)] * (uint32_t)LY::LB + (p & 0xFFFFu);
+    }
+    return true;
+}
+
+// Components in bin order: all records of a component in one LDS round, and the accumulators
+// held by lanes 0..NKACC-1.
+template <class LY>
+__device__ inline void fast_steps(FastLds& F, const WTabs<LY>& W, uint64_t m, int nrec, double& accd,
+                                  float& accf) {
+    const int l = lane_id();
+    const bool fl = kacc_is_float(l);
+    while (m) {
+        const int j = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        if (l < NKACC) {
+            const double c0 = F.con[j * NCON + 2 * l], c1 = F.con[j * NCON + 2 * l + 1];
+            if (fl) {
+                accf += (float)c0;
+                accf += (float)c1;
+            } else {
+                accd += c0;
+                accd += c1;
+            }
+        }
+        const int nr = rli(nrec, j);
+        if (l >= 16 && l < 16 + nr) {
+            const int rr = l - 16;
+            const float w = F.recw[j * MAXREC + rr];
+            if (w > 0.0f) {
+                float* p = W.rows + F.rec[j * MAXREC + rr];
+                *p += w;
+            }
+        }
+    }
+}
+
+__device__ inline double rld(double v, int l) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+__device__ inline void lanes_to_accum(CellAccum& a, double accd, float accf) {
+    a.edep = rld(accd, K_EDEP);
+    a.E_cut = rld(accd, K_ECUT);
+    a.E_nuc = rld(accd, K_ENUC);
+    a.E_bnd = rld(accd, K_EBND);
+    a.w_cut = rlf(accf, K_WCUT);
+    a.w_nuc = rlf(accf, K_WNUC);
+    a.w_bnd = rlf(accf, K_WBND);
+    a.drop_w = rld(accd, K_DROPW);
+    a.drop_E = rld(accd, K_DROPE);
+    a.cl_win = rld(accd, K_CLWIN);
+    a.cl_wout = rld(accd, K_CLWOUT);
+    a.cl_Ein = rld(accd, K_CLEIN);
+    a.cl_Eout = rld(accd, K_CLEOUT);
+}
+
+__device__ inline void accum_to_lanes(const CellAccum& a, double& accd, float& accf) {
+    const int l = lane_id();
+    accd = l == K_EDEP ? a.edep : l == K_ECUT ? a.E_cut : l == K_ENUC ? a.E_nuc : l == K_EBND ? a.E_bnd
+         : l == K_DROPW ? a.drop_w : l == K_DROPE ? a.drop_E : l == K_CLWIN ? a.cl_win
+         : l == K_CLWOUT ? a.cl_wout : l == K_CLEIN ? a.cl_Ein : l == K_CLEOUT ? a.cl_Eout : 0.0;
+    accf = l == K_WCUT ? a.w_cut : l == K_WNUC ? a.w_nuc : l == K_WBND ? a.w_bnd : 0.0f;
+}
+
+// Dynamic LDS of the transport: the energy edges and the range tables (4 * N_E + 1 floats), so
+// that the binary searches and interpolations of every component read LDS.
+inline size_t transport_dyn_lds(int NE) { return sizeof(float) * (4 * (size_t)NE + 1); }
+
+template <class LY, bool FINE>
+__global__ __launch_bounds__(64) void k_transport(Tabs Tg, Tp P, Psi in, Psi out, Bkt bk,
                                                   const uint32_t* __restrict__ list,
                                                   const uint32_t* __restrict__ nlist,
                                                   const uint32_t* __restrict__ ltag, const uint32_t* ctl,
                                                   Tally tal, double* acc, uint32_t* flags) {
     if (ctl[C_DONE]) return;
     __shared__ float s_rows[LY::NL * LY::LB];
+    __shared__ FastLds s_fast;
+    extern __shared__ float s_tab[];
+    Tabs T = Tg;
+    {
+        const int NE = Tg.NE;
+        float* ee = s_tab;
+        float* R = ee + NE + 1;
+        float* lR = R + NE;
+        float* lE = lR + NE;
+        for (int i = threadIdx.x; i <= NE; i += blockDim.x) ee[i] = Tg.ee[i];
+        for (int i = threadIdx.x; i < NE; i += blockDim.x) {
+            R[i] = Tg.R[i];
+            lR[i] = Tg.lR[i];
+            lE[i] = Tg.lE[i];
+        }
+        __syncthreads();
+        T.ee = ee;
+        T.R = R;
+        T.lR = lR;
+        T.lE = lE;
+    }
     const uint32_t tag = 2 * ctl[C_EP] + (FINE ? 1u : 0u);
     const int l = lane_id();
     const uint32_t n = *nlist;
@@ -651,6 +976,22 @@ __global__ __launch_bounds__(64) void k_transport(Tabs T, Tp P, Psi in, Psi out,
         W.n1 = W.n2 = W.n3 = W.n4 = 0u;
         W.used = 0;
         W.rows = s_rows;
+        W.orig = 0u;
+        // psi_out slots that already hold weight (the coarse gather into a fine cell) are
+        // staged in LDS rows as well and written back at the end of the cell
+        for (uint32_t s = 0; s < W.own0 && W.used < (uint32_t)LY::NL; ++s) {
+            const float* src = out.pool + (size_t)rlu(W.r0, (int)s) * LY::LB;
+            float* row = W.rows + (size_t)W.used * LY::LB;
+#pragma unroll
+            for (int k = 0; k < LY::PL; ++k) row[l + 64 * k] = src[l + 64 * k];
+            if (l == (int)s) {
+                W.orig = W.r0;
+                W.r0 = CACHED | W.used;
+            }
+            W.used = W.used + 1;
+        }
+        double accd = 0.0;  // lane a < NKACC holds accumulator a of the cell
+        float accf = 0.0f;
         // lateral +z neighbours listed in this phase (their buckets cannot take foreign emissions)
         uint32_t latlisted;
         {
@@ -958,6 +1299,17 @@ __global__ __launch_bounds__(64) void k_transport(Tabs T, Tp P, Psi in, Psi out,
                     }
                 }
                 // ---------------- ordered apply, bin order ----------------
+                {
+                    const LaneOut o = fast_prepare<LY, FINE>(s_fast, r, v[k] > 0.0f, ix, iz, Nx, T.Nz);
+                    if (fast_claims<LY>(s_fast, W, o, out, bk, flags)) {
+                        fast_steps<LY>(s_fast, W, m, o.nrec, accd, accf);
+                        a.comps += (uint32_t)__popcll(m);
+                        m = 0;
+                    } else {
+                        lanes_to_accum(a, accd, accf);
+                    }
+                }
+                const bool serial = m != 0;
                 while (m) {
                     const int j = __ffsll((unsigned long long)m) - 1;
                     m &= m - 1;
@@ -1152,9 +1504,11 @@ __global__ __launch_bounds__(64) void k_transport(Tabs T, Tp P, Psi in, Psi out,
                         }
                     }
                 }
+                if (serial) accum_to_lanes(a, accd, accf);
             }
         }
         // ---- end of cell: tables and payloads, tallies, per-cell accumulators ----
+        lanes_to_accum(a, accd, accf);
         flush_tables<LY>(W, out, bk, cell, li, flags);
         if (l == 0) {
             tal.edep[cell] += a.edep;

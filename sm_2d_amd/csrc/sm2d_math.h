@@ -160,22 +160,47 @@ SM2D_HD inline double cos_d(double x) {
     }
 }
 
+// erf in FP64 with a fixed cost: |x| <= 1 Maclaurin series (18 terms); 1 < |x| < 6
+// erf = 1 - exp(-x^2) erfcx(|x|) with erfcx as degree-16 Chebyshev series on [1,2], [2,3.5],
+// [3.5,6] (Clenshaw recurrence); |x| >= 6 gives +-1 (erfc(6) < 2.2e-17).  Coefficients from
+// gen_erf_coeffs.py.
+// Maclaurin coefficients of erf(x)/x in x^2
+constexpr double ERF_SMALL[18] = {1.1283791670955126, -0.37612638903183754, 0.11283791670955126, -0.026866170645131252, 0.005223977625442188, -0.0008548327023450852, 0.00012055332981789664, -1.492565035840625e-05, 1.6462114365889246e-06, -1.6365844691234924e-07, 1.4807192815879218e-08, -1.2290555301717926e-09, 9.422759064650411e-11, -6.7113668551641105e-12, 4.4632242632864775e-13, -2.7835162072109212e-14, 1.6342614095367152e-15, -9.063970842808673e-17};
+// erfcx on [1.0, 2.0], relative error 1.11e-15
+constexpr double ERFCX_0[17] = {0.33142728163945206, -0.08500213770674318, 0.009951127341466127, -0.0010810888102208297, 0.0001102324363119606, -1.0636830518200183e-05, 9.775012680520952e-07, -8.597922867975393e-08, 7.267708404039144e-09, -5.92354376740777e-10, 4.668392193897692e-11, -3.5661104057662214e-12, 2.645897641899542e-13, -1.9121009342342266e-14, 1.3381078217890974e-15, -9.434725441780485e-17, -4.368034708275173e-18};
+// erfcx on [2.0, 3.5], relative error 6.66e-16
+constexpr double ERFCX_1[17] = {0.19943078169681194, -0.049380025359927374, 0.00584049638189269, -0.0006632226933018013, 7.259178935511782e-05, -7.68295276597933e-06, 7.883915794572118e-07, -7.861599623216689e-08, 7.632681891791811e-09, -7.22722091081603e-10, 6.683956871903844e-11, -6.045468160968323e-12, 5.353825538120602e-13, -4.64767559382121e-14, 3.949204052985312e-15, -3.3675011888776724e-16, 2.081224107920472e-17};
+// erfcx on [3.5, 6.0], relative error 8.88e-16
+constexpr double ERFCX_2[17] = {0.12011119518612724, -0.030775083678439782, 0.003865397162805734, -0.00047649553104630075, 5.7703715281466306e-05, -6.8706380375185965e-06, 8.049536979224912e-07, -9.285957122895145e-08, 1.0554537585057767e-08, -1.1826607953411862e-09, 1.3071368727667955e-10, -1.4257236555423402e-11, 1.5353302013861495e-12, -1.633113461668731e-13, 1.715853345016225e-14, -1.7882945488732093e-15, 1.8428163989931123e-16};
+
+SM2D_HD inline double cheb16(const double* c, double t) {
+    const double t2 = 2.0 * t;
+    double b1 = 0.0, b2 = 0.0;
+    for (int k = 16; k >= 1; --k) {
+        const double b0 = __builtin_fma(t2, b1, c[k] - b2);
+        b2 = b1;
+        b1 = b0;
+    }
+    return __builtin_fma(t, b1, c[0] - b2);
+}
+
 SM2D_HD inline double erf_d(double x) {
-    double ax = x < 0.0 ? -x : x;
+    const double ax = x < 0.0 ? -x : x;
     if (ax >= 6.0) return x < 0.0 ? -1.0 : 1.0;
     if (ax == 0.0) return x;
-    double x2 = ax * ax;
-    double two_x2 = 2.0 * x2;
-    double term = ax;
-    double sum = ax;
-    // Terms t_n = t_{n-1} * 2x^2 / (2n+1); at most ~2x^2 + 40 of them.
-    for (int n = 1; n < 200; ++n) {
-        term = term * two_x2 / (double)(2 * n + 1);
-        sum = sum + term;
-        if (term < 1e-17 * sum) break;
+    const double x2 = ax * ax;
+    double r;
+    if (ax <= 1.0) {
+        double p = ERF_SMALL[17];
+        for (int n = 16; n >= 0; --n) p = __builtin_fma(p, x2, ERF_SMALL[n]);
+        r = ax * p;
+    } else {
+        double cx;
+        if (ax <= 2.0) cx = cheb16(ERFCX_0, 2.0 * ax - 3.0);
+        else if (ax <= 3.5) cx = cheb16(ERFCX_1, (2.0 * ax - 5.5) / 1.5);
+        else cx = cheb16(ERFCX_2, (2.0 * ax - 9.5) / 2.5);
+        r = 1.0 - exp_d(-x2) * cx;
     }
-    const double two_over_sqrt_pi = 1.12837916709551257390e+00;
-    double r = two_over_sqrt_pi * exp_d(-x2) * sum;
     if (r > 1.0) r = 1.0;
     return x < 0.0 ? -r : r;
 }

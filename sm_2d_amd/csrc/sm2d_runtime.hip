@@ -231,11 +231,12 @@ void launch_iteration(sm2d_ctx* c, const Tp& P) {
         const int e0 = fine ? 6 : 2;
         k_phase_reset<<<1, 64, 0, c->stream>>>(c->ctl.p, c->bpcount.p);
         c->mark(e0);
+        const size_t dyn = transport_dyn_lds(c->g.N_E);
         if (fine)
-            k_transport<LY, true><<<blocks_for(h, 1, 65536), 64, 0, c->stream>>>(
+            k_transport<LY, true><<<blocks_for(h, 1, 65536), 64, dyn, c->stream>>>(
                 T, P, c->psi(in), c->psi(out), c->bkt(), list, nl, c->ltag.p, c->ctl.p, c->tally(), acc, c->flags.p);
         else
-            k_transport<LY, false><<<blocks_for(h, 1, 65536), 64, 0, c->stream>>>(
+            k_transport<LY, false><<<blocks_for(h, 1, 65536), 64, dyn, c->stream>>>(
                 T, P, c->psi(in), c->psi(out), c->bkt(), list, nl, c->ltag.p, c->ctl.p, c->tally(), acc, c->flags.p);
         c->mark(e0 + 1);
         k_dest<<<blocks_for(h, 256, 1024), 256, 0, c->stream>>>(list, nl, c->g.Nx, c->g.Nz, c->Ns, c->dtag.p, c->ctl.p,
@@ -533,6 +534,8 @@ int create(int device, const sm2d_grid* g, int n_spots, sm2d_ctx** out) {
     if (g->Nx <= 0 || g->Nz <= 0 || g->N_theta <= 0 || g->N_E <= 0 || n_spots <= 0 ||
         g->N_theta > 4095 * g->n_theta_local)
         return err("sm2d_ctx_create: bad grid");
+    // the transport stages the energy tables in LDS next to its payload rows (64 KiB per wave)
+    if (g->N_E > 1100) return err("sm2d_ctx_create: N_E above 1100 is not supported", SM2D_ERR_UNSUPPORTED);
     LayoutId lid;
     if (g->n_theta_local == 4 && g->n_e_local == 2 && g->n_x_sub == 8 && g->n_z_sub == 4) lid = L_REF;
     else if (g->n_theta_local == 8 && g->n_e_local == 4 && g->n_x_sub == 4 && g->n_z_sub == 1) lid = L_CFG;
