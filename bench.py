@@ -65,6 +65,53 @@ def cpu_baseline(prob, budget_s: float = 20.0) -> dict:
                       f"cell-bin updates in {dt:.2f} s, serial C++ oracle, -O2)"}
 
 
+def batched_section(hip, n_spots: int, plain_sample: int) -> dict:
+    """The 256-spot workload (pencil beams of 70-230 MeV over +-40 mm with sigma 3-6 mm on water
+    200 x 600 at 0.5 mm): one batched launch sequence for all spots against the plain path
+    (one spot at a time) timed on the first `plain_sample` spots of the same list."""
+    import torch
+    from sm_2d_amd import problems
+    from sm_2d_amd.engine import Engine
+
+    prob = problems.config4(n_spots=n_spots)
+    spots = [prob.make_spot(i) for i in range(n_spots)]
+    tp = prob.make_transport()
+    one = problems.setup_engine(hip, prob, Engine)
+    one.run_spot(spots[0], tp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for sp in spots[:plain_sample]:
+        one.run_spot(sp, tp)
+    torch.cuda.synchronize()
+    dt_plain = time.perf_counter() - t0
+    one.close()
+    eng = problems.setup_engine(hip, prob, Engine, n_spots=n_spots)
+    t0 = time.perf_counter()
+    eng.run_batch(spots[:min(8, n_spots)], tp)
+    t_warm = time.perf_counter() - t0
+    eng.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    esum, _, a = eng.run_batch(spots, tp)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    tm = eng.timing()
+    tuner = eng.tuner_report()
+    eng.close()
+    plain_sps = plain_sample / dt_plain
+    batch_sps = n_spots / dt
+    return {"workload": f"{n_spots} pencil beams, E0 U(70,230) MeV, x0 U(-40,40) mm, sigma_x U(3,6) mm, "
+                        "water 200x600 @ 0.5 mm, 256 log E x 512 theta, 128 local bins",
+            "batched_spots_per_s": batch_sps, "batched_s": dt,
+            "batched_updates_per_s": a.cell_bin_updates / dt,
+            "batched_iterations": a.iterations, "batched_cell_bin_updates": a.cell_bin_updates,
+            "plain_spots_per_s": plain_sps, "plain_sample_spots": plain_sample,
+            "speedup_spots_per_s": batch_sps / plain_sps, "surrogate": False,
+            "warm_batch_s": t_warm,
+            "stage_ms": {k: getattr(tm, k) for k in ("mask_ms", "transport_ms", "gather_ms", "audit_ms")},
+            "transport_tuner": tuner}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +119,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--batch-spots", type=int, default=256)
+    ap.add_argument("--plain-sample", type=int, default=8)
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (spots shard "
@@ -104,6 +154,7 @@ def main() -> int:
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     tm = eng.timing()
+    tuner = eng.tuner_report()
     a = audits[-1]
     achieved = 8.0 * tm.cell_bin_updates / (tm.transport_ms * 1e-3) / 1e9 if tm.transport_ms > 0 else 0.0
     traffic = _profiles_traffic("transport_config2.txt")
@@ -145,11 +196,14 @@ def main() -> int:
             "launches": tm.transport_launches,
             "avg_launch_us": tm.transport_ms * 1e3 / max(1, tm.transport_launches),
         },
+        "transport_tuner": tuner,
         "audit": {"max_cell_W_closure": a.max_cell_W_closure, "W_fail_iters": a.W_fail_iters,
                   "E_fail_iters": a.E_fail_iters, "terminated": a.terminated,
                   "E_boundary_MeV": a.E_boundary, "E_dep_MeV": a.E_dep},
     }
     eng.close()
+    if not args.no_batch:
+        out["batched"] = batched_section(hip, args.batch_spots, args.plain_sample)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(prob, args.cpu_budget)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

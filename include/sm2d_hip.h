@@ -234,6 +234,15 @@ int sm2d_k5_audit(sm2d_ctx* ctx, const double* tallies_delta, double W_src_out, 
 /* Copies psi_in <-> psi_out (load the second buffer for sm2d_k5_audit). */
 int sm2d_swap(sm2d_ctx* ctx);
 
+/* Transport launch shape: waves per listed cell (1, 2 or 4), or 0 for the built-in tuner, which
+ * times the three shapes per phase and launch size class on the first chunk of iterations of
+ * that class and then uses the fastest (no counterpart in the reference; every shape gives the
+ * same results).  The environment variable SM2D_TRANSPORT_WAVES sets the initial value. */
+int sm2d_set_transport_waves(sm2d_ctx* ctx, int waves);
+/* JSON summary of the tuner's measurements (per phase and size class: mean launch time per
+ * shape and the chosen shape), truncated to `capacity` bytes. */
+int sm2d_tuner_report(sm2d_ctx* ctx, char* buf, int capacity);
+
 /* Turns stage timing on (1) or off (0) and zeroes the counters; sm2d_get_timing reads them. */
 int sm2d_set_profiling(sm2d_ctx* ctx, int on);
 int sm2d_get_timing(sm2d_ctx* ctx, sm2d_timing* out);

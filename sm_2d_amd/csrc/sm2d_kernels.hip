@@ -382,7 +382,8 @@ struct WTabs {
     uint32_t own0;                // own psi_out slots present when the cell started
     uint32_t orig;                // global reference of a preloaded own slot (lane < own0)
     uint32_t used;                // LDS rows in use (wave-uniform)
-    float* rows;                  // this wave's LDS rows [NL][LB]
+    uint32_t cap;                 // LDS rows available
+    float* rows;                  // this wave's LDS rows [cap][LB]
 
     __device__ uint32_t gb(int t) const { return t == 0 ? b0 : t == 1 ? b1 : t == 2 ? b2 : t == 3 ? b3 : b4; }
     __device__ uint32_t gr(int t) const { return t == 0 ? r0 : t == 1 ? r1 : t == 2 ? r2 : t == 3 ? r3 : r4; }
@@ -411,7 +412,7 @@ struct WTabs {
         if (n < (uint32_t)KB) {
             const int s = (int)n;
             uint32_t ref;
-            if (used < (uint32_t)LY::NL) {
+            if (used < cap) {
                 ref = CACHED | used;
                 float* row = rows + (size_t)used * LY::LB;
 #pragma unroll
@@ -525,6 +526,7 @@ __device__ inline float emit(WTabs<LY>& W, CellAccum& a, const Tabs& T, Bkt& bk,
     const int el = b.eb % LY::EL;
     const uint32_t bE = (uint32_t)(b.eb / LY::EL);
     float dropped = 0.0f;
+#pragma unroll
     for (int ti = 0; ti < 2; ++ti) {
         const float tw = ti == 0 ? b.tw0 : b.tw1;
         if (tw <= 0.0f) continue;
@@ -569,12 +571,20 @@ __device__ inline void flush_tables(WTabs<LY>& W, Psi& out, Bkt& bk, int cell, u
         for (int k = 0; k < LY::PL; ++k) dst[l + 64 * k] = src[l + 64 * k];
     }
     const uint64_t mo = __ballot(l < 32 && (uint32_t)l >= W.own0 && (uint32_t)l < W.n0 && (W.r0 & CACHED));
-    uint32_t base = 0;
     const uint32_t co = (uint32_t)__popcll(mo);
-    if (co) {
-        if (l == 0) base = atomicAdd(out.pcount, co);
-        base = rlu(base, 0);
+    uint64_t mf[4];
+    uint32_t cb = 0;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        mf[f] = __ballot(l < 32 && (uint32_t)l < W.gn(1 + f) && (W.gr(1 + f) & CACHED));
+        cb += (uint32_t)__popcll(mf[f]);
     }
+    // both pool reservations are issued before either result is used
+    uint32_t base = 0, bbase = 0;
+    if (l == 0 && co) base = atomicAdd(out.pcount, co);
+    if (l == 1 && cb) bbase = atomicAdd(bk.pcount, cb);
+    base = rlu(base, 0);
+    bbase = rlu(bbase, 1);
     bool ok = !co || base + co <= out.pcap;
     if (!ok && l == 0) atomicOr(&flags[F_PSI_FULL], 1u);
     if (ok) {
@@ -594,20 +604,11 @@ __device__ inline void flush_tables(WTabs<LY>& W, Psi& out, Bkt& bk, int cell, u
         if (l == 0) out.ns[cell] = W.n0;
     }
     // buckets of faces 0..3
-    uint64_t mf[4];
-    uint32_t cb = 0;
-    for (int f = 0; f < 4; ++f) {
-        mf[f] = __ballot(l < 32 && (uint32_t)l < W.gn(1 + f) && (W.gr(1 + f) & CACHED));
-        cb += (uint32_t)__popcll(mf[f]);
-    }
-    base = 0;
-    if (cb) {
-        if (l == 0) base = atomicAdd(bk.pcount, cb);
-        base = rlu(base, 0);
-    }
+    base = bbase;
     ok = !cb || base + cb <= bk.pcap;
     if (!ok && l == 0) atomicOr(&flags[F_BKT_FULL], 1u);
     uint32_t k0 = 0;
+#pragma unroll
     for (int f = 0; f < 4; ++f) {
         const uint32_t nf = ok ? W.gn(1 + f) : 0u;
         if (ok) {
@@ -662,6 +663,7 @@ struct FastLds {
     float recw[64 * MAXREC];
     uint32_t key[64 * MAXKEY];
     uint32_t krow[64 * MAXKEY];
+    int nkey[64], nrec[64];
 };
 
 struct LaneOut {
@@ -676,6 +678,7 @@ __device__ inline void rec_emit(FastLds& F, LaneOut& o, int t, const Bins& b, fl
     const int l = lane_id();
     const int el = b.eb % LY::EL;
     const uint32_t bE = (uint32_t)(b.eb / LY::EL);
+#pragma unroll
     for (int ti = 0; ti < 2; ++ti) {
         const float tw = ti == 0 ? b.tw0 : b.tw1;
         if (tw <= 0.0f) continue;
@@ -821,23 +824,34 @@ __device__ inline LaneOut fast_prepare(FastLds& F, const Comp& r, bool live, int
     return o;
 }
 
-// Claims of the chunk's distinct keys in order of first appearance; false when a claim needs
+// Claims of the batch's distinct keys in order of first appearance; false when a claim needs
 // the serial path (full table, global payload, pool overflow).  Resolves record addresses.
 template <class LY>
-__device__ inline bool fast_claims(FastLds& F, WTabs<LY>& W, const LaneOut& o, Psi& out, Bkt& bk,
-                                   uint32_t* flags) {
+__device__ inline bool fast_claims(FastLds& F, WTabs<LY>& W, Psi& out, Bkt& bk, uint32_t* flags) {
     const int l = lane_id();
-    uint32_t pend = (o.nkey > 0) ? ((1u << o.nkey) - 1u) : 0u;
+    const int nkey = F.nkey[l], nrec = F.nrec[l];
+    uint32_t k0 = EMPTY, k1 = EMPTY, k2 = EMPTY, k3 = EMPTY, k4 = EMPTY;
+    if (nkey > 0) k0 = F.key[l * MAXKEY + 0];
+    if (nkey > 1) k1 = F.key[l * MAXKEY + 1];
+    if (nkey > 2) k2 = F.key[l * MAXKEY + 2];
+    if (nkey > 3) k3 = F.key[l * MAXKEY + 3];
+    if (nkey > 4) k4 = F.key[l * MAXKEY + 4];
+    uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;
+    uint32_t pend = (nkey > 0) ? ((1u << nkey) - 1u) : 0u;
     while (true) {
         const uint64_t hp = __ballot(pend != 0u);
         if (!hp) break;
         const int j = __ffsll((unsigned long long)hp) - 1;
         const int kk = __ffs(rlu(pend, j)) - 1;
-        const uint32_t K = F.key[j * MAXKEY + kk];
+        const uint32_t mine = kk == 0 ? k0 : kk == 1 ? k1 : kk == 2 ? k2 : kk == 3 ? k3 : k4;
+        const uint32_t K = rlu(mine, j);
         uint32_t hit = 0;
-        for (int q = 0; q < MAXKEY; ++q)
-            if ((pend >> q) & 1u)
-                if (F.key[l * MAXKEY + q] == K) hit |= 1u << q;
+        hit |= (k0 == K) ? 1u : 0u;
+        hit |= (k1 == K) ? 2u : 0u;
+        hit |= (k2 == K) ? 4u : 0u;
+        hit |= (k3 == K) ? 8u : 0u;
+        hit |= (k4 == K) ? 16u : 0u;
+        hit &= pend;
         pend &= ~hit;
         const int t = (int)(K >> 24);
         const uint32_t bid = K & 0xFFFFFFu;
@@ -847,44 +861,64 @@ __device__ inline bool fast_claims(FastLds& F, WTabs<LY>& W, const LaneOut& o, P
         if (s < 0) return false;
         const uint32_t ref = rlu(W.gr(t), s);
         if (!(ref & CACHED)) return false;
-        for (int q = 0; q < MAXKEY; ++q)
-            if ((hit >> q) & 1u) F.krow[l * MAXKEY + q] = ref & ~CACHED;
+        const uint32_t row = ref & ~CACHED;
+        if (hit & 1u) r0 = row;
+        if (hit & 2u) r1 = row;
+        if (hit & 4u) r2 = row;
+        if (hit & 8u) r3 = row;
+        if (hit & 16u) r4 = row;
     }
-    for (int rr = 0; rr < o.nrec; ++rr) {
+    for (int rr = 0; rr < nrec; ++rr) {
         const uint32_t p = F.rec[l * MAXREC + rr];
-        F.rec[l * MAXREC + rr] = F.krow[l * MAXKEY + (p >> 16)] * (uint32_t)LY::LB + (p & 0xFFFFu);
+        const uint32_t kidx = p >> 16;
+        const uint32_t row = kidx == 0 ? r0 : kidx == 1 ? r1 : kidx == 2 ? r2 : kidx == 3 ? r3 : r4;
+        F.rec[l * MAXREC + rr] = row * (uint32_t)LY::LB + (p & 0xFFFFu);
     }
     return true;
 }
 
-// Components in bin order: all records of a component in one LDS round, and the accumulators
-// held by lanes 0..NKACC-1.
+// Components in bin order (the live lanes of a batch are 0..cnt-1).  Every record of a component
+// is added with one LDS atomic-add instruction (its destinations are distinct); LDS executes a
+// wave's instructions in issue order, so the adds to any one payload bin happen in component
+// order, as in the serial form.  Lanes 0..NKACC-1 add the component's contributions to the
+// accumulator they hold.  Components are fetched in groups of 8 so that one wait covers a group.
 template <class LY>
-__device__ inline void fast_steps(FastLds& F, const WTabs<LY>& W, uint64_t m, int nrec, double& accd,
-                                  float& accf) {
+__device__ inline void fast_steps(FastLds& F, const WTabs<LY>& W, uint64_t m, double& accd, float& accf) {
+    constexpr int G8 = 4;
+    const int cnt = __popcll(m);
     const int l = lane_id();
     const bool fl = kacc_is_float(l);
-    while (m) {
-        const int j = __ffsll((unsigned long long)m) - 1;
-        m &= m - 1;
-        if (l < NKACC) {
-            const double c0 = F.con[j * NCON + 2 * l], c1 = F.con[j * NCON + 2 * l + 1];
-            if (fl) {
-                accf += (float)c0;
-                accf += (float)c1;
-            } else {
-                accd += c0;
-                accd += c1;
-            }
+    const bool accl = l < NKACC;
+    const int rr = l - 16;
+    const bool recl = rr >= 0 && rr < MAXREC;
+    for (int j0 = 0; j0 < cnt; j0 += G8) {
+        double c0[G8], c1[G8];
+        float w[G8];
+        uint32_t ad[G8];
+        int nr[G8];
+#pragma unroll
+        for (int u = 0; u < G8; ++u) {
+            const int j = min(j0 + u, cnt - 1);
+            c0[u] = accl ? F.con[j * NCON + 2 * l] : 0.0;
+            c1[u] = accl ? F.con[j * NCON + 2 * l + 1] : 0.0;
+            nr[u] = F.nrec[j];
+            ad[u] = recl ? F.rec[j * MAXREC + rr] : 0u;
+            w[u] = recl ? F.recw[j * MAXREC + rr] : 0.0f;
         }
-        const int nr = rli(nrec, j);
-        if (l >= 16 && l < 16 + nr) {
-            const int rr = l - 16;
-            const float w = F.recw[j * MAXREC + rr];
-            if (w > 0.0f) {
-                float* p = W.rows + F.rec[j * MAXREC + rr];
-                *p += w;
+#pragma unroll
+        for (int u = 0; u < G8; ++u) {
+            if (j0 + u >= cnt) break;
+            if (accl) {
+                if (fl) {
+                    accf += (float)c0[u];
+                    accf += (float)c1[u];
+                } else {
+                    accd += c0[u];
+                    accd += c1[u];
+                }
             }
+            if (recl && rr < nr[u] && w[u] > 0.0f)
+                __hip_atomic_fetch_add(W.rows + ad[u], w[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         }
     }
 }
@@ -924,15 +958,535 @@ __device__ inline void accum_to_lanes(const CellAccum& a, double& accd, float& a
 // that the binary searches and interpolations of every component read LDS.
 inline size_t transport_dyn_lds(int NE) { return sizeof(float) * (4 * (size_t)NE + 1); }
 
+// Per-bin physics of one component (the per-bin algorithm of k2_coarsetransport.cu:76-704 and
+// k3_finetransport.cu:94-760) plus the table-independent parts of its apply: one lane, one
+// (slot, local bin) of the listed cell.
+struct CellGeo {
+    int lc, iz, ix, Nx, NT, NE;
+    float dx, dz, hdx, hdz, tmin, dth, rh, X0, A, B, sx0;
+    uint32_t latlisted;
+};
+
 template <class LY, bool FINE>
-__global__ __launch_bounds__(64) void k_transport(Tabs Tg, Tp P, Psi in, Psi out, Bkt bk,
-                                                  const uint32_t* __restrict__ list,
-                                                  const uint32_t* __restrict__ nlist,
-                                                  const uint32_t* __restrict__ ltag, const uint32_t* ctl,
-                                                  Tally tal, double* acc, uint32_t* flags) {
+__device__ inline Comp transport_component(const Tabs& T, const Tp& P, const CellGeo& G, bool live, float w_in,
+                                           int lidx_in, uint32_t slot, uint32_t bT, uint32_t bEb,
+                                           uint32_t* flags, bool count_unsup) {
+    const int lc = G.lc, iz = G.iz, ix = G.ix, Nx = G.Nx, NT = G.NT, NE = G.NE;
+    const float dx = G.dx, dz = G.dz, hdx = G.hdx, hdz = G.hdz, tmin = G.tmin, dth = G.dth;
+    const float rh = G.rh, X0 = G.X0, A = G.A, B = G.B, sx0 = G.sx0;
+    const uint32_t latlisted = G.latlisted;
+    Comp r;
+    r.kind = 2;
+    r.face = -1;
+    r.lidx = lidx_in;
+    r.w = live ? w_in : 0.0f;
+    r.edep = r.w_rem = r.E_rem = r.w_new = r.E_new = 0.0f;
+    r.th_new = r.x_new = r.z_new = 0.0f;
+    r.wcf = r.wl = r.wr = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 11; ++q) r.sw[q] = 0.0f;
+    if (live) {
+        const int lidx = r.lidx;
+        const float weight = w_in;
+        int tbin = (int)bT * LY::TL + LY::lt(lidx);
+        int ebin = (int)bEb * LY::EL + LY::le(lidx);
+        if (FINE) {
+            tbin = max(0, min(tbin, NT - 1));
+            ebin = max(0, min(ebin, NE - 1));
+        }
+        const float E = ecenter(T.ee, ebin);
+        r.E = E;
+        const bool pruned = FINE ? (weight < 1e-15f) : (weight < 1e-12f);
+        if (pruned) {
+            r.kind = 0;
+        } else if (E <= 0.1f) {
+            r.kind = 1;
+        } else {
+            const float theta = FINE ? T.te[tbin] + 0.5f * dth : tmin + (tbin + 0.5f) * dth;
+            const float xc = dx * ((float)(2 * LY::lx(lidx) + 1 - LY::XS) / (float)(2 * LY::XS));
+            const float zc = dz * ((float)(2 * LY::lz(lidx) + 1 - LY::ZS) / (float)(2 * LY::ZS));
+            const float mu = cosf_(theta), eta = sinf_(theta);
+            float ds, dE, E_new, edep, w_new, w_rem, E_rem, sx, sxt, sth;
+            unsigned pseed;
+            if (FINE) {
+                float s1 = (mu > 0) ? (hdz - zc) / mu : 1e30f;
+                float s2 = (mu < 0) ? (-hdz - zc) / mu : 1e30f;
+                float s3 = (eta > 0) ? (hdx - xc) / eta : 1e30f;
+                float s4 = (eta < 0) ? (-hdx - xc) / eta : 1e30f;
+                float sb = fmaxf(fminf(fminf(s1, s2), fminf(s3, s4)), 0.0f);
+                ds = fminf(max_step(T, E, rh, P.rule), sb * 1.001f);
+                float mdE = deposition(T, E, ds, rh);
+                if (P.strag) {
+                    unsigned sd = (unsigned)((lc * 7 + (int)slot * 13 + lidx * 17) ^ 0x5DEECE66DL);
+                    float sE = straggling(E, ds, rh);
+                    dE = (sE <= 0.0f) ? mdE : fmaxf(mdE + gauss(sd) * sE, 0.0f);
+                } else {
+                    dE = mdE;
+                }
+                dE = fminf(dE, E);
+                E_new = E - dE;
+                edep = dE * weight;
+                if (P.nuc) {
+                    float surv = expf_(-(nuc_sigma(E) * rh) * ds);
+                    w_new = weight * surv;
+                    w_rem = weight - w_new;
+                    E_rem = w_rem * E;
+                } else {
+                    w_new = weight;
+                    w_rem = 0.0f;
+                    E_rem = 0.0f;
+                }
+                float smm = fmaxf(ds, 0.0f);
+                float Em = fmaxf(E - 0.5f * dE, T.ee[0]);
+                float Tp_ = scat_power(Em, smm, X0);
+                sth = sqrtf_(fmaxf(Tp_ * smm, 0.0f));
+                float dC = 2.0f * B * smm + A * smm * smm + Tp_ * smm * smm * smm / 3.0f;
+                dC = fmaxf(dC, 0.0f);
+                sx = fmaxf(sqrtf_(dC), 0.01f);
+                sxt = 0.001f;
+                pseed = (unsigned)fminf((float)iz * dz * 1000.0f, 4294967295.0f);
+            } else {
+                float cstep = fminf(P.step_coarse, fminf(dx, dz));
+                float crs = cstep;
+                bool split = false;
+                if (E > P.E_on + 1e-6f) {
+                    float Rc = lookup_R(T, E);
+                    float Rf = lookup_R(T, fmaxf(P.E_on, T.ee[0]));
+                    float rtf = (Rc - Rf) / rh;
+                    if (rtf > 0.0f && rtf < crs) {
+                        crs = rtf;
+                        cstep = crs;
+                        split = true;
+                    }
+                }
+                dE = fminf(deposition(T, E, crs, rh), E);
+                E_new = E - dE;
+                edep = dE * weight;
+                float surv = expf_(-(nuc_sigma(E) * rh) * crs);
+                w_new = weight * surv;
+                w_rem = weight - w_new;
+                E_rem = w_rem * E;
+                if (split && E_new > P.E_on) {
+                    float ex = E_new - P.E_on;
+                    if (ex > 0.0f && w_new > 0.0f) edep += ex * w_new;
+                    E_new = P.E_on;
+                }
+                float ps = fmaxf(0.0f, (float)iz * dz + (zc + 0.5f * dz));
+                float smm = fmaxf(crs, 0.0f);
+                float st0 = (ps > 0.0f) ? highland(E, ps, X0) : 0.0f;
+                float Ao = st0 * st0;
+                float Bo = 0.5f * Ao * ps;
+                float Co = sx0 * sx0 + (Ao * ps * ps) / 3.0f;
+                float Em = fmaxf(E - 0.5f * dE, T.ee[0]);
+                float Tp_ = scat_power(Em, smm, X0);
+                float An = fmaxf(Ao + Tp_ * smm, 0.0f);
+                float Cn = fmaxf(Co + 2.0f * Bo * smm + Ao * smm * smm +
+                                     (1.0f / 3.0f) * Tp_ * smm * smm * smm,
+                                 0.0f);
+                sth = sqrtf_(fmaxf(An - Ao, 0.0f));
+                float dC = fmaxf(Cn - Co, 0.0f);
+                float unres = fminf(0.35f, 0.04f + 0.003f * ps);
+                sx = fmaxf(sqrtf_(fmaxf(dC + unres * Co, 0.0f)), 0.01f);
+                sxt = fmaxf(sinf_(fminf(sth, 1.57f)) * smm / 1.7320508f, 0.01f);
+                pseed = (unsigned)fminf(ps * 1000.0f, 4294967295.0f);
+                ds = cstep;
+            }
+            unsigned ts = ((unsigned)(lc + 1) * 2654435761u) ^
+                          ((unsigned)(slot + 1) * 2246822519u) ^
+                          ((unsigned)(lidx + 1) * 3266489917u) ^ (unsigned)(ebin + 17) ^
+                          (pseed * 747796405u);
+            float tsc = (sth <= 0.0f) ? 0.0f : gauss(ts) * sth;
+            float thn = fmaxf(T.te[0], fminf(theta + tsc, T.te[NT]));
+            float mun = cosf_(thn), etn = sinf_(thn);
+            float xn = xc + etn * ds, zn = zc + mun * ds;
+            int face = -1;
+            if (zn >= hdz - 0.001f) face = 0;
+            else if (zn <= -hdz + 0.001f) face = 1;
+            else if (xn >= hdx - 0.001f) face = 2;
+            else if (xn <= -hdx + 0.001f) face = 3;
+            xn = fmaxf(-hdx, fminf(xn, hdx));
+            zn = fmaxf(-hdz, fminf(zn, hdz));
+            r.edep = edep;
+            r.w_rem = w_rem;
+            r.E_rem = E_rem;
+            r.w_new = w_new;
+            r.E_new = E_new;
+            r.face = face;
+            r.th_new = thn;
+            r.x_new = xn;
+            r.z_new = zn;
+            if (E_new > 0.1f) {
+                if (face == 0) {
+                    // device_emit_lateral_spread weights (device_bucket.cuh:880-916)
+                    float s = fmaxf(sxt, 1e-6f);
+                    float xmin = xn - 0.5f * 11.0f * dx;
+                    float prev = gcdf(xmin, xn, s), sum = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < 11; ++q) {
+                        float cur = gcdf(xmin + (q + 1) * dx, xn, s);
+                        r.sw[q] = cur - prev;
+                        prev = cur;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 11; ++q) sum += r.sw[q];
+                    if (sum > 1e-10f) {
+#pragma unroll
+                        for (int q = 0; q < 11; ++q) r.sw[q] /= sum;
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 11; ++q) r.sw[q] = (q == 5) ? 1.0f : 0.0f;
+                    }
+                    sum = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < 11; ++q) {
+                        r.sw[q] = fmaxf(0.0f, r.sw[q]);
+                        sum += r.sw[q];
+                    }
+                    if (sum < 1e-12f) {
+#pragma unroll
+                        for (int q = 0; q < 11; ++q) r.sw[q] = (q == 5) ? 1.0f : 0.0f;
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 11; ++q) r.sw[q] /= sum;
+                    }
+                } else if (face < 0) {
+                    // device_gaussian_spread_weights_subcell + in-cell fraction and
+                    // tails (k3_finetransport.cu:585-653)
+                    float s = fmaxf(FINE ? sxt : sx, 1e-6f);
+                    float dxs = dx / LY::XS;
+                    float xmin = -dx * 0.5f;
+                    float prev = gcdf(xmin, xn, s), sum = 0.0f;
+                    float w8[LY::XS];
+#pragma unroll
+                    for (int q = 0; q < LY::XS; ++q) {
+                        float cur = gcdf(xmin + (q + 1) * dxs, xn, s);
+                        w8[q] = cur - prev;
+                        prev = cur;
+                    }
+#pragma unroll
+                    for (int q = 0; q < LY::XS; ++q) sum += w8[q];
+                    if (sum > 1e-10f) {
+#pragma unroll
+                        for (int q = 0; q < LY::XS; ++q) w8[q] /= sum;
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < LY::XS; ++q) w8[q] = (q == LY::XS / 2) ? 1.0f : 0.0f;
+                    }
+                    float win = gcdf(hdx, xn, sx) - gcdf(-hdx, xn, sx);
+                    float wcf = fminf(1.0f, fmaxf(0.0f, win));
+                    float wss = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < LY::XS; ++q) {
+                        w8[q] = fmaxf(0.0f, w8[q]);
+                        wss += w8[q];
+                    }
+                    if (wss < 1e-12f) {
+#pragma unroll
+                        for (int q = 0; q < LY::XS; ++q) w8[q] = (q == LY::XS / 2) ? 1.0f : 0.0f;
+                        wss = 1.0f;
+                    }
+#pragma unroll
+                    for (int q = 0; q < LY::XS; ++q) r.sw[q] = (w8[q] / wss) * wcf;
+                    float wl = 0.0f, wr = 0.0f;
+                    if (wcf < 1.0f - 1e-6f) {
+                        float wlr = fmaxf(0.0f, gcdf(-hdx, xn, sx));
+                        float wrr = fmaxf(0.0f, 1.0f - gcdf(hdx, xn, sx));
+                        float wtt = fmaxf(0.0f, 1.0f - wcf);
+                        float wts = wlr + wrr;
+                        if (wtt > 0.0f && wts > 1e-12f) {
+                            float sc = wtt / wts;
+                            wl = wlr * sc;
+                            wr = wrr * sc;
+                        }
+                    }
+                    r.wcf = wcf;
+                    r.wl = wl;
+                    r.wr = wr;
+                }
+            }
+        }
+    }
+    // ------- emission binning and the table-independent parts of the apply -------
+    r.bn = Bins{0, 0, 0, 0.0f, 0.0f, 0.0f};
+    r.i0 = r.i1 = r.i2 = 0;
+    r.g0 = r.g1 = r.g2 = 0.0f;
+    if (live && r.kind == 2 && r.E_new > 0.1f) {
+        r.bn = emission_bins(T, r.th_new, r.E_new);
+        const float w_new = r.w_new;
+        if (r.face >= 0) {
+            r.g0 = ecenter(T.ee, find_bin(T.ee, NE, r.E_new));
+            if (r.face == 0) {
+                float drop = 0.0f, lat_bnd = 0.0f;
+                uint32_t unsup = 0;
+#pragma unroll
+                for (int q = 0; q < 11; ++q) {
+                    const float wf = r.sw[q];
+                    if (wf <= 0.0f) continue;
+                    const int it = ix + q - 5;
+                    if (it < 0 || it >= Nx) {
+                        lat_bnd += w_new * wf;
+                        continue;
+                    }
+                    if (q == 5) {
+                        drop += 0.0f;  // the centre emission's drop, zero unless a pool fills
+                    } else {
+                        // another cell's +z bucket: a drop when that cell is not listed
+                        // in this phase (device_bucket.cuh:953-963); a listed target
+                        // would need a cross-wave ordered write, which this path rejects
+                        if ((latlisted >> q) & 1u) ++unsup;
+                        drop += w_new * wf;
+                    }
+                }
+                if (unsup && count_unsup) atomicAdd(&flags[F_UNSUPPORTED], unsup);
+                r.g1 = drop;
+                r.g2 = lat_bnd;
+            } else {
+                const int face = r.face;
+                const float xon = (face == 2) ? -dx * 0.5f : (face == 3 ? dx * 0.5f : r.x_new);
+                r.i0 = min(max((int)(((xon / dx) + 0.5f) * LY::XS), 0), LY::XS - 1);
+                if (face == 1) r.i1 = LY::ZS - 1;
+                else r.i1 = min(max((int)(((r.z_new / dz) + 0.5f) * LY::ZS), 0), LY::ZS - 1);
+            }
+        } else {
+            int tbn = (int)((r.th_new - tmin) / dth);
+            tbn = (int)fmaxf(0, fminf(tbn, NT - 1));
+            const int ebn = find_bin(T.ee, NE, r.E_new);
+            r.i0 = tbn;
+            r.i1 = ebn;
+            r.i2 = min(max((int)(((r.z_new / dz) + 0.5f) * LY::ZS), 0), LY::ZS - 1);
+            r.g0 = ecenter(T.ee, ebn);
+        }
+    }
+    return r;
+}
+
+// The serial form of the ordered apply (components in bin order, one at a time): used for a
+// batch whose claims need the closest-block fallback or a global payload.
+template <class LY, bool FINE>
+__device__ inline void serial_apply(const Comp& r, uint64_t m, WTabs<LY>& W, CellAccum& a, const Tabs& T,
+                                    Psi& out, Bkt& bk, uint32_t* flags, const CellGeo& G) {
+    const int l = lane_id();
+    const int ix = G.ix, iz = G.iz, Nx = G.Nx, NT = G.NT, NE = G.NE;
+    const float tmin = G.tmin, dth = G.dth;
+    (void)NE;
+    while (m) {
+        const int j = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        const int kind = rli(r.kind, j);
+        const float w = rlf(r.w, j), E = rlf(r.E, j);
+        a.comps++;
+        a.cl_win += (double)w;
+        a.cl_Ein += (double)w * (double)E;
+        if (kind == 0) {
+            a.drop_w += (double)w;
+            a.drop_E += (double)(E * w);
+            a.cl_wout += (double)w;
+            a.cl_Eout += (double)(E * w);
+            continue;
+        }
+        if (kind == 1) {
+            a.w_cut += w;
+            a.E_cut += E * w;
+            a.cl_wout += (double)w;
+            a.cl_Eout += (double)(E * w);
+            continue;
+        }
+        const float edep = rlf(r.edep, j), w_rem = rlf(r.w_rem, j), E_rem = rlf(r.E_rem, j);
+        const float w_new = rlf(r.w_new, j), E_new = rlf(r.E_new, j);
+        const int face = rli(r.face, j);
+        a.edep += edep;
+        a.w_nuc += w_rem;
+        a.E_nuc += E_rem;
+        a.cl_wout += (double)w_rem;
+        a.cl_Eout += (double)edep + (double)E_rem;
+        if (E_new <= 0.1f) {
+            a.w_cut += w_new;
+            a.E_cut += E_new * w_new;
+            a.cl_wout += (double)w_new;
+            a.cl_Eout += (double)(E_new * w_new);
+            continue;
+        }
+        Bins bn;
+        bn.bc = rli(r.bn.bc, j);
+        bn.tb1 = rli(r.bn.tb1, j);
+        bn.eb = rli(r.bn.eb, j);
+        bn.tw0 = rlf(r.bn.tw0, j);
+        bn.tw1 = rlf(r.bn.tw1, j);
+        bn.Ecen = rlf(r.bn.Ecen, j);
+        if (face >= 0) {
+            bool nb = (face == 0) ? (iz + 1 < T.Nz) : (face == 1) ? (iz > 0)
+                    : (face == 2) ? (ix + 1 < Nx) : (ix > 0);
+            if (!nb) {
+                a.w_bnd += w_new;
+                a.E_bnd += E_new * w_new;
+                a.cl_wout += (double)w_new;
+                a.cl_Eout += (double)(E_new * w_new);
+                continue;
+            }
+            const float E_emit = rlf(r.g0, j);
+            float drop = 0.0f;
+            if (face == 0) {
+                const float wf5 = rlf(r.sw[5], j);
+                const float lat_bnd = rlf(r.g2, j);
+                float d5 = 0.0f;
+                if (wf5 > 0.0f) d5 = emit<LY>(W, a, T, bk, flags, 0, bn, w_new * wf5, LY::XS / 2, 0);
+                if (d5 == 0.0f) {
+                    drop = rlf(r.g1, j);
+                } else {  // the lateral sum with the centre's drop in its place (q = 5)
+#pragma unroll
+                    for (int q = 0; q < 11; ++q) {
+                        const float wf = rlf(r.sw[q], j);
+                        if (wf <= 0.0f) continue;
+                        const int it = ix + q - 5;
+                        if (it < 0 || it >= Nx) continue;
+                        drop += (q == 5) ? d5 : w_new * wf;
+                    }
+                }
+                if (lat_bnd > 0.0f) {
+                    a.w_bnd += lat_bnd;
+                    a.E_bnd += E_new * lat_bnd;
+                    a.cl_wout += (double)lat_bnd;
+                    a.cl_Eout += (double)(E_new * lat_bnd);
+                }
+                if (drop > 0.0f) {
+                    a.drop_w += (double)drop;
+                    a.drop_E += (double)(E_new * drop);
+                    a.cl_wout += (double)drop;
+                    a.cl_Eout += (double)(E_new * drop);
+                }
+                if (FINE) {
+                    float tw = fmaxf(0.0f, w_new - lat_bnd - drop);
+                    if (tw > 0.0f) {
+                        a.edep += (double)((E_new - E_emit) * tw);
+                        a.cl_Eout += (double)((E_new - E_emit) * tw);
+                    }
+                }
+            } else {
+                const int xsn = rli(r.i0, j), zsn = rli(r.i1, j);
+                drop = emit<LY>(W, a, T, bk, flags, face, bn, w_new, xsn, zsn);
+                if (drop > 0.0f) {
+                    a.drop_w += (double)drop;
+                    a.drop_E += (double)(E_new * drop);
+                    a.cl_wout += (double)drop;
+                    a.cl_Eout += (double)(E_new * drop);
+                }
+                if (FINE) {
+                    float tw = fmaxf(0.0f, w_new - drop);
+                    if (tw > 0.0f) {
+                        a.edep += (double)((E_new - E_emit) * tw);
+                        a.cl_Eout += (double)((E_new - E_emit) * tw);
+                    }
+                }
+            }
+            continue;
+        }
+        // ---- remain in the cell ----
+        const int tbn = rli(r.i0, j), ebn = rli(r.i1, j), zst = rli(r.i2, j);
+        const uint32_t bidn = ((uint32_t)(ebn / LY::EL) << 12) | ((uint32_t)(tbn / LY::TL) & 0xFFF);
+        uint32_t sel;
+        const int os = W.claim(0, bidn, sel, out.pool, out.pcount, out.pcap, flags, F_PSI_FULL);
+        if (os < 0) {
+            a.drop_w += (double)w_new;
+            a.drop_E += (double)(E_new * w_new);
+            a.cl_wout += (double)w_new;
+            a.cl_Eout += (double)(E_new * w_new);
+            continue;
+        }
+        int tbe = tbn, ebe = ebn;
+        float E_emit = rlf(r.g0, j);
+        if (sel != bidn) {
+            int bte = (int)(sel & 0xFFF), bee = (int)((sel >> 12) & 0xFFF);
+            tbe = max(bte * LY::TL, min(tbe, bte * LY::TL + LY::TL - 1));
+            ebe = max(bee * LY::EL, min(ebe, bee * LY::EL + LY::EL - 1));
+            E_emit = T.ee[ebe] + 0.5f * (T.ee[ebe + 1] - T.ee[ebe]);
+        }
+        const uint32_t oref = rlu(W.r0, os);
+        float mywf = 0.0f;
+#pragma unroll
+        for (int q = 0; q < LY::XS; ++q) {
+            float wf = rlf(r.sw[q], j);
+            if (l == q) mywf = wf;
+            if (wf > 0.0f) {
+                float wsp = w_new * wf;
+                a.cl_wout += (double)wsp;
+                a.cl_Eout += (double)wsp * (double)E_emit;
+            }
+        }
+        if (l < LY::XS && mywf > 0.0f)
+            W.add(oref, LY::enc(tbe % LY::TL, ebe % LY::EL, l, zst), w_new * mywf, out.pool);
+        const float wcf = rlf(r.wcf, j), wl = rlf(r.wl, j), wr = rlf(r.wr, j);
+        if (wcf < 1.0f - 1e-6f) {
+            const int zsb = zst;
+            if (wl > 0.0f) {
+                float wsl = w_new * wl;
+                if (ix > 0) {
+                    float d = emit<LY>(W, a, T, bk, flags, 3, bn, wsl, LY::XS - 1, zsb);
+                    if (d > 0.0f) {
+                        a.drop_w += (double)d;
+                        a.drop_E += (double)(E_new * d);
+                        a.cl_wout += (double)d;
+                        a.cl_Eout += (double)(E_new * d);
+                    }
+                } else {
+                    a.w_bnd += wsl;
+                    a.E_bnd += E_new * wsl;
+                    a.cl_wout += (double)wsl;
+                    a.cl_Eout += (double)(E_new * wsl);
+                }
+            }
+            if (wr > 0.0f) {
+                float wsr = w_new * wr;
+                if (ix < Nx - 1) {
+                    float d = emit<LY>(W, a, T, bk, flags, 2, bn, wsr, 0, zsb);
+                    if (d > 0.0f) {
+                        a.drop_w += (double)d;
+                        a.drop_E += (double)(E_new * d);
+                        a.cl_wout += (double)d;
+                        a.cl_Eout += (double)(E_new * d);
+                    }
+                } else {
+                    a.w_bnd += wsr;
+                    a.E_bnd += E_new * wsr;
+                    a.cl_wout += (double)wsr;
+                    a.cl_Eout += (double)(E_new * wsr);
+                }
+            }
+        }
+        if (FINE) {
+            float rf = wcf;
+            if (ix > 0) rf += wl;
+            if (ix < Nx - 1) rf += wr;
+            rf = fminf(1.0f, fmaxf(0.0f, rf));
+            float rw = w_new * rf;
+            if (rw > 0.0f) {
+                a.edep += (double)((E_new - E_emit) * rw);
+                a.cl_Eout += (double)((E_new - E_emit) * rw);
+            }
+        }
+    }
+}
+
+// Item queue of a cell: its non-zero (slot, local bin) components in the serial order
+// (slots ascending, local bins ascending), processed in batches of 64 by the four waves.
+// TW waves work on one listed cell: they evaluate the physics of TW batches at once, wave 0
+// applies them in order.  TW is a launch choice (1, 2 or 4) made by the runtime's tuner; all
+// choices give identical results.
+constexpr int QCAP = 512;
+
+template <class LY, bool FINE, int TW>
+__global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Psi out, Bkt bk,
+                                                   const uint32_t* __restrict__ list,
+                                                   const uint32_t* __restrict__ nlist,
+                                                   const uint32_t* __restrict__ ltag, const uint32_t* ctl,
+                                                   Tally tal, double* acc, uint32_t* flags) {
     if (ctl[C_DONE]) return;
     __shared__ float s_rows[LY::NL * LY::LB];
-    __shared__ FastLds s_fast;
+    __shared__ FastLds s_fast[TW];
+    __shared__ uint32_t s_q[QCAP];
+    __shared__ float s_qw[QCAP];
+    __shared__ uint32_t s_bid[KB], s_ref[KB], s_cnt[KB], s_off[KB];
+    __shared__ uint64_t s_mask[TW];
+    __shared__ uint32_t s_round[2];
     extern __shared__ float s_tab[];
     Tabs T = Tg;
     {
@@ -954,587 +1508,194 @@ __global__ __launch_bounds__(64) void k_transport(Tabs Tg, Tp P, Psi in, Psi out
         T.lE = lE;
     }
     const uint32_t tag = 2 * ctl[C_EP] + (FINE ? 1u : 0u);
-    const int l = lane_id();
+    const int l = lane_id(), wv = (int)(threadIdx.x >> 6);
+    const uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
     const uint32_t n = *nlist;
-    const int Nx = T.Nx, NT = T.NT, NE = T.NE;
-    const float dx = T.dx, dz = T.dz, hdx = dx * 0.5f, hdz = dz * 0.5f;
-    const float tmin = T.te[0], tmax = T.te[NT];
-    const float dth = (tmax - tmin) / NT;
+    CellGeo G;
+    G.Nx = T.Nx;
+    G.NT = T.NT;
+    G.NE = T.NE;
+    G.dx = T.dx;
+    G.dz = T.dz;
+    G.hdx = T.dx * 0.5f;
+    G.hdz = T.dz * 0.5f;
+    G.tmin = T.te[0];
+    G.dth = (T.te[T.NT] - G.tmin) / T.NT;
     for (uint32_t li = blockIdx.x; li < n; li += gridDim.x) {
         const int cell = (int)list[li];
-        const int sp = cell / T.Ns, lc = cell - sp * T.Ns;  // spot, cell within the spot
-        const int iz = lc / Nx, ix = lc - iz * Nx;
-        const float rh = T.rho[iz];
-        const float X0 = X0W * T.x0r[iz];
-        WTabs<LY> W;
-        W.n0 = out.ns[cell];
-        W.own0 = W.n0;
-        W.b0 = (l < 32 && (uint32_t)l < W.n0) ? out.bid[(size_t)cell * KB + l] : EMPTY;
-        W.r0 = (l < 32 && (uint32_t)l < W.n0) ? out.sref[(size_t)cell * KB + l] : 0u;
-        W.b1 = W.b2 = W.b3 = W.b4 = EMPTY;
-        W.r1 = W.r2 = W.r3 = W.r4 = 0u;
-        W.n1 = W.n2 = W.n3 = W.n4 = 0u;
-        W.used = 0;
-        W.rows = s_rows;
-        W.orig = 0u;
-        // psi_out slots that already hold weight (the coarse gather into a fine cell) are
-        // staged in LDS rows as well and written back at the end of the cell
-        for (uint32_t s = 0; s < W.own0 && W.used < (uint32_t)LY::NL; ++s) {
-            const float* src = out.pool + (size_t)rlu(W.r0, (int)s) * LY::LB;
-            float* row = W.rows + (size_t)W.used * LY::LB;
-#pragma unroll
-            for (int k = 0; k < LY::PL; ++k) row[l + 64 * k] = src[l + 64 * k];
-            if (l == (int)s) {
-                W.orig = W.r0;
-                W.r0 = CACHED | W.used;
-            }
-            W.used = W.used + 1;
+        const int sp = cell / T.Ns;
+        G.lc = cell - sp * T.Ns;  // cell within the spot
+        G.iz = G.lc / G.Nx;
+        G.ix = G.lc - G.iz * G.Nx;
+        G.rh = T.rho[G.iz];
+        G.X0 = X0W * T.x0r[G.iz];
+        G.A = T.feA[sp * T.Nz + G.iz];
+        G.B = T.feB[sp * T.Nz + G.iz];
+        G.sx0 = T.sx0[sp];
+        {   // lateral +z neighbours listed in this phase (their buckets cannot take foreign emissions)
+            const int off = l - 5, it = G.ix + off;
+            const bool q = l < 11 && off != 0 && it >= 0 && it < G.Nx && ltag[cell + off] == tag;
+            G.latlisted = (uint32_t)__ballot(q);
         }
-        double accd = 0.0;  // lane a < NKACC holds accumulator a of the cell
-        float accf = 0.0f;
-        // lateral +z neighbours listed in this phase (their buckets cannot take foreign emissions)
-        uint32_t latlisted;
-        {
-            const int off = l - 5, it = ix + off;
-            const bool q = l < 11 && off != 0 && it >= 0 && it < Nx && ltag[cell + off] == tag;
-            latlisted = (uint32_t)__ballot(q);
-        }
-        CellAccum a = {};
         const uint32_t ns = in.ns[cell];
-        const float A = T.feA[sp * T.Nz + iz], B = T.feB[sp * T.Nz + iz];
-        const float sx0 = T.sx0[sp];
-        for (uint32_t slot = 0; slot < ns; ++slot) {
-            const uint32_t bid = in.bid[(size_t)cell * KB + slot];
-            if (bid == EMPTY) continue;
-            const uint32_t bT = bid & 0xFFF, bEb = (bid >> 12) & 0xFFF;
-            const float* vin = in.pool + (size_t)in.sref[(size_t)cell * KB + slot] * LY::LB;
-            float v[LY::PL];
+        if (threadIdx.x < ns) {
+            s_bid[threadIdx.x] = in.bid[(size_t)cell * KB + threadIdx.x];
+            s_ref[threadIdx.x] = in.sref[(size_t)cell * KB + threadIdx.x];
+        }
+        // wave 0 owns the tables and the accumulators of the cell
+        WTabs<LY> W;
+        uint32_t comps = 0;  // components of the cell (wave 0)
+        double accd = 0.0;
+        float accf = 0.0f;
+        if (wv == 0) {
+            W.n0 = out.ns[cell];
+            W.own0 = W.n0;
+            W.b0 = (l < 32 && (uint32_t)l < W.n0) ? out.bid[(size_t)cell * KB + l] : EMPTY;
+            W.r0 = (l < 32 && (uint32_t)l < W.n0) ? out.sref[(size_t)cell * KB + l] : 0u;
+            W.b1 = W.b2 = W.b3 = W.b4 = EMPTY;
+            W.r1 = W.r2 = W.r3 = W.r4 = 0u;
+            W.n1 = W.n2 = W.n3 = W.n4 = 0u;
+            W.used = 0;
+            W.cap = (uint32_t)LY::NL;
+            W.rows = s_rows;
+            W.orig = 0u;
+            // psi_out slots that already hold weight (the coarse gather into a fine cell) are
+            // staged in LDS rows as well and written back at the end of the cell
+            for (uint32_t s = 0; s < W.own0 && W.used < (uint32_t)LY::NL; ++s) {
+                const float* src = out.pool + (size_t)rlu(W.r0, (int)s) * LY::LB;
+                float* row = W.rows + (size_t)W.used * LY::LB;
 #pragma unroll
-            for (int k = 0; k < LY::PL; ++k) v[k] = vin[l + 64 * k];
-#pragma unroll
-            for (int k = 0; k < LY::PL; ++k) {
-                uint64_t m = __ballot(v[k] > 0.0f);
-                if (!m) continue;
-                // ---------------- physics, one bin per lane ----------------
-                Comp r;
-                r.kind = 2;
-                r.face = -1;
-                r.lidx = l + 64 * k;
-                r.w = v[k];
-                r.edep = r.w_rem = r.E_rem = r.w_new = r.E_new = 0.0f;
-                r.th_new = r.x_new = r.z_new = 0.0f;
-                r.wcf = r.wl = r.wr = 0.0f;
-#pragma unroll
-                for (int q = 0; q < 11; ++q) r.sw[q] = 0.0f;
-                if (v[k] > 0.0f) {
-                    const int lidx = r.lidx;
-                    const float weight = v[k];
-                    int tbin = (int)bT * LY::TL + LY::lt(lidx);
-                    int ebin = (int)bEb * LY::EL + LY::le(lidx);
-                    if (FINE) {
-                        tbin = max(0, min(tbin, NT - 1));
-                        ebin = max(0, min(ebin, NE - 1));
-                    }
-                    const float E = ecenter(T.ee, ebin);
-                    r.E = E;
-                    const bool pruned = FINE ? (weight < 1e-15f) : (weight < 1e-12f);
-                    if (pruned) {
-                        r.kind = 0;
-                    } else if (E <= 0.1f) {
-                        r.kind = 1;
-                    } else {
-                        const float theta = FINE ? T.te[tbin] + 0.5f * dth : tmin + (tbin + 0.5f) * dth;
-                        const float xc = dx * ((float)(2 * LY::lx(lidx) + 1 - LY::XS) / (float)(2 * LY::XS));
-                        const float zc = dz * ((float)(2 * LY::lz(lidx) + 1 - LY::ZS) / (float)(2 * LY::ZS));
-                        const float mu = cosf_(theta), eta = sinf_(theta);
-                        float ds, dE, E_new, edep, w_new, w_rem, E_rem, sx, sxt, sth;
-                        unsigned pseed;
-                        if (FINE) {
-                            float s1 = (mu > 0) ? (hdz - zc) / mu : 1e30f;
-                            float s2 = (mu < 0) ? (-hdz - zc) / mu : 1e30f;
-                            float s3 = (eta > 0) ? (hdx - xc) / eta : 1e30f;
-                            float s4 = (eta < 0) ? (-hdx - xc) / eta : 1e30f;
-                            float sb = fmaxf(fminf(fminf(s1, s2), fminf(s3, s4)), 0.0f);
-                            ds = fminf(max_step(T, E, rh, P.rule), sb * 1.001f);
-                            float mdE = deposition(T, E, ds, rh);
-                            if (P.strag) {
-                                unsigned sd = (unsigned)((lc * 7 + (int)slot * 13 + lidx * 17) ^ 0x5DEECE66DL);
-                                float sE = straggling(E, ds, rh);
-                                dE = (sE <= 0.0f) ? mdE : fmaxf(mdE + gauss(sd) * sE, 0.0f);
-                            } else {
-                                dE = mdE;
-                            }
-                            dE = fminf(dE, E);
-                            E_new = E - dE;
-                            edep = dE * weight;
-                            if (P.nuc) {
-                                float surv = expf_(-(nuc_sigma(E) * rh) * ds);
-                                w_new = weight * surv;
-                                w_rem = weight - w_new;
-                                E_rem = w_rem * E;
-                            } else {
-                                w_new = weight;
-                                w_rem = 0.0f;
-                                E_rem = 0.0f;
-                            }
-                            float smm = fmaxf(ds, 0.0f);
-                            float Em = fmaxf(E - 0.5f * dE, T.ee[0]);
-                            float Tp_ = scat_power(Em, smm, X0);
-                            sth = sqrtf_(fmaxf(Tp_ * smm, 0.0f));
-                            float dC = 2.0f * B * smm + A * smm * smm + Tp_ * smm * smm * smm / 3.0f;
-                            dC = fmaxf(dC, 0.0f);
-                            sx = fmaxf(sqrtf_(dC), 0.01f);
-                            sxt = 0.001f;
-                            pseed = (unsigned)fminf((float)iz * dz * 1000.0f, 4294967295.0f);
-                        } else {
-                            float cstep = fminf(P.step_coarse, fminf(dx, dz));
-                            float crs = cstep;
-                            bool split = false;
-                            if (E > P.E_on + 1e-6f) {
-                                float Rc = lookup_R(T, E);
-                                float Rf = lookup_R(T, fmaxf(P.E_on, T.ee[0]));
-                                float rtf = (Rc - Rf) / rh;
-                                if (rtf > 0.0f && rtf < crs) {
-                                    crs = rtf;
-                                    cstep = crs;
-                                    split = true;
-                                }
-                            }
-                            dE = fminf(deposition(T, E, crs, rh), E);
-                            E_new = E - dE;
-                            edep = dE * weight;
-                            float surv = expf_(-(nuc_sigma(E) * rh) * crs);
-                            w_new = weight * surv;
-                            w_rem = weight - w_new;
-                            E_rem = w_rem * E;
-                            if (split && E_new > P.E_on) {
-                                float ex = E_new - P.E_on;
-                                if (ex > 0.0f && w_new > 0.0f) edep += ex * w_new;
-                                E_new = P.E_on;
-                            }
-                            float ps = fmaxf(0.0f, (float)iz * dz + (zc + 0.5f * dz));
-                            float smm = fmaxf(crs, 0.0f);
-                            float st0 = (ps > 0.0f) ? highland(E, ps, X0) : 0.0f;
-                            float Ao = st0 * st0;
-                            float Bo = 0.5f * Ao * ps;
-                            float Co = sx0 * sx0 + (Ao * ps * ps) / 3.0f;
-                            float Em = fmaxf(E - 0.5f * dE, T.ee[0]);
-                            float Tp_ = scat_power(Em, smm, X0);
-                            float An = fmaxf(Ao + Tp_ * smm, 0.0f);
-                            float Cn = fmaxf(Co + 2.0f * Bo * smm + Ao * smm * smm +
-                                                 (1.0f / 3.0f) * Tp_ * smm * smm * smm,
-                                             0.0f);
-                            sth = sqrtf_(fmaxf(An - Ao, 0.0f));
-                            float dC = fmaxf(Cn - Co, 0.0f);
-                            float unres = fminf(0.35f, 0.04f + 0.003f * ps);
-                            sx = fmaxf(sqrtf_(fmaxf(dC + unres * Co, 0.0f)), 0.01f);
-                            sxt = fmaxf(sinf_(fminf(sth, 1.57f)) * smm / 1.7320508f, 0.01f);
-                            pseed = (unsigned)fminf(ps * 1000.0f, 4294967295.0f);
-                            ds = cstep;
-                        }
-                        unsigned ts = ((unsigned)(lc + 1) * 2654435761u) ^
-                                      ((unsigned)(slot + 1) * 2246822519u) ^
-                                      ((unsigned)(lidx + 1) * 3266489917u) ^ (unsigned)(ebin + 17) ^
-                                      (pseed * 747796405u);
-                        float tsc = (sth <= 0.0f) ? 0.0f : gauss(ts) * sth;
-                        float thn = fmaxf(T.te[0], fminf(theta + tsc, T.te[NT]));
-                        float mun = cosf_(thn), etn = sinf_(thn);
-                        float xn = xc + etn * ds, zn = zc + mun * ds;
-                        int face = -1;
-                        if (zn >= hdz - 0.001f) face = 0;
-                        else if (zn <= -hdz + 0.001f) face = 1;
-                        else if (xn >= hdx - 0.001f) face = 2;
-                        else if (xn <= -hdx + 0.001f) face = 3;
-                        xn = fmaxf(-hdx, fminf(xn, hdx));
-                        zn = fmaxf(-hdz, fminf(zn, hdz));
-                        r.edep = edep;
-                        r.w_rem = w_rem;
-                        r.E_rem = E_rem;
-                        r.w_new = w_new;
-                        r.E_new = E_new;
-                        r.face = face;
-                        r.th_new = thn;
-                        r.x_new = xn;
-                        r.z_new = zn;
-                        if (E_new > 0.1f) {
-                            if (face == 0) {
-                                // device_emit_lateral_spread weights (device_bucket.cuh:880-916)
-                                float s = fmaxf(sxt, 1e-6f);
-                                float xmin = xn - 0.5f * 11.0f * dx;
-                                float prev = gcdf(xmin, xn, s), sum = 0.0f;
-#pragma unroll
-                                for (int q = 0; q < 11; ++q) {
-                                    float cur = gcdf(xmin + (q + 1) * dx, xn, s);
-                                    r.sw[q] = cur - prev;
-                                    prev = cur;
-                                }
-#pragma unroll
-                                for (int q = 0; q < 11; ++q) sum += r.sw[q];
-                                if (sum > 1e-10f) {
-#pragma unroll
-                                    for (int q = 0; q < 11; ++q) r.sw[q] /= sum;
-                                } else {
-#pragma unroll
-                                    for (int q = 0; q < 11; ++q) r.sw[q] = (q == 5) ? 1.0f : 0.0f;
-                                }
-                                sum = 0.0f;
-#pragma unroll
-                                for (int q = 0; q < 11; ++q) {
-                                    r.sw[q] = fmaxf(0.0f, r.sw[q]);
-                                    sum += r.sw[q];
-                                }
-                                if (sum < 1e-12f) {
-#pragma unroll
-                                    for (int q = 0; q < 11; ++q) r.sw[q] = (q == 5) ? 1.0f : 0.0f;
-                                } else {
-#pragma unroll
-                                    for (int q = 0; q < 11; ++q) r.sw[q] /= sum;
-                                }
-                            } else if (face < 0) {
-                                // device_gaussian_spread_weights_subcell + in-cell fraction and
-                                // tails (k3_finetransport.cu:585-653)
-                                float s = fmaxf(FINE ? sxt : sx, 1e-6f);
-                                float dxs = dx / LY::XS;
-                                float xmin = -dx * 0.5f;
-                                float prev = gcdf(xmin, xn, s), sum = 0.0f;
-                                float w8[LY::XS];
-#pragma unroll
-                                for (int q = 0; q < LY::XS; ++q) {
-                                    float cur = gcdf(xmin + (q + 1) * dxs, xn, s);
-                                    w8[q] = cur - prev;
-                                    prev = cur;
-                                }
-#pragma unroll
-                                for (int q = 0; q < LY::XS; ++q) sum += w8[q];
-                                if (sum > 1e-10f) {
-#pragma unroll
-                                    for (int q = 0; q < LY::XS; ++q) w8[q] /= sum;
-                                } else {
-#pragma unroll
-                                    for (int q = 0; q < LY::XS; ++q) w8[q] = (q == LY::XS / 2) ? 1.0f : 0.0f;
-                                }
-                                float win = gcdf(hdx, xn, sx) - gcdf(-hdx, xn, sx);
-                                float wcf = fminf(1.0f, fmaxf(0.0f, win));
-                                float wss = 0.0f;
-#pragma unroll
-                                for (int q = 0; q < LY::XS; ++q) {
-                                    w8[q] = fmaxf(0.0f, w8[q]);
-                                    wss += w8[q];
-                                }
-                                if (wss < 1e-12f) {
-#pragma unroll
-                                    for (int q = 0; q < LY::XS; ++q) w8[q] = (q == LY::XS / 2) ? 1.0f : 0.0f;
-                                    wss = 1.0f;
-                                }
-#pragma unroll
-                                for (int q = 0; q < LY::XS; ++q) r.sw[q] = (w8[q] / wss) * wcf;
-                                float wl = 0.0f, wr = 0.0f;
-                                if (wcf < 1.0f - 1e-6f) {
-                                    float wlr = fmaxf(0.0f, gcdf(-hdx, xn, sx));
-                                    float wrr = fmaxf(0.0f, 1.0f - gcdf(hdx, xn, sx));
-                                    float wtt = fmaxf(0.0f, 1.0f - wcf);
-                                    float wts = wlr + wrr;
-                                    if (wtt > 0.0f && wts > 1e-12f) {
-                                        float sc = wtt / wts;
-                                        wl = wlr * sc;
-                                        wr = wrr * sc;
-                                    }
-                                }
-                                r.wcf = wcf;
-                                r.wl = wl;
-                                r.wr = wr;
-                            }
-                        }
-                    }
+                for (int k = 0; k < LY::PL; ++k) row[l + 64 * k] = src[l + 64 * k];
+                if (l == (int)s) {
+                    W.orig = W.r0;
+                    W.r0 = CACHED | W.used;
                 }
-                // ------- emission binning and the table-independent parts of the apply -------
-                r.bn = Bins{0, 0, 0, 0.0f, 0.0f, 0.0f};
-                r.i0 = r.i1 = r.i2 = 0;
-                r.g0 = r.g1 = r.g2 = 0.0f;
-                if (v[k] > 0.0f && r.kind == 2 && r.E_new > 0.1f) {
-                    r.bn = emission_bins(T, r.th_new, r.E_new);
-                    const float w_new = r.w_new;
-                    if (r.face >= 0) {
-                        r.g0 = ecenter(T.ee, find_bin(T.ee, NE, r.E_new));
-                        if (r.face == 0) {
-                            float drop = 0.0f, lat_bnd = 0.0f;
-                            uint32_t unsup = 0;
-#pragma unroll
-                            for (int q = 0; q < 11; ++q) {
-                                const float wf = r.sw[q];
-                                if (wf <= 0.0f) continue;
-                                const int it = ix + q - 5;
-                                if (it < 0 || it >= Nx) {
-                                    lat_bnd += w_new * wf;
-                                    continue;
-                                }
-                                if (q == 5) {
-                                    drop += 0.0f;  // the centre emission's drop, zero unless a pool fills
-                                } else {
-                                    // another cell's +z bucket: a drop when that cell is not listed
-                                    // in this phase (device_bucket.cuh:953-963); a listed target
-                                    // would need a cross-wave ordered write, which this path rejects
-                                    if ((latlisted >> q) & 1u) ++unsup;
-                                    drop += w_new * wf;
-                                }
-                            }
-                            if (unsup) atomicAdd(&flags[F_UNSUPPORTED], unsup);
-                            r.g1 = drop;
-                            r.g2 = lat_bnd;
-                        } else {
-                            const int face = r.face;
-                            const float xon = (face == 2) ? -dx * 0.5f : (face == 3 ? dx * 0.5f : r.x_new);
-                            r.i0 = min(max((int)(((xon / dx) + 0.5f) * LY::XS), 0), LY::XS - 1);
-                            if (face == 1) r.i1 = LY::ZS - 1;
-                            else r.i1 = min(max((int)(((r.z_new / dz) + 0.5f) * LY::ZS), 0), LY::ZS - 1);
-                        }
-                    } else {
-                        int tbn = (int)((r.th_new - tmin) / dth);
-                        tbn = (int)fmaxf(0, fminf(tbn, NT - 1));
-                        const int ebn = find_bin(T.ee, NE, r.E_new);
-                        r.i0 = tbn;
-                        r.i1 = ebn;
-                        r.i2 = min(max((int)(((r.z_new / dz) + 0.5f) * LY::ZS), 0), LY::ZS - 1);
-                        r.g0 = ecenter(T.ee, ebn);
-                    }
-                }
-                // ---------------- ordered apply, bin order ----------------
-                {
-                    const LaneOut o = fast_prepare<LY, FINE>(s_fast, r, v[k] > 0.0f, ix, iz, Nx, T.Nz);
-                    if (fast_claims<LY>(s_fast, W, o, out, bk, flags)) {
-                        fast_steps<LY>(s_fast, W, m, o.nrec, accd, accf);
-                        a.comps += (uint32_t)__popcll(m);
-                        m = 0;
-                    } else {
-                        lanes_to_accum(a, accd, accf);
-                    }
-                }
-                const bool serial = m != 0;
-                while (m) {
-                    const int j = __ffsll((unsigned long long)m) - 1;
-                    m &= m - 1;
-                    const int kind = rli(r.kind, j);
-                    const float w = rlf(r.w, j), E = rlf(r.E, j);
-                    a.comps++;
-                    a.cl_win += (double)w;
-                    a.cl_Ein += (double)w * (double)E;
-                    if (kind == 0) {
-                        a.drop_w += (double)w;
-                        a.drop_E += (double)(E * w);
-                        a.cl_wout += (double)w;
-                        a.cl_Eout += (double)(E * w);
-                        continue;
-                    }
-                    if (kind == 1) {
-                        a.w_cut += w;
-                        a.E_cut += E * w;
-                        a.cl_wout += (double)w;
-                        a.cl_Eout += (double)(E * w);
-                        continue;
-                    }
-                    const float edep = rlf(r.edep, j), w_rem = rlf(r.w_rem, j), E_rem = rlf(r.E_rem, j);
-                    const float w_new = rlf(r.w_new, j), E_new = rlf(r.E_new, j);
-                    const int face = rli(r.face, j);
-                    a.edep += edep;
-                    a.w_nuc += w_rem;
-                    a.E_nuc += E_rem;
-                    a.cl_wout += (double)w_rem;
-                    a.cl_Eout += (double)edep + (double)E_rem;
-                    if (E_new <= 0.1f) {
-                        a.w_cut += w_new;
-                        a.E_cut += E_new * w_new;
-                        a.cl_wout += (double)w_new;
-                        a.cl_Eout += (double)(E_new * w_new);
-                        continue;
-                    }
-                    Bins bn;
-                    bn.bc = rli(r.bn.bc, j);
-                    bn.tb1 = rli(r.bn.tb1, j);
-                    bn.eb = rli(r.bn.eb, j);
-                    bn.tw0 = rlf(r.bn.tw0, j);
-                    bn.tw1 = rlf(r.bn.tw1, j);
-                    bn.Ecen = rlf(r.bn.Ecen, j);
-                    if (face >= 0) {
-                        bool nb = (face == 0) ? (iz + 1 < T.Nz) : (face == 1) ? (iz > 0)
-                                : (face == 2) ? (ix + 1 < Nx) : (ix > 0);
-                        if (!nb) {
-                            a.w_bnd += w_new;
-                            a.E_bnd += E_new * w_new;
-                            a.cl_wout += (double)w_new;
-                            a.cl_Eout += (double)(E_new * w_new);
-                            continue;
-                        }
-                        const float E_emit = rlf(r.g0, j);
-                        float drop = 0.0f;
-                        if (face == 0) {
-                            const float wf5 = rlf(r.sw[5], j);
-                            const float lat_bnd = rlf(r.g2, j);
-                            float d5 = 0.0f;
-                            if (wf5 > 0.0f) d5 = emit<LY>(W, a, T, bk, flags, 0, bn, w_new * wf5, LY::XS / 2, 0);
-                            if (d5 == 0.0f) {
-                                drop = rlf(r.g1, j);
-                            } else {  // the lateral sum with the centre's drop in its place (q = 5)
-                                for (int q = 0; q < 11; ++q) {
-                                    const float wf = rlf(r.sw[q], j);
-                                    if (wf <= 0.0f) continue;
-                                    const int it = ix + q - 5;
-                                    if (it < 0 || it >= Nx) continue;
-                                    drop += (q == 5) ? d5 : w_new * wf;
-                                }
-                            }
-                            if (lat_bnd > 0.0f) {
-                                a.w_bnd += lat_bnd;
-                                a.E_bnd += E_new * lat_bnd;
-                                a.cl_wout += (double)lat_bnd;
-                                a.cl_Eout += (double)(E_new * lat_bnd);
-                            }
-                            if (drop > 0.0f) {
-                                a.drop_w += (double)drop;
-                                a.drop_E += (double)(E_new * drop);
-                                a.cl_wout += (double)drop;
-                                a.cl_Eout += (double)(E_new * drop);
-                            }
-                            if (FINE) {
-                                float tw = fmaxf(0.0f, w_new - lat_bnd - drop);
-                                if (tw > 0.0f) {
-                                    a.edep += (double)((E_new - E_emit) * tw);
-                                    a.cl_Eout += (double)((E_new - E_emit) * tw);
-                                }
-                            }
-                        } else {
-                            const int xsn = rli(r.i0, j), zsn = rli(r.i1, j);
-                            drop = emit<LY>(W, a, T, bk, flags, face, bn, w_new, xsn, zsn);
-                            if (drop > 0.0f) {
-                                a.drop_w += (double)drop;
-                                a.drop_E += (double)(E_new * drop);
-                                a.cl_wout += (double)drop;
-                                a.cl_Eout += (double)(E_new * drop);
-                            }
-                            if (FINE) {
-                                float tw = fmaxf(0.0f, w_new - drop);
-                                if (tw > 0.0f) {
-                                    a.edep += (double)((E_new - E_emit) * tw);
-                                    a.cl_Eout += (double)((E_new - E_emit) * tw);
-                                }
-                            }
-                        }
-                        continue;
-                    }
-                    // ---- remain in the cell ----
-                    const int tbn = rli(r.i0, j), ebn = rli(r.i1, j), zst = rli(r.i2, j);
-                    const uint32_t bidn = ((uint32_t)(ebn / LY::EL) << 12) | ((uint32_t)(tbn / LY::TL) & 0xFFF);
-                    uint32_t sel;
-                    const int os = W.claim(0, bidn, sel, out.pool, out.pcount, out.pcap, flags, F_PSI_FULL);
-                    if (os < 0) {
-                        a.drop_w += (double)w_new;
-                        a.drop_E += (double)(E_new * w_new);
-                        a.cl_wout += (double)w_new;
-                        a.cl_Eout += (double)(E_new * w_new);
-                        continue;
-                    }
-                    int tbe = tbn, ebe = ebn;
-                    float E_emit = rlf(r.g0, j);
-                    if (sel != bidn) {
-                        int bte = (int)(sel & 0xFFF), bee = (int)((sel >> 12) & 0xFFF);
-                        tbe = max(bte * LY::TL, min(tbe, bte * LY::TL + LY::TL - 1));
-                        ebe = max(bee * LY::EL, min(ebe, bee * LY::EL + LY::EL - 1));
-                        E_emit = T.ee[ebe] + 0.5f * (T.ee[ebe + 1] - T.ee[ebe]);
-                    }
-                    const uint32_t oref = rlu(W.r0, os);
-                    float mywf = 0.0f;
-#pragma unroll
-                    for (int q = 0; q < LY::XS; ++q) {
-                        float wf = rlf(r.sw[q], j);
-                        if (l == q) mywf = wf;
-                        if (wf > 0.0f) {
-                            float wsp = w_new * wf;
-                            a.cl_wout += (double)wsp;
-                            a.cl_Eout += (double)wsp * (double)E_emit;
-                        }
-                    }
-                    if (l < LY::XS && mywf > 0.0f)
-                        W.add(oref, LY::enc(tbe % LY::TL, ebe % LY::EL, l, zst), w_new * mywf, out.pool);
-                    const float wcf = rlf(r.wcf, j), wl = rlf(r.wl, j), wr = rlf(r.wr, j);
-                    if (wcf < 1.0f - 1e-6f) {
-                        const int zsb = zst;
-                        if (wl > 0.0f) {
-                            float wsl = w_new * wl;
-                            if (ix > 0) {
-                                float d = emit<LY>(W, a, T, bk, flags, 3, bn, wsl, LY::XS - 1, zsb);
-                                if (d > 0.0f) {
-                                    a.drop_w += (double)d;
-                                    a.drop_E += (double)(E_new * d);
-                                    a.cl_wout += (double)d;
-                                    a.cl_Eout += (double)(E_new * d);
-                                }
-                            } else {
-                                a.w_bnd += wsl;
-                                a.E_bnd += E_new * wsl;
-                                a.cl_wout += (double)wsl;
-                                a.cl_Eout += (double)(E_new * wsl);
-                            }
-                        }
-                        if (wr > 0.0f) {
-                            float wsr = w_new * wr;
-                            if (ix < Nx - 1) {
-                                float d = emit<LY>(W, a, T, bk, flags, 2, bn, wsr, 0, zsb);
-                                if (d > 0.0f) {
-                                    a.drop_w += (double)d;
-                                    a.drop_E += (double)(E_new * d);
-                                    a.cl_wout += (double)d;
-                                    a.cl_Eout += (double)(E_new * d);
-                                }
-                            } else {
-                                a.w_bnd += wsr;
-                                a.E_bnd += E_new * wsr;
-                                a.cl_wout += (double)wsr;
-                                a.cl_Eout += (double)(E_new * wsr);
-                            }
-                        }
-                    }
-                    if (FINE) {
-                        float rf = wcf;
-                        if (ix > 0) rf += wl;
-                        if (ix < Nx - 1) rf += wr;
-                        rf = fminf(1.0f, fmaxf(0.0f, rf));
-                        float rw = w_new * rf;
-                        if (rw > 0.0f) {
-                            a.edep += (double)((E_new - E_emit) * rw);
-                            a.cl_Eout += (double)((E_new - E_emit) * rw);
-                        }
-                    }
-                }
-                if (serial) accum_to_lanes(a, accd, accf);
+                W.used = W.used + 1;
             }
+        }
+        __syncthreads();
+        uint32_t s0 = 0;
+        while (s0 < ns) {
+            // non-zero components per slot
+            for (uint32_t s = s0 + wv; s < ns; s += TW) {
+                uint32_t cnt = 0;
+                if (s_bid[s] != EMPTY) {
+                    const float* vin = in.pool + (size_t)s_ref[s] * LY::LB;
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k) cnt += (uint32_t)__popcll(__ballot(vin[l + 64 * k] > 0.0f));
+                }
+                if (l == 0) s_cnt[s] = cnt;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {  // the slots of this round: as many as fit the queue
+                uint32_t off = 0, s1 = s0;
+                while (s1 < ns && off + s_cnt[s1] <= (uint32_t)QCAP) {
+                    s_off[s1] = off;
+                    off += s_cnt[s1];
+                    ++s1;
+                }
+                s_round[0] = s1;
+                s_round[1] = off;
+            }
+            __syncthreads();
+            const uint32_t s1 = s_round[0], total = s_round[1];
+            for (uint32_t s = s0 + wv; s < s1; s += TW) {
+                if (s_bid[s] == EMPTY) continue;
+                const float* vin = in.pool + (size_t)s_ref[s] * LY::LB;
+                uint32_t pos = s_off[s];
+#pragma unroll
+                for (int k = 0; k < LY::PL; ++k) {
+                    const float w = vin[l + 64 * k];
+                    const uint64_t mk = __ballot(w > 0.0f);
+                    if (w > 0.0f) {
+                        const uint32_t p = pos + (uint32_t)__popcll(mk & below);
+                        s_q[p] = (s << 16) | (uint32_t)(l + 64 * k);
+                        s_qw[p] = w;
+                    }
+                    pos += (uint32_t)__popcll(mk);
+                }
+            }
+            __syncthreads();
+            const uint32_t nb = (total + 63) / 64;
+            for (uint32_t b0 = 0; b0 < nb; b0 += TW) {
+                const uint32_t b = b0 + (uint32_t)wv;
+                if (b < nb) {  // physics of batch b by wave wv
+                    const uint32_t i = b * 64 + (uint32_t)l;
+                    const bool live = i < total;
+                    const uint32_t it = live ? s_q[i] : 0u;
+                    const uint32_t slot = it >> 16, bid = s_bid[slot];
+                    const Comp r = transport_component<LY, FINE>(T, P, G, live, live ? s_qw[i] : 0.0f,
+                                                                 (int)(it & 0xFFFFu), slot, bid & 0xFFF,
+                                                                 (bid >> 12) & 0xFFF, flags, true);
+                    const LaneOut o = fast_prepare<LY, FINE>(s_fast[wv], r, live, G.ix, G.iz, G.Nx, T.Nz);
+                    s_fast[wv].nkey[l] = o.nkey;
+                    s_fast[wv].nrec[l] = o.nrec;
+                    const uint64_t m = __ballot(live);
+                    if (l == 0) s_mask[wv] = m;
+                }
+                __syncthreads();
+                if (wv == 0) {  // ordered apply of the batches, in order
+                    for (uint32_t bb = b0; bb < nb && bb < b0 + TW; ++bb) {
+                        FastLds& F = s_fast[bb - b0];
+                        const uint64_t m = s_mask[bb - b0];
+                        if (fast_claims<LY>(F, W, out, bk, flags)) {
+                            fast_steps<LY>(F, W, m, accd, accf);
+                            comps += (uint32_t)__popcll(m);
+                        } else {
+                            // the batch again, in wave 0, for the serial form
+                            const uint32_t i = bb * 64 + (uint32_t)l;
+                            const bool live = i < total;
+                            const uint32_t it = live ? s_q[i] : 0u;
+                            const uint32_t slot = it >> 16, bid = s_bid[slot];
+                            const Comp r = transport_component<LY, FINE>(T, P, G, live, live ? s_qw[i] : 0.0f,
+                                                                         (int)(it & 0xFFFFu), slot, bid & 0xFFF,
+                                                                         (bid >> 12) & 0xFFF, flags, false);
+                            CellAccum a = {};
+                            lanes_to_accum(a, accd, accf);
+                            a.comps = comps;
+                            serial_apply<LY, FINE>(r, m, W, a, T, out, bk, flags, G);
+                            accum_to_lanes(a, accd, accf);
+                            comps = a.comps;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            s0 = s1;
         }
         // ---- end of cell: tables and payloads, tallies, per-cell accumulators ----
-        lanes_to_accum(a, accd, accf);
-        flush_tables<LY>(W, out, bk, cell, li, flags);
-        if (l == 0) {
-            tal.edep[cell] += a.edep;
-            tal.w_cut[cell] += a.w_cut;
-            tal.E_cut[cell] += a.E_cut;
-            tal.w_nuc[cell] += a.w_nuc;
-            tal.E_nuc[cell] += a.E_nuc;
-            tal.w_bnd[cell] += a.w_bnd;
-            tal.E_bnd[cell] += a.E_bnd;
-            double* o = acc + (size_t)li * NACC;
-            o[A_EDEP] = a.edep;
-            o[A_WCUT] = a.w_cut;
-            o[A_ECUT] = a.E_cut;
-            o[A_WNUC] = a.w_nuc;
-            o[A_ENUC] = a.E_nuc;
-            o[A_WBND] = a.w_bnd;
-            o[A_EBND] = a.E_bnd;
-            o[A_DROPW] = a.drop_w;
-            o[A_DROPE] = a.drop_E;
-            o[A_CLWIN] = a.cl_win;
-            o[A_CLWOUT] = a.cl_wout;
-            o[A_CLEIN] = a.cl_Ein;
-            o[A_CLEOUT] = a.cl_Eout;
-            o[A_COMPS] = (double)a.comps;
-            o[A_UPD] = (double)((size_t)ns * LY::LB);
+        if (wv == 0) {
+            CellAccum a = {};
+            lanes_to_accum(a, accd, accf);
+            a.comps = comps;
+            flush_tables<LY>(W, out, bk, cell, li, flags);
+            if (l == 0) {
+                tal.edep[cell] += a.edep;
+                tal.w_cut[cell] += a.w_cut;
+                tal.E_cut[cell] += a.E_cut;
+                tal.w_nuc[cell] += a.w_nuc;
+                tal.E_nuc[cell] += a.E_nuc;
+                tal.w_bnd[cell] += a.w_bnd;
+                tal.E_bnd[cell] += a.E_bnd;
+                double* o = acc + (size_t)li * NACC;
+                o[A_EDEP] = a.edep;
+                o[A_WCUT] = a.w_cut;
+                o[A_ECUT] = a.E_cut;
+                o[A_WNUC] = a.w_nuc;
+                o[A_ENUC] = a.E_nuc;
+                o[A_WBND] = a.w_bnd;
+                o[A_EBND] = a.E_bnd;
+                o[A_DROPW] = a.drop_w;
+                o[A_DROPE] = a.drop_E;
+                o[A_CLWIN] = a.cl_win;
+                o[A_CLWOUT] = a.cl_wout;
+                o[A_CLEIN] = a.cl_Ein;
+                o[A_CLEOUT] = a.cl_Eout;
+                o[A_COMPS] = (double)a.comps;
+                o[A_UPD] = (double)((size_t)ns * LY::LB);
+            }
         }
+        __syncthreads();
     }
 }
 
@@ -1568,49 +1729,104 @@ __global__ void k_dest(const uint32_t* list, const uint32_t* nlist, int Nx, int 
 }
 
 // K4: gather the neighbours' buckets into psi_out of one destination (faces 0..3, bucket slots
-// in order), one wavefront per destination.
+// in order), one wavefront per destination.  The destination's payloads are staged in LDS rows
+// (existing ones loaded, new ones zeroed), the adds happen in LDS in the serial order, and the
+// new payloads get their pool slots with one reservation when the destination is done.  The
+// next bucket payload is fetched while the current one is added.
 template <class LY>
-__global__ __launch_bounds__(256) void k_gather(Tabs T, Psi out, Bkt bk, const uint32_t* __restrict__ dlist,
-                                                const uint32_t* __restrict__ ndest,
-                                                const uint32_t* __restrict__ ltag,
-                                                const uint32_t* __restrict__ lpos, const uint32_t* ctl, int fine,
-                                                uint32_t dcap, double* k4drop_w, double* k4drop_E, uint32_t* flags) {
+__global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const uint32_t* __restrict__ dlist,
+                                               const uint32_t* __restrict__ ndest,
+                                               const uint32_t* __restrict__ ltag,
+                                               const uint32_t* __restrict__ lpos, const uint32_t* ctl, int fine,
+                                               uint32_t dcap, double* k4drop_w, double* k4drop_E, uint32_t* flags) {
     if (ctl[C_DONE]) return;
+    constexpr int NLG = 4096 / LY::LB;  // 16 KiB of payload rows
+    __shared__ float s_rows[NLG * LY::LB];
     const uint32_t tag = 2 * ctl[C_EP] + (uint32_t)fine;
     const int l = lane_id();
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+    const uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
     const uint32_t n = min(*ndest, dcap);  // k_dest flags an overflow; the host reports it
     const int Nx = T.Nx, Nz = T.Nz;
-    for (uint32_t i = wave; i < n; i += nw) {
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
         const int d = (int)dlist[i];
         const int lc = d % T.Ns;
         const int ix = lc % Nx, iz = lc / Nx;
-        uint32_t n_out = out.ns[d];
-        uint32_t obid = (l < 32 && (uint32_t)l < n_out) ? out.bid[(size_t)d * KB + l] : EMPTY;
-        uint32_t oref = (l < 32 && (uint32_t)l < n_out) ? out.sref[(size_t)d * KB + l] : 0u;
-        double dw = 0.0, de = 0.0;
+        WTabs<LY> W;
+        W.n0 = out.ns[d];
+        W.own0 = W.n0;
+        W.b0 = (l < 32 && (uint32_t)l < W.n0) ? out.bid[(size_t)d * KB + l] : EMPTY;
+        W.r0 = (l < 32 && (uint32_t)l < W.n0) ? out.sref[(size_t)d * KB + l] : 0u;
+        W.b1 = W.b2 = W.b3 = W.b4 = EMPTY;
+        W.r1 = W.r2 = W.r3 = W.r4 = 0u;
+        W.n1 = W.n2 = W.n3 = W.n4 = 0u;
+        W.used = 0;
+        W.cap = (uint32_t)NLG;
+        W.rows = s_rows;
+        W.orig = 0u;
+        for (uint32_t s = 0; s < W.own0 && W.used < W.cap; ++s) {
+            const float* src = out.pool + (size_t)rlu(W.r0, (int)s) * LY::LB;
+            float* row = W.rows + (size_t)W.used * LY::LB;
+#pragma unroll
+            for (int k = 0; k < LY::PL; ++k) row[l + 64 * k] = src[l + 64 * k];
+            if (l == (int)s) {
+                W.orig = W.r0;
+                W.r0 = CACHED | W.used;
+            }
+            W.used = W.used + 1;
+        }
+        // the four faces' bucket tables (lane s < 32: slot s)
+        uint32_t fb[4], fr[4], nf[4];
+#pragma unroll
         for (int face = 0; face < 4; ++face) {
             int src = -1;
             if (face == 0 && iz > 0) src = d - Nx;
             if (face == 1 && iz + 1 < Nz) src = d + Nx;
             if (face == 2 && ix > 0) src = d - 1;
             if (face == 3 && ix + 1 < Nx) src = d + 1;
-            if (src < 0 || ltag[src] != tag) continue;
-            const uint32_t pos = lpos[src];
-            const uint32_t base = (pos * 4 + face) * KB;  // source face == receiving face index
-            const uint32_t nb = bk.n[pos * 4 + face];
-            for (uint32_t s = 0; s < nb; ++s) {
-                const uint32_t bid = bk.bid[base + s];
-                if (bid == EMPTY) continue;
-                const float* bv = bk.pool + (size_t)bk.sref[base + s] * LY::LB;
-                float v[LY::PL];
+            nf[face] = 0;
+            fb[face] = EMPTY;
+            fr[face] = 0;
+            if (src >= 0 && ltag[src] == tag) {
+                const uint32_t pos = lpos[src];
+                const uint32_t base = (pos * 4 + face) * KB;  // source face == receiving face index
+                nf[face] = bk.n[pos * 4 + face];
+                if (l < 32 && (uint32_t)l < nf[face]) {
+                    fb[face] = bk.bid[base + l];
+                    fr[face] = bk.sref[base + l];
+                }
+            }
+        }
+        const uint32_t ne = nf[0] + nf[1] + nf[2] + nf[3];
+        auto entry = [&](uint32_t e, uint32_t& bid, uint32_t& ref) {
+            int f = 0;
+            while (e >= nf[f]) {
+                e -= nf[f];
+                ++f;
+            }
+            const uint32_t bb = f == 0 ? fb[0] : f == 1 ? fb[1] : f == 2 ? fb[2] : fb[3];
+            const uint32_t rr = f == 0 ? fr[0] : f == 1 ? fr[1] : f == 2 ? fr[2] : fr[3];
+            bid = rlu(bb, (int)e);
+            ref = rlu(rr, (int)e);
+        };
+        double dw = 0.0, de = 0.0;
+        float v[LY::PL], vn[LY::PL];
+        uint32_t bid = EMPTY, bref = 0;
+        if (ne) {
+            entry(0, bid, bref);
 #pragma unroll
-                for (int k = 0; k < LY::PL; ++k) v[k] = bv[l + 64 * k];
+            for (int k = 0; k < LY::PL; ++k) v[k] = bid != EMPTY ? bk.pool[(size_t)bref * LY::LB + l + 64 * k] : 0.0f;
+        }
+        for (uint32_t e = 0; e < ne; ++e) {
+            uint32_t bidn = EMPTY, brefn = 0;
+            if (e + 1 < ne) {
+                entry(e + 1, bidn, brefn);
+#pragma unroll
+                for (int k = 0; k < LY::PL; ++k)
+                    vn[k] = bidn != EMPTY ? bk.pool[(size_t)brefn * LY::LB + l + 64 * k] : 0.0f;
+            }
+            if (bid != EMPTY) {
                 uint32_t sel;
-                int os = table_find_or_claim(obid, oref, n_out, bid, sel, out.pool, out.pcount, out.pcap,
-                                             out.bid + (size_t)d * KB, out.sref + (size_t)d * KB, LY::LB,
-                                             flags, F_PSI_FULL);
+                const int os = W.claim(0, bid, sel, out.pool, out.pcount, out.pcap, flags, F_PSI_FULL);
                 if (os < 0) {
                     double pw = 0.0, pe = 0.0;
 #pragma unroll
@@ -1624,27 +1840,63 @@ __global__ __launch_bounds__(256) void k_gather(Tabs T, Psi out, Bkt bk, const u
                     }
                     dw += wave_sum(pw);
                     de += wave_sum(pe);
-                    continue;
-                }
-                const uint32_t ref = rlu(oref, os);
-                float* dst = out.pool + (size_t)ref * LY::LB;
-                if (sel == bid) {
-#pragma unroll
-                    for (int k = 0; k < LY::PL; ++k)
-                        if (v[k] > 0.0f) dst[l + 64 * k] += v[k];
                 } else {
-                    // closest-block fallback: remapped bins may coincide, add in lidx order
-                    for (int k = 0; k < LY::PL; ++k) {
-                        for (int j = 0; j < 64; ++j) {
-                            float w = rlf(v[k], j);
-                            if (w > 0.0f && l == 0) dst[remap<LY>(j + 64 * k, bid, sel)] += w;
+                    const uint32_t ref = rlu(W.r0, os);
+                    if (sel == bid) {
+#pragma unroll
+                        for (int k = 0; k < LY::PL; ++k)
+                            if (v[k] > 0.0f) W.add(ref, l + 64 * k, v[k], out.pool);
+                    } else {
+                        // closest-block fallback: remapped bins may coincide, add in lidx order
+                        for (int k = 0; k < LY::PL; ++k) {
+                            for (int j = 0; j < 64; ++j) {
+                                float w = rlf(v[k], j);
+                                if (w > 0.0f && l == 0) W.add(ref, remap<LY>(j + 64 * k, bid, sel), w, out.pool);
+                            }
                         }
                     }
                 }
             }
+            bid = bidn;
+            bref = brefn;
+#pragma unroll
+            for (int k = 0; k < LY::PL; ++k) v[k] = vn[k];
+        }
+        // staged payloads back to the pool: existing rows in place, new rows with one reservation
+        const uint64_t mp = __ballot(l < 32 && (uint32_t)l < W.own0 && (W.r0 & CACHED));
+        for (uint64_t m = mp; m; m &= m - 1) {
+            const int s = __ffsll((unsigned long long)m) - 1;
+            const float* src = W.rows + (size_t)(rlu(W.r0, s) & ~CACHED) * LY::LB;
+            float* dst = out.pool + (size_t)rlu(W.orig, s) * LY::LB;
+#pragma unroll
+            for (int k = 0; k < LY::PL; ++k) dst[l + 64 * k] = src[l + 64 * k];
+        }
+        const uint64_t mo = __ballot(l < 32 && (uint32_t)l >= W.own0 && (uint32_t)l < W.n0 && (W.r0 & CACHED));
+        const uint32_t co = (uint32_t)__popcll(mo);
+        uint32_t base = 0;
+        if (co) {
+            if (l == 0) base = atomicAdd(out.pcount, co);
+            base = rlu(base, 0);
+        }
+        const bool ok = !co || base + co <= out.pcap;
+        if (!ok && l == 0) atomicOr(&flags[F_PSI_FULL], 1u);
+        if (ok) {
+            uint32_t ref = W.r0;
+            if ((mo >> l) & 1ull) ref = base + (uint32_t)__popcll(mo & below);
+            for (uint64_t m = mo; m; m &= m - 1) {
+                const int s = __ffsll((unsigned long long)m) - 1;
+                const float* src = W.rows + (size_t)(rlu(W.r0, s) & ~CACHED) * LY::LB;
+                float* dst = out.pool + (size_t)rlu(ref, s) * LY::LB;
+#pragma unroll
+                for (int k = 0; k < LY::PL; ++k) dst[l + 64 * k] = src[l + 64 * k];
+            }
+            if (l < 32 && (uint32_t)l >= W.own0 && (uint32_t)l < W.n0) {
+                out.bid[(size_t)d * KB + l] = W.b0;
+                out.sref[(size_t)d * KB + l] = ref;
+            }
+            if (l == 0) out.ns[d] = W.n0;
         }
         if (l == 0) {
-            out.ns[d] = n_out;
             k4drop_w[d] += dw;
             k4drop_E[d] += de;
         }

@@ -10,6 +10,7 @@
 // synchronous calls per iteration.
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -61,6 +62,35 @@ constexpr int CHUNK = 32;           // iterations queued per host read-back
 
 }  // namespace
 
+// Launch-shape tuner of the transport: waves per listed cell (1, 2 or 4; every choice gives the
+// same results).  The choice is made per phase and per size class of the launch (floor(log2) of
+// the listed cells of the previous chunk): the first chunk of a class runs the three shapes on
+// alternating iterations with events around each launch; later chunks use the fastest average.
+struct Tuner {
+    static constexpr int NC = 3;
+    static constexpr int NB = 24;
+    double ms[2][NB][NC] = {};
+    int cnt[2][NB][NC] = {};
+    bool done[2][NB] = {};
+    int forced = 0;           // 0 = tuned, else the fixed waves per cell
+    static int tw(int c) { return c == 0 ? 1 : c == 1 ? 2 : 4; }
+    static int bucket(uint32_t listed) {
+        int b = 0;
+        while (b + 1 < NB && (2u << b) <= listed) ++b;
+        return b;
+    }
+    int best(int ph, int b) const {
+        int bc = 2;
+        double bt = 1e300;
+        for (int c = 0; c < NC; ++c)
+            if (cnt[ph][b][c] && ms[ph][b][c] / cnt[ph][b][c] < bt) {
+                bt = ms[ph][b][c] / cnt[ph][b][c];
+                bc = c;
+            }
+        return bc;
+    }
+};
+
 struct sm2d_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -86,6 +116,10 @@ struct sm2d_ctx {
     DBuf<double> injacc;
     std::vector<double> src;  // per spot: W in/out/slot, E in/out/slot, E representation loss
     uint32_t hint = 64;       // expected listed cells per phase (launch sizing only)
+    uint32_t hint_ph[2] = {64, 64};  // listed cells of the last chunk, coarse / fine
+    Tuner tune;
+    int cand[2] = {2, 2};             // transport shape of the iteration being queued
+    std::vector<int> cand_log;        // shapes of the chunk's iterations (2 per iteration)
     uint32_t hint_occ = 64;   // expected occupied cells per buffer (launch sizing only)
     // stage timing (eager launches with events on `stream`)
     // Events are recorded for every queued iteration of a chunk (12 per iteration) and read
@@ -95,8 +129,9 @@ struct sm2d_ctx {
     std::vector<hipEvent_t> ev;
     int ev_it = 0;  // iteration slot within the current chunk
     sm2d_timing timing{};
+    bool rec = false;  // events recorded this chunk (profiling or tuner exploration)
     void mark(int k) {
-        if (prof) (void)hipEventRecord(ev[(size_t)ev_it * EPI + k], stream);
+        if (rec) (void)hipEventRecord(ev[(size_t)ev_it * EPI + k], stream);
     }
 
     Tabs tabs() const {
@@ -232,18 +267,27 @@ void launch_iteration(sm2d_ctx* c, const Tp& P) {
         k_phase_reset<<<1, 64, 0, c->stream>>>(c->ctl.p, c->bpcount.p);
         c->mark(e0);
         const size_t dyn = transport_dyn_lds(c->g.N_E);
-        if (fine)
-            k_transport<LY, true><<<blocks_for(h, 1, 65536), 64, dyn, c->stream>>>(
-                T, P, c->psi(in), c->psi(out), c->bkt(), list, nl, c->ltag.p, c->ctl.p, c->tally(), acc, c->flags.p);
-        else
-            k_transport<LY, false><<<blocks_for(h, 1, 65536), 64, dyn, c->stream>>>(
-                T, P, c->psi(in), c->psi(out), c->bkt(), list, nl, c->ltag.p, c->ctl.p, c->tally(), acc, c->flags.p);
+        const int tw = Tuner::tw(c->cand[fine]);
+        const int gb = blocks_for(std::max<uint32_t>(c->hint_ph[fine], 64), 1, 65536);
+#define SM2D_TRANSPORT(F, W)                                                                              \
+    k_transport<LY, F, W><<<gb, 64 * W, dyn, c->stream>>>(T, P, c->psi(in), c->psi(out), c->bkt(), list, nl, \
+                                                         c->ltag.p, c->ctl.p, c->tally(), acc, c->flags.p)
+        if (fine) {
+            if (tw == 1) SM2D_TRANSPORT(true, 1);
+            else if (tw == 2) SM2D_TRANSPORT(true, 2);
+            else SM2D_TRANSPORT(true, 4);
+        } else {
+            if (tw == 1) SM2D_TRANSPORT(false, 1);
+            else if (tw == 2) SM2D_TRANSPORT(false, 2);
+            else SM2D_TRANSPORT(false, 4);
+        }
+#undef SM2D_TRANSPORT
         c->mark(e0 + 1);
         k_dest<<<blocks_for(h, 256, 1024), 256, 0, c->stream>>>(list, nl, c->g.Nx, c->g.Nz, c->Ns, c->dtag.p, c->ctl.p,
                                                                 fine, c->dlist.p, c->ctl.p + C_NDEST, c->dcap,
                                                                 c->bitmap.p, c->flags.p);
         c->mark(e0 + 2);
-        k_gather<LY><<<blocks_for(4ull * h, 4, 8192), 256, 0, c->stream>>>(
+        k_gather<LY><<<blocks_for(4ull * h, 1, 65536), 64, 0, c->stream>>>(
             T, c->psi(out), c->bkt(), c->dlist.p, c->ctl.p + C_NDEST, c->ltag.p, c->lpos.p, c->ctl.p, fine, c->dcap,
             c->k4w.p, c->k4e.p, c->flags.p);
         c->mark(e0 + 3);
@@ -354,10 +398,29 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
     std::vector<double> rows((size_t)HROWS * NV);
     while (queued < n_iter) {
         const int R = std::min(chunk, n_iter - queued);
+        // transport shapes of this chunk (see Tuner)
+        int bkt[2];
+        bool explore[2];
+        for (int ph = 0; ph < 2; ++ph) {
+            bkt[ph] = Tuner::bucket(c->hint_ph[ph] / 2);
+            explore[ph] = !c->tune.forced && bkt[ph] > 0 && !c->tune.done[ph][bkt[ph]];
+        }
+        c->rec = c->prof || explore[0] || explore[1];
+        c->cand_log.assign((size_t)2 * R, 0);
         for (int r = 0; r < R; ++r) {
+            for (int ph = 0; ph < 2; ++ph) {
+                int cd;
+                if (c->tune.forced) cd = c->tune.forced == 1 ? 0 : c->tune.forced == 2 ? 1 : 2;
+                else if (bkt[ph] == 0) cd = 0;
+                else if (explore[ph]) cd = r % Tuner::NC;
+                else cd = c->tune.best(ph, bkt[ph]);
+                c->cand[ph] = cd;
+                c->cand_log[(size_t)2 * r + ph] = cd;
+            }
             c->ev_it = r;
             launch_iteration<LY>(c, P);
         }
+        const int first_it = queued;  // global index of the chunk's first iteration
         queued += R;
         HIPCHK(hipGetLastError());
         uint32_t ctl[NCTL], f[NFLAG];
@@ -401,9 +464,28 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
             }
         }
         collect_timing(c, upto - audited);
+        for (int ph = 0; ph < 2; ++ph) {
+            if (!explore[ph]) continue;
+            const int e0 = ph ? 6 : 2;
+            for (int it = std::max(audited, first_it); it < upto; ++it) {
+                const int r = it - first_it;
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, c->ev[(size_t)r * sm2d_ctx::EPI + e0],
+                                        c->ev[(size_t)r * sm2d_ctx::EPI + e0 + 1]) != hipSuccess)
+                    continue;
+                const int cd = c->cand_log[(size_t)2 * r + ph];
+                c->tune.ms[ph][bkt[ph]][cd] += ms;
+                c->tune.cnt[ph][bkt[ph]][cd] += 1;
+            }
+            bool all = true;
+            for (int cd = 0; cd < Tuner::NC; ++cd) all = all && c->tune.cnt[ph][bkt[ph]][cd] >= 2;
+            if (all) c->tune.done[ph][bkt[ph]] = true;
+        }
         audited = upto;
         maxlist = std::max(ctl[C_NACT], ctl[C_NCRS]);
         c->hint = std::max<uint32_t>(64, 2 * maxlist);
+        c->hint_ph[0] = std::max<uint32_t>(2, 2 * ctl[C_NCRS]);
+        c->hint_ph[1] = std::max<uint32_t>(2, 2 * ctl[C_NACT]);
         c->hint_occ = std::max<uint32_t>(64, 2 * std::max(ctl[C_NOCC0], ctl[C_NOCC1]));
         if (au) au->iterations = done ? (int)ctl[C_DONE_IT] : (int)ctl[C_IT];
         if (done) {
@@ -551,15 +633,29 @@ int create(int device, const sm2d_grid* g, int n_spots, sm2d_ctx** out) {
     c->S = n_spots;
     c->Ns = (int)Ns;
     c->N = (int)N;
-    // Capacities: listed cells per phase <= 48 rows of every spot; occupied cells and
-    // destinations <= 5x that; payload slots <= 16 per listed cell (at least 64 Ki).  An
-    // overflow stops the transport and is reported as SM2D_ERR_CAPACITY.
-    c->lcap = (uint32_t)std::min<size_t>(N, (size_t)n_spots * 48 * g->Nx + 1024);
+    // Capacities: listed cells per phase <= 48 rows of every spot (24 in a batch, whose spots
+    // are not all at their widest at once); occupied cells and destinations <= 5x that; payload
+    // slots <= 16 (8 in a batch) per listed cell, at least 64 Ki.  An overflow stops the
+    // transport and is reported as SM2D_ERR_CAPACITY.
+    const size_t rows = n_spots > 1 ? 24 : 48, per_listed = n_spots > 1 ? 8 : 16;
+    c->lcap = (uint32_t)std::min<size_t>(N, (size_t)n_spots * rows * g->Nx + 1024);
     c->ocap = (uint32_t)std::min<size_t>(N, 5ull * c->lcap);
     c->dcap = c->ocap;
-    c->pcap = (uint32_t)std::min<size_t>(N * KB, std::max<size_t>(65536, 16ull * c->lcap));
+    c->pcap = (uint32_t)std::min<size_t>(N * KB, std::max<size_t>(65536, per_listed * c->lcap));
     c->nwords = (uint32_t)((N + 31) / 32);
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    {   // the transport's dynamic LDS (energy tables) comes on top of its static LDS
+        const int dyn = (int)transport_dyn_lds(g->N_E);
+        const void* fns[12] = {
+            (const void*)&k_transport<LayoutRef, true, 1>, (const void*)&k_transport<LayoutRef, true, 2>,
+            (const void*)&k_transport<LayoutRef, true, 4>, (const void*)&k_transport<LayoutRef, false, 1>,
+            (const void*)&k_transport<LayoutRef, false, 2>, (const void*)&k_transport<LayoutRef, false, 4>,
+            (const void*)&k_transport<LayoutCfg, true, 1>, (const void*)&k_transport<LayoutCfg, true, 2>,
+            (const void*)&k_transport<LayoutCfg, true, 4>, (const void*)&k_transport<LayoutCfg, false, 1>,
+            (const void*)&k_transport<LayoutCfg, false, 2>, (const void*)&k_transport<LayoutCfg, false, 4>};
+        for (int k = (lid == L_REF ? 0 : 6); k < (lid == L_REF ? 6 : 12); ++k)
+            HIPCHK(hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+    }
     for (int k = 0; k < 2; ++k) {
         HIPCHK(c->bid[k].alloc(N * KB));
         HIPCHK(c->ns[k].alloc(N));
@@ -616,6 +712,10 @@ int create(int device, const sm2d_grid* g, int n_spots, sm2d_ctx** out) {
     HIPCHK(hipMemcpyAsync(c->rho.p, ones.data(), sizeof(float) * g->Nz, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->x0r.p, ones.data(), sizeof(float) * g->Nz, hipMemcpyHostToDevice, c->stream));
     c->hint = (uint32_t)std::min<size_t>(c->lcap, (size_t)n_spots * g->Nx);
+    c->hint_ph[0] = c->hint_ph[1] = c->hint;
+    c->ev.resize((size_t)CHUNK * sm2d_ctx::EPI);
+    for (hipEvent_t& e : c->ev) HIPCHK(hipEventCreate(&e));
+    if (const char* f = std::getenv("SM2D_TRANSPORT_WAVES")) c->tune.forced = std::atoi(f);
     int rc = zero_state(c.get());
     if (rc) return rc;
     *out = c.release();
@@ -1055,14 +1155,48 @@ int sm2d_run_batch(sm2d_ctx* c, const sm2d_spot* spots, int n, const sm2d_transp
 int sm2d_set_profiling(sm2d_ctx* c, int on) {
     if (!c) return err("null context");
     HIPCHK(hipSetDevice(c->device));
-    if (c->ev.empty()) {
-        c->ev.resize((size_t)CHUNK * sm2d_ctx::EPI);
-        for (hipEvent_t& e : c->ev) HIPCHK(hipEventCreate(&e));
-    }
     c->prof = on != 0;
     std::memset(&c->timing, 0, sizeof(c->timing));
     c->timing.struct_size = sizeof(c->timing);
     c->timing.enabled = c->prof ? 1 : 0;
+    return SM2D_OK;
+}
+
+int sm2d_set_transport_waves(sm2d_ctx* c, int waves) {
+    if (!c) return err("null context");
+    if (waves != 0 && waves != 1 && waves != 2 && waves != 4) return err("waves per cell must be 0, 1, 2 or 4");
+    c->tune.forced = waves;
+    return SM2D_OK;
+}
+
+int sm2d_tuner_report(sm2d_ctx* c, char* buf, int cap) {
+    if (!c || !buf || cap <= 0) return err("sm2d_tuner_report: null argument");
+    std::string o = "{";
+    const char* names[2] = {"coarse", "fine"};
+    for (int ph = 0; ph < 2; ++ph) {
+        o += std::string(ph ? ", " : "") + "\"" + names[ph] + "\": {";
+        bool first = true;
+        for (int b = 0; b < Tuner::NB; ++b) {
+            bool any = false;
+            for (int cd = 0; cd < Tuner::NC; ++cd) any = any || c->tune.cnt[ph][b][cd];
+            if (!any) continue;
+            char t[256];
+            std::snprintf(t, sizeof(t), "%s\"%u-%u\": {\"best_waves\": %d", first ? "" : ", ", 1u << b,
+                          (2u << b) - 1u, Tuner::tw(c->tune.best(ph, b)));
+            o += t;
+            for (int cd = 0; cd < Tuner::NC; ++cd)
+                if (c->tune.cnt[ph][b][cd]) {
+                    std::snprintf(t, sizeof(t), ", \"us_%dw\": %.2f", Tuner::tw(cd),
+                                  1e3 * c->tune.ms[ph][b][cd] / c->tune.cnt[ph][b][cd]);
+                    o += t;
+                }
+            o += "}";
+            first = false;
+        }
+        o += "}";
+    }
+    o += "}";
+    std::snprintf(buf, (size_t)cap, "%s", o.c_str());
     return SM2D_OK;
 }
 

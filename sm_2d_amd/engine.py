@@ -264,6 +264,16 @@ class Engine:
         self.L.call("run_spot", self.h, C.byref(spot), C.byref(tp), _d(edep), C.byref(a))
         return edep, a
 
+    def set_transport_waves(self, waves: int):
+        """Fixes the transport launch shape (1, 2 or 4 waves per listed cell; 0 = tuned)."""
+        self.L.call("set_transport_waves", self.h, waves)
+
+    def tuner_report(self) -> dict:
+        import json
+        buf = C.create_string_buffer(8192)
+        self.L.call("tuner_report", self.h, buf, len(buf))
+        return json.loads(buf.value.decode())
+
     def run_batch(self, spots, tp: abi.Transport, per_spot: bool = False):
         """Transports `spots` together; returns (summed EdepC [Nz, Nx] FP64, per-spot EdepC
         [n, Nz, Nx] FP32 or None, batch audit)."""

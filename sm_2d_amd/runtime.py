@@ -24,6 +24,8 @@ def load_library() -> Library:
         "ctx_create_batch": [C.c_int, C.POINTER(abi.Grid), C.c_int, C.POINTER(C.c_void_p)],
         "run_batch": [C.c_void_p, C.POINTER(abi.Spot), C.c_int, C.POINTER(abi.Transport),
                       abi.dp, abi.fp, C.POINTER(abi.Audit)],
+        "set_transport_waves": [C.c_void_p, C.c_int],
+        "tuner_report": [C.c_void_p, C.c_char_p, C.c_int],
     })
     lib.lib.sm2d_abi_version.restype = C.c_int
     if lib.lib.sm2d_abi_version() != 1:

@@ -218,3 +218,20 @@ def test_slab_phantom_bit_exact(hip, orc):
     assert np.array_equal(eg, eo)
     assert ag.cell_bin_updates == ao.cell_bin_updates
     assert ag.W_fail_iters == 0
+
+
+@pytest.mark.parametrize("waves", [1, 2, 4])
+@pytest.mark.parametrize("prob", [
+    problems.reference_grid_case(Nx=40, Nz=80, E0=60.0, sigma_x=2.5, n_samples=400, max_iterations=300),
+    problems.config1(n_samples=400),
+], ids=["refgrid60_LB256", "config1_LB128"])
+def test_transport_shapes_bit_exact(hip, orc, prob, waves):
+    """Every launch shape of the transport (waves per listed cell) reproduces the oracle."""
+    g, o = _pair(hip, orc, prob)
+    g.set_transport_waves(waves)
+    sp, tp = prob.make_spot(), prob.make_transport()
+    eg, ag = g.run_spot(sp, tp)
+    eo, ao = o.run_spot(sp, tp)
+    assert ag.iterations == ao.iterations
+    assert np.array_equal(eg, eo)
+    assert ag.cell_bin_updates == ao.cell_bin_updates
