@@ -119,6 +119,8 @@ typedef struct sm2d_timing {
     uint64_t iterations, transport_launches, gather_launches;
     double mask_ms, transport_ms, gather_ms, audit_ms;
     uint64_t cell_bin_updates;   /* updates done by the timed transport launches */
+    double coarse_transport_ms, fine_transport_ms;  /* transport_ms split by phase */
+    uint64_t coarse_updates, fine_updates;          /* cell_bin_updates split by phase */
 } sm2d_timing;
 
 typedef struct sm2d_ctx sm2d_ctx;

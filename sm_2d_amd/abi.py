@@ -142,6 +142,8 @@ class Timing(C.Structure):
         ("mask_ms", C.c_double), ("transport_ms", C.c_double), ("gather_ms", C.c_double),
         ("audit_ms", C.c_double),
         ("cell_bin_updates", C.c_uint64),
+        ("coarse_transport_ms", C.c_double), ("fine_transport_ms", C.c_double),
+        ("coarse_updates", C.c_uint64), ("fine_updates", C.c_uint64),
     ]
 
     def __init__(self):

@@ -1911,8 +1911,9 @@ __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const ui
 // ------------------------------------------------------------------------------------------
 enum { R_WIN, R_EIN, R_WOUT, R_EOUT, R_K4W, R_K4E, NR1 };
 // Audit vector: [0,NR1) cell sums, [NR1,NR1+NACC) phase sums, then max cell W closure, max
-// fine-cell E closure and the number of listed cells of the iteration.
-constexpr int NAUD = NR1 + NACC + 3;
+// fine-cell E closure, the number of listed cells of the iteration and the cell-bin updates of
+// its fine phase.
+constexpr int NAUD = NR1 + NACC + 4;
 
 // Stage 1a: one wavefront per list entry i: W_in / E_in from K1's per-entry sums of psi_in,
 // W_out / E_out of psi_out's i-th occupied cell (coalesced payload reads, FP64 butterfly) and
@@ -2002,7 +2003,7 @@ __global__ __launch_bounds__(256) void k_audit_final(const double* part, const u
                                                      const uint32_t* nf, double* outv, double* hist,
                                                      uint32_t hist_rows, const uint32_t* ctl, int force) {
     if (ctl[C_DONE] && !force) return;
-    constexpr int NV = NAUD - 1;  // reduced entries; the list size is appended
+    constexpr int NV = NAUD - 2;  // reduced entries; the list size and fine updates are appended
     __shared__ double sh[NV][256];
     double r[NV];
     for (int k = 0; k < NV; ++k) r[k] = 0.0;
@@ -2033,6 +2034,12 @@ __global__ __launch_bounds__(256) void k_audit_final(const double* part, const u
     double* dst = hist ? hist + (size_t)((ctl[C_IT] - 1u) % hist_rows) * NAUD : outv;
     if (threadIdx.x < NV) dst[threadIdx.x] = sh[threadIdx.x][0];
     if (threadIdx.x == NV) dst[NV] = (double)((acc_c ? *nc : 0u) + (acc_f ? *nf : 0u));
+    if (threadIdx.x == 0) {  // fine-phase updates (integers, exact in FP64 for any realistic count)
+        double u = 0.0;
+        const uint32_t n = acc_f ? *nf : 0u;
+        for (uint32_t i = 0; i < n; ++i) u += acc_f[(size_t)i * NACC + A_UPD];
+        dst[NV + 1] = u;
+    }
 }
 
 // ------------------------------------------------------------------------------------------

@@ -57,6 +57,7 @@ struct DBuf {
 
 enum LayoutId { L_REF = 0, L_CFG = 1 };
 constexpr int NV = NAUD;  // audit vector
+constexpr int IDX_LIST = NAUD - 2, IDX_FUPD = NAUD - 1;
 constexpr int HROWS = 64;           // audit rows per chunk
 constexpr int CHUNK = 32;           // iterations queued per host read-back
 
@@ -329,6 +330,8 @@ void collect_timing(sm2d_ctx* c, int n_valid) {
         };
         c->timing.mask_ms += span(0, 1);
         c->timing.transport_ms += span(2, 3) + span(6, 7);
+        c->timing.coarse_transport_ms += span(2, 3);
+        c->timing.fine_transport_ms += span(6, 7);
         c->timing.gather_ms += span(3, 5) + span(7, 9);
         c->timing.audit_ms += span(9, 11);
         c->timing.transport_launches += 2;
@@ -454,10 +457,14 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
                 au->max_cell_W_closure = std::max(au->max_cell_W_closure, v[NR1 + NACC]);
                 au->max_cell_E_closure = std::max(au->max_cell_E_closure, v[NR1 + NACC + 1]);
                 au->cell_bin_updates += (uint64_t)a[A_UPD];
-                au->processed_cells += (uint64_t)v[NV - 1];
+                au->processed_cells += (uint64_t)v[IDX_LIST];
                 au->processed_components += (uint64_t)a[A_COMPS];
             }
-            if (c->prof) c->timing.cell_bin_updates += (uint64_t)v[NR1 + A_UPD];
+            if (c->prof) {
+                c->timing.cell_bin_updates += (uint64_t)v[NR1 + A_UPD];
+                c->timing.fine_updates += (uint64_t)v[IDX_FUPD];
+                c->timing.coarse_updates += (uint64_t)v[NR1 + A_UPD] - (uint64_t)v[IDX_FUPD];
+            }
             if (tp->fail_fast_on_audit && (!r.W_pass || !r.E_pass)) {
                 c->in = in0 ^ (it & 1);  // psi_in of the failing iteration
                 return err("K5 audit fail-fast", SM2D_ERR_AUDIT);
