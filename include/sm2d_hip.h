@@ -97,6 +97,8 @@ typedef struct sm2d_audit {
                                          occupied slots x local bins */
     uint64_t processed_cells, processed_components;
     uint64_t unsupported_events;      /* lateral emissions into another listed cell's bucket */
+    int surrogate_active;             /* 1 when the fine transport used the learned physics */
+    int surrogate_fallback;           /* 1 when a K5 drift switched it back to the plain physics */
 } sm2d_audit;
 
 /* Per-iteration K5 report: AuditReport of src/cuda/kernels/k5_audit.cuh:5-28, with the weight
@@ -248,6 +250,59 @@ int sm2d_tuner_report(sm2d_ctx* ctx, char* buf, int capacity);
 /* Turns stage timing on (1) or off (0) and zeroes the counters; sm2d_get_timing reads them. */
 int sm2d_set_profiling(sm2d_ctx* ctx, int on);
 int sm2d_get_timing(sm2d_ctx* ctx, sm2d_timing* out);
+
+/* --- learned physics of the fine transport ------------------------------------------------
+ * No counterpart in the reference.  A 4-128-128-128-33 MLP (GELU; BF16 weights, FP32
+ * accumulation on the matrix cores) maps (ln E, log10 dR, rho, ln X0 ratio) to the per-substep
+ * straggling width, scattering width and nuclear attenuation exponent that K3 evaluates
+ * with device_energy_straggling_sigma / device_highland_sigma / device_apply_nuclear_attenuation
+ * (src/cuda/device/device_physics.cuh:50-234).  Outputs: 32 logits of a joint distribution over
+ * 4 x 8 log-spaced width classes (the widths are its log-moments) and ln(a + 1e-9).  It is
+ * trained on the GPU on samples of the plain physics; the transport reads a table of it. */
+typedef struct sm2d_mlp sm2d_mlp;
+
+typedef struct sm2d_mlp_train_cfg {
+    uint32_t struct_size;
+    int n_samples;   /* training samples drawn from the plain physics (e.g. 1 << 22) */
+    int batch;       /* mini-batch, a multiple of 64 */
+    int steps;       /* Adam steps */
+    float lr;        /* learning rate (stepped down to 0.3x and 0.1x in the last half) */
+    float w_att;     /* weight of the attenuation term of the loss */
+    float w_mom;     /* weight of the decoded-width (log-moment) term of the loss */
+    uint32_t seed;   /* sample seed */
+} sm2d_mlp_train_cfg;
+
+int sm2d_mlp_create(int device, uint32_t seed, sm2d_mlp** out);
+int sm2d_mlp_destroy(sm2d_mlp* mlp);
+int sm2d_mlp_param_count(void);
+/* FP32 master parameters (W0[128][4], b0, W1[128][128], b1, W2[128][128], b2, W3[33][128], b3)
+ * and the width dictionaries (ln sigma_E lo/hi, ln sigma_theta lo/hi; may be NULL). */
+int sm2d_mlp_get_params(sm2d_mlp* mlp, float* params, float* dict4);
+int sm2d_mlp_set_params(sm2d_mlp* mlp, const float* params, const float* dict4);
+/* Raw outputs z[B][33] of normalised inputs x[B][4]. */
+int sm2d_mlp_forward(sm2d_mlp* mlp, const float* x, int B, float* z);
+/* Mean loss and its gradient (parameter layout above) of B samples (a multiple of 64):
+ * labels[B][3] = (ln sigma_E, ln sigma_theta, ln(a + 1e-9)); loss = soft-target cross-entropy
+ * + w_mom * (squared errors of the decoded ln widths) + w_att * (attenuation output error)^2. */
+int sm2d_mlp_grad(sm2d_mlp* mlp, const float* x, const float* labels, int B, float w_att,
+                  float w_mom, float* loss, float* grad);
+/* Trains on samples of the plain physics drawn with the tables of ctx; loss_hist[n_hist]
+ * receives the mean loss of consecutive blocks of steps. */
+int sm2d_mlp_train(sm2d_mlp* mlp, sm2d_ctx* ctx, const sm2d_mlp_train_cfg* cfg, float* loss_hist,
+                   int n_hist);
+/* Held-out accuracy on n fresh samples: RMS and largest errors of ln sigma_E, ln sigma_theta
+ * and of the attenuation exponent relative to its largest value (err6[6]). */
+int sm2d_mlp_evaluate(sm2d_mlp* mlp, sm2d_ctx* ctx, int n, uint32_t seed, float* err6);
+/* Builds the context's surrogate table (the network at every material row x energy bin x
+ * n_nodes log-spaced steps in [1e-4, 10] mm) and turns the learned physics on.  Fine-transport
+ * substeps outside the training range use the plain physics; a failed K5 weight check switches
+ * the rest of the run back to it (sm2d_audit.surrogate_fallback). */
+int sm2d_surrogate_attach(sm2d_ctx* ctx, sm2d_mlp* mlp, int n_nodes);
+int sm2d_surrogate_enable(sm2d_ctx* ctx, int on);
+/* The attached table [n_mat][N_E][n_nodes][3] (ln sigma_E, ln sigma_theta, a; NaN outside the
+ * training range) and the material index of every depth row (-1: plain physics). */
+int sm2d_surrogate_export(sm2d_ctx* ctx, float* table, int capacity, int* n_mat, int* n_nodes,
+                          int32_t* rowmat);
 
 /* --- self-test of the shared elementary functions on the device -------------------------- */
 /* Evaluates f(x[i]) for i < n on the GPU: which = 0 exp, 1 log, 2 sin, 3 cos, 4 erf. */

@@ -36,6 +36,7 @@ def load(omp: bool = False) -> Library:
     return Library(path, "orc_", {
         "eval": [C.c_void_p, C.c_int, abi.fp, abi.fp, abi.fp, C.c_int],
         "set_threads": [C.c_void_p, C.c_int],
+        "surrogate_set": [C.c_void_p, abi.fp, C.c_int, C.c_int, abi.i32p, C.c_int],
     })
 
 
@@ -49,3 +50,12 @@ class OracleEngine(Engine):
                     None if yy is None else yy.ctypes.data_as(abi.fp),
                     out.ctypes.data_as(abi.fp), len(x))
         return out
+
+    def surrogate_set(self, table, n_mat: int, n_nodes: int, rowmat, on: bool = True):
+        """Uses the library's learned-physics table (the layout of sm2d_surrogate_export)."""
+        import numpy as np
+        t = np.ascontiguousarray(table, np.float32)
+        r = np.ascontiguousarray(rowmat, np.int32)
+        self._surr_keep = (t, r)
+        self.L.call("surrogate_set", self.h, t.ctypes.data_as(abi.fp), n_mat, n_nodes,
+                    r.ctypes.data_as(abi.i32p), 1 if on else 0)

@@ -259,6 +259,10 @@ struct orc_ctx {
     int LB = 0, Tl = 0, El = 0, Xs = 0, Zs = 0, N = 0;
     std::vector<float> th_edges, E_edges, R, S, lE, lR, lS;
     std::vector<float> feA, feB, rho, x0r;
+    // learned physics table of the fine transport (same layout and lookup as the library)
+    std::vector<float> surr;
+    std::vector<int> rowmat;
+    int surr_nodes = 0, surr_on = 0, surr_active = 0;
     Psi psi[2];
     int in = 0;  // index of psi_in
     std::vector<uint8_t> mask, mask_prev;
@@ -388,6 +392,29 @@ struct orc_ctx {
 };
 
 namespace {
+
+// Learned straggling width, scattering width and attenuation exponent from the table (the
+// library's surrogate_lookup, restated).
+bool surrogate_lookup(const orc_ctx* c, int ebin, int iz, float dR, float& sE, float& sth, float& a) {
+    if (c->surr.empty()) return false;
+    const int m = c->rowmat[iz];
+    const float DR_LO = 1e-4f, DR_HI = 10.0f;
+    if (m < 0 || !(dR >= DR_LO && dR <= DR_HI)) return false;
+    const float lo = logf_(DR_LO), hi = logf_(DR_HI);
+    const float pj = (logf_(dR) - lo) / (hi - lo) * (float)(c->surr_nodes - 1);
+    const int j = std::min((int)pj, c->surr_nodes - 2);
+    const float f = pj - (float)j;
+    const float* t0 = c->surr.data() + ((size_t)(m * c->g.N_E + ebin) * c->surr_nodes + j) * 3;
+    const float* t1 = t0 + 3;
+    const float lsE = t0[0] + f * (t1[0] - t0[0]);
+    const float lsT = t0[1] + f * (t1[1] - t0[1]);
+    const float aa = t0[2] + f * (t1[2] - t0[2]);
+    if (!(lsE == lsE && lsT == lsT && aa == aa)) return false;
+    sE = expf_(lsE);
+    sth = expf_(lsT);
+    a = fmaxf(aa, 0.0f);
+    return true;
+}
 
 // ---- device physics (src/cuda/device/device_physics.cuh) ---------------------------------
 constexpr float MP = 938.272f, ME = 0.511f, X0W = 360.8f, KV = 0.307f, ZA = 0.555f;
@@ -699,9 +726,12 @@ struct Phase {
                     float sp = c->max_step(E, rh, tp->step_rule);
                     ds = fminf(sp, sb * 1.001f);
                     float mean_dE = c->deposition(E, ds, rh);
+                    float s_sE = 0.0f, s_sth = 0.0f, s_a = 0.0f;
+                    const bool use_s = c->surr_active && ds > 0.0f && ds / X0 >= 1e-6f &&
+                                       surrogate_lookup(c, ebin, iz, rh * ds, s_sE, s_sth, s_a);
                     if (tp->enable_straggling) {
                         unsigned seed = (unsigned)((cell * 7 + slot * 13 + lidx * 17) ^ 0x5DEECE66DL);
-                        float sE = straggling_sigma(E, ds, rh);
+                        float sE = use_s ? s_sE : straggling_sigma(E, ds, rh);
                         if (sE <= 0.0f) dE = mean_dE;
                         else dE = fmaxf(mean_dE + gaussian(seed) * sE, 0.0f);
                     } else {
@@ -711,7 +741,7 @@ struct Phase {
                     E_new = E - dE;
                     edep = dE * weight;
                     if (tp->enable_nuclear) {
-                        float surv = expf_(-(nuclear_sigma(E) * rh) * ds);
+                        float surv = use_s ? expf_(-s_a) : expf_(-(nuclear_sigma(E) * rh) * ds);
                         w_new = weight * surv;
                         w_rem = weight - w_new;
                         E_rem = w_rem * E;
@@ -722,8 +752,8 @@ struct Phase {
                     float A = c->feA.empty() ? 0.0f : c->feA[iz];
                     float B = c->feB.empty() ? 0.0f : c->feB[iz];
                     float Em = fmaxf(E - 0.5f * dE, c->E_min());
-                    float T = scattering_power(Em, smm, X0);
-                    sth = sqrtf_(fmaxf(T * smm, 0.0f));
+                    float T = use_s ? (s_sth * s_sth) / smm : scattering_power(Em, smm, X0);
+                    sth = use_s ? s_sth : sqrtf_(fmaxf(T * smm, 0.0f));
                     float dC = 2.0f * B * smm + A * smm * smm + T * smm * smm * smm / 3.0f;
                     dC = fmaxf(dC, 0.0f);
                     sigma_x = fmaxf(sqrtf_(dC), 0.01f);
@@ -1444,6 +1474,18 @@ int orc_fe_upload(orc_ctx* c, const float* A, const float* B) {
     return SM2D_OK;
 }
 
+// The library's learned-physics table (sm2d_surrogate_export layout); on = 0 turns it off.
+int orc_surrogate_set(orc_ctx* c, const float* table, int n_mat, int n_nodes, const int32_t* rowmat, int on) {
+    if (on) {
+        if (!table || !rowmat || n_mat <= 0 || n_nodes < 2) return fail("orc_surrogate_set: bad argument");
+        c->surr.assign(table, table + (size_t)n_mat * c->g.N_E * n_nodes * 3);
+        c->rowmat.assign(rowmat, rowmat + c->g.Nz);
+        c->surr_nodes = n_nodes;
+    }
+    c->surr_on = on ? 1 : 0;
+    return SM2D_OK;
+}
+
 int orc_material_upload(orc_ctx* c, const float* rho, const float* x0r) {
     if (!rho) c->rho.assign(c->g.Nz, 1.0f); else c->rho.assign(rho, rho + c->g.Nz);
     if (!x0r) c->x0r.assign(c->g.Nz, 1.0f); else c->x0r.assign(x0r, x0r + c->g.Nz);
@@ -1630,7 +1672,14 @@ static int run_iters(orc_ctx* c, const sm2d_transport* tp, float sx0, int n_iter
                      sm2d_k5_report* last, sm2d_audit* au) {
     int bon = b_threshold(c, tp->E_fine_on), boff = b_threshold(c, tp->E_fine_off);
     std::vector<int> act, crs;
+    // learned physics: a drift detected by K5 takes effect at the next block of 32 iterations,
+    // the granularity at which the library reads its audits back
+    const int chunk = tp->fail_fast_on_audit ? 1 : 32;
+    bool drift = false;
+    c->surr_active = c->surr_on;
+    if (au) au->surrogate_active = c->surr_active;
     for (int it = 0; it < n_iter; ++it) {
+        if (drift && it % chunk == 0) c->surr_active = 0;
         c->mask_prev = c->mask;
         k1(c, bon, boff, tp->weight_active_min, c->mask_prev.data(), c->mask.data());
         compact(c, c->mask.data(), act, crs);
@@ -1648,6 +1697,10 @@ static int run_iters(orc_ctx* c, const sm2d_transport* tp, float sx0, int n_iter
         k5(c, dW, dE, c->src_W_out, c->src_W_slot, c->src_E_out, c->src_E_slot, drop_w, drop_E,
            first ? 1 : 0, &r);
         if (last) *last = r;
+        if (c->surr_active && (!r.W_pass || !std::isfinite(r.E_error))) {
+            drift = true;
+            if (au) au->surrogate_fallback = 1;
+        }
         if (au) {
             au->W_transport_drop += drop_w;
             au->E_transport_drop += drop_E;

@@ -44,6 +44,7 @@ int orc_lut_upload(orc_ctx* c, const float* theta_edges, const float* E_edges, c
                    const float* S, const float* log_E, const float* log_R, const float* log_S);
 int orc_fe_upload(orc_ctx* c, const float* A, const float* B);
 int orc_material_upload(orc_ctx* c, const float* rho, const float* x0_ratio);
+int orc_surrogate_set(orc_ctx* c, const float* table, int n_mat, int n_nodes, const int32_t* rowmat, int on);
 int orc_reset(orc_ctx* c);
 int orc_inject(orc_ctx* c, const sm2d_spot* spot, sm2d_audit* audit);
 int orc_psi_load(orc_ctx* c, int n_rows, const int32_t* cell, const int32_t* slot,

@@ -102,6 +102,7 @@ class Audit(C.Structure):
         ("cell_bin_updates", C.c_uint64),
         ("processed_cells", C.c_uint64), ("processed_components", C.c_uint64),
         ("unsupported_events", C.c_uint64),
+        ("surrogate_active", C.c_int), ("surrogate_fallback", C.c_int),
     ]
 
     def __init__(self):
@@ -162,6 +163,19 @@ u8p = C.POINTER(C.c_uint8)
 ip = C.POINTER(C.c_int)
 
 # Prototypes shared by libsm2d_hip.so (prefix "sm2d_") and the oracle (prefix "orc_").
+class MlpTrainCfg(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32),
+        ("n_samples", C.c_int), ("batch", C.c_int), ("steps", C.c_int),
+        ("lr", C.c_float), ("w_att", C.c_float), ("w_mom", C.c_float), ("seed", C.c_uint32),
+    ]
+
+    def __init__(self, **kw):
+        d = dict(n_samples=1 << 22, batch=4096, steps=3000, lr=2e-3, w_att=1.0, w_mom=1.0, seed=7)
+        d.update(kw)
+        super().__init__(struct_size=C.sizeof(MlpTrainCfg), **d)
+
+
 COMMON_PROTOS = {
     "energy_grid_log": [C.c_float, C.c_float, C.c_int, fp, fp],
     "energy_grid_piecewise": [fp, C.c_int, fp, fp, ip],

@@ -56,17 +56,21 @@ struct Tabs {
     const float* sx0;  // initial beam sigma_x per spot [S] (coarse moments)
     const float* rho;  // density per row [Nz]
     const float* x0r;  // radiation-length ratio per row [Nz]
+    const float* surr;    // learned-physics table [materials][NE][surr_nodes][3] or null
+    const int* rowmat;    // material index of every row in that table (-1: plain physics)
+    int surr_nodes;
 };
 
 // Device control block (u32 words).
 enum Ctl {
     C_NACT = 0, C_NCRS = 1, C_NDEST = 2, C_DONE = 3, C_EP = 4, C_IT = 5, C_DONE_IT = 6,
-    C_NOCC0 = 8, C_NOCC1 = 9, NCTL = 16
+    C_NOCC0 = 8, C_NOCC1 = 9, C_NDEST1 = 10, C_BPC0 = 11, C_BPC1 = 12, NCTL = 16
 };
 
 struct Tp {
     float E_on, E_off, wmin, step_coarse;
     int b_on, b_off, rule, strag, nuc;
+    int surr;  // fine transport takes the straggling, scattering and attenuation from the table
 };
 
 struct Psi {

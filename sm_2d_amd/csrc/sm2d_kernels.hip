@@ -14,8 +14,34 @@
 // K5  k_audit_*      ConservationAudit (k5_audit.cu:22-167) as fixed-order FP64 reductions
 // K6  host           buffer swap; the next iteration resets only the cells that were occupied
 #include "sm2d_device.h"
+#include "sm2d_mlp.hip"
 
 namespace sm2d {
+
+// Learned straggling width, scattering width and attenuation exponent of a fine-transport
+// substep from the surrogate table (linear in ln dR between the two nearest nodes); false when
+// the row's material, the step or the table entry is outside the network's training range.
+__device__ inline bool surrogate_lookup(const Tabs& T, int ebin, int iz, float dR, float& sE, float& sth,
+                                        float& a) {
+    if (!T.surr) return false;
+    const int m = T.rowmat[iz];
+    if (m < 0 || !(dR >= mlp::DR_LO && dR <= mlp::DR_HI)) return false;
+    const float lo = logf_(mlp::DR_LO), hi = logf_(mlp::DR_HI);
+    const float pj = (logf_(dR) - lo) / (hi - lo) * (float)(T.surr_nodes - 1);
+    const int j = min((int)pj, T.surr_nodes - 2);
+    const float f = pj - (float)j;
+    const float* t0 = T.surr + ((size_t)(m * T.NE + ebin) * T.surr_nodes + j) * 3;
+    const float* t1 = t0 + 3;
+    const float lsE = t0[0] + f * (t1[0] - t0[0]);
+    const float lsT = t0[1] + f * (t1[1] - t0[1]);
+    const float aa = t0[2] + f * (t1[2] - t0[2]);
+    if (!(lsE == lsE && lsT == lsT && aa == aa)) return false;
+    sE = expf_(lsE);
+    sth = expf_(lsT);
+    a = fmaxf(aa, 0.0f);
+    return true;
+}
+
 
 // ------------------------------------------------------------------------------------------
 // Utility kernels
@@ -30,14 +56,21 @@ __global__ void k_begin(uint32_t* ctl) {
     if (threadIdx.x == 0 && blockIdx.x == 0 && !ctl[C_DONE]) {
         ctl[C_EP] += 1;
         ctl[C_IT] += 1;
-        ctl[C_NDEST] = 0;
+        ctl[C_NDEST] = ctl[C_NDEST1] = 0;
+        ctl[C_BPC0] = ctl[C_BPC1] = 0;
     }
 }
 
-// Resets the slot tables of the cells occupied in a buffer (its occupied-cell list) and the
-// buffer's payload pool.
-__global__ void k_clear(Psi p, const uint32_t* __restrict__ occ, const uint32_t* nocc, const uint32_t* ctl) {
+// Opens an iteration (as k_begin, from the first thread) and resets the slot tables of the cells
+// occupied in the output buffer (its occupied-cell list) and that buffer's payload pool.
+__global__ void k_clear(Psi p, const uint32_t* __restrict__ occ, const uint32_t* nocc, uint32_t* ctl) {
     if (ctl[C_DONE]) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl[C_EP] += 1;
+        ctl[C_IT] += 1;
+        ctl[C_NDEST] = ctl[C_NDEST1] = 0;
+        ctl[C_BPC0] = ctl[C_BPC1] = 0;
+    }
     const uint32_t n = *nocc;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t c = occ[i];
@@ -250,6 +283,155 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t* __restrict__ occ, 
         }
         __syncthreads();
     }
+}
+
+// One-workgroup form of k_flag_count + k_scan<2> + k_emit for short occupied lists: tiles of 1024
+// entries in order, ranks from wave ballots, running bases per list.  Same lists, tags, counts,
+// capacity check and end-of-transport test.
+__global__ __launch_bounds__(1024) void k_compact_small(const uint32_t* __restrict__ occ, const uint32_t* nocc,
+                                                        const uint8_t* __restrict__ flag, uint32_t* act,
+                                                        uint32_t* crs, uint32_t* ltag, uint32_t* lpos,
+                                                        uint32_t* bitmap, uint32_t* ctl, uint32_t cap,
+                                                        uint32_t* flags) {
+    if (ctl[C_DONE]) {
+        if (threadIdx.x == 0) {
+            ctl[C_NACT] = 0;
+            ctl[C_NCRS] = 0;
+        }
+        return;
+    }
+    __shared__ uint32_t wc[2][16];
+    __shared__ uint32_t base[2];
+    const uint32_t n = *nocc, ep = ctl[C_EP];
+    const int w = threadIdx.x >> 6, l = lane_id();
+    const uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
+    if (threadIdx.x < 2) base[threadIdx.x] = 0;
+    __syncthreads();
+    // Pass 1: list lengths, so that an over-long list writes nothing (as the tiled form).
+    uint32_t ca = 0, cc = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += 1024) {
+        const uint8_t f = flag[i];
+        ca += f == 1;
+        cc += f == 2;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        ca += __shfl_xor(ca, o);
+        cc += __shfl_xor(cc, o);
+    }
+    if (l == 0) {
+        atomicAdd(&base[0], ca);
+        atomicAdd(&base[1], cc);
+    }
+    __syncthreads();
+    const uint32_t na = base[0], nc = base[1];
+    if (na > cap || nc > cap || (na == 0 && nc == 0)) {
+        if (threadIdx.x == 0) {
+            const bool over = na > cap || nc > cap;
+            if (over) atomicOr(&flags[F_DEST_FULL], 1u);
+            ctl[C_NACT] = over ? 0u : na;
+            ctl[C_NCRS] = over ? 0u : nc;
+            ctl[C_DONE] = 1;
+            ctl[C_DONE_IT] = ctl[C_IT];
+        }
+        return;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) base[threadIdx.x] = 0;
+    __syncthreads();
+    // Pass 2: ordered emission.
+    for (uint32_t t0 = 0; t0 < n; t0 += 1024) {
+        const uint32_t i = t0 + threadIdx.x;
+        const uint8_t f = i < n ? flag[i] : 0;
+        const uint64_t ba = __ballot(f == 1), bc = __ballot(f == 2);
+        if (l == 0) {
+            wc[0][w] = __popcll(ba);
+            wc[1][w] = __popcll(bc);
+        }
+        __syncthreads();
+        if (f) {
+            const int k = f - 1;
+            uint32_t pos = base[k];
+            for (int j = 0; j < w; ++j) pos += wc[k][j];
+            pos += __popcll((k == 0 ? ba : bc) & below);
+            const uint32_t c = occ[i];
+            if (k == 0) {
+                act[pos] = c;
+                ltag[c] = 2 * ep + 1;
+            } else {
+                crs[pos] = c;
+                ltag[c] = 2 * ep;
+            }
+            lpos[c] = pos;
+            atomicOr(&bitmap[c >> 5], 1u << (c & 31));
+        }
+        __syncthreads();
+        if (threadIdx.x < 2) {
+            uint32_t s = 0;
+            for (int j = 0; j < 16; ++j) s += wc[threadIdx.x][j];
+            base[threadIdx.x] += s;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ctl[C_NACT] = na;
+        ctl[C_NCRS] = nc;
+    }
+}
+
+// One-workgroup form of k_occ_count + k_scan<1> + k_occ_emit for small grids.
+__global__ __launch_bounds__(1024) void k_occ_small(uint32_t* bitmap, uint32_t nwords, uint32_t* occ, uint32_t cap,
+                                                    uint32_t* nocc, uint32_t* ctl, uint32_t* flags) {
+    if (ctl[C_DONE]) return;
+    __shared__ uint32_t sc[1024];
+    __shared__ uint32_t base;
+    if (threadIdx.x == 0) base = 0;
+    __syncthreads();
+    // Pass 1: total, so that an over-long list leaves the bitmap and list untouched (as the
+    // tiled form).
+    uint32_t cnt = 0;
+    for (uint32_t i = threadIdx.x; i < nwords; i += 1024) cnt += __popc(bitmap[i]);
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+    if (lane_id() == 0) atomicAdd(&base, cnt);
+    __syncthreads();
+    const uint32_t total = base;
+    if (total > cap) {
+        if (threadIdx.x == 0) {
+            atomicOr(&flags[F_DEST_FULL], 1u);
+            ctl[C_DONE] = 1;
+            ctl[C_DONE_IT] = ctl[C_IT];
+            *nocc = 0u;
+        }
+        return;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) base = 0;
+    __syncthreads();
+    for (uint32_t t0 = 0; t0 < nwords; t0 += 1024) {
+        const uint32_t i = t0 + threadIdx.x;
+        uint32_t b = i < nwords ? bitmap[i] : 0u;
+        sc[threadIdx.x] = __popc(b);
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const uint32_t a = threadIdx.x >= (uint32_t)o ? sc[threadIdx.x - o] : 0;
+            __syncthreads();
+            sc[threadIdx.x] += a;
+            __syncthreads();
+        }
+        uint32_t pos = base + sc[threadIdx.x] - __popc(b);
+        if (b) {
+            bitmap[i] = 0u;
+            while (b) {
+                const int k = __ffs(b) - 1;
+                b &= b - 1;
+                occ[pos] = i * 32 + (uint32_t)k;
+                ++pos;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) base += sc[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *nocc = total;
 }
 
 // Occupied-cell list of the buffer just written: per-tile popcounts of the occupancy bitmap
@@ -1016,9 +1198,12 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
                 float sb = fmaxf(fminf(fminf(s1, s2), fminf(s3, s4)), 0.0f);
                 ds = fminf(max_step(T, E, rh, P.rule), sb * 1.001f);
                 float mdE = deposition(T, E, ds, rh);
+                float s_sE = 0.0f, s_sth = 0.0f, s_a = 0.0f;
+                const bool use_s = P.surr && ds > 0.0f && ds / X0 >= 1e-6f &&
+                                   surrogate_lookup(T, ebin, iz, rh * ds, s_sE, s_sth, s_a);
                 if (P.strag) {
                     unsigned sd = (unsigned)((lc * 7 + (int)slot * 13 + lidx * 17) ^ 0x5DEECE66DL);
-                    float sE = straggling(E, ds, rh);
+                    float sE = use_s ? s_sE : straggling(E, ds, rh);
                     dE = (sE <= 0.0f) ? mdE : fmaxf(mdE + gauss(sd) * sE, 0.0f);
                 } else {
                     dE = mdE;
@@ -1027,7 +1212,7 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
                 E_new = E - dE;
                 edep = dE * weight;
                 if (P.nuc) {
-                    float surv = expf_(-(nuc_sigma(E) * rh) * ds);
+                    float surv = use_s ? expf_(-s_a) : expf_(-(nuc_sigma(E) * rh) * ds);
                     w_new = weight * surv;
                     w_rem = weight - w_new;
                     E_rem = w_rem * E;
@@ -1038,8 +1223,8 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
                 }
                 float smm = fmaxf(ds, 0.0f);
                 float Em = fmaxf(E - 0.5f * dE, T.ee[0]);
-                float Tp_ = scat_power(Em, smm, X0);
-                sth = sqrtf_(fmaxf(Tp_ * smm, 0.0f));
+                float Tp_ = use_s ? (s_sth * s_sth) / smm : scat_power(Em, smm, X0);
+                sth = use_s ? s_sth : sqrtf_(fmaxf(Tp_ * smm, 0.0f));
                 float dC = 2.0f * B * smm + A * smm * smm + Tp_ * smm * smm * smm / 3.0f;
                 dC = fmaxf(dC, 0.0f);
                 sx = fmaxf(sqrtf_(dC), 0.01f);
@@ -2001,7 +2186,8 @@ __global__ __launch_bounds__(256) void k_audit_tiles(const double* __restrict__ 
 __global__ __launch_bounds__(256) void k_audit_final(const double* part, const uint32_t* nin, const uint32_t* nout,
                                                      const double* acc_c, const uint32_t* nc, const double* acc_f,
                                                      const uint32_t* nf, double* outv, double* hist,
-                                                     uint32_t hist_rows, const uint32_t* ctl, int force) {
+                                                     uint32_t hist_rows, const uint32_t* ctl, int force,
+                                                     const double* __restrict__ pe) {
     if (ctl[C_DONE] && !force) return;
     constexpr int NV = NAUD - 2;  // reduced entries; the list size and fine updates are appended
     __shared__ double sh[NV][256];
@@ -2009,6 +2195,24 @@ __global__ __launch_bounds__(256) void k_audit_final(const double* part, const u
     for (int k = 0; k < NV; ++k) r[k] = 0.0;
     const uint32_t n_in = nin ? *nin : 0u, n_out = *nout;
     const uint32_t ntile = (max(n_in, n_out) + 255) / 256;
+    if (pe) {  // stage 1 in this workgroup (short lists): the same tile sums, in the same order
+        double* wpart = const_cast<double*>(part);
+        for (uint32_t tile = 0; tile < ntile; ++tile) {
+            const uint32_t i = tile * 256 + threadIdx.x;
+            const uint32_t n = max(n_in, n_out);
+            for (int k = 0; k < NR1; ++k) sh[k][threadIdx.x] = i < n ? pe[(size_t)i * NR1 + k] : 0.0;
+            __syncthreads();
+            for (int o = 128; o > 0; o >>= 1) {
+                if ((int)threadIdx.x < o)
+                    for (int k = 0; k < NR1; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
+                __syncthreads();
+            }
+            if (threadIdx.x < NR1) wpart[tile * NR1 + threadIdx.x] = sh[threadIdx.x][0];
+            __syncthreads();
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
     for (uint32_t b = threadIdx.x; b < ntile; b += 256)
         for (int k = 0; k < NR1; ++k) r[k] += part[b * NR1 + k];
     for (int ph = 0; ph < 2; ++ph) {

@@ -106,7 +106,7 @@ struct sm2d_ctx {
     DBuf<float> pool[2];
     int in = 0;
     DBuf<uint32_t> bitmap;
-    DBuf<uint32_t> bbid, bn, bsref, bpcount;
+    DBuf<uint32_t> bbid, bn, bsref;
     DBuf<float> bpool;
     DBuf<uint32_t> maskv, ltag, lpos, dtag, act, crs, dlist, ctl, tcount, tcount2, flags;
     DBuf<uint8_t> flag;
@@ -115,6 +115,10 @@ struct sm2d_ctx {
     DBuf<sm2d_spot> spots;
     DBuf<InjSample> samples;
     DBuf<double> injacc;
+    // learned physics of the fine transport (sm2d_surrogate_attach)
+    DBuf<float> surr;
+    DBuf<int> rowmat;
+    int surr_nodes = 0, surr_nmat = 0, surr_on = 0, surr_drift = 0;
     std::vector<double> src;  // per spot: W in/out/slot, E in/out/slot, E representation loss
     uint32_t hint = 64;       // expected listed cells per phase (launch sizing only)
     uint32_t hint_ph[2] = {64, 64};  // listed cells of the last chunk, coarse / fine
@@ -142,6 +146,7 @@ struct sm2d_ctx {
         t.dx = g.dx; t.dz = g.dz; t.x_min = g.x_min; t.z_min = g.z_min;
         t.te = te.p; t.ee = ee.p; t.R = R.p; t.lR = lR.p; t.lE = lE.p;
         t.feA = feA.p; t.feB = feB.p; t.sx0 = sx0.p; t.rho = rho.p; t.x0r = x0r.p;
+        t.surr = surr.p; t.rowmat = rowmat.p; t.surr_nodes = surr_nodes;
         return t;
     }
     Psi psi(int k) const {
@@ -150,9 +155,13 @@ struct sm2d_ctx {
         p.pcount = pcount[k].p; p.pcap = pcap;
         return p;
     }
-    Bkt bkt() const {
+    // Bucket tables of a phase: both phases share the pool (the coarse gather has consumed its
+    // buckets before the fine transport starts) with a payload counter of their own in ctl.
+    Bkt bkt(int fine) const {
         Bkt b;
-        b.bid = bbid.p; b.n = bn.p; b.sref = bsref.p; b.pool = bpool.p; b.pcount = bpcount.p; b.pcap = pcap;
+        b.bid = bbid.p; b.n = bn.p; b.sref = bsref.p; b.pool = bpool.p;
+        b.pcount = ctl.p + (fine ? C_BPC1 : C_BPC0);
+        b.pcap = pcap;
         return b;
     }
     Tally tally() const {
@@ -198,6 +207,7 @@ Tp make_tp(const sm2d_ctx* c, const sm2d_transport* t) {
     p.rule = t->step_rule;
     p.strag = t->enable_straggling;
     p.nuc = t->enable_nuclear;
+    p.surr = (c->surr_on && !c->surr_drift && c->surr.p) ? 1 : 0;
     return p;
 }
 
@@ -207,16 +217,16 @@ __global__ void k_ctl_reset(uint32_t* ctl) {
     ctl[C_DONE_IT] = 0;
 }
 
-// Resets the per-phase bucket pool and destination count (skipped once the transport ended).
-__global__ void k_phase_reset(uint32_t* ctl, uint32_t* bpcount) {
-    if (!ctl[C_DONE]) {
-        ctl[C_NDEST] = 0;
-        *bpcount = 0;
-    }
-}
+// Lists up to these lengths are built by one workgroup (one launch instead of three).
+constexpr uint32_t SMALL_LIST = 8192, SMALL_WORDS = 16384, SMALL_AUDIT = 2048;
 
 // Occupancy bitmap -> ascending occupied-cell list of buffer k.
 void build_occ(sm2d_ctx* c, int k) {
+    if (c->nwords <= SMALL_WORDS) {
+        k_occ_small<<<1, 1024, 0, c->stream>>>(c->bitmap.p, c->nwords, c->occ[k].p, c->ocap, c->nocc(k), c->ctl.p,
+                                               c->flags.p);
+        return;
+    }
     const int nt = blocks_for(c->nwords, 256, 1 << 20);
     k_occ_count<<<nt, 256, 0, c->stream>>>(c->bitmap.p, c->nwords, c->tcount2.p, c->ctl.p);
     // n_items = nwords is held in ctl[15]
@@ -232,21 +242,29 @@ void launch_mask_compact(sm2d_ctx* c, const Tp& P, int in, int grid_entries) {
     const int gt = blocks_for(grid_entries, 256, 4096);
     k_mask<LY><<<ge, 64, 0, c->stream>>>(c->tabs(), P, c->psi(in), c->occ[in].p, c->nocc(in), c->maskv.p, c->ctl.p,
                                          c->flag.p, c->win.p, c->ein.p);
+    if ((uint32_t)grid_entries <= SMALL_LIST) {
+        k_compact_small<<<1, 1024, 0, c->stream>>>(c->occ[in].p, c->nocc(in), c->flag.p, c->act.p, c->crs.p,
+                                                   c->ltag.p, c->lpos.p, c->bitmap.p, c->ctl.p, c->lcap, c->flags.p);
+        return;
+    }
     k_flag_count<<<gt, 256, 0, c->stream>>>(c->flag.p, c->nocc(in), c->tcount.p, c->ctl.p);
     k_scan<2><<<1, 1024, 0, c->stream>>>(c->tcount.p, c->nocc(in), 256, c->ctl.p, nullptr, c->lcap, c->flags.p);
     k_emit<<<gt, 256, 0, c->stream>>>(c->occ[in].p, c->nocc(in), c->flag.p, c->tcount.p, c->act.p, c->crs.p,
                                       c->ltag.p, c->lpos.p, c->bitmap.p, c->ctl.p);
 }
 
-// K5 stage 1 over (nin entries of K1 sums, nout occupied cells of buffer k).
+// K5 stage 1 over (nin entries of K1 sums, nout occupied cells of buffer k).  Returns the
+// per-cell sums for k_audit_final to tile itself (short lists), else null after the tile pass.
 template <class LY>
-void launch_audit_cells(sm2d_ctx* c, int k, const uint32_t* nin, bool k4, int force, int grid_entries) {
+const double* launch_audit_cells(sm2d_ctx* c, int k, const uint32_t* nin, bool k4, int force, int grid_entries) {
     const int ge = blocks_for(grid_entries, 1, 1 << 16);
     const int gt = blocks_for(grid_entries, 256, 4096);
     k_audit_cells<LY><<<ge, 64, 0, c->stream>>>(c->tabs(), c->psi(k), nin, c->win.p, c->ein.p, c->occ[k].p,
                                                 c->nocc(k), k4 ? c->k4w.p : nullptr, k4 ? c->k4e.p : nullptr,
                                                 c->pe.p, c->ctl.p, force);
+    if ((uint32_t)grid_entries <= SMALL_AUDIT) return c->pe.p;
     k_audit_tiles<<<gt, 256, 0, c->stream>>>(c->pe.p, nin, c->nocc(k), c->part.p, c->ctl.p, force);
+    return nullptr;
 }
 
 template <class LY>
@@ -255,8 +273,7 @@ void launch_iteration(sm2d_ctx* c, const Tp& P) {
     Tabs T = c->tabs();
     const uint32_t h = std::max<uint32_t>(c->hint, 64);
     const int gm = blocks_for(4ull * h, 256, 4096);  // occupied-list tiles
-    k_begin<<<1, 64, 0, c->stream>>>(c->ctl.p);
-    k_clear<<<gm, 256, 0, c->stream>>>(c->psi(out), c->occ[out].p, c->nocc(out), c->ctl.p);
+    k_clear<<<gm, 256, 0, c->stream>>>(c->psi(out), c->occ[out].p, c->nocc(out), c->ctl.p);  // opens the iteration
     c->mark(0);
     launch_mask_compact<LY>(c, P, in, (int)c->hint_occ);
     c->mark(1);
@@ -265,13 +282,13 @@ void launch_iteration(sm2d_ctx* c, const Tp& P) {
         const uint32_t* nl = c->ctl.p + (fine ? C_NACT : C_NCRS);
         double* acc = fine ? c->acc_f.p : c->acc_c.p;
         const int e0 = fine ? 6 : 2;
-        k_phase_reset<<<1, 64, 0, c->stream>>>(c->ctl.p, c->bpcount.p);
+        uint32_t* ndest = c->ctl.p + (fine ? C_NDEST1 : C_NDEST);
         c->mark(e0);
         const size_t dyn = transport_dyn_lds(c->g.N_E);
         const int tw = Tuner::tw(c->cand[fine]);
         const int gb = blocks_for(std::max<uint32_t>(c->hint_ph[fine], 64), 1, 65536);
 #define SM2D_TRANSPORT(F, W)                                                                              \
-    k_transport<LY, F, W><<<gb, 64 * W, dyn, c->stream>>>(T, P, c->psi(in), c->psi(out), c->bkt(), list, nl, \
+    k_transport<LY, F, W><<<gb, 64 * W, dyn, c->stream>>>(T, P, c->psi(in), c->psi(out), c->bkt(fine), list, nl, \
                                                          c->ltag.p, c->ctl.p, c->tally(), acc, c->flags.p)
         if (fine) {
             if (tw == 1) SM2D_TRANSPORT(true, 1);
@@ -285,19 +302,20 @@ void launch_iteration(sm2d_ctx* c, const Tp& P) {
 #undef SM2D_TRANSPORT
         c->mark(e0 + 1);
         k_dest<<<blocks_for(h, 256, 1024), 256, 0, c->stream>>>(list, nl, c->g.Nx, c->g.Nz, c->Ns, c->dtag.p, c->ctl.p,
-                                                                fine, c->dlist.p, c->ctl.p + C_NDEST, c->dcap,
+                                                                fine, c->dlist.p, ndest, c->dcap,
                                                                 c->bitmap.p, c->flags.p);
         c->mark(e0 + 2);
         k_gather<LY><<<blocks_for(4ull * h, 1, 65536), 64, 0, c->stream>>>(
-            T, c->psi(out), c->bkt(), c->dlist.p, c->ctl.p + C_NDEST, c->ltag.p, c->lpos.p, c->ctl.p, fine, c->dcap,
+            T, c->psi(out), c->bkt(fine), c->dlist.p, ndest, c->ltag.p, c->lpos.p, c->ctl.p, fine, c->dcap,
             c->k4w.p, c->k4e.p, c->flags.p);
         c->mark(e0 + 3);
     }
     build_occ(c, out);
     c->mark(10);
-    launch_audit_cells<LY>(c, out, c->nocc(in), true, 0, (int)c->hint_occ);
+    const double* pe = launch_audit_cells<LY>(c, out, c->nocc(in), true, 0, (int)c->hint_occ);
     k_audit_final<<<1, 256, 0, c->stream>>>(c->part.p, c->nocc(in), c->nocc(out), c->acc_c.p, c->ctl.p + C_NCRS,
-                                            c->acc_f.p, c->ctl.p + C_NACT, nullptr, c->hist.p, HROWS, c->ctl.p, 0);
+                                            c->acc_f.p, c->ctl.p + C_NACT, nullptr, c->hist.p, HROWS, c->ctl.p, 0,
+                                            pe);
     c->mark(11);
     c->in = out;  // K6
 }
@@ -305,9 +323,9 @@ void launch_iteration(sm2d_ctx* c, const Tp& P) {
 // Totals (W, E) of buffer k over its occupied list, after the transport (forced pass).
 template <class LY>
 int buffer_totals(sm2d_ctx* c, int k, double& W, double& E) {
-    launch_audit_cells<LY>(c, k, nullptr, false, 1, (int)std::max<uint32_t>(c->hint_occ, 1024));
+    const double* pe = launch_audit_cells<LY>(c, k, nullptr, false, 1, (int)std::max<uint32_t>(c->hint_occ, 1024));
     k_audit_final<<<1, 256, 0, c->stream>>>(c->part.p, nullptr, c->nocc(k), nullptr, nullptr, nullptr, nullptr,
-                                            c->k5v.p, nullptr, 1, c->ctl.p, 1);
+                                            c->k5v.p, nullptr, 1, c->ctl.p, 1, pe);
     HIPCHK(hipGetLastError());
     double v[NV];
     HIPCHK(hipMemcpyAsync(v, c->k5v.p, sizeof(v), hipMemcpyDeviceToHost, c->stream));
@@ -385,7 +403,9 @@ void fill_report(const double* v, double Wso, double Wss, double Eso, double Ess
 template <class LY>
 int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, sm2d_k5_report* last,
               sm2d_audit* au) {
+    c->surr_drift = 0;
     Tp P = make_tp(c, tp);
+    if (au) au->surrogate_active = P.surr;
     k_ctl_reset<<<1, 1, 0, c->stream>>>(c->ctl.p);
     HIPCHK(hipGetLastError());
     const int in0 = c->in;
@@ -442,6 +462,12 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
             sm2d_k5_report r;
             fill_report(v, Wso, Wss, Eso, Ess, (src_terms && it == 0) ? 1 : 0, &r);
             if (last) *last = r;
+            // drift of the learned physics: a failed weight audit or a non-finite energy sum
+            // switches the remaining iterations to the plain physics
+            if (P.surr && (!r.W_pass || !std::isfinite(r.E_error))) {
+                c->surr_drift = 1;
+                if (au) au->surrogate_fallback = 1;
+            }
             if (au) {
                 const double* a = v + NR1;
                 au->W_transport_drop += a[A_DROPW] + v[R_K4W];
@@ -489,6 +515,7 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
             if (all) c->tune.done[ph][bkt[ph]] = true;
         }
         audited = upto;
+        if (P.surr && c->surr_drift) P = make_tp(c, tp);
         maxlist = std::max(ctl[C_NACT], ctl[C_NCRS]);
         c->hint = std::max<uint32_t>(64, 2 * maxlist);
         c->hint_ph[0] = std::max<uint32_t>(2, 2 * ctl[C_NCRS]);
@@ -675,7 +702,6 @@ int create(int device, const sm2d_grid* g, int n_spots, sm2d_ctx** out) {
     HIPCHK(c->bbid.alloc((size_t)c->lcap * 4 * KB));
     HIPCHK(c->bn.alloc((size_t)c->lcap * 4));
     HIPCHK(c->bsref.alloc((size_t)c->lcap * 4 * KB));
-    HIPCHK(c->bpcount.alloc(1));
     HIPCHK(c->bpool.alloc((size_t)c->pcap * c->LB));
     for (DBuf<uint32_t>* b : {&c->maskv, &c->ltag, &c->lpos, &c->dtag}) HIPCHK(b->alloc(N));
     HIPCHK(c->act.alloc(c->ocap));
@@ -745,6 +771,8 @@ int upload_fe_for(sm2d_ctx* c, int s, const sm2d_spot& sp, std::vector<float>* s
 }
 
 }  // namespace
+
+#include "sm2d_surrogate.hip"
 
 extern "C" {
 
@@ -1049,6 +1077,8 @@ static int finish_audit(sm2d_ctx* c, sm2d_audit* au, const sm2d_audit& it, int n
     au->cell_bin_updates = it.cell_bin_updates;
     au->processed_components = it.processed_components;
     au->processed_cells = it.processed_cells;
+    au->surrogate_active = it.surrogate_active;
+    au->surrogate_fallback = it.surrogate_fallback;
     for (int s = 0; s < n_spots; ++s) {
         au->W_source_injected += c->src[s * 7 + 0];
         au->W_source_out_of_grid += c->src[s * 7 + 1];

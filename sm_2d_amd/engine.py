@@ -274,6 +274,23 @@ class Engine:
         self.L.call("tuner_report", self.h, buf, len(buf))
         return json.loads(buf.value.decode())
 
+    def surrogate_attach(self, mlp: "Mlp", n_nodes: int = 64):
+        """Builds the learned-physics table of this context and turns it on."""
+        self.L.call("surrogate_attach", self.h, mlp.h, n_nodes)
+
+    def surrogate_enable(self, on: bool):
+        self.L.call("surrogate_enable", self.h, 1 if on else 0)
+
+    def surrogate_export(self):
+        """(table [n_mat][N_E][n_nodes][3], n_mat, n_nodes, rowmat [Nz])."""
+        nm, nn = C.c_int(), C.c_int()
+        self.L.call("surrogate_export", self.h, None, 0, C.byref(nm), C.byref(nn), None)
+        tab = np.zeros(nm.value * self.grid.N_E * nn.value * 3, np.float32)
+        rows = np.zeros(self.grid.Nz, np.int32)
+        self.L.call("surrogate_export", self.h, _f(tab), len(tab), C.byref(nm), C.byref(nn),
+                    rows.ctypes.data_as(abi.i32p))
+        return tab, nm.value, nn.value, rows
+
     def run_batch(self, spots, tp: abi.Transport, per_spot: bool = False):
         """Transports `spots` together; returns (summed EdepC [Nz, Nx] FP64, per-spot EdepC
         [n, Nz, Nx] FP32 or None, batch audit)."""
@@ -285,3 +302,62 @@ class Engine:
         self.L.call("run_batch", self.h, arr, n, C.byref(tp), _d(edep),
                     _f(per) if per is not None else None, C.byref(a))
         return edep, per, a
+
+
+class Mlp:
+    """The learned-physics network of the fine transport (sm2d_mlp_* of the C ABI)."""
+
+    def __init__(self, lib: Library, seed: int = 1, device: int = 0):
+        self.L = lib
+        h = C.c_void_p()
+        lib.call("mlp_create", device, seed, C.byref(h))
+        self.h = h
+        self.n = lib.lib.sm2d_mlp_param_count()
+
+    def close(self):
+        if self.h:
+            self.L.call("mlp_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.zeros(self.n, np.float32)
+        d = np.zeros(4, np.float32)
+        self.L.call("mlp_get_params", self.h, _f(p), _f(d))
+        return p, d
+
+    def set_params(self, p=None, dict4=None):
+        pp = None if p is None else np.ascontiguousarray(p, np.float32)
+        dd = None if dict4 is None else np.ascontiguousarray(dict4, np.float32)
+        self.L.call("mlp_set_params", self.h, None if pp is None else _f(pp), None if dd is None else _f(dd))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        z = np.zeros((len(x), 33), np.float32)
+        self.L.call("mlp_forward", self.h, _f(x), len(x), _f(z))
+        return z
+
+    def grad(self, x, labels, w_att: float = 1.0, w_mom: float = 1.0):
+        x = np.ascontiguousarray(x, np.float32)
+        lb = np.ascontiguousarray(labels, np.float32)
+        g = np.zeros(self.n, np.float32)
+        loss = np.zeros(1, np.float32)
+        self.L.call("mlp_grad", self.h, _f(x), _f(lb), len(x), C.c_float(w_att), C.c_float(w_mom),
+                    _f(loss), _f(g))
+        return float(loss[0]), g
+
+    def train(self, eng: Engine, n_hist: int = 20, **cfg):
+        c = abi.MlpTrainCfg(**cfg)
+        hist = np.zeros(n_hist, np.float32)
+        self.L.call("mlp_train", self.h, eng.h, C.byref(c), _f(hist), n_hist)
+        return hist
+
+    def evaluate(self, eng: Engine, n: int = 65536, seed: int = 99):
+        e = np.zeros(6, np.float32)
+        self.L.call("mlp_evaluate", self.h, eng.h, n, seed, _f(e))
+        return e

@@ -26,8 +26,20 @@ def load_library() -> Library:
                       abi.dp, abi.fp, C.POINTER(abi.Audit)],
         "set_transport_waves": [C.c_void_p, C.c_int],
         "tuner_report": [C.c_void_p, C.c_char_p, C.c_int],
+        "mlp_create": [C.c_int, C.c_uint32, C.POINTER(C.c_void_p)],
+        "mlp_destroy": [C.c_void_p],
+        "mlp_get_params": [C.c_void_p, abi.fp, abi.fp],
+        "mlp_set_params": [C.c_void_p, abi.fp, abi.fp],
+        "mlp_forward": [C.c_void_p, abi.fp, C.c_int, abi.fp],
+        "mlp_grad": [C.c_void_p, abi.fp, abi.fp, C.c_int, C.c_float, C.c_float, abi.fp, abi.fp],
+        "mlp_train": [C.c_void_p, C.c_void_p, C.POINTER(abi.MlpTrainCfg), abi.fp, C.c_int],
+        "mlp_evaluate": [C.c_void_p, C.c_void_p, C.c_int, C.c_uint32, abi.fp],
+        "surrogate_attach": [C.c_void_p, C.c_void_p, C.c_int],
+        "surrogate_enable": [C.c_void_p, C.c_int],
+        "surrogate_export": [C.c_void_p, abi.fp, C.c_int, abi.ip, abi.ip, abi.i32p],
     })
     lib.lib.sm2d_abi_version.restype = C.c_int
+    lib.lib.sm2d_mlp_param_count.restype = C.c_int
     if lib.lib.sm2d_abi_version() != 1:
         raise RuntimeError("libsm2d_hip.so ABI version mismatch")
     _LIB = lib
