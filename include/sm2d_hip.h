@@ -303,6 +303,10 @@ int sm2d_surrogate_enable(sm2d_ctx* ctx, int on);
  * training range) and the material index of every depth row (-1: plain physics). */
 int sm2d_surrogate_export(sm2d_ctx* ctx, float* table, int capacity, int* n_mat, int* n_nodes,
                           int32_t* rowmat);
+/* Installs a table in the layout above (one exported and edited, or tabulated by other means)
+ * and turns the learned physics on; rowmat[Nz] in [-1, n_mat). */
+int sm2d_surrogate_set_table(sm2d_ctx* ctx, const float* table, int n_mat, int n_nodes,
+                             const int32_t* rowmat);
 
 /* --- self-test of the shared elementary functions on the device -------------------------- */
 /* Evaluates f(x[i]) for i < n on the GPU: which = 0 exp, 1 log, 2 sin, 3 cos, 4 erf. */

@@ -454,4 +454,27 @@ int sm2d_surrogate_export(sm2d_ctx* c, float* table, int capacity, int* n_mat, i
     return SM2D_OK;
 }
 
+// Installs a caller-built table in the layout sm2d_surrogate_export returns (for instance a
+// table exported from one context and edited, or tabulated by other means).  Entries of a row
+// whose material index is -1 are never read; NaN entries send the lookup back to the plain
+// physics like the out-of-range rows.
+int sm2d_surrogate_set_table(sm2d_ctx* c, const float* table, int n_mat, int n_nodes, const int32_t* rowmat) {
+    if (!c || !table || !rowmat || n_mat < 1 || n_nodes < 2) return err("sm2d_surrogate_set_table: bad argument");
+    if (!c->tables) return err("sm2d_surrogate_set_table: context tables not uploaded");
+    for (int z = 0; z < c->g.Nz; ++z)
+        if (rowmat[z] < -1 || rowmat[z] >= n_mat) return err("sm2d_surrogate_set_table: material index out of range");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t n = (size_t)n_mat * c->g.N_E * n_nodes * 3;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(c->surr.alloc(n));
+    HIPCHK(c->rowmat.alloc(c->g.Nz));
+    HIPCHK(hipMemcpyAsync(c->surr.p, table, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->rowmat.p, rowmat, sizeof(int) * c->g.Nz, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->surr_nodes = n_nodes;
+    c->surr_nmat = n_mat;
+    c->surr_on = 1;
+    return SM2D_OK;
+}
+
 }  // extern "C"

@@ -291,6 +291,14 @@ class Engine:
                     rows.ctypes.data_as(abi.i32p))
         return tab, nm.value, nn.value, rows
 
+    def surrogate_set_table(self, table, n_mat: int, n_nodes: int, rowmat):
+        """Installs a table in the layout of surrogate_export and turns the learned physics on."""
+        tab = np.ascontiguousarray(table, np.float32)
+        rows = np.ascontiguousarray(rowmat, np.int32)
+        if tab.size != n_mat * self.grid.N_E * n_nodes * 3 or rows.size != self.grid.Nz:
+            raise ValueError("surrogate table / rowmat size does not match the grid")
+        self.L.call("surrogate_set_table", self.h, _f(tab), n_mat, n_nodes, rows.ctypes.data_as(abi.i32p))
+
     def run_batch(self, spots, tp: abi.Transport, per_spot: bool = False):
         """Transports `spots` together; returns (summed EdepC [Nz, Nx] FP64, per-spot EdepC
         [n, Nz, Nx] FP32 or None, batch audit)."""

@@ -37,6 +37,7 @@ def load_library() -> Library:
         "surrogate_attach": [C.c_void_p, C.c_void_p, C.c_int],
         "surrogate_enable": [C.c_void_p, C.c_int],
         "surrogate_export": [C.c_void_p, abi.fp, C.c_int, abi.ip, abi.ip, abi.i32p],
+        "surrogate_set_table": [C.c_void_p, abi.fp, C.c_int, C.c_int, abi.i32p],
     })
     lib.lib.sm2d_abi_version.restype = C.c_int
     lib.lib.sm2d_mlp_param_count.restype = C.c_int
