@@ -246,6 +246,12 @@ int sm2d_set_transport_waves(sm2d_ctx* ctx, int waves);
 /* JSON summary of the tuner's measurements (per phase and size class: mean launch time per
  * shape and the chosen shape), truncated to `capacity` bytes. */
 int sm2d_tuner_report(sm2d_ctx* ctx, char* buf, int capacity);
+/* Chunks of iterations are captured as HIP graphs and replayed when nothing they depend on has
+ * changed (on by default; SM2D_GRAPHS=0 in the environment turns it off at context creation).
+ * Off (0): every kernel is launched on the stream.  Chunks with stage timing or tuner
+ * measurements on are always launched on the stream.  No counterpart in the reference (its
+ * pipeline launches every kernel from the host, k1k6_pipeline.cu:1663-2011). */
+int sm2d_set_graphs(sm2d_ctx* ctx, int on);
 
 /* Turns stage timing on (1) or off (0) and zeroes the counters; sm2d_get_timing reads them. */
 int sm2d_set_profiling(sm2d_ctx* ctx, int on);

@@ -61,24 +61,31 @@ __global__ void k_begin(uint32_t* ctl) {
     }
 }
 
-// Opens an iteration (as k_begin, from the first thread) and resets the slot tables of the cells
-// occupied in the output buffer (its occupied-cell list) and that buffer's payload pool.
-__global__ void k_clear(Psi p, const uint32_t* __restrict__ occ, const uint32_t* nocc, uint32_t* ctl) {
-    if (ctl[C_DONE]) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl[C_EP] += 1;
-        ctl[C_IT] += 1;
-        ctl[C_NDEST] = ctl[C_NDEST1] = 0;
-        ctl[C_BPC0] = ctl[C_BPC1] = 0;
-    }
-    const uint32_t n = *nocc;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+__device__ inline void open_iteration(uint32_t* ctl) {
+    ctl[C_EP] += 1;
+    ctl[C_IT] += 1;
+    ctl[C_NDEST] = ctl[C_NDEST1] = 0;
+    ctl[C_BPC0] = ctl[C_BPC1] = 0;
+}
+
+// Resets the slot tables of the cells on an occupied-cell list and the buffer's payload pool
+// (thread t of nthr).
+__device__ inline void clear_cells(Psi p, const uint32_t* __restrict__ occ, uint32_t n, uint32_t t, uint32_t nthr) {
+    for (uint32_t i = t; i < n; i += nthr) {
         const uint32_t c = occ[i];
         const uint32_t k = p.ns[c];
         for (uint32_t s = 0; s < k; ++s) p.bid[(size_t)c * KB + s] = EMPTY;
         p.ns[c] = 0;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) *p.pcount = 0;
+    if (t == 0) *p.pcount = 0;
+}
+
+// Opens an iteration (as k_begin, from the first thread) and clears the output buffer over its
+// occupied-cell list.  Iterations after the first of a run are opened by k_audit_final instead.
+__global__ void k_clear(Psi p, const uint32_t* __restrict__ occ, const uint32_t* nocc, uint32_t* ctl) {
+    if (ctl[C_DONE]) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) open_iteration(ctl);
+    clear_cells(p, occ, *nocc, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -382,7 +389,7 @@ __global__ __launch_bounds__(1024) void k_compact_small(const uint32_t* __restri
 __global__ __launch_bounds__(1024) void k_occ_small(uint32_t* bitmap, uint32_t nwords, uint32_t* occ, uint32_t cap,
                                                     uint32_t* nocc, uint32_t* ctl, uint32_t* flags) {
     if (ctl[C_DONE]) return;
-    __shared__ uint32_t sc[1024];
+    __shared__ uint32_t sc[16];
     __shared__ uint32_t base;
     if (threadIdx.x == 0) base = 0;
     __syncthreads();
@@ -404,20 +411,26 @@ __global__ __launch_bounds__(1024) void k_occ_small(uint32_t* bitmap, uint32_t n
         return;
     }
     __syncthreads();
-    if (threadIdx.x == 0) base = 0;
-    __syncthreads();
+    const int w = threadIdx.x >> 6, l = lane_id();
+    uint32_t run = 0;  // words before this tile's
     for (uint32_t t0 = 0; t0 < nwords; t0 += 1024) {
         const uint32_t i = t0 + threadIdx.x;
         uint32_t b = i < nwords ? bitmap[i] : 0u;
-        sc[threadIdx.x] = __popc(b);
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            const uint32_t a = threadIdx.x >= (uint32_t)o ? sc[threadIdx.x - o] : 0;
-            __syncthreads();
-            sc[threadIdx.x] += a;
-            __syncthreads();
+        const uint32_t cb = __popc(b);
+        uint32_t x = cb;  // inclusive scan within the wave
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (l >= o) x += y;
         }
-        uint32_t pos = base + sc[threadIdx.x] - __popc(b);
+        if (l == 63) sc[w] = x;
+        __syncthreads();
+        uint32_t pos = run + x - cb;
+        uint32_t tile = 0;
+        for (int j = 0; j < 16; ++j) {
+            const uint32_t v = sc[j];
+            if (j < w) pos += v;
+            tile += v;
+        }
         if (b) {
             bitmap[i] = 0u;
             while (b) {
@@ -427,8 +440,7 @@ __global__ __launch_bounds__(1024) void k_occ_small(uint32_t* bitmap, uint32_t n
                 ++pos;
             }
         }
-        __syncthreads();
-        if (threadIdx.x == 0) base += sc[1023];
+        run += tile;
         __syncthreads();
     }
     if (threadIdx.x == 0) *nocc = total;
@@ -1658,13 +1670,33 @@ __device__ inline void serial_apply(const Comp& r, uint64_t m, WTabs<LY>& W, Cel
 // choices give identical results.
 constexpr int QCAP = 512;
 
+// Destination-set output of a transport phase.
+struct Dest {
+    uint32_t* dtag;    // per-cell tag of the phase that listed it as a destination
+    uint32_t* dlist;   // destinations, in first-marked order
+    uint32_t* ndest;   // their count
+    uint32_t cap;      // capacity of dlist
+    uint32_t* bitmap;  // occupancy bitmap of psi_out
+};
+
+__device__ inline void dest_mark(const Dest& D, uint32_t d, uint32_t tag, uint32_t* flags) {
+    const uint32_t old = D.dtag[d];
+    if (old == tag) return;
+    if (atomicCAS(&D.dtag[d], old, tag) == old) {
+        const uint32_t p = atomicAdd(D.ndest, 1u);
+        if (p < D.cap) D.dlist[p] = d;
+        else atomicOr(&flags[F_DEST_FULL], 1u);
+        atomicOr(&D.bitmap[d >> 5], 1u << (d & 31));
+    }
+}
+
 template <class LY, bool FINE, int TW>
 __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Psi out, Bkt bk,
                                                    const uint32_t* __restrict__ list,
                                                    const uint32_t* __restrict__ nlist,
                                                    const uint32_t* __restrict__ ltag, const uint32_t* ctl,
-                                                   Tally tal, double* acc, uint32_t* flags) {
-    if (ctl[C_DONE]) return;
+                                                   Tally tal, double* acc, uint32_t* flags, Dest D) {
+    if (ctl[C_DONE] || blockIdx.x >= *nlist) return;  // no cell for this workgroup
     __shared__ float s_rows[LY::NL * LY::LB];
     __shared__ FastLds s_fast[TW];
     __shared__ uint32_t s_q[QCAP];
@@ -1721,6 +1753,13 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
             const int off = l - 5, it = G.ix + off;
             const bool q = l < 11 && off != 0 && it >= 0 && it < G.Nx && ltag[cell + off] == tag;
             G.latlisted = (uint32_t)__ballot(q);
+        }
+        if (wv == 0 && l < 4) {  // the cell's neighbours join the phase's destination set
+            const int d = l == 0   ? (G.iz + 1 < T.Nz ? cell + G.Nx : -1)
+                          : l == 1 ? (G.iz > 0 ? cell - G.Nx : -1)
+                          : l == 2 ? (G.ix + 1 < G.Nx ? cell + 1 : -1)
+                                   : (G.ix > 0 ? cell - 1 : -1);
+            if (d >= 0) dest_mark(D, (uint32_t)d, tag, flags);
         }
         const uint32_t ns = in.ns[cell];
         if (threadIdx.x < ns) {
@@ -1885,7 +1924,8 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
 }
 
 // Destination set of a phase: the four neighbours of every listed cell, deduplicated with a
-// per-cell tag (the order of the resulting list does not affect any result).
+// per-cell tag (the order of the resulting list does not affect any result).  The transport
+// kernel marks them as it takes each cell (dest_mark); k_dest is the stand-alone form.
 __global__ void k_dest(const uint32_t* list, const uint32_t* nlist, int Nx, int Nz, int Ns, uint32_t* dtag,
                        const uint32_t* ctl, int fine, uint32_t* dlist, uint32_t* ndest, uint32_t cap,
                        uint32_t* bitmap, uint32_t* flags) {
@@ -2186,35 +2226,28 @@ __global__ __launch_bounds__(256) void k_audit_tiles(const double* __restrict__ 
 __global__ __launch_bounds__(256) void k_audit_final(const double* part, const uint32_t* nin, const uint32_t* nout,
                                                      const double* acc_c, const uint32_t* nc, const double* acc_f,
                                                      const uint32_t* nf, double* outv, double* hist,
-                                                     uint32_t hist_rows, const uint32_t* ctl, int force,
-                                                     const double* __restrict__ pe) {
+                                                     uint32_t hist_rows, uint32_t* ctl, int force,
+                                                     const double* __restrict__ pe, Psi nxt,
+                                                     const uint32_t* __restrict__ nxt_occ, const uint32_t* nxt_n) {
     if (ctl[C_DONE] && !force) return;
+    if (blockIdx.x > 0) {  // clear the next iteration's output buffer (this iteration's input)
+        clear_cells(nxt, nxt_occ, *nxt_n, (blockIdx.x - 1) * blockDim.x + threadIdx.x, (gridDim.x - 1) * blockDim.x);
+        return;
+    }
     constexpr int NV = NAUD - 2;  // reduced entries; the list size and fine updates are appended
     __shared__ double sh[NV][256];
     double r[NV];
     for (int k = 0; k < NV; ++k) r[k] = 0.0;
     const uint32_t n_in = nin ? *nin : 0u, n_out = *nout;
     const uint32_t ntile = (max(n_in, n_out) + 255) / 256;
-    if (pe) {  // stage 1 in this workgroup (short lists): the same tile sums, in the same order
-        double* wpart = const_cast<double*>(part);
-        for (uint32_t tile = 0; tile < ntile; ++tile) {
-            const uint32_t i = tile * 256 + threadIdx.x;
-            const uint32_t n = max(n_in, n_out);
-            for (int k = 0; k < NR1; ++k) sh[k][threadIdx.x] = i < n ? pe[(size_t)i * NR1 + k] : 0.0;
-            __syncthreads();
-            for (int o = 128; o > 0; o >>= 1) {
-                if ((int)threadIdx.x < o)
-                    for (int k = 0; k < NR1; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
-                __syncthreads();
-            }
-            if (threadIdx.x < NR1) wpart[tile * NR1 + threadIdx.x] = sh[threadIdx.x][0];
-            __syncthreads();
-        }
-        __threadfence_block();
-        __syncthreads();
+    if (pe) {  // short lists: the per-cell sums directly (entry i to thread i mod 256, in order)
+        const uint32_t n = max(n_in, n_out);
+        for (uint32_t i = threadIdx.x; i < n; i += 256)
+            for (int k = 0; k < NR1; ++k) r[k] += pe[(size_t)i * NR1 + k];
+    } else {
+        for (uint32_t b = threadIdx.x; b < ntile; b += 256)
+            for (int k = 0; k < NR1; ++k) r[k] += part[b * NR1 + k];
     }
-    for (uint32_t b = threadIdx.x; b < ntile; b += 256)
-        for (int k = 0; k < NR1; ++k) r[k] += part[b * NR1 + k];
     for (int ph = 0; ph < 2; ++ph) {
         const double* acc = ph ? acc_f : acc_c;
         const uint32_t n = acc ? (ph ? *nf : *nc) : 0u;
@@ -2243,6 +2276,10 @@ __global__ __launch_bounds__(256) void k_audit_final(const double* part, const u
         const uint32_t n = acc_f ? *nf : 0u;
         for (uint32_t i = 0; i < n; ++i) u += acc_f[(size_t)i * NACC + A_UPD];
         dst[NV + 1] = u;
+    }
+    if (gridDim.x > 1) {  // open the next iteration once every thread has read the counters
+        __syncthreads();
+        if (threadIdx.x == 0) open_iteration(ctl);
     }
 }
 

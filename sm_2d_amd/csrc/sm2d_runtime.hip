@@ -135,6 +135,20 @@ struct sm2d_ctx {
     int ev_it = 0;  // iteration slot within the current chunk
     sm2d_timing timing{};
     bool rec = false;  // events recorded this chunk (profiling or tuner exploration)
+    // Captured chunks of iterations (hipGraph), keyed by everything their launches depend on.
+    struct ChunkGraph {
+        std::vector<uint8_t> key;
+        hipGraphExec_t exec = nullptr;
+        uint64_t used = 0;
+    };
+    std::vector<ChunkGraph> graphs;
+    bool graphs_on = true;
+    uint64_t graph_clock = 0, graph_captures = 0, graph_replays = 0;
+    void graphs_clear() {
+        for (ChunkGraph& g : graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
+    }
     void mark(int k) {
         if (rec) (void)hipEventRecord(ev[(size_t)ev_it * EPI + k], stream);
     }
@@ -175,6 +189,14 @@ struct sm2d_ctx {
 
 namespace {
 
+// Launch sizes follow the list-length hints rounded up to powers of two, so that captured chunks
+// of iterations can be replayed while the lists keep growing or shrinking.
+uint32_t pow2_at_least(uint32_t x) {
+    uint32_t p = 1;
+    while (p < x && p < (1u << 31)) p <<= 1;
+    return p;
+}
+
 int blocks_for(uint64_t items, int per_block, int cap) {
     uint64_t b = (items + per_block - 1) / per_block;
     return (int)std::max<uint64_t>(1, std::min<uint64_t>(b, (uint64_t)cap));
@@ -198,6 +220,7 @@ int b_threshold(const sm2d_ctx* c, float E) {  // compute_b_E_threshold (k1k6_pi
 
 Tp make_tp(const sm2d_ctx* c, const sm2d_transport* t) {
     Tp p;
+    std::memset(&p, 0, sizeof(p));  // padding too: the bytes key the chunk graphs
     p.E_on = t->E_fine_on;
     p.E_off = t->E_fine_off;
     p.wmin = t->weight_active_min;
@@ -217,8 +240,15 @@ __global__ void k_ctl_reset(uint32_t* ctl) {
     ctl[C_DONE_IT] = 0;
 }
 
+__global__ void k_undo_open(uint32_t* ctl) {
+    if (!ctl[C_DONE]) {
+        ctl[C_EP] -= 1;
+        ctl[C_IT] -= 1;
+    }
+}
+
 // Lists up to these lengths are built by one workgroup (one launch instead of three).
-constexpr uint32_t SMALL_LIST = 8192, SMALL_WORDS = 16384, SMALL_AUDIT = 2048;
+constexpr uint32_t SMALL_LIST = 8192, SMALL_WORDS = 16384, SMALL_AUDIT = 8192;
 
 // Occupancy bitmap -> ascending occupied-cell list of buffer k.
 void build_occ(sm2d_ctx* c, int k) {
@@ -268,14 +298,17 @@ const double* launch_audit_cells(sm2d_ctx* c, int k, const uint32_t* nin, bool k
 }
 
 template <class LY>
-void launch_iteration(sm2d_ctx* c, const Tp& P) {
+// opener: open the iteration here (first of a run); open_next: the closing audit kernel opens
+// the next iteration and clears its output buffer, which is this iteration's input.
+void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next) {
     const int in = c->in, out = 1 - c->in;
     Tabs T = c->tabs();
-    const uint32_t h = std::max<uint32_t>(c->hint, 64);
+    const uint32_t h = pow2_at_least(std::max<uint32_t>(c->hint, 64));
+    const uint32_t hocc = pow2_at_least(c->hint_occ);
     const int gm = blocks_for(4ull * h, 256, 4096);  // occupied-list tiles
-    k_clear<<<gm, 256, 0, c->stream>>>(c->psi(out), c->occ[out].p, c->nocc(out), c->ctl.p);  // opens the iteration
+    if (opener) k_clear<<<gm, 256, 0, c->stream>>>(c->psi(out), c->occ[out].p, c->nocc(out), c->ctl.p);
     c->mark(0);
-    launch_mask_compact<LY>(c, P, in, (int)c->hint_occ);
+    launch_mask_compact<LY>(c, P, in, (int)hocc);
     c->mark(1);
     for (int fine = 0; fine < 2; ++fine) {
         const uint32_t* list = fine ? c->act.p : c->crs.p;
@@ -283,13 +316,19 @@ void launch_iteration(sm2d_ctx* c, const Tp& P) {
         double* acc = fine ? c->acc_f.p : c->acc_c.p;
         const int e0 = fine ? 6 : 2;
         uint32_t* ndest = c->ctl.p + (fine ? C_NDEST1 : C_NDEST);
+        Dest D;
+        D.dtag = c->dtag.p;
+        D.dlist = c->dlist.p;
+        D.ndest = ndest;
+        D.cap = c->dcap;
+        D.bitmap = c->bitmap.p;
         c->mark(e0);
         const size_t dyn = transport_dyn_lds(c->g.N_E);
         const int tw = Tuner::tw(c->cand[fine]);
-        const int gb = blocks_for(std::max<uint32_t>(c->hint_ph[fine], 64), 1, 65536);
+        const int gb = blocks_for(pow2_at_least(std::max<uint32_t>(c->hint_ph[fine], 64)), 1, 65536);
 #define SM2D_TRANSPORT(F, W)                                                                              \
     k_transport<LY, F, W><<<gb, 64 * W, dyn, c->stream>>>(T, P, c->psi(in), c->psi(out), c->bkt(fine), list, nl, \
-                                                         c->ltag.p, c->ctl.p, c->tally(), acc, c->flags.p)
+                                                         c->ltag.p, c->ctl.p, c->tally(), acc, c->flags.p, D)
         if (fine) {
             if (tw == 1) SM2D_TRANSPORT(true, 1);
             else if (tw == 2) SM2D_TRANSPORT(true, 2);
@@ -301,9 +340,6 @@ void launch_iteration(sm2d_ctx* c, const Tp& P) {
         }
 #undef SM2D_TRANSPORT
         c->mark(e0 + 1);
-        k_dest<<<blocks_for(h, 256, 1024), 256, 0, c->stream>>>(list, nl, c->g.Nx, c->g.Nz, c->Ns, c->dtag.p, c->ctl.p,
-                                                                fine, c->dlist.p, ndest, c->dcap,
-                                                                c->bitmap.p, c->flags.p);
         c->mark(e0 + 2);
         k_gather<LY><<<blocks_for(4ull * h, 1, 65536), 64, 0, c->stream>>>(
             T, c->psi(out), c->bkt(fine), c->dlist.p, ndest, c->ltag.p, c->lpos.p, c->ctl.p, fine, c->dcap,
@@ -312,10 +348,10 @@ void launch_iteration(sm2d_ctx* c, const Tp& P) {
     }
     build_occ(c, out);
     c->mark(10);
-    const double* pe = launch_audit_cells<LY>(c, out, c->nocc(in), true, 0, (int)c->hint_occ);
-    k_audit_final<<<1, 256, 0, c->stream>>>(c->part.p, c->nocc(in), c->nocc(out), c->acc_c.p, c->ctl.p + C_NCRS,
-                                            c->acc_f.p, c->ctl.p + C_NACT, nullptr, c->hist.p, HROWS, c->ctl.p, 0,
-                                            pe);
+    const double* pe = launch_audit_cells<LY>(c, out, c->nocc(in), true, 0, (int)hocc);
+    k_audit_final<<<open_next ? 1 + gm : 1, 256, 0, c->stream>>>(
+        c->part.p, c->nocc(in), c->nocc(out), c->acc_c.p, c->ctl.p + C_NCRS, c->acc_f.p, c->ctl.p + C_NACT, nullptr,
+        c->hist.p, HROWS, c->ctl.p, 0, pe, c->psi(in), c->occ[in].p, c->nocc(in));
     c->mark(11);
     c->in = out;  // K6
 }
@@ -325,7 +361,7 @@ template <class LY>
 int buffer_totals(sm2d_ctx* c, int k, double& W, double& E) {
     const double* pe = launch_audit_cells<LY>(c, k, nullptr, false, 1, (int)std::max<uint32_t>(c->hint_occ, 1024));
     k_audit_final<<<1, 256, 0, c->stream>>>(c->part.p, nullptr, c->nocc(k), nullptr, nullptr, nullptr, nullptr,
-                                            c->k5v.p, nullptr, 1, c->ctl.p, 1, pe);
+                                            c->k5v.p, nullptr, 1, c->ctl.p, 1, pe, Psi{}, nullptr, nullptr);
     HIPCHK(hipGetLastError());
     double v[NV];
     HIPCHK(hipMemcpyAsync(v, c->k5v.p, sizeof(v), hipMemcpyDeviceToHost, c->stream));
@@ -400,6 +436,76 @@ void fill_report(const double* v, double Wso, double Wss, double Eso, double Ess
     r->E_pass = (std::isfinite(r->E_error) && r->E_error < 1e-5) ? 1 : 0;
 }
 
+template <class T>
+void key_put(std::vector<uint8_t>& k, const T& v) {
+    const uint8_t* b = reinterpret_cast<const uint8_t*>(&v);
+    k.insert(k.end(), b, b + sizeof(T));
+}
+
+// Queues R iterations as one graph launch.  A chunk is captured once per key (iteration count,
+// buffer parity, opening mode, transport shapes, launch sizes, physics switches, surrogate table)
+// and replayed afterwards; every launch of the chunk reads its lists, counts and epochs from the
+// device control block, so a replay is the same work as the eager launches.
+template <class LY>
+int launch_chunk_graph(sm2d_ctx* c, const Tp& P, int R, bool first_opens, bool fused_open) {
+    std::vector<uint8_t> key;
+    key_put(key, R);
+    key_put(key, c->in);
+    key_put(key, (int)first_opens);
+    key_put(key, (int)fused_open);
+    key_put(key, c->cand[0]);
+    key_put(key, c->cand[1]);
+    key_put(key, pow2_at_least(std::max<uint32_t>(c->hint, 64)));
+    key_put(key, pow2_at_least(c->hint_occ));
+    key_put(key, pow2_at_least(std::max<uint32_t>(c->hint_ph[0], 64)));
+    key_put(key, pow2_at_least(std::max<uint32_t>(c->hint_ph[1], 64)));
+    key_put(key, P);
+    key_put(key, (const void*)c->surr.p);
+    key_put(key, (const void*)c->rowmat.p);
+    key_put(key, c->surr_nodes);
+    ++c->graph_clock;
+    for (sm2d_ctx::ChunkGraph& g : c->graphs)
+        if (g.key == key) {
+            g.used = c->graph_clock;
+            HIPCHK(hipGraphLaunch(g.exec, c->stream));
+            if (R & 1) c->in = 1 - c->in;
+            ++c->graph_replays;
+            return SM2D_OK;
+        }
+    const int in0 = c->in;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int r = 0; r < R; ++r) launch_iteration<LY>(c, P, first_opens && r == 0, fused_open);
+    hipGraph_t graph = nullptr;
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+    if (le != hipSuccess || ce != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        c->in = in0;
+        return err(std::string("chunk capture: ") + hipGetErrorString(le != hipSuccess ? le : ce), SM2D_ERR_HIP);
+    }
+    sm2d_ctx::ChunkGraph g;
+    g.key = std::move(key);
+    g.used = c->graph_clock;
+    const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+        c->in = in0;
+        return err(std::string("chunk graph instantiate: ") + hipGetErrorString(ie), SM2D_ERR_HIP);
+    }
+    HIPCHK(hipGraphLaunch(g.exec, c->stream));
+    ++c->graph_captures;
+    constexpr size_t MAX_GRAPHS = 48;
+    if (c->graphs.size() >= MAX_GRAPHS) {  // drop the least recently used
+        size_t lru = 0;
+        for (size_t i = 1; i < c->graphs.size(); ++i)
+            if (c->graphs[i].used < c->graphs[lru].used) lru = i;
+        (void)hipGraphExecDestroy(c->graphs[lru].exec);
+        c->graphs.erase(c->graphs.begin() + (long)lru);
+    }
+    c->graphs.push_back(std::move(g));
+    return SM2D_OK;
+}
+
 template <class LY>
 int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, sm2d_k5_report* last,
               sm2d_audit* au) {
@@ -410,6 +516,8 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
     HIPCHK(hipGetLastError());
     const int in0 = c->in;
     const int chunk = tp->fail_fast_on_audit ? 1 : CHUNK;
+    const bool fused_open = !tp->fail_fast_on_audit;
+    const int opened_ahead = fused_open ? 1 : 0;  // ctl[C_IT] counts the next, already opened, iteration
     int queued = 0, audited = 0;
     double Wso = 0, Wss = 0, Eso = 0, Ess = 0;
     for (int s = 0; s < c->S; ++s) {
@@ -430,7 +538,19 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
         }
         c->rec = c->prof || explore[0] || explore[1];
         c->cand_log.assign((size_t)2 * R, 0);
-        for (int r = 0; r < R; ++r) {
+        if (!c->rec && c->graphs_on) {
+            // the same shapes for every iteration of the chunk (no exploration)
+            for (int ph = 0; ph < 2; ++ph) {
+                int cd;
+                if (c->tune.forced) cd = c->tune.forced == 1 ? 0 : c->tune.forced == 2 ? 1 : 2;
+                else if (bkt[ph] == 0) cd = 0;
+                else cd = c->tune.best(ph, bkt[ph]);
+                c->cand[ph] = cd;
+                for (int r = 0; r < R; ++r) c->cand_log[(size_t)2 * r + ph] = cd;
+            }
+            const int rc = launch_chunk_graph<LY>(c, P, R, fused_open ? queued == 0 : true, fused_open);
+            if (rc) return rc;
+        } else for (int r = 0; r < R; ++r) {
             for (int ph = 0; ph < 2; ++ph) {
                 int cd;
                 if (c->tune.forced) cd = c->tune.forced == 1 ? 0 : c->tune.forced == 2 ? 1 : 2;
@@ -441,7 +561,8 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
                 c->cand_log[(size_t)2 * r + ph] = cd;
             }
             c->ev_it = r;
-            launch_iteration<LY>(c, P);
+            // fail-fast keeps psi_in of a failing iteration, so there every iteration opens itself
+            launch_iteration<LY>(c, P, fused_open ? (queued == 0 && r == 0) : true, fused_open);
         }
         const int first_it = queued;  // global index of the chunk's first iteration
         queued += R;
@@ -455,7 +576,7 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
         int rc = check_flags(c, f);
         if (rc) return rc;
         const bool done = ctl[C_DONE] != 0;
-        const int upto = done ? (int)ctl[C_DONE_IT] - 1 : (int)ctl[C_IT];  // iterations with an audit row
+        const int upto = done ? (int)ctl[C_DONE_IT] - 1 : (int)ctl[C_IT] - opened_ahead;  // audited iterations
         uint32_t maxlist = 0;
         for (int it = audited; it < upto; ++it) {
             const double* v = rows.data() + (size_t)(it % HROWS) * NV;
@@ -521,7 +642,7 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
         c->hint_ph[0] = std::max<uint32_t>(2, 2 * ctl[C_NCRS]);
         c->hint_ph[1] = std::max<uint32_t>(2, 2 * ctl[C_NACT]);
         c->hint_occ = std::max<uint32_t>(64, 2 * std::max(ctl[C_NOCC0], ctl[C_NOCC1]));
-        if (au) au->iterations = done ? (int)ctl[C_DONE_IT] : (int)ctl[C_IT];
+        if (au) au->iterations = done ? (int)ctl[C_DONE_IT] : (int)ctl[C_IT] - opened_ahead;
         if (done) {
             if (au) au->terminated = 1;
             c->in = in0 ^ (int)((ctl[C_DONE_IT] - 1u) & 1u);  // the buffer of the terminating iteration
@@ -529,6 +650,10 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
         }
     }
     if (c->prof) c->timing.iterations += (uint64_t)audited;
+    // A run that stopped at its iteration count has opened one more iteration: take it back, so
+    // that the epochs (and the mask hysteresis that follows them) continue in the next run.
+    if (fused_open) k_undo_open<<<1, 1, 0, c->stream>>>(c->ctl.p);
+    HIPCHK(hipGetLastError());
     return SM2D_OK;
 }
 
@@ -749,6 +874,7 @@ int create(int device, const sm2d_grid* g, int n_spots, sm2d_ctx** out) {
     c->ev.resize((size_t)CHUNK * sm2d_ctx::EPI);
     for (hipEvent_t& e : c->ev) HIPCHK(hipEventCreate(&e));
     if (const char* f = std::getenv("SM2D_TRANSPORT_WAVES")) c->tune.forced = std::atoi(f);
+    if (const char* f = std::getenv("SM2D_GRAPHS")) c->graphs_on = std::atoi(f) != 0;
     int rc = zero_state(c.get());
     if (rc) return rc;
     *out = c.release();
@@ -791,6 +917,7 @@ int sm2d_ctx_destroy(sm2d_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     for (hipEvent_t& e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    c->graphs_clear();
     hipStream_t s = c->stream;
     delete c;
     if (s) (void)hipStreamDestroy(s);
@@ -1206,6 +1333,15 @@ int sm2d_set_transport_waves(sm2d_ctx* c, int waves) {
     return SM2D_OK;
 }
 
+int sm2d_set_graphs(sm2d_ctx* c, int on) {
+    if (!c) return err("null context");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->graphs_on = on != 0;
+    if (!c->graphs_on) c->graphs_clear();
+    return SM2D_OK;
+}
+
 int sm2d_tuner_report(sm2d_ctx* c, char* buf, int cap) {
     if (!c || !buf || cap <= 0) return err("sm2d_tuner_report: null argument");
     std::string o = "{";
@@ -1232,6 +1368,11 @@ int sm2d_tuner_report(sm2d_ctx* c, char* buf, int cap) {
         }
         o += "}";
     }
+    char t[160];
+    std::snprintf(t, sizeof(t), ", \"chunk_graphs\": {\"on\": %d, \"captured\": %llu, \"replayed\": %llu}",
+                  c->graphs_on ? 1 : 0, (unsigned long long)c->graph_captures,
+                  (unsigned long long)c->graph_replays);
+    o += t;
     o += "}";
     std::snprintf(buf, (size_t)cap, "%s", o.c_str());
     return SM2D_OK;

@@ -420,6 +420,7 @@ int sm2d_surrogate_attach(sm2d_ctx* c, sm2d_mlp* M, int n_nodes) {
     for (int i = 0; i < total; ++i)
         if (!hv[i]) tab[3 * i] = tab[3 * i + 1] = tab[3 * i + 2] = NAN;
     tab.resize((size_t)total * 3);
+    c->graphs_clear();
     HIPCHK(c->surr.alloc(tab.size()));
     HIPCHK(c->rowmat.alloc(Nz));
     HIPCHK(hipMemcpyAsync(c->surr.p, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, c->stream));
@@ -466,6 +467,7 @@ int sm2d_surrogate_set_table(sm2d_ctx* c, const float* table, int n_mat, int n_n
     HIPCHK(hipSetDevice(c->device));
     const size_t n = (size_t)n_mat * c->g.N_E * n_nodes * 3;
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->graphs_clear();
     HIPCHK(c->surr.alloc(n));
     HIPCHK(c->rowmat.alloc(c->g.Nz));
     HIPCHK(hipMemcpyAsync(c->surr.p, table, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));

@@ -268,6 +268,10 @@ class Engine:
         """Fixes the transport launch shape (1, 2 or 4 waves per listed cell; 0 = tuned)."""
         self.L.call("set_transport_waves", self.h, waves)
 
+    def set_graphs(self, on: bool):
+        """Captured-and-replayed chunks of iterations (True, the default) or eager launches."""
+        self.L.call("set_graphs", self.h, 1 if on else 0)
+
     def tuner_report(self) -> dict:
         import json
         buf = C.create_string_buffer(8192)

@@ -235,3 +235,50 @@ def test_transport_shapes_bit_exact(hip, orc, prob, waves):
     assert ag.iterations == ao.iterations
     assert np.array_equal(eg, eo)
     assert ag.cell_bin_updates == ao.cell_bin_updates
+
+
+@pytest.mark.parametrize("split", [[1, 3], [2, 1, 1]], ids=["1+3", "2+1+1"])
+def test_split_iteration_runs_match_one_run(hip, orc, split):
+    """Iterations queued over several calls continue exactly where the last call stopped (epochs,
+    mask hysteresis, buffer parity), so the state equals one call over all of them and the
+    oracle's."""
+    prob = problems.reference_grid_case(Nx=40, Nz=40)
+    g, o = _pair(hip, orc, prob)
+    rng = np.random.default_rng(11)
+    NEb = (g.grid.N_E + g.grid.n_e_local - 1) // g.grid.n_e_local
+    NTb = g.grid.N_theta // g.grid.n_theta_local
+    rows = _random_psi(rng, g.N, g.LB, NEb, NTb, n_cells=120, max_slots=8)
+    tp = prob.make_transport(sigma_x_initial=2.0)
+    for e in (g, o):
+        e.psi_load(rows)
+    for n in split:
+        tg, _, _ = g.run_iterations(tp, n)
+    to, _, _ = o.run_iterations(tp, sum(split))
+    bg, rg = g.psi_dump()
+    bo, ro = o.psi_dump()
+    assert np.array_equal(bg, bo)
+    assert _rows_key(rg) == _rows_key(ro)
+    assert np.array_equal(tg, to)
+
+
+def test_chunk_graphs_replay_bit_exact(hip, orc):
+    """Chunks of iterations captured as graphs and replayed (with eager host work and other
+    launches between the replays) give the oracle's dose on every run, as do eager launches."""
+    prob = problems.reference_grid_case(Nx=40, Nz=80, E0=60.0, sigma_x=2.5, n_samples=400,
+                                        max_iterations=300)
+    g, o = _pair(hip, orc, prob)
+    sp, tp = prob.make_spot(), prob.make_transport()
+    eo, ao = o.run_spot(sp, tp)
+    g.set_graphs(True)
+    for rep in range(3):
+        eg, ag = g.run_spot(sp, tp)
+        assert ag.iterations == ao.iterations, rep
+        assert np.array_equal(eg, eo), rep
+        assert ag.cell_bin_updates == ao.cell_bin_updates, rep
+        g.psi_dump()  # eager copies between replays
+    rep = g.tuner_report()["chunk_graphs"]
+    assert rep["on"] == 1 and rep["captured"] > 0 and rep["replayed"] > 0
+    g.set_graphs(False)
+    eg, ag = g.run_spot(sp, tp)
+    assert np.array_equal(eg, eo)
+    assert ag.iterations == ao.iterations
