@@ -296,14 +296,23 @@ int sm2d_mlp_grad(sm2d_mlp* mlp, const float* x, const float* labels, int B, flo
  * receives the mean loss of consecutive blocks of steps. */
 int sm2d_mlp_train(sm2d_mlp* mlp, sm2d_ctx* ctx, const sm2d_mlp_train_cfg* cfg, float* loss_hist,
                    int n_hist);
-/* Held-out accuracy on n fresh samples: RMS and largest errors of ln sigma_E, ln sigma_theta
- * and of the attenuation exponent relative to its largest value (err6[6]). */
+/* Held-out accuracy on n fresh samples: RMS errors of ln sigma_E, ln sigma_theta and
+ * ln(a + 1e-9) (a: attenuation exponent), then the largest errors of the same (err6[6]). */
 int sm2d_mlp_evaluate(sm2d_mlp* mlp, sm2d_ctx* ctx, int n, uint32_t seed, float* err6);
 /* Builds the context's surrogate table (the network at every material row x energy bin x
- * n_nodes log-spaced steps in [1e-4, 10] mm) and turns the learned physics on.  Fine-transport
- * substeps outside the training range use the plain physics; a failed K5 weight check switches
- * the rest of the run back to it (sm2d_audit.surrogate_fallback). */
+ * n_nodes log-spaced steps in [1e-4, 10] mm), validates it against the plain physics at its
+ * nodes (sm2d_surrogate_check with the context tolerance, 0.25 by default) and turns the
+ * learned physics on.  Fine-transport substeps outside the training range or at rejected nodes
+ * use the plain physics; a failed K5 weight check or a non-finite K5 energy sum switches the
+ * rest of the run back to it (sm2d_audit.surrogate_fallback). */
 int sm2d_surrogate_attach(sm2d_ctx* ctx, sm2d_mlp* mlp, int n_nodes);
+/* Re-validates the attached table: nodes whose learned ln sigma_E or ln sigma_theta differ from
+ * the plain physics by more than tol, or whose attenuation exponent a differs by more than
+ * tol x (a + 1e-4 / mm x dR), are sent back to the plain physics.  stats4 (may be null) = (nodes checked,
+ * nodes rejected, largest deviation, RMS deviation); tol <= 0 only measures. */
+int sm2d_surrogate_check(sm2d_ctx* ctx, float tol, float* stats4);
+/* Statistics of the last validation (attach or check). */
+int sm2d_surrogate_stats(sm2d_ctx* ctx, float* stats4);
 int sm2d_surrogate_enable(sm2d_ctx* ctx, int on);
 /* The attached table [n_mat][N_E][n_nodes][3] (ln sigma_E, ln sigma_theta, a; NaN outside the
  * training range) and the material index of every depth row (-1: plain physics). */

@@ -270,8 +270,9 @@ __device__ inline float hash_u01(uint32_t seed, uint64_t idx, uint32_t stream) {
 }
 
 // Training samples from the plain physics: E log-uniform in [0.1, 250] MeV, rho uniform in
-// [0.2, 2], X0 ratio log-uniform in [0.5, 4], ds = (device step rule) x u with u log-uniform in
-// [1e-3, 1] (steps cut short by cell boundaries); labels from the same functions K3 calls:
+// [0.2, 2], X0 ratio log-uniform in [0.5, 4], ds log-uniform between the table's smallest step
+// (rho ds = 1e-4 mm) and the device step rule's step (steps cut short by cell boundaries
+// included); labels from the same functions K3 calls:
 // straggling width, scattering width sqrt(T(E_mid) ds) with E_mid = E - mean dE / 2, and the
 // attenuation exponent sigma_n(E) rho ds.  Samples outside the input range or with a step below
 // the Highland threshold (ds / X0 < 1e-6) are redrawn (at most 16 times, then replaced by a
@@ -287,7 +288,9 @@ __global__ void k_gen_samples(Tabs T, uint32_t seed, int n, uint16_t* __restrict
             const float Et = expf_(logf_(E_LO) + u0 * (logf_(E_HI) - logf_(E_LO)));
             const float rt = RHO_LO + u1 * (RHO_HI - RHO_LO);
             const float xt = expf_(logf_(X0R_LO) + u2 * (logf_(X0R_HI) - logf_(X0R_LO)));
-            const float dt = max_step(T, Et, rt, 0) * expf_(logf_(1e-3f) * (1.0f - u3));
+            // substep log-uniform between the table's smallest step and the largest step
+            const float smax = max_step(T, Et, rt, 0), smin = fminf(DR_LO / rt, smax);
+            const float dt = smax * expf_(logf_(smin / smax) * (1.0f - u3));
             if (dt / (X0W * xt) < 1e-6f || !encode_input(Et, rt * dt, rt, xt, x)) continue;
             E = Et;
             rho = rt;
@@ -354,6 +357,33 @@ __global__ void k_table_inputs(Tabs T, const float* __restrict__ mrho, const flo
         const bool ok = encode_input(E, dR, mrho[m], mx0r[m], x);
         valid[i] = ok ? 1 : 0;
         for (int k = 0; k < KP0; ++k) X[(size_t)i * KP0 + k] = k < NIN ? f2bf(x[k]) : 0;
+    }
+}
+
+// The plain physics at the table nodes (the labels the network is trained on): out3[i] =
+// (ln sigma_E, ln sigma_theta, a) of material m, energy bin e (its centre) and step node j.
+// valid[i]: the node can be reached by a fine-transport substep (in the input range, above the
+// Highland threshold, and the node below it within 1.5 x the device step rule's largest step).
+__global__ void k_table_plain(Tabs T, const float* __restrict__ mrho, const float* __restrict__ mx0r, int nmat,
+                              int NJ, float* __restrict__ out3, uint8_t* __restrict__ valid) {
+    const int total = nmat * T.NE * NJ;
+    const float lo = logf_(DR_LO), hi = logf_(DR_HI);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int j = i % NJ, e = (i / NJ) % T.NE, m = i / (NJ * T.NE);
+        const float E = ecenter(T.ee, e);
+        const float dR = expf_(lo + (hi - lo) * (float)j / (float)(NJ - 1));
+        const float dRprev = expf_(lo + (hi - lo) * (float)max(j - 1, 0) / (float)(NJ - 1));
+        const float rho = mrho[m], x0r = mx0r[m], ds = dR / rho;
+        float x[4];
+        const bool reach = dRprev / rho <= 1.5f * max_step(T, E, rho, 0);
+        valid[i] = (reach && encode_input(E, dR, rho, x0r, x) && ds / (X0W * x0r) >= 1e-6f) ? 1 : 0;
+        const float sE = straggling(E, ds, rho);
+        const float mdE = deposition(T, E, ds, rho);
+        const float Em = fmaxf(E - 0.5f * mdE, T.ee[0]);
+        const float sth = sqrtf_(fmaxf(scat_power(Em, ds, X0W * x0r) * ds, 0.0f));
+        out3[3 * (size_t)i + 0] = logf_(fmaxf(sE, 1e-30f));
+        out3[3 * (size_t)i + 1] = logf_(fmaxf(sth, 1e-30f));
+        out3[3 * (size_t)i + 2] = nuc_sigma(E) * rho * ds;
     }
 }
 

@@ -119,6 +119,8 @@ struct sm2d_ctx {
     DBuf<float> surr;
     DBuf<int> rowmat;
     int surr_nodes = 0, surr_nmat = 0, surr_on = 0, surr_drift = 0;
+    float surr_tol = 0.25f;   // validation tolerance of an attached network's table
+    float surr_stats[4] = {}; // last validation: nodes checked, flagged, max and RMS deviation
     std::vector<double> src;  // per spot: W in/out/slot, E in/out/slot, E representation loss
     uint32_t hint = 64;       // expected listed cells per phase (launch sizing only)
     uint32_t hint_ph[2] = {64, 64};  // listed cells of the last chunk, coarse / fine

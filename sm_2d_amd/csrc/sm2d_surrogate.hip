@@ -144,6 +144,78 @@ int mlp_upload_x(sm2d_mlp* M, const float* x, int B, int Bp) {
     return SM2D_OK;
 }
 
+// Compares the attached table with the plain physics at its nodes and sends every node whose
+// learned ln sigma_E or ln sigma_theta differs by more than `tol`, or whose attenuation exponent
+// a differs by more than tol x (a + 1e-4 / mm x dR), back to the plain physics (NaN entries).
+// stats4 = (nodes checked, nodes flagged, largest deviation, RMS deviation).
+int surrogate_validate(sm2d_ctx* c, float tol, float* stats4) {
+    const int Nz = c->g.Nz, NE = c->g.N_E, nmat = c->surr_nmat, NJ = c->surr_nodes;
+    std::vector<int> rows(Nz);
+    std::vector<float> rho(Nz), x0r(Nz);
+    HIPCHK(hipMemcpyAsync(rows.data(), c->rowmat.p, sizeof(int) * Nz, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(rho.data(), c->rho.p, sizeof(float) * Nz, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(x0r.data(), c->x0r.p, sizeof(float) * Nz, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<float> mr(nmat, 1.0f), mx(nmat, 1.0f);
+    std::vector<uint8_t> used(nmat, 0);
+    for (int z = 0; z < Nz; ++z)
+        if (rows[z] >= 0 && !used[rows[z]]) {
+            used[rows[z]] = 1;
+            mr[rows[z]] = rho[z];
+            mx[rows[z]] = x0r[z];
+        }
+    const size_t total = (size_t)nmat * NE * NJ;
+    DBuf<float> dmr, dmx, plain;
+    DBuf<uint8_t> valid;
+    HIPCHK(dmr.alloc(nmat));
+    HIPCHK(dmx.alloc(nmat));
+    HIPCHK(plain.alloc(total * 3));
+    HIPCHK(valid.alloc(total));
+    HIPCHK(hipMemcpyAsync(dmr.p, mr.data(), sizeof(float) * nmat, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dmx.p, mx.data(), sizeof(float) * nmat, hipMemcpyHostToDevice, c->stream));
+    k_table_plain<<<std::min<size_t>((total + 255) / 256, 8192), 256, 0, c->stream>>>(
+        c->tabs(), dmr.p, dmx.p, nmat, NJ, plain.p, valid.p);
+    HIPCHK(hipGetLastError());
+    std::vector<float> tab(total * 3), pl(total * 3);
+    std::vector<uint8_t> hv(total);
+    HIPCHK(hipMemcpyAsync(tab.data(), c->surr.p, sizeof(float) * tab.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(pl.data(), plain.p, sizeof(float) * pl.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(hv.data(), valid.p, total, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double n = 0, flagged = 0, mx_dev = 0, ss = 0;
+    for (size_t i = 0; i < total; ++i) {
+        const int m = (int)(i / ((size_t)NE * NJ));
+        if (!used[m] || !hv[i]) continue;
+        float* t = &tab[3 * i];
+        if (!(t[0] == t[0] && t[1] == t[1] && t[2] == t[2])) continue;  // already plain
+        const double d = std::max(std::fabs((double)t[0] - pl[3 * i]), std::fabs((double)t[1] - pl[3 * i + 1]));
+        // attenuation: relative error, floored at a cross-section of 1e-4 / mm (sigma_n vanishes
+        // below 5 MeV, where any small exponent is harmless)
+        const int j = (int)(i % (size_t)NJ);
+        const double dR = std::exp(std::log(1e-4) + (std::log(10.0) - std::log(1e-4)) * j / (double)(NJ - 1));
+        const double da = std::fabs((double)std::max(t[2], 0.0f) - pl[3 * i + 2]) / (pl[3 * i + 2] + 1e-4 * dR);
+        const double dev = std::max(d, da);
+        n += 1;
+        ss += dev * dev;
+        mx_dev = std::max(mx_dev, dev);
+        if (tol > 0.0f && dev > tol) {
+            t[0] = t[1] = t[2] = NAN;
+            flagged += 1;
+        }
+    }
+    if (flagged > 0) {
+        HIPCHK(hipMemcpyAsync(c->surr.p, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    c->surr_stats[0] = (float)n;
+    c->surr_stats[1] = (float)flagged;
+    c->surr_stats[2] = (float)mx_dev;
+    c->surr_stats[3] = n > 0 ? (float)std::sqrt(ss / n) : 0.0f;
+    if (stats4)
+        for (int k = 0; k < 4; ++k) stats4[k] = c->surr_stats[k];
+    return SM2D_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -347,11 +419,10 @@ int sm2d_mlp_evaluate(sm2d_mlp* M, sm2d_ctx* c, int n, uint32_t seed, float* err
     HIPCHK(hipMemcpyAsync(pr.data(), out3.p, sizeof(float) * pr.size(), hipMemcpyDeviceToHost, M->stream));
     HIPCHK(hipMemcpyAsync(lb.data(), M->w.labels.p, sizeof(float) * lb.size(), hipMemcpyDeviceToHost, M->stream));
     HIPCHK(hipStreamSynchronize(M->stream));
-    double s0 = 0, s1 = 0, s2 = 0, m0 = 0, m1 = 0, m2 = 0, amax = 0;
-    for (int i = 0; i < n; ++i) amax = std::max(amax, (double)std::exp(lb[3 * i + 2]) - 1e-9);
+    double s0 = 0, s1 = 0, s2 = 0, m0 = 0, m1 = 0, m2 = 0;
     for (int i = 0; i < n; ++i) {
         const double d0 = pr[3 * i] - lb[3 * i], d1 = pr[3 * i + 1] - lb[3 * i + 1];
-        const double d2 = (pr[3 * i + 2] - (std::exp((double)lb[3 * i + 2]) - 1e-9)) / std::max(amax, 1e-30);
+        const double d2 = std::log((double)pr[3 * i + 2] + 1e-9) - (double)lb[3 * i + 2];
         s0 += d0 * d0;
         s1 += d1 * d1;
         s2 += d2 * d2;
@@ -429,6 +500,20 @@ int sm2d_surrogate_attach(sm2d_ctx* c, sm2d_mlp* M, int n_nodes) {
     c->surr_nodes = n_nodes;
     c->surr_nmat = nmat;
     c->surr_on = 1;
+    return surrogate_validate(c, c->surr_tol, nullptr);
+}
+
+int sm2d_surrogate_check(sm2d_ctx* c, float tol, float* stats4) {
+    if (!c || !c->surr.p) return err("sm2d_surrogate_check: no surrogate table attached");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->surr_tol = tol;
+    return surrogate_validate(c, tol, stats4);
+}
+
+int sm2d_surrogate_stats(sm2d_ctx* c, float* stats4) {
+    if (!c || !stats4) return err("sm2d_surrogate_stats: null argument");
+    for (int k = 0; k < 4; ++k) stats4[k] = c->surr_stats[k];
     return SM2D_OK;
 }
 

@@ -282,6 +282,19 @@ class Engine:
         """Builds the learned-physics table of this context and turns it on."""
         self.L.call("surrogate_attach", self.h, mlp.h, n_nodes)
 
+    def surrogate_check(self, tol: float = 0.25):
+        """Re-validates the attached table against the plain physics; returns (nodes checked,
+        nodes rejected, largest deviation, RMS deviation)."""
+        st = np.zeros(4, np.float32)
+        self.L.call("surrogate_check", self.h, C.c_float(tol), _f(st))
+        return tuple(float(v) for v in st)
+
+    def surrogate_stats(self):
+        """Statistics of the last validation (attach or check)."""
+        st = np.zeros(4, np.float32)
+        self.L.call("surrogate_stats", self.h, _f(st))
+        return tuple(float(v) for v in st)
+
     def surrogate_enable(self, on: bool):
         self.L.call("surrogate_enable", self.h, 1 if on else 0)
 

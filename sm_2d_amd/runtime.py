@@ -32,6 +32,8 @@ HIP_PROTOS = {
     "surrogate_enable": [C.c_void_p, C.c_int],
     "surrogate_export": [C.c_void_p, abi.fp, C.c_int, abi.ip, abi.ip, abi.i32p],
     "surrogate_set_table": [C.c_void_p, abi.fp, C.c_int, C.c_int, abi.i32p],
+    "surrogate_check": [C.c_void_p, C.c_float, abi.fp],
+    "surrogate_stats": [C.c_void_p, abi.fp],
 }
 
 
