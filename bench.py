@@ -113,6 +113,72 @@ def batched_section(hip, n_spots: int, plain_sample: int) -> dict:
             "transport_tuner": tuner}
 
 
+def surrogate_section(hip) -> dict:
+    """The learned fine-transport physics against the plain physics on a 60 MeV water spot whose
+    whole track runs in the fine transport (where the learned widths act): one-time training and
+    table build, ms per spot of both, and the dose comparison (total deposited energy, 4 x 4
+    block means over blocks above 10 % of the maximum)."""
+    import numpy as np
+    import torch
+    from sm_2d_amd import problems
+    from sm_2d_amd.engine import Engine, Mlp
+    prob = problems.reference_grid_case(Nx=40, Nz=80, E0=60.0, sigma_x=2.0, n_samples=600,
+                                        max_iterations=400)
+    prob.transport.update(E_fine_on=300.0, E_fine_off=300.0)
+    eng = problems.setup_engine(hip, prob, Engine)
+    sp, tp = prob.make_spot(), prob.make_transport()
+    eng.run_spot(sp, tp)
+
+    def timed(n=3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            e, a = eng.run_spot(sp, tp)
+        torch.cuda.synchronize()
+        return e, a, (time.perf_counter() - t0) / n * 1e3
+
+    ep, ap, ms_plain = timed()
+    m = Mlp(hip, seed=11)
+    t0 = time.perf_counter()
+    cfg = dict(n_samples=1 << 21, batch=4096, steps=4000, lr=2e-3, w_mom=4.0, w_att=4.0)
+    hist = m.train(eng, n_hist=4, **cfg)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    ev = m.evaluate(eng, 65536, seed=123)
+    t0 = time.perf_counter()
+    eng.surrogate_attach(m, 64)
+    t_attach = time.perf_counter() - t0
+    st = eng.surrogate_stats()
+    eng.run_spot(sp, tp)
+    es, as_, ms_surr = timed()
+
+    def blk(d, b=4):
+        nz, nx = d.shape
+        return d[: nz // b * b, : nx // b * b].reshape(nz // b, b, nx // b, b).sum((1, 3))
+
+    bp, bs = blk(ep), blk(es)
+    mk = bp > 0.1 * bp.max()
+    rel = np.abs(bs - bp)[mk] / bp[mk]
+    m.close()
+    eng.close()
+    flops_train = 3 * 75008 * cfg["batch"] * cfg["steps"]
+    return {"workload": "60 MeV water spot, 40 x 80 cells @ 1 mm (reference grid), all energies in "
+                        "the fine transport, 600 source samples",
+            "train": {"samples": cfg["n_samples"], "steps": cfg["steps"], "batch": cfg["batch"],
+                      "seconds": t_train, "loss_first_last": [float(hist[0]), float(hist[-1])],
+                      "mfma_tflops_achieved": flops_train / t_train / 1e12,
+                      "heldout_rms_ln_sigmaE_sigmaTheta_lnA": [float(v) for v in ev[:3]]},
+            "table": {"build_s": t_attach, "nodes_checked": st[0], "nodes_rejected": st[1],
+                      "max_dev": st[2], "rms_dev": st[3]},
+            "ms_per_spot_plain": ms_plain, "ms_per_spot_learned": ms_surr,
+            "speedup": ms_plain / ms_surr,
+            "dose": {"total_rel": float(es.sum() / ep.sum() - 1.0),
+                     "block4_mean_rel": float(rel.mean()),
+                     "block4_p95_rel": float(np.percentile(rel, 95))},
+            "audit": {"surrogate_active": as_.surrogate_active, "fallback": as_.surrogate_fallback,
+                      "max_cell_W_closure": as_.max_cell_W_closure, "W_fail_iters": as_.W_fail_iters}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +189,7 @@ def main() -> int:
     ap.add_argument("--batch-spots", type=int, default=256)
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device to device three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -231,6 +298,8 @@ def main() -> int:
     eng.close()
     if not args.no_batch:
         out["batched"] = batched_section(hip, args.batch_spots, args.plain_sample)
+    if not args.no_surrogate:
+        out["learned_physics"] = surrogate_section(hip)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(prob, args.cpu_budget)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
