@@ -1117,6 +1117,75 @@ __device__ inline void fast_steps(FastLds& F, const WTabs<LY>& W, uint64_t m, do
     }
 }
 
+// Size of the per-workgroup owner table of the record rounds (hashed payload addresses).  Kept
+// at 2 KiB so that two 2-wave transport workgroups still fit one CU's LDS.
+constexpr int NOWN = 512;
+
+__device__ inline uint32_t own_slot(uint32_t addr) { return (addr * 2654435761u) >> 23; }
+
+// The same adds as fast_steps, record-parallel: lane l holds component l's records (distinct
+// addresses within a component).  Each round every pending record bids its index (component-major)
+// for its address's owner slot; the lowest bid per slot adds its weight and leaves.  Records of
+// one address therefore add in component order, one per round, as in the serial form; records
+// of addresses that share a slot only wait.  own[] holds ~0u between rounds.  The accumulator
+// lanes then take the components in order.
+template <class LY>
+__device__ inline void fast_rounds(FastLds& F, const WTabs<LY>& W, uint64_t m, double& accd, float& accf,
+                                   uint32_t* own) {
+    const int cnt = __popcll(m);
+    const int l = lane_id();
+    const int nr = (l < cnt) ? F.nrec[l] : 0;
+    uint32_t pend = 0;
+    for (int r = 0; r < nr; ++r)
+        if (F.recw[l * MAXREC + r] > 0.0f) pend |= 1u << r;
+    while (__ballot(pend != 0u)) {
+        for (uint32_t q = pend; q; q &= q - 1) {
+            const int r = __ffs(q) - 1;
+            atomicMin(&own[own_slot(F.rec[l * MAXREC + r])], (uint32_t)(l * MAXREC + r));
+        }
+        uint32_t done = 0;
+        for (uint32_t q = pend; q; q &= q - 1) {
+            const int r = __ffs(q) - 1;
+            const uint32_t ad = F.rec[l * MAXREC + r];
+            if (own[own_slot(ad)] == (uint32_t)(l * MAXREC + r)) {
+                __hip_atomic_fetch_add(W.rows + ad, F.recw[l * MAXREC + r], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
+                done |= 1u << r;
+            }
+        }
+        for (uint32_t q = pend; q; q &= q - 1) own[own_slot(F.rec[l * MAXREC + (__ffs(q) - 1)])] = ~0u;
+        pend &= ~done;
+    }
+    // accumulators: lanes 0..NKACC-1, components in order
+    if (l < NKACC) {
+        const bool fl = kacc_is_float(l);
+        int j = 0;
+        for (; j + 4 <= cnt; j += 4) {
+            double c[8];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                c[2 * u] = F.con[(j + u) * NCON + 2 * l];
+                c[2 * u + 1] = F.con[(j + u) * NCON + 2 * l + 1];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (fl) accf += (float)c[u];
+                else accd += c[u];
+            }
+        }
+        for (; j < cnt; ++j) {
+            const double c0 = F.con[j * NCON + 2 * l], c1 = F.con[j * NCON + 2 * l + 1];
+            if (fl) {
+                accf += (float)c0;
+                accf += (float)c1;
+            } else {
+                accd += c0;
+                accd += c1;
+            }
+        }
+    }
+}
+
 __device__ inline double rld(double v, int l) {
     const uint64_t u = __builtin_bit_cast(uint64_t, v);
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
@@ -1704,7 +1773,9 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
     __shared__ uint32_t s_bid[KB], s_ref[KB], s_cnt[KB], s_off[KB];
     __shared__ uint64_t s_mask[TW];
     __shared__ uint32_t s_round[2];
+    __shared__ uint32_t s_own[NOWN];
     extern __shared__ float s_tab[];
+    for (int i = threadIdx.x; i < NOWN; i += blockDim.x) s_own[i] = ~0u;
     Tabs T = Tg;
     {
         const int NE = Tg.NE;
@@ -1863,7 +1934,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                         FastLds& F = s_fast[bb - b0];
                         const uint64_t m = s_mask[bb - b0];
                         if (fast_claims<LY>(F, W, out, bk, flags)) {
-                            fast_steps<LY>(F, W, m, accd, accf);
+                            fast_rounds<LY>(F, W, m, accd, accf, s_own);
                             comps += (uint32_t)__popcll(m);
                         } else {
                             // the batch again, in wave 0, for the serial form

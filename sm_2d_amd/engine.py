@@ -61,13 +61,16 @@ class Library:
         if extra_protos:
             protos.update(extra_protos)
         abi.bind(self.lib, prefix, protos)
-        self.lib[prefix + "last_error"].restype = C.c_char_p
-        self.lib[prefix + "last_error"].argtypes = []
+        # attribute access caches the function object (item access would make a new one)
+        self._last_error = getattr(self.lib, prefix + "last_error")
+        self._last_error.restype = C.c_char_p
+        self._last_error.argtypes = []
 
     def call(self, name: str, *args) -> int:
         rc = getattr(self.lib, self.prefix + name)(*args)
         if rc != abi.OK:
-            msg = self.lib[self.prefix + "last_error"]().decode(errors="replace")
+            raw = self._last_error()
+            msg = raw.decode(errors="replace") if raw else ""
             raise LibError(f"{self.prefix}{name} failed ({rc}): {msg}")
         return rc
 
