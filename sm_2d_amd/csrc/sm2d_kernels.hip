@@ -579,17 +579,42 @@ struct WTabs {
     uint32_t cap;                 // LDS rows available
     float* rows;                  // this wave's LDS rows [cap][LB]
 
-    __device__ uint32_t gb(int t) const { return t == 0 ? b0 : t == 1 ? b1 : t == 2 ? b2 : t == 3 ? b3 : b4; }
-    __device__ uint32_t gr(int t) const { return t == 0 ? r0 : t == 1 ? r1 : t == 2 ? r2 : t == 3 ? r3 : r4; }
-    __device__ uint32_t gn(int t) const { return t == 0 ? n0 : t == 1 ? n1 : t == 2 ? n2 : t == 3 ? n3 : n4; }
-    __device__ void sb(int t, uint32_t v) {
-        if (t == 0) b0 = v; else if (t == 1) b1 = v; else if (t == 2) b2 = v; else if (t == 3) b3 = v; else b4 = v;
+    // Getters read every field and combine them with masks: a table-dependent branch or address
+    // would make the compiler keep the whole struct in scratch memory.
+    static __device__ uint32_t pick(int t, uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, uint32_t a4) {
+        const uint32_t m0 = 0u - (uint32_t)(t == 0), m1 = 0u - (uint32_t)(t == 1), m2 = 0u - (uint32_t)(t == 2),
+                       m3 = 0u - (uint32_t)(t == 3), m4 = 0u - (uint32_t)(t == 4);
+        return (a0 & m0) | (a1 & m1) | (a2 & m2) | (a3 & m3) | (a4 & m4);
     }
-    __device__ void sr(int t, uint32_t v) {
-        if (t == 0) r0 = v; else if (t == 1) r1 = v; else if (t == 2) r2 = v; else if (t == 3) r3 = v; else r4 = v;
+    __device__ uint32_t gb(int t) const { return pick(t, b0, b1, b2, b3, b4); }
+    __device__ uint32_t gr(int t) const { return pick(t, r0, r1, r2, r3, r4); }
+    __device__ uint32_t gn(int t) const { return pick(t, n0, n1, n2, n3, n4); }
+    // Setters write every field through a select (no store through a table-dependent address),
+    // so the compiler keeps the tables in registers.
+    static __device__ uint32_t blend(bool c, uint32_t v, uint32_t old) {
+        const uint32_t m = 0u - (uint32_t)c;
+        return (v & m) | (old & ~m);
+    }
+    __device__ void sb(int t, uint32_t v, bool on = true) {
+        b0 = blend(on && t == 0, v, b0);
+        b1 = blend(on && t == 1, v, b1);
+        b2 = blend(on && t == 2, v, b2);
+        b3 = blend(on && t == 3, v, b3);
+        b4 = blend(on && t == 4, v, b4);
+    }
+    __device__ void sr(int t, uint32_t v, bool on = true) {
+        r0 = blend(on && t == 0, v, r0);
+        r1 = blend(on && t == 1, v, r1);
+        r2 = blend(on && t == 2, v, r2);
+        r3 = blend(on && t == 3, v, r3);
+        r4 = blend(on && t == 4, v, r4);
     }
     __device__ void sn(int t, uint32_t v) {
-        if (t == 0) n0 = v; else if (t == 1) n1 = v; else if (t == 2) n2 = v; else if (t == 3) n3 = v; else n4 = v;
+        n0 = blend(t == 0, v, n0);
+        n1 = blend(t == 1, v, n1);
+        n2 = blend(t == 2, v, n2);
+        n3 = blend(t == 3, v, n3);
+        n4 = blend(t == 4, v, n4);
     }
 
     // Same search / claim / closest-block fallback as table_find_or_claim; a claimed slot gets a
@@ -624,10 +649,8 @@ struct WTabs {
                 for (int k = 0; k < LY::PL; ++k) gpool[(size_t)ref * LY::LB + l + 64 * k] = 0.0f;
                 __threadfence_block();
             }
-            if (l == s) {
-                sb(t, want);
-                sr(t, ref);
-            }
+            sb(t, want, l == s);
+            sr(t, ref, l == s);
             sn(t, n + 1);
             return s;
         }
