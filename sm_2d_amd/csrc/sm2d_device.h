@@ -127,8 +127,15 @@ __device__ inline double wave_sum(double v) {  // fixed butterfly: deterministic
 }
 
 // ---- tables ---------------------------------------------------------------------------------
-// device_find_bin_edges (src/cuda/device/device_lut.cuh:34-49)
-__device__ inline int find_bin(const float* __restrict__ e, int NE, float E) {
+// device_find_bin_edges (src/cuda/device/device_lut.cuh:34-49).  With a guess g >= 0 the bins
+// g, g-1, g-2 are tried first (strictly increasing edges: a bin that brackets E is the binary
+// search's answer), so lookups near a known bin need no search; the result never depends on g.
+__device__ inline int find_bin(const float* __restrict__ e, int NE, float E, int g = -1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int b = g - k;
+        if (g >= 0 && b >= 0 && b < NE && e[b] <= E && E < e[b + 1]) return b;
+    }
     if (E <= e[0]) return 0;
     if (E >= e[NE]) return NE - 1;
     int lo = 0, hi = NE;
@@ -142,22 +149,35 @@ __device__ inline float ecenter(const float* __restrict__ e, int b) {
     float lo = e[b], hi = e[b + 1];
     return lo + 1.00f * ((hi - lo) * 0.5f);
 }
-// device_lookup_R (device_lut.cuh:66-85)
-__device__ inline float lookup_R(const Tabs& T, float E) {
+// device_lookup_R (device_lut.cuh:66-85); g: bin guess (see find_bin)
+__device__ inline float lookup_R(const Tabs& T, float E, int g = -1) {
     const float Emin = T.ee[0], Emax = T.ee[T.NE];
     float Ec = fmaxf(Emin, fminf(E, Emax));
-    int b = find_bin(T.ee, T.NE, Ec);
+    int b = find_bin(T.ee, T.NE, Ec, g);
     if (b == T.NE - 1 || Ec >= Emax) return T.R[T.NE - 1];
     float lv = logf_(Ec);
     return expf_(T.lR[b] + (T.lR[b + 1] - T.lR[b]) * (lv - T.lE[b]) / (T.lE[b + 1] - T.lE[b]));
 }
-// device_lookup_E_inverse (device_lut.cuh:110-138): lower_bound bin, no upper clamp
-__device__ inline float E_inverse(const Tabs& T, float Rin) {
+// device_lookup_E_inverse (device_lut.cuh:110-138): lower_bound bin, no upper clamp.  With a
+// guess g whose R[g] >= Rin, the lower bound is found walking down from g (at most 3 steps, R
+// non-decreasing) before the binary search; the result never depends on g.
+__device__ inline float E_inverse(const Tabs& T, float Rin, int g = -1) {
     if (Rin <= 0.0f) return T.ee[0];
-    int lo = 0, hi = T.NE;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (T.R[mid] < Rin) lo = mid + 1; else hi = mid;
+    int lo = -1;
+    if (g >= 0 && g < T.NE && T.R[g] >= Rin) {
+#pragma unroll
+        for (int k = 0; k < 3 && lo < 0; ++k) {
+            const int c = g - k;
+            if (c == 0 || T.R[c - 1] < Rin) lo = c;
+        }
+    }
+    if (lo < 0) {
+        int l0 = 0, hi = T.NE;
+        while (l0 < hi) {
+            int mid = (l0 + hi) >> 1;
+            if (T.R[mid] < Rin) l0 = mid + 1; else hi = mid;
+        }
+        lo = l0;
     }
     int b = max(0, min(lo, T.NE - 2));
     float lv = logf_(Rin);
@@ -167,14 +187,15 @@ __device__ inline float E_inverse(const Tabs& T, float Rin) {
 }
 // device_compute_energy_deposition (device_lut.cuh:145-157); a geometric step ds in density
 // rho consumes rho*ds of water range.
-__device__ inline float deposition(const Tabs& T, float E, float ds, float rh) {
-    float Rn = lookup_R(T, E) - rh * ds;
-    float En = (Rn <= 0) ? T.ee[0] : E_inverse(T, Rn);
+// g: bin guess of E (the inverse search starts one bin above it)
+__device__ inline float deposition(const Tabs& T, float E, float ds, float rh, int g = -1) {
+    float Rn = lookup_R(T, E, g) - rh * ds;
+    float En = (Rn <= 0) ? T.ee[0] : E_inverse(T, Rn, g < 0 ? -1 : min(g + 1, T.NE - 1));
     return E - En;
 }
 // device_compute_max_step (device_lut.cuh:168-214) or the 2 % rule (step_control.hpp:18-66)
-__device__ inline float max_step(const Tabs& T, float E, float rh, int rule) {
-    float Rw = lookup_R(T, E) / rh;
+__device__ inline float max_step(const Tabs& T, float E, float rh, int rule, int g = -1) {
+    float Rw = lookup_R(T, E, g) / rh;
     if (rule == 1) {
         float d = 0.02f * Rw, f = 1.0f;
         if (E < 5.0f) { f = 0.2f; d = fminf(d, 0.1f); }

@@ -687,7 +687,7 @@ struct Bins {
     float tw0, tw1, Ecen;
 };
 
-__device__ inline Bins emission_bins(const Tabs& T, float theta, float E) {
+__device__ inline Bins emission_bins(const Tabs& T, float theta, float E, int eguess = -1) {
     const int NT = T.NT, NE = T.NE;
     const float tmin = T.te[0], tmax = T.te[NT];
     const float dth = (tmax - tmin) / NT;
@@ -707,7 +707,7 @@ __device__ inline Bins emission_bins(const Tabs& T, float theta, float E) {
         tw0 = 1.0f - tw1;
     }
     float Ec = fmaxf(T.ee[0], fminf(E, T.ee[NE]));
-    int eb = find_bin(T.ee, NE, Ec);
+    int eb = find_bin(T.ee, NE, Ec, eguess);
     eb = max(0, min(eb, NE - 1));
     b.bc = bc;
     b.tb1 = tb1;
@@ -1250,6 +1250,7 @@ inline size_t transport_dyn_lds(int NE) { return sizeof(float) * (4 * (size_t)NE
 struct CellGeo {
     int lc, iz, ix, Nx, NT, NE;
     float dx, dz, hdx, hdz, tmin, dth, rh, X0, A, B, sx0;
+    float R_on;  // water range at the fine-transport threshold energy (a launch constant)
     uint32_t latlisted;
 };
 
@@ -1262,6 +1263,7 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
     const float rh = G.rh, X0 = G.X0, A = G.A, B = G.B, sx0 = G.sx0;
     const uint32_t latlisted = G.latlisted;
     Comp r;
+    int eg = -1;  // energy bin of the component (a starting guess for the bin searches)
     r.kind = 2;
     r.face = -1;
     r.lidx = lidx_in;
@@ -1281,6 +1283,7 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
             ebin = max(0, min(ebin, NE - 1));
         }
         const float E = ecenter(T.ee, ebin);
+        eg = ebin;
         r.E = E;
         const bool pruned = FINE ? (weight < 1e-15f) : (weight < 1e-12f);
         if (pruned) {
@@ -1300,8 +1303,8 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
                 float s3 = (eta > 0) ? (hdx - xc) / eta : 1e30f;
                 float s4 = (eta < 0) ? (-hdx - xc) / eta : 1e30f;
                 float sb = fmaxf(fminf(fminf(s1, s2), fminf(s3, s4)), 0.0f);
-                ds = fminf(max_step(T, E, rh, P.rule), sb * 1.001f);
-                float mdE = deposition(T, E, ds, rh);
+                ds = fminf(max_step(T, E, rh, P.rule, ebin), sb * 1.001f);
+                float mdE = deposition(T, E, ds, rh, ebin);
                 float s_sE = 0.0f, s_sth = 0.0f, s_a = 0.0f;
                 const bool use_s = P.surr && ds > 0.0f && ds / X0 >= 1e-6f &&
                                    surrogate_lookup(T, ebin, iz, rh * ds, s_sE, s_sth, s_a);
@@ -1339,8 +1342,8 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
                 float crs = cstep;
                 bool split = false;
                 if (E > P.E_on + 1e-6f) {
-                    float Rc = lookup_R(T, E);
-                    float Rf = lookup_R(T, fmaxf(P.E_on, T.ee[0]));
+                    float Rc = lookup_R(T, E, ebin);
+                    float Rf = G.R_on;
                     float rtf = (Rc - Rf) / rh;
                     if (rtf > 0.0f && rtf < crs) {
                         crs = rtf;
@@ -1348,7 +1351,7 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
                         split = true;
                     }
                 }
-                dE = fminf(deposition(T, E, crs, rh), E);
+                dE = fminf(deposition(T, E, crs, rh, ebin), E);
                 E_new = E - dE;
                 edep = dE * weight;
                 float surv = expf_(-(nuc_sigma(E) * rh) * crs);
@@ -1404,18 +1407,46 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
             r.th_new = thn;
             r.x_new = xn;
             r.z_new = zn;
-            if (E_new > 0.1f) {
-                if (face == 0) {
-                    // device_emit_lateral_spread weights (device_bucket.cuh:880-916)
-                    float s = fmaxf(sxt, 1e-6f);
-                    float xmin = xn - 0.5f * 11.0f * dx;
-                    float prev = gcdf(xmin, xn, s), sum = 0.0f;
+            if (E_new > 0.1f && face <= 0) {
+                // Both spreads need Gaussian CDF values at up to 12 points; they are evaluated
+                // in one uniform loop so that a wave holding both kinds of component runs the
+                // CDF once per point, not once per branch.
+                // face 0: device_emit_lateral_spread weights (device_bucket.cuh:880-916), 11
+                // cells, points xmin + k dx (k = 0..11), width sxt.
+                // inside: device_gaussian_spread_weights_subcell + in-cell fraction and tails
+                // (k3_finetransport.cu:585-653), points -dx/2 + k dx/XS (k = 0..XS), width
+                // s, then +-dx/2 with width sx.
+                const bool f0 = face == 0;
+                const float s0w = fmaxf(sxt, 1e-6f);
+                const float s1w = fmaxf(FINE ? sxt : sx, 1e-6f);
+                const float dxs = dx / LY::XS;
+                const float xa = f0 ? xn - 0.5f * 11.0f * dx : -dx * 0.5f;
+                float g[12];
 #pragma unroll
-                    for (int q = 0; q < 11; ++q) {
-                        float cur = gcdf(xmin + (q + 1) * dx, xn, s);
-                        r.sw[q] = cur - prev;
-                        prev = cur;
+                for (int q = 0; q < 12; ++q) {
+                    float x, sq;
+                    if (f0) {
+                        x = q == 0 ? xa : xa + q * dx;
+                        sq = s0w;
+                    } else if (q <= LY::XS) {
+                        x = q == 0 ? xa : xa + q * dxs;
+                        sq = s1w;
+                    } else if (q == LY::XS + 1) {
+                        x = hdx;
+                        sq = sx;
+                    } else if (q == LY::XS + 2) {
+                        x = -hdx;
+                        sq = sx;
+                    } else {
+                        x = xn;
+                        sq = 1.0f;
                     }
+                    g[q] = gcdf(x, xn, sq);
+                }
+                if (f0) {
+                    float sum = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < 11; ++q) r.sw[q] = g[q + 1] - g[q];
 #pragma unroll
                     for (int q = 0; q < 11; ++q) sum += r.sw[q];
                     if (sum > 1e-10f) {
@@ -1438,20 +1469,11 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
 #pragma unroll
                         for (int q = 0; q < 11; ++q) r.sw[q] /= sum;
                     }
-                } else if (face < 0) {
-                    // device_gaussian_spread_weights_subcell + in-cell fraction and
-                    // tails (k3_finetransport.cu:585-653)
-                    float s = fmaxf(FINE ? sxt : sx, 1e-6f);
-                    float dxs = dx / LY::XS;
-                    float xmin = -dx * 0.5f;
-                    float prev = gcdf(xmin, xn, s), sum = 0.0f;
+                } else {
+                    float sum = 0.0f;
                     float w8[LY::XS];
 #pragma unroll
-                    for (int q = 0; q < LY::XS; ++q) {
-                        float cur = gcdf(xmin + (q + 1) * dxs, xn, s);
-                        w8[q] = cur - prev;
-                        prev = cur;
-                    }
+                    for (int q = 0; q < LY::XS; ++q) w8[q] = g[q + 1] - g[q];
 #pragma unroll
                     for (int q = 0; q < LY::XS; ++q) sum += w8[q];
                     if (sum > 1e-10f) {
@@ -1461,7 +1483,8 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
 #pragma unroll
                         for (int q = 0; q < LY::XS; ++q) w8[q] = (q == LY::XS / 2) ? 1.0f : 0.0f;
                     }
-                    float win = gcdf(hdx, xn, sx) - gcdf(-hdx, xn, sx);
+                    const float gp = g[LY::XS + 1], gm = g[LY::XS + 2];  // CDF at +dx/2 and -dx/2
+                    float win = gp - gm;
                     float wcf = fminf(1.0f, fmaxf(0.0f, win));
                     float wss = 0.0f;
 #pragma unroll
@@ -1478,8 +1501,8 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
                     for (int q = 0; q < LY::XS; ++q) r.sw[q] = (w8[q] / wss) * wcf;
                     float wl = 0.0f, wr = 0.0f;
                     if (wcf < 1.0f - 1e-6f) {
-                        float wlr = fmaxf(0.0f, gcdf(-hdx, xn, sx));
-                        float wrr = fmaxf(0.0f, 1.0f - gcdf(hdx, xn, sx));
+                        float wlr = fmaxf(0.0f, gm);
+                        float wrr = fmaxf(0.0f, 1.0f - gp);
                         float wtt = fmaxf(0.0f, 1.0f - wcf);
                         float wts = wlr + wrr;
                         if (wtt > 0.0f && wts > 1e-12f) {
@@ -1500,10 +1523,10 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
     r.i0 = r.i1 = r.i2 = 0;
     r.g0 = r.g1 = r.g2 = 0.0f;
     if (live && r.kind == 2 && r.E_new > 0.1f) {
-        r.bn = emission_bins(T, r.th_new, r.E_new);
+        r.bn = emission_bins(T, r.th_new, r.E_new, eg);
         const float w_new = r.w_new;
         if (r.face >= 0) {
-            r.g0 = ecenter(T.ee, find_bin(T.ee, NE, r.E_new));
+            r.g0 = ecenter(T.ee, find_bin(T.ee, NE, r.E_new, eg));
             if (r.face == 0) {
                 float drop = 0.0f, lat_bnd = 0.0f;
                 uint32_t unsup = 0;
@@ -1539,7 +1562,7 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
         } else {
             int tbn = (int)((r.th_new - tmin) / dth);
             tbn = (int)fmaxf(0, fminf(tbn, NT - 1));
-            const int ebn = find_bin(T.ee, NE, r.E_new);
+            const int ebn = find_bin(T.ee, NE, r.E_new, eg);
             r.i0 = tbn;
             r.i1 = ebn;
             r.i2 = min(max((int)(((r.z_new / dz) + 0.5f) * LY::ZS), 0), LY::ZS - 1);
@@ -1832,6 +1855,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
     G.hdz = T.dz * 0.5f;
     G.tmin = T.te[0];
     G.dth = (T.te[T.NT] - G.tmin) / T.NT;
+    G.R_on = lookup_R(T, fmaxf(P.E_on, T.ee[0]));
     for (uint32_t li = blockIdx.x; li < n; li += gridDim.x) {
         const int cell = (int)list[li];
         const int sp = cell / T.Ns;
@@ -1894,15 +1918,29 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
         __syncthreads();
         uint32_t s0 = 0;
         while (s0 < ns) {
-            // non-zero components per slot
-            for (uint32_t s = s0 + wv; s < ns; s += TW) {
-                uint32_t cnt = 0;
-                if (s_bid[s] != EMPTY) {
-                    const float* vin = in.pool + (size_t)s_ref[s] * LY::LB;
+            // non-zero components per slot (the payloads of SU slots per wave are fetched at once)
+            constexpr int SU = LY::PL >= 8 ? 1 : 8 / LY::PL;
+            for (uint32_t s = s0 + wv; s < ns; s += SU * TW) {
+                float v[SU][LY::PL];
 #pragma unroll
-                    for (int k = 0; k < LY::PL; ++k) cnt += (uint32_t)__popcll(__ballot(vin[l + 64 * k] > 0.0f));
+                for (int u = 0; u < SU; ++u) {
+                    const uint32_t su = s + (uint32_t)(u * TW);
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k) v[u][k] = 0.0f;
+                    if (su < ns && s_bid[su] != EMPTY) {
+                        const float* vin = in.pool + (size_t)s_ref[su] * LY::LB;
+#pragma unroll
+                        for (int k = 0; k < LY::PL; ++k) v[u][k] = vin[l + 64 * k];
+                    }
                 }
-                if (l == 0) s_cnt[s] = cnt;
+#pragma unroll
+                for (int u = 0; u < SU; ++u) {
+                    const uint32_t su = s + (uint32_t)(u * TW);
+                    uint32_t cnt = 0;
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k) cnt += (uint32_t)__popcll(__ballot(v[u][k] > 0.0f));
+                    if (l == 0 && su < ns) s_cnt[su] = cnt;
+                }
             }
             __syncthreads();
             if (threadIdx.x == 0) {  // the slots of this round: as many as fit the queue
@@ -1917,20 +1955,35 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
             }
             __syncthreads();
             const uint32_t s1 = s_round[0], total = s_round[1];
-            for (uint32_t s = s0 + wv; s < s1; s += TW) {
-                if (s_bid[s] == EMPTY) continue;
-                const float* vin = in.pool + (size_t)s_ref[s] * LY::LB;
-                uint32_t pos = s_off[s];
+            for (uint32_t s = s0 + wv; s < s1; s += SU * TW) {
+                float v[SU][LY::PL];
 #pragma unroll
-                for (int k = 0; k < LY::PL; ++k) {
-                    const float w = vin[l + 64 * k];
-                    const uint64_t mk = __ballot(w > 0.0f);
-                    if (w > 0.0f) {
-                        const uint32_t p = pos + (uint32_t)__popcll(mk & below);
-                        s_q[p] = (s << 16) | (uint32_t)(l + 64 * k);
-                        s_qw[p] = w;
+                for (int u = 0; u < SU; ++u) {
+                    const uint32_t su = s + (uint32_t)(u * TW);
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k) v[u][k] = 0.0f;
+                    if (su < s1 && s_bid[su] != EMPTY) {
+                        const float* vin = in.pool + (size_t)s_ref[su] * LY::LB;
+#pragma unroll
+                        for (int k = 0; k < LY::PL; ++k) v[u][k] = vin[l + 64 * k];
                     }
-                    pos += (uint32_t)__popcll(mk);
+                }
+#pragma unroll
+                for (int u = 0; u < SU; ++u) {
+                    const uint32_t su = s + (uint32_t)(u * TW);
+                    if (su >= s1) break;
+                    uint32_t pos = s_off[su];
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k) {
+                        const float w = v[u][k];
+                        const uint64_t mk = __ballot(w > 0.0f);
+                        if (w > 0.0f) {
+                            const uint32_t p = pos + (uint32_t)__popcll(mk & below);
+                            s_q[p] = (su << 16) | (uint32_t)(l + 64 * k);
+                            s_qw[p] = w;
+                        }
+                        pos += (uint32_t)__popcll(mk);
+                    }
                 }
             }
             __syncthreads();
