@@ -324,7 +324,8 @@ int sm2d_surrogate_set_table(sm2d_ctx* ctx, const float* table, int n_mat, int n
                              const int32_t* rowmat);
 
 /* --- self-test of the shared elementary functions on the device -------------------------- */
-/* Evaluates f(x[i]) for i < n on the GPU: which = 0 exp, 1 log, 2 sin, 3 cos, 4 erf. */
+/* Evaluates f(x[i]) for i < n on the GPU: which = 0 exp, 1 log, 2 sin, 3 cos, 4 erf, 5 sin and
+ * 6 cos through the shared-reduction sincos. */
 int sm2d_math_eval(int which, const float* x, float* y, int n);
 
 #ifdef __cplusplus

@@ -1399,6 +1399,18 @@ int orc_math_eval(int which, const float* x, float* y, int n) {
             case 2: y[i] = sinf_(x[i]); break;
             case 3: y[i] = cosf_(x[i]); break;
             case 4: y[i] = erff_(x[i]); break;
+            case 5: {  // sin, then cos, through the shared-reduction sincos
+                float sn, cs;
+                sm2d::sincosf_(x[i], sn, cs);
+                y[i] = sn;
+                break;
+            }
+            case 6: {
+                float sn, cs;
+                sm2d::sincosf_(x[i], sn, cs);
+                y[i] = cs;
+                break;
+            }
             default: return fail("bad function");
         }
     }

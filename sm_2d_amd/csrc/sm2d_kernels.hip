@@ -1294,7 +1294,8 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
             const float theta = FINE ? T.te[tbin] + 0.5f * dth : tmin + (tbin + 0.5f) * dth;
             const float xc = dx * ((float)(2 * LY::lx(lidx) + 1 - LY::XS) / (float)(2 * LY::XS));
             const float zc = dz * ((float)(2 * LY::lz(lidx) + 1 - LY::ZS) / (float)(2 * LY::ZS));
-            const float mu = cosf_(theta), eta = sinf_(theta);
+            float mu, eta;
+            sincosf_(theta, eta, mu);
             float ds, dE, E_new, edep, w_new, w_rem, E_rem, sx, sxt, sth;
             unsigned pseed;
             if (FINE) {
@@ -1389,7 +1390,8 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
                           (pseed * 747796405u);
             float tsc = (sth <= 0.0f) ? 0.0f : gauss(ts) * sth;
             float thn = fmaxf(T.te[0], fminf(theta + tsc, T.te[NT]));
-            float mun = cosf_(thn), etn = sinf_(thn);
+            float mun, etn;
+            sincosf_(thn, etn, mun);
             float xn = xc + etn * ds, zn = zc + mun * ds;
             int face = -1;
             if (zn >= hdz - 0.001f) face = 0;
@@ -2611,6 +2613,13 @@ __global__ void k_math_eval(int which, const float* x, float* y, int n) {
         case 1: y[i] = logf_(v); break;
         case 2: y[i] = sinf_(v); break;
         case 3: y[i] = cosf_(v); break;
+        case 5:
+        case 6: {
+            float sn, cs;
+            sincosf_(v, sn, cs);
+            y[i] = which == 5 ? sn : cs;
+            break;
+        }
         default: y[i] = erff_(v); break;
     }
 }

@@ -138,6 +138,16 @@ SM2D_HD inline double reduce_pio2(double x, int& quadrant) {
     return r;
 }
 
+// sin and cos together: one reduction, both polynomials, the quadrant picks signs and roles
+// (the values are those of sin_d and cos_d; no branch on the quadrant).
+SM2D_HD inline void sincos_d(double x, double& sn, double& cs) {
+    int q = 0;
+    const double r = reduce_pio2(x, q);
+    const double sp = sin_poly(r), cp = cos_poly(r);
+    sn = q == 0 ? sp : q == 1 ? cp : q == 2 ? -sp : -cp;
+    cs = q == 0 ? cp : q == 1 ? -sp : q == 2 ? -cp : sp;
+}
+
 SM2D_HD inline double sin_d(double x) {
     int q = 0;
     double r = reduce_pio2(x, q);
@@ -209,6 +219,12 @@ SM2D_HD inline float expf_(float x) { return (float)exp_d((double)x); }
 SM2D_HD inline float logf_(float x) { return (float)log_d((double)x); }
 SM2D_HD inline float sinf_(float x) { return (float)sin_d((double)x); }
 SM2D_HD inline float cosf_(float x) { return (float)cos_d((double)x); }
+SM2D_HD inline void sincosf_(float x, float& sn, float& cs) {
+    double s, c;
+    sincos_d((double)x, s, c);
+    sn = (float)s;
+    cs = (float)c;
+}
 SM2D_HD inline float erff_(float x) { return (float)erf_d((double)x); }
 SM2D_HD inline float sqrtf_(float x) { return (float)__builtin_sqrt((double)x); }
 

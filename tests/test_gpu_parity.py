@@ -31,7 +31,7 @@ def _pair(hip, orc, prob):
 
 def test_math_functions_bit_exact(hip, orc):
     rng = np.random.default_rng(1)
-    ranges = {0: (-80, 80), 1: (1e-12, 1e6), 2: (-7, 7), 3: (-7, 7), 4: (-7, 7)}
+    ranges = {0: (-80, 80), 1: (1e-12, 1e6), 2: (-7, 7), 3: (-7, 7), 4: (-7, 7), 5: (-7, 7), 6: (-7, 7)}
     for which, (a, b) in ranges.items():
         if which == 1:
             x = np.exp(rng.uniform(np.log(a), np.log(b), 200_000)).astype(np.float32)

@@ -40,3 +40,12 @@ def test_special_values(orc):
     assert np.isneginf(orc.math_eval(1, np.array([0.0], np.float32))[0])
     assert orc.math_eval(0, np.array([-200.0], np.float32))[0] == 0.0
     del x
+
+
+def test_sincos_equals_sin_and_cos(orc):
+    """The shared-reduction sincos returns exactly sin_d and cos_d (all quadrants)."""
+    rng = np.random.default_rng(9)
+    x = np.concatenate([rng.uniform(-40.0, 40.0, 200_000),
+                        np.array([0.0, -0.0, np.pi / 4, np.pi / 2, np.pi, -np.pi / 2])]).astype(np.float32)
+    assert np.array_equal(orc.math_eval(5, x).view(np.uint32), orc.math_eval(2, x).view(np.uint32))
+    assert np.array_equal(orc.math_eval(6, x).view(np.uint32), orc.math_eval(3, x).view(np.uint32))
