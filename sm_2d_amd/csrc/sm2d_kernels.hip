@@ -114,21 +114,41 @@ __global__ __launch_bounds__(64) void k_mask(Tabs T, Tp P, Psi in, const uint32_
         const bool off = __ballot(mb != EMPTY && bE <= (uint32_t)P.b_off) != 0;
         double W64 = 0.0, E64 = 0.0;
         bool neg = false;
-        for (uint32_t s = 0; s < ns; ++s) {
-            const uint32_t b = rlu(mb, (int)s);
-            if (b == EMPTY) continue;
-            const float* v = in.pool + (size_t)rlu(mr, (int)s) * LY::LB;
-            const int bEs = (int)((b >> 12) & 0xFFF);
+        // slots in order; the payloads of SU slots are fetched together
+        constexpr int SU = LY::PL >= 8 ? 1 : 8 / LY::PL;
+        for (uint32_t s0 = 0; s0 < ns; s0 += SU) {
+            float v[SU][LY::PL];
+            int bEs[SU];
 #pragma unroll
-            for (int k = 0; k < LY::PL; ++k) {
-                const int lidx = l + 64 * k;
-                const float w = v[lidx];
-                if (w > 0.0f) {
-                    W64 += (double)w;
-                    const int eb = min(max(bEs * LY::EL + LY::le(lidx), 0), T.NE - 1);
-                    E64 += (double)w * (double)ecenter(T.ee, eb);
-                } else if (w < 0.0f || w != w) {
-                    neg = true;
+            for (int u = 0; u < SU; ++u) {
+                const uint32_t b = rlu(mb, (int)(s0 + u));  // EMPTY beyond ns
+                bEs[u] = b == EMPTY ? -1 : (int)((b >> 12) & 0xFFF);
+#pragma unroll
+                for (int k = 0; k < LY::PL; ++k) v[u][k] = 0.0f;
+                if (b != EMPTY) {
+                    const float* vp = in.pool + (size_t)rlu(mr, (int)(s0 + u)) * LY::LB;
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k) v[u][k] = vp[l + 64 * k];
+                }
+            }
+            float ec[SU][LY::PL];  // bin centres, fetched without waiting for the weights
+#pragma unroll
+            for (int u = 0; u < SU; ++u)
+#pragma unroll
+                for (int k = 0; k < LY::PL; ++k)
+                    ec[u][k] = ecenter(T.ee, min(max(bEs[u] * LY::EL + LY::le(l + 64 * k), 0), T.NE - 1));
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                if (bEs[u] < 0) continue;
+#pragma unroll
+                for (int k = 0; k < LY::PL; ++k) {
+                    const float w = v[u][k];
+                    if (w > 0.0f) {
+                        W64 += (double)w;
+                        E64 += (double)w * (double)ec[u][k];
+                    } else if (w < 0.0f || w != w) {
+                        neg = true;
+                    }
                 }
             }
         }
@@ -2314,19 +2334,38 @@ __global__ __launch_bounds__(64) void k_audit_cells(Tabs T, Psi out, const uint3
             const uint32_t mb = (l < (int)ns) ? out.bid[(size_t)c * KB + l] : EMPTY;
             const uint32_t mr = (l < (int)ns) ? out.sref[(size_t)c * KB + l] : 0u;
             double W = 0.0, E = 0.0;
-            for (uint32_t s = 0; s < ns; ++s) {
-                const uint32_t b = rlu(mb, (int)s);
-                if (b == EMPTY) continue;
-                const float* v = out.pool + (size_t)rlu(mr, (int)s) * LY::LB;
-                const int bE = (int)((b >> 12) & 0xFFF);
+            constexpr int SU = LY::PL >= 8 ? 1 : 8 / LY::PL;  // slots fetched together
+            for (uint32_t s0 = 0; s0 < ns; s0 += SU) {
+                float v[SU][LY::PL];
+                int bEs[SU];
 #pragma unroll
-                for (int k = 0; k < LY::PL; ++k) {
-                    const int lidx = l + 64 * k;
-                    const float w = v[lidx];
-                    if (w <= 0.0f) continue;
-                    W += (double)w;
-                    const int eb = min(max(bE * LY::EL + LY::le(lidx), 0), T.NE - 1);
-                    E += (double)w * (double)ecenter(T.ee, eb);
+                for (int u = 0; u < SU; ++u) {
+                    const uint32_t b = rlu(mb, (int)(s0 + u));  // EMPTY beyond ns
+                    bEs[u] = b == EMPTY ? -1 : (int)((b >> 12) & 0xFFF);
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k) v[u][k] = 0.0f;
+                    if (b != EMPTY) {
+                        const float* vp = out.pool + (size_t)rlu(mr, (int)(s0 + u)) * LY::LB;
+#pragma unroll
+                        for (int k = 0; k < LY::PL; ++k) v[u][k] = vp[l + 64 * k];
+                    }
+                }
+                float ec[SU][LY::PL];  // bin centres, fetched without waiting for the weights
+#pragma unroll
+                for (int u = 0; u < SU; ++u)
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k)
+                        ec[u][k] = ecenter(T.ee, min(max(bEs[u] * LY::EL + LY::le(l + 64 * k), 0), T.NE - 1));
+#pragma unroll
+                for (int u = 0; u < SU; ++u) {
+                    if (bEs[u] < 0) continue;
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k) {
+                        const float w = v[u][k];
+                        if (w <= 0.0f) continue;
+                        W += (double)w;
+                        E += (double)w * (double)ec[u][k];
+                    }
                 }
             }
             r[R_WOUT] = wave_sum(W);
