@@ -637,6 +637,41 @@ struct WTabs {
         n4 = blend(t == 4, v, n4);
     }
 
+    // psi_out slots that already hold weight are staged in LDS rows as well and written back
+    // when the cell ends: slot s takes row s while rows remain (used is 0 here).  The payloads
+    // of SU slots are fetched together.
+    __device__ void stage_own(const Psi& out) {
+        const int l = lane_id();
+        constexpr int SU = LY::PL >= 8 ? 1 : 8 / LY::PL;
+        const uint32_t n = own0 < cap ? own0 : cap;
+        for (uint32_t s0 = 0; s0 < n; s0 += SU) {
+            float v[SU][LY::PL];
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const uint32_t sl = s0 + (uint32_t)u;
+                if (sl < n) {
+                    const float* src = out.pool + (size_t)rlu(r0, (int)sl) * LY::LB;
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k) v[u][k] = src[l + 64 * k];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const uint32_t sl = s0 + (uint32_t)u;
+                if (sl < n) {
+                    float* row = rows + (size_t)sl * LY::LB;
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k) row[l + 64 * k] = v[u][k];
+                }
+            }
+        }
+        if ((uint32_t)l < n) {
+            orig = r0;
+            r0 = CACHED | (uint32_t)l;
+        }
+        used = n;
+    }
+
     // Same search / claim / closest-block fallback as table_find_or_claim; a claimed slot gets a
     // zeroed LDS row while rows remain, else a zeroed global payload.
     // fallback = false: a full table without `want` returns -2 instead of the closest block.
@@ -1923,19 +1958,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
             W.cap = (uint32_t)LY::NL;
             W.rows = s_rows;
             W.orig = 0u;
-            // psi_out slots that already hold weight (the coarse gather into a fine cell) are
-            // staged in LDS rows as well and written back at the end of the cell
-            for (uint32_t s = 0; s < W.own0 && W.used < (uint32_t)LY::NL; ++s) {
-                const float* src = out.pool + (size_t)rlu(W.r0, (int)s) * LY::LB;
-                float* row = W.rows + (size_t)W.used * LY::LB;
-#pragma unroll
-                for (int k = 0; k < LY::PL; ++k) row[l + 64 * k] = src[l + 64 * k];
-                if (l == (int)s) {
-                    W.orig = W.r0;
-                    W.r0 = CACHED | W.used;
-                }
-                W.used = W.used + 1;
-            }
+            W.stage_own(out);  // e.g. the coarse gather into a fine cell
         }
         __syncthreads();
         uint32_t s0 = 0;
@@ -2157,17 +2180,7 @@ __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const ui
         W.cap = (uint32_t)NLG;
         W.rows = s_rows;
         W.orig = 0u;
-        for (uint32_t s = 0; s < W.own0 && W.used < W.cap; ++s) {
-            const float* src = out.pool + (size_t)rlu(W.r0, (int)s) * LY::LB;
-            float* row = W.rows + (size_t)W.used * LY::LB;
-#pragma unroll
-            for (int k = 0; k < LY::PL; ++k) row[l + 64 * k] = src[l + 64 * k];
-            if (l == (int)s) {
-                W.orig = W.r0;
-                W.r0 = CACHED | W.used;
-            }
-            W.used = W.used + 1;
-        }
+        W.stage_own(out);
         // the four faces' bucket tables (lane s < 32: slot s)
         uint32_t fb[4], fr[4], nf[4];
 #pragma unroll
