@@ -207,7 +207,9 @@ int sm2d_reset(sm2d_ctx* ctx);
  * k1k6_pipeline.cu:300-551) into psi_in; accounting goes to *audit (may be NULL). */
 int sm2d_inject(sm2d_ctx* ctx, const sm2d_spot* spot, sm2d_audit* audit);
 /* Loads psi_in from rows (cell, slot, bid, lidx, weight); rows of one (cell, slot) must
- * carry the same bid, slots of one cell must form a prefix 0..n-1. */
+ * carry the same bid, slots of one cell must form a prefix 0..n-1, and every row's global
+ * (theta, E) bin must lie on the grid (SM2D_ERR_ARG otherwise: the local bins of a partial
+ * last block past N_theta or N_E have no edges). */
 int sm2d_psi_load(sm2d_ctx* ctx, int n_rows, const int32_t* cell, const int32_t* slot,
                   const uint32_t* bid, const int32_t* lidx, const float* w);
 /* Exports psi_in: bids[Nx*Nz*32] (may be NULL) and the rows with weight != 0 in

@@ -655,7 +655,7 @@ struct Phase {
         Psi& po = c->pout();
         CellAcc& a = acc[li];
         const sm2d_grid& g = c->g;
-        const int Nx = g.Nx, Nz = g.Nz, NT = g.N_theta, NE = g.N_E;
+        const int Nx = g.Nx, NT = g.N_theta, NE = g.N_E;
         const float dx = g.dx, dz = g.dz;
         const float* te = c->th_edges.data();
         const float* ee = c->E_edges.data();
@@ -1628,6 +1628,11 @@ int orc_psi_load(orc_ctx* c, int n, const int32_t* cell, const int32_t* slot, co
         int cl = cell[i], s = slot[i];
         if (cl < 0 || cl >= c->N || s < 0 || s >= KB || lidx[i] < 0 || lidx[i] >= c->LB)
             return fail("psi row out of range");
+        int t, e, x, z;
+        c->dec(lidx[i], t, e, x, z);
+        if ((bid[i] >> 24) != 0 || (int)(bid[i] & 0xFFF) * c->Tl + t >= c->g.N_theta ||
+            (int)((bid[i] >> 12) & 0xFFF) * c->El + e >= c->g.N_E)
+            return fail("psi bin outside the grid");
         while (p.nslot[cl] <= s) {
             int k = p.nslot[cl];
             p.claim(cl, EMPTY);

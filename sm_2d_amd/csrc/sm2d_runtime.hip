@@ -1011,6 +1011,12 @@ int sm2d_psi_load(sm2d_ctx* c, int n, const int32_t* cell, const int32_t* slot, 
         int cl = cell[i], s = slot[i];
         if (cl < 0 || (size_t)cl >= N || s < 0 || s >= KB || lidx[i] < 0 || lidx[i] >= LB)
             return err("psi_load: row out of range");
+        // the bin must lie on the grid: a partial last block has local bins past N_theta or
+        // N_E, which no emission produces and whose energy centre has no upper edge
+        const int TL = c->g.n_theta_local, EL = c->g.n_e_local;
+        if ((bid[i] >> 24) != 0 || (int)(bid[i] & 0xFFF) * TL + lidx[i] % TL >= c->g.N_theta ||
+            (int)((bid[i] >> 12) & 0xFFF) * EL + (lidx[i] / TL) % EL >= c->g.N_E)
+            return err("psi_load: bin outside the grid");
         size_t k = (size_t)cl * KB + s;
         if (hb[k] != SM2D_EMPTY && hb[k] != bid[i]) return err("psi_load: rows of one slot carry different bids");
         if (where[k] < 0) {

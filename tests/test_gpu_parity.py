@@ -76,7 +76,22 @@ def test_injection_and_one_iteration_bit_exact(hip, orc, case):
     assert aug.cell_bin_updates == auo.cell_bin_updates
 
 
-def _random_psi(rng, N, LB, NE_blocks, NT_blocks, n_cells=200, max_slots=6):
+def _on_grid(gr, bid, LB):
+    """Local bins of block `bid` whose global (theta, E) bins lie on the grid (a partial last
+    block has local bins past N_theta or N_E, which psi_load rejects)."""
+    TL, EL = gr.n_theta_local, gr.n_e_local
+    l = np.arange(LB)
+    t = (bid & 0xFFF) * TL + l % TL
+    e = ((bid >> 12) & 0xFFF) * EL + (l // TL) % EL
+    return l[(t < gr.N_theta) & (e < gr.N_E)]
+
+
+def _blocks(gr):
+    return (gr.N_E + gr.n_e_local - 1) // gr.n_e_local, (gr.N_theta + gr.n_theta_local - 1) // gr.n_theta_local
+
+
+def _random_psi(rng, gr, N, LB, n_cells=200, max_slots=6):
+    NE_blocks, NT_blocks = _blocks(gr)
     cells = rng.choice(N, size=n_cells, replace=False)
     rows = {k: [] for k in ("cell", "slot", "bid", "lidx", "w")}
     for c in sorted(cells.tolist()):
@@ -85,8 +100,9 @@ def _random_psi(rng, N, LB, NE_blocks, NT_blocks, n_cells=200, max_slots=6):
         while len(bids) < ns:
             bids.add(int(rng.integers(0, NT_blocks)) | (int(rng.integers(0, NE_blocks)) << 12))
         for s, b in enumerate(sorted(bids)):
-            nl = int(rng.integers(1, 12))
-            for l in rng.choice(LB, size=nl, replace=False).tolist():
+            ok = _on_grid(gr, b, LB)
+            nl = min(int(rng.integers(1, 12)), len(ok))
+            for l in rng.choice(ok, size=nl, replace=False).tolist():
                 w = float(10 ** rng.uniform(-14, -2))
                 rows["cell"].append(c)
                 rows["slot"].append(s)
@@ -102,9 +118,7 @@ def test_mask_compact_random_bit_exact(hip, orc, layout):
     g, o = _pair(hip, orc, prob)
     LB = g.LB
     rng = np.random.default_rng(3)
-    NEb = (g.grid.N_E + g.grid.n_e_local - 1) // g.grid.n_e_local
-    NTb = g.grid.N_theta // g.grid.n_theta_local
-    rows = _random_psi(rng, g.N, LB, NEb, NTb)
+    rows = _random_psi(rng, g.grid, g.N, LB)
     prev = (rng.random(g.N) < 0.3).astype(np.uint8)
     for wmin in (1e-12, 1e-6):
         tp = prob.make_transport(weight_active_min=wmin)
@@ -125,9 +139,7 @@ def test_random_psi_iterations_bit_exact(hip, orc, layout):
     prob = problems.reference_grid_case(Nx=40, Nz=40, layout=layout)
     g, o = _pair(hip, orc, prob)
     rng = np.random.default_rng(11)
-    NEb = (g.grid.N_E + g.grid.n_e_local - 1) // g.grid.n_e_local
-    NTb = g.grid.N_theta // g.grid.n_theta_local
-    rows = _random_psi(rng, g.N, g.LB, NEb, NTb, n_cells=120, max_slots=8)
+    rows = _random_psi(rng, g.grid, g.N, g.LB, n_cells=120, max_slots=8)
     tp = prob.make_transport(sigma_x_initial=2.0)
     for e in (g, o):
         e.psi_load(rows)
@@ -245,9 +257,7 @@ def test_split_iteration_runs_match_one_run(hip, orc, split):
     prob = problems.reference_grid_case(Nx=40, Nz=40)
     g, o = _pair(hip, orc, prob)
     rng = np.random.default_rng(11)
-    NEb = (g.grid.N_E + g.grid.n_e_local - 1) // g.grid.n_e_local
-    NTb = g.grid.N_theta // g.grid.n_theta_local
-    rows = _random_psi(rng, g.N, g.LB, NEb, NTb, n_cells=120, max_slots=8)
+    rows = _random_psi(rng, g.grid, g.N, g.LB, n_cells=120, max_slots=8)
     tp = prob.make_transport(sigma_x_initial=2.0)
     for e in (g, o):
         e.psi_load(rows)
@@ -282,3 +292,68 @@ def test_chunk_graphs_replay_bit_exact(hip, orc):
     eg, ag = g.run_spot(sp, tp)
     assert np.array_equal(eg, eo)
     assert ag.iterations == ao.iterations
+
+
+def _saturating_psi(rng, g, LB):
+    """Cells spaced 7 apart in x and 3 in z (no lateral emission into another listed cell), each
+    with 32 occupied blocks over the upper half of the energy range and 8-39 weighted bins per
+    block: the transport's own and bucket tables fill (32 slots) and its LDS rows run out, so the
+    closest-block fallback and the serial apply with global payloads are taken."""
+    NEb, NTb = _blocks(g)
+    rows = {k: [] for k in ("cell", "slot", "bid", "lidx", "w")}
+    for c in [iz * g.Nx + ix for iz in range(1, g.Nz - 1, 3) for ix in range(3, g.Nx - 2, 7)]:
+        bids = set()
+        while len(bids) < 32:
+            bids.add(int(rng.integers(0, NTb)) | (int(rng.integers(NEb // 2, NEb)) << 12))
+        for s, b in enumerate(sorted(bids)):
+            ok = _on_grid(g, b, LB)
+            for lidx in rng.choice(ok, size=min(int(rng.integers(8, 40)), len(ok)), replace=False).tolist():
+                rows["cell"].append(c)
+                rows["slot"].append(s)
+                rows["bid"].append(b)
+                rows["lidx"].append(lidx)
+                rows["w"].append(float(10 ** rng.uniform(-6, -2)))
+    return {k: np.asarray(v) for k, v in rows.items()}
+
+
+@pytest.mark.parametrize("layout", [problems.LAYOUT_REF, problems.LAYOUT_CFG])
+def test_saturated_slot_tables_bit_exact(hip, orc, layout):
+    prob = problems.reference_grid_case(Nx=42, Nz=12, layout=layout)
+    g, o = _pair(hip, orc, prob)
+    rows = _saturating_psi(np.random.default_rng(5), g.grid, g.LB)
+    tp = prob.make_transport(sigma_x_initial=2.0)
+    for e in (g, o):
+        e.psi_load(rows)
+    tg, _, ag = g.run_iterations(tp, 2)
+    to, _, ao = o.run_iterations(tp, 2)
+    bg, rg = g.psi_dump()
+    bo, ro = o.psi_dump()
+    assert np.array_equal(bg, bo)
+    assert _rows_key(rg) == _rows_key(ro)
+    assert np.array_equal(tg, to)
+    assert ag.cell_bin_updates == ao.cell_bin_updates
+    assert ag.W_transport_drop == pytest.approx(ao.W_transport_drop, rel=1e-12, abs=1e-18)
+    if layout == problems.LAYOUT_CFG:  # the tables did fill
+        assert max(len(np.unique(ro["slot"][ro["cell"] == c])) for c in np.unique(ro["cell"])) == 32
+
+
+def test_psi_load_rejects_bins_off_the_grid(hip, orc):
+    """The 1029-bin energy grid in blocks of 4 ends in a partial block: its bins past N_E are
+    rejected by both loaders, the last bins on the grid load."""
+    from sm_2d_amd.engine import LibError
+    prob = problems.reference_grid_case(Nx=8, Nz=8, layout=problems.LAYOUT_CFG)
+    g, o = _pair(hip, orc, prob)
+    NEb, NTb = _blocks(g.grid)
+    b = (NTb - 1) | ((NEb - 1) << 12)
+    ok = _on_grid(g.grid, b, g.LB)
+    bad = np.setdiff1d(np.arange(g.LB), ok)
+    assert len(ok) > 0 and len(bad) > 0
+
+    def row(lidx):
+        return {"cell": np.array([9]), "slot": np.array([0]), "bid": np.array([b], np.uint32),
+                "lidx": np.array([lidx]), "w": np.array([1e-3])}
+
+    for e in (g, o):
+        e.psi_load(row(int(ok[-1])))
+        with pytest.raises(LibError, match="outside the grid"):
+            e.psi_load(row(int(bad[0])))

@@ -16,9 +16,10 @@ from sm_2d_amd import abi
 
 def _k1_engine(orc, Nx=4, Nz=4):
     from oracle.pyoracle import OracleEngine
-    # log grid 0.1-300 MeV / 256 bins, N_E_local = 4 as in the reference's K1 tests
-    tab = orc.make_tables(8, -0.35, 0.35, ("log", 0.1, 300.0, 256))
-    grid = abi.Grid(Nx=Nx, Nz=Nz, dx=1.0, dz=1.0, x_min=-2.0, z_min=0.0, N_theta=8,
+    # log grid 0.1-300 MeV / 256 bins, N_E_local = 4 as in the reference's K1 tests; 96 angular
+    # bins so that the tests' angular block 10 lies on the grid
+    tab = orc.make_tables(96, -0.35, 0.35, ("log", 0.1, 300.0, 256))
+    grid = abi.Grid(Nx=Nx, Nz=Nz, dx=1.0, dz=1.0, x_min=-2.0, z_min=0.0, N_theta=96,
                     theta_min=-0.35, theta_max=0.35, N_E=256, n_theta_local=8, n_e_local=4,
                     n_x_sub=4, n_z_sub=1)
     return OracleEngine(orc, grid, tab), tab
@@ -84,6 +85,28 @@ def test_compute_b_E_trigger_ranges(orc):
     b10 = (np.searchsorted(e, 10.0, side="right") - 1) // 4
     b50 = (np.searchsorted(e, 50.0, side="right") - 1) // 4
     assert 30 < b10 < 45 and 45 < b50 < 55 and b50 > b10
+
+
+def test_psi_load_rejects_bins_off_the_grid(orc):
+    """A block may reach past the grid (a partial last block); its local bins there have no
+    energy centre, so loading one is an error while the last bins on the grid load."""
+    from sm_2d_amd.engine import LibError
+    from oracle.pyoracle import OracleEngine
+    for n_e in (256, 255):  # 64 whole blocks of 4 / a last block with 3 bins on the grid
+        tab = orc.make_tables(96, -0.35, 0.35, ("log", 0.1, 300.0, n_e))
+        grid = abi.Grid(Nx=2, Nz=2, dx=1.0, dz=1.0, x_min=-1.0, z_min=0.0, N_theta=96,
+                        theta_min=-0.35, theta_max=0.35, N_E=n_e, n_theta_local=8, n_e_local=4,
+                        n_x_sub=4, n_z_sub=1)
+        e = OracleEngine(orc, grid, tab)
+        last_e = (n_e - 1) % 4  # local energy index of the last bin on the grid
+        e.psi_load(_row(0, 11, 63, 7 + 8 * last_e, 1.0))
+        with pytest.raises(LibError, match="outside the grid"):
+            e.psi_load(_row(0, 11, 64, 0, 1.0))
+        with pytest.raises(LibError, match="outside the grid"):
+            e.psi_load(_row(0, 12, 0, 0, 1.0))
+        if last_e < 3:
+            with pytest.raises(LibError, match="outside the grid"):
+                e.psi_load(_row(0, 11, 63, 8 * (last_e + 1), 1.0))
 
 
 def test_k5_drop_channel_kat(orc):
