@@ -183,6 +183,39 @@ int sm2d_material_upload(sm2d_ctx* ctx, const float* rho, const float* x0_ratio)
 int sm2d_run_spot(sm2d_ctx* ctx, const sm2d_spot* spot, const sm2d_transport* tp,
                   double* edep_out, sm2d_audit* audit);
 
+/* --- device memory planning ------------------------------------------------------------- */
+
+/* Device memory of a context, replacing the dense estimate and check of run_memory_preflight
+ * (src/perf/memory_preflight.cpp:43-160, called at src/gpu/gpu_transport_runner.cpp:111-139).
+ * The library's phase space is a sparse store with bounded capacities, so its plan lists
+ * those capacities and the exact bytes sm2d_ctx_create(_batch) allocates for them (the
+ * enumeration the allocation itself uses); dense_bytes is what the reference's dense store
+ * would need for the same grid.  Tables and networks of the learned physics are extra. */
+typedef struct sm2d_memory_plan {
+    uint32_t struct_size;
+    uint32_t listed_cells;    /* listed cells per phase and iteration */
+    uint32_t occupied_cells;  /* occupied cells (and destinations) per iteration */
+    uint32_t payload_slots;   /* payload slots of each ψ buffer and of the buckets */
+    uint64_t psi_bytes;       /* both ψ buffers: slot tables, slot counts, payload pools */
+    uint64_t bucket_bytes;    /* outflow buckets of the listed cells */
+    uint64_t cell_bytes;      /* per-cell masks, tags and tallies */
+    uint64_t other_bytes;     /* lists, audit rows, counters, grids and tables */
+    uint64_t total_bytes;
+    uint64_t dense_bytes;     /* estimate_dense_k1k6_memory of the reference for this grid */
+    uint64_t free_bytes;      /* sm2d_memory_preflight: free device memory */
+    uint64_t usable_bytes;    /* free_bytes x margin */
+    int ok;                   /* total_bytes <= usable_bytes (1 from sm2d_memory_estimate) */
+} sm2d_memory_plan;
+
+/* The plan for a grid and spot count; host only (no device is touched). */
+int sm2d_memory_estimate(const sm2d_grid* grid, int n_spots, sm2d_memory_plan* out);
+/* The plan checked against the device's free memory times margin (0 < margin <= 1, the
+ * reference's preflight_vram_margin); out->ok tells whether sm2d_ctx_create fits. */
+int sm2d_memory_preflight(int device, const sm2d_grid* grid, int n_spots, float margin,
+                          sm2d_memory_plan* out);
+/* Device bytes a context's creation allocated (equals its plan's total_bytes). */
+int sm2d_ctx_memory(sm2d_ctx* ctx, uint64_t* bytes);
+
 /* --- batched spots ------------------------------------------------------------------------ */
 
 /* Creates a context holding `n_spots` independent copies of the grid (spot s owns cells
@@ -198,6 +231,10 @@ int sm2d_ctx_create_batch(int device, const sm2d_grid* grid, int n_spots, sm2d_c
  * batch totals (iterations = the batch's iteration count). */
 int sm2d_run_batch(sm2d_ctx* ctx, const sm2d_spot* spots, int n, const sm2d_transport* tp,
                    double* edep_sum, float* edep_per_spot, sm2d_audit* audit);
+/* Spot s's FP64 EdepC[Nz*Nx] of the last sm2d_run_batch (0 <= s < its n), before any FP32
+ * rounding: the values sm2d_run_spot returns for that spot alone.  No counterpart in the
+ * reference (one process per spot there, batch_run.py:200-238). */
+int sm2d_batch_spot_edep(sm2d_ctx* ctx, int s, double* edep_out);
 
 /* --- stage-level entry points (used by the parity tests) -------------------------------- */
 

@@ -8,12 +8,19 @@
 #include <tuple>
 #include <vector>
 
+#include <cstdio>
+#include <cstring>
+#include <exception>
+
+#include "../include/sm2d_io.h"
 #include "core/block_encoding.hpp"
+#include "core/config_loader.hpp"
 #include "core/grids.hpp"
 #include "core/local_bins.hpp"
 #include "lut/r_lut.hpp"
 #include "physics/highland.hpp"
 #include "physics/step_control.hpp"
+#include "validation/deterministic_beam.hpp"
 
 extern "C" {
 
@@ -84,5 +91,77 @@ float ref_z_offset(int zs, float dz) { return get_z_offset_from_bin(zs, dz); }
 int ref_x_sub_bin(float off, float dx) { return get_x_sub_bin(off, dx); }
 int ref_z_sub_bin(float off, float dz) { return get_z_sub_bin(off, dz); }
 int ref_local_bins(void) { return LOCAL_BINS; }
+
+// The reference's INI loader (config_loader.hpp:281-416) into the library's flat struct.
+// Returns 0 (valid), 1 (loaded; validate() threw: msg holds its message) or 2 (load threw).
+int ref_config_load(const char* path, sm2d_run_config* o, char* msg, int cap) {
+    sm_2d::IncidentParticleConfig c;
+    try {
+        c = sm_2d::load_incident_particle_config(path);
+    } catch (const std::exception& e) {
+        std::snprintf(msg, cap, "%s", e.what());
+        return 2;
+    }
+    o->particle_type = (int)c.particle_type;
+    o->particle_mass_amu = c.particle_mass_amu;
+    o->particle_charge_e = c.particle_charge_e;
+    o->beam_profile = (int)c.beam_profile;
+    o->W_total = c.W_total;
+    o->E_mean = c.energy.mean_E0; o->E_sigma = c.energy.sigma_E;
+    o->E_min = c.energy.min_E; o->E_max = c.energy.max_E;
+    o->x0 = c.spatial.x0; o->z0 = c.spatial.z0; o->sigma_x = c.spatial.sigma_x;
+    o->sigma_z = c.spatial.sigma_z; o->width_x = c.spatial.width_x; o->width_z = c.spatial.width_z;
+    o->radius = c.spatial.radius;
+    o->theta0 = c.angular.theta0; o->phi0 = c.angular.phi0;
+    o->sigma_theta = c.angular.sigma_theta; o->sigma_phi = c.angular.sigma_phi;
+    o->n_samples = c.sampling.n_samples; o->random_seed = c.sampling.random_seed;
+    o->use_stratified = c.sampling.use_stratified;
+    o->Nx = c.grid.Nx; o->Nz = c.grid.Nz; o->dx = c.grid.dx; o->dz = c.grid.dz;
+    o->max_steps = c.grid.max_steps;
+    const sm_2d::TransportConfig& t = c.transport;
+    o->N_theta = t.N_theta; o->N_theta_local = t.N_theta_local; o->N_E_local = t.N_E_local;
+    o->n_energy_groups = (int)t.energy_groups.size();
+    for (int k = 0; k < o->n_energy_groups && k < SM2D_MAX_ENERGY_GROUPS; ++k) {
+        o->energy_groups[k][0] = t.energy_groups[k].E_min_MeV;
+        o->energy_groups[k][1] = t.energy_groups[k].E_max_MeV;
+        o->energy_groups[k][2] = t.energy_groups[k].dE_MeV;
+    }
+    o->E_fine_on = t.E_fine_on; o->E_fine_off = t.E_fine_off; o->E_trigger = t.E_trigger;
+    o->weight_active_min = t.weight_active_min; o->E_coarse_max = t.E_coarse_max;
+    o->step_coarse = t.step_coarse; o->n_steps_per_cell = t.n_steps_per_cell;
+    o->fine_batch_max_cells = t.fine_batch_max_cells; o->fine_halo_cells = t.fine_halo_cells;
+    o->preflight_vram_margin = t.preflight_vram_margin; o->validation_mode = t.validation_mode;
+    o->fail_fast_on_audit = t.fail_fast_on_audit; o->debug_dumps = t.debug_dumps;
+    o->max_iterations = t.max_iterations; o->log_level = t.log_level;
+    std::snprintf(o->output_dir, SM2D_PATH_MAX, "%s", c.output.output_dir.c_str());
+    std::snprintf(o->dose_2d_file, SM2D_PATH_MAX, "%s", c.output.dose_2d_file.c_str());
+    std::snprintf(o->pdd_file, SM2D_PATH_MAX, "%s", c.output.pdd_file.c_str());
+    std::snprintf(o->let_file, SM2D_PATH_MAX, "%s", c.output.let_file.c_str());
+    std::snprintf(o->format, sizeof o->format, "%s", c.output.format.c_str());
+    o->normalize_dose = c.output.normalize_dose; o->save_2d = c.output.save_2d;
+    o->save_pdd = c.output.save_pdd; o->save_lat_profiles = c.output.save_lat_profiles;
+    try {
+        c.validate();
+    } catch (const std::exception& e) {
+        std::snprintf(msg, cap, "%s", e.what());
+        return 1;
+    }
+    return 0;
+}
+
+// get_depth_dose and find_bragg_peak_z (src/validation/deterministic_beam.cpp:207-240).
+int ref_depth_dose(const double* edep, int Nx, int Nz, double* dd, int* peak) {
+    SimulationResult r;
+    r.Nx = Nx;
+    r.Nz = Nz;
+    r.dx = r.dz = 1.0f;
+    r.edep.assign(Nz, std::vector<double>(Nx));
+    for (int j = 0; j < Nz; ++j)
+        for (int i = 0; i < Nx; ++i) r.edep[j][i] = edep[(size_t)j * Nx + i];
+    std::vector<double> d = get_depth_dose(r);
+    for (int j = 0; j < Nz; ++j) dd[j] = d[j];
+    *peak = find_bragg_peak_z(r);
+    return 0;
+}
 
 }  // extern "C"

@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsm2d_hip.so")
+RUNNER = os.path.join(HERE, "sm2d_run")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SM2D_ARCH", "gfx950")
 
@@ -34,17 +35,23 @@ def _run(cmd: list[str]) -> None:
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     host_o = os.path.join(BUILD, "sm2d_host.o")
+    io_o = os.path.join(BUILD, "sm2d_io.o")
     dev_o = os.path.join(BUILD, "sm2d_runtime.o")
     steps = [
         ["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, "sm2d_host.cpp"), "-o", host_o],
+        ["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, "sm2d_io.cpp"), "-o", io_o],
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "sm2d_runtime.hip"), "-o", dev_o],
-        [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", host_o, dev_o, "-o", LIB + ".tmp"],
+        [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", host_o, io_o, dev_o, "-o", LIB + ".tmp"],
     ]
     for cmd in steps:
         if verbose:
             print(" ".join(cmd), flush=True)
         _run(cmd)
     os.replace(LIB + ".tmp", LIB)
+    # the INI driver (run_simulation's counterpart), linked against the library beside it
+    _run(["g++", *HOST_FLAGS, os.path.join(CSRC, "sm2d_run.cpp"), "-o", RUNNER + ".tmp",
+          "-L" + HERE, "-lsm2d_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(RUNNER + ".tmp", RUNNER)
     return LIB
 
 

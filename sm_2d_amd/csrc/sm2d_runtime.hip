@@ -44,6 +44,7 @@ template <class T>
 struct DBuf {
     T* p = nullptr;
     size_t n = 0;
+    size_t bytes_for(size_t count) const { return sizeof(T) * (count ? count : 1); }
     hipError_t alloc(size_t count) {
         if (p) (void)hipFree(p);
         p = nullptr;
@@ -99,6 +100,8 @@ struct sm2d_ctx {
     LayoutId layout = L_REF;
     int LB = 256, S = 1, Ns = 0, N = 0;
     uint32_t pcap = 0, lcap = 0, dcap = 0, ocap = 0, nwords = 0;
+    uint64_t bytes = 0;  // device bytes of the buffers sm2d_ctx_create allocated
+    int batch_n = 0;     // spots of the last sm2d_run_batch (their EdepC stays readable)
     bool tables = false;
     DBuf<float> te, ee, R, Sp, lE, lR, lS, feA, feB, sx0, rho, x0r;
     std::vector<float> h_ee;
@@ -772,19 +775,97 @@ __global__ void k_edep_reduce(const double* edep, int n_spots, int Ns, double* s
     }
 }
 
-int create(int device, const sm2d_grid* g, int n_spots, sm2d_ctx** out) {
-    if (!g || !out || g->struct_size != sizeof(sm2d_grid)) return err("sm2d_ctx_create: grid struct_size");
+// Capacities: listed cells per phase <= 48 rows of every spot (24 in a batch, whose spots are
+// not all at their widest at once); occupied cells and destinations <= 5x that; payload slots
+// <= 16 (8 in a batch) per listed cell, at least 64 Ki.  An overflow stops the transport and
+// is reported as SM2D_ERR_CAPACITY.
+void plan_caps(sm2d_ctx* c) {
+    const size_t N = (size_t)c->N, S = (size_t)c->S;
+    const size_t rows = S > 1 ? 24 : 48, per_listed = S > 1 ? 8 : 16;
+    c->lcap = (uint32_t)std::min<size_t>(N, S * rows * c->g.Nx + 1024);
+    c->ocap = (uint32_t)std::min<size_t>(N, 5ull * c->lcap);
+    c->dcap = c->ocap;
+    c->pcap = (uint32_t)std::min<size_t>(N * KB, std::max<size_t>(65536, per_listed * c->lcap));
+    c->nwords = (uint32_t)((N + 31) / 32);
+}
+
+enum BufKind { BK_PSI, BK_BUCKET, BK_CELL, BK_OTHER, NBK };
+
+// Every device buffer of a context with its element count and kind, in allocation order.
+// `a(buffer, count, kind)` allocates it (sm2d_ctx_create) or only counts its bytes
+// (sm2d_memory_estimate), so the estimate is the allocation itself.
+template <class A>
+int each_buffer(sm2d_ctx* c, A&& a) {
+    const size_t N = (size_t)c->N, Ns = (size_t)c->Ns, S = (size_t)c->S;
+    const size_t NT = (size_t)c->g.N_theta, NE = (size_t)c->g.N_E, Nz = (size_t)c->g.Nz;
+    for (int k = 0; k < 2; ++k) {
+        HIPCHK(a(c->bid[k], N * KB, BK_PSI));
+        HIPCHK(a(c->ns[k], N, BK_PSI));
+        HIPCHK(a(c->sref[k], N * KB, BK_PSI));
+        HIPCHK(a(c->pcount[k], 1, BK_PSI));
+        HIPCHK(a(c->pool[k], (size_t)c->pcap * c->LB, BK_PSI));
+        HIPCHK(a(c->occ[k], c->ocap, BK_PSI));
+    }
+    HIPCHK(a(c->bitmap, c->nwords, BK_OTHER));
+    HIPCHK(a(c->bbid, (size_t)c->lcap * 4 * KB, BK_BUCKET));
+    HIPCHK(a(c->bn, (size_t)c->lcap * 4, BK_BUCKET));
+    HIPCHK(a(c->bsref, (size_t)c->lcap * 4 * KB, BK_BUCKET));
+    HIPCHK(a(c->bpool, (size_t)c->pcap * c->LB, BK_BUCKET));
+    for (DBuf<uint32_t>* b : {&c->maskv, &c->ltag, &c->lpos, &c->dtag}) HIPCHK(a(*b, N, BK_CELL));
+    HIPCHK(a(c->act, c->ocap, BK_OTHER));
+    HIPCHK(a(c->crs, c->ocap, BK_OTHER));
+    HIPCHK(a(c->dlist, c->dcap, BK_OTHER));
+    HIPCHK(a(c->ctl, NCTL, BK_OTHER));
+    HIPCHK(a(c->tcount, 2 * ((size_t)c->ocap / 256 + 2), BK_OTHER));
+    HIPCHK(a(c->tcount2, (size_t)c->nwords / 256 + 2, BK_OTHER));
+    HIPCHK(a(c->flags, NFLAG, BK_OTHER));
+    HIPCHK(a(c->flag, c->ocap, BK_OTHER));
+    for (DBuf<double>* b : {&c->edep, &c->E_cut, &c->E_nuc, &c->E_bnd, &c->k4w, &c->k4e})
+        HIPCHK(a(*b, N, BK_CELL));
+    HIPCHK(a(c->win, c->ocap, BK_OTHER));
+    HIPCHK(a(c->ein, c->ocap, BK_OTHER));
+    HIPCHK(a(c->acc_c, (size_t)c->ocap * NACC, BK_OTHER));
+    HIPCHK(a(c->acc_f, (size_t)c->ocap * NACC, BK_OTHER));
+    HIPCHK(a(c->pe, (size_t)c->ocap * NR1, BK_OTHER));
+    HIPCHK(a(c->part, ((size_t)c->ocap / 256 + 2) * NR1, BK_OTHER));
+    HIPCHK(a(c->k5v, NV, BK_OTHER));
+    HIPCHK(a(c->hist, (size_t)HROWS * NV, BK_OTHER));
+    HIPCHK(a(c->edep_sum, Ns, BK_OTHER));
+    for (DBuf<float>* b : {&c->w_cut, &c->w_nuc, &c->w_bnd}) HIPCHK(a(*b, N, BK_CELL));
+    HIPCHK(a(c->spots, S, BK_OTHER));
+    HIPCHK(a(c->injacc, S * 6, BK_OTHER));
+    HIPCHK(a(c->te, NT + 1, BK_OTHER));
+    HIPCHK(a(c->ee, NE + 1, BK_OTHER));
+    for (DBuf<float>* b : {&c->R, &c->Sp, &c->lE, &c->lR, &c->lS}) HIPCHK(a(*b, NE, BK_OTHER));
+    HIPCHK(a(c->feA, S * Nz, BK_OTHER));
+    HIPCHK(a(c->feB, S * Nz, BK_OTHER));
+    HIPCHK(a(c->sx0, S, BK_OTHER));
+    HIPCHK(a(c->rho, Nz, BK_OTHER));
+    HIPCHK(a(c->x0r, Nz, BK_OTHER));
+    return SM2D_OK;
+}
+
+// Grid and layout checks of a context (shared by the estimate and the creation).
+int check_grid(const sm2d_grid* g, int n_spots, LayoutId* lid) {
+    if (!g || g->struct_size != sizeof(sm2d_grid)) return err("sm2d_ctx_create: grid struct_size");
     if (g->Nx <= 0 || g->Nz <= 0 || g->N_theta <= 0 || g->N_E <= 0 || n_spots <= 0 ||
         g->N_theta > 4095 * g->n_theta_local)
         return err("sm2d_ctx_create: bad grid");
     // the transport stages the energy tables in LDS next to its payload rows (64 KiB per wave)
     if (g->N_E > 1100) return err("sm2d_ctx_create: N_E above 1100 is not supported", SM2D_ERR_UNSUPPORTED);
-    LayoutId lid;
-    if (g->n_theta_local == 4 && g->n_e_local == 2 && g->n_x_sub == 8 && g->n_z_sub == 4) lid = L_REF;
-    else if (g->n_theta_local == 8 && g->n_e_local == 4 && g->n_x_sub == 4 && g->n_z_sub == 1) lid = L_CFG;
+    if (g->n_theta_local == 4 && g->n_e_local == 2 && g->n_x_sub == 8 && g->n_z_sub == 4) *lid = L_REF;
+    else if (g->n_theta_local == 8 && g->n_e_local == 4 && g->n_x_sub == 4 && g->n_z_sub == 1) *lid = L_CFG;
     else return err("sm2d_ctx_create: local-bin layout must be (4,2,8,4) or (8,4,4,1)");
+    if ((size_t)g->Nx * g->Nz * (size_t)n_spots >= (1ull << 31)) return err("sm2d_ctx_create: too many cells");
+    return SM2D_OK;
+}
+
+int create(int device, const sm2d_grid* g, int n_spots, sm2d_ctx** out) {
+    if (!out) return err("sm2d_ctx_create: out is NULL");
+    LayoutId lid;
+    int rc_alloc = check_grid(g, n_spots, &lid);
+    if (rc_alloc) return rc_alloc;
     const size_t Ns = (size_t)g->Nx * g->Nz, N = Ns * (size_t)n_spots;
-    if (N >= (1ull << 31) / 1) return err("sm2d_ctx_create: too many cells");
     HIPCHK(hipSetDevice(device));
     std::unique_ptr<sm2d_ctx> c(new sm2d_ctx());
     c->device = device;
@@ -794,16 +875,7 @@ int create(int device, const sm2d_grid* g, int n_spots, sm2d_ctx** out) {
     c->S = n_spots;
     c->Ns = (int)Ns;
     c->N = (int)N;
-    // Capacities: listed cells per phase <= 48 rows of every spot (24 in a batch, whose spots
-    // are not all at their widest at once); occupied cells and destinations <= 5x that; payload
-    // slots <= 16 (8 in a batch) per listed cell, at least 64 Ki.  An overflow stops the
-    // transport and is reported as SM2D_ERR_CAPACITY.
-    const size_t rows = n_spots > 1 ? 24 : 48, per_listed = n_spots > 1 ? 8 : 16;
-    c->lcap = (uint32_t)std::min<size_t>(N, (size_t)n_spots * rows * g->Nx + 1024);
-    c->ocap = (uint32_t)std::min<size_t>(N, 5ull * c->lcap);
-    c->dcap = c->ocap;
-    c->pcap = (uint32_t)std::min<size_t>(N * KB, std::max<size_t>(65536, per_listed * c->lcap));
-    c->nwords = (uint32_t)((N + 31) / 32);
+    plan_caps(c.get());
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     {   // the transport's dynamic LDS (energy tables) comes on top of its static LDS
         const int dyn = (int)transport_dyn_lds(g->N_E);
@@ -817,49 +889,12 @@ int create(int device, const sm2d_grid* g, int n_spots, sm2d_ctx** out) {
         for (int k = (lid == L_REF ? 0 : 6); k < (lid == L_REF ? 6 : 12); ++k)
             HIPCHK(hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
     }
-    for (int k = 0; k < 2; ++k) {
-        HIPCHK(c->bid[k].alloc(N * KB));
-        HIPCHK(c->ns[k].alloc(N));
-        HIPCHK(c->sref[k].alloc(N * KB));
-        HIPCHK(c->pcount[k].alloc(1));
-        HIPCHK(c->pool[k].alloc((size_t)c->pcap * c->LB));
-        HIPCHK(c->occ[k].alloc(c->ocap));
-    }
-    HIPCHK(c->bitmap.alloc(c->nwords));
-    HIPCHK(c->bbid.alloc((size_t)c->lcap * 4 * KB));
-    HIPCHK(c->bn.alloc((size_t)c->lcap * 4));
-    HIPCHK(c->bsref.alloc((size_t)c->lcap * 4 * KB));
-    HIPCHK(c->bpool.alloc((size_t)c->pcap * c->LB));
-    for (DBuf<uint32_t>* b : {&c->maskv, &c->ltag, &c->lpos, &c->dtag}) HIPCHK(b->alloc(N));
-    HIPCHK(c->act.alloc(c->ocap));
-    HIPCHK(c->crs.alloc(c->ocap));
-    HIPCHK(c->dlist.alloc(c->dcap));
-    HIPCHK(c->ctl.alloc(NCTL));
-    HIPCHK(c->tcount.alloc(2 * ((size_t)c->ocap / 256 + 2)));
-    HIPCHK(c->tcount2.alloc((size_t)c->nwords / 256 + 2));
-    HIPCHK(c->flags.alloc(NFLAG));
-    HIPCHK(c->flag.alloc(c->ocap));
-    for (DBuf<double>* b : {&c->edep, &c->E_cut, &c->E_nuc, &c->E_bnd, &c->k4w, &c->k4e}) HIPCHK(b->alloc(N));
-    HIPCHK(c->win.alloc(c->ocap));
-    HIPCHK(c->ein.alloc(c->ocap));
-    HIPCHK(c->acc_c.alloc((size_t)c->ocap * NACC));
-    HIPCHK(c->acc_f.alloc((size_t)c->ocap * NACC));
-    HIPCHK(c->pe.alloc((size_t)c->ocap * NR1));
-    HIPCHK(c->part.alloc(((size_t)c->ocap / 256 + 2) * NR1));
-    HIPCHK(c->k5v.alloc(NV));
-    HIPCHK(c->hist.alloc((size_t)HROWS * NV));
-    HIPCHK(c->edep_sum.alloc(Ns));
-    for (DBuf<float>* b : {&c->w_cut, &c->w_nuc, &c->w_bnd}) HIPCHK(b->alloc(N));
-    HIPCHK(c->spots.alloc(n_spots));
-    HIPCHK(c->injacc.alloc((size_t)n_spots * 6));
-    HIPCHK(c->te.alloc(g->N_theta + 1));
-    HIPCHK(c->ee.alloc(g->N_E + 1));
-    for (DBuf<float>* b : {&c->R, &c->Sp, &c->lE, &c->lR, &c->lS}) HIPCHK(b->alloc(g->N_E));
-    HIPCHK(c->feA.alloc((size_t)n_spots * g->Nz));
-    HIPCHK(c->feB.alloc((size_t)n_spots * g->Nz));
-    HIPCHK(c->sx0.alloc(n_spots));
-    HIPCHK(c->rho.alloc(g->Nz));
-    HIPCHK(c->x0r.alloc(g->Nz));
+    c->bytes = 0;
+    rc_alloc = each_buffer(c.get(), [&](auto& b, size_t n, int) {
+        c->bytes += b.bytes_for(n);
+        return b.alloc(n);
+    });
+    if (rc_alloc) return rc_alloc;
     c->src.assign((size_t)n_spots * 7, 0.0);
     HIPCHK(hipMemsetAsync(c->maskv.p, 0, sizeof(uint32_t) * N, c->stream));
     HIPCHK(hipMemsetAsync(c->ltag.p, 0, sizeof(uint32_t) * N, c->stream));
@@ -911,6 +946,71 @@ int sm2d_ctx_create(int device, const sm2d_grid* g, sm2d_ctx** out) { return cre
 
 int sm2d_ctx_create_batch(int device, const sm2d_grid* g, int n_spots, sm2d_ctx** out) {
     return create(device, g, n_spots, out);
+}
+
+int sm2d_memory_estimate(const sm2d_grid* g, int n_spots, sm2d_memory_plan* out) {
+    if (!out || out->struct_size != sizeof(sm2d_memory_plan)) return err("sm2d_memory_estimate: struct_size");
+    LayoutId lid;
+    int rc = check_grid(g, n_spots, &lid);
+    if (rc) return rc;
+    sm2d_ctx c;  // host-side planning copy: capacities only, nothing is allocated
+    c.g = *g;
+    c.LB = g->n_theta_local * g->n_e_local * g->n_x_sub * g->n_z_sub;
+    c.S = n_spots;
+    c.Ns = g->Nx * g->Nz;
+    c.N = c.Ns * n_spots;
+    plan_caps(&c);
+    uint64_t kind[NBK] = {0, 0, 0, 0};
+    rc = each_buffer(&c, [&](auto& b, size_t n, int k) {
+        kind[k] += b.bytes_for(n);
+        return hipSuccess;
+    });
+    if (rc) return rc;
+    const uint32_t keep = out->struct_size;
+    *out = sm2d_memory_plan{};
+    out->struct_size = keep;
+    out->listed_cells = c.lcap;
+    out->occupied_cells = c.ocap;
+    out->payload_slots = c.pcap;
+    out->psi_bytes = kind[BK_PSI];
+    out->bucket_bytes = kind[BK_BUCKET];
+    out->cell_bytes = kind[BK_CELL];
+    out->other_bytes = kind[BK_OTHER];
+    out->total_bytes = kind[0] + kind[1] + kind[2] + kind[3];
+    // the reference's dense store for the same grid (estimate_dense_k1k6_memory,
+    // src/perf/memory_preflight.cpp:22-105) with this layout's local bins: two full ψ
+    // buffers, four buckets per cell, 114 B of pipeline state per cell (2 masks, 2 lists and
+    // the 14 current/previous tally arrays and 2 weight sums of memory_preflight.cpp:22-39),
+    // two counters, the audit report, the edges, the LUT and 64 MiB of runtime overhead
+    const uint64_t cells = (uint64_t)c.N, LB = (uint64_t)c.LB;
+    const uint64_t psic_cell = 32ull * (4 + LB * 4), face = 32ull * (4 + 2 + LB * 4) + 12;
+    out->dense_bytes = cells * (2 * psic_cell + 4 * face + 114) + 8 + 256 +
+                       4ull * (g->N_theta + 1) + 4ull * (g->N_E + 1) + 4ull * (6ull * g->N_E + 1) +
+                       (64ull << 20);
+    out->ok = 1;
+    return SM2D_OK;
+}
+
+int sm2d_memory_preflight(int device, const sm2d_grid* g, int n_spots, float margin,
+                          sm2d_memory_plan* out) {
+    int rc = sm2d_memory_estimate(g, n_spots, out);
+    if (rc) return rc;
+    if (!(margin > 0.0f && margin <= 1.0f)) return err("sm2d_memory_preflight: margin must be in (0,1]");
+    HIPCHK(hipSetDevice(device));
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    out->free_bytes = fr;
+    out->usable_bytes = (uint64_t)std::floor((double)fr * (double)margin);
+    out->ok = out->total_bytes <= out->usable_bytes;
+    return SM2D_OK;
+}
+
+int sm2d_ctx_memory(sm2d_ctx* c, uint64_t* bytes) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!bytes) return err("sm2d_ctx_memory: bytes is NULL");
+    *bytes = c->bytes;
+    return SM2D_OK;
 }
 
 int sm2d_ctx_destroy(sm2d_ctx* c) {
@@ -1316,12 +1416,24 @@ int sm2d_run_batch(sm2d_ctx* c, const sm2d_spot* spots, int n, const sm2d_transp
     if (edep_per_spot)
         HIPCHK(hipMemcpyAsync(edep_per_spot, per.p, sizeof(float) * n * c->Ns, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->batch_n = n;
     double s7[7];
     rc = tally_totals(c, n, s7);
     if (rc) return rc;
     au->E_dep = s7[0]; au->W_cutoff = s7[1]; au->E_cutoff = s7[2]; au->W_nuclear = s7[3];
     au->E_nuclear = s7[4]; au->W_boundary = s7[5]; au->E_boundary = s7[6];
     return finish_audit(c, au, it, n);
+}
+
+int sm2d_batch_spot_edep(sm2d_ctx* c, int s, double* edep_out) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!edep_out || s < 0 || s >= c->batch_n) return err("sm2d_batch_spot_edep: spot index outside the last batch");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(edep_out, c->edep.p + (size_t)s * c->Ns, sizeof(double) * c->Ns, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SM2D_OK;
 }
 
 int sm2d_set_profiling(sm2d_ctx* c, int on) {

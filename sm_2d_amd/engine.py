@@ -331,6 +331,12 @@ class Engine:
                     _f(per) if per is not None else None, C.byref(a))
         return edep, per, a
 
+    def batch_spot_edep(self, s: int) -> np.ndarray:
+        """Spot s's FP64 EdepC [Nz, Nx] of the last run_batch (equal to run_spot's)."""
+        e = np.zeros((self.grid.Nz, self.grid.Nx), np.float64)
+        self.L.call("batch_spot_edep", self.h, s, _d(e))
+        return e
+
 
 class Mlp:
     """The learned-physics network of the fine transport (sm2d_mlp_* of the C ABI)."""

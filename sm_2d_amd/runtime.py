@@ -5,6 +5,7 @@ import ctypes as C
 import os
 
 from . import abi
+from . import io as sio
 from .engine import Library
 
 _LIB = None
@@ -17,6 +18,7 @@ HIP_PROTOS = {
     "ctx_create_batch": [C.c_int, C.POINTER(abi.Grid), C.c_int, C.POINTER(C.c_void_p)],
     "run_batch": [C.c_void_p, C.POINTER(abi.Spot), C.c_int, C.POINTER(abi.Transport),
                   abi.dp, abi.fp, C.POINTER(abi.Audit)],
+    "batch_spot_edep": [C.c_void_p, C.c_int, abi.dp],
     "set_transport_waves": [C.c_void_p, C.c_int],
     "tuner_report": [C.c_void_p, C.c_char_p, C.c_int],
     "set_graphs": [C.c_void_p, C.c_int],
@@ -38,14 +40,15 @@ HIP_PROTOS = {
 
 
 def load_library() -> Library:
-    """Loads the in-tree HIP library.  There is no CPU fallback: a missing library or a
-    missing GPU raises."""
+    """Loads the in-tree HIP library.  There is no CPU fallback: a missing library raises, and
+    so does every transport entry point without a GPU (the host-only configuration, file
+    and planning functions of sm_2d_amd.io work anywhere)."""
     global _LIB
     if _LIB is not None:
         return _LIB
     if not os.path.exists(abi.LIB_PATH):
         raise RuntimeError(f"{abi.LIB_PATH} is missing; build it with `python -m sm_2d_amd.build`")
-    lib = Library(abi.LIB_PATH, "sm2d_", HIP_PROTOS)
+    lib = Library(abi.LIB_PATH, "sm2d_", {**HIP_PROTOS, **sio.PROTOS})
     lib.lib.sm2d_abi_version.restype = C.c_int
     lib.lib.sm2d_mlp_param_count.restype = C.c_int
     if lib.lib.sm2d_abi_version() != 1:

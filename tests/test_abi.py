@@ -1,6 +1,7 @@
-"""The C ABI surface: every function declared in include/sm2d_hip.h is exported by
-libsm2d_hip.so, and the ctypes mirror (sm_2d_amd/abi.py) has the C layout of each struct.
+"""The C ABI surface: every function declared in include/*.h is exported by libsm2d_hip.so, and
+the ctypes mirrors (sm_2d_amd/abi.py, sm_2d_amd/io.py) have the C layout of each struct.
 No GPU work is done here."""
+import glob
 import ctypes as C
 import os
 import re
@@ -8,14 +9,14 @@ import subprocess
 
 import pytest
 
-from sm_2d_amd import abi
+from sm_2d_amd import abi, io
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "sm2d_hip.h")
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
 
 
 def declared_functions():
-    text = open(HEADER).read()
+    text = "\n".join(open(h).read() for h in HEADERS)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(sm2d_\w+)\s*\(", text, flags=re.M)))
 
@@ -36,12 +37,14 @@ def test_library_exports_every_declared_symbol():
 
 
 STRUCTS = {"sm2d_grid": abi.Grid, "sm2d_transport": abi.Transport, "sm2d_spot": abi.Spot,
-           "sm2d_audit": abi.Audit, "sm2d_k5_report": abi.K5Report, "sm2d_timing": abi.Timing}
+           "sm2d_audit": abi.Audit, "sm2d_k5_report": abi.K5Report, "sm2d_timing": abi.Timing,
+           "sm2d_run_config": io.RunConfig, "sm2d_memory_plan": io.MemoryPlan}
 
 
 def test_ctypes_structs_match_c_layout(tmp_path):
     src = tmp_path / "sz.c"
-    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
+    lines = ['#include <stdio.h>', '#include <stddef.h>'] + [f'#include "{h}"' for h in HEADERS]
+    lines.append("int main(void){")
     for name, cls in STRUCTS.items():
         lines.append(f'printf("{name} %zu\\n", sizeof({name}));')
         for f, _ in cls._fields_:
