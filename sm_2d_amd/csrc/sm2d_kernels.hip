@@ -1096,10 +1096,14 @@ __device__ inline LaneOut fast_prepare(FastLds& F, const Comp& r, bool live, int
     return o;
 }
 
-// Claims of the batch's distinct keys in order of first appearance; false when a claim needs
-// the serial path (full table, global payload, pool overflow).  Resolves record addresses.
+// Claims of the batch's distinct keys in order of first appearance, with the serial claims'
+// outcome: a key found in its table takes that slot, any other key takes its table's next free
+// slot and the next LDS row.  The loop over distinct keys only decides; the tables and rows
+// change once every key is decided and the capacities are checked, so a false return (a table
+// would overflow into the closest-block fallback, a slot's payload is not in LDS, or the LDS
+// rows run out) leaves them untouched for the serial path.  Resolves record addresses.
 template <class LY>
-__device__ inline bool fast_claims(FastLds& F, WTabs<LY>& W, Psi& out, Bkt& bk, uint32_t* flags) {
+__device__ inline bool fast_claims(FastLds& F, WTabs<LY>& W) {
     const int l = lane_id();
     const int nkey = F.nkey[l], nrec = F.nrec[l];
     uint32_t k0 = EMPTY, k1 = EMPTY, k2 = EMPTY, k3 = EMPTY, k4 = EMPTY;
@@ -1110,6 +1114,9 @@ __device__ inline bool fast_claims(FastLds& F, WTabs<LY>& W, Psi& out, Bkt& bk, 
     if (nkey > 4) k4 = F.key[l * MAXKEY + 4];
     uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;
     uint32_t pend = (nkey > 0) ? ((1u << nkey) - 1u) : 0u;
+    const uint32_t used0 = W.used;
+    uint32_t nnew = 0;  // new slots so far, in order of first appearance (wave-uniform)
+    uint32_t cnt = 0;   // new slots per table, 6 bits per table (wave-uniform)
     while (true) {
         const uint64_t hp = __ballot(pend != 0u);
         if (!hp) break;
@@ -1126,19 +1133,46 @@ __device__ inline bool fast_claims(FastLds& F, WTabs<LY>& W, Psi& out, Bkt& bk, 
         hit &= pend;
         pend &= ~hit;
         const int t = (int)(K >> 24);
-        const uint32_t bid = K & 0xFFFFFFu;
-        uint32_t sel;
-        const int s = (t == 0) ? W.claim(0, bid, sel, out.pool, out.pcount, out.pcap, flags, F_PSI_FULL, false)
-                               : W.claim(t, bid, sel, bk.pool, bk.pcount, bk.pcap, flags, F_BKT_FULL, false);
-        if (s < 0) return false;
-        const uint32_t ref = rlu(W.gr(t), s);
-        if (!(ref & CACHED)) return false;
-        const uint32_t row = ref & ~CACHED;
+        const uint64_t found = __ballot(l < 32 && W.gb(t) == (K & 0xFFFFFFu));
+        uint32_t row;
+        if (found) {
+            const uint32_t ref = rlu(W.gr(t), __ffsll((unsigned long long)found) - 1);
+            if (!(ref & CACHED)) return false;
+            row = ref & ~CACHED;
+        } else {
+            if (l == 0 && nnew < 64 * MAXKEY) F.krow[nnew] = K;
+            row = used0 + nnew;
+            ++nnew;
+            cnt += 1u << (6 * t);
+        }
         if (hit & 1u) r0 = row;
         if (hit & 2u) r1 = row;
         if (hit & 4u) r2 = row;
         if (hit & 8u) r3 = row;
         if (hit & 16u) r4 = row;
+    }
+    if (nnew) {
+        if (used0 + nnew > W.cap) return false;  // (cap < 64, so no 6-bit count overflowed)
+#pragma unroll
+        for (int t = 0; t < 5; ++t)
+            if (W.gn(t) + ((cnt >> (6 * t)) & 63u) > (uint32_t)KB) return false;
+        // new slot i (first-appearance order) of table t takes the table's next slot and row used0 + i
+        const uint32_t kv = ((uint32_t)l < nnew) ? F.krow[l] : EMPTY;
+        uint32_t seen = 0;
+        for (uint32_t i = 0; i < nnew; ++i) {
+            const uint32_t K = rlu(kv, (int)i);
+            const int t = (int)(K >> 24);
+            const uint32_t slot = W.gn(t) + ((seen >> (6 * t)) & 63u);
+            seen += 1u << (6 * t);
+            W.sb(t, K & 0xFFFFFFu, (uint32_t)l == slot);
+            W.sr(t, CACHED | (used0 + i), (uint32_t)l == slot);
+            float* row = W.rows + (size_t)(used0 + i) * LY::LB;
+#pragma unroll
+            for (int k = 0; k < LY::PL; ++k) row[l + 64 * k] = 0.0f;
+        }
+#pragma unroll
+        for (int t = 0; t < 5; ++t) W.sn(t, W.gn(t) + ((cnt >> (6 * t)) & 63u));
+        W.used = used0 + nnew;
     }
     for (int rr = 0; rr < nrec; ++rr) {
         const uint32_t p = F.rec[l * MAXREC + rr];
@@ -2054,7 +2088,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                     for (uint32_t bb = b0; bb < nb && bb < b0 + TW; ++bb) {
                         FastLds& F = s_fast[bb - b0];
                         const uint64_t m = s_mask[bb - b0];
-                        if (fast_claims<LY>(F, W, out, bk, flags)) {
+                        if (fast_claims<LY>(F, W)) {
                             fast_rounds<LY>(F, W, m, accd, accf, s_own);
                             comps += (uint32_t)__popcll(m);
                         } else {
