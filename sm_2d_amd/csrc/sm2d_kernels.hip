@@ -97,6 +97,103 @@ __global__ void k_clear(Psi p, const uint32_t* __restrict__ occ, const uint32_t*
 // w_min (or a weight is negative), where lane 0 forms the serial FP32 sum itself.
 // Per-entry flag: 1 = active (fine), 2 = coarse (mask 0 with weight), 0 = nothing.
 // ------------------------------------------------------------------------------------------
+// The FP64 weight and energy of one occupied cell (one wavefront, slots in order, SU slots'
+// payloads and bin centres fetched together, positive weights only) and whether a weight is
+// negative or NaN (`bad`) or NaN (`nan`).
+template <class LY>
+__device__ inline void k1_sums(const Tabs& T, const Psi& buf, uint32_t ns, uint32_t mb, uint32_t mr, double& W64,
+                               double& E64, bool& bad, bool& nan) {
+    const int l = lane_id();
+    W64 = 0.0;
+    E64 = 0.0;
+    bool neg = false, nn = false;
+    constexpr int SU = LY::PL >= 8 ? 1 : 8 / LY::PL;
+    for (uint32_t s0 = 0; s0 < ns; s0 += SU) {
+        float v[SU][LY::PL];
+        int bEs[SU];
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            const uint32_t b = rlu(mb, (int)(s0 + u));  // EMPTY beyond ns
+            bEs[u] = b == EMPTY ? -1 : (int)((b >> 12) & 0xFFF);
+#pragma unroll
+            for (int k = 0; k < LY::PL; ++k) v[u][k] = 0.0f;
+            if (b != EMPTY) {
+                const float* vp = buf.pool + (size_t)rlu(mr, (int)(s0 + u)) * LY::LB;
+#pragma unroll
+                for (int k = 0; k < LY::PL; ++k) v[u][k] = vp[l + 64 * k];
+            }
+        }
+        float ec[SU][LY::PL];  // bin centres, fetched without waiting for the weights
+#pragma unroll
+        for (int u = 0; u < SU; ++u)
+#pragma unroll
+            for (int k = 0; k < LY::PL; ++k)
+                ec[u][k] = ecenter(T.ee, min(max(bEs[u] * LY::EL + LY::le(l + 64 * k), 0), T.NE - 1));
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            if (bEs[u] < 0) continue;
+#pragma unroll
+            for (int k = 0; k < LY::PL; ++k) {
+                const float w = v[u][k];
+                if (w > 0.0f) {
+                    W64 += (double)w;
+                    E64 += (double)w * (double)ec[u][k];
+                } else if (w < 0.0f || w != w) {
+                    neg = true;
+                    nn = nn || (w != w);
+                }
+            }
+        }
+    }
+    W64 = wave_sum(W64);
+    E64 = wave_sum(E64);
+    bad = __ballot(neg) != 0;
+    nan = __ballot(nn) != 0;
+}
+
+// K1 decision for occupied cell c of buffer `buf` in epoch ep (k1_activemask.cu:6-48): any slot
+// at or below the on / off energy blocks, the weight test and the hysteresis through the tag of
+// epoch ep - 1.  A tag of epoch ep already present was decided by the previous iteration's
+// closing audit from the same phase space, and is kept.  Lane 0 writes the tag; returns the
+// list flag (1 active, 2 coarse, 0 nothing).
+template <class LY>
+__device__ inline uint8_t k1_decide(const Tp& P, const Psi& buf, uint32_t c, uint32_t ns, uint32_t mb, double W64,
+                                    bool bad, uint32_t ep, uint32_t* maskv) {
+    const int l = lane_id();
+    const uint32_t bE = (mb >> 12) & 0xFFF;
+    const bool on = __ballot(mb != EMPTY && bE <= (uint32_t)P.b_on) != 0;
+    const bool off = __ballot(mb != EMPTY && bE <= (uint32_t)P.b_off) != 0;
+    const double wm = (double)P.wmin;
+    bool pass;
+    if (!bad && W64 > wm * (1.0 + 1e-3)) pass = true;
+    else if (!bad && W64 < wm * (1.0 - 1e-3)) pass = false;
+    else {
+        float W = 0.0f;  // the reference's serial FP32 sum (k1_activemask.cu:24-34)
+        if (l == 0)
+            for (uint32_t s = 0; s < ns; ++s) {
+                const uint32_t b = buf.bid[(size_t)c * KB + s];
+                if (b == EMPTY) continue;
+                const float* v = buf.pool + (size_t)buf.sref[(size_t)c * KB + s] * LY::LB;
+                for (int q = 0; q < LY::LB; ++q) W += v[q];
+            }
+        pass = rlf(W, 0) > P.wmin;
+    }
+    uint8_t f = 0;
+    if (l == 0 && ns) {
+        const uint32_t tag = maskv[c];
+        bool m;
+        if ((tag >> 1) == ep) {
+            m = (tag & 1u) != 0;
+        } else {
+            const bool prev = tag == (((ep - 1u) << 1) | 1u);
+            m = (on || (prev && off)) && pass;
+            maskv[c] = (ep << 1) | (m ? 1u : 0u);
+        }
+        f = m ? 1 : (W64 > 0.0 ? 2 : 0);
+    }
+    return f;
+}
+
 template <class LY>
 __global__ __launch_bounds__(64) void k_mask(Tabs T, Tp P, Psi in, const uint32_t* __restrict__ occ,
                                              const uint32_t* nocc, uint32_t* maskv, const uint32_t* ctl,
@@ -109,75 +206,11 @@ __global__ __launch_bounds__(64) void k_mask(Tabs T, Tp P, Psi in, const uint32_
         const uint32_t ns = in.ns[c];
         const uint32_t mb = (l < (int)ns) ? in.bid[(size_t)c * KB + l] : EMPTY;
         const uint32_t mr = (l < (int)ns) ? in.sref[(size_t)c * KB + l] : 0u;
-        const uint32_t bE = (mb >> 12) & 0xFFF;
-        const bool on = __ballot(mb != EMPTY && bE <= (uint32_t)P.b_on) != 0;
-        const bool off = __ballot(mb != EMPTY && bE <= (uint32_t)P.b_off) != 0;
-        double W64 = 0.0, E64 = 0.0;
-        bool neg = false;
-        // slots in order; the payloads of SU slots are fetched together
-        constexpr int SU = LY::PL >= 8 ? 1 : 8 / LY::PL;
-        for (uint32_t s0 = 0; s0 < ns; s0 += SU) {
-            float v[SU][LY::PL];
-            int bEs[SU];
-#pragma unroll
-            for (int u = 0; u < SU; ++u) {
-                const uint32_t b = rlu(mb, (int)(s0 + u));  // EMPTY beyond ns
-                bEs[u] = b == EMPTY ? -1 : (int)((b >> 12) & 0xFFF);
-#pragma unroll
-                for (int k = 0; k < LY::PL; ++k) v[u][k] = 0.0f;
-                if (b != EMPTY) {
-                    const float* vp = in.pool + (size_t)rlu(mr, (int)(s0 + u)) * LY::LB;
-#pragma unroll
-                    for (int k = 0; k < LY::PL; ++k) v[u][k] = vp[l + 64 * k];
-                }
-            }
-            float ec[SU][LY::PL];  // bin centres, fetched without waiting for the weights
-#pragma unroll
-            for (int u = 0; u < SU; ++u)
-#pragma unroll
-                for (int k = 0; k < LY::PL; ++k)
-                    ec[u][k] = ecenter(T.ee, min(max(bEs[u] * LY::EL + LY::le(l + 64 * k), 0), T.NE - 1));
-#pragma unroll
-            for (int u = 0; u < SU; ++u) {
-                if (bEs[u] < 0) continue;
-#pragma unroll
-                for (int k = 0; k < LY::PL; ++k) {
-                    const float w = v[u][k];
-                    if (w > 0.0f) {
-                        W64 += (double)w;
-                        E64 += (double)w * (double)ec[u][k];
-                    } else if (w < 0.0f || w != w) {
-                        neg = true;
-                    }
-                }
-            }
-        }
-        W64 = wave_sum(W64);
-        E64 = wave_sum(E64);
-        const bool anyneg = __ballot(neg) != 0;
-        const double wm = (double)P.wmin;
-        bool pass;
-        if (!anyneg && W64 > wm * (1.0 + 1e-3)) pass = true;
-        else if (!anyneg && W64 < wm * (1.0 - 1e-3)) pass = false;
-        else {
-            float W = 0.0f;  // the reference's serial FP32 sum (k1_activemask.cu:24-34)
-            if (l == 0)
-                for (uint32_t s = 0; s < ns; ++s) {
-                    const uint32_t b = in.bid[(size_t)c * KB + s];
-                    if (b == EMPTY) continue;
-                    const float* v = in.pool + (size_t)in.sref[(size_t)c * KB + s] * LY::LB;
-                    for (int q = 0; q < LY::LB; ++q) W += v[q];
-                }
-            pass = rlf(W, 0) > P.wmin;
-        }
+        double W64, E64;
+        bool bad, nan;
+        k1_sums<LY>(T, in, ns, mb, mr, W64, E64, bad, nan);
+        const uint8_t f = k1_decide<LY>(P, in, c, ns, mb, W64, bad, ep, maskv);
         if (l == 0) {
-            uint8_t f = 0;
-            if (ns) {
-                const bool prev = maskv[c] == (((ep - 1u) << 1) | 1u);
-                const bool m = (on || (prev && off)) && pass;
-                maskv[c] = (ep << 1) | (m ? 1u : 0u);
-                f = m ? 1 : (W64 > 0.0 ? 2 : 0);
-            }
             flag[i] = f;
             win[i] = W64;
             ein[i] = E64;
@@ -2361,13 +2394,15 @@ constexpr int NAUD = NR1 + NACC + 4;
 // the K4 drops booked on that cell (consumed).  Per-entry values go to pe[i * NR1 + k].
 // `force` runs the pass after the transport ended (totals of a buffer for the host).
 template <class LY>
-__global__ __launch_bounds__(64) void k_audit_cells(Tabs T, Psi out, const uint32_t* nin, const double* win,
-                                                    const double* ein, const uint32_t* __restrict__ occ_out,
+__global__ __launch_bounds__(64) void k_audit_cells(Tabs T, Psi out, const uint32_t* nin, double* win,
+                                                    double* ein, const uint32_t* __restrict__ occ_out,
                                                     const uint32_t* nout, double* k4w, double* k4e, double* pe,
-                                                    const uint32_t* ctl, int force) {
+                                                    const uint32_t* ctl, int force, Tp P, uint32_t* maskv,
+                                                    uint8_t* flag) {
     if (ctl[C_DONE] && !force) return;
     const uint32_t n_in = nin ? *nin : 0u, n_out = *nout;
     const uint32_t n = max(n_in, n_out);
+    const uint32_t ep1 = ctl[C_EP] + 1u;  // the next iteration's epoch
     const int l = lane_id();
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
         double r[NR1] = {0, 0, 0, 0, 0, 0};
@@ -2380,49 +2415,29 @@ __global__ __launch_bounds__(64) void k_audit_cells(Tabs T, Psi out, const uint3
             const uint32_t ns = out.ns[c];
             const uint32_t mb = (l < (int)ns) ? out.bid[(size_t)c * KB + l] : EMPTY;
             const uint32_t mr = (l < (int)ns) ? out.sref[(size_t)c * KB + l] : 0u;
-            double W = 0.0, E = 0.0;
-            constexpr int SU = LY::PL >= 8 ? 1 : 8 / LY::PL;  // slots fetched together
-            for (uint32_t s0 = 0; s0 < ns; s0 += SU) {
-                float v[SU][LY::PL];
-                int bEs[SU];
-#pragma unroll
-                for (int u = 0; u < SU; ++u) {
-                    const uint32_t b = rlu(mb, (int)(s0 + u));  // EMPTY beyond ns
-                    bEs[u] = b == EMPTY ? -1 : (int)((b >> 12) & 0xFFF);
-#pragma unroll
-                    for (int k = 0; k < LY::PL; ++k) v[u][k] = 0.0f;
-                    if (b != EMPTY) {
-                        const float* vp = out.pool + (size_t)rlu(mr, (int)(s0 + u)) * LY::LB;
-#pragma unroll
-                        for (int k = 0; k < LY::PL; ++k) v[u][k] = vp[l + 64 * k];
-                    }
-                }
-                float ec[SU][LY::PL];  // bin centres, fetched without waiting for the weights
-#pragma unroll
-                for (int u = 0; u < SU; ++u)
-#pragma unroll
-                    for (int k = 0; k < LY::PL; ++k)
-                        ec[u][k] = ecenter(T.ee, min(max(bEs[u] * LY::EL + LY::le(l + 64 * k), 0), T.NE - 1));
-#pragma unroll
-                for (int u = 0; u < SU; ++u) {
-                    if (bEs[u] < 0) continue;
-#pragma unroll
-                    for (int k = 0; k < LY::PL; ++k) {
-                        const float w = v[u][k];
-                        if (w <= 0.0f) continue;
-                        W += (double)w;
-                        E += (double)w * (double)ec[u][k];
-                    }
-                }
-            }
-            r[R_WOUT] = wave_sum(W);
-            r[R_EOUT] = wave_sum(E);
+            double W, E;
+            bool bad, nan;
+            k1_sums<LY>(T, out, ns, mb, mr, W, E, bad, nan);
+            // a NaN weight makes the totals NaN (the audit fails), as a plain sum would
+            r[R_WOUT] = nan ? __builtin_nan("") : W;
+            r[R_EOUT] = nan ? __builtin_nan("") : E;
             if (k4w && l == 0) {
                 r[R_K4W] = k4w[c];
                 r[R_K4E] = k4e[c];
                 if (r[R_K4W] != 0.0 || r[R_K4E] != 0.0) {
                     k4w[c] = 0.0;
                     k4e[c] = 0.0;
+                }
+            }
+            // psi_out is the next iteration's psi_in with this same occupied list: its K1 decision,
+            // list flag and input sums are those the next iteration's K1 would compute (entry i is
+            // read above before it is replaced here, by the same wavefront)
+            if (maskv) {
+                const uint8_t f = k1_decide<LY>(P, out, c, ns, mb, W, bad, ep1, maskv);
+                if (l == 0) {
+                    flag[i] = f;
+                    win[i] = W;
+                    ein[i] = E;
                 }
             }
         }

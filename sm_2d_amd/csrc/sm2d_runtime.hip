@@ -270,13 +270,15 @@ void build_occ(sm2d_ctx* c, int k) {
     k_occ_emit<<<nt, 256, 0, c->stream>>>(c->bitmap.p, c->nwords, c->tcount2.p, c->occ[k].p, c->ocap, c->ctl.p);
 }
 
-// K1 + K2 on psi_in (buffer `in`): mask, flag counts, ordered compaction.
+// K1 + K2 on psi_in (buffer `in`): mask, flag counts, ordered compaction.  mask = false: the
+// previous iteration's closing audit already decided the mask and flags (k_audit_cells).
 template <class LY>
-void launch_mask_compact(sm2d_ctx* c, const Tp& P, int in, int grid_entries) {
+void launch_mask_compact(sm2d_ctx* c, const Tp& P, int in, int grid_entries, bool mask = true) {
     const int ge = blocks_for(grid_entries, 1, 1 << 16);
     const int gt = blocks_for(grid_entries, 256, 4096);
-    k_mask<LY><<<ge, 64, 0, c->stream>>>(c->tabs(), P, c->psi(in), c->occ[in].p, c->nocc(in), c->maskv.p, c->ctl.p,
-                                         c->flag.p, c->win.p, c->ein.p);
+    if (mask)
+        k_mask<LY><<<ge, 64, 0, c->stream>>>(c->tabs(), P, c->psi(in), c->occ[in].p, c->nocc(in), c->maskv.p,
+                                             c->ctl.p, c->flag.p, c->win.p, c->ein.p);
     if ((uint32_t)grid_entries <= SMALL_LIST) {
         k_compact_small<<<1, 1024, 0, c->stream>>>(c->occ[in].p, c->nocc(in), c->flag.p, c->act.p, c->crs.p,
                                                    c->ltag.p, c->lpos.p, c->bitmap.p, c->ctl.p, c->lcap, c->flags.p);
@@ -290,13 +292,16 @@ void launch_mask_compact(sm2d_ctx* c, const Tp& P, int in, int grid_entries) {
 
 // K5 stage 1 over (nin entries of K1 sums, nout occupied cells of buffer k).  Returns the
 // per-cell sums for k_audit_final to tile itself (short lists), else null after the tile pass.
+// With P, buffer k's K1 decision for the next iteration is made in the same pass.
 template <class LY>
-const double* launch_audit_cells(sm2d_ctx* c, int k, const uint32_t* nin, bool k4, int force, int grid_entries) {
+const double* launch_audit_cells(sm2d_ctx* c, int k, const uint32_t* nin, bool k4, int force, int grid_entries,
+                                 const Tp* P = nullptr) {
     const int ge = blocks_for(grid_entries, 1, 1 << 16);
     const int gt = blocks_for(grid_entries, 256, 4096);
     k_audit_cells<LY><<<ge, 64, 0, c->stream>>>(c->tabs(), c->psi(k), nin, c->win.p, c->ein.p, c->occ[k].p,
                                                 c->nocc(k), k4 ? c->k4w.p : nullptr, k4 ? c->k4e.p : nullptr,
-                                                c->pe.p, c->ctl.p, force);
+                                                c->pe.p, c->ctl.p, force, P ? *P : Tp{}, P ? c->maskv.p : nullptr,
+                                                c->flag.p);
     if ((uint32_t)grid_entries <= SMALL_AUDIT) return c->pe.p;
     k_audit_tiles<<<gt, 256, 0, c->stream>>>(c->pe.p, nin, c->nocc(k), c->part.p, c->ctl.p, force);
     return nullptr;
@@ -313,7 +318,8 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next) {
     const int gm = blocks_for(4ull * h, 256, 4096);  // occupied-list tiles
     if (opener) k_clear<<<gm, 256, 0, c->stream>>>(c->psi(out), c->occ[out].p, c->nocc(out), c->ctl.p);
     c->mark(0);
-    launch_mask_compact<LY>(c, P, in, (int)hocc);
+    // an iteration opened by the previous one's closing kernel has its K1 decision from there
+    launch_mask_compact<LY>(c, P, in, (int)hocc, opener);
     c->mark(1);
     for (int fine = 0; fine < 2; ++fine) {
         const uint32_t* list = fine ? c->act.p : c->crs.p;
@@ -353,7 +359,7 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next) {
     }
     build_occ(c, out);
     c->mark(10);
-    const double* pe = launch_audit_cells<LY>(c, out, c->nocc(in), true, 0, (int)hocc);
+    const double* pe = launch_audit_cells<LY>(c, out, c->nocc(in), true, 0, (int)hocc, open_next ? &P : nullptr);
     k_audit_final<<<open_next ? 1 + gm : 1, 256, 0, c->stream>>>(
         c->part.p, c->nocc(in), c->nocc(out), c->acc_c.p, c->ctl.p + C_NCRS, c->acc_f.p, c->ctl.p + C_NACT, nullptr,
         c->hist.p, HROWS, c->ctl.p, 0, pe, c->psi(in), c->occ[in].p, c->nocc(in));
