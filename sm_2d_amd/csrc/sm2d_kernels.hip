@@ -345,32 +345,30 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t* __restrict__ occ, 
     }
 }
 
-// One-workgroup form of k_flag_count + k_scan<2> + k_emit for short occupied lists: tiles of 1024
-// entries in order, ranks from wave ballots, running bases per list.  Same lists, tags, counts,
-// capacity check and end-of-transport test.
-__global__ __launch_bounds__(1024) void k_compact_small(const uint32_t* __restrict__ occ, const uint32_t* nocc,
-                                                        const uint8_t* __restrict__ flag, uint32_t* act,
-                                                        uint32_t* crs, uint32_t* ltag, uint32_t* lpos,
-                                                        uint32_t* bitmap, uint32_t* ctl, uint32_t cap,
-                                                        uint32_t* flags) {
-    if (ctl[C_DONE]) {
-        if (threadIdx.x == 0) {
-            ctl[C_NACT] = 0;
-            ctl[C_NCRS] = 0;
-        }
-        return;
-    }
-    __shared__ uint32_t wc[2][16];
-    __shared__ uint32_t base[2];
-    const uint32_t n = *nocc, ep = ctl[C_EP];
+// Where the ordered compaction writes (K2 lists, their tags and positions, the occupied bits of
+// the listed cells); occ / flag: the occupied list being compacted and its K1 flags.
+struct ListOut {
+    const uint32_t* occ;
+    const uint8_t* flag;
+    uint32_t *act, *crs, *ltag, *lpos, *bitmap, *flags;
+    uint32_t cap;
+};
+
+// One-workgroup ordered compaction of n list entries (k_flag_count + k_scan<2> + k_emit in one
+// block of blockDim.x threads, at most 16 waves): tiles in order, ranks from wave ballots,
+// running bases per list; lists tagged for epoch ep.  Empty or over-long lists end the
+// transport (done_it = the iteration that found them).  wc[2][16], base[2]: shared scratch.
+__device__ inline void compact_lists(const ListOut& L, uint32_t n, uint32_t ep, uint32_t done_it, uint32_t* ctl,
+                                     uint32_t (*wc)[16], uint32_t* base) {
+    const uint32_t nthr = blockDim.x, nw = nthr / 64;
     const int w = threadIdx.x >> 6, l = lane_id();
     const uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
     if (threadIdx.x < 2) base[threadIdx.x] = 0;
     __syncthreads();
     // Pass 1: list lengths, so that an over-long list writes nothing (as the tiled form).
     uint32_t ca = 0, cc = 0;
-    for (uint32_t i = threadIdx.x; i < n; i += 1024) {
-        const uint8_t f = flag[i];
+    for (uint32_t i = threadIdx.x; i < n; i += nthr) {
+        const uint8_t f = L.flag[i];
         ca += f == 1;
         cc += f == 2;
     }
@@ -384,24 +382,24 @@ __global__ __launch_bounds__(1024) void k_compact_small(const uint32_t* __restri
     }
     __syncthreads();
     const uint32_t na = base[0], nc = base[1];
-    if (na > cap || nc > cap || (na == 0 && nc == 0)) {
+    __syncthreads();
+    if (na > L.cap || nc > L.cap || (na == 0 && nc == 0)) {
         if (threadIdx.x == 0) {
-            const bool over = na > cap || nc > cap;
-            if (over) atomicOr(&flags[F_DEST_FULL], 1u);
+            const bool over = na > L.cap || nc > L.cap;
+            if (over) atomicOr(&L.flags[F_DEST_FULL], 1u);
             ctl[C_NACT] = over ? 0u : na;
             ctl[C_NCRS] = over ? 0u : nc;
             ctl[C_DONE] = 1;
-            ctl[C_DONE_IT] = ctl[C_IT];
+            ctl[C_DONE_IT] = done_it;
         }
         return;
     }
-    __syncthreads();
     if (threadIdx.x < 2) base[threadIdx.x] = 0;
     __syncthreads();
     // Pass 2: ordered emission.
-    for (uint32_t t0 = 0; t0 < n; t0 += 1024) {
+    for (uint32_t t0 = 0; t0 < n; t0 += nthr) {
         const uint32_t i = t0 + threadIdx.x;
-        const uint8_t f = i < n ? flag[i] : 0;
+        const uint8_t f = i < n ? L.flag[i] : 0;
         const uint64_t ba = __ballot(f == 1), bc = __ballot(f == 2);
         if (l == 0) {
             wc[0][w] = __popcll(ba);
@@ -413,21 +411,21 @@ __global__ __launch_bounds__(1024) void k_compact_small(const uint32_t* __restri
             uint32_t pos = base[k];
             for (int j = 0; j < w; ++j) pos += wc[k][j];
             pos += __popcll((k == 0 ? ba : bc) & below);
-            const uint32_t c = occ[i];
+            const uint32_t c = L.occ[i];
             if (k == 0) {
-                act[pos] = c;
-                ltag[c] = 2 * ep + 1;
+                L.act[pos] = c;
+                L.ltag[c] = 2 * ep + 1;
             } else {
-                crs[pos] = c;
-                ltag[c] = 2 * ep;
+                L.crs[pos] = c;
+                L.ltag[c] = 2 * ep;
             }
-            lpos[c] = pos;
-            atomicOr(&bitmap[c >> 5], 1u << (c & 31));
+            L.lpos[c] = pos;
+            atomicOr(&L.bitmap[c >> 5], 1u << (c & 31));
         }
         __syncthreads();
         if (threadIdx.x < 2) {
             uint32_t s = 0;
-            for (int j = 0; j < 16; ++j) s += wc[threadIdx.x][j];
+            for (uint32_t j = 0; j < nw; ++j) s += wc[threadIdx.x][j];
             base[threadIdx.x] += s;
         }
         __syncthreads();
@@ -436,6 +434,21 @@ __global__ __launch_bounds__(1024) void k_compact_small(const uint32_t* __restri
         ctl[C_NACT] = na;
         ctl[C_NCRS] = nc;
     }
+}
+
+// The compaction as a kernel of its own (1024 threads): the first iteration of a run or of a
+// chunk, and lists too long for the closing audit kernel's workgroup.
+__global__ __launch_bounds__(1024) void k_compact_small(ListOut L, const uint32_t* nocc, uint32_t* ctl) {
+    if (ctl[C_DONE]) {
+        if (threadIdx.x == 0) {
+            ctl[C_NACT] = 0;
+            ctl[C_NCRS] = 0;
+        }
+        return;
+    }
+    __shared__ uint32_t wc[2][16];
+    __shared__ uint32_t base[2];
+    compact_lists(L, *nocc, ctl[C_EP], ctl[C_IT], ctl, wc, base);
 }
 
 // One-workgroup form of k_occ_count + k_scan<1> + k_occ_emit for small grids.
@@ -2478,7 +2491,8 @@ __global__ __launch_bounds__(256) void k_audit_final(const double* part, const u
                                                      const uint32_t* nf, double* outv, double* hist,
                                                      uint32_t hist_rows, uint32_t* ctl, int force,
                                                      const double* __restrict__ pe, Psi nxt,
-                                                     const uint32_t* __restrict__ nxt_occ, const uint32_t* nxt_n) {
+                                                     const uint32_t* __restrict__ nxt_occ, const uint32_t* nxt_n,
+                                                     ListOut L) {
     if (ctl[C_DONE] && !force) return;
     if (blockIdx.x > 0) {  // clear the next iteration's output buffer (this iteration's input)
         clear_cells(nxt, nxt_occ, *nxt_n, (blockIdx.x - 1) * blockDim.x + threadIdx.x, (gridDim.x - 1) * blockDim.x);
@@ -2529,6 +2543,12 @@ __global__ __launch_bounds__(256) void k_audit_final(const double* part, const u
     }
     if (gridDim.x > 1) {  // open the next iteration once every thread has read the counters
         __syncthreads();
+        if (L.occ) {  // the next iteration's lists from the K1 flags of psi_out (k_audit_cells)
+            __shared__ uint32_t wc[2][16];
+            __shared__ uint32_t base[2];
+            compact_lists(L, n_out, ctl[C_EP] + 1u, ctl[C_IT] + 1u, ctl, wc, base);
+            __syncthreads();
+        }
         if (threadIdx.x == 0) open_iteration(ctl);
     }
 }

@@ -270,18 +270,34 @@ void build_occ(sm2d_ctx* c, int k) {
     k_occ_emit<<<nt, 256, 0, c->stream>>>(c->bitmap.p, c->nwords, c->tcount2.p, c->occ[k].p, c->ocap, c->ctl.p);
 }
 
+// Compaction targets of buffer `in`'s occupied list (occ = null: no compaction).
+ListOut list_out(sm2d_ctx* c, int in) {
+    ListOut L;
+    L.occ = c->occ[in].p;
+    L.flag = c->flag.p;
+    L.act = c->act.p;
+    L.crs = c->crs.p;
+    L.ltag = c->ltag.p;
+    L.lpos = c->lpos.p;
+    L.bitmap = c->bitmap.p;
+    L.flags = c->flags.p;
+    L.cap = c->lcap;
+    return L;
+}
+
 // K1 + K2 on psi_in (buffer `in`): mask, flag counts, ordered compaction.  mask = false: the
-// previous iteration's closing audit already decided the mask and flags (k_audit_cells).
+// previous iteration's closing audit already decided the mask and flags (k_audit_cells);
+// compact = false: it also built the lists (k_audit_final).
 template <class LY>
-void launch_mask_compact(sm2d_ctx* c, const Tp& P, int in, int grid_entries, bool mask = true) {
+void launch_mask_compact(sm2d_ctx* c, const Tp& P, int in, int grid_entries, bool mask = true, bool compact = true) {
     const int ge = blocks_for(grid_entries, 1, 1 << 16);
     const int gt = blocks_for(grid_entries, 256, 4096);
     if (mask)
         k_mask<LY><<<ge, 64, 0, c->stream>>>(c->tabs(), P, c->psi(in), c->occ[in].p, c->nocc(in), c->maskv.p,
                                              c->ctl.p, c->flag.p, c->win.p, c->ein.p);
+    if (!compact) return;
     if ((uint32_t)grid_entries <= SMALL_LIST) {
-        k_compact_small<<<1, 1024, 0, c->stream>>>(c->occ[in].p, c->nocc(in), c->flag.p, c->act.p, c->crs.p,
-                                                   c->ltag.p, c->lpos.p, c->bitmap.p, c->ctl.p, c->lcap, c->flags.p);
+        k_compact_small<<<1, 1024, 0, c->stream>>>(list_out(c, in), c->nocc(in), c->ctl.p);
         return;
     }
     k_flag_count<<<gt, 256, 0, c->stream>>>(c->flag.p, c->nocc(in), c->tcount.p, c->ctl.p);
@@ -310,7 +326,10 @@ const double* launch_audit_cells(sm2d_ctx* c, int k, const uint32_t* nin, bool k
 template <class LY>
 // opener: open the iteration here (first of a run); open_next: the closing audit kernel opens
 // the next iteration and clears its output buffer, which is this iteration's input.
-void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next) {
+// compacted: the previous iteration's closing kernel built this iteration's lists; compact_next:
+// this iteration's closing kernel builds the next one's (short lists only).
+void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, bool compacted = false,
+                      bool compact_next = false) {
     const int in = c->in, out = 1 - c->in;
     Tabs T = c->tabs();
     const uint32_t h = pow2_at_least(std::max<uint32_t>(c->hint, 64));
@@ -319,7 +338,7 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next) {
     if (opener) k_clear<<<gm, 256, 0, c->stream>>>(c->psi(out), c->occ[out].p, c->nocc(out), c->ctl.p);
     c->mark(0);
     // an iteration opened by the previous one's closing kernel has its K1 decision from there
-    launch_mask_compact<LY>(c, P, in, (int)hocc, opener);
+    launch_mask_compact<LY>(c, P, in, (int)hocc, opener, !compacted);
     c->mark(1);
     for (int fine = 0; fine < 2; ++fine) {
         const uint32_t* list = fine ? c->act.p : c->crs.p;
@@ -360,9 +379,11 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next) {
     build_occ(c, out);
     c->mark(10);
     const double* pe = launch_audit_cells<LY>(c, out, c->nocc(in), true, 0, (int)hocc, open_next ? &P : nullptr);
+    ListOut L = list_out(c, out);
+    if (!(open_next && compact_next)) L.occ = nullptr;
     k_audit_final<<<open_next ? 1 + gm : 1, 256, 0, c->stream>>>(
         c->part.p, c->nocc(in), c->nocc(out), c->acc_c.p, c->ctl.p + C_NCRS, c->acc_f.p, c->ctl.p + C_NACT, nullptr,
-        c->hist.p, HROWS, c->ctl.p, 0, pe, c->psi(in), c->occ[in].p, c->nocc(in));
+        c->hist.p, HROWS, c->ctl.p, 0, pe, c->psi(in), c->occ[in].p, c->nocc(in), L);
     c->mark(11);
     c->in = out;  // K6
 }
@@ -371,8 +392,9 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next) {
 template <class LY>
 int buffer_totals(sm2d_ctx* c, int k, double& W, double& E) {
     const double* pe = launch_audit_cells<LY>(c, k, nullptr, false, 1, (int)std::max<uint32_t>(c->hint_occ, 1024));
+    ListOut none{};
     k_audit_final<<<1, 256, 0, c->stream>>>(c->part.p, nullptr, c->nocc(k), nullptr, nullptr, nullptr, nullptr,
-                                            c->k5v.p, nullptr, 1, c->ctl.p, 1, pe, Psi{}, nullptr, nullptr);
+                                            c->k5v.p, nullptr, 1, c->ctl.p, 1, pe, Psi{}, nullptr, nullptr, none);
     HIPCHK(hipGetLastError());
     double v[NV];
     HIPCHK(hipMemcpyAsync(v, c->k5v.p, sizeof(v), hipMemcpyDeviceToHost, c->stream));
@@ -485,7 +507,10 @@ int launch_chunk_graph(sm2d_ctx* c, const Tp& P, int R, bool first_opens, bool f
         }
     const int in0 = c->in;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int r = 0; r < R; ++r) launch_iteration<LY>(c, P, first_opens && r == 0, fused_open);
+    // inside a chunk, each iteration's closing kernel builds the next one's short lists
+    const bool fold = fused_open && pow2_at_least(c->hint_occ) <= SMALL_LIST;
+    for (int r = 0; r < R; ++r)
+        launch_iteration<LY>(c, P, first_opens && r == 0, fused_open, fold && r > 0, fold && r + 1 < R);
     hipGraph_t graph = nullptr;
     const hipError_t le = hipGetLastError();
     const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
@@ -573,7 +598,9 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
             }
             c->ev_it = r;
             // fail-fast keeps psi_in of a failing iteration, so there every iteration opens itself
-            launch_iteration<LY>(c, P, fused_open ? (queued == 0 && r == 0) : true, fused_open);
+            const bool fold = fused_open && pow2_at_least(c->hint_occ) <= SMALL_LIST;
+            launch_iteration<LY>(c, P, fused_open ? (queued == 0 && r == 0) : true, fused_open, fold && r > 0,
+                                 fold && r + 1 < R);
         }
         const int first_it = queued;  // global index of the chunk's first iteration
         queued += R;
