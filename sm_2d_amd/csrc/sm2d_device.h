@@ -61,7 +61,8 @@ struct Tabs {
     int surr_nodes;
 };
 
-// Device control block (u32 words).
+// Device control block (u32 words).  C_DONE: 0 running, 1 ended (empty lists or a capacity
+// overflow), 2 paused (the next iteration needs a phase the queued chunk left out).
 enum Ctl {
     C_NACT = 0, C_NCRS = 1, C_NDEST = 2, C_DONE = 3, C_EP = 4, C_IT = 5, C_DONE_IT = 6,
     C_NOCC0 = 8, C_NOCC1 = 9, C_NDEST1 = 10, C_BPC0 = 11, C_BPC1 = 12, NCTL = 16

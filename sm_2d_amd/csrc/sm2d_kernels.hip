@@ -346,12 +346,16 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t* __restrict__ occ, 
 }
 
 // Where the ordered compaction writes (K2 lists, their tags and positions, the occupied bits of
-// the listed cells); occ / flag: the occupied list being compacted and its K1 flags.
+// the listed cells); occ / flag: the occupied list being compacted and its K1 flags.  skip: the
+// phases whose kernels the queued chunk leaves out (bit 0 coarse, bit 1 fine); lists that need
+// one of them pause the transport (ctl[C_DONE] = 2) before the iteration runs, and the host
+// queues the rest again with both phases.
 struct ListOut {
     const uint32_t* occ;
     const uint8_t* flag;
     uint32_t *act, *crs, *ltag, *lpos, *bitmap, *flags;
     uint32_t cap;
+    uint32_t skip;
 };
 
 // One-workgroup ordered compaction of n list entries (k_flag_count + k_scan<2> + k_emit in one
@@ -433,6 +437,7 @@ __device__ inline void compact_lists(const ListOut& L, uint32_t n, uint32_t ep, 
     if (threadIdx.x == 0) {
         ctl[C_NACT] = na;
         ctl[C_NCRS] = nc;
+        if (((L.skip & 2u) && na) || ((L.skip & 1u) && nc)) ctl[C_DONE] = 2;
     }
 }
 

@@ -102,6 +102,7 @@ struct sm2d_ctx {
     uint32_t pcap = 0, lcap = 0, dcap = 0, ocap = 0, nwords = 0;
     uint64_t bytes = 0;  // device bytes of the buffers sm2d_ctx_create allocated
     int batch_n = 0;     // spots of the last sm2d_run_batch (their EdepC stays readable)
+    uint64_t skip_chunks = 0, pauses = 0;  // graph chunks without a phase, and their pauses
     bool tables = false;
     DBuf<float> te, ee, R, Sp, lE, lR, lS, feA, feB, sx0, rho, x0r;
     std::vector<float> h_ee;
@@ -282,6 +283,7 @@ ListOut list_out(sm2d_ctx* c, int in) {
     L.bitmap = c->bitmap.p;
     L.flags = c->flags.p;
     L.cap = c->lcap;
+    L.skip = 0;
     return L;
 }
 
@@ -289,7 +291,8 @@ ListOut list_out(sm2d_ctx* c, int in) {
 // previous iteration's closing audit already decided the mask and flags (k_audit_cells);
 // compact = false: it also built the lists (k_audit_final).
 template <class LY>
-void launch_mask_compact(sm2d_ctx* c, const Tp& P, int in, int grid_entries, bool mask = true, bool compact = true) {
+void launch_mask_compact(sm2d_ctx* c, const Tp& P, int in, int grid_entries, bool mask = true, bool compact = true,
+                         uint32_t skip = 0) {
     const int ge = blocks_for(grid_entries, 1, 1 << 16);
     const int gt = blocks_for(grid_entries, 256, 4096);
     if (mask)
@@ -297,7 +300,9 @@ void launch_mask_compact(sm2d_ctx* c, const Tp& P, int in, int grid_entries, boo
                                              c->ctl.p, c->flag.p, c->win.p, c->ein.p);
     if (!compact) return;
     if ((uint32_t)grid_entries <= SMALL_LIST) {
-        k_compact_small<<<1, 1024, 0, c->stream>>>(list_out(c, in), c->nocc(in), c->ctl.p);
+        ListOut L = list_out(c, in);
+        L.skip = skip;
+        k_compact_small<<<1, 1024, 0, c->stream>>>(L, c->nocc(in), c->ctl.p);
         return;
     }
     k_flag_count<<<gt, 256, 0, c->stream>>>(c->flag.p, c->nocc(in), c->tcount.p, c->ctl.p);
@@ -327,9 +332,11 @@ template <class LY>
 // opener: open the iteration here (first of a run); open_next: the closing audit kernel opens
 // the next iteration and clears its output buffer, which is this iteration's input.
 // compacted: the previous iteration's closing kernel built this iteration's lists; compact_next:
-// this iteration's closing kernel builds the next one's (short lists only).
+// this iteration's closing kernel builds the next one's (short lists only).  skip: phases left
+// out (bit 0 coarse, bit 1 fine; short lists only): their lists are empty in every iteration that
+// runs, since lists that need them pause the transport first.
 void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, bool compacted = false,
-                      bool compact_next = false) {
+                      bool compact_next = false, uint32_t skip = 0) {
     const int in = c->in, out = 1 - c->in;
     Tabs T = c->tabs();
     const uint32_t h = pow2_at_least(std::max<uint32_t>(c->hint, 64));
@@ -338,9 +345,10 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
     if (opener) k_clear<<<gm, 256, 0, c->stream>>>(c->psi(out), c->occ[out].p, c->nocc(out), c->ctl.p);
     c->mark(0);
     // an iteration opened by the previous one's closing kernel has its K1 decision from there
-    launch_mask_compact<LY>(c, P, in, (int)hocc, opener, !compacted);
+    launch_mask_compact<LY>(c, P, in, (int)hocc, opener, !compacted, skip);
     c->mark(1);
     for (int fine = 0; fine < 2; ++fine) {
+        if (skip & (1u << fine)) continue;
         const uint32_t* list = fine ? c->act.p : c->crs.p;
         const uint32_t* nl = c->ctl.p + (fine ? C_NACT : C_NCRS);
         double* acc = fine ? c->acc_f.p : c->acc_c.p;
@@ -380,6 +388,7 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
     c->mark(10);
     const double* pe = launch_audit_cells<LY>(c, out, c->nocc(in), true, 0, (int)hocc, open_next ? &P : nullptr);
     ListOut L = list_out(c, out);
+    L.skip = skip;
     if (!(open_next && compact_next)) L.occ = nullptr;
     k_audit_final<<<open_next ? 1 + gm : 1, 256, 0, c->stream>>>(
         c->part.p, c->nocc(in), c->nocc(out), c->acc_c.p, c->ctl.p + C_NCRS, c->acc_f.p, c->ctl.p + C_NACT, nullptr,
@@ -480,9 +489,10 @@ void key_put(std::vector<uint8_t>& k, const T& v) {
 // and replayed afterwards; every launch of the chunk reads its lists, counts and epochs from the
 // device control block, so a replay is the same work as the eager launches.
 template <class LY>
-int launch_chunk_graph(sm2d_ctx* c, const Tp& P, int R, bool first_opens, bool fused_open) {
+int launch_chunk_graph(sm2d_ctx* c, const Tp& P, int R, bool first_opens, bool fused_open, uint32_t skip = 0) {
     std::vector<uint8_t> key;
     key_put(key, R);
+    key_put(key, skip);
     key_put(key, c->in);
     key_put(key, (int)first_opens);
     key_put(key, (int)fused_open);
@@ -510,7 +520,8 @@ int launch_chunk_graph(sm2d_ctx* c, const Tp& P, int R, bool first_opens, bool f
     // inside a chunk, each iteration's closing kernel builds the next one's short lists
     const bool fold = fused_open && pow2_at_least(c->hint_occ) <= SMALL_LIST;
     for (int r = 0; r < R; ++r)
-        launch_iteration<LY>(c, P, first_opens && r == 0, fused_open, fold && r > 0, fold && r + 1 < R);
+        launch_iteration<LY>(c, P, first_opens && r == 0, fused_open, fold && r > 0, fold && r + 1 < R,
+                             fold ? skip : 0u);
     hipGraph_t graph = nullptr;
     const hipError_t le = hipGetLastError();
     const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
@@ -555,6 +566,10 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
     const bool fused_open = !tp->fail_fast_on_audit;
     const int opened_ahead = fused_open ? 1 : 0;  // ctl[C_IT] counts the next, already opened, iteration
     int queued = 0, audited = 0;
+    // phases whose lists were empty when the last chunk ended; a graph chunk leaves their kernels
+    // out (short lists) until a list needs them, which pauses the transport (once per run)
+    uint32_t last_nact = 1, last_ncrs = 1;
+    bool paused_once = false;
     double Wso = 0, Wss = 0, Eso = 0, Ess = 0;
     for (int s = 0; s < c->S; ++s) {
         Wso += c->src[s * 7 + 1];
@@ -584,7 +599,14 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
                 c->cand[ph] = cd;
                 for (int r = 0; r < R; ++r) c->cand_log[(size_t)2 * r + ph] = cd;
             }
-            const int rc = launch_chunk_graph<LY>(c, P, R, fused_open ? queued == 0 : true, fused_open);
+            uint32_t skip = 0;
+            if (fused_open && queued > 0 && !paused_once) {
+                if (last_ncrs == 0) skip |= 1u;
+                if (last_nact == 0) skip |= 2u;
+                if (skip == 3u) skip = 0;
+            }
+            if (skip) c->skip_chunks++;
+            const int rc = launch_chunk_graph<LY>(c, P, R, fused_open ? queued == 0 : true, fused_open, skip);
             if (rc) return rc;
         } else for (int r = 0; r < R; ++r) {
             for (int ph = 0; ph < 2; ++ph) {
@@ -613,7 +635,7 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
         HIPCHK(hipStreamSynchronize(c->stream));
         int rc = check_flags(c, f);
         if (rc) return rc;
-        const bool done = ctl[C_DONE] != 0;
+        const bool done = ctl[C_DONE] == 1, paused = ctl[C_DONE] == 2;
         const int upto = done ? (int)ctl[C_DONE_IT] - 1 : (int)ctl[C_IT] - opened_ahead;  // audited iterations
         uint32_t maxlist = 0;
         for (int it = audited; it < upto; ++it) {
@@ -681,6 +703,17 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
         c->hint_ph[1] = std::max<uint32_t>(2, 2 * ctl[C_NACT]);
         c->hint_occ = std::max<uint32_t>(64, 2 * std::max(ctl[C_NOCC0], ctl[C_NOCC1]));
         if (au) au->iterations = done ? (int)ctl[C_DONE_IT] : (int)ctl[C_IT] - opened_ahead;
+        last_nact = ctl[C_NACT];
+        last_ncrs = ctl[C_NCRS];
+        if (paused) {  // the iterations from `upto` on did not run: queue them again with both phases
+            const uint32_t zero = 0;
+            HIPCHK(hipMemcpyAsync(c->ctl.p + C_DONE, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            queued = upto;
+            c->in = in0 ^ (upto & 1);
+            paused_once = true;
+            c->pauses++;
+        }
         if (done) {
             if (au) au->terminated = 1;
             c->in = in0 ^ (int)((ctl[C_DONE_IT] - 1u) & 1u);  // the buffer of the terminating iteration
@@ -1521,10 +1554,13 @@ int sm2d_tuner_report(sm2d_ctx* c, char* buf, int cap) {
         }
         o += "}";
     }
-    char t[160];
-    std::snprintf(t, sizeof(t), ", \"chunk_graphs\": {\"on\": %d, \"captured\": %llu, \"replayed\": %llu}",
+    char t[256];
+    std::snprintf(t, sizeof(t),
+                  ", \"chunk_graphs\": {\"on\": %d, \"captured\": %llu, \"replayed\": %llu, "
+                  "\"phase_skipping_chunks\": %llu, \"pauses\": %llu}",
                   c->graphs_on ? 1 : 0, (unsigned long long)c->graph_captures,
-                  (unsigned long long)c->graph_replays);
+                  (unsigned long long)c->graph_replays, (unsigned long long)c->skip_chunks,
+                  (unsigned long long)c->pauses);
     o += t;
     o += "}";
     std::snprintf(buf, (size_t)cap, "%s", o.c_str());

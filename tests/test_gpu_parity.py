@@ -294,6 +294,33 @@ def test_chunk_graphs_replay_bit_exact(hip, orc):
     assert ag.iterations == ao.iterations
 
 
+@pytest.mark.parametrize("fine_on", [10.0, 300.0], ids=["coarse-then-fine", "fine-only"])
+def test_phase_skipping_chunks_bit_exact(hip, orc, fine_on):
+    """Graph chunks leave out a phase whose list was empty when the previous chunk ended.  With
+    a 60 MeV beam (0.5 mm cells) the fine phase starts after the first chunks: its first list pauses the
+    transport before that iteration runs and the host queues the rest again with both phases.
+    With every energy in the fine phase the coarse kernels stay out while the coarse list is
+    empty.  Both equal the oracle."""
+    prob = problems.reference_grid_case(Nx=40, Nz=80, E0=60.0, sigma_x=2.0, n_samples=300,
+                                        max_iterations=400)
+    prob.transport.update(E_fine_on=fine_on, E_fine_off=fine_on + 1.0)
+    g, o = _pair(hip, orc, prob)
+    sp, tp = prob.make_spot(), prob.make_transport()
+    eo, ao = o.run_spot(sp, tp)
+    g.set_graphs(True)
+    g.set_transport_waves(2)  # no tuner exploration: every chunk is a graph chunk
+    eg, ag = g.run_spot(sp, tp)
+    assert ag.iterations == ao.iterations and ag.terminated == ao.terminated
+    assert np.array_equal(eg, eo)
+    assert ag.cell_bin_updates == ao.cell_bin_updates
+    rep = g.tuner_report()["chunk_graphs"]
+    assert rep["phase_skipping_chunks"] > 0
+    # the fine phase's arrival pauses once; with every energy fine, a coarse list can still come
+    # back (cells whose weight fell below w_min are coarse-listed), and after one pause a run
+    # keeps both phases
+    assert rep["pauses"] == 1 if fine_on == 10.0 else rep["pauses"] <= 1
+
+
 def _saturating_psi(rng, g, LB):
     """Cells spaced 7 apart in x and 3 in z (no lateral emission into another listed cell), each
     with 32 occupied blocks over the upper half of the energy range and 8-39 weighted bins per
