@@ -16,6 +16,8 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "../../include/sm2d_hip.h"
 #include "sm2d_kernels.hip"
 
@@ -81,6 +83,8 @@ struct Tuner {
         while (b + 1 < NB && (2u << b) <= listed) ++b;
         return b;
     }
+    // The fastest shape; four waves (the default) when it is within 5 % of the fastest, so that
+    // near-ties do not flip with timing noise from one process to the next.
     int best(int ph, int b) const {
         int bc = 2;
         double bt = 1e300;
@@ -89,6 +93,7 @@ struct Tuner {
                 bt = ms[ph][b][c] / cnt[ph][b][c];
                 bc = c;
             }
+        if (cnt[ph][b][2] && ms[ph][b][2] / cnt[ph][b][2] <= 1.05 * bt) bc = 2;
         return bc;
     }
 };
@@ -360,13 +365,25 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
         D.ndest = ndest;
         D.cap = c->dcap;
         D.bitmap = c->bitmap.p;
-        c->mark(e0);
         const size_t dyn = transport_dyn_lds(c->g.N_E);
         const int tw = Tuner::tw(c->cand[fine]);
         const int gb = blocks_for(pow2_at_least(std::max<uint32_t>(c->hint_ph[fine], 64)), 1, 65536);
-#define SM2D_TRANSPORT(F, W)                                                                              \
-    k_transport<LY, F, W><<<gb, 64 * W, dyn, c->stream>>>(T, P, c->psi(in), c->psi(out), c->bkt(fine), list, nl, \
-                                                         c->ltag.p, c->ctl.p, c->tally(), acc, c->flags.p, D)
+        // timed launches take their events from the dispatch itself (the kernel's own start and
+        // end), so stage times and the tuner see the kernel without marker packets around it
+        hipEvent_t e_start = c->rec ? c->ev[(size_t)c->ev_it * sm2d_ctx::EPI + e0] : nullptr;
+        hipEvent_t e_stop = c->rec ? c->ev[(size_t)c->ev_it * sm2d_ctx::EPI + e0 + 1] : nullptr;
+#define SM2D_TRANSPORT(F, W)                                                                                  \
+    do {                                                                                                      \
+        if (c->rec)                                                                                           \
+            hipExtLaunchKernelGGL(k_transport<LY, F, W>, dim3(gb), dim3(64 * W), (uint32_t)dyn, c->stream,    \
+                                  e_start, e_stop, 0u, T, P, c->psi(in), c->psi(out), c->bkt(fine), list, nl, \
+                                  (const uint32_t*)c->ltag.p, (const uint32_t*)c->ctl.p, c->tally(), acc,     \
+                                  c->flags.p, D);                                                             \
+        else                                                                                                  \
+            k_transport<LY, F, W><<<gb, 64 * W, dyn, c->stream>>>(T, P, c->psi(in), c->psi(out), c->bkt(fine), \
+                                                                 list, nl, c->ltag.p, c->ctl.p, c->tally(),   \
+                                                                 acc, c->flags.p, D);                         \
+    } while (0)
         if (fine) {
             if (tw == 1) SM2D_TRANSPORT(true, 1);
             else if (tw == 2) SM2D_TRANSPORT(true, 2);
@@ -377,7 +394,6 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
             else SM2D_TRANSPORT(false, 4);
         }
 #undef SM2D_TRANSPORT
-        c->mark(e0 + 1);
         c->mark(e0 + 2);
         k_gather<LY><<<blocks_for(4ull * h, 1, 65536), 64, 0, c->stream>>>(
             T, c->psi(out), c->bkt(fine), c->dlist.p, ndest, c->ltag.p, c->lpos.p, c->ctl.p, fine, c->dcap,
