@@ -192,6 +192,8 @@ def main() -> int:
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device to device three times (HBM counter calibration)")
+    ap.add_argument("--only", choices=("batch", "learned"),
+                    help="print only that section (for profiling one workload); not the metric line")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -213,6 +215,13 @@ def main() -> int:
     torch.zeros(1, device="cuda:0")
     from sm_2d_amd import problems, runtime
     from sm_2d_amd.engine import Engine
+
+    if args.only:
+        hip = runtime.load_library()
+        sec = (batched_section(hip, args.batch_spots, args.plain_sample) if args.only == "batch"
+               else surrogate_section(hip))
+        print(json.dumps({"only": args.only, "section": sec}), flush=True)
+        return 0
 
     prob = problems.config2()
     hip = runtime.load_library()

@@ -204,13 +204,34 @@ __global__ void k_gelu_back(const float* __restrict__ dA, const float* __restric
     }
 }
 
-// Bias gradients: db[n] = sum_b dZ[b][n] in a fixed order (one thread per column).
-__global__ void k_colsum(const float* __restrict__ dZ, int Bn, int ld, int nvalid, float* db) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nvalid) return;
-    float s = 0.0f;
-    for (int b = 0; b < Bn; ++b) s += dZ[(size_t)b * ld + n];
-    db[n] = s;
+// Bias gradients: db[n] = sum_b dZ[b][n] in a fixed order.  A workgroup of 16 waves takes 64
+// columns (lane = column, so each row is read coalesced); wave w sums rows w, w+16, ... into four
+// running sums (rows 64k + w + 16j go to sum j), and the 64 partial sums of a column are then
+// added in (wave, sum) order.
+constexpr int COLSUM_WAVES = 16;
+__global__ __launch_bounds__(64 * COLSUM_WAVES) void k_colsum(const float* __restrict__ dZ, int Bn, int ld,
+                                                              int nvalid, float* db) {
+    __shared__ float part[COLSUM_WAVES * 4][64];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + l;
+    float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (n < nvalid) {
+        for (int b0 = 0; b0 < Bn; b0 += 4 * COLSUM_WAVES) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int b = b0 + j * COLSUM_WAVES + w;
+                if (b < Bn) s[j] += dZ[(size_t)b * ld + n];
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) part[w * 4 + j][l] = s[j];
+    __syncthreads();
+    if (w == 0 && n < nvalid) {
+        float t = 0.0f;
+        for (int k = 0; k < COLSUM_WAVES * 4; ++k) t += part[k][l];
+        db[n] = t;
+    }
 }
 
 // Fixed-order sum of per-sample losses (one workgroup).

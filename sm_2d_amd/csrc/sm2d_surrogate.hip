@@ -95,7 +95,7 @@ int mlp_backward(sm2d_mlp* M, int Bn, float w_att, float w_mom) {
     if ((rc = gemm(st, w.dZb4.p, 1, NOP, w.A3.p, NH, 1, NOP, NH, Bn, KSPLIT_W, w.P.p, NOUT, NH, nullptr, 0,
                    g + OFF_W3, NH, nullptr, 0)))
         return rc;
-    k_colsum<<<1, 64, 0, st>>>(w.dZ4.p, Bn, NOP, NOUT, g + OFF_B3);
+    k_colsum<<<1, 64 * COLSUM_WAVES, 0, st>>>(w.dZ4.p, Bn, NOP, NOUT, g + OFF_B3);
     if ((rc = gemm(st, w.dZb4.p, NOP, 1, wb + BOFF_W3, NH, 1, Bn, NH, NOP, 1, w.P.p, Bn, NH, nullptr, 0, w.dA.p, NH,
                    nullptr, 0)))
         return rc;
@@ -119,7 +119,7 @@ int mlp_backward(sm2d_mlp* M, int Bn, float w_att, float w_mom) {
         if ((rc = gemm(st, w.dZb.p, 1, NH, y.Ain, y.ldin, 1, NH, y.kin, Bn, KSPLIT_W, w.P.p, NH, nval, nullptr, 0,
                        g + y.offW, nval, nullptr, 0)))
             return rc;
-        k_colsum<<<(NH + 63) / 64, 64, 0, st>>>(w.dZ.p, Bn, NH, NH, g + y.offB);
+        k_colsum<<<(NH + 63) / 64, 64 * COLSUM_WAVES, 0, st>>>(w.dZ.p, Bn, NH, NH, g + y.offB);
         if (li < 2) {  // dA of the previous layer = dZ W
             if ((rc = gemm(st, w.dZb.p, NH, 1, wb + (li == 0 ? BOFF_W2 : BOFF_W1), NH, 1, Bn, NH, NH, 1, w.P.p, Bn, NH,
                            nullptr, 0, w.dA.p, NH, nullptr, 0)))
