@@ -205,31 +205,34 @@ __global__ void k_gelu_back(const float* __restrict__ dA, const float* __restric
 }
 
 // Bias gradients: db[n] = sum_b dZ[b][n] in a fixed order.  A workgroup of 16 waves takes 64
-// columns (lane = column, so each row is read coalesced); wave w sums rows w, w+16, ... into four
-// running sums (rows 64k + w + 16j go to sum j), and the 64 partial sums of a column are then
-// added in (wave, sum) order.
+// columns (lane = column, so each row is read coalesced).  Wave w owns rows w, w+16, ...; it loads
+// them 16 at a time (all in flight together) and adds them in row order into one running sum per
+// lane, and the 16 waves' sums of a column are then added in wave order.
 constexpr int COLSUM_WAVES = 16;
 __global__ __launch_bounds__(64 * COLSUM_WAVES) void k_colsum(const float* __restrict__ dZ, int Bn, int ld,
                                                               int nvalid, float* db) {
-    __shared__ float part[COLSUM_WAVES * 4][64];
+    constexpr int U = 16;
+    __shared__ float part[COLSUM_WAVES][64];
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int n = blockIdx.x * 64 + l;
-    float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float s = 0.0f;
     if (n < nvalid) {
-        for (int b0 = 0; b0 < Bn; b0 += 4 * COLSUM_WAVES) {
+        for (int b0 = w; b0 < Bn; b0 += U * COLSUM_WAVES) {
+            float v[U];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int b = b0 + j * COLSUM_WAVES + w;
-                if (b < Bn) s[j] += dZ[(size_t)b * ld + n];
+            for (int j = 0; j < U; ++j) {
+                const int b = b0 + j * COLSUM_WAVES;
+                v[j] = b < Bn ? dZ[(size_t)b * ld + n] : 0.0f;
             }
+#pragma unroll
+            for (int j = 0; j < U; ++j) s += v[j];
         }
     }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) part[w * 4 + j][l] = s[j];
+    part[w][l] = s;
     __syncthreads();
     if (w == 0 && n < nvalid) {
         float t = 0.0f;
-        for (int k = 0; k < COLSUM_WAVES * 4; ++k) t += part[k][l];
+        for (int k = 0; k < COLSUM_WAVES; ++k) t += part[k][l];
         db[n] = t;
     }
 }
