@@ -1,0 +1,89 @@
+"""GPU parity at the benchmark workloads' full sizes.
+
+The plain 150 MeV spot on water 100 x 200 mm at 0.5 mm (200 x 400) and the 230 MeV spot
+through seeded water / bone / lung slabs on 100 x 300 mm at 0.25 mm (400 x 1200) are compared
+with the CPU oracle bit for bit: the oracle transports them in about 4 s and 45 s.  The
+256-spot batch on 200 x 600 cells is checked spot by spot against the single-spot GPU path
+(every spot), its FP64 sum against the ordered sum of those single spots, and its two extreme
+spots (lowest and highest energy) against the oracle.  Every run must also close the per-cell
+weight balance below 1e-6 and flag no audit failure.
+"""
+import numpy as np
+import pytest
+
+from sm_2d_amd import problems
+from sm_2d_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+REPORT_SUMS = ("E_dep", "W_nuclear", "E_nuclear", "W_boundary", "E_boundary", "W_cutoff", "E_cutoff",
+               "W_leftover")
+
+
+def _oracle_spot(orc, prob, i=0):
+    from oracle.pyoracle import OracleEngine
+    o = problems.setup_engine(orc, prob, OracleEngine, spot_index=i)
+    return o.run_spot(prob.make_spot(i), prob.make_transport())
+
+
+def _same_spot(eg, ag, eo, ao):
+    assert ag.terminated == ao.terminated == 1
+    assert ag.iterations == ao.iterations
+    assert np.array_equal(eg, eo)
+    assert ag.cell_bin_updates == ao.cell_bin_updates
+    assert ag.processed_components == ao.processed_components
+    # K5 report sums: FP64 reductions in different (fixed) orders on the two sides
+    for k in REPORT_SUMS:
+        assert getattr(ag, k) == pytest.approx(getattr(ao, k), rel=1e-12, abs=1e-15), k
+    assert ag.max_cell_W_closure < 1e-6 and ag.W_fail_iters == 0
+
+
+@pytest.mark.parametrize("make", [problems.config2, problems.config3],
+                         ids=["150MeV_water_200x400", "230MeV_slabs_400x1200"])
+def test_full_size_spot_bit_exact(hip, orc, make):
+    prob = make()
+    print(prob.name, flush=True)
+    g = problems.setup_engine(hip, prob, Engine)
+    sp, tp = prob.make_spot(), prob.make_transport()
+    eg, ag = g.run_spot(sp, tp)
+    # a second run on the same context repeats the first bit for bit
+    eg2, ag2 = g.run_spot(sp, tp)
+    assert np.array_equal(eg, eg2) and ag.iterations == ag2.iterations
+    g.close()
+    eo, ao = _oracle_spot(orc, prob)
+    _same_spot(eg, ag, eo, ao)
+    # the beam deposits from the entrance row on
+    assert eg[0].sum() > 0
+
+
+def test_full_size_batch_256_spots(hip, orc):
+    prob = problems.config4()
+    print(prob.name, flush=True)
+    n = len(prob.spots)
+    spots = [prob.make_spot(i) for i in range(n)]
+    tp = prob.make_transport()
+    batch = problems.setup_engine(hip, prob, Engine, n_spots=n)
+    esum, per, ab = batch.run_batch(spots, tp, per_spot=True)
+    batch.close()
+    assert ab.terminated == 1 and ab.W_fail_iters == 0 and ab.max_cell_W_closure < 1e-6
+
+    single = problems.setup_engine(hip, prob, Engine)
+    total = None
+    updates = 0
+    singles = {}
+    E0 = np.array([s["E0"] for s in prob.spots])
+    extremes = (int(E0.argmin()), int(E0.argmax()))
+    for i, sp in enumerate(spots):
+        e, a = single.run_spot(sp, tp)
+        assert a.terminated == 1 and a.W_fail_iters == 0, i
+        assert np.array_equal(per[i], e.astype(np.float32)), i
+        total = e.copy() if total is None else total + e  # spot order, as the batch sums
+        updates += a.cell_bin_updates
+        if i in extremes:
+            singles[i] = (e, a)
+    single.close()
+    assert np.array_equal(esum, total)
+    assert ab.cell_bin_updates == updates
+    for i in extremes:
+        eo, ao = _oracle_spot(orc, prob, i)
+        _same_spot(*singles[i], eo, ao)
