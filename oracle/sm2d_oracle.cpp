@@ -9,7 +9,8 @@
 // per-component physics come from sm_2d_amd/csrc/sm2d_math.h (the same source the GPU
 // kernels compile), so that GPU and oracle agree bit for bit.  Host table builders
 // (grids, R-LUT, Fermi–Eyges precompute, post-convolution) use the C library exactly as the
-// reference's host code does.
+// reference's host code does.  Built with -DORC_FP64, the transport step runs in binary64
+// instead (the FP32 results' error is stated against that build).
 //
 // Pinning (what checks each part; tests/ names the test):
 //  - grids, R-LUT: bit-exact against the reference's own grids.cpp / r_lut.cpp compiled
@@ -179,6 +180,31 @@ float host_highland(float E, float ds, float X0 = 360.8f) {
     float b = fmaxf(1.0f + 0.038f * logf(t), 0.25f);
     return (13.6f * 1.0f / (beta * p)) * sqrtf(t) * b;
 }
+
+// ---------------------------------------------------------------------------------------
+// FP64 build (-DORC_FP64 -> _build/liboracle_sm2d_fp64.so).  From here to the C entry
+// points every binary32 quantity of the transport step is binary64: phase-space weights,
+// per-component physics, emission and spread weights, the FP32 weight tallies and the
+// tables the context holds.  The transcendentals become the C library's double functions.
+// Inputs stay the binary32 values the FP32 path gets: grids, range table, Fermi-Eyges
+// moments, the injected phase space and the model's constants (the float literals).  So
+// this build is the same algorithm evaluated in double, and the FP32 results' error is
+// measured against it.
+// ---------------------------------------------------------------------------------------
+#ifdef ORC_FP64
+#define float double
+#define fmaxf fmax
+#define fminf fmin
+#define fabsf fabs
+#define floorf floor
+#define expf_(x) std::exp((double)(x))
+#define logf_(x) std::log((double)(x))
+#define sqrtf_(x) std::sqrt((double)(x))
+#define sinf_(x) std::sin((double)(x))
+#define cosf_(x) std::cos((double)(x))
+#define erff_(x) std::erf((double)(x))
+#endif
+using real_t = float;  // the transport step's arithmetic type (double in the FP64 build)
 
 // ---------------------------------------------------------------------------------------
 // Problem description and phase-space storage
@@ -1262,6 +1288,20 @@ void run_phase(orc_ctx* c, const sm2d_transport* tp, float sx0, const std::vecto
 
 }  // namespace
 
+#ifdef ORC_FP64
+#undef float
+#undef fmaxf
+#undef fminf
+#undef fabsf
+#undef floorf
+#undef expf_
+#undef logf_
+#undef sqrtf_
+#undef sinf_
+#undef cosf_
+#undef erff_
+#endif
+
 // ---------------------------------------------------------------------------------------
 // C interface
 // ---------------------------------------------------------------------------------------
@@ -1453,7 +1493,7 @@ int orc_ctx_create(const sm2d_grid* g, orc_ctx** out) {
     c->edep.assign(c->N, 0); c->E_cut.assign(c->N, 0); c->E_nuc.assign(c->N, 0); c->E_bnd.assign(c->N, 0);
     c->W_cut.assign(c->N, 0); c->W_nuc.assign(c->N, 0); c->W_bnd.assign(c->N, 0);
     c->bk.resize((size_t)c->N * 4);
-    c->bkval.reset(new float[(size_t)c->N * 4 * KB * c->LB]);
+    c->bkval.reset(new real_t[(size_t)c->N * 4 * KB * c->LB]);
     for (size_t i = 0; i < c->bk.size(); ++i) {
         c->bk[i].n = 0;
         c->bk[i].val = c->bkval.get() + i * KB * c->LB;
@@ -1528,8 +1568,8 @@ static bool inject_one(orc_ctx* c, int cell, float theta, float E, float w, floa
     x = fmaxf(0.0f, fminf(x, g.dx));
     z = fmaxf(0.0f, fminf(z, g.dz));
     int xs = c->xbin(x - g.dx * 0.5f), zs = c->zbin(z - g.dz * 0.5f);
-    const float* te = c->th_edges.data();
-    const float* ee = c->E_edges.data();
+    const real_t* te = c->th_edges.data();
+    const real_t* ee = c->E_edges.data();
     int NT = g.N_theta, NE = g.N_E;
     float tmin = te[0], tmax = te[NT], Emin = ee[0], Emax = ee[NE];
     theta = fmaxf(tmin, fminf(theta, tmax));
@@ -1656,7 +1696,7 @@ int orc_psi_dump(orc_ctx* c, uint32_t* bids, int cap, int32_t* cell, int32_t* sl
     int n = 0;
     for (int cl = 0; cl < c->N; ++cl) {
         for (int s = 0; s < p.nslot[cl]; ++s) {
-            const float* v = p.v(cl, s);
+            const real_t* v = p.v(cl, s);
             for (int l = 0; l < c->LB; ++l) {
                 if (v[l] == 0.0f) continue;
                 if (n < cap) {
@@ -1792,11 +1832,11 @@ int orc_run_spot(orc_ctx* c, const sm2d_spot* sp, const sm2d_transport* tp, doub
     orc_reset(c);
     // FE moments of this spot (init_pipeline_state, k1k6_pipeline.cu:1061-1212)
     {
-        std::vector<float> st(c->g.Nz);
-        c->feA.assign(c->g.Nz, 0.0f);
-        c->feB.assign(c->g.Nz, 0.0f);
-        int r0 = orc_fe_precompute(sp->E0, sp->sigma_x, c->g.Nz, c->g.dz, c->feA.data(), c->feB.data(), st.data());
+        std::vector<float> st(c->g.Nz), A(c->g.Nz, 0.0f), B(c->g.Nz, 0.0f);
+        int r0 = orc_fe_precompute(sp->E0, sp->sigma_x, c->g.Nz, c->g.dz, A.data(), B.data(), st.data());
         if (r0) return r0;
+        c->feA.assign(A.begin(), A.end());
+        c->feB.assign(B.begin(), B.end());
     }
     int rc = orc_inject(c, sp, au);
     if (rc) return rc;

@@ -1,4 +1,5 @@
-"""Readers for the reference's one-step dumps under tests/golden/one_step_case_{A,B,C,D}.
+"""Readers for the reference's one-step dumps under tests/golden/one_step_case_{A,B,C,D}, and
+the dose-error measure used against the binary64 oracle.
 
 The dumps were written by the reference's debug path (src/cuda/k1k6_pipeline.cu:195-291 for
 the raw ψ rows, :2127-2157 for the channel totals) for the INI cases stored beside them.
@@ -84,4 +85,18 @@ def rows_as_set(rows: dict, wdigits: int) -> dict:
     for c, b, l, w in zip(rows["cell"], rows["bid"], rows["lidx"], rows["w"]):
         key = (int(c), int(b), int(l))
         out[key] = out.get(key, 0.0) + float(w)
+    return out
+
+
+def dose_errors(e, e64):
+    """Largest relative difference of dose grid `e` from `e64` on voxels above 1 % and 10 % of
+    Dmax (keys 0.01, 0.1), and of the PDD on depths above 1 % of its maximum (key "pdd")."""
+    D = e64.max()
+    out = {}
+    for f in (0.01, 0.1):
+        m = e64 > f * D
+        out[f] = float((np.abs(e - e64)[m] / e64[m]).max())
+    p, p64 = e.sum(axis=1), e64.sum(axis=1)
+    m = p64 > 0.01 * p64.max()
+    out["pdd"] = float((np.abs(p - p64)[m] / p64[m]).max())
     return out

@@ -1,4 +1,5 @@
-"""GPU parity at the benchmark workloads' full sizes.
+"""GPU parity at the benchmark workloads' full sizes, and the error of the FP32 results against
+the oracle's binary64 build.
 
 The plain 150 MeV spot on water 100 x 200 mm at 0.5 mm (200 x 400) and the 230 MeV spot
 through seeded water / bone / lung slabs on 100 x 300 mm at 0.25 mm (400 x 1200) are compared
@@ -7,6 +8,12 @@ with the CPU oracle bit for bit: the oracle transports them in about 4 s and 45 
 (every spot), its FP64 sum against the ordered sum of those single spots, and its two extreme
 spots (lowest and highest energy) against the oracle.  Every run must also close the per-cell
 weight balance below 1e-6 and flag no audit failure.
+
+Error against binary64 (the same transport step with every FP32 quantity in double), measured
+on these runs as the largest relative difference on voxels above 1 % / 10 % of Dmax and on the
+PDD: 1.71e-4 / 1.71e-4 / 1.44e-4 for the 150 MeV water spot and 1.65e-3 / 7.9e-4 / 6.3e-4 for
+the slab phantom, where density jumps amplify the FP32 rounding of the range-energy steps.  The
+bounds below hold these with a margin.
 """
 import numpy as np
 import pytest
@@ -14,15 +21,22 @@ import pytest
 from sm_2d_amd import problems
 from sm_2d_amd.engine import Engine
 
+from fixtures import dose_errors
+
 pytestmark = pytest.mark.gpu
 
 REPORT_SUMS = ("E_dep", "W_nuclear", "E_nuclear", "W_boundary", "E_boundary", "W_cutoff", "E_cutoff",
                "W_leftover")
 
 
-def _oracle_spot(orc, prob, i=0):
-    from oracle.pyoracle import OracleEngine
-    o = problems.setup_engine(orc, prob, OracleEngine, spot_index=i)
+# largest relative error against binary64 allowed on voxels > 1 % Dmax, > 10 % Dmax, PDD
+FP64_BOUNDS = {"150MeV_water_200x400": (3e-4, 3e-4, 3e-4),
+               "230MeV_slabs_400x1200": (3e-3, 1.5e-3, 1.2e-3)}
+
+
+def _oracle_spot(orc, prob, i=0, fp64=False):
+    from oracle.pyoracle import OracleEngine, load
+    o = problems.setup_engine(load(fp64=True) if fp64 else orc, prob, OracleEngine, spot_index=i)
     return o.run_spot(prob.make_spot(i), prob.make_transport())
 
 
@@ -38,9 +52,10 @@ def _same_spot(eg, ag, eo, ao):
     assert ag.max_cell_W_closure < 1e-6 and ag.W_fail_iters == 0
 
 
-@pytest.mark.parametrize("make", [problems.config2, problems.config3],
+@pytest.mark.parametrize("make,name", [(problems.config2, "150MeV_water_200x400"),
+                                       (problems.config3, "230MeV_slabs_400x1200")],
                          ids=["150MeV_water_200x400", "230MeV_slabs_400x1200"])
-def test_full_size_spot_bit_exact(hip, orc, make):
+def test_full_size_spot_bit_exact(hip, orc, make, name):
     prob = make()
     print(prob.name, flush=True)
     g = problems.setup_engine(hip, prob, Engine)
@@ -54,6 +69,14 @@ def test_full_size_spot_bit_exact(hip, orc, make):
     _same_spot(eg, ag, eo, ao)
     # the beam deposits from the entrance row on
     assert eg[0].sum() > 0
+    # error of the FP32 dose against the binary64 build of the oracle
+    e64, a64 = _oracle_spot(orc, prob, fp64=True)
+    assert a64.iterations == ag.iterations and a64.max_cell_W_closure < 1e-12
+    err = dose_errors(eg, e64)
+    print(f"{name}: error vs FP64: dose>1% {err[0.01]:.3e}, dose>10% {err[0.1]:.3e}, "
+          f"PDD {err['pdd']:.3e}, E_dep {abs(ag.E_dep - a64.E_dep) / a64.E_dep:.3e}", flush=True)
+    b1, b10, bp = FP64_BOUNDS[name]
+    assert err[0.01] < b1 and err[0.1] < b10 and err["pdd"] < bp
 
 
 def test_full_size_batch_256_spots(hip, orc):
