@@ -8,8 +8,10 @@ inputs are synthetic (the seeded Gaussian source of the reference injector).
 
 Reported: cell-bin updates/s (value), spots/s, ms per spot, the roofline of the dominant
 kernel (transport, HBM-bound: 8 B per cell-bin update = one FP32 read of psi_in + one FP32
-write of psi_out, divided by the kernel time from HIP events on the library's stream) and the
-CPU oracle timed on the same host on a bounded sample of the same workload.
+write of psi_out, divided by the kernel time from HIP events on the library's stream), the same
+for the K4 gather (8 B per non-zero bucket value moved = one FP32 read of the bucket + one FP32
+write of psi_out) and the CPU oracle timed on the same host on a bounded sample of the same
+workload.
 """
 from __future__ import annotations
 
@@ -27,7 +29,7 @@ HBM_PEAK_GBS = 8000.0
 
 
 def _profiles_traffic(name: str):
-    """HBM bytes per transport launch measured by rocprofv3 counters, when a summary exists."""
+    """HBM bytes per launch of a kernel measured by rocprofv3 counters, when its summary exists."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
@@ -259,6 +261,9 @@ def main() -> int:
         dom_ms, dom_upd = tm.fine_transport_ms, tm.fine_updates
     achieved = 8.0 * dom_upd / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
     traffic = _profiles_traffic("transport_config2.txt")
+    # K4: the coarse-phase gather (one launch per iteration while profiling)
+    g_ms, g_val = tm.coarse_gather_ms, tm.coarse_gather_values
+    g_achieved = 8.0 * g_val / (g_ms * 1e-3) / 1e9 if g_ms > 0 else 0.0
     out = {
         "metric": "cell-bin updates/s + spots/s (batched+surrogate vs plain) at <=1% dose diff",
         "value": updates / dt,
@@ -298,6 +303,19 @@ def main() -> int:
             "launches": tm.iterations,
             "avg_launch_us": dom_ms * 1e3 / max(1, tm.iterations),
             "updates_per_launch": dom_upd / max(1, tm.iterations),
+        },
+        "roofline_k4": {
+            "bound": "hbm",
+            "kernel": "k_gather (coarse phase)",
+            "achieved": g_achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": g_achieved / HBM_PEAK_GBS,
+            "traffic": _profiles_traffic("gather_config2.txt"),
+            "algorithmic_bytes_per_value": 8,
+            "launches": tm.iterations,
+            "avg_launch_us": g_ms * 1e3 / max(1, tm.iterations),
+            "values_per_launch": g_val / max(1, tm.iterations),
         },
         "transport_tuner": tuner,
         "audit": {"max_cell_W_closure": a.max_cell_W_closure, "W_fail_iters": a.W_fail_iters,

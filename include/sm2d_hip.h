@@ -123,6 +123,9 @@ typedef struct sm2d_timing {
     uint64_t cell_bin_updates;   /* updates done by the timed transport launches */
     double coarse_transport_ms, fine_transport_ms;  /* transport_ms split by phase */
     uint64_t coarse_updates, fine_updates;          /* cell_bin_updates split by phase */
+    double coarse_gather_ms, fine_gather_ms;        /* gather_ms split by phase */
+    uint64_t coarse_gather_values, fine_gather_values;  /* non-zero bucket values the timed
+                                                          gather launches moved into psi_out */
 } sm2d_timing;
 
 typedef struct sm2d_ctx sm2d_ctx;

@@ -145,6 +145,8 @@ class Timing(C.Structure):
         ("cell_bin_updates", C.c_uint64),
         ("coarse_transport_ms", C.c_double), ("fine_transport_ms", C.c_double),
         ("coarse_updates", C.c_uint64), ("fine_updates", C.c_uint64),
+        ("coarse_gather_ms", C.c_double), ("fine_gather_ms", C.c_double),
+        ("coarse_gather_values", C.c_uint64), ("fine_gather_values", C.c_uint64),
     ]
 
     def __init__(self):

@@ -2234,13 +2234,15 @@ __global__ void k_dest(const uint32_t* list, const uint32_t* nlist, int Nx, int 
 // in order), one wavefront per destination.  The destination's payloads are staged in LDS rows
 // (existing ones loaded, new ones zeroed), the adds happen in LDS in the serial order, and the
 // new payloads get their pool slots with one reservation when the destination is done.  The
-// next bucket payload is fetched while the current one is added.
-template <class LY>
+// next bucket payload is fetched while the current one is added.  COUNT (profiling launches
+// only) also adds the destination's non-zero bucket values to *moved.
+template <class LY, bool COUNT>
 __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const uint32_t* __restrict__ dlist,
                                                const uint32_t* __restrict__ ndest,
                                                const uint32_t* __restrict__ ltag,
                                                const uint32_t* __restrict__ lpos, const uint32_t* ctl, int fine,
-                                               uint32_t dcap, double* k4drop_w, double* k4drop_E, uint32_t* flags) {
+                                               uint32_t dcap, double* k4drop_w, double* k4drop_E, uint32_t* flags,
+                                               uint32_t* moved) {
     if (ctl[C_DONE]) return;
     constexpr int NLG = 4096 / LY::LB;  // 16 KiB of payload rows
     __shared__ float s_rows[NLG * LY::LB];
@@ -2301,6 +2303,7 @@ __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const ui
             ref = rlu(rr, (int)e);
         };
         double dw = 0.0, de = 0.0;
+        uint32_t nmoved = 0;  // non-zero bucket values of this destination (profiling only)
         float v[LY::PL], vn[LY::PL];
         uint32_t bid = EMPTY, bref = 0;
         if (ne) {
@@ -2317,6 +2320,10 @@ __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const ui
                     vn[k] = bidn != EMPTY ? bk.pool[(size_t)brefn * LY::LB + l + 64 * k] : 0.0f;
             }
             if (bid != EMPTY) {
+                if constexpr (COUNT) {
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k) nmoved += (uint32_t)__popcll(__ballot(v[k] > 0.0f));
+                }
                 uint32_t sel;
                 const int os = W.claim(0, bid, sel, out.pool, out.pcount, out.pcap, flags, F_PSI_FULL);
                 if (os < 0) {
@@ -2391,6 +2398,7 @@ __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const ui
         if (l == 0) {
             k4drop_w[d] += dw;
             k4drop_E[d] += de;
+            if (COUNT && nmoved) atomicAdd(moved, nmoved);
         }
     }
 }

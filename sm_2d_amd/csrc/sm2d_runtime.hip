@@ -118,6 +118,7 @@ struct sm2d_ctx {
     DBuf<uint32_t> bbid, bn, bsref;
     DBuf<float> bpool;
     DBuf<uint32_t> maskv, ltag, lpos, dtag, act, crs, dlist, ctl, tcount, tcount2, flags;
+    DBuf<uint32_t> gmoved;  // profiling: bucket values moved per (iteration slot, phase) of a chunk
     DBuf<uint8_t> flag;
     DBuf<double> edep, E_cut, E_nuc, E_bnd, win, ein, k4w, k4e, acc_c, acc_f, pe, part, k5v, hist, edep_sum;
     DBuf<float> w_cut, w_nuc, w_bnd, edep_f;
@@ -394,11 +395,20 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
             else SM2D_TRANSPORT(false, 4);
         }
 #undef SM2D_TRANSPORT
-        c->mark(e0 + 2);
-        k_gather<LY><<<blocks_for(4ull * h, 1, 65536), 64, 0, c->stream>>>(
-            T, c->psi(out), c->bkt(fine), c->dlist.p, ndest, c->ltag.p, c->lpos.p, c->ctl.p, fine, c->dcap,
-            c->k4w.p, c->k4e.p, c->flags.p);
-        c->mark(e0 + 3);
+        uint32_t* moved = c->prof ? c->gmoved.p + 2 * c->ev_it + fine : nullptr;
+        const unsigned gg = blocks_for(4ull * h, 1, 65536);
+        if (c->rec)
+            hipExtLaunchKernelGGL(c->prof ? k_gather<LY, true> : k_gather<LY, false>, dim3(gg), dim3(64), 0u,
+                                  c->stream,
+                                  c->ev[(size_t)c->ev_it * sm2d_ctx::EPI + e0 + 2],
+                                  c->ev[(size_t)c->ev_it * sm2d_ctx::EPI + e0 + 3], 0u, T, c->psi(out),
+                                  c->bkt(fine), (const uint32_t*)c->dlist.p, (const uint32_t*)ndest,
+                                  (const uint32_t*)c->ltag.p, (const uint32_t*)c->lpos.p,
+                                  (const uint32_t*)c->ctl.p, fine, c->dcap, c->k4w.p, c->k4e.p, c->flags.p, moved);
+        else
+            k_gather<LY, false><<<gg, 64, 0, c->stream>>>(T, c->psi(out), c->bkt(fine), c->dlist.p, ndest, c->ltag.p,
+                                                   c->lpos.p, c->ctl.p, fine, c->dcap, c->k4w.p, c->k4e.p,
+                                                   c->flags.p, moved);
     }
     build_occ(c, out);
     c->mark(10);
@@ -433,6 +443,11 @@ int buffer_totals(sm2d_ctx* c, int k, double& W, double& E) {
 // after the transport ended return at once and are not counted).
 void collect_timing(sm2d_ctx* c, int n_valid) {
     if (!c->prof) return;
+    uint32_t moved[2 * CHUNK] = {};
+    if (n_valid > 0 &&
+        hipMemcpy(moved, c->gmoved.p, sizeof(uint32_t) * 2 * std::min(n_valid, CHUNK), hipMemcpyDeviceToHost) !=
+            hipSuccess)
+        return;
     for (int r = 0; r < n_valid; ++r) {
         hipEvent_t* e = c->ev.data() + (size_t)r * sm2d_ctx::EPI;
         auto span = [&](int a, int b) -> double {
@@ -444,7 +459,11 @@ void collect_timing(sm2d_ctx* c, int n_valid) {
         c->timing.transport_ms += span(2, 3) + span(6, 7);
         c->timing.coarse_transport_ms += span(2, 3);
         c->timing.fine_transport_ms += span(6, 7);
-        c->timing.gather_ms += span(3, 5) + span(7, 9);
+        c->timing.gather_ms += span(4, 5) + span(8, 9);
+        c->timing.coarse_gather_ms += span(4, 5);
+        c->timing.fine_gather_ms += span(8, 9);
+        c->timing.coarse_gather_values += moved[2 * r];
+        c->timing.fine_gather_values += moved[2 * r + 1];
         c->timing.audit_ms += span(9, 11);
         c->timing.transport_launches += 2;
         c->timing.gather_launches += 2;
@@ -604,6 +623,7 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
             explore[ph] = !c->tune.forced && bkt[ph] > 0 && !c->tune.done[ph][bkt[ph]];
         }
         c->rec = c->prof || explore[0] || explore[1];
+        if (c->prof) HIPCHK(hipMemsetAsync(c->gmoved.p, 0, sizeof(uint32_t) * 2 * CHUNK, c->stream));
         c->cand_log.assign((size_t)2 * R, 0);
         if (!c->rec && c->graphs_on) {
             // the same shapes for every iteration of the chunk (no exploration)
@@ -898,6 +918,7 @@ int each_buffer(sm2d_ctx* c, A&& a) {
     HIPCHK(a(c->crs, c->ocap, BK_OTHER));
     HIPCHK(a(c->dlist, c->dcap, BK_OTHER));
     HIPCHK(a(c->ctl, NCTL, BK_OTHER));
+    HIPCHK(a(c->gmoved, (size_t)2 * CHUNK, BK_OTHER));
     HIPCHK(a(c->tcount, 2 * ((size_t)c->ocap / 256 + 2), BK_OTHER));
     HIPCHK(a(c->tcount2, (size_t)c->nwords / 256 + 2, BK_OTHER));
     HIPCHK(a(c->flags, NFLAG, BK_OTHER));

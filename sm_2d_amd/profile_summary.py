@@ -6,9 +6,12 @@
 
     python -m sm_2d_amd.profile_summary traffic --fetch DIR --write DIR --cal-fetch DIR
         --cal-write DIR --kernel REGEX --algorithmic-bytes B [--copy-bytes N]
+        [--stride S --offset O]
         HBM bytes per dispatch of the kernels matching REGEX from two counter runs
         (rocprofv3 --pmc FETCH_SIZE, then --pmc WRITE_SIZE), each scaled by what the same
         counter reads for a plain device copy of N bytes (bench.py --copy-calibration).
+        --stride/--offset keep every S-th matching dispatch from the O-th on, in dispatch
+        order (a kernel launched once per phase: stride 2 separates the phases).
 """
 from __future__ import annotations
 
@@ -44,13 +47,14 @@ def stats(d: str, cmd: str) -> str:
     return "\n".join(lines)
 
 
-def _counter(d: str, regex: str) -> list[float]:
+def _counter(d: str, regex: str, stride: int = 1, offset: int = 0) -> list[float]:
     pat = re.compile(regex)
-    vals = []
+    rows = []
     for r in csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))):
         if pat.search(r["Kernel_Name"]):
-            vals.append(float(r["Counter_Value"]))
-    return vals
+            rows.append((int(r["Dispatch_Id"]), float(r["Counter_Value"])))
+    rows.sort()
+    return [v for _, v in rows[offset::stride]]
 
 
 def traffic(a) -> str:
@@ -61,8 +65,8 @@ def traffic(a) -> str:
         raise SystemExit("no copy dispatch in the calibration runs")
     f_fetch = copy_kib / max(cf)
     f_write = copy_kib / max(cw)
-    fk = _counter(a.fetch, a.kernel)
-    wk = _counter(a.write, a.kernel)
+    fk = _counter(a.fetch, a.kernel, a.stride, a.offset)
+    wk = _counter(a.write, a.kernel, a.stride, a.offset)
     if not fk or not wk:
         raise SystemExit(f"no dispatch of {a.kernel!r} in the counter runs")
     fetch = sum(fk) / len(fk)
@@ -94,6 +98,8 @@ def main(argv=None) -> int:
     t.add_argument("--algorithmic-bytes", type=float, required=True)
     t.add_argument("--copy-bytes", type=int, default=256 * 1024 * 1024)
     t.add_argument("--copy-kernel", default="copyBuffer")
+    t.add_argument("--stride", type=int, default=1)
+    t.add_argument("--offset", type=int, default=0)
     a = ap.parse_args(argv)
     print(stats(a.dir, a.cmd) if a.what == "stats" else traffic(a))
     return 0
