@@ -67,10 +67,13 @@ def cpu_baseline(prob, budget_s: float = 20.0) -> dict:
                       f"cell-bin updates in {dt:.2f} s, serial C++ oracle, -O2)"}
 
 
-def batched_section(hip, n_spots: int, plain_sample: int) -> dict:
+def batched_section(hip, n_spots: int, plain_sample: int, mlp=None) -> dict:
     """The 256-spot workload (pencil beams of 70-230 MeV over +-40 mm with sigma 3-6 mm on water
     200 x 600 at 0.5 mm): one batched launch sequence for all spots against the plain path
-    (one spot at a time) timed on the first `plain_sample` spots of the same list."""
+    (one spot at a time) timed on the first `plain_sample` spots of the same list.  With a
+    trained network `mlp`, the same batch again with the learned fine-transport physics
+    attached: spots/s against the plain path, and its summed dose against the plain batch's on
+    voxels above 10 % of the maximum."""
     import torch
     from sm_2d_amd import problems
     from sm_2d_amd.engine import Engine
@@ -100,9 +103,31 @@ def batched_section(hip, n_spots: int, plain_sample: int) -> dict:
     eng.run_batch(spots, tp)
     tm = eng.timing()
     tuner = eng.tuner_report()
-    eng.close()
     plain_sps = plain_sample / dt_plain
     batch_sps = n_spots / dt
+    learned = None
+    if mlp is not None:
+        import numpy as np
+        eng.set_profiling(False)
+        eng.surrogate_attach(mlp, 64)
+        eng.run_batch(spots[:min(8, n_spots)], tp)  # captures the chunk graphs again
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        es, _, al = eng.run_batch(spots, tp)
+        torch.cuda.synchronize()
+        dt_l = time.perf_counter() - t0
+        m10 = esum > 0.1 * esum.max()
+        rel = np.abs(es - esum)[m10] / esum[m10]
+        learned = {"batched_learned_spots_per_s": n_spots / dt_l, "batched_learned_s": dt_l,
+                   "speedup_vs_plain_spots_per_s": n_spots / dt_l / plain_sps,
+                   "dose_vs_plain_batch": {"total_rel": float(es.sum() / esum.sum() - 1.0),
+                                           "voxel_gt10pct_mean_rel": float(rel.mean()),
+                                           "voxel_gt10pct_p95_rel": float(np.percentile(rel, 95)),
+                                           "voxel_gt10pct_max_rel": float(rel.max()),
+                                           "voxels_within_1pct": float((rel <= 0.01).mean())},
+                   "audit": {"surrogate_active": al.surrogate_active, "fallback": al.surrogate_fallback,
+                             "max_cell_W_closure": al.max_cell_W_closure, "W_fail_iters": al.W_fail_iters}}
+    eng.close()
     return {"workload": f"{n_spots} pencil beams, E0 U(70,230) MeV, x0 U(-40,40) mm, sigma_x U(3,6) mm, "
                         "water 200x600 @ 0.5 mm, 256 log E x 512 theta, 128 local bins",
             "batched_spots_per_s": batch_sps, "batched_s": dt,
@@ -110,12 +135,13 @@ def batched_section(hip, n_spots: int, plain_sample: int) -> dict:
             "batched_iterations": a.iterations, "batched_cell_bin_updates": a.cell_bin_updates,
             "plain_spots_per_s": plain_sps, "plain_sample_spots": plain_sample,
             "speedup_spots_per_s": batch_sps / plain_sps, "surrogate": False,
+            "batched_learned": learned,
             "warm_batch_s": t_warm,
             "stage_ms": {k: getattr(tm, k) for k in ("mask_ms", "transport_ms", "gather_ms", "audit_ms")},
             "transport_tuner": tuner}
 
 
-def surrogate_section(hip) -> dict:
+def surrogate_section(hip, keep: list | None = None) -> dict:
     """The learned fine-transport physics against the plain physics on a 60 MeV water spot whose
     whole track runs in the fine transport (where the learned widths act): one-time training and
     table build, ms per spot of both, and the dose comparison (total deposited energy, 4 x 4
@@ -161,7 +187,10 @@ def surrogate_section(hip) -> dict:
     bp, bs = blk(ep), blk(es)
     mk = bp > 0.1 * bp.max()
     rel = np.abs(bs - bp)[mk] / bp[mk]
-    m.close()
+    if keep is not None:
+        keep.append(m)  # the trained network, for the batched section
+    else:
+        m.close()
     eng.close()
     flops_train = 3 * 75008 * cfg["batch"] * cfg["steps"]
     return {"workload": "60 MeV water spot, 40 x 80 cells @ 1 mm (reference grid), all energies in "
@@ -323,10 +352,14 @@ def main() -> int:
                   "E_boundary_MeV": a.E_boundary, "E_dep_MeV": a.E_dep},
     }
     eng.close()
-    if not args.no_batch:
-        out["batched"] = batched_section(hip, args.batch_spots, args.plain_sample)
+    trained = []
     if not args.no_surrogate:
-        out["learned_physics"] = surrogate_section(hip)
+        out["learned_physics"] = surrogate_section(hip, keep=trained)
+    if not args.no_batch:
+        out["batched"] = batched_section(hip, args.batch_spots, args.plain_sample,
+                                         mlp=trained[0] if trained else None)
+    for m in trained:
+        m.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(prob, args.cpu_budget)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

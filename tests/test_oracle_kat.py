@@ -11,7 +11,7 @@ import math
 import numpy as np
 import pytest
 
-from sm_2d_amd import abi
+from sm_2d_amd import abi, problems
 
 
 def _k1_engine(orc, Nx=4, Nz=4):
@@ -234,3 +234,31 @@ def test_device_step_rule(phys):
         elif Ei < 20:
             d = min(d, 0.5)
         assert g == pytest.approx(d, rel=1e-6)
+
+
+def _min_E_block_and_fine(orc, prob, n_iter):
+    from oracle.pyoracle import OracleEngine
+    e = problems.setup_engine(orc, prob, OracleEngine)
+    tp = prob.make_transport()
+    e.inject(prob.make_spot())
+    _, r0 = e.psi_dump()
+    fine = 0
+    for _ in range(n_iter // 10):
+        _, act, _ = e.mask_compact(tp)
+        fine += len(act)
+        e.run_iterations(tp, 10)
+    _, r = e.psi_dump()
+    return int(((r0["bid"] >> 12) & 0xFFF).min()), int(((r["bid"] >> 12) & 0xFFF).min()), fine
+
+
+def test_coarse_steps_lock_to_wide_energy_bins(orc):
+    """K2 takes a component's energy at its bin centre and re-bins E - dE
+    (k2_coarsetransport.cu:167-170, 235, 488).  On the 256-bin log grid (bins 3.1 % wide) a
+    0.5 mm step loses less than half a bin, so a 70 MeV component stays in its injected energy
+    block and never reaches the fine phase (E <= 10 MeV); on the reference runner's piecewise
+    1029-bin grid the same beam slows down and the fine list fills."""
+    b0, b1, fine = _min_E_block_and_fine(orc, problems.config1(n_samples=100), 80)
+    assert b1 == b0 and fine == 0
+    b0, b1, fine = _min_E_block_and_fine(
+        orc, problems.reference_grid_case(Nx=40, Nz=100, E0=70.0, sigma_x=2.0, n_samples=100), 100)
+    assert b1 < b0 and fine > 0
