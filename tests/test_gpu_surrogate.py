@@ -104,6 +104,37 @@ def test_learned_transport_bit_exact_vs_oracle(hip, trained, orc):
     assert ag.max_cell_W_closure < 1e-6 and ag.W_fail_iters == 0
 
 
+def test_batched_learned_transport_equals_single_spots(hip, trained):
+    """The learned physics in a batch context: three all-fine spots transported together with
+    the single-spot context's table give, spot by spot, the single-spot learned EdepC, and the
+    FP64 sum of those in spot order."""
+    prob, _, m, _ = trained
+    single = problems.setup_engine(hip, prob, Engine)
+    single.surrogate_attach(m, 64)
+    tab, nm, nn, rows = single.surrogate_export()
+    spots = []
+    for i, (E0, x0, sx) in enumerate([(60.0, 0.0, 2.0), (45.0, -3.0, 1.5), (55.0, 4.0, 2.5)]):
+        sp = prob.make_spot()
+        sp.E0, sp.x0, sp.sigma_x, sp.random_seed = E0, x0, sx, 77 + i
+        spots.append(sp)
+    tp = prob.make_transport()
+    ref = []
+    for sp in spots:
+        e, a = single.run_spot(sp, tp)
+        assert a.surrogate_active == 1 and a.surrogate_fallback == 0
+        ref.append(e)
+    single.close()
+    batch = problems.setup_engine(hip, prob, Engine, n_spots=3)
+    batch.surrogate_set_table(tab, nm, nn, rows)
+    esum, per, ab = batch.run_batch(spots, tp, per_spot=True)
+    batch.close()
+    assert ab.surrogate_active == 1 and ab.surrogate_fallback == 0
+    for k in range(3):
+        assert np.array_equal(per[k], ref[k].astype(np.float32)), k
+    assert np.array_equal(esum, ref[0] + ref[1] + ref[2])
+    assert ab.max_cell_W_closure < 1e-6 and ab.W_fail_iters == 0
+
+
 def test_out_of_range_material_uses_plain_physics(hip, trained):
     prob, _, m, _ = trained
     g = problems.setup_engine(hip, prob, Engine)
