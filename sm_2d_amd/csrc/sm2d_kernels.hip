@@ -456,22 +456,39 @@ __global__ __launch_bounds__(1024) void k_compact_small(ListOut L, const uint32_
     compact_lists(L, *nocc, ctl[C_EP], ctl[C_IT], ctl, wc, base);
 }
 
-// One-workgroup form of k_occ_count + k_scan<1> + k_occ_emit for small grids.
+// One-workgroup form of k_occ_count + k_scan<1> + k_occ_emit for small grids (nwords <= 16384):
+// thread t loads its contiguous run of ceil(nwords / 1024) <= 16 bitmap words once, and one
+// workgroup scan of the per-thread popcounts gives its first list position, so the list comes
+// out ascending.  An over-long list leaves the bitmap and the list untouched (as the tiled form).
 __global__ __launch_bounds__(1024) void k_occ_small(uint32_t* bitmap, uint32_t nwords, uint32_t* occ, uint32_t cap,
                                                     uint32_t* nocc, uint32_t* ctl, uint32_t* flags) {
     if (ctl[C_DONE]) return;
+    constexpr int K = 16;
     __shared__ uint32_t sc[16];
-    __shared__ uint32_t base;
-    if (threadIdx.x == 0) base = 0;
-    __syncthreads();
-    // Pass 1: total, so that an over-long list leaves the bitmap and list untouched (as the
-    // tiled form).
+    const uint32_t per = (nwords + 1023u) / 1024u;
+    const uint32_t w0 = threadIdx.x * per;
+    uint32_t wv[K];
     uint32_t cnt = 0;
-    for (uint32_t i = threadIdx.x; i < nwords; i += 1024) cnt += __popc(bitmap[i]);
-    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
-    if (lane_id() == 0) atomicAdd(&base, cnt);
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const uint32_t i = w0 + (uint32_t)j;
+        wv[j] = ((uint32_t)j < per && i < nwords) ? bitmap[i] : 0u;
+        cnt += __popc(wv[j]);
+    }
+    const int w = threadIdx.x >> 6, l = lane_id();
+    uint32_t x = cnt;  // inclusive scan within the wave
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (l >= o) x += y;
+    }
+    if (l == 63) sc[w] = x;
     __syncthreads();
-    const uint32_t total = base;
+    uint32_t pos = x - cnt, total = 0;
+    for (int j = 0; j < 16; ++j) {
+        const uint32_t v = sc[j];
+        if (j < w) pos += v;
+        total += v;
+    }
     if (total > cap) {
         if (threadIdx.x == 0) {
             atomicOr(&flags[F_DEST_FULL], 1u);
@@ -481,28 +498,11 @@ __global__ __launch_bounds__(1024) void k_occ_small(uint32_t* bitmap, uint32_t n
         }
         return;
     }
-    __syncthreads();
-    const int w = threadIdx.x >> 6, l = lane_id();
-    uint32_t run = 0;  // words before this tile's
-    for (uint32_t t0 = 0; t0 < nwords; t0 += 1024) {
-        const uint32_t i = t0 + threadIdx.x;
-        uint32_t b = i < nwords ? bitmap[i] : 0u;
-        const uint32_t cb = __popc(b);
-        uint32_t x = cb;  // inclusive scan within the wave
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(x, o);
-            if (l >= o) x += y;
-        }
-        if (l == 63) sc[w] = x;
-        __syncthreads();
-        uint32_t pos = run + x - cb;
-        uint32_t tile = 0;
-        for (int j = 0; j < 16; ++j) {
-            const uint32_t v = sc[j];
-            if (j < w) pos += v;
-            tile += v;
-        }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        uint32_t b = wv[j];
         if (b) {
+            const uint32_t i = w0 + (uint32_t)j;
             bitmap[i] = 0u;
             while (b) {
                 const int k = __ffs(b) - 1;
@@ -511,8 +511,6 @@ __global__ __launch_bounds__(1024) void k_occ_small(uint32_t* bitmap, uint32_t n
                 ++pos;
             }
         }
-        run += tile;
-        __syncthreads();
     }
     if (threadIdx.x == 0) *nocc = total;
 }
