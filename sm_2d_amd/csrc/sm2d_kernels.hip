@@ -152,10 +152,12 @@ __device__ inline void k1_sums(const Tabs& T, const Psi& buf, uint32_t ns, uint3
 }
 
 // K1 decision for occupied cell c of buffer `buf` in epoch ep (k1_activemask.cu:6-48): any slot
-// at or below the on / off energy blocks, the weight test and the hysteresis through the tag of
-// epoch ep - 1.  A tag of epoch ep already present was decided by the previous iteration's
-// closing audit from the same phase space, and is kept.  Lane 0 writes the tag; returns the
-// list flag (1 active, 2 coarse, 0 nothing).
+// at or below the on / off energy blocks, the weight test and the hysteresis through the mask of
+// epoch ep - 1.  A mask tag is (epoch << 2) | (mask of epoch - 1) << 1 | mask of epoch, so a
+// cell whose epoch-ep decision was already made (the previous iteration's closing audit decides
+// it ahead, possibly in an earlier call with other thresholds) is decided again from the same
+// previous mask with the current thresholds.  Lane 0 writes the tag; returns the list flag (1
+// active, 2 coarse, 0 nothing).
 template <class LY>
 __device__ inline uint8_t k1_decide(const Tp& P, const Psi& buf, uint32_t c, uint32_t ns, uint32_t mb, double W64,
                                     bool bad, uint32_t ep, uint32_t* maskv) {
@@ -180,15 +182,10 @@ __device__ inline uint8_t k1_decide(const Tp& P, const Psi& buf, uint32_t c, uin
     }
     uint8_t f = 0;
     if (l == 0 && ns) {
-        const uint32_t tag = maskv[c];
-        bool m;
-        if ((tag >> 1) == ep) {
-            m = (tag & 1u) != 0;
-        } else {
-            const bool prev = tag == (((ep - 1u) << 1) | 1u);
-            m = (on || (prev && off)) && pass;
-            maskv[c] = (ep << 1) | (m ? 1u : 0u);
-        }
+        const uint32_t tag = maskv[c], te = tag >> 2;
+        const bool prev = (te == ep) ? ((tag & 2u) != 0) : (te == ep - 1u && (tag & 1u));
+        const bool m = (on || (prev && off)) && pass;
+        maskv[c] = (ep << 2) | (prev ? 2u : 0u) | (m ? 1u : 0u);
         f = m ? 1 : (W64 > 0.0 ? 2 : 0);
     }
     return f;

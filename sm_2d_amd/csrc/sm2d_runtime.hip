@@ -1310,7 +1310,7 @@ int sm2d_k1k2_mask_compact(sm2d_ctx* c, const sm2d_transport* tp, const uint8_t*
     const uint32_t ep = ctl[C_EP] + 1;  // the epoch k_begin is about to open
     std::vector<uint32_t> mv(c->N, 0);
     if (mask_prev)
-        for (int i = 0; i < c->N; ++i) mv[i] = mask_prev[i] ? (((ep - 1u) << 1) | 1u) : 0u;
+        for (int i = 0; i < c->N; ++i) mv[i] = mask_prev[i] ? (((ep - 1u) << 2) | 1u) : 0u;
     HIPCHK(hipMemcpyAsync(c->maskv.p, mv.data(), sizeof(uint32_t) * c->N, hipMemcpyHostToDevice, c->stream));
     Tp P = make_tp(c, tp);
     k_ctl_reset<<<1, 1, 0, c->stream>>>(c->ctl.p);
@@ -1331,7 +1331,7 @@ int sm2d_k1k2_mask_compact(sm2d_ctx* c, const sm2d_transport* tp, const uint8_t*
     HIPCHK(hipMemcpyAsync(k.data(), c->crs.p, sizeof(uint32_t) * nc, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (mask_out)
-        for (int i = 0; i < c->N; ++i) mask_out[i] = (mv[i] == ((ep << 1) | 1u)) ? 1 : 0;
+        for (int i = 0; i < c->N; ++i) mask_out[i] = ((mv[i] >> 2) == ep && (mv[i] & 1u)) ? 1 : 0;
     for (uint32_t i = 0; i < na; ++i) active[i] = (int32_t)a[i];
     for (uint32_t i = 0; i < nc; ++i) coarse[i] = (int32_t)k[i];
     *n_active = (int)na;

@@ -271,6 +271,32 @@ def test_split_iteration_runs_match_one_run(hip, orc, split):
     assert np.array_equal(tg, to)
 
 
+@pytest.mark.parametrize("change", [dict(E_fine_on=30.0, E_fine_off=33.0), dict(weight_active_min=1e-5)],
+                         ids=["fine_on", "weight_min"])
+def test_split_runs_with_changed_thresholds(hip, orc, change):
+    """A call whose K1 thresholds differ from the previous call's decides its first iteration's
+    mask with its own thresholds (the previous call's closing audit decided that iteration ahead
+    with the old ones), from the same previous-iteration mask as the oracle."""
+    prob = problems.reference_grid_case(Nx=40, Nz=40)
+    g, o = _pair(hip, orc, prob)
+    rng = np.random.default_rng(17)
+    rows = _random_psi(rng, g.grid, g.N, g.LB, n_cells=150, max_slots=8)
+    tp1 = prob.make_transport(sigma_x_initial=2.0)
+    tp2 = prob.make_transport(sigma_x_initial=2.0, **change)
+    for e in (g, o):
+        e.psi_load(rows)
+    for tp in (tp1, tp2, tp1):
+        tg, _, ag = g.run_iterations(tp, 2)
+        to, _, ao = o.run_iterations(tp, 2)
+        assert ag.processed_cells == ao.processed_cells
+        assert ag.cell_bin_updates == ao.cell_bin_updates
+    bg, rg = g.psi_dump()
+    bo, ro = o.psi_dump()
+    assert np.array_equal(bg, bo)
+    assert _rows_key(rg) == _rows_key(ro)
+    assert np.array_equal(tg, to)
+
+
 def test_chunk_graphs_replay_bit_exact(hip, orc):
     """Chunks of iterations captured as graphs and replayed (with eager host work and other
     launches between the replays) give the oracle's dose on every run, as do eager launches."""
