@@ -1,17 +1,25 @@
 """Throughput of the K1-K6 transport step on one MI355X.
 
-A step is one complete pencil-beam spot of the benchmark workload (injection, every K1..K6
-iteration until the phase space is empty, EdepC read-back) through the HIP library.  The
-workload is the 150 MeV water spot on 200 x 400 cells of 0.5 mm with 256 log-spaced energy
-bins (0.1-250 MeV), 512 angular bins over [-pi/2, pi/2] and 128 local bins per block;
-inputs are synthetic (the seeded Gaussian source of the reference injector).
+Headline workload (value): 256 pencil-beam spots transported together (one batched launch
+sequence per step, chunks of iterations replayed as HIP graphs) with the learned fine-transport
+physics attached: energies U(70, 230) MeV, lateral offsets U(-40, 40) mm, sigma_x U(3, 6) mm,
+1 % energy spread, on water 200 x 600 cells of 0.5 mm, 256 log-spaced energy bins
+(0.1-250 MeV), 512 angular bins over [-pi/2, pi/2], 128 local bins per block.  Inputs are
+synthetic (the seeded Gaussian source of the reference injector) and resident on the device
+side before the timed region; a step ends with every spot's EdepC summed on the host.
 
-Reported: cell-bin updates/s (value), spots/s, ms per spot, the roofline of the dominant
-kernel (transport, HBM-bound: 8 B per cell-bin update = one FP32 read of psi_in + one FP32
-write of psi_out, divided by the kernel time from HIP events on the library's stream), the same
-for the K4 gather (8 B per non-zero bucket value moved = one FP32 read of the bucket + one FP32
-write of psi_out) and the CPU oracle timed on the same host on a bounded sample of the same
-workload.
+A step is one sm2d_run_batch call over all 256 spots.  Reported beside it:
+  - the plain path (one spot at a time) over the same 256 spots, and the plain batch;
+  - the summed dose of batched+learned against the plain path, and the fine-transport work
+    (cell-bin updates and learned substeps) that shows where the learned physics acts;
+  - the roofline of the dominant kernel (the coarse transport: 8 B of algorithmic HBM traffic
+    per cell-bin update, one FP32 read of psi_in and one FP32 write of psi_out) from HIP events
+    on the kernel's own dispatches, and its HBM traffic from rocprofv3 counters (profiles/);
+  - the same batch on the reference runner's piecewise 1029-bin energy grid, on water and on
+    the seeded slab phantom (400 x 1200 at 0.25 mm), where the fine transport and the learned
+    physics run, with the plain path's seed-to-seed dose difference as its noise floor;
+  - the 150 MeV single spot on 200 x 400 cells (plain path, the per-spot latency);
+  - the CPU oracle (-O3 port of the reference step) on the host cores, on a sample of the spots.
 """
 from __future__ import annotations
 
@@ -42,172 +50,288 @@ def _profiles_traffic(name: str):
     return None
 
 
-def cpu_baseline(prob, budget_s: float = 20.0) -> dict:
-    """The oracle (a serial C++ port of the reference step) on the same spot.  The spot is run
-    with a bounded iteration count chosen from a short probe so the sample takes ~budget_s."""
-    from oracle import pyoracle
-    from sm_2d_amd import problems
-
-    orc = pyoracle.load()
-    eng = problems.setup_engine(orc, prob, pyoracle.OracleEngine)
-    sp = prob.make_spot()
-    probe_iters = 20
-    t0 = time.perf_counter()
-    _, a = eng.run_spot(sp, prob.make_transport(max_iterations=probe_iters))
-    dt = time.perf_counter() - t0
-    full = prob.transport["max_iterations"]
-    iters = full if dt <= 0 else int(min(full, max(probe_iters, probe_iters * budget_s / dt)))
-    t0 = time.perf_counter()
-    _, a = eng.run_spot(sp, prob.make_transport(max_iterations=iters))
-    dt = time.perf_counter() - t0
-    eng.close()
-    return {"value": a.cell_bin_updates / dt, "unit": "cell-bin updates/s", "cores": 1,
-            "kind": "port",
-            "sample": f"first {a.iterations} iterations of the same spot ({a.cell_bin_updates} "
-                      f"cell-bin updates in {dt:.2f} s, serial C++ oracle, -O2)"}
+def _log(msg: str) -> None:
+    """Progress of the sections on stderr (the metric line alone goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def batched_section(hip, n_spots: int, plain_sample: int, mlp=None) -> dict:
-    """The 256-spot workload (pencil beams of 70-230 MeV over +-40 mm with sigma 3-6 mm on water
-    200 x 600 at 0.5 mm): one batched launch sequence for all spots against the plain path
-    (one spot at a time) timed on the first `plain_sample` spots of the same list.  With a
-    trained network `mlp`, the same batch again with the learned fine-transport physics
-    attached: spots/s against the plain path, and its summed dose against the plain batch's on
-    voxels above 10 % of the maximum."""
+def _sync():
     import torch
+    torch.cuda.synchronize()
+
+
+def dose_diff(test, ref) -> dict:
+    """Dose difference of `test` against `ref` on the voxels above 10 % of ref's maximum: local
+    (|d| / ref) and global (|d| / ref max) relative differences, and the total."""
+    import numpy as np
+    m = ref > 0.1 * ref.max()
+    d = np.abs(test - ref)[m]
+    loc = d / ref[m]
+    glo = d / ref.max()
+    return {"voxels": int(m.sum()), "total_rel": float(test.sum() / ref.sum() - 1.0),
+            "local_mean": float(loc.mean()), "local_p95": float(np.percentile(loc, 95)),
+            "local_max": float(loc.max()), "local_within_1pct": float((loc <= 0.01).mean()),
+            "global_mean": float(glo.mean()), "global_p95": float(np.percentile(glo, 95)),
+            "global_max": float(glo.max()), "global_within_1pct": float((glo <= 0.01).mean())}
+
+
+def _audit(a) -> dict:
+    return {"iterations": a.iterations, "terminated": a.terminated,
+            "cell_bin_updates": a.cell_bin_updates, "fine_cell_bin_updates": a.fine_cell_bin_updates,
+            "surrogate_active": a.surrogate_active, "surrogate_substeps": a.surrogate_substeps,
+            "surrogate_fallback": a.surrogate_fallback, "max_cell_W_closure": a.max_cell_W_closure,
+            "W_fail_iters": a.W_fail_iters, "max_W_error": a.max_W_error,
+            "W_leftover": a.W_leftover, "E_dep_MeV": a.E_dep}
+
+
+def plain_spots(hip, prob, spots, tp):
+    """One spot at a time through sm2d_run_spot (the reference's call pattern); returns the
+    summed EdepC in spot order, the seconds and the cell-bin updates."""
+    import numpy as np
     from sm_2d_amd import problems
     from sm_2d_amd.engine import Engine
-
-    prob = problems.config4(n_spots=n_spots)
-    spots = [prob.make_spot(i) for i in range(n_spots)]
-    tp = prob.make_transport()
     one = problems.setup_engine(hip, prob, Engine)
     one.run_spot(spots[0], tp)
-    torch.cuda.synchronize()
+    _sync()
+    esum = np.zeros((prob.grid["Nz"], prob.grid["Nx"]))
+    upd = 0
     t0 = time.perf_counter()
-    for sp in spots[:plain_sample]:
-        one.run_spot(sp, tp)
-    torch.cuda.synchronize()
-    dt_plain = time.perf_counter() - t0
-    one.close()
-    eng = problems.setup_engine(hip, prob, Engine, n_spots=n_spots)
-    t0 = time.perf_counter()
-    eng.run_batch(spots[:min(8, n_spots)], tp)
-    t_warm = time.perf_counter() - t0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    esum, _, a = eng.run_batch(spots, tp)
-    torch.cuda.synchronize()
+    for sp in spots:
+        e, a = one.run_spot(sp, tp)
+        esum += e
+        upd += a.cell_bin_updates
+    _sync()
     dt = time.perf_counter() - t0
-    eng.set_profiling(True)  # stage times from a second, eagerly launched run
+    one.close()
+    return esum, dt, upd
+
+
+def roofline_from_timing(tm, traffic):
+    """Roofline of the coarse transport from a profiled (eagerly launched, event-timed) run."""
+    achieved = 8.0 * tm.coarse_updates / (tm.coarse_transport_ms * 1e-3) / 1e9 if tm.coarse_transport_ms else 0.0
+    launches = max(1, tm.transport_launches // 2)
+    return {"bound": "hbm", "kernel": "k_transport<Layout<8,4,4,1>, coarse, tuned waves> (all coarse launches)",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic, "algorithmic_bytes_per_update": 8, "launches": launches,
+            "avg_launch_us": tm.coarse_transport_ms * 1e3 / launches,
+            "updates_per_launch": tm.coarse_updates / launches,
+            "algorithmic_bytes_per_launch": 8.0 * tm.coarse_updates / launches}
+
+
+def headline(hip, mlp, args) -> dict:
+    """The 256-spot batch with the learned physics attached (the metric's workload)."""
+    from sm_2d_amd import problems
+    from sm_2d_amd.engine import Engine
+    n = args.batch_spots
+    prob = problems.config4(n_spots=n)
+    spots = [prob.make_spot(i) for i in range(n)]
+    tp = prob.make_transport()
+    eng = problems.setup_engine(hip, prob, Engine, n_spots=n)
+    plain_batch, _, ap = eng.run_batch(spots, tp)  # plain batch (also captures the chunk graphs)
+    if mlp is not None:
+        eng.surrogate_attach(mlp, 64)
+        table = eng.surrogate_stats()
+    for _ in range(args.warmup):
+        eng.run_batch(spots, tp)
+    _sync()
+    t0 = time.perf_counter()
+    upd = 0
+    for _ in range(args.steps):
+        esum, _, a = eng.run_batch(spots, tp)
+        upd += a.cell_bin_updates
+    _sync()
+    dt = time.perf_counter() - t0
+    # second pass, same batch, eagerly launched with HIP events on the transport dispatches
+    eng.set_profiling(True)
     eng.run_batch(spots, tp)
     tm = eng.timing()
+    eng.set_profiling(False)
     tuner = eng.tuner_report()
-    plain_sps = plain_sample / dt_plain
-    batch_sps = n_spots / dt
-    learned = None
-    if mlp is not None:
-        import numpy as np
-        eng.set_profiling(False)
-        eng.surrogate_attach(mlp, 64)
-        eng.run_batch(spots[:min(8, n_spots)], tp)  # captures the chunk graphs again
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        es, _, al = eng.run_batch(spots, tp)
-        torch.cuda.synchronize()
-        dt_l = time.perf_counter() - t0
-        m10 = esum > 0.1 * esum.max()
-        rel = np.abs(es - esum)[m10] / esum[m10]
-        learned = {"batched_learned_spots_per_s": n_spots / dt_l, "batched_learned_s": dt_l,
-                   "speedup_vs_plain_spots_per_s": n_spots / dt_l / plain_sps,
-                   "dose_vs_plain_batch": {"total_rel": float(es.sum() / esum.sum() - 1.0),
-                                           "voxel_gt10pct_mean_rel": float(rel.mean()),
-                                           "voxel_gt10pct_p95_rel": float(np.percentile(rel, 95)),
-                                           "voxel_gt10pct_max_rel": float(rel.max()),
-                                           "voxels_within_1pct": float((rel <= 0.01).mean())},
-                   "audit": {"surrogate_active": al.surrogate_active, "fallback": al.surrogate_fallback,
-                             "max_cell_W_closure": al.max_cell_W_closure, "W_fail_iters": al.W_fail_iters}}
     eng.close()
-    return {"workload": f"{n_spots} pencil beams, E0 U(70,230) MeV, x0 U(-40,40) mm, sigma_x U(3,6) mm, "
-                        "water 200x600 @ 0.5 mm, 256 log E x 512 theta, 128 local bins",
-            "batched_spots_per_s": batch_sps, "batched_s": dt,
-            "batched_updates_per_s": a.cell_bin_updates / dt,
-            "batched_iterations": a.iterations, "batched_cell_bin_updates": a.cell_bin_updates,
-            "plain_spots_per_s": plain_sps, "plain_sample_spots": plain_sample,
-            "speedup_spots_per_s": batch_sps / plain_sps, "surrogate": False,
-            "batched_learned": learned,
-            "warm_batch_s": t_warm,
-            "stage_ms": {k: getattr(tm, k) for k in ("mask_ms", "transport_ms", "gather_ms", "audit_ms")},
-            "transport_tuner": tuner}
+    out = {"value": upd / dt, "spots_per_s": n * args.steps / dt, "ms_per_step": dt / args.steps * 1e3,
+           "ms_per_spot": dt / args.steps / n * 1e3, "audit": _audit(a),
+           "roofline": roofline_from_timing(tm, _profiles_traffic("transport_batch256.txt")),
+           "stage_ms_per_step": {k: getattr(tm, k) for k in ("mask_ms", "transport_ms", "gather_ms", "audit_ms")},
+           "transport_tuner": tuner,
+           "learned_physics_attached": mlp is not None,
+           "dose_vs_plain_batch": dose_diff(esum, plain_batch), "plain_batch_audit": _audit(ap)}
+    if mlp is not None:
+        out["learned_table"] = {"nodes_checked": table[0], "nodes_rejected": table[1]}
+    if not args.no_plain:
+        _log("plain path over all spots")
+        ep, dtp, updp = plain_spots(hip, prob, spots, tp)
+        out["plain"] = {"spots": n, "seconds": dtp, "spots_per_s": n / dtp, "updates_per_s": updp / dtp}
+        out["speedup_spots_per_s_vs_plain"] = out["spots_per_s"] / out["plain"]["spots_per_s"]
+        out["dose_vs_plain"] = dose_diff(esum, ep)
+        out["plain_equals_batch_bitwise"] = bool((ep == plain_batch).all())
+    return out
 
 
-def surrogate_section(hip, keep: list | None = None) -> dict:
-    """The learned fine-transport physics against the plain physics on a 60 MeV water spot whose
-    whole track runs in the fine transport (where the learned widths act): one-time training and
-    table build, ms per spot of both, and the dose comparison (total deposited energy, 4 x 4
-    block means over blocks above 10 % of the maximum)."""
-    import numpy as np
-    import torch
+def fine_grid_variant(hip, mlp, name: str, n: int, plain_sample: int, slabs: bool) -> dict:
+    """The batch on the reference runner's piecewise 1029-bin energy grid, where components slow
+    down into the fine transport: plain batch, the same with the learned physics, the plain
+    batch with other source seeds (the dose noise floor), and a sample of plain single spots."""
+    from sm_2d_amd import problems
+    from sm_2d_amd.engine import Engine
+    prob = problems.config5(n_spots=n) if slabs else problems.config4_fine_grid(n_spots=n)
+    spots = [prob.make_spot(i) for i in range(n)]
+    reseeded = [prob.make_spot(i) for i in range(n)]
+    for sp in reseeded:
+        sp.random_seed += 7919
+    tp = prob.make_transport()
+    eng = problems.setup_engine(hip, prob, Engine, n_spots=n)
+    eng.run_batch(spots, tp)
+    _sync()
+    t0 = time.perf_counter()
+    e_plain, _, a_plain = eng.run_batch(spots, tp)
+    _sync()
+    dt_plain = time.perf_counter() - t0
+    e_seed, _, _ = eng.run_batch(reseeded, tp)
+    out = {"workload": prob.note, "spots": n, "plain_batch_s": dt_plain,
+           "plain_batch_spots_per_s": n / dt_plain, "plain_batch_audit": _audit(a_plain),
+           "plain_reseeded_vs_plain": dose_diff(e_seed, e_plain)}
+    if mlp is not None:
+        eng.surrogate_attach(mlp, 64)
+        st = eng.surrogate_stats()
+        eng.run_batch(spots, tp)
+        _sync()
+        t0 = time.perf_counter()
+        e_l, _, a_l = eng.run_batch(spots, tp)
+        _sync()
+        dt_l = time.perf_counter() - t0
+        out.update({"learned_batch_s": dt_l, "learned_batch_spots_per_s": n / dt_l,
+                    "learned_audit": _audit(a_l), "learned_vs_plain": dose_diff(e_l, e_plain),
+                    "learned_table": {"nodes_checked": st[0], "nodes_rejected": st[1]},
+                    "learned_over_plain_batch_speed": dt_plain / dt_l})
+    eng.close()
+    if plain_sample:
+        _, dts, upds = plain_spots(hip, prob, spots[:plain_sample], tp)
+        out["plain_single"] = {"spots": plain_sample, "seconds": dts, "spots_per_s": plain_sample / dts}
+        out["plain_batch_over_single"] = out["plain_batch_spots_per_s"] / out["plain_single"]["spots_per_s"]
+        if mlp is not None:
+            out["learned_batch_over_plain_single"] = out["learned_batch_spots_per_s"] / out["plain_single"]["spots_per_s"]
+    return out
+
+
+def single_spot(hip, steps: int) -> dict:
+    """The 150 MeV spot on 200 x 400 cells through the plain path (per-spot latency)."""
+    from sm_2d_amd import problems
+    from sm_2d_amd.engine import Engine
+    prob = problems.config2()
+    eng = problems.setup_engine(hip, prob, Engine)
+    sp, tp = prob.make_spot(), prob.make_transport()
+    eng.run_spot(sp, tp)
+    _sync()
+    t0 = time.perf_counter()
+    upd = 0
+    for _ in range(steps):
+        _, a = eng.run_spot(sp, tp)
+        upd += a.cell_bin_updates
+    _sync()
+    dt = time.perf_counter() - t0
+    eng.set_profiling(True)
+    for _ in range(steps):
+        eng.run_spot(sp, tp)
+    tm = eng.timing()
+    eng.close()
+    g_ms, g_val = tm.coarse_gather_ms, tm.coarse_gather_values
+    g_ach = 8.0 * g_val / (g_ms * 1e-3) / 1e9 if g_ms > 0 else 0.0
+    launches = max(1, tm.iterations)
+    return {"workload": "plain K1-K6, one 150 MeV spot (sigma_x 3.8 mm, 1% sigma_E, 2 mrad), water "
+                        "200x400 @ 0.5 mm, 256 log E x 512 theta, 128 local bins",
+            "ms_per_spot": dt / steps * 1e3, "updates_per_s": upd / dt, "iterations": a.iterations,
+            "cell_bin_updates": a.cell_bin_updates,
+            "roofline": roofline_from_timing(tm, _profiles_traffic("transport_config2.txt")),
+            "roofline_k4": {"bound": "hbm", "kernel": "k_gather (coarse phase)", "achieved": g_ach,
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": g_ach / HBM_PEAK_GBS,
+                            "traffic": _profiles_traffic("gather_config2.txt"),
+                            "avg_launch_us": g_ms * 1e3 / launches, "values_per_launch": g_val / launches},
+            "stage_ms_per_spot": {k: getattr(tm, k) / steps for k in ("mask_ms", "transport_ms", "gather_ms", "audit_ms")}}
+
+
+def learned_physics(hip) -> tuple:
+    """Trains the network on the GPU (forward and backward kernels on the matrix cores) on
+    samples of the plain physics; returns it with the training figures."""
     from sm_2d_amd import problems
     from sm_2d_amd.engine import Engine, Mlp
     prob = problems.reference_grid_case(Nx=40, Nz=80, E0=60.0, sigma_x=2.0, n_samples=600,
                                         max_iterations=400)
-    prob.transport.update(E_fine_on=300.0, E_fine_off=300.0)
     eng = problems.setup_engine(hip, prob, Engine)
-    sp, tp = prob.make_spot(), prob.make_transport()
-    eng.run_spot(sp, tp)
-
-    def timed(n=3):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(n):
-            e, a = eng.run_spot(sp, tp)
-        torch.cuda.synchronize()
-        return e, a, (time.perf_counter() - t0) / n * 1e3
-
-    ep, ap, ms_plain = timed()
     m = Mlp(hip, seed=11)
+    cfg = dict(n_samples=1 << 22, batch=4096, steps=4000, lr=2e-3, w_mom=4.0, w_att=4.0)
+    _sync()
     t0 = time.perf_counter()
-    cfg = dict(n_samples=1 << 21, batch=4096, steps=4000, lr=2e-3, w_mom=4.0, w_att=4.0)
     hist = m.train(eng, n_hist=4, **cfg)
-    torch.cuda.synchronize()
+    _sync()
     t_train = time.perf_counter() - t0
     ev = m.evaluate(eng, 65536, seed=123)
-    t0 = time.perf_counter()
-    eng.surrogate_attach(m, 64)
-    t_attach = time.perf_counter() - t0
-    st = eng.surrogate_stats()
-    eng.run_spot(sp, tp)
-    es, as_, ms_surr = timed()
-
-    def blk(d, b=4):
-        nz, nx = d.shape
-        return d[: nz // b * b, : nx // b * b].reshape(nz // b, b, nx // b, b).sum((1, 3))
-
-    bp, bs = blk(ep), blk(es)
-    mk = bp > 0.1 * bp.max()
-    rel = np.abs(bs - bp)[mk] / bp[mk]
-    if keep is not None:
-        keep.append(m)  # the trained network, for the batched section
-    else:
-        m.close()
     eng.close()
-    flops_train = 3 * 75008 * cfg["batch"] * cfg["steps"]
-    return {"workload": "60 MeV water spot, 40 x 80 cells @ 1 mm (reference grid), all energies in "
-                        "the fine transport, 600 source samples",
-            "train": {"samples": cfg["n_samples"], "steps": cfg["steps"], "batch": cfg["batch"],
-                      "seconds": t_train, "loss_first_last": [float(hist[0]), float(hist[-1])],
-                      "mfma_tflops_achieved": flops_train / t_train / 1e12,
-                      "heldout_rms_ln_sigmaE_sigmaTheta_lnA": [float(v) for v in ev[:3]]},
-            "table": {"build_s": t_attach, "nodes_checked": st[0], "nodes_rejected": st[1],
-                      "max_dev": st[2], "rms_dev": st[3]},
-            "ms_per_spot_plain": ms_plain, "ms_per_spot_learned": ms_surr,
-            "speedup": ms_plain / ms_surr,
-            "dose": {"total_rel": float(es.sum() / ep.sum() - 1.0),
-                     "block4_mean_rel": float(rel.mean()),
-                     "block4_p95_rel": float(np.percentile(rel, 95))},
-            "audit": {"surrogate_active": as_.surrogate_active, "fallback": as_.surrogate_fallback,
-                      "max_cell_W_closure": as_.max_cell_W_closure, "W_fail_iters": as_.W_fail_iters}}
+    flops = 3 * 75008 * cfg["batch"] * cfg["steps"]
+    return m, {"samples": cfg["n_samples"], "steps": cfg["steps"], "batch": cfg["batch"],
+               "seconds": t_train, "loss_first_last": [float(hist[0]), float(hist[-1])],
+               "mfma_tflops_achieved": flops / t_train / 1e12,
+               "heldout_rms_ln_sigmaE_sigmaTheta_lnA": [float(v) for v in ev[:3]]}
+
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(budget_s: float, n_spots: int) -> dict:
+    """The oracle (a C++ port of the reference step, -O3 -march=x86-64-v3) on the host cores:
+    spots of the headline workload run to completion, one thread, then spots on every core the
+    process may use (at most 16, the host share of one GPU), one context per thread.  Context
+    set-up is outside the timed spans."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import pyoracle
+    from sm_2d_amd import problems
+
+    lib = pyoracle.load(fast=True)
+    prob = problems.config4(n_spots=n_spots)
+    tp = prob.make_transport()
+
+    def run(idx, barrier=None):
+        eng = problems.setup_engine(lib, prob, pyoracle.OracleEngine)
+        if barrier is not None:
+            barrier.wait()
+        t0 = time.perf_counter()
+        upd = 0
+        for i in idx:
+            _, a = eng.run_spot(prob.make_spot(i), tp)
+            upd += a.cell_bin_updates
+        t1 = time.perf_counter()
+        eng.close()
+        return upd, t0, t1
+
+    u, t0, t1 = run([0])
+    t_one = t1 - t0
+    k1 = max(1, min(n_spots, int(budget_s / 2 / max(t_one, 1e-3))))
+    u1, dt1 = u, t_one
+    if k1 > 1:
+        u, t0, t1 = run(list(range(1, k1)))
+        u1, dt1 = u1 + u, dt1 + (t1 - t0)
+    cores = min(16, len(os.sched_getaffinity(0)))
+    per = max(1, min(int(budget_s / 2 / max(t_one, 1e-3)), n_spots // cores))
+    groups = [[(t * per + j) % n_spots for j in range(per)] for t in range(cores)]
+    bar = threading.Barrier(cores)
+    with ThreadPoolExecutor(cores) as ex:
+        res = list(ex.map(lambda g: run(g, bar), groups))
+    un = sum(r[0] for r in res)
+    dtn = max(r[2] for r in res) - min(r[1] for r in res)
+    return {"value": un / dtn, "unit": "cell-bin updates/s", "cores": cores, "kind": "port",
+            "spots_per_s": cores * per / dtn,
+            "one_core": {"value": u1 / dt1, "spots_per_s": k1 / dt1, "spots": k1, "seconds": dt1},
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(),
+            "sample": f"{cores} threads x {per} spots of the 256-spot workload run to completion "
+                      f"({un} cell-bin updates in {dtn:.1f} s); one thread: spots 0..{k1 - 1} "
+                      f"({u1} updates in {dt1:.1f} s); oracle -O3 -march=x86-64-v3, one context per thread"}
 
 
 def main() -> int:
@@ -215,15 +339,17 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--batch-spots", type=int, default=256)
-    ap.add_argument("--plain-sample", type=int, default=8)
-    ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=24.0)
+    ap.add_argument("--no-surrogate", action="store_true", help="headline without the learned physics")
+    ap.add_argument("--no-plain", action="store_true", help="skip the one-spot-at-a-time pass")
+    ap.add_argument("--no-variants", action="store_true", help="skip the 1029-bin-grid batches")
+    ap.add_argument("--variant-spots", type=int, default=256)
+    ap.add_argument("--variant-plain-sample", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device to device three times (HBM counter calibration)")
-    ap.add_argument("--only", choices=("batch", "learned"),
+    ap.add_argument("--only", choices=("headline", "single", "fine_grid", "slabs"),
                     help="print only that section (for profiling one workload); not the metric line")
     args = ap.parse_args()
     if args.copy_calibration:
@@ -237,131 +363,77 @@ def main() -> int:
         print(f"copied {4 * n} bytes x 3", flush=True)
         return 0
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (spots shard "
-              f"across GPUs by a longest-first schedule, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (spots shard across "
+              f"GPUs by a longest-first schedule with one all-reduce of the summed dose)", file=sys.stderr)
         return 2
 
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.zeros(1, device="cuda:0")
-    from sm_2d_amd import problems, runtime
-    from sm_2d_amd.engine import Engine
+    from sm_2d_amd import runtime
+    hip = runtime.load_library()
 
     if args.only:
-        hip = runtime.load_library()
-        sec = (batched_section(hip, args.batch_spots, args.plain_sample) if args.only == "batch"
-               else surrogate_section(hip))
+        _log(f"section {args.only}")
+        mlp = None
+        if args.only != "single" and not args.no_surrogate:
+            mlp, _ = learned_physics(hip)
+        if args.only == "headline":
+            args.no_plain = True
+            sec = headline(hip, mlp, args)
+        elif args.only == "single":
+            sec = single_spot(hip, args.steps)
+        else:
+            sec = fine_grid_variant(hip, mlp, args.only, args.variant_spots, 0, args.only == "slabs")
         print(json.dumps({"only": args.only, "section": sec}), flush=True)
         return 0
 
-    prob = problems.config2()
-    hip = runtime.load_library()
-    eng = problems.setup_engine(hip, prob, Engine)
-    sp, tp = prob.make_spot(), prob.make_transport()
-    for _ in range(args.warmup):
-        eng.run_spot(sp, tp)
-    # Timed region: the production launch path (chunks of iterations replayed as HIP graphs).
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    updates = 0
-    audits = []
-    for _ in range(args.steps):
-        _, a = eng.run_spot(sp, tp)
-        updates += a.cell_bin_updates
-        audits.append(a)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    # Second timed region, same steps, for the per-kernel times: HIP events around every stage of
-    # every iteration on the library's stream (graph capture drops event records, so these chunks
-    # are launched eagerly; the kernels themselves are the same).
-    eng.set_profiling(True)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        eng.run_spot(sp, tp)
-    torch.cuda.synchronize()
-    dt_prof = time.perf_counter() - t1
-    tm = eng.timing()
-    tuner = eng.tuner_report()
-    a = audits[-1]
-    # dominant kernel: the coarse-phase transport launch (the fine phase of this workload is empty)
-    dom_ms, dom_upd = tm.coarse_transport_ms, tm.coarse_updates
-    if tm.fine_transport_ms > tm.coarse_transport_ms:
-        dom_ms, dom_upd = tm.fine_transport_ms, tm.fine_updates
-    achieved = 8.0 * dom_upd / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
-    traffic = _profiles_traffic("transport_config2.txt")
-    # K4: the coarse-phase gather (one launch per iteration while profiling)
-    g_ms, g_val = tm.coarse_gather_ms, tm.coarse_gather_values
-    g_achieved = 8.0 * g_val / (g_ms * 1e-3) / 1e9 if g_ms > 0 else 0.0
+    _log("training the learned physics")
+    mlp, train = (None, None) if args.no_surrogate else learned_physics(hip)
+    _log("headline batch")
+    h = headline(hip, mlp, args)
     out = {
         "metric": "cell-bin updates/s + spots/s (batched+surrogate vs plain) at <=1% dose diff",
-        "value": updates / dt,
+        "value": h["value"],
         "unit": "cell-bin updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt / args.steps * 1e3,
+        "ms_per_step": h["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded Gaussian source, water phantom)",
+        "data": "synthetic (seeded Gaussian sources, water phantom)",
         "config": {
-            "workload": "plain K1-K6, one 150 MeV spot (sigma_x 3.8 mm, 1% sigma_E, 2 mrad), "
-                        "water 200x400 cells @ 0.5 mm, 256 log E bins x 512 theta bins, "
-                        "128 local bins (8 theta x 4 E x 4 x_sub)",
-            "path": "plain",
-            "spots_per_step": 1,
-            "iterations_per_spot": a.iterations,
-            "cell_bin_updates_per_spot": a.cell_bin_updates,
+            "workload": f"{args.batch_spots} pencil beams batched, learned fine-transport physics "
+                        f"attached: E0 U(70,230) MeV, x0 U(-40,40) mm, sigma_x U(3,6) mm, 1% sigma_E, "
+                        f"water 200x600 cells @ 0.5 mm, 256 log E bins (0.1-250 MeV) x 512 theta bins "
+                        f"over [-pi/2, pi/2], 128 local bins (8 theta x 4 E x 4 x_sub)",
+            "spots_per_step": args.batch_spots,
+            "path": "batched+learned" if mlp is not None else "batched",
         },
-        "spots_per_s": args.steps / dt,
-        "ms_per_spot": dt / args.steps * 1e3,
-        "ms_per_spot_eager_with_events": dt_prof / args.steps * 1e3,
-        "stage_ms_per_spot": {k: getattr(tm, k) / args.steps for k in
-                              ("mask_ms", "transport_ms", "gather_ms", "audit_ms")},
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_transport (coarse phase)" if dom_ms == tm.coarse_transport_ms else "k_transport (fine phase)",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "algorithmic_bytes_per_update": 8,
-            "launches": tm.iterations,
-            "avg_launch_us": dom_ms * 1e3 / max(1, tm.iterations),
-            "updates_per_launch": dom_upd / max(1, tm.iterations),
-        },
-        "roofline_k4": {
-            "bound": "hbm",
-            "kernel": "k_gather (coarse phase)",
-            "achieved": g_achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": g_achieved / HBM_PEAK_GBS,
-            "traffic": _profiles_traffic("gather_config2.txt"),
-            "algorithmic_bytes_per_value": 8,
-            "launches": tm.iterations,
-            "avg_launch_us": g_ms * 1e3 / max(1, tm.iterations),
-            "values_per_launch": g_val / max(1, tm.iterations),
-        },
-        "transport_tuner": tuner,
-        "audit": {"max_cell_W_closure": a.max_cell_W_closure, "W_fail_iters": a.W_fail_iters,
-                  "E_fail_iters": a.E_fail_iters, "terminated": a.terminated,
-                  "E_boundary_MeV": a.E_boundary, "E_dep_MeV": a.E_dep},
+        "spots_per_s": h["spots_per_s"],
+        "ms_per_spot": h["ms_per_spot"],
+        "roofline": h["roofline"],
+        "headline": h,
     }
-    eng.close()
-    trained = []
-    if not args.no_surrogate:
-        out["learned_physics"] = surrogate_section(hip, keep=trained)
-    if not args.no_batch:
-        out["batched"] = batched_section(hip, args.batch_spots, args.plain_sample,
-                                         mlp=trained[0] if trained else None)
-    for m in trained:
-        m.close()
+    if train is not None:
+        out["learned_training"] = train
+    if not args.no_variants:
+        _log("fine-grid variants")
+        out["fine_grid_water"] = fine_grid_variant(hip, mlp, "fine_grid", args.variant_spots,
+                                                   args.variant_plain_sample, slabs=False)
+        _log("fine-grid slab variant")
+        out["fine_grid_slabs"] = fine_grid_variant(hip, mlp, "slabs", args.variant_spots,
+                                                   args.variant_plain_sample, slabs=True)
+    _log("single spot")
+    out["single_spot"] = single_spot(hip, args.steps)
+    if mlp is not None:
+        mlp.close()
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(prob, args.cpu_budget)
+        _log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_budget, args.batch_spots)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
     return 0

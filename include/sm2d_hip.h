@@ -99,6 +99,9 @@ typedef struct sm2d_audit {
     uint64_t unsupported_events;      /* lateral emissions into another listed cell's bucket */
     int surrogate_active;             /* 1 when the fine transport used the learned physics */
     int surrogate_fallback;           /* 1 when a K5 drift switched it back to the plain physics */
+    uint64_t fine_cell_bin_updates;   /* the part of cell_bin_updates done by the fine transport */
+    uint64_t surrogate_substeps;      /* fine-transport substeps whose straggling, scattering and
+                                         attenuation came from the learned-physics table */
 } sm2d_audit;
 
 /* Per-iteration K5 report: AuditReport of src/cuda/kernels/k5_audit.cuh:5-28, with the weight

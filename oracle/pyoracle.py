@@ -16,6 +16,8 @@ LIB = os.path.join(HERE, "_build", "liboracle_sm2d.so")
 LIB_OMP = os.path.join(HERE, "_build", "liboracle_sm2d_omp.so")
 # the same transport step evaluated in binary64 (the FP32 results' error is stated against it)
 LIB_FP64 = os.path.join(HERE, "_build", "liboracle_sm2d_fp64.so")
+# -O3 -march=x86-64-v3 build of the same source (the CPU baseline; results equal the -O2 build's)
+LIB_FAST = os.path.join(HERE, "_build", "liboracle_sm2d_fast.so")
 
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -31,8 +33,8 @@ def build(quiet: bool = True) -> None:
         raise RuntimeError("oracle build failed:\n" + (out.stdout or "") + (out.stderr or ""))
 
 
-def load(omp: bool = False, fp64: bool = False) -> Library:
-    path = LIB_FP64 if fp64 else LIB_OMP if omp else LIB
+def load(omp: bool = False, fp64: bool = False, fast: bool = False) -> Library:
+    path = LIB_FP64 if fp64 else LIB_OMP if omp else LIB_FAST if fast else LIB
     if not os.path.exists(path):
         build()
     return Library(path, "orc_", {

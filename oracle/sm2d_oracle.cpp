@@ -550,7 +550,7 @@ struct CellAcc {
     float w_cut = 0, w_nuc = 0, w_bnd = 0;
     double drop_w = 0, drop_E = 0, prune_w = 0, prune_E = 0;
     double cl_w_in = 0, cl_w_out = 0, cl_E_in = 0, cl_E_out = 0;
-    uint64_t comps = 0, updates = 0;
+    uint64_t comps = 0, updates = 0, surr = 0;
 };
 
 }  // namespace
@@ -559,6 +559,40 @@ struct CellAcc {
 // One transport phase (K2 coarse for `fine` = false, K3 fine for `fine` = true)
 // ---------------------------------------------------------------------------------------
 namespace {
+
+// Angular split of an emitted component (device_emit_component_to_bucket_4d,
+// device_bucket.cuh:452-476): the nearest theta-bin centre keeps 1 - f^2 of the weight and the
+// neighbouring bin on the side of theta takes f^2, f = |theta - centre| / dtheta (none at the
+// grid's edges).  The split conserves weight; its mean angle is centre + sign * f^2 * dtheta.
+void theta_split(const orc_ctx* c, float theta, int tb[2], float tw[2]) {
+    const float* te = c->th_edges.data();
+    int NT = c->g.N_theta;
+    float tmin = te[0], tmax = te[NT];
+    float dth = (tmax - tmin) / NT;
+    float tc = fmaxf(tmin, fminf(theta, tmax));
+    float c0 = tmin + 0.5f * dth;
+    float cont = (tc - c0) / dth;
+    int bc = (int)floorf(cont + 0.5f);
+    bc = std::max(0, std::min(bc, NT - 1));
+    float tcen = c0 + (float)bc * dth;
+    float delta = tc - tcen;
+    tb[0] = tb[1] = bc;
+    tw[0] = 1.0f;
+    tw[1] = 0.0f;
+    if (delta > 0.0f && bc < NT - 1) tb[1] = bc + 1;
+    else if (delta < 0.0f && bc > 0) tb[1] = bc - 1;
+    if (tb[1] != tb[0]) {
+        float fr = fminf(1.0f, fabsf(delta) / dth);
+        tw[1] = fr * fr;
+        tw[0] = 1.0f - tw[1];
+    }
+}
+
+// Seed of a component's angular draw (k3_finetransport.cu:355-366, k2_coarsetransport.cu:339-350).
+unsigned theta_seed(int cell, int slot, int lidx, int ebin, unsigned path_seed) {
+    return ((unsigned)(cell + 1) * 2654435761u) ^ ((unsigned)(slot + 1) * 2246822519u) ^
+           ((unsigned)(lidx + 1) * 3266489917u) ^ (unsigned)(ebin + 17) ^ (path_seed * 747796405u);
+}
 
 struct Phase {
     orc_ctx* c;
@@ -580,26 +614,10 @@ struct Phase {
     // device_emit_component_to_bucket_4d (device_bucket.cuh:431-533); returns dropped weight.
     float emit4d(Bucket* bk, float theta, float E, float weight, int xs, int zs, CellAcc& a) {
         if (weight <= 0.0f) return 0.0f;
-        const float* te = c->th_edges.data();
-        int NT = c->g.N_theta, NE = c->g.N_E;
-        float tmin = te[0], tmax = te[NT];
-        float dth = (tmax - tmin) / NT;
-        float tc = fmaxf(tmin, fminf(theta, tmax));
-        float c0 = tmin + 0.5f * dth;
-        float cont = (tc - c0) / dth;
-        int bc = (int)floorf(cont + 0.5f);
-        bc = std::max(0, std::min(bc, NT - 1));
-        float tcen = c0 + (float)bc * dth;
-        float delta = tc - tcen;
-        int tb[2] = {bc, bc};
-        float tw[2] = {1.0f, 0.0f};
-        if (delta > 0.0f && bc < NT - 1) tb[1] = bc + 1;
-        else if (delta < 0.0f && bc > 0) tb[1] = bc - 1;
-        if (tb[1] != tb[0]) {
-            float fr = fminf(1.0f, fabsf(delta) / dth);
-            tw[1] = fr * fr;
-            tw[0] = 1.0f - tw[1];
-        }
+        int NE = c->g.N_E;
+        int tb[2];
+        float tw[2];
+        theta_split(c, theta, tb, tw);
         const float* ee = c->E_edges.data();
         float Ec = fmaxf(ee[0], fminf(E, ee[NE]));
         int eb = c->find_bin(Ec);
@@ -755,6 +773,7 @@ struct Phase {
                     float s_sE = 0.0f, s_sth = 0.0f, s_a = 0.0f;
                     const bool use_s = c->surr_active && ds > 0.0f && ds / X0 >= 1e-6f &&
                                        surrogate_lookup(c, ebin, iz, rh * ds, s_sE, s_sth, s_a);
+                    if (use_s) a.surr++;
                     if (tp->enable_straggling) {
                         unsigned seed = (unsigned)((cell * 7 + slot * 13 + lidx * 17) ^ 0x5DEECE66DL);
                         float sE = use_s ? s_sE : straggling_sigma(E, ds, rh);
@@ -837,10 +856,7 @@ struct Phase {
                     ds = cstep;
                 }
                 // Angular diffusion (k3_finetransport.cu:355-366, k2_coarsetransport.cu:339-350)
-                unsigned tseed = ((unsigned)(cell + 1) * 2654435761u) ^
-                                 ((unsigned)(slot + 1) * 2246822519u) ^
-                                 ((unsigned)(lidx + 1) * 3266489917u) ^ (unsigned)(ebin + 17) ^
-                                 (path_seed * 747796405u);
+                unsigned tseed = theta_seed(cell, slot, lidx, ebin, path_seed);
                 float tsc = (sth <= 0.0f) ? 0.0f : gaussian(tseed) * sth;
                 float th_new = theta + tsc;
                 th_new = fmaxf(te[0], fminf(th_new, te[NT]));
@@ -1278,6 +1294,8 @@ void run_phase(orc_ctx* c, const sm2d_transport* tp, float sx0, const std::vecto
         if (fine && a.cl_E_in > 0) cl_E = std::max(cl_E, std::fabs(a.cl_E_in - a.cl_E_out) / a.cl_E_in);
         if (au) {
             au->cell_bin_updates += a.updates;
+            if (fine) au->fine_cell_bin_updates += a.updates;
+            au->surrogate_substeps += a.surr;
             au->processed_cells += 1;
             au->processed_components += a.comps;
         }
@@ -1471,6 +1489,18 @@ int orc_eval(orc_ctx* c, int which, const float* x, const float* y, float* out, 
             case 7: out[i] = straggling_sigma(a, b, 1.0f); break;
             case 8: out[i] = nuclear_sigma(a); break;
             case 9: out[i] = (float)c->find_bin(a); break;
+            case 10: case 11: {  // theta split of an emission at angle a: neighbour weight, side
+                int tb[2];
+                real_t tw[2];
+                theta_split(c, a, tb, tw);
+                out[i] = which == 10 ? (float)tw[1] : (float)(tb[1] - tb[0]);
+                break;
+            }
+            case 12: {  // the angular draw of component (cell a, slot 0, local bin b, E bin 0, path 0)
+                unsigned sd = theta_seed((int)a, 0, (int)b, 0, 0u);
+                out[i] = gaussian(sd);
+                break;
+            }
             default: return fail("bad eval id");
         }
     }

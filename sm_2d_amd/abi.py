@@ -103,6 +103,7 @@ class Audit(C.Structure):
         ("processed_cells", C.c_uint64), ("processed_components", C.c_uint64),
         ("unsupported_events", C.c_uint64),
         ("surrogate_active", C.c_int), ("surrogate_fallback", C.c_int),
+        ("fine_cell_bin_updates", C.c_uint64), ("surrogate_substeps", C.c_uint64),
     ]
 
     def __init__(self):

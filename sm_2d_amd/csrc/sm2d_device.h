@@ -105,7 +105,7 @@ struct Tally {  // cumulative per-cell tallies (src/cuda/k1k6_pipeline.cuh:174-1
 // Per listed cell and iteration (FP64), reduced by K5 in list order.
 enum Acc {
     A_EDEP, A_WCUT, A_ECUT, A_WNUC, A_ENUC, A_WBND, A_EBND, A_DROPW, A_DROPE,
-    A_CLWIN, A_CLWOUT, A_CLEIN, A_CLEOUT, A_COMPS, A_UPD, NACC
+    A_CLWIN, A_CLWOUT, A_CLEIN, A_CLEOUT, A_COMPS, A_UPD, A_SURR, NACC
 };
 
 // Device error flags.

@@ -822,6 +822,7 @@ __device__ inline Bins emission_bins(const Tabs& T, float theta, float E, int eg
 // Wave-uniform broadcast of one component's results (lane j -> all lanes).
 struct Comp {
     int kind;     // 0 prune, 1 cutoff before the step, 2 transported
+    int surr;     // 1: the learned-physics table gave the substep's widths
     float w, E;
     float edep, w_rem, E_rem, w_new, E_new;
     int face;     // -1 stays in the cell
@@ -1401,6 +1402,7 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
     int eg = -1;  // energy bin of the component (a starting guess for the bin searches)
     r.kind = 2;
     r.face = -1;
+    r.surr = 0;
     r.lidx = lidx_in;
     r.w = live ? w_in : 0.0f;
     r.edep = r.w_rem = r.E_rem = r.w_new = r.E_new = 0.0f;
@@ -1444,6 +1446,7 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
                 float s_sE = 0.0f, s_sth = 0.0f, s_a = 0.0f;
                 const bool use_s = P.surr && ds > 0.0f && ds / X0 >= 1e-6f &&
                                    surrogate_lookup(T, ebin, iz, rh * ds, s_sE, s_sth, s_a);
+                r.surr = use_s ? 1 : 0;
                 if (P.strag) {
                     unsigned sd = (unsigned)((lc * 7 + (int)slot * 13 + lidx * 17) ^ 0x5DEECE66DL);
                     float sE = use_s ? s_sE : straggling(E, ds, rh);
@@ -1956,6 +1959,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
     __shared__ uint32_t s_bid[KB], s_ref[KB], s_cnt[KB], s_off[KB];
     __shared__ uint64_t s_mask[TW];
     __shared__ uint32_t s_round[2];
+    __shared__ uint32_t s_surr;  // learned-physics substeps of the cell
     __shared__ uint32_t s_own[NOWN];
     extern __shared__ float s_tab[];
     for (int i = threadIdx.x; i < NOWN; i += blockDim.x) s_own[i] = ~0u;
@@ -2017,6 +2021,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
             if (d >= 0) dest_mark(D, (uint32_t)d, tag, flags);
         }
         const uint32_t ns = in.ns[cell];
+        if (threadIdx.x == 0) s_surr = 0;
         if (threadIdx.x < ns) {
             s_bid[threadIdx.x] = in.bid[(size_t)cell * KB + threadIdx.x];
             s_ref[threadIdx.x] = in.sref[(size_t)cell * KB + threadIdx.x];
@@ -2127,7 +2132,9 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                     s_fast[wv].nkey[l] = o.nkey;
                     s_fast[wv].nrec[l] = o.nrec;
                     const uint64_t m = __ballot(live);
+                    const uint32_t nsur = (uint32_t)__popcll(__ballot(live && r.kind == 2 && r.surr));
                     if (l == 0) s_mask[wv] = m;
+                    if (FINE && l == 0 && nsur) atomicAdd(&s_surr, nsur);
                 }
                 __syncthreads();
                 if (wv == 0) {  // ordered apply of the batches, in order
@@ -2189,6 +2196,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                 o[A_CLEOUT] = a.cl_Eout;
                 o[A_COMPS] = (double)a.comps;
                 o[A_UPD] = (double)((size_t)ns * LY::LB);
+                o[A_SURR] = (double)s_surr;
             }
         }
         __syncthreads();

@@ -700,6 +700,8 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
                 au->max_cell_W_closure = std::max(au->max_cell_W_closure, v[NR1 + NACC]);
                 au->max_cell_E_closure = std::max(au->max_cell_E_closure, v[NR1 + NACC + 1]);
                 au->cell_bin_updates += (uint64_t)a[A_UPD];
+                au->fine_cell_bin_updates += (uint64_t)v[IDX_FUPD];
+                au->surrogate_substeps += (uint64_t)a[A_SURR];
                 au->processed_cells += (uint64_t)v[IDX_LIST];
                 au->processed_components += (uint64_t)a[A_COMPS];
             }
@@ -1417,6 +1419,8 @@ static int finish_audit(sm2d_ctx* c, sm2d_audit* au, const sm2d_audit& it, int n
     au->processed_cells = it.processed_cells;
     au->surrogate_active = it.surrogate_active;
     au->surrogate_fallback = it.surrogate_fallback;
+    au->fine_cell_bin_updates = it.fine_cell_bin_updates;
+    au->surrogate_substeps = it.surrogate_substeps;
     for (int s = 0; s < n_spots; ++s) {
         au->W_source_injected += c->src[s * 7 + 0];
         au->W_source_out_of_grid += c->src[s * 7 + 1];

@@ -138,6 +138,49 @@ def config4(n_spots: int = 256, layout=LAYOUT_CFG, n_samples=1000, seed: int = 2
         dict(E_fine_on=10.0, E_fine_off=11.0, max_iterations=2000), spots)
 
 
+def config4_fine_grid(n_spots: int = 256, n_samples=1000, seed: int = 2024) -> Problem:
+    """The batch of config4 on the reference runner's default piecewise 1029-bin energy grid
+    (src/include/core/incident_particle_config.hpp:164-171).  Coarse steps lose more than half
+    a bin there, so components slow down into the fine transport (where the learned physics
+    acts).  A few fine components reach a fixed point of the reference algorithm (a hash-seeded
+    draw with dE = 0 that keeps the component in its sub-bin, repeated with the same seed every
+    iteration), so the run is capped at Nz + 50 iterations and reports the leftover weight."""
+    p = config4(n_spots=n_spots, n_samples=n_samples, seed=seed)
+    p.name = f"config4_fine_grid_{n_spots}spots_200x600"
+    p.energy = "piecewise"
+    p.transport = dict(p.transport, max_iterations=600 + 50)
+    p.note = (f"{n_spots} pencil beams, E0 U(70,230) MeV, x0 U(-40,40) mm, sigma_x U(3,6) mm, water "
+              "200x600 @ 0.5 mm, piecewise 1029-bin E grid (reference runner default) x 512 theta, "
+              "128 local bins, 650 iterations")
+    return p
+
+
+def config5(n_spots: int = 256, energy="piecewise", n_samples=1000, seed: int = 2024,
+            slab_seed: int = 4) -> Problem:
+    """The config4 spot distribution on the config3 slab phantom (water / bone / lung slabs,
+    100 x 300 mm at 0.25 mm = 400 x 1200 cells): the learned-physics workload.  With the
+    256-bin log grid of the benchmark configurations no component reaches the fine transport
+    (coarse steps re-centre every component in its energy bin); the piecewise 1029-bin grid
+    (default) is where the fine transport and the learned physics act (capped at Nz + 50
+    iterations, see config4_fine_grid)."""
+    p = config4(n_spots=n_spots, n_samples=n_samples, seed=seed)
+    p3 = config3(slab_seed=slab_seed)
+    p.grid = p3.grid
+    p.materials = p3.materials
+    if energy == "piecewise":
+        p.energy = "piecewise"
+        p.transport = dict(p.transport, max_iterations=1200 + 50)
+        tag, grid_txt = "fine_grid", "piecewise 1029-bin E grid (reference runner default), 1250 iterations"
+    else:
+        p.energy = ("log", 0.1, 250.0, 256)
+        p.transport = dict(p.transport, max_iterations=3000)
+        tag, grid_txt = "log256", "256 log E bins 0.1-250 MeV"
+    p.name = f"config5_{tag}_{n_spots}spots_400x1200"
+    p.note = (f"{n_spots} pencil beams, E0 U(70,230) MeV, x0 U(-40,40) mm, sigma_x U(3,6) mm, seeded "
+              f"water/bone/lung slabs 400x1200 @ 0.25 mm, {grid_txt} x 512 theta, 128 local bins")
+    return p
+
+
 def setup_engine(lib: Library, prob: Problem, engine_cls=None, spot_index: int = 0,
                  n_spots: int = 1):
     """Builds tables, creates a context (of `n_spots` spot slots) and uploads FE moments for
