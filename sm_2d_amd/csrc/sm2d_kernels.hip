@@ -969,17 +969,21 @@ enum {
     K_EDEP, K_ECUT, K_ENUC, K_EBND, K_WCUT, K_WNUC, K_WBND,
     K_DROPW, K_DROPE, K_CLWIN, K_CLWOUT, K_CLEIN, K_CLEOUT, NKACC
 };
-constexpr int MAXREC = 12;  // x_sub spread (<= 8) + two two-bin side emissions
 constexpr int MAXKEY = 5;   // own block + two two-bin side emissions
 constexpr int NCON = 2 * NKACC;
 
 __device__ inline bool kacc_is_float(int a) { return a == K_WCUT || a == K_WNUC || a == K_WBND; }
 
 // LDS scratch of one transport wave for one chunk of 64 components.
+template <class LY>
 struct FastLds {
-    double con[64 * NCON];
-    uint32_t rec[64 * MAXREC];   // (key index << 16 | lidx), then the LDS payload address
-    float recw[64 * MAXREC];
+    static constexpr int MR = LY::XS + 4;  // records per lane: x_sub spread + two two-bin side emissions
+    // per-accumulator contributions in the serial order, two per accumulator: binary32 values
+    // (the FP64 accumulators add them widened); K_CLEIN holds (w, E), whose FP64 product is added
+    float con[64 * NCON];
+    double cl[64 * 2];           // per-component FP64 sums of K_CLWOUT and K_CLEOUT
+    uint32_t rec[64 * MR];   // (key index << 16 | lidx), then the LDS payload address
+    float recw[64 * MR];
     uint32_t key[64 * MAXKEY];
     uint32_t krow[64 * MAXKEY];
     int nkey[64], nrec[64];
@@ -992,7 +996,7 @@ struct LaneOut {
 };
 
 template <class LY>
-__device__ inline void rec_emit(FastLds& F, LaneOut& o, int t, const Bins& b, float weight, int xs, int zs) {
+__device__ inline void rec_emit(FastLds<LY>& F, LaneOut& o, int t, const Bins& b, float weight, int xs, int zs) {
     if (weight <= 0.0f) return;
     const int l = lane_id();
     const int el = b.eb % LY::EL;
@@ -1008,56 +1012,59 @@ __device__ inline void rec_emit(FastLds& F, LaneOut& o, int t, const Bins& b, fl
         const int k = o.nkey++;
         F.key[l * MAXKEY + k] = ((uint32_t)t << 24) | bid;
         const int r = o.nrec++;
-        F.rec[l * MAXREC + r] = ((uint32_t)k << 16) | (uint32_t)lidx;
-        F.recw[l * MAXREC + r] = we;
+        F.rec[l * FastLds<LY>::MR + r] = ((uint32_t)k << 16) | (uint32_t)lidx;
+        F.recw[l * FastLds<LY>::MR + r] = we;
         o.clwout += we;
         o.cleout += (double)we * (double)b.Ecen;
     }
 }
 
-__device__ inline void con_set(FastLds& F, int a, int second, double v) {
+template <class LY>
+__device__ inline void con_set(FastLds<LY>& F, int a, int second, float v) {
     F.con[lane_id() * NCON + 2 * a + second] = v;
 }
 
 // The component of this lane (see the serial apply for the meaning of each quantity).
 template <class LY, bool FINE>
-__device__ inline LaneOut fast_prepare(FastLds& F, const Comp& r, bool live, int ix, int iz, int Nx, int Nz) {
+__device__ inline LaneOut fast_prepare(FastLds<LY>& F, const Comp& r, bool live, int ix, int iz, int Nx, int Nz) {
     const int l = lane_id();
     LaneOut o;
     o.nkey = o.nrec = o.nbnd = 0;
     o.clwout = o.cleout = 0.0;
-    for (int q = 0; q < NCON; ++q) F.con[l * NCON + q] = 0.0;
+    for (int q = 0; q < NCON; ++q) F.con[l * NCON + q] = 0.0f;
+    F.cl[l * 2] = F.cl[l * 2 + 1] = 0.0;
     if (!live) return o;
     const float w = r.w, E = r.E;
-    con_set(F, K_CLWIN, 0, (double)w);
-    con_set(F, K_CLEIN, 0, (double)w * (double)E);
+    con_set(F, K_CLWIN, 0, w);
+    con_set(F, K_CLEIN, 0, w);
+    con_set(F, K_CLEIN, 1, E);
     if (r.kind == 0) {
-        con_set(F, K_DROPW, 0, (double)w);
-        con_set(F, K_DROPE, 0, (double)(E * w));
+        con_set(F, K_DROPW, 0, w);
+        con_set(F, K_DROPE, 0, E * w);
         o.clwout = (double)w;
         o.cleout = (double)(E * w);
     } else if (r.kind == 1) {
-        con_set(F, K_WCUT, 0, (double)w);
-        con_set(F, K_ECUT, 0, (double)(E * w));
+        con_set(F, K_WCUT, 0, w);
+        con_set(F, K_ECUT, 0, E * w);
         o.clwout = (double)w;
         o.cleout = (double)(E * w);
     } else {
         const float w_new = r.w_new, E_new = r.E_new;
-        con_set(F, K_EDEP, 0, (double)r.edep);
-        con_set(F, K_WNUC, 0, (double)r.w_rem);
-        con_set(F, K_ENUC, 0, (double)r.E_rem);
+        con_set(F, K_EDEP, 0, r.edep);
+        con_set(F, K_WNUC, 0, r.w_rem);
+        con_set(F, K_ENUC, 0, r.E_rem);
         o.clwout += (double)r.w_rem;
         o.cleout += (double)r.edep + (double)r.E_rem;
         auto bnd = [&](float wb) {
-            con_set(F, K_WBND, o.nbnd, (double)wb);
-            con_set(F, K_EBND, o.nbnd, (double)(E_new * wb));
+            con_set(F, K_WBND, o.nbnd, wb);
+            con_set(F, K_EBND, o.nbnd, E_new * wb);
             o.nbnd++;
             o.clwout += (double)wb;
             o.cleout += (double)(E_new * wb);
         };
         if (E_new <= 0.1f) {
-            con_set(F, K_WCUT, 0, (double)w_new);
-            con_set(F, K_ECUT, 0, (double)(E_new * w_new));
+            con_set(F, K_WCUT, 0, w_new);
+            con_set(F, K_ECUT, 0, E_new * w_new);
             o.clwout += (double)w_new;
             o.cleout += (double)(E_new * w_new);
         } else if (r.face >= 0) {
@@ -1071,15 +1078,15 @@ __device__ inline LaneOut fast_prepare(FastLds& F, const Comp& r, bool live, int
                 if (wf5 > 0.0f) rec_emit<LY>(F, o, 1, r.bn, w_new * wf5, LY::XS / 2, 0);
                 if (lat_bnd > 0.0f) bnd(lat_bnd);
                 if (drop > 0.0f) {
-                    con_set(F, K_DROPW, 0, (double)drop);
-                    con_set(F, K_DROPE, 0, (double)(E_new * drop));
+                    con_set(F, K_DROPW, 0, drop);
+                    con_set(F, K_DROPE, 0, E_new * drop);
                     o.clwout += (double)drop;
                     o.cleout += (double)(E_new * drop);
                 }
                 if (FINE) {
                     const float tw = fmaxf(0.0f, w_new - lat_bnd - drop);
                     if (tw > 0.0f) {
-                        con_set(F, K_EDEP, 1, (double)((E_new - E_emit) * tw));
+                        con_set(F, K_EDEP, 1, (E_new - E_emit) * tw);
                         o.cleout += (double)((E_new - E_emit) * tw);
                     }
                 }
@@ -1089,7 +1096,7 @@ __device__ inline LaneOut fast_prepare(FastLds& F, const Comp& r, bool live, int
                 if (FINE) {
                     const float tw = fmaxf(0.0f, w_new - 0.0f);
                     if (tw > 0.0f) {
-                        con_set(F, K_EDEP, 1, (double)((E_new - E_emit) * tw));
+                        con_set(F, K_EDEP, 1, (E_new - E_emit) * tw);
                         o.cleout += (double)((E_new - E_emit) * tw);
                     }
                 }
@@ -1108,8 +1115,8 @@ __device__ inline LaneOut fast_prepare(FastLds& F, const Comp& r, bool live, int
                     o.clwout += (double)wsp;
                     o.cleout += (double)wsp * (double)E_emit;
                     const int rr = o.nrec++;
-                    F.rec[l * MAXREC + rr] = (uint32_t)LY::enc(tbn % LY::TL, ebn % LY::EL, q, zst);
-                    F.recw[l * MAXREC + rr] = wsp;
+                    F.rec[l * FastLds<LY>::MR + rr] = (uint32_t)LY::enc(tbn % LY::TL, ebn % LY::EL, q, zst);
+                    F.recw[l * FastLds<LY>::MR + rr] = wsp;
                 }
             }
             const float wcf = r.wcf, wl = r.wl, wr = r.wr;
@@ -1132,14 +1139,14 @@ __device__ inline LaneOut fast_prepare(FastLds& F, const Comp& r, bool live, int
                 rf = fminf(1.0f, fmaxf(0.0f, rf));
                 const float rw = w_new * rf;
                 if (rw > 0.0f) {
-                    con_set(F, K_EDEP, 1, (double)((E_new - E_emit) * rw));
+                    con_set(F, K_EDEP, 1, (E_new - E_emit) * rw);
                     o.cleout += (double)((E_new - E_emit) * rw);
                 }
             }
         }
     }
-    con_set(F, K_CLWOUT, 0, o.clwout);
-    con_set(F, K_CLEOUT, 0, o.cleout);
+    F.cl[l * 2] = o.clwout;
+    F.cl[l * 2 + 1] = o.cleout;
     return o;
 }
 
@@ -1150,7 +1157,7 @@ __device__ inline LaneOut fast_prepare(FastLds& F, const Comp& r, bool live, int
 // would overflow into the closest-block fallback, a slot's payload is not in LDS, or the LDS
 // rows run out) leaves them untouched for the serial path.  Resolves record addresses.
 template <class LY>
-__device__ inline bool fast_claims(FastLds& F, WTabs<LY>& W) {
+__device__ inline bool fast_claims(FastLds<LY>& F, WTabs<LY>& W) {
     const int l = lane_id();
     const int nkey = F.nkey[l], nrec = F.nrec[l];
     uint32_t k0 = EMPTY, k1 = EMPTY, k2 = EMPTY, k3 = EMPTY, k4 = EMPTY;
@@ -1222,58 +1229,12 @@ __device__ inline bool fast_claims(FastLds& F, WTabs<LY>& W) {
         W.used = used0 + nnew;
     }
     for (int rr = 0; rr < nrec; ++rr) {
-        const uint32_t p = F.rec[l * MAXREC + rr];
+        const uint32_t p = F.rec[l * FastLds<LY>::MR + rr];
         const uint32_t kidx = p >> 16;
         const uint32_t row = kidx == 0 ? r0 : kidx == 1 ? r1 : kidx == 2 ? r2 : kidx == 3 ? r3 : r4;
-        F.rec[l * MAXREC + rr] = row * (uint32_t)LY::LB + (p & 0xFFFFu);
+        F.rec[l * FastLds<LY>::MR + rr] = row * (uint32_t)LY::LB + (p & 0xFFFFu);
     }
     return true;
-}
-
-// Components in bin order (the live lanes of a batch are 0..cnt-1).  Every record of a component
-// is added with one LDS atomic-add instruction (its destinations are distinct); LDS executes a
-// wave's instructions in issue order, so the adds to any one payload bin happen in component
-// order, as in the serial form.  Lanes 0..NKACC-1 add the component's contributions to the
-// accumulator they hold.  Components are fetched in groups of 8 so that one wait covers a group.
-template <class LY>
-__device__ inline void fast_steps(FastLds& F, const WTabs<LY>& W, uint64_t m, double& accd, float& accf) {
-    constexpr int G8 = 4;
-    const int cnt = __popcll(m);
-    const int l = lane_id();
-    const bool fl = kacc_is_float(l);
-    const bool accl = l < NKACC;
-    const int rr = l - 16;
-    const bool recl = rr >= 0 && rr < MAXREC;
-    for (int j0 = 0; j0 < cnt; j0 += G8) {
-        double c0[G8], c1[G8];
-        float w[G8];
-        uint32_t ad[G8];
-        int nr[G8];
-#pragma unroll
-        for (int u = 0; u < G8; ++u) {
-            const int j = min(j0 + u, cnt - 1);
-            c0[u] = accl ? F.con[j * NCON + 2 * l] : 0.0;
-            c1[u] = accl ? F.con[j * NCON + 2 * l + 1] : 0.0;
-            nr[u] = F.nrec[j];
-            ad[u] = recl ? F.rec[j * MAXREC + rr] : 0u;
-            w[u] = recl ? F.recw[j * MAXREC + rr] : 0.0f;
-        }
-#pragma unroll
-        for (int u = 0; u < G8; ++u) {
-            if (j0 + u >= cnt) break;
-            if (accl) {
-                if (fl) {
-                    accf += (float)c0[u];
-                    accf += (float)c1[u];
-                } else {
-                    accd += c0[u];
-                    accd += c1[u];
-                }
-            }
-            if (recl && rr < nr[u] && w[u] > 0.0f)
-                __hip_atomic_fetch_add(W.rows + ad[u], w[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-        }
-    }
 }
 
 // Size of the per-workgroup owner table of the record rounds (hashed payload addresses).  Kept
@@ -1282,64 +1243,82 @@ constexpr int NOWN = 512;
 
 __device__ inline uint32_t own_slot(uint32_t addr) { return (addr * 2654435761u) >> 23; }
 
-// The same adds as fast_steps, record-parallel: lane l holds component l's records (distinct
+// The ordered adds, record-parallel: lane l holds component l's records (distinct
 // addresses within a component).  Each round every pending record bids its index (component-major)
 // for its address's owner slot; the lowest bid per slot adds its weight and leaves.  Records of
 // one address therefore add in component order, one per round, as in the serial form; records
 // of addresses that share a slot only wait.  own[] holds ~0u between rounds.  The accumulator
 // lanes then take the components in order.
 template <class LY>
-__device__ inline void fast_rounds(FastLds& F, const WTabs<LY>& W, uint64_t m, double& accd, float& accf,
+__device__ inline void fast_rounds(FastLds<LY>& F, const WTabs<LY>& W, uint64_t m, double& accd, float& accf,
                                    uint32_t* own) {
     const int cnt = __popcll(m);
     const int l = lane_id();
     const int nr = (l < cnt) ? F.nrec[l] : 0;
     uint32_t pend = 0;
     for (int r = 0; r < nr; ++r)
-        if (F.recw[l * MAXREC + r] > 0.0f) pend |= 1u << r;
+        if (F.recw[l * FastLds<LY>::MR + r] > 0.0f) pend |= 1u << r;
     while (__ballot(pend != 0u)) {
         for (uint32_t q = pend; q; q &= q - 1) {
             const int r = __ffs(q) - 1;
-            atomicMin(&own[own_slot(F.rec[l * MAXREC + r])], (uint32_t)(l * MAXREC + r));
+            atomicMin(&own[own_slot(F.rec[l * FastLds<LY>::MR + r])], (uint32_t)(l * FastLds<LY>::MR + r));
         }
         uint32_t done = 0;
         for (uint32_t q = pend; q; q &= q - 1) {
             const int r = __ffs(q) - 1;
-            const uint32_t ad = F.rec[l * MAXREC + r];
-            if (own[own_slot(ad)] == (uint32_t)(l * MAXREC + r)) {
-                __hip_atomic_fetch_add(W.rows + ad, F.recw[l * MAXREC + r], __ATOMIC_RELAXED,
+            const uint32_t ad = F.rec[l * FastLds<LY>::MR + r];
+            if (own[own_slot(ad)] == (uint32_t)(l * FastLds<LY>::MR + r)) {
+                __hip_atomic_fetch_add(W.rows + ad, F.recw[l * FastLds<LY>::MR + r], __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WAVEFRONT);
                 done |= 1u << r;
             }
         }
-        for (uint32_t q = pend; q; q &= q - 1) own[own_slot(F.rec[l * MAXREC + (__ffs(q) - 1)])] = ~0u;
+        for (uint32_t q = pend; q; q &= q - 1) own[own_slot(F.rec[l * FastLds<LY>::MR + (__ffs(q) - 1)])] = ~0u;
         pend &= ~done;
     }
     // accumulators: lanes 0..NKACC-1, components in order
     if (l < NKACC) {
-        const bool fl = kacc_is_float(l);
+        // float accumulators add both contributions; K_CLEIN adds the FP64 product w E;
+        // K_CLWOUT / K_CLEOUT add the component's FP64 sum; the others add both widened
+        const int kind = kacc_is_float(l) ? 0 : l == K_CLEIN ? 1 : (l == K_CLWOUT || l == K_CLEOUT) ? 2 : 3;
+        const int cli = l == K_CLEOUT ? 1 : 0;
         int j = 0;
         for (; j + 4 <= cnt; j += 4) {
-            double c[8];
+            float c[8];
+            double d[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 c[2 * u] = F.con[(j + u) * NCON + 2 * l];
                 c[2 * u + 1] = F.con[(j + u) * NCON + 2 * l + 1];
+                d[u] = F.cl[(j + u) * 2 + cli];
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                if (fl) accf += (float)c[u];
-                else accd += c[u];
+            for (int u = 0; u < 4; ++u) {
+                if (kind == 0) {
+                    accf += c[2 * u];
+                    accf += c[2 * u + 1];
+                } else if (kind == 1) {
+                    accd += (double)c[2 * u] * (double)c[2 * u + 1];
+                } else if (kind == 2) {
+                    accd += d[u];
+                } else {
+                    accd += (double)c[2 * u];
+                    accd += (double)c[2 * u + 1];
+                }
             }
         }
         for (; j < cnt; ++j) {
-            const double c0 = F.con[j * NCON + 2 * l], c1 = F.con[j * NCON + 2 * l + 1];
-            if (fl) {
-                accf += (float)c0;
-                accf += (float)c1;
+            const float c0 = F.con[j * NCON + 2 * l], c1 = F.con[j * NCON + 2 * l + 1];
+            if (kind == 0) {
+                accf += c0;
+                accf += c1;
+            } else if (kind == 1) {
+                accd += (double)c0 * (double)c1;
+            } else if (kind == 2) {
+                accd += F.cl[j * 2 + cli];
             } else {
-                accd += c0;
-                accd += c1;
+                accd += (double)c0;
+                accd += (double)c1;
             }
         }
     }
@@ -1952,8 +1931,11 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                                                    const uint32_t* __restrict__ ltag, const uint32_t* ctl,
                                                    Tally tal, double* acc, uint32_t* flags, Dest D) {
     if (ctl[C_DONE] || blockIdx.x >= *nlist) return;  // no cell for this workgroup
-    __shared__ float s_rows[LY::NL * LY::LB];
-    __shared__ FastLds s_fast[TW];
+    // LDS payload rows: 4-wave workgroups (few listed cells, one workgroup per CU) keep the full
+    // set; 1- and 2-wave ones (many listed cells) half, so that more workgroups fit a CU
+    constexpr int NLT = TW >= 4 ? LY::NL : LY::NL / 2;
+    __shared__ float s_rows[NLT * LY::LB];
+    __shared__ FastLds<LY> s_fast[TW];
     __shared__ uint32_t s_q[QCAP];
     __shared__ float s_qw[QCAP];
     __shared__ uint32_t s_bid[KB], s_ref[KB], s_cnt[KB], s_off[KB];
@@ -2040,7 +2022,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
             W.r1 = W.r2 = W.r3 = W.r4 = 0u;
             W.n1 = W.n2 = W.n3 = W.n4 = 0u;
             W.used = 0;
-            W.cap = (uint32_t)LY::NL;
+            W.cap = (uint32_t)NLT;
             W.rows = s_rows;
             W.orig = 0u;
             W.stage_own(out);  // e.g. the coarse gather into a fine cell
@@ -2139,7 +2121,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                 __syncthreads();
                 if (wv == 0) {  // ordered apply of the batches, in order
                     for (uint32_t bb = b0; bb < nb && bb < b0 + TW; ++bb) {
-                        FastLds& F = s_fast[bb - b0];
+                        FastLds<LY>& F = s_fast[bb - b0];
                         const uint64_t m = s_mask[bb - b0];
                         if (fast_claims<LY>(F, W)) {
                             fast_rounds<LY>(F, W, m, accd, accf, s_own);

@@ -1,33 +1,40 @@
-"""The project's elementary functions (sm_2d_amd/csrc/sm2d_math.h) against numpy's binary64
-functions rounded once to binary32: they must agree on (practically) every input — a wrong
-polynomial coefficient, reduction constant or quadrant would show as mass disagreement."""
+"""The project's binary32 elementary functions (sm_2d_amd/csrc/sm2d_math.h) against the
+correctly rounded binary32 result (numpy / scipy in binary64): the error bound of each function
+in units in the last place of the result, and the share of inputs rounded correctly.  A wrong
+polynomial coefficient, reduction constant or quadrant shows as errors of many units."""
 import numpy as np
 import pytest
 
 
-@pytest.mark.parametrize("which,fn,lo,hi,log", [
-    (0, np.exp, -85.0, 85.0, False),
-    (1, np.log, 1e-30, 1e30, True),
-    (2, np.sin, -7.0, 7.0, False),
-    (3, np.cos, -7.0, 7.0, False),
-    (4, None, -7.0, 7.0, False),
+def _ulp_error(y, ref64):
+    ref32 = ref64.astype(np.float32)
+    return np.abs(y.astype(np.float64) - ref64) / np.spacing(np.abs(ref32)).astype(np.float64)
+
+
+# (eval id, function, sample, max ulp, min share correctly rounded); the bounds hold with a
+# margin over 10^6 inputs (measured: exp 0.90, log 0.86, sin 1.40, cos 1.42, erf 1.38 ulp)
+@pytest.mark.parametrize("which,fn,sample,bound,cr", [
+    (0, np.exp, ("lin", -87.0, 88.0), 1.0, 0.90),
+    (1, np.log, ("log", 1e-38, 1e38), 1.0, 0.90),
+    (2, np.sin, ("lin", -7.0, 7.0), 1.5, 0.80),
+    (3, np.cos, ("lin", -7.0, 7.0), 1.5, 0.80),
+    (4, None, ("lin", -5.0, 5.0), 1.5, 0.90),
+    (4, None, ("lin", -1e-3, 1e-3), 0.5, 0.99),
 ])
-def test_elementary_functions_correctly_rounded(orc, which, fn, lo, hi, log):
+def test_elementary_function_error_bounds(orc, which, fn, sample, bound, cr):
     rng = np.random.default_rng(which)
-    if log:
-        x = np.exp(rng.uniform(np.log(lo), np.log(hi), 300_000)).astype(np.float32)
+    kind, lo, hi = sample
+    if kind == "log":
+        x = np.exp(rng.uniform(np.log(lo), np.log(hi), 400_000)).astype(np.float32)
     else:
-        x = rng.uniform(lo, hi, 300_000).astype(np.float32)
+        x = rng.uniform(lo, hi, 400_000).astype(np.float32)
     y = orc.math_eval(which, x)
     if which == 4:
-        from scipy.special import erf
-        ref = erf(x.astype(np.float64)).astype(np.float32)
-    else:
-        ref = fn(x.astype(np.float64)).astype(np.float32)
-    mism = np.count_nonzero(y.view(np.uint32) != ref.view(np.uint32))
-    assert mism <= 3, mism  # only inputs within ~1e-16 of a rounding boundary may differ
-    ulp = np.abs(y.view(np.int32).astype(np.int64) - ref.view(np.int32).astype(np.int64))
-    assert ulp.max() <= 1
+        from scipy.special import erf as fn
+    ref = fn(x.astype(np.float64))
+    err = _ulp_error(y, ref)
+    assert err.max() <= bound, (float(err.max()), float(x[np.argmax(err)]))
+    assert np.mean(y == ref.astype(np.float32)) >= cr
 
 
 def test_special_values(orc):
@@ -43,7 +50,7 @@ def test_special_values(orc):
 
 
 def test_sincos_equals_sin_and_cos(orc):
-    """The shared-reduction sincos returns exactly sin_d and cos_d (all quadrants)."""
+    """The shared-reduction sincos returns exactly sinf_ and cosf_ (all quadrants)."""
     rng = np.random.default_rng(9)
     x = np.concatenate([rng.uniform(-40.0, 40.0, 200_000),
                         np.array([0.0, -0.0, np.pi / 4, np.pi / 2, np.pi, -np.pi / 2])]).astype(np.float32)
