@@ -96,7 +96,8 @@ typedef struct sm2d_audit {
     uint64_t cell_bin_updates;        /* sum over iterations and processed cells of
                                          occupied slots x local bins */
     uint64_t processed_cells, processed_components;
-    uint64_t unsupported_events;      /* lateral emissions into another listed cell's bucket */
+    uint64_t cross_cell_emissions;    /* +z lateral-spread emissions into another listed cell's
+                                         bucket (device_bucket.cuh:952-983) */
     int surrogate_active;             /* 1 when the fine transport used the learned physics */
     int surrogate_fallback;           /* 1 when a K5 drift switched it back to the plain physics */
     uint64_t fine_cell_bin_updates;   /* the part of cell_bin_updates done by the fine transport */
@@ -135,6 +136,8 @@ typedef struct sm2d_ctx sm2d_ctx;
 
 const char* sm2d_last_error(void);
 int sm2d_abi_version(void);
+/* Name of HIP device `device` (get_gpu_name_internal, src/include/cuda/gpu_transport_wrapper.hpp:95). */
+int sm2d_device_name(int device, char* buf, int capacity);
 
 /* --- host-side table builders (run on the CPU, identical to the reference's host code) --- */
 
@@ -367,6 +370,15 @@ int sm2d_surrogate_export(sm2d_ctx* ctx, float* table, int capacity, int* n_mat,
  * and turns the learned physics on; rowmat[Nz] in [-1, n_mat). */
 int sm2d_surrogate_set_table(sm2d_ctx* ctx, const float* table, int n_mat, int n_nodes,
                              const int32_t* rowmat);
+
+/* Stage clocks of the transport kernel (diagnostic build only, python -m sm_2d_amd.build
+ * --stage-clocks; the normal build returns SM2D_ERR_UNSUPPORTED): out16[k] receives the
+ * microseconds the first thread of the transport workgroups spent in stage k, summed over cells
+ * (0 tags and destination marks, 1 own psi_out staging, 2 count and queue, 3 physics, 4 wait for
+ * other waves, 5 claims, 6 ordered adds, 7 serial apply, 8 end-of-cell barrier, 9 table flush,
+ * 10 tallies), out16[15] the cells; reset = 1 zeroes the sums afterwards.  No counterpart in the
+ * reference. */
+int sm2d_stage_clocks(int device, double* out16, int reset);
 
 /* --- self-test of the shared elementary functions on the device -------------------------- */
 /* Evaluates f(x[i]) for i < n on the GPU: which = 0 exp, 1 log, 2 sin, 3 cos, 4 erf, 5 sin and

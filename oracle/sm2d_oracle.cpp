@@ -303,7 +303,7 @@ struct orc_ctx {
     std::unique_ptr<float[]> bkval;
     std::vector<int> phase_of;  // 1 = listed in current phase
     int n_threads = 1;
-    uint64_t unsupported = 0;
+    uint64_t cross_cell = 0;  // lateral emissions into another listed cell's +z bucket
     bool water = true;
 
     Psi& pin() { return psi[in]; }
@@ -662,9 +662,23 @@ struct Phase {
         return c->enc(td, ed, x, z);
     }
 
+    // Lateral emissions into another listed cell's +z bucket, applied after every cell of the
+    // phase has made its own emissions (apply_foreign).
+    struct Foreign {
+        int src_li, cell;
+        float theta, E, w;
+        int xs;
+    };
+    std::vector<Foreign> foreign;
+
     // device_emit_lateral_spread (device_bucket.cuh:857-987); returns dropped weight,
-    // *bnd receives the out-of-domain weight.
-    float emit_lateral(int cell, int tz, float theta, float E, float weight, float xo, float sx,
+    // *bnd receives the out-of-domain weight.  The reference emits into a neighbour's +z bucket
+    // with device atomics while that neighbour's own thread emits into it too, so the order of
+    // a bucket's claims and FP32 adds is a thread race.  The order taken here, which the device
+    // memory model allows (each bucket location orders its relaxed atomics independently), is:
+    // a bucket's own cell first, then the emissions of other cells in ascending source-cell and
+    // component order.  The emitted weight is booked to the source cell when it is emitted.
+    float emit_lateral(int li, int cell, int tz, float theta, float E, float weight, float xo, float sx,
                        float* bnd, CellAcc& a) {
         if (weight <= 0.0f) return 0.0f;
         float dropped = 0.0f;
@@ -687,11 +701,42 @@ struct Phase {
             int ec = it + iz * Nx;
             Bucket* bk = bucket(ec, ZP);
             if (!bk) { dropped += weight * wf; continue; }
-            if (ec != cell) c->unsupported++;
             int xs = off < 0 ? c->Xs - 1 : (off > 0 ? 0 : c->Xs / 2);
+            if (ec != cell) {
+                c->cross_cell++;
+                const float wl = weight * wf;
+                int tb[2];
+                float tw[2];
+                theta_split(c, theta, tb, tw);
+                const float* ee = c->E_edges.data();
+                int eb = c->find_bin(fmaxf(ee[0], fminf(E, ee[c->g.N_E])));
+                eb = std::max(0, std::min(eb, c->g.N_E - 1));
+                for (int ti = 0; ti < 2; ++ti) {
+                    if (tw[ti] <= 0.0f) continue;
+                    const float we = wl * tw[ti];
+                    a.cl_w_out += we;
+                    a.cl_E_out += (double)we * (double)c->ecenter(eb);
+                }
+                foreign.push_back(Foreign{li, ec, theta, E, wl, xs});
+                continue;
+            }
             dropped += emit4d(bk, theta, E, weight * wf, xs, 0, a);
         }
         return dropped;
+    }
+
+    // The deferred lateral emissions, in production order (ascending source cell, component
+    // order, target offset): each target bucket takes them after its own cell's emissions.
+    void apply_foreign() {
+        for (const Foreign& f : foreign) {
+            CellAcc booked;  // already booked to the source cell when emitted
+            float d = emit4d(bucket(f.cell, ZP), f.theta, f.E, f.w, f.xs, 0, booked);
+            if (d > 0.0f) {  // a full bucket without an occupied slot: cannot happen (32 slots)
+                acc[f.src_li].drop_w += (double)d;
+                acc[f.src_li].drop_E += (double)(f.E * d);
+            }
+        }
+        foreign.clear();
     }
 
     void run_cell(int li, int cell) {
@@ -896,7 +941,7 @@ struct Phase {
                     float E_emit = c->ecenter(c->find_bin(E_new));
                     if (face == ZP) {
                         float lat_bnd = 0.0f;
-                        float drop = emit_lateral(cell, iz + 1, th_new, E_new, w_new, x_new,
+                        float drop = emit_lateral(li, cell, iz + 1, th_new, E_new, w_new, x_new,
                                                   sigma_x_tr, &lat_bnd, a);
                         if (lat_bnd > 0.0f) {
                             a.w_bnd += lat_bnd;
@@ -1276,6 +1321,7 @@ void run_phase(orc_ctx* c, const sm2d_transport* tp, float sx0, const std::vecto
     // Cells write disjoint ψ_out cells and their own buckets, so the loop parallelises; the
     // claim bookkeeping of ψ_out (occupied-cell list) is made thread-safe by a final rebuild.
     for (size_t i = 0; i < list.size(); ++i) ph.run_cell((int)i, list[i]);
+    ph.apply_foreign();
     for (size_t i = 0; i < list.size(); ++i) {
         int cell = list[i];
         CellAcc& a = ph.acc[i];
@@ -1586,7 +1632,7 @@ int orc_reset(orc_ctx* c) {
     std::fill(c->W_bnd.begin(), c->W_bnd.end(), 0);
     c->src_W_in = c->src_W_out = c->src_W_slot = 0;
     c->src_E_in = c->src_E_out = c->src_E_slot = c->src_E_repr = 0;
-    c->unsupported = 0;
+    c->cross_cell = 0;
     return SM2D_OK;
 }
 
@@ -1801,7 +1847,7 @@ static int run_iters(orc_ctx* c, const sm2d_transport* tp, float sx0, int n_iter
             if (!r.E_pass) au->E_fail_iters++;
             au->max_cell_W_closure = std::max(au->max_cell_W_closure, clw);
             au->max_cell_E_closure = std::max(au->max_cell_E_closure, clE);
-            au->unsupported_events = c->unsupported;
+            au->cross_cell_emissions = c->cross_cell;
         }
         if (tp->fail_fast_on_audit && (!r.W_pass || !r.E_pass)) return fail("K5 audit fail-fast", SM2D_ERR_AUDIT);
         c->in = 1 - c->in;  // K6

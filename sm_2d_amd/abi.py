@@ -101,7 +101,7 @@ class Audit(C.Structure):
         ("max_cell_W_closure", C.c_double), ("max_cell_E_closure", C.c_double),
         ("cell_bin_updates", C.c_uint64),
         ("processed_cells", C.c_uint64), ("processed_components", C.c_uint64),
-        ("unsupported_events", C.c_uint64),
+        ("cross_cell_emissions", C.c_uint64),
         ("surrogate_active", C.c_int), ("surrogate_fallback", C.c_int),
         ("fine_cell_bin_updates", C.c_uint64), ("surrogate_substeps", C.c_uint64),
     ]

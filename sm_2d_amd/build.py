@@ -32,22 +32,34 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError("build failed: " + " ".join(cmd) + "\n" + r.stdout + r.stderr)
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(BUILD, exist_ok=True)
-    host_o = os.path.join(BUILD, "sm2d_host.o")
-    io_o = os.path.join(BUILD, "sm2d_io.o")
-    dev_o = os.path.join(BUILD, "sm2d_runtime.o")
+# Diagnostic build with the transport kernel's stage clocks (sm2d_stage_clocks): same sources and
+# work, into its own directory (load it with SM2D_LIB=<path>).
+CLOCKS_LIB = os.path.join(HERE, "_clocks", "libsm2d_hip.so")
+
+
+def build(verbose: bool = False, stage_clocks: bool = False) -> str:
+    out_dir = os.path.dirname(CLOCKS_LIB) if stage_clocks else HERE
+    bdir = os.path.join(BUILD, "clocks") if stage_clocks else BUILD
+    lib = CLOCKS_LIB if stage_clocks else LIB
+    os.makedirs(bdir, exist_ok=True)
+    os.makedirs(out_dir, exist_ok=True)
+    host_o = os.path.join(bdir, "sm2d_host.o")
+    io_o = os.path.join(bdir, "sm2d_io.o")
+    dev_o = os.path.join(bdir, "sm2d_runtime.o")
+    extra = ["-DSM2D_STAGE_CLOCKS"] if stage_clocks else []
     steps = [
         ["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, "sm2d_host.cpp"), "-o", host_o],
         ["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, "sm2d_io.cpp"), "-o", io_o],
-        [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "sm2d_runtime.hip"), "-o", dev_o],
-        [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", host_o, io_o, dev_o, "-o", LIB + ".tmp"],
+        [HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, "sm2d_runtime.hip"), "-o", dev_o],
+        [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", host_o, io_o, dev_o, "-o", lib + ".tmp"],
     ]
     for cmd in steps:
         if verbose:
             print(" ".join(cmd), flush=True)
         _run(cmd)
-    os.replace(LIB + ".tmp", LIB)
+    os.replace(lib + ".tmp", lib)
+    if stage_clocks:
+        return lib
     # the INI driver (run_simulation's counterpart), linked against the library beside it
     _run(["g++", *HOST_FLAGS, os.path.join(CSRC, "sm2d_run.cpp"), "-o", RUNNER + ".tmp",
           "-L" + HERE, "-lsm2d_hip", "-Wl,-rpath,$ORIGIN"])
@@ -56,4 +68,4 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, stage_clocks="--stage-clocks" in sys.argv))

@@ -63,9 +63,11 @@ struct Tabs {
 
 // Device control block (u32 words).  C_DONE: 0 running, 1 ended (empty lists or a capacity
 // overflow), 2 paused (the next iteration needs a phase the queued chunk left out).
+// C_FNT counts the cells whose +z bucket takes lateral emissions of other cells (k_foreign
+// applies them and returns it to 0).
 enum Ctl {
     C_NACT = 0, C_NCRS = 1, C_NDEST = 2, C_DONE = 3, C_EP = 4, C_IT = 5, C_DONE_IT = 6,
-    C_NOCC0 = 8, C_NOCC1 = 9, C_NDEST1 = 10, C_BPC0 = 11, C_BPC1 = 12, NCTL = 16
+    C_NOCC0 = 8, C_NOCC1 = 9, C_NDEST1 = 10, C_BPC0 = 11, C_BPC1 = 12, C_FNT = 13, NCTL = 16
 };
 
 struct Tp {
@@ -90,6 +92,8 @@ struct Bkt {
     float* pool;
     uint32_t* pcount;
     uint32_t pcap;
+    uint32_t* ftag;   // [cap] phase tag of a listed cell whose +z bucket takes other cells' lateral emissions
+    uint32_t* flist;  // [cap] list positions of those cells (count in ctl[C_FNT])
 };
 
 struct Tally {  // cumulative per-cell tallies (src/cuda/k1k6_pipeline.cuh:174-180)
@@ -109,7 +113,8 @@ enum Acc {
 };
 
 // Device error flags.
-enum Flag { F_PSI_FULL, F_BKT_FULL, F_UNSUPPORTED, F_DEST_FULL, NFLAG };
+// F_CROSS counts lateral emissions into another listed cell's +z bucket (not an error).
+enum Flag { F_PSI_FULL, F_BKT_FULL, F_CROSS, F_DEST_FULL, NFLAG };
 
 // ---- wave helpers -----------------------------------------------------------------------
 __device__ inline int lane_id() { return (int)(threadIdx.x & 63); }

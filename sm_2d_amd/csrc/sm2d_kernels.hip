@@ -18,6 +18,38 @@
 
 namespace sm2d {
 
+// Diagnostic build (-DSM2D_STAGE_CLOCKS, python -m sm_2d_amd.build --stage-clocks): the first
+// thread of every transport workgroup adds the constant-rate clock ticks of each stage of a
+// cell to stage_clk[stage] (STAGES below; [15] counts cells).  The work is the same as in the
+// normal build; the normal build compiles the stamps away.
+#ifdef SM2D_STAGE_CLOCKS
+__device__ unsigned long long stage_clk[16];
+#define SM2D_STAMP(k)                                                   \
+    do {                                                                \
+        if (threadIdx.x == 0) {                                         \
+            const unsigned long long t_ = wall_clock64();               \
+            atomicAdd(&stage_clk[k], t_ - clk_prev);                    \
+            clk_prev = t_;                                              \
+        }                                                               \
+    } while (0)
+#define SM2D_STAMP_START() unsigned long long clk_prev = wall_clock64()
+#define SM2D_STAMP_RESTART() clk_prev = wall_clock64()
+#else
+#define SM2D_STAMP(k) \
+    do {              \
+    } while (0)
+#define SM2D_STAMP_START() \
+    do {                   \
+    } while (0)
+#define SM2D_STAMP_RESTART() \
+    do {                     \
+    } while (0)
+#endif
+// stages: 0 tags and destination marks, 1 own psi_out staging, 2 count and queue of non-zero
+// bins, 3 physics and per-lane preparation (wave 0's batches), 4 wait for the other waves'
+// physics, 5 claims, 6 ordered adds and accumulators, 7 serial apply, 8 end-of-round barrier,
+// 9 table flush, 10 tallies.
+
 // Learned straggling width, scattering width and attenuation exponent of a fine-transport
 // substep from the surrogate table (linear in ln dR between the two nearest nodes); false when
 // the row's material, the step or the table entry is outside the network's training range.
@@ -822,6 +854,7 @@ __device__ inline Bins emission_bins(const Tabs& T, float theta, float E, int eg
 // Wave-uniform broadcast of one component's results (lane j -> all lanes).
 struct Comp {
     int kind;     // 0 prune, 1 cutoff before the step, 2 transported
+    uint32_t fmask;  // face 0: lateral targets q (bit q) that are other listed cells
     int surr;     // 1: the learned-physics table gave the substep's widths
     float w, E;
     float edep, w_rem, E_rem, w_new, E_new;
@@ -995,6 +1028,28 @@ struct LaneOut {
     double clwout, cleout;
 };
 
+// Weight and energy of a component's lateral emissions into other listed cells' +z buckets,
+// booked to the source cell when emitted (they are applied after the phase, apply_foreign): per
+// target q in fmask and per theta part, the weight (w_new sw[q]) tw and its energy at the
+// emission bin's centre.
+__device__ inline void foreign_booking(const Comp& r, double& clw, double& cle) {
+#pragma unroll
+    for (int q = 0; q < 11; ++q) {
+        if (!((r.fmask >> q) & 1u)) continue;
+        const float wl = r.w_new * r.sw[q];
+        if (r.bn.tw0 > 0.0f) {
+            const float we = wl * r.bn.tw0;
+            clw += (double)we;
+            cle += (double)we * (double)r.bn.Ecen;
+        }
+        if (r.bn.tw1 > 0.0f) {
+            const float we = wl * r.bn.tw1;
+            clw += (double)we;
+            cle += (double)we * (double)r.bn.Ecen;
+        }
+    }
+}
+
 template <class LY>
 __device__ inline void rec_emit(FastLds<LY>& F, LaneOut& o, int t, const Bins& b, float weight, int xs, int zs) {
     if (weight <= 0.0f) return;
@@ -1076,6 +1131,7 @@ __device__ inline LaneOut fast_prepare(FastLds<LY>& F, const Comp& r, bool live,
                 const float E_emit = r.g0, drop = r.g1, lat_bnd = r.g2;
                 const float wf5 = r.sw[5];
                 if (wf5 > 0.0f) rec_emit<LY>(F, o, 1, r.bn, w_new * wf5, LY::XS / 2, 0);
+                foreign_booking(r, o.clwout, o.cleout);
                 if (lat_bnd > 0.0f) bnd(lat_bnd);
                 if (drop > 0.0f) {
                     con_set(F, K_DROPW, 0, drop);
@@ -1279,47 +1335,20 @@ __device__ inline void fast_rounds(FastLds<LY>& F, const WTabs<LY>& W, uint64_t 
     // accumulators: lanes 0..NKACC-1, components in order
     if (l < NKACC) {
         // float accumulators add both contributions; K_CLEIN adds the FP64 product w E;
-        // K_CLWOUT / K_CLEOUT add the component's FP64 sum; the others add both widened
-        const int kind = kacc_is_float(l) ? 0 : l == K_CLEIN ? 1 : (l == K_CLWOUT || l == K_CLEOUT) ? 2 : 3;
+        // K_CLWOUT / K_CLEOUT add the component's FP64 sum; the others add both widened.  Every
+        // lane evaluates both forms and keeps its own (no divergence in the apply wave); adding
+        // +0.0 leaves an FP64 accumulator unchanged.
+        const bool fl = kacc_is_float(l), prod = l == K_CLEIN, sum = l == K_CLWOUT || l == K_CLEOUT;
         const int cli = l == K_CLEOUT ? 1 : 0;
-        int j = 0;
-        for (; j + 4 <= cnt; j += 4) {
-            float c[8];
-            double d[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                c[2 * u] = F.con[(j + u) * NCON + 2 * l];
-                c[2 * u + 1] = F.con[(j + u) * NCON + 2 * l + 1];
-                d[u] = F.cl[(j + u) * 2 + cli];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (kind == 0) {
-                    accf += c[2 * u];
-                    accf += c[2 * u + 1];
-                } else if (kind == 1) {
-                    accd += (double)c[2 * u] * (double)c[2 * u + 1];
-                } else if (kind == 2) {
-                    accd += d[u];
-                } else {
-                    accd += (double)c[2 * u];
-                    accd += (double)c[2 * u + 1];
-                }
-            }
-        }
-        for (; j < cnt; ++j) {
+        for (int j = 0; j < cnt; ++j) {
             const float c0 = F.con[j * NCON + 2 * l], c1 = F.con[j * NCON + 2 * l + 1];
-            if (kind == 0) {
-                accf += c0;
-                accf += c1;
-            } else if (kind == 1) {
-                accd += (double)c0 * (double)c1;
-            } else if (kind == 2) {
-                accd += F.cl[j * 2 + cli];
-            } else {
-                accd += (double)c0;
-                accd += (double)c1;
-            }
+            const double cs = F.cl[j * 2 + cli];
+            const double d0 = prod ? (double)c0 * (double)c1 : sum ? cs : (double)c0;
+            const double d1 = (prod || sum) ? 0.0 : (double)c1;
+            const float f = (accf + c0) + c1;
+            accf = fl ? f : accf;
+            accd += d0;
+            accd += d1;
         }
     }
 }
@@ -1371,8 +1400,7 @@ struct CellGeo {
 
 template <class LY, bool FINE>
 __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const CellGeo& G, bool live, float w_in,
-                                           int lidx_in, uint32_t slot, uint32_t bT, uint32_t bEb,
-                                           uint32_t* flags, bool count_unsup) {
+                                           int lidx_in, uint32_t slot, uint32_t bT, uint32_t bEb) {
     const int lc = G.lc, iz = G.iz, ix = G.ix, Nx = G.Nx, NT = G.NT, NE = G.NE;
     const float dx = G.dx, dz = G.dz, hdx = G.hdx, hdz = G.hdz, tmin = G.tmin, dth = G.dth;
     const float rh = G.rh, X0 = G.X0, A = G.A, B = G.B, sx0 = G.sx0;
@@ -1381,6 +1409,7 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
     int eg = -1;  // energy bin of the component (a starting guess for the bin searches)
     r.kind = 2;
     r.face = -1;
+    r.fmask = 0;
     r.surr = 0;
     r.lidx = lidx_in;
     r.w = live ? w_in : 0.0f;
@@ -1648,7 +1677,7 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
             r.g0 = ecenter(T.ee, find_bin(T.ee, NE, r.E_new, eg));
             if (r.face == 0) {
                 float drop = 0.0f, lat_bnd = 0.0f;
-                uint32_t unsup = 0;
+                uint32_t fm = 0;
 #pragma unroll
                 for (int q = 0; q < 11; ++q) {
                     const float wf = r.sw[q];
@@ -1660,15 +1689,17 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
                     }
                     if (q == 5) {
                         drop += 0.0f;  // the centre emission's drop, zero unless a pool fills
+                    } else if ((latlisted >> q) & 1u) {
+                        // another listed cell's +z bucket: emitted after every cell of the
+                        // phase has made its own emissions (apply_foreign)
+                        fm |= 1u << q;
                     } else {
-                        // another cell's +z bucket: a drop when that cell is not listed
-                        // in this phase (device_bucket.cuh:953-963); a listed target
-                        // would need a cross-wave ordered write, which this path rejects
-                        if ((latlisted >> q) & 1u) ++unsup;
+                        // a cell not listed in this phase has no bucket: a drop
+                        // (device_bucket.cuh:953-963)
                         drop += w_new * wf;
                     }
                 }
-                if (unsup && count_unsup) atomicAdd(&flags[F_UNSUPPORTED], unsup);
+                r.fmask = fm;
                 r.g1 = drop;
                 r.g2 = lat_bnd;
             } else {
@@ -1759,15 +1790,31 @@ __device__ inline void serial_apply(const Comp& r, uint64_t m, WTabs<LY>& W, Cel
             if (face == 0) {
                 const float wf5 = rlf(r.sw[5], j);
                 const float lat_bnd = rlf(r.g2, j);
+                const uint32_t fmj = rlu(r.fmask, j);
                 float d5 = 0.0f;
                 if (wf5 > 0.0f) d5 = emit<LY>(W, a, T, bk, flags, 0, bn, w_new * wf5, LY::XS / 2, 0);
+#pragma unroll
+                for (int q = 0; q < 11; ++q) {  // lateral emissions into other listed cells
+                    if (!((fmj >> q) & 1u)) continue;
+                    const float wl = w_new * rlf(r.sw[q], j);
+                    if (bn.tw0 > 0.0f) {
+                        const float we = wl * bn.tw0;
+                        a.cl_wout += (double)we;
+                        a.cl_Eout += (double)we * (double)bn.Ecen;
+                    }
+                    if (bn.tw1 > 0.0f) {
+                        const float we = wl * bn.tw1;
+                        a.cl_wout += (double)we;
+                        a.cl_Eout += (double)we * (double)bn.Ecen;
+                    }
+                }
                 if (d5 == 0.0f) {
                     drop = rlf(r.g1, j);
                 } else {  // the lateral sum with the centre's drop in its place (q = 5)
 #pragma unroll
                     for (int q = 0; q < 11; ++q) {
                         const float wf = rlf(r.sw[q], j);
-                        if (wf <= 0.0f) continue;
+                        if (wf <= 0.0f || ((fmj >> q) & 1u)) continue;
                         const int it = ix + q - 5;
                         if (it < 0 || it >= Nx) continue;
                         drop += (q == 5) ? d5 : w_new * wf;
@@ -1897,6 +1944,103 @@ __device__ inline void serial_apply(const Comp& r, uint64_t m, WTabs<LY>& W, Cel
     }
 }
 
+// Per-cell part of the transport geometry of cell `cell` (one wavefront): position, material,
+// Fermi-Eyges moments of its spot and the lateral +z neighbours listed in this phase.
+__device__ inline void set_cell_geo(CellGeo& G, const Tabs& T, int cell, const uint32_t* ltag, uint32_t tag) {
+    const int l = lane_id();
+    const int sp = cell / T.Ns;
+    G.lc = cell - sp * T.Ns;  // cell within the spot
+    G.iz = G.lc / G.Nx;
+    G.ix = G.lc - G.iz * G.Nx;
+    G.rh = T.rho[G.iz];
+    G.X0 = X0W * T.x0r[G.iz];
+    G.A = T.feA[sp * T.Nz + G.iz];
+    G.B = T.feB[sp * T.Nz + G.iz];
+    G.sx0 = T.sx0[sp];
+    const int off = l - 5, it = G.ix + off;
+    const bool q = l < 11 && off != 0 && it >= 0 && it < G.Nx && ltag[cell + off] == tag;
+    G.latlisted = (uint32_t)__ballot(q);
+}
+
+// The lateral emissions of a phase into other listed cells' +z buckets (device_emit_lateral_spread,
+// device_bucket.cuh:952-983), applied by one wavefront once every cell of the phase has made its
+// own emissions.  For each marked target bucket, the sources S (the listed cells up to five to
+// either side, in ascending cell order) are transported again, component by component in their
+// serial order, with the same function and inputs (so the same results), and each component's
+// emission toward the target is claimed and added in that order: source-cell order, component
+// order, theta part.  This is one of the orders the reference's device atomics can produce: each
+// bucket location orders its relaxed atomics independently, so the bucket's own cell may come
+// first.  The weights were booked to their sources when emitted.
+template <class LY, bool FINE>
+__device__ void apply_foreign(const Tabs& T, const Tp& P, CellGeo G, const Psi& in, Bkt& bk,
+                              const uint32_t* __restrict__ list, const uint32_t* __restrict__ ltag, uint32_t tag,
+                              uint32_t nf, uint32_t* flags) {
+    const int l = lane_id();
+    for (uint32_t k = 0; k < nf; ++k) {
+        const uint32_t pT = bk.flist[k];
+        const int tc = (int)list[pT];
+        const int ixT = (tc % T.Ns) % T.Nx;
+        const uint32_t base = (pT * 4 + 0) * KB;  // the target's +z bucket (face 0)
+        uint32_t n = bk.n[pT * 4];
+        uint32_t mybid = (l < 32 && (uint32_t)l < n) ? bk.bid[base + l] : EMPTY;
+        uint32_t myref = (l < 32 && (uint32_t)l < n) ? bk.sref[base + l] : 0u;
+        for (int off = 5; off >= -5; --off) {  // source S = target - off, ascending
+            if (off == 0) continue;
+            const int ixS = ixT - off;
+            if (ixS < 0 || ixS >= T.Nx) continue;
+            const int S = tc - off;
+            if (ltag[S] != tag) continue;
+            set_cell_geo(G, T, S, ltag, tag);
+            if (G.iz + 1 >= T.Nz) continue;  // +z exits at the last row leave the grid
+            const int q = off + 5;           // the target's index in the source's lateral spread
+            const int xs = off > 0 ? 0 : LY::XS - 1;
+            const uint32_t ns = in.ns[S];
+            for (uint32_t sl = 0; sl < ns; ++sl) {
+                const uint32_t bid = in.bid[(size_t)S * KB + sl];
+                if (bid == EMPTY) continue;
+                const float* vin = in.pool + (size_t)in.sref[(size_t)S * KB + sl] * LY::LB;
+                for (int kk = 0; kk < LY::PL; ++kk) {  // local bins ascending
+                    const float w = vin[l + 64 * kk];
+                    const bool live = w > 0.0f;
+                    if (!__ballot(live)) continue;
+                    const Comp r = transport_component<LY, FINE>(T, P, G, live, live ? w : 0.0f, l + 64 * kk, sl,
+                                                                 bid & 0xFFF, (bid >> 12) & 0xFFF);
+                    float wq = 0.0f;
+#pragma unroll
+                    for (int u = 0; u < 11; ++u)
+                        if (u == q) wq = r.sw[u];
+                    const bool em = live && r.kind == 2 && r.E_new > 0.1f && r.face == 0 && ((r.fmask >> q) & 1u);
+                    for (uint64_t m = __ballot(em); m; m &= m - 1) {
+                        const int j = __ffsll((unsigned long long)m) - 1;
+                        const float wl = rlf(r.w_new, j) * rlf(wq, j);
+                        const int bc = rli(r.bn.bc, j), tb1 = rli(r.bn.tb1, j), eb = rli(r.bn.eb, j);
+                        const float tw0 = rlf(r.bn.tw0, j), tw1 = rlf(r.bn.tw1, j);
+                        const int el = eb % LY::EL;
+                        const uint32_t bE = (uint32_t)(eb / LY::EL);
+#pragma unroll
+                        for (int ti = 0; ti < 2; ++ti) {
+                            const float tw = ti == 0 ? tw0 : tw1;
+                            if (tw <= 0.0f) continue;
+                            const float we = wl * tw;
+                            const int t = ti == 0 ? bc : tb1;
+                            const uint32_t want = ((uint32_t)(t / LY::TL) & 0xFFF) | ((bE & 0xFFF) << 12);
+                            const int lidx = LY::enc(t % LY::TL, el, xs, 0);
+                            uint32_t sel;
+                            const int s2 = table_find_or_claim(mybid, myref, n, want, sel, bk.pool, bk.pcount, bk.pcap,
+                                                               bk.bid + base, bk.sref + base, LY::LB, flags, F_BKT_FULL);
+                            if (s2 < 0) continue;  // the bucket pool is full: the run ends with its flag
+                            const int l2 = (sel == want) ? lidx : remap<LY>(lidx, want, sel);
+                            const uint32_t ref = rlu(myref, s2);
+                            if (l == 0) bk.pool[(size_t)ref * LY::LB + l2] += we;
+                        }
+                    }
+                }
+            }
+        }
+        if (l == 0) bk.n[pT * 4] = n;
+    }
+}
+
 // Item queue of a cell: its non-zero (slot, local bin) components in the serial order
 // (slots ascending, local bins ascending), processed in batches of 64 by the four waves.
 // TW waves work on one listed cell: they evaluate the physics of TW batches at once, wave 0
@@ -1928,7 +2072,8 @@ template <class LY, bool FINE, int TW>
 __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Psi out, Bkt bk,
                                                    const uint32_t* __restrict__ list,
                                                    const uint32_t* __restrict__ nlist,
-                                                   const uint32_t* __restrict__ ltag, const uint32_t* ctl,
+                                                   const uint32_t* __restrict__ ltag,
+                                                   const uint32_t* __restrict__ lpos, uint32_t* ctl,
                                                    Tally tal, double* acc, uint32_t* flags, Dest D) {
     if (ctl[C_DONE] || blockIdx.x >= *nlist) return;  // no cell for this workgroup
     // LDS payload rows: 4-wave workgroups (few listed cells, one workgroup per CU) keep the full
@@ -1979,22 +2124,11 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
     G.tmin = T.te[0];
     G.dth = (T.te[T.NT] - G.tmin) / T.NT;
     G.R_on = lookup_R(T, fmaxf(P.E_on, T.ee[0]));
+    SM2D_STAMP_START();
     for (uint32_t li = blockIdx.x; li < n; li += gridDim.x) {
+        SM2D_STAMP_RESTART();
         const int cell = (int)list[li];
-        const int sp = cell / T.Ns;
-        G.lc = cell - sp * T.Ns;  // cell within the spot
-        G.iz = G.lc / G.Nx;
-        G.ix = G.lc - G.iz * G.Nx;
-        G.rh = T.rho[G.iz];
-        G.X0 = X0W * T.x0r[G.iz];
-        G.A = T.feA[sp * T.Nz + G.iz];
-        G.B = T.feB[sp * T.Nz + G.iz];
-        G.sx0 = T.sx0[sp];
-        {   // lateral +z neighbours listed in this phase (their buckets cannot take foreign emissions)
-            const int off = l - 5, it = G.ix + off;
-            const bool q = l < 11 && off != 0 && it >= 0 && it < G.Nx && ltag[cell + off] == tag;
-            G.latlisted = (uint32_t)__ballot(q);
-        }
+        set_cell_geo(G, T, cell, ltag, tag);
         if (wv == 0 && l < 4) {  // the cell's neighbours join the phase's destination set
             const int d = l == 0   ? (G.iz + 1 < T.Nz ? cell + G.Nx : -1)
                           : l == 1 ? (G.iz > 0 ? cell - G.Nx : -1)
@@ -2008,6 +2142,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
             s_bid[threadIdx.x] = in.bid[(size_t)cell * KB + threadIdx.x];
             s_ref[threadIdx.x] = in.sref[(size_t)cell * KB + threadIdx.x];
         }
+        SM2D_STAMP(0);
         // wave 0 owns the tables and the accumulators of the cell
         WTabs<LY> W;
         uint32_t comps = 0;  // components of the cell (wave 0)
@@ -2028,6 +2163,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
             W.stage_own(out);  // e.g. the coarse gather into a fine cell
         }
         __syncthreads();
+        SM2D_STAMP(1);
         uint32_t s0 = 0;
         while (s0 < ns) {
             // non-zero components per slot (the payloads of SU slots per wave are fetched at once)
@@ -2099,6 +2235,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                 }
             }
             __syncthreads();
+            SM2D_STAMP(2);
             const uint32_t nb = (total + 63) / 64;
             for (uint32_t b0 = 0; b0 < nb; b0 += TW) {
                 const uint32_t b = b0 + (uint32_t)wv;
@@ -2109,7 +2246,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                     const uint32_t slot = it >> 16, bid = s_bid[slot];
                     const Comp r = transport_component<LY, FINE>(T, P, G, live, live ? s_qw[i] : 0.0f,
                                                                  (int)(it & 0xFFFFu), slot, bid & 0xFFF,
-                                                                 (bid >> 12) & 0xFFF, flags, true);
+                                                                 (bid >> 12) & 0xFFF);
                     const LaneOut o = fast_prepare<LY, FINE>(s_fast[wv], r, live, G.ix, G.iz, G.Nx, T.Nz);
                     s_fast[wv].nkey[l] = o.nkey;
                     s_fast[wv].nrec[l] = o.nrec;
@@ -2117,14 +2254,34 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                     const uint32_t nsur = (uint32_t)__popcll(__ballot(live && r.kind == 2 && r.surr));
                     if (l == 0) s_mask[wv] = m;
                     if (FINE && l == 0 && nsur) atomicAdd(&s_surr, nsur);
+                    SM2D_STAMP(3);
+                    // lateral emissions into other listed cells' +z buckets: mark the targets for
+                    // apply_foreign at the end of the launch
+                    const uint32_t fm = (live && r.kind == 2 && r.E_new > 0.1f && r.face == 0 && G.iz + 1 < T.Nz)
+                                            ? r.fmask : 0u;
+                    if (__ballot(fm != 0u)) {
+                        for (uint32_t b = fm; b; b &= b - 1) {
+                            const int q = __ffs(b) - 1;
+                            const uint32_t pT = lpos[cell + q - 5];
+                            if (atomicExch(&bk.ftag[pT], tag) != tag) bk.flist[atomicAdd(&ctl[C_FNT], 1u)] = pT;
+                        }
+                        const uint32_t nc = (uint32_t)__popc(fm);
+                        uint32_t tot = nc;
+                        for (int o2 = 32; o2 > 0; o2 >>= 1) tot += __shfl_xor(tot, o2);
+                        if (l == 0) atomicAdd(&flags[F_CROSS], tot);
+                    }
                 }
                 __syncthreads();
+                SM2D_STAMP(4);
                 if (wv == 0) {  // ordered apply of the batches, in order
                     for (uint32_t bb = b0; bb < nb && bb < b0 + TW; ++bb) {
                         FastLds<LY>& F = s_fast[bb - b0];
                         const uint64_t m = s_mask[bb - b0];
-                        if (fast_claims<LY>(F, W)) {
+                        const bool fast = fast_claims<LY>(F, W);
+                        SM2D_STAMP(5);
+                        if (fast) {
                             fast_rounds<LY>(F, W, m, accd, accf, s_own);
+                            SM2D_STAMP(6);
                             comps += (uint32_t)__popcll(m);
                         } else {
                             // the batch again, in wave 0, for the serial form
@@ -2134,11 +2291,12 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                             const uint32_t slot = it >> 16, bid = s_bid[slot];
                             const Comp r = transport_component<LY, FINE>(T, P, G, live, live ? s_qw[i] : 0.0f,
                                                                          (int)(it & 0xFFFFu), slot, bid & 0xFFF,
-                                                                         (bid >> 12) & 0xFFF, flags, false);
+                                                                         (bid >> 12) & 0xFFF);
                             CellAccum a = {};
                             lanes_to_accum(a, accd, accf);
                             a.comps = comps;
                             serial_apply<LY, FINE>(r, m, W, a, T, out, bk, flags, G);
+                            SM2D_STAMP(7);
                             accum_to_lanes(a, accd, accf);
                             comps = a.comps;
                         }
@@ -2153,7 +2311,9 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
             CellAccum a = {};
             lanes_to_accum(a, accd, accf);
             a.comps = comps;
+            SM2D_STAMP(8);
             flush_tables<LY>(W, out, bk, cell, li, flags);
+            SM2D_STAMP(9);
             if (l == 0) {
                 tal.edep[cell] += a.edep;
                 tal.w_cut[cell] += a.w_cut;
@@ -2180,9 +2340,36 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                 o[A_UPD] = (double)((size_t)ns * LY::LB);
                 o[A_SURR] = (double)s_surr;
             }
+            SM2D_STAMP(10);
+#ifdef SM2D_STAGE_CLOCKS
+            if (threadIdx.x == 0) atomicAdd(&stage_clk[15], 1ull);
+#endif
         }
         __syncthreads();
     }
+}
+
+// The phase's lateral emissions into other listed cells' +z buckets (apply_foreign), after the
+// transport launch that marked them: one wavefront; returns at once when nothing was marked.
+template <class LY, bool FINE>
+__global__ __launch_bounds__(64) void k_foreign(Tabs T, Tp P, Psi in, Bkt bk, const uint32_t* __restrict__ list,
+                                                const uint32_t* __restrict__ ltag, uint32_t* ctl, uint32_t* flags) {
+    const uint32_t nf = ctl[C_FNT];
+    if (ctl[C_DONE] || nf == 0) return;
+    const uint32_t tag = 2 * ctl[C_EP] + (FINE ? 1u : 0u);
+    CellGeo G;
+    G.Nx = T.Nx;
+    G.NT = T.NT;
+    G.NE = T.NE;
+    G.dx = T.dx;
+    G.dz = T.dz;
+    G.hdx = T.dx * 0.5f;
+    G.hdz = T.dz * 0.5f;
+    G.tmin = T.te[0];
+    G.dth = (T.te[T.NT] - G.tmin) / T.NT;
+    G.R_on = lookup_R(T, fmaxf(P.E_on, T.ee[0]));
+    apply_foreign<LY, FINE>(T, P, G, in, bk, list, ltag, tag, nf, flags);
+    if (threadIdx.x == 0) ctl[C_FNT] = 0u;
 }
 
 // Destination set of a phase: the four neighbours of every listed cell, deduplicated with a

@@ -115,7 +115,7 @@ struct sm2d_ctx {
     DBuf<float> pool[2];
     int in = 0;
     DBuf<uint32_t> bitmap;
-    DBuf<uint32_t> bbid, bn, bsref;
+    DBuf<uint32_t> bbid, bn, bsref, bftag, bflist;
     DBuf<float> bpool;
     DBuf<uint32_t> maskv, ltag, lpos, dtag, act, crs, dlist, ctl, tcount, tcount2, flags;
     DBuf<uint32_t> gmoved;  // profiling: bucket values moved per (iteration slot, phase) of a chunk
@@ -188,6 +188,8 @@ struct sm2d_ctx {
         b.bid = bbid.p; b.n = bn.p; b.sref = bsref.p; b.pool = bpool.p;
         b.pcount = ctl.p + (fine ? C_BPC1 : C_BPC0);
         b.pcap = pcap;
+        b.ftag = bftag.p;
+        b.flist = bflist.p;
         return b;
     }
     Tally tally() const {
@@ -378,11 +380,13 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
         if (c->rec)                                                                                           \
             hipExtLaunchKernelGGL(k_transport<LY, F, W>, dim3(gb), dim3(64 * W), (uint32_t)dyn, c->stream,    \
                                   e_start, e_stop, 0u, T, P, c->psi(in), c->psi(out), c->bkt(fine), list, nl, \
-                                  (const uint32_t*)c->ltag.p, (const uint32_t*)c->ctl.p, c->tally(), acc,     \
+                                  (const uint32_t*)c->ltag.p, (const uint32_t*)c->lpos.p, c->ctl.p,           \
+                                  c->tally(), acc,                                                            \
                                   c->flags.p, D);                                                             \
         else                                                                                                  \
             k_transport<LY, F, W><<<gb, 64 * W, dyn, c->stream>>>(T, P, c->psi(in), c->psi(out), c->bkt(fine), \
-                                                                 list, nl, c->ltag.p, c->ctl.p, c->tally(),   \
+                                                                 list, nl, c->ltag.p, c->lpos.p, c->ctl.p,    \
+                                                                 c->tally(),                                  \
                                                                  acc, c->flags.p, D);                         \
     } while (0)
         if (fine) {
@@ -395,6 +399,10 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
             else SM2D_TRANSPORT(false, 4);
         }
 #undef SM2D_TRANSPORT
+        if (fine) k_foreign<LY, true><<<1, 64, 0, c->stream>>>(T, P, c->psi(in), c->bkt(fine), list, c->ltag.p,
+                                                             c->ctl.p, c->flags.p);
+        else k_foreign<LY, false><<<1, 64, 0, c->stream>>>(T, P, c->psi(in), c->bkt(fine), list, c->ltag.p,
+                                                           c->ctl.p, c->flags.p);
         uint32_t* moved = c->prof ? c->gmoved.p + 2 * c->ev_it + fine : nullptr;
         const unsigned gg = blocks_for(4ull * h, 1, 65536);
         if (c->rec)
@@ -915,6 +923,8 @@ int each_buffer(sm2d_ctx* c, A&& a) {
     HIPCHK(a(c->bn, (size_t)c->lcap * 4, BK_BUCKET));
     HIPCHK(a(c->bsref, (size_t)c->lcap * 4 * KB, BK_BUCKET));
     HIPCHK(a(c->bpool, (size_t)c->pcap * c->LB, BK_BUCKET));
+    HIPCHK(a(c->bftag, c->lcap, BK_BUCKET));
+    HIPCHK(a(c->bflist, c->lcap, BK_BUCKET));
     for (DBuf<uint32_t>* b : {&c->maskv, &c->ltag, &c->lpos, &c->dtag}) HIPCHK(a(*b, N, BK_CELL));
     HIPCHK(a(c->act, c->ocap, BK_OTHER));
     HIPCHK(a(c->crs, c->ocap, BK_OTHER));
@@ -1002,6 +1012,7 @@ int create(int device, const sm2d_grid* g, int n_spots, sm2d_ctx** out) {
     if (rc_alloc) return rc_alloc;
     c->src.assign((size_t)n_spots * 7, 0.0);
     HIPCHK(hipMemsetAsync(c->maskv.p, 0, sizeof(uint32_t) * N, c->stream));
+    HIPCHK(hipMemsetAsync(c->bftag.p, 0, sizeof(uint32_t) * c->lcap, c->stream));
     HIPCHK(hipMemsetAsync(c->ltag.p, 0, sizeof(uint32_t) * N, c->stream));
     HIPCHK(hipMemsetAsync(c->dtag.p, 0, sizeof(uint32_t) * N, c->stream));
     HIPCHK(hipMemsetAsync(c->ctl.p, 0, sizeof(uint32_t) * NCTL, c->stream));
@@ -1051,6 +1062,39 @@ int sm2d_ctx_create(int device, const sm2d_grid* g, sm2d_ctx** out) { return cre
 
 int sm2d_ctx_create_batch(int device, const sm2d_grid* g, int n_spots, sm2d_ctx** out) {
     return create(device, g, n_spots, out);
+}
+
+int sm2d_stage_clocks(int device, double* out16, int reset) {
+#ifdef SM2D_STAGE_CLOCKS
+    if (!out16) return err("sm2d_stage_clocks: no output");
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipDeviceSynchronize());
+    unsigned long long v[16];
+    HIPCHK(hipMemcpyFromSymbol(v, HIP_SYMBOL(stage_clk), sizeof(v)));
+    int khz = 0;
+    HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+    for (int k = 0; k < 15; ++k) out16[k] = khz > 0 ? (double)v[k] * 1e3 / (double)khz : 0.0;
+    out16[15] = (double)v[15];
+    if (reset) {
+        std::memset(v, 0, sizeof(v));
+        HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(stage_clk), v, sizeof(v)));
+    }
+    return SM2D_OK;
+#else
+    (void)device;
+    (void)out16;
+    (void)reset;
+    return err("stage clocks need the diagnostic build (python -m sm_2d_amd.build --stage-clocks)",
+               SM2D_ERR_UNSUPPORTED);
+#endif
+}
+
+int sm2d_device_name(int device, char* buf, int capacity) {
+    if (!buf || capacity <= 0) return err("sm2d_device_name: no buffer");
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, device));
+    std::snprintf(buf, (size_t)capacity, "%s (%s)", p.name, p.gcnArchName);
+    return SM2D_OK;
 }
 
 int sm2d_memory_estimate(const sm2d_grid* g, int n_spots, sm2d_memory_plan* out) {
@@ -1357,7 +1401,7 @@ int sm2d_run_iterations(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int s
     if (au) {
         uint32_t f[NFLAG];
         HIPCHK(hipMemcpy(f, c->flags.p, sizeof(f), hipMemcpyDeviceToHost));
-        au->unsupported_events = f[F_UNSUPPORTED];
+        au->cross_cell_emissions = f[F_CROSS];
     }
     if (tallies) return download_tallies(c, tallies, 0);
     return SM2D_OK;
@@ -1436,10 +1480,7 @@ static int finish_audit(sm2d_ctx* c, sm2d_audit* au, const sm2d_audit& it, int n
     }
     uint32_t f[NFLAG];
     HIPCHK(hipMemcpy(f, c->flags.p, sizeof(f), hipMemcpyDeviceToHost));
-    au->unsupported_events = f[F_UNSUPPORTED];
-    if (f[F_UNSUPPORTED])
-        return err("lateral emission into another listed cell's bucket is not supported on the GPU path",
-                   SM2D_ERR_UNSUPPORTED);
+    au->cross_cell_emissions = f[F_CROSS];
     return SM2D_OK;
 }
 

@@ -36,6 +36,8 @@ HIP_PROTOS = {
     "surrogate_set_table": [C.c_void_p, abi.fp, C.c_int, C.c_int, abi.i32p],
     "surrogate_check": [C.c_void_p, C.c_float, abi.fp],
     "surrogate_stats": [C.c_void_p, abi.fp],
+    "stage_clocks": [C.c_int, abi.dp, C.c_int],
+    "device_name": [C.c_int, C.c_char_p, C.c_int],
 }
 
 
@@ -46,9 +48,11 @@ def load_library() -> Library:
     global _LIB
     if _LIB is not None:
         return _LIB
-    if not os.path.exists(abi.LIB_PATH):
-        raise RuntimeError(f"{abi.LIB_PATH} is missing; build it with `python -m sm_2d_amd.build`")
-    lib = Library(abi.LIB_PATH, "sm2d_", {**HIP_PROTOS, **sio.PROTOS})
+    # SM2D_LIB selects another build of the same sources (the stage-clock diagnostic build)
+    path = os.environ.get("SM2D_LIB", abi.LIB_PATH)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing; build it with `python -m sm_2d_amd.build`")
+    lib = Library(path, "sm2d_", {**HIP_PROTOS, **sio.PROTOS})
     lib.lib.sm2d_abi_version.restype = C.c_int
     lib.lib.sm2d_mlp_param_count.restype = C.c_int
     if lib.lib.sm2d_abi_version() != 1:

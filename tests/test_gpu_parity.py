@@ -151,7 +151,7 @@ def test_random_psi_iterations_bit_exact(hip, orc, layout):
     assert _rows_key(rg) == _rows_key(ro)
     assert np.array_equal(tg, to)
     assert aug.cell_bin_updates == auo.cell_bin_updates
-    assert aug.unsupported_events == 0
+    assert aug.cross_cell_emissions == auo.cross_cell_emissions
 
 
 @pytest.mark.parametrize("prob", [
@@ -410,3 +410,50 @@ def test_psi_load_rejects_bins_off_the_grid(hip, orc):
         e.psi_load(row(int(ok[-1])))
         with pytest.raises(LibError, match="outside the grid"):
             e.psi_load(row(int(bad[0])))
+
+
+@pytest.mark.parametrize("layout", [problems.LAYOUT_REF, problems.LAYOUT_CFG])
+def test_cross_cell_lateral_emission_bit_exact(hip, orc, layout):
+    """Cells of 0.05 mm are narrower than the coarse transport's lateral spread, so +z exits emit
+    into the +z buckets of listed neighbours (device_bucket.cuh:952-983) throughout the spot: the
+    GPU applies them after the phase's own emissions, in source-cell and component order, as the
+    oracle does."""
+    prob = problems.reference_grid_case(Nx=60, Nz=60, E0=70.0, sigma_x=0.4, n_samples=300,
+                                        max_iterations=200, layout=layout)
+    prob.grid.update(dx=0.05, dz=0.05, x_min=-1.5)
+    g, o = _pair(hip, orc, prob)
+    sp, tp = prob.make_spot(), prob.make_transport()
+    eg, ag = g.run_spot(sp, tp)
+    eo, ao = o.run_spot(sp, tp)
+    assert ao.cross_cell_emissions > 100_000
+    assert ag.cross_cell_emissions == ao.cross_cell_emissions
+    assert ag.iterations == ao.iterations
+    assert np.array_equal(eg, eo)
+    assert ag.cell_bin_updates == ao.cell_bin_updates
+    assert ag.W_transport_drop == pytest.approx(ao.W_transport_drop, rel=1e-12, abs=1e-18)
+    assert ag.max_cell_W_closure < 1e-6 and ag.W_fail_iters == 0
+
+
+def test_cross_cell_routing_matches_oracle(hip, orc):
+    """One component on 0.02 mm cells with its neighbours listed: the phase space after one
+    iteration equals the oracle's, slot for slot (tests/test_oracle_cross_cell.py pins the
+    oracle's routing)."""
+    prob = problems.Problem("cross_cell", problems._grid(9, 4, 0.02, 0.02, 36, -0.35, 0.35, problems.LAYOUT_REF),
+                            "piecewise", dict(E_fine_on=10.0, E_fine_off=11.0, max_iterations=1), [dict(E0=150.0)])
+    g, o = _pair(hip, orc, prob)
+    bid = 4 | (314 << 12)
+    lidx = 2 + 4 * (1 + 2 * 4)
+    cells = [4, 0, 1, 2, 3, 5, 6, 7, 8]
+    rows = dict(cell=np.array(cells), slot=np.zeros(9, int), bid=np.full(9, bid), lidx=np.full(9, lidx),
+                w=np.array([1.0e-3] + [1e-13] * 8, np.float32))
+    tp = prob.make_transport(sigma_x_initial=3.0)
+    for e in (g, o):
+        e.reset()
+        e.psi_load(rows)
+    _, _, ag = g.run_iterations(tp, 1)
+    _, _, ao = o.run_iterations(tp, 1)
+    bg, rg = g.psi_dump()
+    bo, ro = o.psi_dump()
+    assert np.array_equal(bg, bo)
+    assert _rows_key(rg) == _rows_key(ro)
+    assert ag.cross_cell_emissions == ao.cross_cell_emissions > 0
