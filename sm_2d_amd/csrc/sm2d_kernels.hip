@@ -1340,15 +1340,27 @@ __device__ inline void fast_rounds(FastLds<LY>& F, const WTabs<LY>& W, uint64_t 
         // +0.0 leaves an FP64 accumulator unchanged.
         const bool fl = kacc_is_float(l), prod = l == K_CLEIN, sum = l == K_CLWOUT || l == K_CLEOUT;
         const int cli = l == K_CLEOUT ? 1 : 0;
-        for (int j = 0; j < cnt; ++j) {
-            const float c0 = F.con[j * NCON + 2 * l], c1 = F.con[j * NCON + 2 * l + 1];
-            const double cs = F.cl[j * 2 + cli];
-            const double d0 = prod ? (double)c0 * (double)c1 : sum ? cs : (double)c0;
-            const double d1 = (prod || sum) ? 0.0 : (double)c1;
-            const float f = (accf + c0) + c1;
-            accf = fl ? f : accf;
-            accd += d0;
-            accd += d1;
+        constexpr int G = 8;  // components whose contributions are fetched before they are added
+        for (int j0 = 0; j0 < cnt; j0 += G) {
+            float c0[G], c1[G];
+            double cs[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                const int j = min(j0 + u, cnt - 1);
+                c0[u] = F.con[j * NCON + 2 * l];
+                c1[u] = F.con[j * NCON + 2 * l + 1];
+                cs[u] = F.cl[j * 2 + cli];
+            }
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                if (j0 + u >= cnt) break;
+                const double d0 = prod ? (double)c0[u] * (double)c1[u] : sum ? cs[u] : (double)c0[u];
+                const double d1 = (prod || sum) ? 0.0 : (double)c1[u];
+                const float f = (accf + c0[u]) + c1[u];
+                accf = fl ? f : accf;
+                accd += d0;
+                accd += d1;
+            }
         }
     }
 }
@@ -2077,8 +2089,9 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                                                    Tally tal, double* acc, uint32_t* flags, Dest D) {
     if (ctl[C_DONE] || blockIdx.x >= *nlist) return;  // no cell for this workgroup
     // LDS payload rows: 4-wave workgroups (few listed cells, one workgroup per CU) keep the full
-    // set; 1- and 2-wave ones (many listed cells) half, so that more workgroups fit a CU
-    constexpr int NLT = TW >= 4 ? LY::NL : LY::NL / 2;
+    // set (24 KiB); 2-wave ones 12 KiB and 1-wave ones 14 KiB, so that three (four) of them fit
+    // a CU with the 256-bin energy tables
+    constexpr int NLT = TW >= 4 ? LY::NL : TW == 2 ? LY::NL / 2 : (LY::NL * 7) / 12;
     __shared__ float s_rows[NLT * LY::LB];
     __shared__ FastLds<LY> s_fast[TW];
     __shared__ uint32_t s_q[QCAP];

@@ -151,3 +151,27 @@ def test_encodings_match_reference(ref):
         assert ref.ref_x_offset(xs, C.c_float(0.5)) == np.float32(0.5) * np.float32((2 * xs + 1 - 8) / 16)
     for zs in range(4):
         assert ref.ref_z_offset(zs, C.c_float(0.5)) == np.float32(0.5) * np.float32((2 * zs + 1 - 4) / 8)
+
+
+def test_two_percent_step_rule_matches_reference(ref, orc):
+    """The 2 % range step rule (step_rule = 1) against the reference's host
+    compute_max_step_physics (step_control.hpp:18-66) on the same R-LUT (the log grid 0.1-300 MeV
+    of 256 bins that ref_log_lut_eval holds).  Wherever the oracle's range lookup (the device rule
+    of device_lut.cuh:66-85) returns the reference host lookup's value, the step is equal bit for
+    bit, across every refinement branch (E < 5, 10, 20, 50 MeV and above, the 0.05 mm floor)."""
+    from oracle.pyoracle import OracleEngine
+    from sm_2d_amd import problems
+    prob = problems.reference_grid_case(Nx=10, Nz=10)
+    prob.energy = ("log", 0.1, 300.0, 256)
+    eng = problems.setup_engine(orc, prob, OracleEngine)
+    E = np.concatenate([np.geomspace(0.15, 250.0, 4000),
+                        [4.99, 5.0, 9.99, 10.0, 19.99, 20.0, 49.99, 50.0]]).astype(np.float32)
+    R_o, S_o = eng.eval(0, E), eng.eval(4, E)
+    R_r, S_r = np.zeros_like(E), np.zeros_like(E)
+    ref.ref_log_lut_eval(0, _f(E), None, _f(R_r), len(E))
+    ref.ref_log_lut_eval(4, _f(E), None, _f(S_r), len(E))
+    same = R_o == R_r
+    assert same.mean() > 0.8
+    assert np.array_equal(S_o[same], S_r[same])
+    assert np.allclose(S_o, S_r, rtol=2e-6)
+    assert S_o.min() == np.float32(0.05)  # the floor is reached near the end of range
