@@ -289,6 +289,7 @@ struct orc_ctx {
     std::vector<float> surr;
     std::vector<int> rowmat;
     int surr_nodes = 0, surr_on = 0, surr_active = 0;
+    float sdev_max = 0;  // monitor maximum of the current iteration
     Psi psi[2];
     int in = 0;  // index of psi_in
     std::vector<uint8_t> mask, mask_prev;
@@ -543,6 +544,28 @@ void spread_weights_sub(float* w, float xm, float s, float dx, int N) {
 
 enum Face { ZP = 0, ZM = 1, XP = 2, XM = 3 };
 
+// Run-time monitor of the learned physics (the library's K5 drift check): every
+// SURR_MONITOR_EVERY-th cell of a spot compares its learned widths with the plain ones; a largest
+// deviation above SURR_DRIFT_TOL, a weight balance off by 1e-6 or a non-finite energy balance
+// switches the following iterations to the plain physics.
+constexpr int SURR_MONITOR_EVERY = 8;
+constexpr float SURR_DRIFT_TOL = 0.5f;
+
+// Deviation of learned (sigma_E, sigma_theta, a) from the plain physics of the same substep: the
+// largest of the |ln sigma| differences and |a - a_plain| / (a_plain + 1e-4 / mm x rh ds), the
+// criterion the library also applies to each table node before it accepts the table.
+float learned_deviation(const orc_ctx* c, float E, float ds, float rh, float X0, float mdE, float sE,
+                        float sth, float a) {
+    const float sE_p = straggling_sigma(E, ds, rh);
+    const float Em_p = fmaxf(E - 0.5f * mdE, c->E_min());
+    const float sth_p = sqrtf_(fmaxf(scattering_power(Em_p, ds, X0) * ds, 0.0f));
+    const float a_p = nuclear_sigma(E) * rh * ds;
+    const float d0 = fabsf(logf_(fmaxf(sE, 1e-30f)) - logf_(fmaxf(sE_p, 1e-30f)));
+    const float d1 = fabsf(logf_(fmaxf(sth, 1e-30f)) - logf_(fmaxf(sth_p, 1e-30f)));
+    const float d2 = fabsf(a - a_p) / (a_p + 1e-4f * rh * ds);
+    return fmaxf(d0, fmaxf(d1, d2));
+}
+
 // Per-cell results of one transport pass (the seven accumulators of
 // k3_finetransport.cu:140-146 plus drop channels and closure sums).
 struct CellAcc {
@@ -551,6 +574,7 @@ struct CellAcc {
     double drop_w = 0, drop_E = 0, prune_w = 0, prune_E = 0;
     double cl_w_in = 0, cl_w_out = 0, cl_E_in = 0, cl_E_out = 0;
     uint64_t comps = 0, updates = 0, surr = 0;
+    float sdev = 0;  // largest learned-vs-plain deviation of the cell's substeps (monitored cells)
 };
 
 }  // namespace
@@ -819,6 +843,8 @@ struct Phase {
                     const bool use_s = c->surr_active && ds > 0.0f && ds / X0 >= 1e-6f &&
                                        surrogate_lookup(c, ebin, iz, rh * ds, s_sE, s_sth, s_a);
                     if (use_s) a.surr++;
+                    if (use_s && cell % SURR_MONITOR_EVERY == 0)
+                        a.sdev = fmaxf(a.sdev, learned_deviation(c, E, ds, rh, X0, mean_dE, s_sE, s_sth, s_a));
                     if (tp->enable_straggling) {
                         unsigned seed = (unsigned)((cell * 7 + slot * 13 + lidx * 17) ^ 0x5DEECE66DL);
                         float sE = use_s ? s_sE : straggling_sigma(E, ds, rh);
@@ -1338,6 +1364,7 @@ void run_phase(orc_ctx* c, const sm2d_transport* tp, float sx0, const std::vecto
         drop_E += a.drop_E + a.prune_E;
         if (a.cl_w_in > 0) cl_w = std::max(cl_w, std::fabs(a.cl_w_in - a.cl_w_out) / a.cl_w_in);
         if (fine && a.cl_E_in > 0) cl_E = std::max(cl_E, std::fabs(a.cl_E_in - a.cl_E_out) / a.cl_E_in);
+        if (fine) c->sdev_max = fmaxf(c->sdev_max, a.sdev);
         if (au) {
             au->cell_bin_updates += a.updates;
             if (fine) au->fine_cell_bin_updates += a.updates;
@@ -1805,14 +1832,10 @@ static int run_iters(orc_ctx* c, const sm2d_transport* tp, float sx0, int n_iter
                      sm2d_k5_report* last, sm2d_audit* au) {
     int bon = b_threshold(c, tp->E_fine_on), boff = b_threshold(c, tp->E_fine_off);
     std::vector<int> act, crs;
-    // learned physics: a drift detected by K5 takes effect at the next block of 32 iterations,
-    // the granularity at which the library reads its audits back
-    const int chunk = tp->fail_fast_on_audit ? 1 : 32;
-    bool drift = false;
     c->surr_active = c->surr_on;
     if (au) au->surrogate_active = c->surr_active;
     for (int it = 0; it < n_iter; ++it) {
-        if (drift && it % chunk == 0) c->surr_active = 0;
+        c->sdev_max = 0;
         c->mask_prev = c->mask;
         k1(c, bon, boff, tp->weight_active_min, c->mask_prev.data(), c->mask.data());
         compact(c, c->mask.data(), act, crs);
@@ -1830,9 +1853,16 @@ static int run_iters(orc_ctx* c, const sm2d_transport* tp, float sx0, int n_iter
         k5(c, dW, dE, c->src_W_out, c->src_W_slot, c->src_E_out, c->src_E_slot, drop_w, drop_E,
            first ? 1 : 0, &r);
         if (last) *last = r;
-        if (c->surr_active && (!r.W_pass || !std::isfinite(r.E_error))) {
-            drift = true;
-            if (au) au->surrogate_fallback = 1;
+        if (c->surr_active) {  // the drift check of K5, on the balances without the source terms
+            const double Wi = r.W_in - r.W_source_out_of_grid - r.W_source_slot_drop;
+            const double Wr = r.W_out + r.W_cutoff + r.W_nuclear + r.W_boundary + r.W_transport_drop;
+            const double Ei = r.E_in - r.E_source_out_of_grid - r.E_source_slot_drop;
+            const double Er = r.E_out + r.E_dep + r.E_cutoff + r.E_nuclear + r.E_boundary + r.E_transport_drop;
+            const double werr = std::fabs(Wi - Wr) / std::max(Wi, 1e-20);
+            if (!(werr < 1e-6) || !std::isfinite(Ei - Er) || !(c->sdev_max <= SURR_DRIFT_TOL)) {
+                c->surr_active = 0;
+                if (au) au->surrogate_fallback = 1;
+            }
         }
         if (au) {
             au->W_transport_drop += drop_w;

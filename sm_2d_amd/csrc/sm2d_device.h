@@ -64,11 +64,21 @@ struct Tabs {
 // Device control block (u32 words).  C_DONE: 0 running, 1 ended (empty lists or a capacity
 // overflow), 2 paused (the next iteration needs a phase the queued chunk left out).
 // C_FNT counts the cells whose +z bucket takes lateral emissions of other cells (k_foreign
-// applies them and returns it to 0).
+// applies them and returns it to 0).  C_DRIFT: set by the closing audit of an iteration whose K5
+// row flags the learned physics (drift_check); from the next iteration on the fine transport
+// uses the plain physics.
 enum Ctl {
-    C_NACT = 0, C_NCRS = 1, C_NDEST = 2, C_DONE = 3, C_EP = 4, C_IT = 5, C_DONE_IT = 6,
+    C_NACT = 0, C_NCRS = 1, C_NDEST = 2, C_DONE = 3, C_EP = 4, C_IT = 5, C_DONE_IT = 6, C_DRIFT = 7,
     C_NOCC0 = 8, C_NOCC1 = 9, C_NDEST1 = 10, C_BPC0 = 11, C_BPC1 = 12, C_FNT = 13, NCTL = 16
 };
+
+// Run-time monitor of the learned physics: in every eighth cell (cell index within its spot
+// divisible by 8) each fine-transport substep that takes the learned widths also evaluates the
+// plain ones; the largest deviation (ln sigma_E, ln sigma_theta, and the attenuation exponent
+// relative to a + 1e-4 / mm x dR, the criterion of the table validation) enters the iteration's
+// K5 row, and a deviation above SURR_DRIFT_TOL switches the rest of the run to the plain physics.
+constexpr int SURR_MONITOR_EVERY = 8;
+constexpr float SURR_DRIFT_TOL = 0.5f;
 
 struct Tp {
     float E_on, E_off, wmin, step_coarse;
@@ -109,7 +119,7 @@ struct Tally {  // cumulative per-cell tallies (src/cuda/k1k6_pipeline.cuh:174-1
 // Per listed cell and iteration (FP64), reduced by K5 in list order.
 enum Acc {
     A_EDEP, A_WCUT, A_ECUT, A_WNUC, A_ENUC, A_WBND, A_EBND, A_DROPW, A_DROPE,
-    A_CLWIN, A_CLWOUT, A_CLEIN, A_CLEOUT, A_COMPS, A_UPD, A_SURR, NACC
+    A_CLWIN, A_CLWOUT, A_CLEIN, A_CLEOUT, A_COMPS, A_UPD, A_SURR, A_SDEV, NACC
 };
 
 // Device error flags.

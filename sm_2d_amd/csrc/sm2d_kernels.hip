@@ -856,6 +856,7 @@ struct Comp {
     int kind;     // 0 prune, 1 cutoff before the step, 2 transported
     uint32_t fmask;  // face 0: lateral targets q (bit q) that are other listed cells
     int surr;     // 1: the learned-physics table gave the substep's widths
+    float sdev;   // monitored cells: deviation of the learned widths from the plain ones
     float w, E;
     float edep, w_rem, E_rem, w_new, E_new;
     int face;     // -1 stays in the cell
@@ -1400,6 +1401,22 @@ __device__ inline void accum_to_lanes(const CellAccum& a, double& accd, float& a
 // that the binary searches and interpolations of every component read LDS.
 inline size_t transport_dyn_lds(int NE) { return sizeof(float) * (4 * (size_t)NE + 1); }
 
+// Deviation of a substep's learned widths from the plain physics of the same substep (E, ds in
+// density rh and radiation length X0, mean energy loss mdE): the largest of |ln sigma_E|,
+// |ln sigma_theta| differences and |a - a_plain| / (a_plain + 1e-4 / mm x rh ds), the quantities
+// and the criterion of the table's node validation (k_table_plain).
+__device__ inline float learned_deviation(const Tabs& T, float E, float ds, float rh, float X0, float mdE,
+                                          float sE, float sth, float a) {
+    const float sE_p = straggling(E, ds, rh);
+    const float Em_p = fmaxf(E - 0.5f * mdE, T.ee[0]);
+    const float sth_p = sqrtf_(fmaxf(scat_power(Em_p, ds, X0) * ds, 0.0f));
+    const float a_p = nuc_sigma(E) * rh * ds;
+    const float d0 = fabsf(logf_(fmaxf(sE, 1e-30f)) - logf_(fmaxf(sE_p, 1e-30f)));
+    const float d1 = fabsf(logf_(fmaxf(sth, 1e-30f)) - logf_(fmaxf(sth_p, 1e-30f)));
+    const float d2 = fabsf(a - a_p) / (a_p + 1e-4f * rh * ds);
+    return fmaxf(d0, fmaxf(d1, d2));
+}
+
 // Per-bin physics of one component (the per-bin algorithm of k2_coarsetransport.cu:76-704 and
 // k3_finetransport.cu:94-760) plus the table-independent parts of its apply: one lane, one
 // (slot, local bin) of the listed cell.
@@ -1408,6 +1425,7 @@ struct CellGeo {
     float dx, dz, hdx, hdz, tmin, dth, rh, X0, A, B, sx0;
     float R_on;  // water range at the fine-transport threshold energy (a launch constant)
     uint32_t latlisted;
+    int monitor;  // the learned physics of this cell's substeps is compared with the plain physics
 };
 
 template <class LY, bool FINE>
@@ -1423,6 +1441,7 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
     r.face = -1;
     r.fmask = 0;
     r.surr = 0;
+    r.sdev = 0.0f;
     r.lidx = lidx_in;
     r.w = live ? w_in : 0.0f;
     r.edep = r.w_rem = r.E_rem = r.w_new = r.E_new = 0.0f;
@@ -1489,6 +1508,7 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
                 }
                 float smm = fmaxf(ds, 0.0f);
                 float Em = fmaxf(E - 0.5f * dE, T.ee[0]);
+                if (use_s && G.monitor) r.sdev = learned_deviation(T, E, ds, rh, X0, mdE, s_sE, s_sth, s_a);
                 float Tp_ = use_s ? (s_sth * s_sth) / smm : scat_power(Em, smm, X0);
                 sth = use_s ? s_sth : sqrtf_(fmaxf(Tp_ * smm, 0.0f));
                 float dC = 2.0f * B * smm + A * smm * smm + Tp_ * smm * smm * smm / 3.0f;
@@ -1969,6 +1989,7 @@ __device__ inline void set_cell_geo(CellGeo& G, const Tabs& T, int cell, const u
     G.A = T.feA[sp * T.Nz + G.iz];
     G.B = T.feB[sp * T.Nz + G.iz];
     G.sx0 = T.sx0[sp];
+    G.monitor = (G.lc % SURR_MONITOR_EVERY) == 0;
     const int off = l - 5, it = G.ix + off;
     const bool q = l < 11 && off != 0 && it >= 0 && it < G.Nx && ltag[cell + off] == tag;
     G.latlisted = (uint32_t)__ballot(q);
@@ -2100,6 +2121,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
     __shared__ uint64_t s_mask[TW];
     __shared__ uint32_t s_round[2];
     __shared__ uint32_t s_surr;  // learned-physics substeps of the cell
+    __shared__ uint32_t s_sdev;  // bits of the largest learned-physics deviation of the cell (>= 0)
     __shared__ uint32_t s_own[NOWN];
     extern __shared__ float s_tab[];
     for (int i = threadIdx.x; i < NOWN; i += blockDim.x) s_own[i] = ~0u;
@@ -2123,6 +2145,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
         T.lE = lE;
     }
     const uint32_t tag = 2 * ctl[C_EP] + (FINE ? 1u : 0u);
+    if (ctl[C_DRIFT]) P.surr = 0;  // K5 flagged the learned physics in an earlier iteration
     const int l = lane_id(), wv = (int)(threadIdx.x >> 6);
     const uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
     const uint32_t n = *nlist;
@@ -2150,7 +2173,10 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
             if (d >= 0) dest_mark(D, (uint32_t)d, tag, flags);
         }
         const uint32_t ns = in.ns[cell];
-        if (threadIdx.x == 0) s_surr = 0;
+        if (threadIdx.x == 0) {
+            s_surr = 0;
+            s_sdev = 0;
+        }
         if (threadIdx.x < ns) {
             s_bid[threadIdx.x] = in.bid[(size_t)cell * KB + threadIdx.x];
             s_ref[threadIdx.x] = in.sref[(size_t)cell * KB + threadIdx.x];
@@ -2267,6 +2293,11 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                     const uint32_t nsur = (uint32_t)__popcll(__ballot(live && r.kind == 2 && r.surr));
                     if (l == 0) s_mask[wv] = m;
                     if (FINE && l == 0 && nsur) atomicAdd(&s_surr, nsur);
+                    if (FINE && G.monitor) {  // non-negative floats order as their bits
+                        uint32_t dv = live ? __float_as_uint(r.sdev) : 0u;
+                        for (int o2 = 32; o2 > 0; o2 >>= 1) dv = max(dv, (uint32_t)__shfl_xor((int)dv, o2));
+                        if (l == 0 && dv) atomicMax(&s_sdev, dv);
+                    }
                     SM2D_STAMP(3);
                     // lateral emissions into other listed cells' +z buckets: mark the targets for
                     // apply_foreign at the end of the launch
@@ -2352,6 +2383,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                 o[A_COMPS] = (double)a.comps;
                 o[A_UPD] = (double)((size_t)ns * LY::LB);
                 o[A_SURR] = (double)s_surr;
+                o[A_SDEV] = (double)__uint_as_float(s_sdev);
             }
             SM2D_STAMP(10);
 #ifdef SM2D_STAGE_CLOCKS
@@ -2370,6 +2402,7 @@ __global__ __launch_bounds__(64) void k_foreign(Tabs T, Tp P, Psi in, Bkt bk, co
     const uint32_t nf = ctl[C_FNT];
     if (ctl[C_DONE] || nf == 0) return;
     const uint32_t tag = 2 * ctl[C_EP] + (FINE ? 1u : 0u);
+    if (ctl[C_DRIFT]) P.surr = 0;
     CellGeo G;
     G.Nx = T.Nx;
     G.NT = T.NT;
@@ -2596,9 +2629,10 @@ __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const ui
 // ------------------------------------------------------------------------------------------
 enum { R_WIN, R_EIN, R_WOUT, R_EOUT, R_K4W, R_K4E, NR1 };
 // Audit vector: [0,NR1) cell sums, [NR1,NR1+NACC) phase sums, then max cell W closure, max
-// fine-cell E closure, the number of listed cells of the iteration and the cell-bin updates of
-// its fine phase.
-constexpr int NAUD = NR1 + NACC + 4;
+// fine-cell E closure, max learned-physics deviation of the monitored fine cells, the number of
+// listed cells of the iteration and the cell-bin updates of its fine phase.
+constexpr int NAUD = NR1 + NACC + 5;
+constexpr int IDX_SDEV = NR1 + NACC + 2;
 
 // Stage 1a: one wavefront per list entry i: W_in / E_in from K1's per-entry sums of psi_in,
 // W_out / E_out of psi_out's i-th occupied cell (coalesced payload reads, FP64 butterfly) and
@@ -2690,7 +2724,7 @@ __global__ __launch_bounds__(256) void k_audit_final(const double* part, const u
                                                      uint32_t hist_rows, uint32_t* ctl, int force,
                                                      const double* __restrict__ pe, Psi nxt,
                                                      const uint32_t* __restrict__ nxt_occ, const uint32_t* nxt_n,
-                                                     ListOut L) {
+                                                     ListOut L, int drift_check) {
     if (ctl[C_DONE] && !force) return;
     if (blockIdx.x > 0) {  // clear the next iteration's output buffer (this iteration's input)
         clear_cells(nxt, nxt_occ, *nxt_n, (blockIdx.x - 1) * blockDim.x + threadIdx.x, (gridDim.x - 1) * blockDim.x);
@@ -2719,6 +2753,7 @@ __global__ __launch_bounds__(256) void k_audit_final(const double* part, const u
             if (o[A_CLWIN] > 0) r[NR1 + NACC] = fmax(r[NR1 + NACC], fabs(o[A_CLWIN] - o[A_CLWOUT]) / o[A_CLWIN]);
             if (ph && o[A_CLEIN] > 0)
                 r[NR1 + NACC + 1] = fmax(r[NR1 + NACC + 1], fabs(o[A_CLEIN] - o[A_CLEOUT]) / o[A_CLEIN]);
+            if (ph) r[IDX_SDEV] = fmax(r[IDX_SDEV], o[A_SDEV]);
         }
     }
     for (int k = 0; k < NV; ++k) sh[k][threadIdx.x] = r[k];
@@ -2732,6 +2767,19 @@ __global__ __launch_bounds__(256) void k_audit_final(const double* part, const u
     }
     double* dst = hist ? hist + (size_t)((ctl[C_IT] - 1u) % hist_rows) * NAUD : outv;
     if (threadIdx.x < NV) dst[threadIdx.x] = sh[threadIdx.x][0];
+    if (drift_check && threadIdx.x == 0) {
+        // K5 flags the learned physics: a learned width far from the plain one in a monitored
+        // cell, a weight balance off by 1e-6 or a non-finite energy balance (source terms, which
+        // enter both sides of the balances, left out); the next iteration runs the plain physics
+        const double W_in = sh[R_WIN][0];
+        const double Wr = sh[R_WOUT][0] + sh[NR1 + A_WCUT][0] + sh[NR1 + A_WNUC][0] + sh[NR1 + A_WBND][0] +
+                          fmax(0.0, sh[NR1 + A_DROPW][0] + sh[R_K4W][0]);
+        const double Er = sh[R_EOUT][0] + sh[NR1 + A_EDEP][0] + sh[NR1 + A_ECUT][0] + sh[NR1 + A_ENUC][0] +
+                          sh[NR1 + A_EBND][0] + fmax(0.0, sh[NR1 + A_DROPE][0] + sh[R_K4E][0]);
+        const double werr = fabs(W_in - Wr) / fmax(W_in, 1e-20);
+        if (!(werr < 1e-6) || !isfinite(sh[R_EIN][0] - Er) || !(sh[IDX_SDEV][0] <= (double)SURR_DRIFT_TOL))
+            ctl[C_DRIFT] = 1u;
+    }
     if (threadIdx.x == NV) dst[NV] = (double)((acc_c ? *nc : 0u) + (acc_f ? *nf : 0u));
     if (threadIdx.x == 0) {  // fine-phase updates (integers, exact in FP64 for any realistic count)
         double u = 0.0;

@@ -128,7 +128,7 @@ struct sm2d_ctx {
     // learned physics of the fine transport (sm2d_surrogate_attach)
     DBuf<float> surr;
     DBuf<int> rowmat;
-    int surr_nodes = 0, surr_nmat = 0, surr_on = 0, surr_drift = 0;
+    int surr_nodes = 0, surr_nmat = 0, surr_on = 0;
     float surr_tol = 0.25f;   // validation tolerance of an attached network's table
     float surr_stats[4] = {}; // last validation: nodes checked, flagged, max and RMS deviation
     std::vector<double> src;  // per spot: W in/out/slot, E in/out/slot, E representation loss
@@ -244,12 +244,13 @@ Tp make_tp(const sm2d_ctx* c, const sm2d_transport* t) {
     p.rule = t->step_rule;
     p.strag = t->enable_straggling;
     p.nuc = t->enable_nuclear;
-    p.surr = (c->surr_on && !c->surr_drift && c->surr.p) ? 1 : 0;
+    p.surr = (c->surr_on && c->surr.p) ? 1 : 0;
     return p;
 }
 
 __global__ void k_ctl_reset(uint32_t* ctl) {
     ctl[C_DONE] = 0;
+    ctl[C_DRIFT] = 0;
     ctl[C_IT] = 0;
     ctl[C_DONE_IT] = 0;
 }
@@ -426,7 +427,7 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
     if (!(open_next && compact_next)) L.occ = nullptr;
     k_audit_final<<<open_next ? 1 + gm : 1, 256, 0, c->stream>>>(
         c->part.p, c->nocc(in), c->nocc(out), c->acc_c.p, c->ctl.p + C_NCRS, c->acc_f.p, c->ctl.p + C_NACT, nullptr,
-        c->hist.p, HROWS, c->ctl.p, 0, pe, c->psi(in), c->occ[in].p, c->nocc(in), L);
+        c->hist.p, HROWS, c->ctl.p, 0, pe, c->psi(in), c->occ[in].p, c->nocc(in), L, P.surr);
     c->mark(11);
     c->in = out;  // K6
 }
@@ -437,7 +438,8 @@ int buffer_totals(sm2d_ctx* c, int k, double& W, double& E) {
     const double* pe = launch_audit_cells<LY>(c, k, nullptr, false, 1, (int)std::max<uint32_t>(c->hint_occ, 1024));
     ListOut none{};
     k_audit_final<<<1, 256, 0, c->stream>>>(c->part.p, nullptr, c->nocc(k), nullptr, nullptr, nullptr, nullptr,
-                                            c->k5v.p, nullptr, 1, c->ctl.p, 1, pe, Psi{}, nullptr, nullptr, none);
+                                            c->k5v.p, nullptr, 1, c->ctl.p, 1, pe, Psi{}, nullptr, nullptr, none,
+                                            0);
     HIPCHK(hipGetLastError());
     double v[NV];
     HIPCHK(hipMemcpyAsync(v, c->k5v.p, sizeof(v), hipMemcpyDeviceToHost, c->stream));
@@ -599,7 +601,6 @@ int launch_chunk_graph(sm2d_ctx* c, const Tp& P, int R, bool first_opens, bool f
 template <class LY>
 int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, sm2d_k5_report* last,
               sm2d_audit* au) {
-    c->surr_drift = 0;
     Tp P = make_tp(c, tp);
     if (au) au->surrogate_active = P.surr;
     k_ctl_reset<<<1, 1, 0, c->stream>>>(c->ctl.p);
@@ -687,12 +688,6 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
             sm2d_k5_report r;
             fill_report(v, Wso, Wss, Eso, Ess, (src_terms && it == 0) ? 1 : 0, &r);
             if (last) *last = r;
-            // drift of the learned physics: a failed weight audit or a non-finite energy sum
-            // switches the remaining iterations to the plain physics
-            if (P.surr && (!r.W_pass || !std::isfinite(r.E_error))) {
-                c->surr_drift = 1;
-                if (au) au->surrogate_fallback = 1;
-            }
             if (au) {
                 const double* a = v + NR1;
                 au->W_transport_drop += a[A_DROPW] + v[R_K4W];
@@ -742,7 +737,9 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
             if (all) c->tune.done[ph][bkt[ph]] = true;
         }
         audited = upto;
-        if (P.surr && c->surr_drift) P = make_tp(c, tp);
+        // K5 of an iteration that flagged the learned physics switched the following ones to the
+        // plain physics on the device (the control word C_DRIFT)
+        if (P.surr && ctl[C_DRIFT] && au) au->surrogate_fallback = 1;
         maxlist = std::max(ctl[C_NACT], ctl[C_NCRS]);
         c->hint = std::max<uint32_t>(64, 2 * maxlist);
         c->hint_ph[0] = std::max<uint32_t>(2, 2 * ctl[C_NCRS]);

@@ -154,16 +154,26 @@ def test_random_psi_iterations_bit_exact(hip, orc, layout):
     assert aug.cross_cell_emissions == auo.cross_cell_emissions
 
 
+def _step_rule_2pct(prob):
+    """The same problem with the fine transport's 2 % energy-loss step rule (step_rule = 1)."""
+    prob.transport = dict(prob.transport, step_rule=1)
+    return prob
+
+
 @pytest.mark.parametrize("prob", [
     problems.reference_grid_case(Nx=60, Nz=100, E0=70.0, sigma_x=3.0, n_samples=500),
     problems.config1(n_samples=500),
-], ids=["refgrid70_LB256", "config1_LB128"])
+    _step_rule_2pct(problems.reference_grid_case(Nx=60, Nz=100, E0=70.0, sigma_x=3.0, n_samples=500)),
+], ids=["refgrid70_LB256", "config1_LB128", "refgrid70_LB256_step_rule_2pct"])
 def test_full_spot_bit_exact(hip, orc, prob):
     g, o = _pair(hip, orc, prob)
     sp = prob.make_spot()
     tp = prob.make_transport()
     eg, ag = g.run_spot(sp, tp)
     eo, ao = o.run_spot(sp, tp)
+    if prob.transport.get("step_rule", 0) == 1:
+        assert ag.fine_cell_bin_updates > 0  # the rule acts in the fine transport
+    assert ag.fine_cell_bin_updates == ao.fine_cell_bin_updates
     assert ag.terminated == ao.terminated
     assert ag.iterations == ao.iterations
     assert np.array_equal(eg, eo)

@@ -181,26 +181,181 @@ def test_validation_rejects_a_perturbed_network(hip, trained, orc):
     bad.close()
 
 
-def test_learned_dose_against_plain(hip, trained):
-    """Integral agreement with the plain physics.  Voxel doses of this transport are not a smooth
-    function of the widths (each component takes one hashed draw into one bin), so they are
-    compared in 4 x 4 blocks; the bounds sit above the measured change of the plain path itself
-    under a 1 % change of the widths (block mean 5 %)."""
+def test_drift_monitor_switches_to_plain_physics(hip, trained, orc):
+    """Run-time monitor: a table whose scattering widths are e times the plain ones (ln sigma_theta
+    + 1 on every accepted node, injected after validation through sm2d_surrogate_set_table) trips
+    the K5 drift check in the first iteration whose monitored cells take a learned substep; the
+    following iterations run the plain physics on the device.  The oracle applies the same check,
+    so the run still equals it bit for bit; the weight balance holds throughout."""
+    from oracle.pyoracle import OracleEngine
+    prob, _, m, _ = trained
+    g = problems.setup_engine(hip, prob, Engine)
+    g.surrogate_attach(m, 64)
+    tab, nm, nn, rows = g.surrogate_export()
+    sp, tp = prob.make_spot(), prob.make_transport()
+    _, a_good = g.run_spot(sp, tp)
+    assert a_good.surrogate_fallback == 0 and a_good.surrogate_substeps > 0
+    skew = tab.reshape(-1, 3).copy()
+    skew[:, 1] += 1.0  # NaN (rejected) nodes stay NaN
+    g.surrogate_set_table(skew.reshape(tab.shape), nm, nn, rows)
+    eg, ag = g.run_spot(sp, tp)
+    assert ag.surrogate_active == 1 and ag.surrogate_fallback == 1
+    assert 0 < ag.surrogate_substeps < a_good.surrogate_substeps // 4
+    assert ag.max_cell_W_closure < 1e-6 and ag.W_fail_iters == 0
+    o = problems.setup_engine(orc, prob, OracleEngine)
+    o.surrogate_set(skew.reshape(tab.shape), nm, nn, rows)
+    eo, ao = o.run_spot(sp, tp)
+    assert ao.surrogate_fallback == 1
+    assert ag.iterations == ao.iterations
+    assert ag.surrogate_substeps == ao.surrogate_substeps
+    assert np.array_equal(eg, eo)
+    # the table switched off by the flag is back on for the next run
+    g.surrogate_set_table(tab, nm, nn, rows)
+    _, a2 = g.run_spot(sp, tp)
+    assert a2.surrogate_fallback == 0 and a2.surrogate_substeps == a_good.surrogate_substeps
+    g.close()
+    o.close()
+
+
+def _learned_spot_vs_oracle(hip, m, prob, i):
+    """One spot of `prob` with the learned table attached, on the GPU and on the oracle (its
+    -O3 build, which equals the default build bit for bit) reading the same table."""
+    from oracle.pyoracle import OracleEngine, load
+    g = problems.setup_engine(hip, prob, Engine, spot_index=i)
+    g.surrogate_attach(m, 64)
+    tab, nm, nn, rows = g.surrogate_export()
+    o = problems.setup_engine(load(fast=True), prob, OracleEngine, spot_index=i)
+    o.surrogate_set(tab, nm, nn, rows)
+    sp, tp = prob.make_spot(i), prob.make_transport()
+    eg, ag = g.run_spot(sp, tp)
+    eo, ao = o.run_spot(sp, tp)
+    g.close()
+    o.close()
+    return eg, ag, eo, ao
+
+
+def _same_learned(eg, ag, eo, ao):
+    # the learned physics acted: fine-transport updates and learned substeps on both sides
+    assert ag.fine_cell_bin_updates > 0 and ag.surrogate_substeps > 0
+    assert ag.surrogate_active == 1 and ag.surrogate_fallback == 0
+    assert ag.iterations == ao.iterations and ag.terminated == ao.terminated
+    assert ag.cell_bin_updates == ao.cell_bin_updates
+    assert ag.fine_cell_bin_updates == ao.fine_cell_bin_updates
+    assert ag.surrogate_substeps == ao.surrogate_substeps
+    assert np.array_equal(eg, eo)
+    assert ag.max_cell_W_closure < 1e-6 and ag.W_fail_iters == 0
+
+
+def test_learned_fine_grid_water_spots_bit_exact(hip, trained):
+    """The benchmark batch's spot distribution on the piecewise 1029-bin energy grid (water
+    200 x 600 at 0.5 mm, 650 iterations): the lowest-energy and the median-energy spot, where
+    the fine transport runs, with the learned physics, equal the oracle bit for bit."""
+    _, _, m, _ = trained
+    prob = problems.config4_fine_grid()
+    E0 = np.array([s["E0"] for s in prob.spots])
+    order = np.argsort(E0)
+    for i in (int(order[0]), int(order[len(order) // 2])):
+        print(prob.name, "spot", i, flush=True)
+        _same_learned(*_learned_spot_vs_oracle(hip, m, prob, i))
+
+
+def test_learned_fine_grid_slabs_spot_bit_exact(hip, trained):
+    """The same spots on the water / bone / lung slab phantom (400 x 1200 at 0.25 mm, piecewise
+    energy grid, 1250 iterations): the lowest-energy spot with the learned physics equals the
+    oracle bit for bit."""
+    _, _, m, _ = trained
+    prob = problems.config5(energy="piecewise")
+    E0 = np.array([s["E0"] for s in prob.spots])
+    i = int(E0.argmin())
+    print(prob.name, "spot", i, flush=True)
+    _same_learned(*_learned_spot_vs_oracle(hip, m, prob, i))
+
+
+def test_learned_physics_idle_on_the_log_grid(hip, trained):
+    """On the 256-bin log energy grid coarse steps re-centre every component in its energy bin,
+    so no component reaches the fine transport: 32 spots of the benchmark distribution on the
+    slab phantom run no fine update and no learned substep, and the dose with the learned table
+    attached equals the plain dose bit for bit."""
+    _, _, m, _ = trained
+    prob = problems.config5(n_spots=32, energy="log")
+    spots = [prob.make_spot(i) for i in range(32)]
+    tp = prob.make_transport()
+    b = problems.setup_engine(hip, prob, Engine, n_spots=32)
+    ep, _, ap = b.run_batch(spots, tp)
+    b.surrogate_attach(m, 64)
+    es, _, as_ = b.run_batch(spots, tp)
+    b.close()
+    assert ap.terminated == 1 and as_.terminated == 1
+    assert as_.surrogate_active == 1 and as_.surrogate_fallback == 0
+    assert ap.fine_cell_bin_updates == 0 and as_.fine_cell_bin_updates == 0
+    assert as_.surrogate_substeps == 0
+    assert np.array_equal(ep, es)
+
+
+def _plain_table(orc, prob, n_nodes=64):
+    """The plain physics of water at the table nodes (energy-bin centres x log-spaced rho ds),
+    from the oracle's routines, in the table layout: (ln sigma_E, ln sigma_theta, a)."""
+    from oracle.pyoracle import OracleEngine
+    o = problems.setup_engine(orc, prob, OracleEngine)
+    e = np.asarray(o.tables.E_edges, np.float32)
+    Ec = (e[:-1] + (e[1:] - e[:-1]) * np.float32(0.5)).astype(np.float32)
+    j = np.arange(n_nodes, dtype=np.float32)
+    lo, hi = np.log(np.float32(1e-4)), np.log(np.float32(10.0))
+    dR = np.exp(lo + (hi - lo) * j / np.float32(n_nodes - 1)).astype(np.float32)
+    E, ds = np.repeat(Ec, n_nodes), np.tile(dR, len(Ec))  # rho = 1
+    sE = o.eval(7, E, ds)
+    Em = np.maximum(E - 0.5 * (E - o.eval(5, E, ds)), e[0]).astype(np.float32)
+    sth = o.eval(6, Em, ds)
+    a = o.eval(8, E) * ds
+    o.close()
+    return np.stack([np.log(np.maximum(sE, 1e-30)), np.log(np.maximum(sth, 1e-30)), a], 1).astype(np.float32)
+
+
+def test_learned_dose_against_plain(hip, trained, orc):
+    """Dose of the learned physics against the plain physics, judged against the plain path's own
+    sensitivity.  A voxel dose of this transport is one realisation of hashed per-substep draws
+    (one draw per component and substep, seeded by cell, slot and bin), not an average: the
+    same plain physics with every width moved by a random 1 % (the plain widths tabulated at the
+    table nodes, ln sigma + N(0, 0.01)) changes the 4 x 4 block doses by about 6 % and the
+    depth-dose curve by about 6 % on this all-fine problem, so no learned model short of the
+    plain formulas themselves meets 1 % per voxel here.  The learned dose must stay within twice
+    that 1 %-perturbation response, and its integral dose within 1 %."""
     prob, _, m, _ = trained
     g = problems.setup_engine(hip, prob, Engine)
     sp, tp = prob.make_spot(), prob.make_transport()
     ep, ap = g.run_spot(sp, tp)
     g.surrogate_attach(m, 64)
+    _, nm, nn, rows = g.surrogate_export()
     es, as_ = g.run_spot(sp, tp)
     assert as_.surrogate_active == 1 and as_.surrogate_fallback == 0
+    assert as_.fine_cell_bin_updates > 0 and as_.surrogate_substeps > 0
     assert abs(es.sum() / ep.sum() - 1.0) < 0.01
+    plain = _plain_table(orc, prob, nn)
+    assert nm == 1 and plain.size == g.grid.N_E * nn * 3
+    rng = np.random.default_rng(7)
+    pert = plain.copy()
+    pert[:, :2] += rng.normal(0.0, 0.01, (len(pert), 2)).astype(np.float32)
+    g.surrogate_set_table(pert.reshape(-1), nm, nn, rows)
+    ef, af = g.run_spot(sp, tp)
+    assert af.surrogate_fallback == 0 and af.surrogate_substeps > 0
+    g.close()
 
     def blk(d, b=4):
         nz, nx = d.shape
         return d[: nz // b * b, : nx // b * b].reshape(nz // b, b, nx // b, b).sum((1, 3))
 
-    bp, bs = blk(ep), blk(es)
+    bp = blk(ep)
     mask = bp > 0.1 * bp.max()
-    rel = np.abs(bs - bp)[mask] / bp[mask]
-    assert rel.mean() < 0.12
-    assert as_.max_cell_W_closure < 1e-6
+    pdd0 = ep.sum(1)
+    pm = pdd0 > 0.1 * pdd0.max()
+
+    def diffs(e):
+        b = blk(e)
+        return (np.abs(b - bp)[mask] / bp[mask]).mean(), (np.abs(e.sum(1) - pdd0)[pm] / pdd0[pm]).mean()
+
+    bl, pl = diffs(es)
+    bf, pf = diffs(ef)
+    print(f"learned vs plain: block {bl:.4f} PDD {pl:.4f}; plain 1% perturbed: block {bf:.4f} PDD {pf:.4f}",
+          flush=True)
+    assert bl < 2.0 * bf and pl < 2.0 * pf
+    assert as_.max_cell_W_closure < 1e-6 and af.max_cell_W_closure < 1e-6
