@@ -283,6 +283,26 @@ int sm2d_run_iterations(sm2d_ctx* ctx, const sm2d_transport* tp, int n_iter,
 int sm2d_k5_audit(sm2d_ctx* ctx, const double* tallies_delta, double W_src_out, double W_src_slot,
                   double E_src_out, double E_src_slot, double W_drop, double E_drop,
                   int include_source_terms, sm2d_k5_report* out);
+/* K1 ActiveMask + K2 CompactActive on caller-provided device buffers, allocating nothing: no
+ * hipMalloc, hipFree or allocating library call, for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation).  No counterpart in
+ * the reference; it takes the arguments of the reference's stage wrapper run_k1_active_mask
+ * (k1k6_pipeline.cu:1261-1294) on its dense phase-space layout (DevicePsiC, device_psic.cuh:39-55):
+ * d_block_ids[Nx*Nz*32] (EMPTY = 0xFFFFFFFF), d_values[Nx*Nz*32*local_bins] (local_bins 128 or
+ * 256), d_mask_prev[Nx*Nz] (may be NULL), block thresholds b_E_fine_on / b_E_fine_off
+ * (compute_b_E_threshold, k1k6_pipeline.cuh:419-422) and weight_active_min.  Outputs:
+ * d_mask[Nx*Nz] (0/1), the active and coarse cell lists in ascending order (d_active, d_coarse:
+ * Nx*Nz entries each) and their lengths d_counts[2].  Work is queued on `stream` (a hipStream_t;
+ * NULL = the default stream) and the call returns without waiting.  The scratch is a fixed
+ * device array of the library (4.3 MB per device, held until the process ends): grids above
+ * 4 194 304 cells are rejected (SM2D_ERR_ARG).  Calls that overlap on one device race on it.
+ * Results equal those of sm2d_k1k2_mask_compact on the same phase space. */
+int sm2d_k1k2_mask_compact_noalloc(int Nx, int Nz, int local_bins, const uint32_t* d_block_ids,
+                                   const float* d_values, const uint8_t* d_mask_prev, uint8_t* d_mask,
+                                   int b_E_fine_on, int b_E_fine_off, float weight_active_min,
+                                   uint32_t* d_active, uint32_t* d_coarse, uint32_t* d_counts,
+                                   void* stream);
+
 /* Copies psi_in <-> psi_out (load the second buffer for sm2d_k5_audit). */
 int sm2d_swap(sm2d_ctx* ctx);
 

@@ -1049,6 +1049,7 @@ int upload_fe_for(sm2d_ctx* c, int s, const sm2d_spot& sp, std::vector<float>* s
 }  // namespace
 
 #include "sm2d_surrogate.hip"
+#include "sm2d_noalloc.hip"
 
 extern "C" {
 
