@@ -18,22 +18,33 @@
 
 namespace sm2d {
 
-// Diagnostic build (-DSM2D_STAGE_CLOCKS, python -m sm_2d_amd.build --stage-clocks): the first
-// thread of every transport workgroup adds the constant-rate clock ticks of each stage of a
-// cell to stage_clk[stage] (STAGES below; [15] counts cells).  The work is the same as in the
-// normal build; the normal build compiles the stamps away.
+// Diagnostic build (-DSM2D_STAGE_CLOCKS, python -m sm_2d_amd.build --stage-clocks): every
+// transport workgroup sums the constant-rate clock ticks of each stage of its cells in registers
+// and its first thread adds the sums to stage_clk[stage] when the workgroup ends (STAGES below;
+// [15] counts cells), so the stamps add no memory traffic inside the stages.  The work is the
+// same as in the normal build; the normal build compiles the stamps away.
 #ifdef SM2D_STAGE_CLOCKS
 __device__ unsigned long long stage_clk[16];
-#define SM2D_STAMP(k)                                                   \
-    do {                                                                \
-        if (threadIdx.x == 0) {                                         \
-            const unsigned long long t_ = wall_clock64();               \
-            atomicAdd(&stage_clk[k], t_ - clk_prev);                    \
-            clk_prev = t_;                                              \
-        }                                                               \
+#define SM2D_STAMP(k)                                     \
+    do {                                                  \
+        const unsigned long long t_ = wall_clock64();     \
+        clk_acc[k] += t_ - clk_prev;                      \
+        clk_prev = t_;                                    \
     } while (0)
-#define SM2D_STAMP_START() unsigned long long clk_prev = wall_clock64()
+#define SM2D_STAMP_START()                                \
+    unsigned long long clk_prev = wall_clock64();         \
+    unsigned long long clk_acc[11] = {};                  \
+    unsigned long long clk_cells = 0
 #define SM2D_STAMP_RESTART() clk_prev = wall_clock64()
+#define SM2D_STAMP_CELL() ++clk_cells
+#define SM2D_STAMP_FLUSH()                                \
+    do {                                                  \
+        if (threadIdx.x == 0) {                           \
+            for (int k_ = 0; k_ < 11; ++k_)               \
+                atomicAdd(&stage_clk[k_], clk_acc[k_]);   \
+            atomicAdd(&stage_clk[15], clk_cells);         \
+        }                                                 \
+    } while (0)
 #else
 #define SM2D_STAMP(k) \
     do {              \
@@ -43,6 +54,12 @@ __device__ unsigned long long stage_clk[16];
     } while (0)
 #define SM2D_STAMP_RESTART() \
     do {                     \
+    } while (0)
+#define SM2D_STAMP_CELL() \
+    do {                  \
+    } while (0)
+#define SM2D_STAMP_FLUSH() \
+    do {                   \
     } while (0)
 #endif
 // stages: 0 tags and destination marks, 1 own psi_out staging, 2 count and queue of non-zero
@@ -1294,8 +1311,8 @@ __device__ inline bool fast_claims(FastLds<LY>& F, WTabs<LY>& W) {
     return true;
 }
 
-// Size of the per-workgroup owner table of the record rounds (hashed payload addresses).  Kept
-// at 2 KiB so that two 2-wave transport workgroups still fit one CU's LDS.
+// Size of the per-workgroup owner table of the record rounds (hashed payload addresses), 2 KiB:
+// addresses that share a slot only wait a round.
 constexpr int NOWN = 512;
 
 __device__ inline uint32_t own_slot(uint32_t addr) { return (addr * 2654435761u) >> 23; }
@@ -2079,7 +2096,7 @@ __device__ void apply_foreign(const Tabs& T, const Tp& P, CellGeo G, const Psi& 
 // TW waves work on one listed cell: they evaluate the physics of TW batches at once, wave 0
 // applies them in order.  TW is a launch choice (1, 2 or 4) made by the runtime's tuner; all
 // choices give identical results.
-constexpr int QCAP = 512;
+constexpr int QCAP = 512;  // non-zero components queued per round (at least one slot's worth)
 
 // Destination-set output of a transport phase.
 struct Dest {
@@ -2111,8 +2128,10 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
     if (ctl[C_DONE] || blockIdx.x >= *nlist) return;  // no cell for this workgroup
     // LDS payload rows: 4-wave workgroups (few listed cells, one workgroup per CU) keep the full
     // set (24 KiB); 2-wave ones 12 KiB and 1-wave ones 14 KiB, so that three (four) of them fit
-    // a CU with the 256-bin energy tables
+    // a CU with the 256-bin energy tables.  (Five 1-wave workgroups with 19 rows each were
+    // measured slower: more cells overflow the rows and take the serial apply.)
     constexpr int NLT = TW >= 4 ? LY::NL : TW == 2 ? LY::NL / 2 : (LY::NL * 7) / 12;
+    static_assert(QCAP >= LY::LB, "a slot's non-zero components must fit one round's queue");
     __shared__ float s_rows[NLT * LY::LB];
     __shared__ FastLds<LY> s_fast[TW];
     __shared__ uint32_t s_q[QCAP];
@@ -2160,10 +2179,38 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
     G.tmin = T.te[0];
     G.dth = (T.te[T.NT] - G.tmin) / T.NT;
     G.R_on = lookup_R(T, fmaxf(P.E_on, T.ee[0]));
+    // The slot headers of a workgroup's next cell (psi_in and psi_out block ids, payload
+    // references and slot counts; lanes < 32 of each wave) are fetched while the current cell is
+    // transported.  Only this cell's workgroup writes its psi_out during the launch.
+    uint32_t nx_cell = blockIdx.x < n ? list[blockIdx.x] : 0u;
+    uint32_t nx_ns = 0, nx_bid = EMPTY, nx_ref = 0, nx_ons = 0, nx_obid = EMPTY, nx_oref = 0;
+    if (blockIdx.x < n) {
+        nx_ns = in.ns[nx_cell];
+        nx_ons = out.ns[nx_cell];
+        if (l < KB) {
+            nx_bid = in.bid[(size_t)nx_cell * KB + l];
+            nx_ref = in.sref[(size_t)nx_cell * KB + l];
+            nx_obid = out.bid[(size_t)nx_cell * KB + l];
+            nx_oref = out.sref[(size_t)nx_cell * KB + l];
+        }
+    }
     SM2D_STAMP_START();
     for (uint32_t li = blockIdx.x; li < n; li += gridDim.x) {
         SM2D_STAMP_RESTART();
-        const int cell = (int)list[li];
+        const int cell = (int)nx_cell;
+        const uint32_t ns = nx_ns, c_bid = nx_bid, c_ref = nx_ref, c_ons = nx_ons, c_obid = nx_obid,
+                       c_oref = nx_oref;
+        if (li + gridDim.x < n) {
+            nx_cell = list[li + gridDim.x];
+            nx_ns = in.ns[nx_cell];
+            nx_ons = out.ns[nx_cell];
+            if (l < KB) {
+                nx_bid = in.bid[(size_t)nx_cell * KB + l];
+                nx_ref = in.sref[(size_t)nx_cell * KB + l];
+                nx_obid = out.bid[(size_t)nx_cell * KB + l];
+                nx_oref = out.sref[(size_t)nx_cell * KB + l];
+            }
+        }
         set_cell_geo(G, T, cell, ltag, tag);
         if (wv == 0 && l < 4) {  // the cell's neighbours join the phase's destination set
             const int d = l == 0   ? (G.iz + 1 < T.Nz ? cell + G.Nx : -1)
@@ -2172,14 +2219,13 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                                    : (G.ix > 0 ? cell - 1 : -1);
             if (d >= 0) dest_mark(D, (uint32_t)d, tag, flags);
         }
-        const uint32_t ns = in.ns[cell];
         if (threadIdx.x == 0) {
             s_surr = 0;
             s_sdev = 0;
         }
         if (threadIdx.x < ns) {
-            s_bid[threadIdx.x] = in.bid[(size_t)cell * KB + threadIdx.x];
-            s_ref[threadIdx.x] = in.sref[(size_t)cell * KB + threadIdx.x];
+            s_bid[threadIdx.x] = c_bid;
+            s_ref[threadIdx.x] = c_ref;
         }
         SM2D_STAMP(0);
         // wave 0 owns the tables and the accumulators of the cell
@@ -2188,10 +2234,10 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
         double accd = 0.0;
         float accf = 0.0f;
         if (wv == 0) {
-            W.n0 = out.ns[cell];
+            W.n0 = c_ons;
             W.own0 = W.n0;
-            W.b0 = (l < 32 && (uint32_t)l < W.n0) ? out.bid[(size_t)cell * KB + l] : EMPTY;
-            W.r0 = (l < 32 && (uint32_t)l < W.n0) ? out.sref[(size_t)cell * KB + l] : 0u;
+            W.b0 = (l < KB && (uint32_t)l < W.n0) ? c_obid : EMPTY;
+            W.r0 = (l < KB && (uint32_t)l < W.n0) ? c_oref : 0u;
             W.b1 = W.b2 = W.b3 = W.b4 = EMPTY;
             W.r1 = W.r2 = W.r3 = W.r4 = 0u;
             W.n1 = W.n2 = W.n3 = W.n4 = 0u;
@@ -2205,9 +2251,35 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
         SM2D_STAMP(1);
         uint32_t s0 = 0;
         while (s0 < ns) {
-            // non-zero components per slot (the payloads of SU slots per wave are fetched at once)
+            // non-zero components per slot (the payloads of SU slots per wave are fetched at once);
+            // when the remaining slots fit SR per wave, their payloads are fetched once, kept in
+            // registers and queued from there
             constexpr int SU = LY::PL >= 8 ? 1 : 8 / LY::PL;
-            for (uint32_t s = s0 + wv; s < ns; s += SU * TW) {
+            constexpr int SR = LY::PL >= 32 ? 1 : 32 / LY::PL;
+            const bool one = ns - s0 <= (uint32_t)(SR * TW);
+            float vr[SR][LY::PL];
+            if (one) {
+#pragma unroll
+                for (int u = 0; u < SR; ++u) {
+                    const uint32_t su = s0 + (uint32_t)wv + (uint32_t)(u * TW);
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k) vr[u][k] = 0.0f;
+                    if (su < ns && s_bid[su] != EMPTY) {
+                        const float* vin = in.pool + (size_t)s_ref[su] * LY::LB;
+#pragma unroll
+                        for (int k = 0; k < LY::PL; ++k) vr[u][k] = vin[l + 64 * k];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < SR; ++u) {
+                    const uint32_t su = s0 + (uint32_t)wv + (uint32_t)(u * TW);
+                    uint32_t cnt = 0;
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k) cnt += (uint32_t)__popcll(__ballot(vr[u][k] > 0.0f));
+                    if (l == 0 && su < ns) s_cnt[su] = cnt;
+                }
+            }
+            for (uint32_t s = s0 + wv; !one && s < ns; s += SU * TW) {
                 float v[SU][LY::PL];
 #pragma unroll
                 for (int u = 0; u < SU; ++u) {
@@ -2242,7 +2314,27 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
             }
             __syncthreads();
             const uint32_t s1 = s_round[0], total = s_round[1];
-            for (uint32_t s = s0 + wv; s < s1; s += SU * TW) {
+            if (one) {
+#pragma unroll
+                for (int u = 0; u < SR; ++u) {
+                    const uint32_t su = s0 + (uint32_t)wv + (uint32_t)(u * TW);
+                    if (su < s1) {
+                        uint32_t pos = s_off[su];
+#pragma unroll
+                        for (int k = 0; k < LY::PL; ++k) {
+                            const float w = vr[u][k];
+                            const uint64_t mk = __ballot(w > 0.0f);
+                            if (w > 0.0f) {
+                                const uint32_t p = pos + (uint32_t)__popcll(mk & below);
+                                s_q[p] = (su << 16) | (uint32_t)(l + 64 * k);
+                                s_qw[p] = w;
+                            }
+                            pos += (uint32_t)__popcll(mk);
+                        }
+                    }
+                }
+            }
+            for (uint32_t s = s0 + wv; !one && s < s1; s += SU * TW) {
                 float v[SU][LY::PL];
 #pragma unroll
                 for (int u = 0; u < SU; ++u) {
@@ -2358,14 +2450,17 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
             SM2D_STAMP(8);
             flush_tables<LY>(W, out, bk, cell, li, flags);
             SM2D_STAMP(9);
+            // the seven cumulative tallies, one lane each (four FP64, three FP32 arrays)
+            if (l < 4) {
+                double* t = l == 0 ? tal.edep : l == 1 ? tal.E_cut : l == 2 ? tal.E_nuc : tal.E_bnd;
+                const double v = l == 0 ? a.edep : l == 1 ? a.E_cut : l == 2 ? a.E_nuc : a.E_bnd;
+                t[cell] += v;
+            } else if (l < 7) {
+                float* t = l == 4 ? tal.w_cut : l == 5 ? tal.w_nuc : tal.w_bnd;
+                const float v = l == 4 ? a.w_cut : l == 5 ? a.w_nuc : a.w_bnd;
+                t[cell] += v;
+            }
             if (l == 0) {
-                tal.edep[cell] += a.edep;
-                tal.w_cut[cell] += a.w_cut;
-                tal.E_cut[cell] += a.E_cut;
-                tal.w_nuc[cell] += a.w_nuc;
-                tal.E_nuc[cell] += a.E_nuc;
-                tal.w_bnd[cell] += a.w_bnd;
-                tal.E_bnd[cell] += a.E_bnd;
                 double* o = acc + (size_t)li * NACC;
                 o[A_EDEP] = a.edep;
                 o[A_WCUT] = a.w_cut;
@@ -2386,12 +2481,11 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                 o[A_SDEV] = (double)__uint_as_float(s_sdev);
             }
             SM2D_STAMP(10);
-#ifdef SM2D_STAGE_CLOCKS
-            if (threadIdx.x == 0) atomicAdd(&stage_clk[15], 1ull);
-#endif
+            SM2D_STAMP_CELL();
         }
         __syncthreads();
     }
+    SM2D_STAMP_FLUSH();
 }
 
 // The phase's lateral emissions into other listed cells' +z buckets (apply_foreign), after the
