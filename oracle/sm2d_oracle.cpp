@@ -12,23 +12,33 @@
 // reference's host code does.  Built with -DORC_FP64, the transport step runs in binary64
 // instead (the FP32 results' error is stated against that build).
 //
+// Summation order where the reference's is a thread race (its kernels add with device atomics):
+// payload adds per destination in component order; per-cell accumulators (tallies, drops,
+// closure sums) one batch sum at a time, each batch of 64 components summed by a fixed
+// butterfly (Batch below).  Both are the library's orders, fixed by construction and checked by
+// the GPU parity tests; no reference fixture can pin them.
+//
 // Pinning (what checks each part; tests/ names the test):
 //  - grids, R-LUT: bit-exact against the reference's own grids.cpp / r_lut.cpp compiled
 //    from source into oracle/_ref (test_oracle_ref.py) and against the NIST nodes.
 //  - encodings: reference KATs of tests/unit/test_local_bins.cpp, test_block_encoding.cpp.
-//  - K1: truth table of tests/kernels/test_k1_activemask.cpp.
-//  - K5: closure KAT of tests/gpu/test_k5_drop_channels_gpu.cu:120-211.
+//  - K1: truth table of tests/kernels/test_k1_activemask.cpp (test_oracle_kat.py).
+//  - K5: closure KAT of tests/gpu/test_k5_drop_channels_gpu.cu:120-211 (test_oracle_kat.py).
 //  - injection + one full iteration (K1, compaction, K2 or K3, K4, K5, K6): the committed
-//    one-step dumps results/one_step_case_{A,B,C,D} (ψ before/after, channel totals).
+//    one-step dumps results/one_step_case_{A,B,C,D} (test_oracle_golden.py).
+//  - theta split and angular draws: ten one-iteration comparisons with the reference's
+//    multi-iteration dumps, the split law and the N(0,1) draw distribution (test_oracle_golden.py).
 //  - physics helpers: closed forms (Highland, nuclear survival, Vavilov regimes, CSDA range
-//    inversion, Gaussian CDF) in test_oracle_physics.py.
+//    inversion, the device step rule) in test_oracle_kat.py; the 2 % step rule against the
+//    reference's host rule compiled into oracle/_ref (test_oracle_ref.py).
 //  - FE precompute: closed-form recurrence checks (A increments = T dz, B, sigma_total^2 =
 //    sigma0^2 + C) and reference log-LUT lookups from oracle/_ref.
-//  - post-convolution: normalised-kernel invariants (row sums preserved away from edges,
-//    symmetry) — parity unpinned beyond these (no reference fixture holds convolved rows).
+//  - post-convolution: the impulse response of every row against the discrete Gaussian of
+//    sigma_MCS = sqrt(sigma_total^2 - sigma_x0^2) built independently, row sums away from the
+//    edges (test_post_convolve.py).
 //  - K4 slot-saturation fallback (closest-bid remap) and lateral emission into another
-//    listed cell's bucket: pinned only by brute force on tiny hand-built cases; the
-//    reference's own order there is a thread race.
+//    listed cell's bucket: brute force on small hand-built cases (test_oracle_cross_cell.py);
+//    the reference's own order there is a thread race.
 #include "sm2d_oracle.h"
 #include "../sm_2d_amd/csrc/sm2d_math.h"
 
@@ -577,6 +587,55 @@ struct CellAcc {
     float sdev = 0;  // largest learned-vs-plain deviation of the cell's substeps (monitored cells)
 };
 
+// The library applies a cell's components in rounds of whole slots (up to BATCH_ROUND
+// components) and batches of 64 consecutive components of a round; each batch adds to every
+// accumulator the sum of its components' contributions, formed by a fixed butterfly over the 64
+// positions (x[i] + x[i ^ o] for o = 32, 16, ..., 1; absent components contribute zero).
+constexpr int BATCH_ROUND = 512;
+
+template <class T>
+T butterfly64(const T* v, int n) {
+    T x[64], y[64];
+    for (int i = 0; i < 64; ++i) x[i] = i < n ? v[i] : T(0);
+    for (int o = 32; o > 0; o >>= 1) {
+        for (int i = 0; i < 64; ++i) y[i] = x[i] + x[i ^ o];
+        for (int i = 0; i < 64; ++i) x[i] = y[i];
+    }
+    return x[0];
+}
+
+struct Batch {
+    CellAcc v[64];
+    int n = 0;
+    CellAcc& next() {
+        v[n] = CellAcc();
+        return v[n++];
+    }
+    template <class T>
+    T sum(T CellAcc::*f) const {
+        T x[64];
+        for (int i = 0; i < n; ++i) x[i] = v[i].*f;
+        return butterfly64(x, n);
+    }
+    void flush(CellAcc& a) {
+        if (!n) return;
+        a.edep += sum(&CellAcc::edep);
+        a.E_cut += sum(&CellAcc::E_cut);
+        a.E_nuc += sum(&CellAcc::E_nuc);
+        a.E_bnd += sum(&CellAcc::E_bnd);
+        a.w_cut += sum(&CellAcc::w_cut);
+        a.w_nuc += sum(&CellAcc::w_nuc);
+        a.w_bnd += sum(&CellAcc::w_bnd);
+        a.drop_w += sum(&CellAcc::drop_w);
+        a.drop_E += sum(&CellAcc::drop_E);
+        a.cl_w_in += sum(&CellAcc::cl_w_in);
+        a.cl_w_out += sum(&CellAcc::cl_w_out);
+        a.cl_E_in += sum(&CellAcc::cl_E_in);
+        a.cl_E_out += sum(&CellAcc::cl_E_out);
+        n = 0;
+    }
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------
@@ -778,14 +837,25 @@ struct Phase {
         const float half_dz = dz * 0.5f, half_dx = dx * 0.5f;
         int ns = pi.nslot[cell];
         a.updates += (uint64_t)ns * c->LB;
+        Batch B;       // the accumulator contributions of the current batch, one per component
+        int rpos = 0;  // the component's position in its round
         for (int slot = 0; slot < ns; ++slot) {
             uint32_t bid = pi.bid[(size_t)cell * KB + slot];
             if (bid == EMPTY) continue;
             uint32_t bT = bid & 0xFFF, bE = (bid >> 12) & 0xFFF;
             const float* vin = pi.v(cell, slot);
+            int cnt = 0;
+            for (int lidx = 0; lidx < c->LB; ++lidx) cnt += vin[lidx] > 0.0f ? 1 : 0;
+            if (rpos + cnt > BATCH_ROUND) {  // the slot opens a new round
+                B.flush(a);
+                rpos = 0;
+            }
             for (int lidx = 0; lidx < c->LB; ++lidx) {
                 float weight = vin[lidx];
-                if (weight <= 0.0f) continue;
+                if (!(weight > 0.0f)) continue;
+                if (rpos % 64 == 0) B.flush(a);
+                ++rpos;
+                CellAcc& cc = B.next();
                 a.comps++;
                 int tl, el, xsub, zsub;
                 c->dec(lidx, tl, el, xsub, zsub);
@@ -795,32 +865,32 @@ struct Phase {
                     tbin = std::max(0, std::min(tbin, NT - 1));
                     ebin = std::max(0, std::min(ebin, NE - 1));
                 }
-                a.cl_w_in += weight;
+                cc.cl_w_in += weight;
                 float tmin = te[0], tmax = te[NT];
                 float dth = (tmax - tmin) / NT;
                 float E = c->ecenter(ebin);
-                a.cl_E_in += (double)weight * (double)E;
+                cc.cl_E_in += (double)weight * (double)E;
                 if (fine) {
                     if (weight < 1e-15f) {  // k3_finetransport.cu:175-184
-                        a.prune_w += (double)weight;
-                        a.prune_E += (double)(E * weight);
-                        a.cl_w_out += weight;
-                        a.cl_E_out += (double)(E * weight);
+                        cc.drop_w += (double)weight;
+                        cc.drop_E += (double)(E * weight);
+                        cc.cl_w_out += weight;
+                        cc.cl_E_out += (double)(E * weight);
                         continue;
                     }
                 } else if (weight < 1e-12f) {  // k2_coarsetransport.cu:172-177
-                    a.prune_w += (double)weight;
-                    a.prune_E += (double)(E * weight);
-                    a.cl_w_out += weight;
-                    a.cl_E_out += (double)(E * weight);
+                    cc.drop_w += (double)weight;
+                    cc.drop_E += (double)(E * weight);
+                    cc.cl_w_out += weight;
+                    cc.cl_E_out += (double)(E * weight);
                     continue;
                 }
                 float theta = fine ? te[tbin] + 0.5f * dth : tmin + (tbin + 0.5f) * dth;
                 if (E <= 0.1f) {
-                    a.w_cut += weight;
-                    a.E_cut += E * weight;
-                    a.cl_w_out += weight;
-                    a.cl_E_out += (double)(E * weight);
+                    cc.w_cut += weight;
+                    cc.E_cut += E * weight;
+                    cc.cl_w_out += weight;
+                    cc.cl_E_out += (double)(E * weight);
                     continue;
                 }
                 float xc = c->xoff(xsub), zc = c->zoff(zsub);
@@ -943,49 +1013,49 @@ struct Phase {
                 x_new = fmaxf(-half_dx, fminf(x_new, half_dx));
                 z_new = fmaxf(-half_dz, fminf(z_new, half_dz));
 
-                a.edep += edep;
-                a.w_nuc += w_rem;
-                a.E_nuc += E_rem;
-                a.cl_w_out += w_rem;
-                a.cl_E_out += (double)edep + (double)E_rem;
+                cc.edep += edep;
+                cc.w_nuc += w_rem;
+                cc.E_nuc += E_rem;
+                cc.cl_w_out += w_rem;
+                cc.cl_E_out += (double)edep + (double)E_rem;
                 if (E_new <= 0.1f) {
-                    a.w_cut += w_new;
-                    a.E_cut += E_new * w_new;
-                    a.cl_w_out += w_new;
-                    a.cl_E_out += (double)(E_new * w_new);
+                    cc.w_cut += w_new;
+                    cc.E_cut += E_new * w_new;
+                    cc.cl_w_out += w_new;
+                    cc.cl_E_out += (double)(E_new * w_new);
                     continue;
                 }
                 if (face >= 0) {
                     int nb = neighbor(cell, face);
                     if (nb < 0) {
-                        a.w_bnd += w_new;
-                        a.E_bnd += E_new * w_new;
-                        a.cl_w_out += w_new;
-                        a.cl_E_out += (double)(E_new * w_new);
+                        cc.w_bnd += w_new;
+                        cc.E_bnd += E_new * w_new;
+                        cc.cl_w_out += w_new;
+                        cc.cl_E_out += (double)(E_new * w_new);
                         continue;
                     }
                     float E_emit = c->ecenter(c->find_bin(E_new));
                     if (face == ZP) {
                         float lat_bnd = 0.0f;
                         float drop = emit_lateral(li, cell, iz + 1, th_new, E_new, w_new, x_new,
-                                                  sigma_x_tr, &lat_bnd, a);
+                                                  sigma_x_tr, &lat_bnd, cc);
                         if (lat_bnd > 0.0f) {
-                            a.w_bnd += lat_bnd;
-                            a.E_bnd += E_new * lat_bnd;
-                            a.cl_w_out += lat_bnd;
-                            a.cl_E_out += (double)(E_new * lat_bnd);
+                            cc.w_bnd += lat_bnd;
+                            cc.E_bnd += E_new * lat_bnd;
+                            cc.cl_w_out += lat_bnd;
+                            cc.cl_E_out += (double)(E_new * lat_bnd);
                         }
                         if (drop > 0.0f) {
-                            a.drop_w += (double)drop;
-                            a.drop_E += (double)(E_new * drop);
-                            a.cl_w_out += drop;
-                            a.cl_E_out += (double)(E_new * drop);
+                            cc.drop_w += (double)drop;
+                            cc.drop_E += (double)(E_new * drop);
+                            cc.cl_w_out += drop;
+                            cc.cl_E_out += (double)(E_new * drop);
                         }
                         if (fine) {
                             float tw = fmaxf(0.0f, w_new - lat_bnd - drop);
                             if (tw > 0.0f) {
-                                a.edep += (double)((E_new - E_emit) * tw);
-                                a.cl_E_out += (double)((E_new - E_emit) * tw);
+                                cc.edep += (double)((E_new - E_emit) * tw);
+                                cc.cl_E_out += (double)((E_new - E_emit) * tw);
                             }
                         }
                     } else {
@@ -998,18 +1068,18 @@ struct Phase {
                         if (face == ZP) zsn = 0;
                         else if (face == ZM) zsn = c->Zs - 1;
                         Bucket* bk = bucket(cell, face);
-                        float drop = emit4d(bk, th_new, E_new, w_new, xsn, zsn, a);
+                        float drop = emit4d(bk, th_new, E_new, w_new, xsn, zsn, cc);
                         if (drop > 0.0f) {
-                            a.drop_w += (double)drop;
-                            a.drop_E += (double)(E_new * drop);
-                            a.cl_w_out += drop;
-                            a.cl_E_out += (double)(E_new * drop);
+                            cc.drop_w += (double)drop;
+                            cc.drop_E += (double)(E_new * drop);
+                            cc.cl_w_out += drop;
+                            cc.cl_E_out += (double)(E_new * drop);
                         }
                         if (fine) {
                             float tw = fmaxf(0.0f, w_new - drop);
                             if (tw > 0.0f) {
-                                a.edep += (double)((E_new - E_emit) * tw);
-                                a.cl_E_out += (double)((E_new - E_emit) * tw);
+                                cc.edep += (double)((E_new - E_emit) * tw);
+                                cc.cl_E_out += (double)((E_new - E_emit) * tw);
                             }
                         }
                     }
@@ -1025,10 +1095,10 @@ struct Phase {
                 int os = find_or_alloc(po.b(cell), po.nslot[cell], bid_new, sel,
                                        [&](int, uint32_t w) { return po.claim(cell, w); });
                 if (os < 0) {
-                    a.drop_w += (double)w_new;
-                    a.drop_E += (double)(E_new * w_new);
-                    a.cl_w_out += w_new;
-                    a.cl_E_out += (double)(E_new * w_new);
+                    cc.drop_w += (double)w_new;
+                    cc.drop_E += (double)(E_new * w_new);
+                    cc.cl_w_out += w_new;
+                    cc.cl_E_out += (double)(E_new * w_new);
                     continue;
                 }
                 int tbe = tbn, ebe = ebn;
@@ -1061,8 +1131,8 @@ struct Phase {
                     if (wf <= 0.0f) continue;
                     float wsp = w_new * wf;
                     vo[c->enc(tln, eln, k, zst)] += wsp;
-                    a.cl_w_out += wsp;
-                    a.cl_E_out += (double)wsp * (double)c->ecenter(ebe);
+                    cc.cl_w_out += wsp;
+                    cc.cl_E_out += (double)wsp * (double)c->ecenter(ebe);
                 }
                 bool aborted = false;
                 if (wcf < 1.0f - 1e-6f) {
@@ -1079,36 +1149,36 @@ struct Phase {
                         float wsl = w_new * wl;
                         if (ix > 0) {
                             Bucket* bk = bucket(cell, XM);
-                            float d = emit4d(bk, th_new, E_new, wsl, XS - 1, c->zbin(z_new), a);
+                            float d = emit4d(bk, th_new, E_new, wsl, XS - 1, c->zbin(z_new), cc);
                             if (d > 0.0f) {
-                                a.drop_w += (double)d;
-                                a.drop_E += (double)(E_new * d);
-                                a.cl_w_out += d;
-                                a.cl_E_out += (double)(E_new * d);
+                                cc.drop_w += (double)d;
+                                cc.drop_E += (double)(E_new * d);
+                                cc.cl_w_out += d;
+                                cc.cl_E_out += (double)(E_new * d);
                             }
                         } else {
-                            a.w_bnd += wsl;
-                            a.E_bnd += E_new * wsl;
-                            a.cl_w_out += wsl;
-                            a.cl_E_out += (double)(E_new * wsl);
+                            cc.w_bnd += wsl;
+                            cc.E_bnd += E_new * wsl;
+                            cc.cl_w_out += wsl;
+                            cc.cl_E_out += (double)(E_new * wsl);
                         }
                     }
                     if (wr > 0.0f) {
                         float wsr = w_new * wr;
                         if (ix < Nx - 1) {
                             Bucket* bk = bucket(cell, XP);
-                            float d = emit4d(bk, th_new, E_new, wsr, 0, c->zbin(z_new), a);
+                            float d = emit4d(bk, th_new, E_new, wsr, 0, c->zbin(z_new), cc);
                             if (d > 0.0f) {
-                                a.drop_w += (double)d;
-                                a.drop_E += (double)(E_new * d);
-                                a.cl_w_out += d;
-                                a.cl_E_out += (double)(E_new * d);
+                                cc.drop_w += (double)d;
+                                cc.drop_E += (double)(E_new * d);
+                                cc.cl_w_out += d;
+                                cc.cl_E_out += (double)(E_new * d);
                             }
                         } else {
-                            a.w_bnd += wsr;
-                            a.E_bnd += E_new * wsr;
-                            a.cl_w_out += wsr;
-                            a.cl_E_out += (double)(E_new * wsr);
+                            cc.w_bnd += wsr;
+                            cc.E_bnd += E_new * wsr;
+                            cc.cl_w_out += wsr;
+                            cc.cl_E_out += (double)(E_new * wsr);
                         }
                     }
                 }
@@ -1120,12 +1190,13 @@ struct Phase {
                     rf = fminf(1.0f, fmaxf(0.0f, rf));
                     float rw = w_new * rf;
                     if (rw > 0.0f) {
-                        a.edep += (double)((E_new - E_emit) * rw);
-                        a.cl_E_out += (double)((E_new - E_emit) * rw);
+                        cc.edep += (double)((E_new - E_emit) * rw);
+                        cc.cl_E_out += (double)((E_new - E_emit) * rw);
                     }
                 }
             }
         }
+        B.flush(a);
     }
     static float sigma_x_tr_or_x(bool fine, float tr, float sx) { return fine ? tr : sx; }
     int neighbor(int cell, int face) {  // device_get_neighbor (device_bucket.cuh:810-829)

@@ -33,14 +33,14 @@ __device__ unsigned long long stage_clk[16];
     } while (0)
 #define SM2D_STAMP_START()                                \
     unsigned long long clk_prev = wall_clock64();         \
-    unsigned long long clk_acc[11] = {};                  \
+    unsigned long long clk_acc[12] = {};                  \
     unsigned long long clk_cells = 0
 #define SM2D_STAMP_RESTART() clk_prev = wall_clock64()
 #define SM2D_STAMP_CELL() ++clk_cells
 #define SM2D_STAMP_FLUSH()                                \
     do {                                                  \
         if (threadIdx.x == 0) {                           \
-            for (int k_ = 0; k_ < 11; ++k_)               \
+            for (int k_ = 0; k_ < 12; ++k_)               \
                 atomicAdd(&stage_clk[k_], clk_acc[k_]);   \
             atomicAdd(&stage_clk[15], clk_cells);         \
         }                                                 \
@@ -64,8 +64,8 @@ __device__ unsigned long long stage_clk[16];
 #endif
 // stages: 0 tags and destination marks, 1 own psi_out staging, 2 count and queue of non-zero
 // bins, 3 physics and per-lane preparation (wave 0's batches), 4 wait for the other waves'
-// physics, 5 claims, 6 ordered adds and accumulators, 7 serial apply, 8 end-of-round barrier,
-// 9 table flush, 10 tallies.
+// physics, 5 claims, 6 ordered adds, 7 serial apply, 8 end-of-round barrier, 9 table flush,
+// 10 tallies, 11 accumulators.
 
 // Learned straggling width, scattering width and attenuation exponent of a fine-transport
 // substep from the surrogate table (linear in ln dR between the two nearest nodes); false when
@@ -1007,21 +1007,20 @@ __device__ inline void flush_tables(WTabs<LY>& W, Psi& out, Bkt& bk, int cell, u
 //  - claim keys (table << 24 | block id) in the serial claim order,
 //  - payload records (key, local bin, weight) whose destinations are distinct within the
 //    component, and
-//  - per-accumulator contributions in the serial order (two per accumulator, the second 0 when
-//    unused; adding +0.0 leaves every accumulator unchanged).
+//  - per-accumulator contributions (its sums from zero, in the serial order).
 // The wave then claims the distinct keys in order of first appearance (= the serial claim
-// order) and walks the components in bin order: per component one LDS read-modify-write round
-// applies all its records, and lanes 0..12 each add the component's contributions to the
-// accumulator they hold.  Every FP32 payload sum and every tally is therefore formed in the
-// serial order.  A chunk whose claims need the closest-block fallback, a global payload or an
-// exhausted pool is applied by the serial code instead.
+// order) and applies the records in rounds in which every destination takes its next record in
+// component order, so every FP32 payload sum is formed in the serial order.  The accumulators
+// (tallies, drops, closure sums) take one sum per batch of 64 components: a fixed butterfly
+// over the batch's lanes (FP64, FP32 for the three FP32 accumulators), added in batch order.
+// A batch whose claims need the closest-block fallback, a global payload or an exhausted pool
+// is applied by the serial code instead, with the same batch sums.
 // ------------------------------------------------------------------------------------------
 enum {
     K_EDEP, K_ECUT, K_ENUC, K_EBND, K_WCUT, K_WNUC, K_WBND,
     K_DROPW, K_DROPE, K_CLWIN, K_CLWOUT, K_CLEIN, K_CLEOUT, NKACC
 };
 constexpr int MAXKEY = 5;   // own block + two two-bin side emissions
-constexpr int NCON = 2 * NKACC;
 
 __device__ inline bool kacc_is_float(int a) { return a == K_WCUT || a == K_WNUC || a == K_WBND; }
 
@@ -1029,10 +1028,6 @@ __device__ inline bool kacc_is_float(int a) { return a == K_WCUT || a == K_WNUC 
 template <class LY>
 struct FastLds {
     static constexpr int MR = LY::XS + 4;  // records per lane: x_sub spread + two two-bin side emissions
-    // per-accumulator contributions in the serial order, two per accumulator: binary32 values
-    // (the FP64 accumulators add them widened); K_CLEIN holds (w, E), whose FP64 product is added
-    float con[64 * NCON];
-    double cl[64 * 2];           // per-component FP64 sums of K_CLWOUT and K_CLEOUT
     uint32_t rec[64 * MR];   // (key index << 16 | lidx), then the LDS payload address
     float recw[64 * MR];
     uint32_t key[64 * MAXKEY];
@@ -1042,9 +1037,41 @@ struct FastLds {
 
 struct LaneOut {
     int nkey, nrec;
-    int nbnd;          // boundary contributions written so far (w_bnd / E_bnd slots)
     double clwout, cleout;
+    CellAccum c;  // the component's accumulator contributions
 };
+
+// Sum over a batch's 64 lanes of one accumulator's contributions, in a fixed butterfly order
+// (every lane ends with the same sum).
+__device__ inline float wave_sumf(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// The batch sums of all accumulators; lane k < NKACC returns accumulator k's sum (the FP32
+// sums widened, which is exact).
+__device__ inline double batch_sum_lane(const CellAccum& c) {
+    // thirteen independent butterflies, interleaved by the compiler
+    const double edep = wave_sum(c.edep), E_cut = wave_sum(c.E_cut), E_nuc = wave_sum(c.E_nuc),
+                 E_bnd = wave_sum(c.E_bnd), drop_w = wave_sum(c.drop_w), drop_E = wave_sum(c.drop_E),
+                 cl_win = wave_sum(c.cl_win), cl_wout = wave_sum(c.cl_wout), cl_Ein = wave_sum(c.cl_Ein),
+                 cl_Eout = wave_sum(c.cl_Eout);
+    const float w_cut = wave_sumf(c.w_cut), w_nuc = wave_sumf(c.w_nuc), w_bnd = wave_sumf(c.w_bnd);
+    const int l = lane_id();
+    return l == K_EDEP ? edep : l == K_ECUT ? E_cut : l == K_ENUC ? E_nuc : l == K_EBND ? E_bnd
+         : l == K_WCUT ? (double)w_cut : l == K_WNUC ? (double)w_nuc : l == K_WBND ? (double)w_bnd
+         : l == K_DROPW ? drop_w : l == K_DROPE ? drop_E : l == K_CLWIN ? cl_win : l == K_CLWOUT ? cl_wout
+         : l == K_CLEIN ? cl_Ein : l == K_CLEOUT ? cl_Eout : 0.0;
+}
+
+// Lanes 0..NKACC-1 hold the cell's accumulators (FP64 in accd, FP32 in accf) and add their
+// batch sum s.
+__device__ inline void add_batch_sum(double s, double& accd, float& accf) {
+    const int l = lane_id();
+    if (l >= NKACC) return;
+    if (kacc_is_float(l)) accf = accf + (float)s;
+    else accd += s;
+}
 
 // Weight and energy of a component's lateral emissions into other listed cells' +z buckets,
 // booked to the source cell when emitted (they are applied after the phase, apply_foreign): per
@@ -1092,9 +1119,18 @@ __device__ inline void rec_emit(FastLds<LY>& F, LaneOut& o, int t, const Bins& b
     }
 }
 
-template <class LY>
-__device__ inline void con_set(FastLds<LY>& F, int a, int second, float v) {
-    F.con[lane_id() * NCON + 2 * a + second] = v;
+// Adds one contribution v to accumulator K of the lane's component (FP64 accumulators widen it).
+template <int K>
+__device__ inline void cadd(CellAccum& c, float v) {
+    if (K == K_EDEP) c.edep += (double)v;
+    else if (K == K_ECUT) c.E_cut += (double)v;
+    else if (K == K_ENUC) c.E_nuc += (double)v;
+    else if (K == K_EBND) c.E_bnd += (double)v;
+    else if (K == K_WCUT) c.w_cut += v;
+    else if (K == K_WNUC) c.w_nuc += v;
+    else if (K == K_WBND) c.w_bnd += v;
+    else if (K == K_DROPW) c.drop_w += (double)v;
+    else if (K == K_DROPE) c.drop_E += (double)v;
 }
 
 // The component of this lane (see the serial apply for the meaning of each quantity).
@@ -1102,42 +1138,39 @@ template <class LY, bool FINE>
 __device__ inline LaneOut fast_prepare(FastLds<LY>& F, const Comp& r, bool live, int ix, int iz, int Nx, int Nz) {
     const int l = lane_id();
     LaneOut o;
-    o.nkey = o.nrec = o.nbnd = 0;
+    o.nkey = o.nrec = 0;
     o.clwout = o.cleout = 0.0;
-    for (int q = 0; q < NCON; ++q) F.con[l * NCON + q] = 0.0f;
-    F.cl[l * 2] = F.cl[l * 2 + 1] = 0.0;
+    o.c = CellAccum{};
     if (!live) return o;
     const float w = r.w, E = r.E;
-    con_set(F, K_CLWIN, 0, w);
-    con_set(F, K_CLEIN, 0, w);
-    con_set(F, K_CLEIN, 1, E);
+    o.c.cl_win = (double)w;
+    o.c.cl_Ein = (double)w * (double)E;
     if (r.kind == 0) {
-        con_set(F, K_DROPW, 0, w);
-        con_set(F, K_DROPE, 0, E * w);
+        cadd<K_DROPW>(o.c, w);
+        cadd<K_DROPE>(o.c, E * w);
         o.clwout = (double)w;
         o.cleout = (double)(E * w);
     } else if (r.kind == 1) {
-        con_set(F, K_WCUT, 0, w);
-        con_set(F, K_ECUT, 0, E * w);
+        cadd<K_WCUT>(o.c, w);
+        cadd<K_ECUT>(o.c, E * w);
         o.clwout = (double)w;
         o.cleout = (double)(E * w);
     } else {
         const float w_new = r.w_new, E_new = r.E_new;
-        con_set(F, K_EDEP, 0, r.edep);
-        con_set(F, K_WNUC, 0, r.w_rem);
-        con_set(F, K_ENUC, 0, r.E_rem);
+        cadd<K_EDEP>(o.c, r.edep);
+        cadd<K_WNUC>(o.c, r.w_rem);
+        cadd<K_ENUC>(o.c, r.E_rem);
         o.clwout += (double)r.w_rem;
         o.cleout += (double)r.edep + (double)r.E_rem;
         auto bnd = [&](float wb) {
-            con_set(F, K_WBND, o.nbnd, wb);
-            con_set(F, K_EBND, o.nbnd, E_new * wb);
-            o.nbnd++;
+            cadd<K_WBND>(o.c, wb);
+            cadd<K_EBND>(o.c, E_new * wb);
             o.clwout += (double)wb;
             o.cleout += (double)(E_new * wb);
         };
         if (E_new <= 0.1f) {
-            con_set(F, K_WCUT, 0, w_new);
-            con_set(F, K_ECUT, 0, E_new * w_new);
+            cadd<K_WCUT>(o.c, w_new);
+            cadd<K_ECUT>(o.c, E_new * w_new);
             o.clwout += (double)w_new;
             o.cleout += (double)(E_new * w_new);
         } else if (r.face >= 0) {
@@ -1152,15 +1185,15 @@ __device__ inline LaneOut fast_prepare(FastLds<LY>& F, const Comp& r, bool live,
                 foreign_booking(r, o.clwout, o.cleout);
                 if (lat_bnd > 0.0f) bnd(lat_bnd);
                 if (drop > 0.0f) {
-                    con_set(F, K_DROPW, 0, drop);
-                    con_set(F, K_DROPE, 0, E_new * drop);
+                    cadd<K_DROPW>(o.c, drop);
+                    cadd<K_DROPE>(o.c, E_new * drop);
                     o.clwout += (double)drop;
                     o.cleout += (double)(E_new * drop);
                 }
                 if (FINE) {
                     const float tw = fmaxf(0.0f, w_new - lat_bnd - drop);
                     if (tw > 0.0f) {
-                        con_set(F, K_EDEP, 1, (E_new - E_emit) * tw);
+                        cadd<K_EDEP>(o.c, (E_new - E_emit) * tw);
                         o.cleout += (double)((E_new - E_emit) * tw);
                     }
                 }
@@ -1170,7 +1203,7 @@ __device__ inline LaneOut fast_prepare(FastLds<LY>& F, const Comp& r, bool live,
                 if (FINE) {
                     const float tw = fmaxf(0.0f, w_new - 0.0f);
                     if (tw > 0.0f) {
-                        con_set(F, K_EDEP, 1, (E_new - E_emit) * tw);
+                        cadd<K_EDEP>(o.c, (E_new - E_emit) * tw);
                         o.cleout += (double)((E_new - E_emit) * tw);
                     }
                 }
@@ -1213,14 +1246,14 @@ __device__ inline LaneOut fast_prepare(FastLds<LY>& F, const Comp& r, bool live,
                 rf = fminf(1.0f, fmaxf(0.0f, rf));
                 const float rw = w_new * rf;
                 if (rw > 0.0f) {
-                    con_set(F, K_EDEP, 1, (E_new - E_emit) * rw);
+                    cadd<K_EDEP>(o.c, (E_new - E_emit) * rw);
                     o.cleout += (double)((E_new - E_emit) * rw);
                 }
             }
         }
     }
-    F.cl[l * 2] = o.clwout;
-    F.cl[l * 2 + 1] = o.cleout;
+    o.c.cl_wout = o.clwout;
+    o.c.cl_Eout = o.cleout;
     return o;
 }
 
@@ -1324,8 +1357,7 @@ __device__ inline uint32_t own_slot(uint32_t addr) { return (addr * 2654435761u)
 // of addresses that share a slot only wait.  own[] holds ~0u between rounds.  The accumulator
 // lanes then take the components in order.
 template <class LY>
-__device__ inline void fast_rounds(FastLds<LY>& F, const WTabs<LY>& W, uint64_t m, double& accd, float& accf,
-                                   uint32_t* own) {
+__device__ inline void fast_rounds(FastLds<LY>& F, const WTabs<LY>& W, uint64_t m, uint32_t* own) {
     const int cnt = __popcll(m);
     const int l = lane_id();
     const int nr = (l < cnt) ? F.nrec[l] : 0;
@@ -1350,37 +1382,6 @@ __device__ inline void fast_rounds(FastLds<LY>& F, const WTabs<LY>& W, uint64_t 
         for (uint32_t q = pend; q; q &= q - 1) own[own_slot(F.rec[l * FastLds<LY>::MR + (__ffs(q) - 1)])] = ~0u;
         pend &= ~done;
     }
-    // accumulators: lanes 0..NKACC-1, components in order
-    if (l < NKACC) {
-        // float accumulators add both contributions; K_CLEIN adds the FP64 product w E;
-        // K_CLWOUT / K_CLEOUT add the component's FP64 sum; the others add both widened.  Every
-        // lane evaluates both forms and keeps its own (no divergence in the apply wave); adding
-        // +0.0 leaves an FP64 accumulator unchanged.
-        const bool fl = kacc_is_float(l), prod = l == K_CLEIN, sum = l == K_CLWOUT || l == K_CLEOUT;
-        const int cli = l == K_CLEOUT ? 1 : 0;
-        constexpr int G = 8;  // components whose contributions are fetched before they are added
-        for (int j0 = 0; j0 < cnt; j0 += G) {
-            float c0[G], c1[G];
-            double cs[G];
-#pragma unroll
-            for (int u = 0; u < G; ++u) {
-                const int j = min(j0 + u, cnt - 1);
-                c0[u] = F.con[j * NCON + 2 * l];
-                c1[u] = F.con[j * NCON + 2 * l + 1];
-                cs[u] = F.cl[j * 2 + cli];
-            }
-#pragma unroll
-            for (int u = 0; u < G; ++u) {
-                if (j0 + u >= cnt) break;
-                const double d0 = prod ? (double)c0[u] * (double)c1[u] : sum ? cs[u] : (double)c0[u];
-                const double d1 = (prod || sum) ? 0.0 : (double)c1[u];
-                const float f = (accf + c0[u]) + c1[u];
-                accf = fl ? f : accf;
-                accd += d0;
-                accd += d1;
-            }
-        }
-    }
 }
 
 __device__ inline double rld(double v, int l) {
@@ -1404,14 +1405,6 @@ __device__ inline void lanes_to_accum(CellAccum& a, double accd, float accf) {
     a.cl_wout = rld(accd, K_CLWOUT);
     a.cl_Ein = rld(accd, K_CLEIN);
     a.cl_Eout = rld(accd, K_CLEOUT);
-}
-
-__device__ inline void accum_to_lanes(const CellAccum& a, double& accd, float& accf) {
-    const int l = lane_id();
-    accd = l == K_EDEP ? a.edep : l == K_ECUT ? a.E_cut : l == K_ENUC ? a.E_nuc : l == K_EBND ? a.E_bnd
-         : l == K_DROPW ? a.drop_w : l == K_DROPE ? a.drop_E : l == K_CLWIN ? a.cl_win
-         : l == K_CLWOUT ? a.cl_wout : l == K_CLEIN ? a.cl_Ein : l == K_CLEOUT ? a.cl_Eout : 0.0;
-    accf = l == K_WCUT ? a.w_cut : l == K_WNUC ? a.w_nuc : l == K_WBND ? a.w_bnd : 0.0f;
 }
 
 // Dynamic LDS of the transport: the energy edges and the range tables (4 * N_E + 1 floats), so
@@ -1772,229 +1765,240 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
 }
 
 // The serial form of the ordered apply (components in bin order, one at a time): used for a
-// batch whose claims need the closest-block fallback or a global payload.
+// batch whose claims need the closest-block fallback or a global payload.  Component j's
+// accumulator contributions are formed in `a` (all lanes, from zero) in the serial order.
 template <class LY, bool FINE>
-__device__ inline void serial_apply(const Comp& r, uint64_t m, WTabs<LY>& W, CellAccum& a, const Tabs& T,
-                                    Psi& out, Bkt& bk, uint32_t* flags, const CellGeo& G) {
+__device__ inline void serial_component(const Comp& r, int j, WTabs<LY>& W, CellAccum& a, const Tabs& T, Psi& out,
+                                        Bkt& bk, uint32_t* flags, const CellGeo& G) {
     const int l = lane_id();
-    const int ix = G.ix, iz = G.iz, Nx = G.Nx, NT = G.NT, NE = G.NE;
+    const int ix = G.ix, iz = G.iz, Nx = G.Nx, NT = G.NT;
     const float tmin = G.tmin, dth = G.dth;
-    (void)NE;
-    while (m) {
-        const int j = __ffsll((unsigned long long)m) - 1;
-        m &= m - 1;
-        const int kind = rli(r.kind, j);
-        const float w = rlf(r.w, j), E = rlf(r.E, j);
-        a.comps++;
-        a.cl_win += (double)w;
-        a.cl_Ein += (double)w * (double)E;
-        if (kind == 0) {
-            a.drop_w += (double)w;
-            a.drop_E += (double)(E * w);
-            a.cl_wout += (double)w;
-            a.cl_Eout += (double)(E * w);
-            continue;
-        }
-        if (kind == 1) {
-            a.w_cut += w;
-            a.E_cut += E * w;
-            a.cl_wout += (double)w;
-            a.cl_Eout += (double)(E * w);
-            continue;
-        }
-        const float edep = rlf(r.edep, j), w_rem = rlf(r.w_rem, j), E_rem = rlf(r.E_rem, j);
-        const float w_new = rlf(r.w_new, j), E_new = rlf(r.E_new, j);
-        const int face = rli(r.face, j);
-        a.edep += edep;
-        a.w_nuc += w_rem;
-        a.E_nuc += E_rem;
-        a.cl_wout += (double)w_rem;
-        a.cl_Eout += (double)edep + (double)E_rem;
-        if (E_new <= 0.1f) {
-            a.w_cut += w_new;
-            a.E_cut += E_new * w_new;
+    (void)NT;
+    (void)tmin;
+    (void)dth;
+    const int kind = rli(r.kind, j);
+    const float w = rlf(r.w, j), E = rlf(r.E, j);
+    a.cl_win += (double)w;
+    a.cl_Ein += (double)w * (double)E;
+    if (kind == 0) {
+        a.drop_w += (double)w;
+        a.drop_E += (double)(E * w);
+        a.cl_wout += (double)w;
+        a.cl_Eout += (double)(E * w);
+        return;
+    }
+    if (kind == 1) {
+        a.w_cut += w;
+        a.E_cut += E * w;
+        a.cl_wout += (double)w;
+        a.cl_Eout += (double)(E * w);
+        return;
+    }
+    const float edep = rlf(r.edep, j), w_rem = rlf(r.w_rem, j), E_rem = rlf(r.E_rem, j);
+    const float w_new = rlf(r.w_new, j), E_new = rlf(r.E_new, j);
+    const int face = rli(r.face, j);
+    a.edep += edep;
+    a.w_nuc += w_rem;
+    a.E_nuc += E_rem;
+    a.cl_wout += (double)w_rem;
+    a.cl_Eout += (double)edep + (double)E_rem;
+    if (E_new <= 0.1f) {
+        a.w_cut += w_new;
+        a.E_cut += E_new * w_new;
+        a.cl_wout += (double)w_new;
+        a.cl_Eout += (double)(E_new * w_new);
+        return;
+    }
+    Bins bn;
+    bn.bc = rli(r.bn.bc, j);
+    bn.tb1 = rli(r.bn.tb1, j);
+    bn.eb = rli(r.bn.eb, j);
+    bn.tw0 = rlf(r.bn.tw0, j);
+    bn.tw1 = rlf(r.bn.tw1, j);
+    bn.Ecen = rlf(r.bn.Ecen, j);
+    if (face >= 0) {
+        bool nb = (face == 0) ? (iz + 1 < T.Nz) : (face == 1) ? (iz > 0)
+                : (face == 2) ? (ix + 1 < Nx) : (ix > 0);
+        if (!nb) {
+            a.w_bnd += w_new;
+            a.E_bnd += E_new * w_new;
             a.cl_wout += (double)w_new;
             a.cl_Eout += (double)(E_new * w_new);
-            continue;
+            return;
         }
-        Bins bn;
-        bn.bc = rli(r.bn.bc, j);
-        bn.tb1 = rli(r.bn.tb1, j);
-        bn.eb = rli(r.bn.eb, j);
-        bn.tw0 = rlf(r.bn.tw0, j);
-        bn.tw1 = rlf(r.bn.tw1, j);
-        bn.Ecen = rlf(r.bn.Ecen, j);
-        if (face >= 0) {
-            bool nb = (face == 0) ? (iz + 1 < T.Nz) : (face == 1) ? (iz > 0)
-                    : (face == 2) ? (ix + 1 < Nx) : (ix > 0);
-            if (!nb) {
-                a.w_bnd += w_new;
-                a.E_bnd += E_new * w_new;
-                a.cl_wout += (double)w_new;
-                a.cl_Eout += (double)(E_new * w_new);
-                continue;
+        const float E_emit = rlf(r.g0, j);
+        float drop = 0.0f;
+        if (face == 0) {
+            const float wf5 = rlf(r.sw[5], j);
+            const float lat_bnd = rlf(r.g2, j);
+            const uint32_t fmj = rlu(r.fmask, j);
+            float d5 = 0.0f;
+            if (wf5 > 0.0f) d5 = emit<LY>(W, a, T, bk, flags, 0, bn, w_new * wf5, LY::XS / 2, 0);
+#pragma unroll
+            for (int q = 0; q < 11; ++q) {  // lateral emissions into other listed cells
+                if (!((fmj >> q) & 1u)) continue;
+                const float wl = w_new * rlf(r.sw[q], j);
+                if (bn.tw0 > 0.0f) {
+                    const float we = wl * bn.tw0;
+                    a.cl_wout += (double)we;
+                    a.cl_Eout += (double)we * (double)bn.Ecen;
+                }
+                if (bn.tw1 > 0.0f) {
+                    const float we = wl * bn.tw1;
+                    a.cl_wout += (double)we;
+                    a.cl_Eout += (double)we * (double)bn.Ecen;
+                }
             }
-            const float E_emit = rlf(r.g0, j);
-            float drop = 0.0f;
-            if (face == 0) {
-                const float wf5 = rlf(r.sw[5], j);
-                const float lat_bnd = rlf(r.g2, j);
-                const uint32_t fmj = rlu(r.fmask, j);
-                float d5 = 0.0f;
-                if (wf5 > 0.0f) d5 = emit<LY>(W, a, T, bk, flags, 0, bn, w_new * wf5, LY::XS / 2, 0);
+            if (d5 == 0.0f) {
+                drop = rlf(r.g1, j);
+            } else {  // the lateral sum with the centre's drop in its place (q = 5)
 #pragma unroll
-                for (int q = 0; q < 11; ++q) {  // lateral emissions into other listed cells
-                    if (!((fmj >> q) & 1u)) continue;
-                    const float wl = w_new * rlf(r.sw[q], j);
-                    if (bn.tw0 > 0.0f) {
-                        const float we = wl * bn.tw0;
-                        a.cl_wout += (double)we;
-                        a.cl_Eout += (double)we * (double)bn.Ecen;
-                    }
-                    if (bn.tw1 > 0.0f) {
-                        const float we = wl * bn.tw1;
-                        a.cl_wout += (double)we;
-                        a.cl_Eout += (double)we * (double)bn.Ecen;
-                    }
+                for (int q = 0; q < 11; ++q) {
+                    const float wf = rlf(r.sw[q], j);
+                    if (wf <= 0.0f || ((fmj >> q) & 1u)) continue;
+                    const int it = ix + q - 5;
+                    if (it < 0 || it >= Nx) continue;
+                    drop += (q == 5) ? d5 : w_new * wf;
                 }
-                if (d5 == 0.0f) {
-                    drop = rlf(r.g1, j);
-                } else {  // the lateral sum with the centre's drop in its place (q = 5)
+            }
+            if (lat_bnd > 0.0f) {
+                a.w_bnd += lat_bnd;
+                a.E_bnd += E_new * lat_bnd;
+                a.cl_wout += (double)lat_bnd;
+                a.cl_Eout += (double)(E_new * lat_bnd);
+            }
+            if (drop > 0.0f) {
+                a.drop_w += (double)drop;
+                a.drop_E += (double)(E_new * drop);
+                a.cl_wout += (double)drop;
+                a.cl_Eout += (double)(E_new * drop);
+            }
+            if (FINE) {
+                float tw = fmaxf(0.0f, w_new - lat_bnd - drop);
+                if (tw > 0.0f) {
+                    a.edep += (double)((E_new - E_emit) * tw);
+                    a.cl_Eout += (double)((E_new - E_emit) * tw);
+                }
+            }
+        } else {
+            const int xsn = rli(r.i0, j), zsn = rli(r.i1, j);
+            drop = emit<LY>(W, a, T, bk, flags, face, bn, w_new, xsn, zsn);
+            if (drop > 0.0f) {
+                a.drop_w += (double)drop;
+                a.drop_E += (double)(E_new * drop);
+                a.cl_wout += (double)drop;
+                a.cl_Eout += (double)(E_new * drop);
+            }
+            if (FINE) {
+                float tw = fmaxf(0.0f, w_new - drop);
+                if (tw > 0.0f) {
+                    a.edep += (double)((E_new - E_emit) * tw);
+                    a.cl_Eout += (double)((E_new - E_emit) * tw);
+                }
+            }
+        }
+        return;
+    }
+    // ---- remain in the cell ----
+    const int tbn = rli(r.i0, j), ebn = rli(r.i1, j), zst = rli(r.i2, j);
+    const uint32_t bidn = ((uint32_t)(ebn / LY::EL) << 12) | ((uint32_t)(tbn / LY::TL) & 0xFFF);
+    uint32_t sel;
+    const int os = W.claim(0, bidn, sel, out.pool, out.pcount, out.pcap, flags, F_PSI_FULL);
+    if (os < 0) {
+        a.drop_w += (double)w_new;
+        a.drop_E += (double)(E_new * w_new);
+        a.cl_wout += (double)w_new;
+        a.cl_Eout += (double)(E_new * w_new);
+        return;
+    }
+    int tbe = tbn, ebe = ebn;
+    float E_emit = rlf(r.g0, j);
+    if (sel != bidn) {
+        int bte = (int)(sel & 0xFFF), bee = (int)((sel >> 12) & 0xFFF);
+        tbe = max(bte * LY::TL, min(tbe, bte * LY::TL + LY::TL - 1));
+        ebe = max(bee * LY::EL, min(ebe, bee * LY::EL + LY::EL - 1));
+        E_emit = T.ee[ebe] + 0.5f * (T.ee[ebe + 1] - T.ee[ebe]);
+    }
+    const uint32_t oref = rlu(W.r0, os);
+    float mywf = 0.0f;
 #pragma unroll
-                    for (int q = 0; q < 11; ++q) {
-                        const float wf = rlf(r.sw[q], j);
-                        if (wf <= 0.0f || ((fmj >> q) & 1u)) continue;
-                        const int it = ix + q - 5;
-                        if (it < 0 || it >= Nx) continue;
-                        drop += (q == 5) ? d5 : w_new * wf;
-                    }
-                }
-                if (lat_bnd > 0.0f) {
-                    a.w_bnd += lat_bnd;
-                    a.E_bnd += E_new * lat_bnd;
-                    a.cl_wout += (double)lat_bnd;
-                    a.cl_Eout += (double)(E_new * lat_bnd);
-                }
-                if (drop > 0.0f) {
-                    a.drop_w += (double)drop;
-                    a.drop_E += (double)(E_new * drop);
-                    a.cl_wout += (double)drop;
-                    a.cl_Eout += (double)(E_new * drop);
-                }
-                if (FINE) {
-                    float tw = fmaxf(0.0f, w_new - lat_bnd - drop);
-                    if (tw > 0.0f) {
-                        a.edep += (double)((E_new - E_emit) * tw);
-                        a.cl_Eout += (double)((E_new - E_emit) * tw);
-                    }
+    for (int q = 0; q < LY::XS; ++q) {
+        float wf = rlf(r.sw[q], j);
+        if (l == q) mywf = wf;
+        if (wf > 0.0f) {
+            float wsp = w_new * wf;
+            a.cl_wout += (double)wsp;
+            a.cl_Eout += (double)wsp * (double)E_emit;
+        }
+    }
+    if (l < LY::XS && mywf > 0.0f)
+        W.add(oref, LY::enc(tbe % LY::TL, ebe % LY::EL, l, zst), w_new * mywf, out.pool);
+    const float wcf = rlf(r.wcf, j), wl = rlf(r.wl, j), wr = rlf(r.wr, j);
+    if (wcf < 1.0f - 1e-6f) {
+        const int zsb = zst;
+        if (wl > 0.0f) {
+            float wsl = w_new * wl;
+            if (ix > 0) {
+                float d = emit<LY>(W, a, T, bk, flags, 3, bn, wsl, LY::XS - 1, zsb);
+                if (d > 0.0f) {
+                    a.drop_w += (double)d;
+                    a.drop_E += (double)(E_new * d);
+                    a.cl_wout += (double)d;
+                    a.cl_Eout += (double)(E_new * d);
                 }
             } else {
-                const int xsn = rli(r.i0, j), zsn = rli(r.i1, j);
-                drop = emit<LY>(W, a, T, bk, flags, face, bn, w_new, xsn, zsn);
-                if (drop > 0.0f) {
-                    a.drop_w += (double)drop;
-                    a.drop_E += (double)(E_new * drop);
-                    a.cl_wout += (double)drop;
-                    a.cl_Eout += (double)(E_new * drop);
+                a.w_bnd += wsl;
+                a.E_bnd += E_new * wsl;
+                a.cl_wout += (double)wsl;
+                a.cl_Eout += (double)(E_new * wsl);
+            }
+        }
+        if (wr > 0.0f) {
+            float wsr = w_new * wr;
+            if (ix < Nx - 1) {
+                float d = emit<LY>(W, a, T, bk, flags, 2, bn, wsr, 0, zsb);
+                if (d > 0.0f) {
+                    a.drop_w += (double)d;
+                    a.drop_E += (double)(E_new * d);
+                    a.cl_wout += (double)d;
+                    a.cl_Eout += (double)(E_new * d);
                 }
-                if (FINE) {
-                    float tw = fmaxf(0.0f, w_new - drop);
-                    if (tw > 0.0f) {
-                        a.edep += (double)((E_new - E_emit) * tw);
-                        a.cl_Eout += (double)((E_new - E_emit) * tw);
-                    }
-                }
-            }
-            continue;
-        }
-        // ---- remain in the cell ----
-        const int tbn = rli(r.i0, j), ebn = rli(r.i1, j), zst = rli(r.i2, j);
-        const uint32_t bidn = ((uint32_t)(ebn / LY::EL) << 12) | ((uint32_t)(tbn / LY::TL) & 0xFFF);
-        uint32_t sel;
-        const int os = W.claim(0, bidn, sel, out.pool, out.pcount, out.pcap, flags, F_PSI_FULL);
-        if (os < 0) {
-            a.drop_w += (double)w_new;
-            a.drop_E += (double)(E_new * w_new);
-            a.cl_wout += (double)w_new;
-            a.cl_Eout += (double)(E_new * w_new);
-            continue;
-        }
-        int tbe = tbn, ebe = ebn;
-        float E_emit = rlf(r.g0, j);
-        if (sel != bidn) {
-            int bte = (int)(sel & 0xFFF), bee = (int)((sel >> 12) & 0xFFF);
-            tbe = max(bte * LY::TL, min(tbe, bte * LY::TL + LY::TL - 1));
-            ebe = max(bee * LY::EL, min(ebe, bee * LY::EL + LY::EL - 1));
-            E_emit = T.ee[ebe] + 0.5f * (T.ee[ebe + 1] - T.ee[ebe]);
-        }
-        const uint32_t oref = rlu(W.r0, os);
-        float mywf = 0.0f;
-#pragma unroll
-        for (int q = 0; q < LY::XS; ++q) {
-            float wf = rlf(r.sw[q], j);
-            if (l == q) mywf = wf;
-            if (wf > 0.0f) {
-                float wsp = w_new * wf;
-                a.cl_wout += (double)wsp;
-                a.cl_Eout += (double)wsp * (double)E_emit;
+            } else {
+                a.w_bnd += wsr;
+                a.E_bnd += E_new * wsr;
+                a.cl_wout += (double)wsr;
+                a.cl_Eout += (double)(E_new * wsr);
             }
         }
-        if (l < LY::XS && mywf > 0.0f)
-            W.add(oref, LY::enc(tbe % LY::TL, ebe % LY::EL, l, zst), w_new * mywf, out.pool);
-        const float wcf = rlf(r.wcf, j), wl = rlf(r.wl, j), wr = rlf(r.wr, j);
-        if (wcf < 1.0f - 1e-6f) {
-            const int zsb = zst;
-            if (wl > 0.0f) {
-                float wsl = w_new * wl;
-                if (ix > 0) {
-                    float d = emit<LY>(W, a, T, bk, flags, 3, bn, wsl, LY::XS - 1, zsb);
-                    if (d > 0.0f) {
-                        a.drop_w += (double)d;
-                        a.drop_E += (double)(E_new * d);
-                        a.cl_wout += (double)d;
-                        a.cl_Eout += (double)(E_new * d);
-                    }
-                } else {
-                    a.w_bnd += wsl;
-                    a.E_bnd += E_new * wsl;
-                    a.cl_wout += (double)wsl;
-                    a.cl_Eout += (double)(E_new * wsl);
-                }
-            }
-            if (wr > 0.0f) {
-                float wsr = w_new * wr;
-                if (ix < Nx - 1) {
-                    float d = emit<LY>(W, a, T, bk, flags, 2, bn, wsr, 0, zsb);
-                    if (d > 0.0f) {
-                        a.drop_w += (double)d;
-                        a.drop_E += (double)(E_new * d);
-                        a.cl_wout += (double)d;
-                        a.cl_Eout += (double)(E_new * d);
-                    }
-                } else {
-                    a.w_bnd += wsr;
-                    a.E_bnd += E_new * wsr;
-                    a.cl_wout += (double)wsr;
-                    a.cl_Eout += (double)(E_new * wsr);
-                }
-            }
-        }
-        if (FINE) {
-            float rf = wcf;
-            if (ix > 0) rf += wl;
-            if (ix < Nx - 1) rf += wr;
-            rf = fminf(1.0f, fmaxf(0.0f, rf));
-            float rw = w_new * rf;
-            if (rw > 0.0f) {
-                a.edep += (double)((E_new - E_emit) * rw);
-                a.cl_Eout += (double)((E_new - E_emit) * rw);
-            }
+    }
+    if (FINE) {
+        float rf = wcf;
+        if (ix > 0) rf += wl;
+        if (ix < Nx - 1) rf += wr;
+        rf = fminf(1.0f, fmaxf(0.0f, rf));
+        float rw = w_new * rf;
+        if (rw > 0.0f) {
+            a.edep += (double)((E_new - E_emit) * rw);
+            a.cl_Eout += (double)((E_new - E_emit) * rw);
         }
     }
 }
 
-// Per-cell part of the transport geometry of cell `cell` (one wavefront): position, material,
-// Fermi-Eyges moments of its spot and the lateral +z neighbours listed in this phase.
+// The batch's components in bin order; lane j of `mine` receives component j's contributions.
+template <class LY, bool FINE>
+__device__ inline void serial_apply(const Comp& r, uint64_t m, WTabs<LY>& W, CellAccum& mine, uint32_t& comps,
+                                    const Tabs& T, Psi& out, Bkt& bk, uint32_t* flags, const CellGeo& G) {
+    const int l = lane_id();
+    while (m) {
+        const int j = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        CellAccum a = {};
+        serial_component<LY, FINE>(r, j, W, a, T, out, bk, flags, G);
+        ++comps;
+        if (l == j) mine = a;
+    }
+}
+
 __device__ inline void set_cell_geo(CellGeo& G, const Tabs& T, int cell, const uint32_t* ltag, uint32_t tag) {
     const int l = lane_id();
     const int sp = cell / T.Ns;
@@ -2138,6 +2142,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
     __shared__ float s_qw[QCAP];
     __shared__ uint32_t s_bid[KB], s_ref[KB], s_cnt[KB], s_off[KB];
     __shared__ uint64_t s_mask[TW];
+    __shared__ double s_bsum[TW][NKACC];  // accumulator sums of each wave's batch
     __shared__ uint32_t s_round[2];
     __shared__ uint32_t s_surr;  // learned-physics substeps of the cell
     __shared__ uint32_t s_sdev;  // bits of the largest learned-physics deviation of the cell (>= 0)
@@ -2212,6 +2217,14 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
             }
         }
         set_cell_geo(G, T, cell, ltag, tag);
+        // the cell's seven cumulative tallies, one lane each (four FP64, three FP32 arrays), are
+        // read now and written back when the cell ends (no other workgroup writes them)
+        double tal_d = 0.0;
+        float tal_f = 0.0f;
+        if (wv == 0) {
+            if (l < 4) tal_d = (l == 0 ? tal.edep : l == 1 ? tal.E_cut : l == 2 ? tal.E_nuc : tal.E_bnd)[cell];
+            else if (l < 7) tal_f = (l == 4 ? tal.w_cut : l == 5 ? tal.w_nuc : tal.w_bnd)[cell];
+        }
         if (wv == 0 && l < 4) {  // the cell's neighbours join the phase's destination set
             const int d = l == 0   ? (G.iz + 1 < T.Nz ? cell + G.Nx : -1)
                           : l == 1 ? (G.iz > 0 ? cell - G.Nx : -1)
@@ -2381,6 +2394,8 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                     const LaneOut o = fast_prepare<LY, FINE>(s_fast[wv], r, live, G.ix, G.iz, G.Nx, T.Nz);
                     s_fast[wv].nkey[l] = o.nkey;
                     s_fast[wv].nrec[l] = o.nrec;
+                    const double bsum = batch_sum_lane(o.c);
+                    if (l < NKACC) s_bsum[wv][l] = bsum;
                     const uint64_t m = __ballot(live);
                     const uint32_t nsur = (uint32_t)__popcll(__ballot(live && r.kind == 2 && r.surr));
                     if (l == 0) s_mask[wv] = m;
@@ -2416,8 +2431,10 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                         const bool fast = fast_claims<LY>(F, W);
                         SM2D_STAMP(5);
                         if (fast) {
-                            fast_rounds<LY>(F, W, m, accd, accf, s_own);
+                            fast_rounds<LY>(F, W, m, s_own);
                             SM2D_STAMP(6);
+                            add_batch_sum(l < NKACC ? s_bsum[bb - b0][l] : 0.0, accd, accf);
+                            SM2D_STAMP(11);
                             comps += (uint32_t)__popcll(m);
                         } else {
                             // the batch again, in wave 0, for the serial form
@@ -2428,13 +2445,10 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                             const Comp r = transport_component<LY, FINE>(T, P, G, live, live ? s_qw[i] : 0.0f,
                                                                          (int)(it & 0xFFFFu), slot, bid & 0xFFF,
                                                                          (bid >> 12) & 0xFFF);
-                            CellAccum a = {};
-                            lanes_to_accum(a, accd, accf);
-                            a.comps = comps;
-                            serial_apply<LY, FINE>(r, m, W, a, T, out, bk, flags, G);
+                            CellAccum mine = {};
+                            serial_apply<LY, FINE>(r, m, W, mine, comps, T, out, bk, flags, G);
+                            add_batch_sum(batch_sum_lane(mine), accd, accf);
                             SM2D_STAMP(7);
-                            accum_to_lanes(a, accd, accf);
-                            comps = a.comps;
                         }
                     }
                 }
@@ -2450,15 +2464,14 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
             SM2D_STAMP(8);
             flush_tables<LY>(W, out, bk, cell, li, flags);
             SM2D_STAMP(9);
-            // the seven cumulative tallies, one lane each (four FP64, three FP32 arrays)
             if (l < 4) {
                 double* t = l == 0 ? tal.edep : l == 1 ? tal.E_cut : l == 2 ? tal.E_nuc : tal.E_bnd;
                 const double v = l == 0 ? a.edep : l == 1 ? a.E_cut : l == 2 ? a.E_nuc : a.E_bnd;
-                t[cell] += v;
+                t[cell] = tal_d + v;
             } else if (l < 7) {
                 float* t = l == 4 ? tal.w_cut : l == 5 ? tal.w_nuc : tal.w_bnd;
                 const float v = l == 4 ? a.w_cut : l == 5 ? a.w_nuc : a.w_bnd;
-                t[cell] += v;
+                t[cell] = tal_f + v;
             }
             if (l == 0) {
                 double* o = acc + (size_t)li * NACC;
