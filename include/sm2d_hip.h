@@ -392,13 +392,15 @@ int sm2d_surrogate_set_table(sm2d_ctx* ctx, const float* table, int n_mat, int n
                              const int32_t* rowmat);
 
 /* Stage clocks of the transport kernel (diagnostic build only, python -m sm_2d_amd.build
- * --stage-clocks; the normal build returns SM2D_ERR_UNSUPPORTED): out16[k] receives the
- * microseconds the first thread of the transport workgroups spent in stage k, summed over cells
- * (0 tags and destination marks, 1 own psi_out staging, 2 count and queue, 3 physics, 4 wait for
- * other waves, 5 claims, 6 ordered adds, 7 serial apply, 8 end-of-cell barrier, 9 table flush,
- * 10 tallies), out16[15] the cells; reset = 1 zeroes the sums afterwards.  No counterpart in the
- * reference. */
-int sm2d_stage_clocks(int device, double* out16, int reset);
+ * --stage-clocks; the normal build returns SM2D_ERR_UNSUPPORTED): out32[k] receives the
+ * microseconds the transport workgroups spent in stage k, summed over cells (0 tags and
+ * destination marks, 1 own psi_out staging, 2 count and queue, 3 physics, 4 wait for other
+ * waves, 5 claims, 6 ordered adds, 7 serial apply, 8 end-of-cell barrier, 9 table flush,
+ * 10 tallies, 11 accumulator batch sums), out32[15] the cells, and out32[16 + k] the parts of
+ * the physics stage (0 set-up and direction, 1 step, energy loss and widths, 2 angular draw and
+ * exit face, 3 spread weights, 4 emission bins, 5 lane preparation, 6 batch sums); reset = 1
+ * zeroes the sums afterwards.  No counterpart in the reference. */
+int sm2d_stage_clocks(int device, double* out32, int reset);
 
 /* --- self-test of the shared elementary functions on the device -------------------------- */
 /* Evaluates f(x[i]) for i < n on the GPU: which = 0 exp, 1 log, 2 sin, 3 cos, 4 erf, 5 sin and

@@ -24,7 +24,26 @@ namespace sm2d {
 // [15] counts cells), so the stamps add no memory traffic inside the stages.  The work is the
 // same as in the normal build; the normal build compiles the stamps away.
 #ifdef SM2D_STAGE_CLOCKS
-__device__ unsigned long long stage_clk[16];
+__device__ unsigned long long stage_clk[32];
+// parts of the physics stage ([16 + k]), stamped inside the per-component physics
+struct SubClk {
+    unsigned long long c[8];
+    unsigned long long p;
+};
+#define SM2D_SUBP , SubClk *sc_
+#define SM2D_SUBA(x) , x
+#define SM2D_SUB_START()                   \
+    do {                                   \
+        if (sc_) sc_->p = wall_clock64();  \
+    } while (0)
+#define SM2D_SUB(k)                                   \
+    do {                                              \
+        if (sc_) {                                    \
+            const unsigned long long t_ = wall_clock64(); \
+            sc_->c[k] += t_ - sc_->p;                 \
+            sc_->p = t_;                              \
+        }                                             \
+    } while (0)
 #define SM2D_STAMP(k)                                     \
     do {                                                  \
         const unsigned long long t_ = wall_clock64();     \
@@ -34,6 +53,8 @@ __device__ unsigned long long stage_clk[16];
 #define SM2D_STAMP_START()                                \
     unsigned long long clk_prev = wall_clock64();         \
     unsigned long long clk_acc[12] = {};                  \
+    SubClk clk_sub = {};                                  \
+    SubClk* sc_ = &clk_sub;                               \
     unsigned long long clk_cells = 0
 #define SM2D_STAMP_RESTART() clk_prev = wall_clock64()
 #define SM2D_STAMP_CELL() ++clk_cells
@@ -42,6 +63,8 @@ __device__ unsigned long long stage_clk[16];
         if (threadIdx.x == 0) {                           \
             for (int k_ = 0; k_ < 12; ++k_)               \
                 atomicAdd(&stage_clk[k_], clk_acc[k_]);   \
+            for (int k_ = 0; k_ < 8; ++k_)                \
+                atomicAdd(&stage_clk[16 + k_], clk_sub.c[k_]); \
             atomicAdd(&stage_clk[15], clk_cells);         \
         }                                                 \
     } while (0)
@@ -60,6 +83,14 @@ __device__ unsigned long long stage_clk[16];
     } while (0)
 #define SM2D_STAMP_FLUSH() \
     do {                   \
+    } while (0)
+#define SM2D_SUBP
+#define SM2D_SUBA(x)
+#define SM2D_SUB_START() \
+    do {                 \
+    } while (0)
+#define SM2D_SUB(k) \
+    do {            \
     } while (0)
 #endif
 // stages: 0 tags and destination marks, 1 own psi_out staging, 2 count and queue of non-zero
@@ -1030,8 +1061,7 @@ struct FastLds {
     static constexpr int MR = LY::XS + 4;  // records per lane: x_sub spread + two two-bin side emissions
     uint32_t rec[64 * MR];   // (key index << 16 | lidx), then the LDS payload address
     float recw[64 * MR];
-    uint32_t key[64 * MAXKEY];
-    uint32_t krow[64 * MAXKEY];
+    uint32_t key[64 * MAXKEY];  // claim keys; the claims reuse it for the new keys in order
     int nkey[64], nrec[64];
 };
 
@@ -1041,10 +1071,36 @@ struct LaneOut {
     CellAccum c;  // the component's accumulator contributions
 };
 
+// Lane i receives lane (i ^ M)'s 32-bit value: DPP quad permutations for M = 1, 2, LDS-free
+// swizzles for M = 4, 8, 16, a permute for M = 32.  Every lane of the wave must be active.
+template <int M>
+__device__ inline int lane_xor(int v) {
+    if constexpr (M == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);  // [1,0,3,2]
+    else if constexpr (M == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // [2,3,0,1]
+    else if constexpr (M == 32) return __shfl_xor(v, 32);
+    else return __builtin_amdgcn_ds_swizzle(v, (M << 10) | 0x1F);  // and 0x1F, xor M (32-lane groups)
+}
+template <int M>
+__device__ inline double lane_xor(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)lane_xor<M>((int)(uint32_t)u), hi = (uint32_t)lane_xor<M>((int)(uint32_t)(u >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+template <int M>
+__device__ inline float lane_xor(float v) {
+    return __int_as_float(lane_xor<M>(__float_as_int(v)));
+}
+
 // Sum over a batch's 64 lanes of one accumulator's contributions, in a fixed butterfly order
-// (every lane ends with the same sum).
-__device__ inline float wave_sumf(float v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+// (v_i + v_(i^o) for o = 32, 16, ..., 1; every lane ends with the same sum).  Full wave only.
+template <class F>
+__device__ inline F bfly_sum(F v) {
+    v += lane_xor<32>(v);
+    v += lane_xor<16>(v);
+    v += lane_xor<8>(v);
+    v += lane_xor<4>(v);
+    v += lane_xor<2>(v);
+    v += lane_xor<1>(v);
     return v;
 }
 
@@ -1052,11 +1108,11 @@ __device__ inline float wave_sumf(float v) {
 // sums widened, which is exact).
 __device__ inline double batch_sum_lane(const CellAccum& c) {
     // thirteen independent butterflies, interleaved by the compiler
-    const double edep = wave_sum(c.edep), E_cut = wave_sum(c.E_cut), E_nuc = wave_sum(c.E_nuc),
-                 E_bnd = wave_sum(c.E_bnd), drop_w = wave_sum(c.drop_w), drop_E = wave_sum(c.drop_E),
-                 cl_win = wave_sum(c.cl_win), cl_wout = wave_sum(c.cl_wout), cl_Ein = wave_sum(c.cl_Ein),
-                 cl_Eout = wave_sum(c.cl_Eout);
-    const float w_cut = wave_sumf(c.w_cut), w_nuc = wave_sumf(c.w_nuc), w_bnd = wave_sumf(c.w_bnd);
+    const double edep = bfly_sum(c.edep), E_cut = bfly_sum(c.E_cut), E_nuc = bfly_sum(c.E_nuc),
+                 E_bnd = bfly_sum(c.E_bnd), drop_w = bfly_sum(c.drop_w), drop_E = bfly_sum(c.drop_E),
+                 cl_win = bfly_sum(c.cl_win), cl_wout = bfly_sum(c.cl_wout), cl_Ein = bfly_sum(c.cl_Ein),
+                 cl_Eout = bfly_sum(c.cl_Eout);
+    const float w_cut = bfly_sum(c.w_cut), w_nuc = bfly_sum(c.w_nuc), w_bnd = bfly_sum(c.w_bnd);
     const int l = lane_id();
     return l == K_EDEP ? edep : l == K_ECUT ? E_cut : l == K_ENUC ? E_nuc : l == K_EBND ? E_bnd
          : l == K_WCUT ? (double)w_cut : l == K_WNUC ? (double)w_nuc : l == K_WBND ? (double)w_bnd
@@ -1263,8 +1319,15 @@ __device__ inline LaneOut fast_prepare(FastLds<LY>& F, const Comp& r, bool live,
 // change once every key is decided and the capacities are checked, so a false return (a table
 // would overflow into the closest-block fallback, a slot's payload is not in LDS, or the LDS
 // rows run out) leaves them untouched for the serial path.  Resolves record addresses.
+// Direct-mapped cache of the current cell's claimed keys (key -> LDS row) in 2 x 64 words of
+// LDS: keys found in it skip the serial loop over distinct keys.  Only slots whose payload is
+// an LDS row enter it, and a cell's slots keep their rows until the cell ends, so a hit gives
+// the loop's answer.
+constexpr int NKC = 64;
+__device__ inline uint32_t kc_slot(uint32_t K) { return (K * 2654435761u) >> 26; }
+
 template <class LY>
-__device__ inline bool fast_claims(FastLds<LY>& F, WTabs<LY>& W) {
+__device__ inline bool fast_claims(FastLds<LY>& F, WTabs<LY>& W, uint32_t* kc) {
     const int l = lane_id();
     const int nkey = F.nkey[l], nrec = F.nrec[l];
     uint32_t k0 = EMPTY, k1 = EMPTY, k2 = EMPTY, k3 = EMPTY, k4 = EMPTY;
@@ -1275,6 +1338,17 @@ __device__ inline bool fast_claims(FastLds<LY>& F, WTabs<LY>& W) {
     if (nkey > 4) k4 = F.key[l * MAXKEY + 4];
     uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;
     uint32_t pend = (nkey > 0) ? ((1u << nkey) - 1u) : 0u;
+    {
+        const uint32_t h0 = kc_slot(k0), h1 = kc_slot(k1), h2 = kc_slot(k2), h3 = kc_slot(k3), h4 = kc_slot(k4);
+        const uint32_t c0 = kc[h0], c1 = kc[h1], c2 = kc[h2], c3 = kc[h3], c4 = kc[h4];
+        const uint32_t q0 = kc[NKC + h0], q1 = kc[NKC + h1], q2 = kc[NKC + h2], q3 = kc[NKC + h3],
+                       q4 = kc[NKC + h4];
+        if ((pend & 1u) && c0 == k0) { r0 = q0; pend &= ~1u; }
+        if ((pend & 2u) && c1 == k1) { r1 = q1; pend &= ~2u; }
+        if ((pend & 4u) && c2 == k2) { r2 = q2; pend &= ~4u; }
+        if ((pend & 8u) && c3 == k3) { r3 = q3; pend &= ~8u; }
+        if ((pend & 16u) && c4 == k4) { r4 = q4; pend &= ~16u; }
+    }
     const uint32_t used0 = W.used;
     uint32_t nnew = 0;  // new slots so far, in order of first appearance (wave-uniform)
     uint32_t cnt = 0;   // new slots per table, 6 bits per table (wave-uniform)
@@ -1300,8 +1374,13 @@ __device__ inline bool fast_claims(FastLds<LY>& F, WTabs<LY>& W) {
             const uint32_t ref = rlu(W.gr(t), __ffsll((unsigned long long)found) - 1);
             if (!(ref & CACHED)) return false;
             row = ref & ~CACHED;
+            if (l == 0) {
+                const uint32_t h = kc_slot(K);
+                kc[h] = K;
+                kc[NKC + h] = row;
+            }
         } else {
-            if (l == 0 && nnew < 64 * MAXKEY) F.krow[nnew] = K;
+            if (l == 0 && nnew < 64 * MAXKEY) F.key[nnew] = K;
             row = used0 + nnew;
             ++nnew;
             cnt += 1u << (6 * t);
@@ -1318,7 +1397,7 @@ __device__ inline bool fast_claims(FastLds<LY>& F, WTabs<LY>& W) {
         for (int t = 0; t < 5; ++t)
             if (W.gn(t) + ((cnt >> (6 * t)) & 63u) > (uint32_t)KB) return false;
         // new slot i (first-appearance order) of table t takes the table's next slot and row used0 + i
-        const uint32_t kv = ((uint32_t)l < nnew) ? F.krow[l] : EMPTY;
+        const uint32_t kv = ((uint32_t)l < nnew) ? F.key[l] : EMPTY;
         uint32_t seen = 0;
         for (uint32_t i = 0; i < nnew; ++i) {
             const uint32_t K = rlu(kv, (int)i);
@@ -1327,6 +1406,11 @@ __device__ inline bool fast_claims(FastLds<LY>& F, WTabs<LY>& W) {
             seen += 1u << (6 * t);
             W.sb(t, K & 0xFFFFFFu, (uint32_t)l == slot);
             W.sr(t, CACHED | (used0 + i), (uint32_t)l == slot);
+            if (l == 0) {
+                const uint32_t h = kc_slot(K);
+                kc[h] = K;
+                kc[NKC + h] = used0 + i;
+            }
             float* row = W.rows + (size_t)(used0 + i) * LY::LB;
 #pragma unroll
             for (int k = 0; k < LY::PL; ++k) row[l + 64 * k] = 0.0f;
@@ -1354,8 +1438,7 @@ __device__ inline uint32_t own_slot(uint32_t addr) { return (addr * 2654435761u)
 // addresses within a component).  Each round every pending record bids its index (component-major)
 // for its address's owner slot; the lowest bid per slot adds its weight and leaves.  Records of
 // one address therefore add in component order, one per round, as in the serial form; records
-// of addresses that share a slot only wait.  own[] holds ~0u between rounds.  The accumulator
-// lanes then take the components in order.
+// of addresses that share a slot only wait.  own[] holds ~0u between rounds.
 template <class LY>
 __device__ inline void fast_rounds(FastLds<LY>& F, const WTabs<LY>& W, uint64_t m, uint32_t* own) {
     const int cnt = __popcll(m);
@@ -1440,7 +1523,8 @@ struct CellGeo {
 
 template <class LY, bool FINE>
 __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const CellGeo& G, bool live, float w_in,
-                                           int lidx_in, uint32_t slot, uint32_t bT, uint32_t bEb) {
+                                           int lidx_in, uint32_t slot, uint32_t bT, uint32_t bEb SM2D_SUBP) {
+    SM2D_SUB_START();
     const int lc = G.lc, iz = G.iz, ix = G.ix, Nx = G.Nx, NT = G.NT, NE = G.NE;
     const float dx = G.dx, dz = G.dz, hdx = G.hdx, hdz = G.hdz, tmin = G.tmin, dth = G.dth;
     const float rh = G.rh, X0 = G.X0, A = G.A, B = G.B, sx0 = G.sx0;
@@ -1482,6 +1566,7 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
             const float zc = dz * ((float)(2 * LY::lz(lidx) + 1 - LY::ZS) / (float)(2 * LY::ZS));
             float mu, eta;
             sincosf_(theta, eta, mu);
+            SM2D_SUB(0);
             float ds, dE, E_new, edep, w_new, w_rem, E_rem, sx, sxt, sth;
             unsigned pseed;
             if (FINE) {
@@ -1572,6 +1657,7 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
                 pseed = (unsigned)fminf(ps * 1000.0f, 4294967295.0f);
                 ds = cstep;
             }
+            SM2D_SUB(1);
             unsigned ts = ((unsigned)(lc + 1) * 2654435761u) ^
                           ((unsigned)(slot + 1) * 2246822519u) ^
                           ((unsigned)(lidx + 1) * 3266489917u) ^ (unsigned)(ebin + 17) ^
@@ -1597,6 +1683,7 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
             r.th_new = thn;
             r.x_new = xn;
             r.z_new = zn;
+            SM2D_SUB(2);
             if (E_new > 0.1f && face <= 0) {
                 // Both spreads need Gaussian CDF values at up to 12 points; they are evaluated
                 // in one uniform loop so that a wave holding both kinds of component runs the
@@ -1708,6 +1795,7 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
             }
         }
     }
+    SM2D_SUB(3);
     // ------- emission binning and the table-independent parts of the apply -------
     r.bn = Bins{0, 0, 0, 0.0f, 0.0f, 0.0f};
     r.i0 = r.i1 = r.i2 = 0;
@@ -1761,6 +1849,7 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
             r.g0 = ecenter(T.ee, ebn);
         }
     }
+    SM2D_SUB(4);
     return r;
 }
 
@@ -2058,7 +2147,7 @@ __device__ void apply_foreign(const Tabs& T, const Tp& P, CellGeo G, const Psi& 
                     const bool live = w > 0.0f;
                     if (!__ballot(live)) continue;
                     const Comp r = transport_component<LY, FINE>(T, P, G, live, live ? w : 0.0f, l + 64 * kk, sl,
-                                                                 bid & 0xFFF, (bid >> 12) & 0xFFF);
+                                                                 bid & 0xFFF, (bid >> 12) & 0xFFF SM2D_SUBA(nullptr));
                     float wq = 0.0f;
 #pragma unroll
                     for (int u = 0; u < 11; ++u)
@@ -2123,7 +2212,7 @@ __device__ inline void dest_mark(const Dest& D, uint32_t d, uint32_t tag, uint32
 }
 
 template <class LY, bool FINE, int TW>
-__global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Psi out, Bkt bk,
+__global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_transport(Tabs Tg, Tp P, Psi in, Psi out, Bkt bk,
                                                    const uint32_t* __restrict__ list,
                                                    const uint32_t* __restrict__ nlist,
                                                    const uint32_t* __restrict__ ltag,
@@ -2147,6 +2236,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
     __shared__ uint32_t s_surr;  // learned-physics substeps of the cell
     __shared__ uint32_t s_sdev;  // bits of the largest learned-physics deviation of the cell (>= 0)
     __shared__ uint32_t s_own[NOWN];
+    __shared__ uint32_t s_kc[2 * NKC];  // claimed-key cache of the cell (fast_claims)
     extern __shared__ float s_tab[];
     for (int i = threadIdx.x; i < NOWN; i += blockDim.x) s_own[i] = ~0u;
     Tabs T = Tg;
@@ -2236,6 +2326,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
             s_surr = 0;
             s_sdev = 0;
         }
+        if (threadIdx.x < NKC) s_kc[threadIdx.x] = EMPTY;
         if (threadIdx.x < ns) {
             s_bid[threadIdx.x] = c_bid;
             s_ref[threadIdx.x] = c_ref;
@@ -2390,12 +2481,14 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                     const uint32_t slot = it >> 16, bid = s_bid[slot];
                     const Comp r = transport_component<LY, FINE>(T, P, G, live, live ? s_qw[i] : 0.0f,
                                                                  (int)(it & 0xFFFFu), slot, bid & 0xFFF,
-                                                                 (bid >> 12) & 0xFFF);
+                                                                 (bid >> 12) & 0xFFF SM2D_SUBA(sc_));
                     const LaneOut o = fast_prepare<LY, FINE>(s_fast[wv], r, live, G.ix, G.iz, G.Nx, T.Nz);
                     s_fast[wv].nkey[l] = o.nkey;
                     s_fast[wv].nrec[l] = o.nrec;
+                    SM2D_SUB(5);
                     const double bsum = batch_sum_lane(o.c);
                     if (l < NKACC) s_bsum[wv][l] = bsum;
+                    SM2D_SUB(6);
                     const uint64_t m = __ballot(live);
                     const uint32_t nsur = (uint32_t)__popcll(__ballot(live && r.kind == 2 && r.surr));
                     if (l == 0) s_mask[wv] = m;
@@ -2428,7 +2521,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                     for (uint32_t bb = b0; bb < nb && bb < b0 + TW; ++bb) {
                         FastLds<LY>& F = s_fast[bb - b0];
                         const uint64_t m = s_mask[bb - b0];
-                        const bool fast = fast_claims<LY>(F, W);
+                        const bool fast = fast_claims<LY>(F, W, s_kc);
                         SM2D_STAMP(5);
                         if (fast) {
                             fast_rounds<LY>(F, W, m, s_own);
@@ -2444,7 +2537,7 @@ __global__ __launch_bounds__(64 * TW) void k_transport(Tabs Tg, Tp P, Psi in, Ps
                             const uint32_t slot = it >> 16, bid = s_bid[slot];
                             const Comp r = transport_component<LY, FINE>(T, P, G, live, live ? s_qw[i] : 0.0f,
                                                                          (int)(it & 0xFFFFu), slot, bid & 0xFFF,
-                                                                         (bid >> 12) & 0xFFF);
+                                                                         (bid >> 12) & 0xFFF SM2D_SUBA(nullptr));
                             CellAccum mine = {};
                             serial_apply<LY, FINE>(r, m, W, mine, comps, T, out, bk, flags, G);
                             add_batch_sum(batch_sum_lane(mine), accd, accf);

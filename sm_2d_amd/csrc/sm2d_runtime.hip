@@ -1062,17 +1062,16 @@ int sm2d_ctx_create_batch(int device, const sm2d_grid* g, int n_spots, sm2d_ctx*
     return create(device, g, n_spots, out);
 }
 
-int sm2d_stage_clocks(int device, double* out16, int reset) {
+int sm2d_stage_clocks(int device, double* out32, int reset) {
 #ifdef SM2D_STAGE_CLOCKS
-    if (!out16) return err("sm2d_stage_clocks: no output");
+    if (!out32) return err("sm2d_stage_clocks: no output");
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipDeviceSynchronize());
-    unsigned long long v[16];
+    unsigned long long v[32];
     HIPCHK(hipMemcpyFromSymbol(v, HIP_SYMBOL(stage_clk), sizeof(v)));
     int khz = 0;
     HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
-    for (int k = 0; k < 15; ++k) out16[k] = khz > 0 ? (double)v[k] * 1e3 / (double)khz : 0.0;
-    out16[15] = (double)v[15];
+    for (int k = 0; k < 32; ++k) out32[k] = k == 15 ? (double)v[k] : khz > 0 ? (double)v[k] * 1e3 / (double)khz : 0.0;
     if (reset) {
         std::memset(v, 0, sizeof(v));
         HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(stage_clk), v, sizeof(v)));
@@ -1080,7 +1079,7 @@ int sm2d_stage_clocks(int device, double* out16, int reset) {
     return SM2D_OK;
 #else
     (void)device;
-    (void)out16;
+    (void)out32;
     (void)reset;
     return err("stage clocks need the diagnostic build (python -m sm_2d_amd.build --stage-clocks)",
                SM2D_ERR_UNSUPPORTED);
