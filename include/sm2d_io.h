@@ -94,6 +94,14 @@ int sm2d_write_psi_csv(const char* path, int iteration, int decimals, const sm2d
                        const float* theta_edges, const float* E_edges, int n_rows,
                        const int32_t* cell, const int32_t* slot, const uint32_t* bid,
                        const int32_t* lidx, const float* w);
+/* Per-cell summary CSV of the same rows (dump_nonzero_cells_to_csv, k1k6_pipeline.cu:29-190):
+ * one line per cell whose FP32 weight sum exceeds 1e-12, in ascending cell order, with the
+ * count of bins above 1e-12, the weighted mean and the range of their bin-centre energies and
+ * angles (theta centres from the uniform bin width), 6 decimals.  Rows whose global bin lies
+ * off the grid are rejected (SM2D_ERR_ARG). */
+int sm2d_write_cells_csv(const char* path, const sm2d_grid* grid, const float* theta_edges,
+                         const float* E_edges, int n_rows, const int32_t* cell, const int32_t* slot,
+                         const uint32_t* bid, const int32_t* lidx, const float* w);
 /* Reads the (cell, slot, bid, lidx, weight) columns of such a file.  *n_rows receives the row
  * count; rows beyond capacity are counted but not stored (call with capacity 0 to size). */
 int sm2d_read_psi_csv(const char* path, int capacity, int32_t* cell, int32_t* slot,

@@ -510,6 +510,71 @@ int sm2d_write_psi_csv(const char* path, int iteration, int decimals, const sm2d
     return std::fclose(f) == 0 ? SM2D_OK : fail(std::string("write failed: ") + path);
 }
 
+int sm2d_write_cells_csv(const char* path, const sm2d_grid* g, const float* te, const float* ee, int n,
+                         const int32_t* cell, const int32_t* slot, const uint32_t* bid, const int32_t* lidx,
+                         const float* w) {
+    if (!path || !g || g->struct_size != sizeof(sm2d_grid) || !te || !ee || n < 0 ||
+        (n > 0 && (!cell || !slot || !bid || !lidx || !w)))
+        return fail("sm2d_write_cells_csv: bad arguments");
+    const int TL = g->n_theta_local, EL = g->n_e_local, XS = g->n_x_sub, ZS = g->n_z_sub;
+    const int LB = TL * EL * XS * ZS;
+    // bin centres as the dump computes them: theta from the uniform width, E from the edges
+    const float t0 = te[0], dth = (te[g->N_theta] - te[0]) / (float)g->N_theta;
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (cell[a] != cell[b]) return cell[a] < cell[b];
+        if (slot[a] != slot[b]) return slot[a] < slot[b];
+        return lidx[a] < lidx[b];
+    });
+    FILE* f = std::fopen(path, "w");
+    if (!f) return fail(std::string("Failed to open ") + path + " for writing");
+    std::fputs("cell,ix,iz,x_mm,z_mm,total_weight,num_slots,mean_E_MeV,mean_theta_rad,min_E_MeV,max_E_MeV,"
+               "min_theta_rad,max_theta_rad\n", f);
+    size_t r = 0;
+    while (r < order.size()) {
+        const int c = cell[order[r]];
+        if (c < 0) {
+            std::fclose(f);
+            return fail("sm2d_write_cells_csv: row out of range");
+        }
+        // one cell: FP32 sums in (slot, lidx) order over the bins above 1e-12
+        float W = 0.f, sE = 0.f, sT = 0.f, mnE = 1e30f, mxE = -1e30f, mnT = 1e30f, mxT = -1e30f;
+        int nz = 0;
+        for (; r < order.size() && cell[order[r]] == c; ++r) {
+            const int k = order[r];
+            if (!(w[k] > 1e-12f)) continue;
+            const int l = lidx[k];
+            if (l < 0 || l >= LB) {
+                std::fclose(f);
+                return fail("sm2d_write_cells_csv: row out of range");
+            }
+            const int tb = (int)(bid[k] & 0xFFF) * TL + l % TL;
+            const int eb = (int)((bid[k] >> 12) & 0xFFF) * EL + (l / TL) % EL;
+            if (tb >= g->N_theta || eb >= g->N_E) {
+                std::fclose(f);
+                return fail("sm2d_write_cells_csv: bin outside the grid");
+            }
+            const float th = t0 + ((float)tb + 0.5f) * dth;
+            const float E = ee[eb] + (ee[eb + 1] - ee[eb]) * 0.5f;
+            W += w[k];
+            ++nz;
+            sE += E * w[k];
+            sT += th * w[k];
+            mnE = std::min(mnE, E);
+            mxE = std::max(mxE, E);
+            mnT = std::min(mnT, th);
+            mxT = std::max(mxT, th);
+        }
+        if (!(W > 1e-12f)) continue;
+        const int ix = c % g->Nx, iz = c / g->Nx;
+        std::fprintf(f, "%d,%d,%d,%.6f,%.6f,%.6f,%d,%.6f,%.6f,%.6f,%.6f,%.6f,%.6f\n", c, ix, iz,
+                     (double)(((float)ix + 0.5f) * g->dx), (double)(((float)iz + 0.5f) * g->dz), (double)W, nz,
+                     (double)(sE / W), (double)(sT / W), (double)mnE, (double)mxE, (double)mnT, (double)mxT);
+    }
+    return std::fclose(f) == 0 ? SM2D_OK : fail(std::string("write failed: ") + path);
+}
+
 int sm2d_read_psi_csv(const char* path, int cap, int32_t* cell, int32_t* slot, uint32_t* bid,
                       int32_t* lidx, float* w, int* n_rows) {
     if (!path || !n_rows || cap < 0 || (cap > 0 && (!cell || !slot || !bid || !lidx || !w)))

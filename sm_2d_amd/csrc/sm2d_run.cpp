@@ -6,12 +6,18 @@
 //   config file, check device memory, build the grids and the range table, run the spot
 //   (injection, K1-K6 loop, MCS post-convolution), report the Bragg peak and write
 //   dose_2d.txt / pdd.txt in the reference's formats.
-// Extra options (none of which the reference has): --device N, --dump-psi writes the
-// injected phase space as debug_ps_raw_iter_00_initial.csv into the output directory.
+// Extra options (none of which the reference has): --device N, and --dump-psi, which writes the
+// phase-space dumps of the reference's debug build (SM2D_ENABLE_DEBUG_DUMPS, gated at run time by
+// transport.debug_dumps) into the output directory: debug_ps_raw_iter_00_initial.csv after the
+// injection (gpu_transport_wrapper.cu:22-120) and, after K4 of iterations 1..10, 50, 100, 150,
+// 200 and 250 (k1k6_pipeline.cu:1925-1941), debug_ps_raw_iter_NN_after_K4.csv and
+// debug_cells_iter_NN_after_K4.csv.  With dumps on, the loop runs in pieces that end at those
+// iterations; split runs give the same results as one run.
 // Errors print "Error: <message>" to stderr and exit with status 1, as the reference does.
 #include <sys/stat.h>
 #include <sys/types.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -126,29 +132,67 @@ int run(const std::string& cfg_path, int device, bool dump_psi) {
         std::printf("Running K1-K6 transport: E0 %g MeV, phase space %d x %d bins, max_iterations %d\n",
                     sp.E0, NT, NE, tp.max_iterations);
 
-    if (dump_psi) {  // the injected phase space in the reference's initial-dump format
-        check(sm2d_inject(ctx, &sp, nullptr), "injection");
-        int n = 0;
-        check(sm2d_psi_dump(ctx, nullptr, 0, nullptr, nullptr, nullptr, nullptr, nullptr, &n), "psi dump");
-        std::vector<int32_t> cell(n), slot(n), lidx(n);
-        std::vector<uint32_t> bid(n);
-        std::vector<float> w(n);
-        check(sm2d_psi_dump(ctx, nullptr, n, cell.data(), slot.data(), bid.data(), lidx.data(), w.data(), &n),
-              "psi dump");
-        const std::string p = out_dir + "/debug_ps_raw_iter_00_initial.csv";
-        check(sm2d_write_psi_csv(p.c_str(), 0, 6, &g, te.data(), ee.data(), n, cell.data(), slot.data(),
-                                 bid.data(), lidx.data(), w.data()),
-              "psi csv");
-        std::printf("  wrote %d raw phase-space rows to %s\n", n, p.c_str());
-    }
-
+    std::vector<float> A(g.Nz), B(g.Nz), st(g.Nz);
+    check(sm2d_fe_precompute(sp.E0, sp.sigma_x, g.Nz, g.dz, A.data(), B.data(), st.data()), "FE moments");
     std::vector<double> edep((size_t)g.Nx * g.Nz, 0.0);
     sm2d_audit au{};
     au.struct_size = sizeof au;
-    check(sm2d_run_spot(ctx, &sp, &tp, edep.data(), &au), "transport");
+    if (dump_psi && cfg.debug_dumps) {
+        // psi_in after injection, then after each dumped iteration (K6 leaves the K4 output there)
+        auto dump = [&](int iter) {
+            int n = 0;
+            check(sm2d_psi_dump(ctx, nullptr, 0, nullptr, nullptr, nullptr, nullptr, nullptr, &n), "psi dump");
+            std::vector<int32_t> cell(n), slot(n), lidx(n);
+            std::vector<uint32_t> bid(n);
+            std::vector<float> w(n);
+            check(sm2d_psi_dump(ctx, nullptr, n, cell.data(), slot.data(), bid.data(), lidx.data(), w.data(), &n),
+                  "psi dump");
+            char name[64];
+            if (iter == 0) std::snprintf(name, sizeof name, "debug_ps_raw_iter_00_initial.csv");
+            else std::snprintf(name, sizeof name, "debug_ps_raw_iter_%02d_after_K4.csv", iter);
+            const std::string p = out_dir + "/" + name;
+            check(sm2d_write_psi_csv(p.c_str(), iter, iter == 0 ? 6 : 8, &g, te.data(), ee.data(), n, cell.data(),
+                                     slot.data(), bid.data(), lidx.data(), w.data()),
+                  "psi csv");
+            std::printf("  DEBUG: Wrote %d raw phase-space rows to %s\n", n, p.c_str());
+            if (iter > 0) {
+                std::snprintf(name, sizeof name, "debug_cells_iter_%02d_after_K4.csv", iter);
+                const std::string q = out_dir + "/" + name;
+                check(sm2d_write_cells_csv(q.c_str(), &g, te.data(), ee.data(), n, cell.data(), slot.data(),
+                                           bid.data(), lidx.data(), w.data()),
+                      "cells csv");
+            }
+        };
+        check(sm2d_reset(ctx), "reset");
+        check(sm2d_fe_upload(ctx, A.data(), B.data()), "FE upload");
+        check(sm2d_inject(ctx, &sp, &au), "injection");
+        dump(0);
+        std::vector<double> tallies((size_t)7 * g.Nx * g.Nz);
+        int done = 0;
+        bool ended = false;
+        while (done < tp.max_iterations && !ended) {
+            // the next dumped iteration: 1..10, 50, 100, 150, 200, 250, then the rest in one piece
+            int next = done < 10 ? done + 1 : (done < 250 ? (done / 50 + 1) * 50 : tp.max_iterations);
+            next = std::min(next, tp.max_iterations);
+            sm2d_audit piece{};
+            piece.struct_size = sizeof piece;
+            check(sm2d_run_iterations(ctx, &tp, next - done, done == 0 ? 1 : 0, tallies.data(), nullptr, &piece),
+                  "transport");
+            au.iterations += piece.iterations;
+            au.W_fail_iters += piece.W_fail_iters;
+            au.max_cell_W_closure = std::max(au.max_cell_W_closure, piece.max_cell_W_closure);
+            ended = piece.terminated != 0;
+            done += piece.iterations;
+            if (!ended && done == next && (done <= 10 || (done <= 250 && done % 50 == 0))) dump(done);
+        }
+        for (size_t i = 0; i < edep.size(); ++i) {
+            edep[i] = tallies[i];
+            au.E_dep += tallies[i];
+        }
+    } else {
+        check(sm2d_run_spot(ctx, &sp, &tp, edep.data(), &au), "transport");
+    }
     // MCS lateral spread as one convolution per depth row (gpu_transport_wrapper.cu:551-616)
-    std::vector<float> A(g.Nz), B(g.Nz), st(g.Nz);
-    check(sm2d_fe_precompute(sp.E0, sp.sigma_x, g.Nz, g.dz, A.data(), B.data(), st.data()), "FE moments");
     check(sm2d_post_convolve(edep.data(), g.Nx, g.Nz, g.dx, st.data(), sp.sigma_x), "post-convolution");
     if (cfg.log_level >= 1)
         std::printf("GPU transport complete: %d iterations, deposited %g MeV, W closure %.3g\n",

@@ -102,6 +102,8 @@ PROTOS = {
     "write_psi_csv": [C.c_char_p, C.c_int, C.c_int, C.POINTER(abi.Grid), abi.fp, abi.fp, C.c_int,
                       abi.i32p, abi.i32p, abi.u32p, abi.i32p, abi.fp],
     "read_psi_csv": [C.c_char_p, C.c_int, abi.i32p, abi.i32p, abi.u32p, abi.i32p, abi.fp, abi.ip],
+    "write_cells_csv": [C.c_char_p, C.POINTER(abi.Grid), abi.fp, abi.fp, C.c_int,
+                        abi.i32p, abi.i32p, abi.u32p, abi.i32p, abi.fp],
     "memory_estimate": [C.POINTER(abi.Grid), C.c_int, C.POINTER(MemoryPlan)],
     "memory_preflight": [C.c_int, C.POINTER(abi.Grid), C.c_int, C.c_float, C.POINTER(MemoryPlan)],
     "ctx_memory": [C.c_void_p, C.POINTER(C.c_uint64)],
@@ -157,6 +159,17 @@ def write_psi_csv(lib, path, rows: dict, grid: abi.Grid, theta_edges, E_edges, i
              te.ctypes.data_as(abi.fp), ee.ctypes.data_as(abi.fp), len(cell),
              cell.ctypes.data_as(abi.i32p), slot.ctypes.data_as(abi.i32p),
              bid.ctypes.data_as(abi.u32p), lidx.ctypes.data_as(abi.i32p), w.ctypes.data_as(abi.fp))
+
+
+def write_cells_csv(lib, path, rows: dict, grid: abi.Grid, theta_edges, E_edges) -> None:
+    """Per-cell summary (debug_cells_iter_NN_<stage>.csv format) of psi_dump rows."""
+    cell, slot, lidx = (_arr(rows[k], np.int32) for k in ("cell", "slot", "lidx"))
+    bid, w = _arr(rows["bid"], np.uint32), _arr(rows["w"], np.float32)
+    te, ee = _arr(theta_edges, np.float32), _arr(E_edges, np.float32)
+    lib.call("write_cells_csv", _b(path), C.byref(grid), te.ctypes.data_as(abi.fp),
+             ee.ctypes.data_as(abi.fp), len(cell), cell.ctypes.data_as(abi.i32p),
+             slot.ctypes.data_as(abi.i32p), bid.ctypes.data_as(abi.u32p),
+             lidx.ctypes.data_as(abi.i32p), w.ctypes.data_as(abi.fp))
 
 
 def read_psi_csv(lib, path) -> dict:

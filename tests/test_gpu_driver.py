@@ -87,11 +87,31 @@ def test_driver_writes_the_oracles_files(hip, orc, tmp_path, layout):
     g, tp, sp = io.config_runtime(hip, cfg)
     tab = orc.make_tables(g.N_theta, g.theta_min, g.theta_max)
     o = OracleEngine(orc, g, tab)
+    A, B, _ = orc.fe_precompute(sp.E0, sp.sigma_x, g.Nz, g.dz)
+    o.upload_fe(A, B)
+    o.reset()
     o.inject(sp)
     _, rows = o.psi_dump()
     io.write_psi_csv(hip, tmp_path / "init.csv", rows, g, tab.theta_edges, tab.E_edges, 0, 6)
     assert (open(tmp_path / "out" / "debug_ps_raw_iter_00_initial.csv").read() ==
             open(tmp_path / "init.csv").read())
+    # after-K4 dumps at the reference's iterations, equal to the oracle's phase space there
+    done, checked = 0, []
+    for it in list(range(1, 11)) + [50, 100]:
+        o.run_iterations(tp, it - done, include_source_terms=(done == 0))
+        done = it
+        raw = tmp_path / "out" / f"debug_ps_raw_iter_{it:02d}_after_K4.csv"
+        cells = tmp_path / "out" / f"debug_cells_iter_{it:02d}_after_K4.csv"
+        _, rows = o.psi_dump()
+        if len(rows["w"]) == 0:  # the transport ended before this iteration: nothing dumped
+            assert not raw.exists() and not cells.exists()
+            break
+        io.write_psi_csv(hip, tmp_path / "k4.csv", rows, g, tab.theta_edges, tab.E_edges, it, 8)
+        io.write_cells_csv(hip, tmp_path / "cells.csv", rows, g, tab.theta_edges, tab.E_edges)
+        assert open(raw).read() == open(tmp_path / "k4.csv").read(), it
+        assert open(cells).read() == open(tmp_path / "cells.csv").read(), it
+        checked.append(it)
+    assert checked[:10] == list(range(1, 11)) and 50 in checked, checked
 
 
 def test_driver_reports_errors(tmp_path):

@@ -12,6 +12,9 @@ Pins:
   raw ψ CSV                 the reference's one-step dumps (tests/golden/one_step_case_*):
                             the initial dump byte for byte from the oracle's injection, the
                             after-K4 dump column by column from its own rows
+  per-cell ψ summary CSV    the reference's debug_cells_iter_NN_after_K4.csv of its 150 MeV run
+                            (iterations 1..10) and of the one-step cases, rewritten from the raw
+                            dumps of the same iterations (tests/golden/multi_step)
   dense memory estimate     the reference's preflight line in output_message.txt:14-22
                             (100 x 320 cells, 36 x 1029 bins: required 5.96 GiB)
 """
@@ -360,6 +363,50 @@ def test_after_k4_dump_format(lib, case, tmp_path):
         fx, fy = x.split(","), y.split(",")
         assert fx[:7] == fy[:7] and fx[8:] == fy[8:]
         assert np.float32(fx[7]) == np.float32(fy[7])
+
+
+def _printed_rows(lib, path):
+    """Rows of a raw dump; a weight printed as 0.00000000 was above the dump's 1e-12 threshold,
+    so it is read back as a tiny positive weight that counts as a bin and adds nothing visible."""
+    rows = io.read_psi_csv(lib, path)
+    rows["w"] = np.where(rows["w"] > 0, rows["w"], np.float32(2e-12)).astype(np.float32)
+    return rows
+
+
+def _same_cells(mine_path, gold_path):
+    a = open(gold_path).read().splitlines()
+    b = open(mine_path).read().splitlines()
+    assert a[0] == b[0] and len(a) == len(b), (len(a), len(b))
+    for x, y in zip(a[1:], b[1:]):
+        fx, fy = x.split(","), y.split(",")
+        assert fx[:5] == fy[:5] and fx[6] == fy[6], (x, y)      # cell, ix, iz, x_mm, z_mm, bins
+        assert fx[9:] == fy[9:], (x, y)                        # min/max E and theta: bin centres
+        # total weight and the weighted means come from weights printed with 8 decimals
+        assert abs(float(fx[5]) - float(fy[5])) <= 1.5e-6, (x, y)
+        assert abs(float(fx[7]) - float(fy[7])) <= 2e-6 * abs(float(fx[7])) + 1.5e-6, (x, y)
+        assert abs(float(fx[8]) - float(fy[8])) <= 1.5e-6, (x, y)
+
+
+@pytest.mark.parametrize("n", range(1, 11))
+def test_cells_dump_matches_reference_multi_step(lib, n, tmp_path):
+    """debug_cells_iter_NN_after_K4.csv of the reference's 150 MeV run, rewritten from the raw
+    dump of the same iteration: cells, positions and bin counts exactly, energy and angle ranges
+    exactly, sums and means within the raw dump's print precision."""
+    from sm_2d_amd import problems
+    g = abi.Grid(**problems._grid(100, 200, 1.0, 1.0, 36, -0.35, 0.35, problems.LAYOUT_REF))
+    tab = lib.make_tables(g.N_theta, g.theta_min, g.theta_max)
+    g.N_E = len(tab.E_rep)
+    rows = _printed_rows(lib, os.path.join(GOLD, "multi_step", f"iter_{n:02d}.csv"))
+    io.write_cells_csv(lib, tmp_path / "c.csv", rows, g, tab.theta_edges, tab.E_edges)
+    _same_cells(tmp_path / "c.csv", os.path.join(GOLD, "multi_step", f"cells_{n:02d}.csv"))
+
+
+@pytest.mark.parametrize("case", ["A", "B", "C", "D"])
+def test_cells_dump_matches_reference_cases(lib, case, tmp_path):
+    g, tab = _case_tables(lib, case)
+    rows = _printed_rows(lib, os.path.join(fixtures.case_dir(case), "debug_ps_raw_iter_01_after_K4.csv"))
+    io.write_cells_csv(lib, tmp_path / "c.csv", rows, g, tab.theta_edges, tab.E_edges)
+    _same_cells(tmp_path / "c.csv", os.path.join(GOLD, "multi_step", f"cells_case_{case}.csv"))
 
 
 def test_psi_csv_skips_tiny_weights_and_sorts(lib, tmp_path):
