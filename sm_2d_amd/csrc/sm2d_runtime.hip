@@ -14,6 +14,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <hip/hip_ext.h>
@@ -121,6 +122,7 @@ struct sm2d_ctx {
     DBuf<uint32_t> gmoved;  // profiling: bucket values moved per (iteration slot, phase) of a chunk
     DBuf<uint8_t> flag;
     DBuf<double> edep, E_cut, E_nuc, E_bnd, win, ein, k4w, k4e, acc_c, acc_f, pe, part, k5v, hist, edep_sum;
+    DBuf<double> tsum;  // tally totals: per-workgroup partial sums, then the seven totals
     DBuf<float> w_cut, w_nuc, w_bnd, edep_f;
     DBuf<sm2d_spot> spots;
     DBuf<InjSample> samples;
@@ -771,6 +773,14 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
     return SM2D_OK;
 }
 
+// A new phase space (injected or loaded) with `no` occupied cells: the first chunk's launches are
+// sized from it, not from the lists the previous run ended with.
+void seed_hints(sm2d_ctx* c, uint32_t no) {
+    const uint32_t h = (uint32_t)std::min<size_t>(c->lcap, std::max<size_t>(64, 2 * (size_t)no));
+    c->hint_occ = std::max<uint32_t>(64, 2 * no);
+    c->hint = c->hint_ph[0] = c->hint_ph[1] = h;
+}
+
 template <class LY>
 int inject_all(sm2d_ctx* c, const sm2d_spot* sp, int n) {
     int nmax = 1;
@@ -795,7 +805,7 @@ int inject_all(sm2d_ctx* c, const sm2d_spot* sp, int n) {
     HIPCHK(hipMemcpyAsync(f, c->flags.p, sizeof(f), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(&no, c->nocc(c->in), sizeof(no), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    c->hint_occ = std::max<uint32_t>(64, 2 * no);
+    seed_hints(c, no);
     int rc = check_flags(c, f);
     if (rc) return rc;
     // represented energy of psi_in (gpu_transport_wrapper.cu:125-180,443-457); per batch
@@ -884,6 +894,46 @@ __global__ void k_edep_reduce(const double* edep, int n_spots, int Ns, double* s
     }
 }
 
+// Totals of the seven cumulative tallies over the first M cells (the spots of the last run), in
+// FP64 and a fixed order: workgroup p of tally k sums its contiguous share of the cells with a
+// fixed tree, then one workgroup adds the TSUM_PARTS partials of each tally in order.
+constexpr int TSUM_PARTS = 512;
+
+__device__ inline double tally_value(const Tally& t, int k, size_t i) {
+    switch (k) {
+        case 0: return t.edep[i];
+        case 1: return (double)t.w_cut[i];
+        case 2: return t.E_cut[i];
+        case 3: return (double)t.w_nuc[i];
+        case 4: return t.E_nuc[i];
+        case 5: return (double)t.w_bnd[i];
+        default: return t.E_bnd[i];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tally_part(Tally t, size_t M, double* part) {
+    __shared__ double sh[256];
+    const int k = (int)blockIdx.y, p = (int)blockIdx.x;
+    const size_t per = (M + TSUM_PARTS - 1) / TSUM_PARTS, i0 = (size_t)p * per, i1 = i0 + per < M ? i0 + per : M;
+    double a = 0.0;
+    for (size_t i = i0 + threadIdx.x; i < i1; i += 256) a += tally_value(t, k, i);
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)k * TSUM_PARTS + p] = sh[0];
+}
+
+__global__ __launch_bounds__(64) void k_tally_final(double* part) {
+    const int k = (int)threadIdx.x;
+    if (k >= 7) return;
+    double a = 0.0;
+    for (int p = 0; p < TSUM_PARTS; ++p) a += part[(size_t)k * TSUM_PARTS + p];
+    part[(size_t)7 * TSUM_PARTS + k] = a;
+}
+
 // Capacities: listed cells per phase <= 48 rows of every spot (24 in a batch, whose spots are
 // not all at their widest at once); occupied cells and destinations <= 5x that; payload slots
 // <= 16 (8 in a batch) per listed cell, at least 64 Ki.  An overflow stops the transport and
@@ -943,6 +993,7 @@ int each_buffer(sm2d_ctx* c, A&& a) {
     HIPCHK(a(c->k5v, NV, BK_OTHER));
     HIPCHK(a(c->hist, (size_t)HROWS * NV, BK_OTHER));
     HIPCHK(a(c->edep_sum, Ns, BK_OTHER));
+    HIPCHK(a(c->tsum, 7 * (size_t)(TSUM_PARTS + 1), BK_OTHER));
     for (DBuf<float>* b : {&c->w_cut, &c->w_nuc, &c->w_bnd}) HIPCHK(a(*b, N, BK_CELL));
     HIPCHK(a(c->spots, S, BK_OTHER));
     HIPCHK(a(c->injacc, S * 6, BK_OTHER));
@@ -1043,6 +1094,39 @@ int upload_fe_for(sm2d_ctx* c, int s, const sm2d_spot& sp, std::vector<float>* s
     HIPCHK(hipMemcpyAsync(c->sx0.p + s, &sp.sigma_x, sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (sigma_total) *sigma_total = st;
+    return SM2D_OK;
+}
+
+// The moments of n spots (host precompute, spread over host threads), then one upload.
+int upload_fe_batch(sm2d_ctx* c, const sm2d_spot* sp, int n) {
+    const int Nz = c->g.Nz;
+    std::vector<float> A((size_t)n * Nz), B((size_t)n * Nz), sx(n);
+    std::vector<int> rcs(n, SM2D_OK);
+    const int nt = std::max(1, std::min<int>({n, 16, (int)std::thread::hardware_concurrency()}));
+    auto work = [&](int t) {
+        std::vector<float> st(Nz);
+        for (int s = t; s < n; s += nt)
+            rcs[s] = sm2d_fe_precompute(sp[s].E0, sp[s].sigma_x, Nz, c->g.dz, A.data() + (size_t)s * Nz,
+                                        B.data() + (size_t)s * Nz, st.data());
+    };
+    if (nt == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t) th.emplace_back(work, t);
+        for (std::thread& x : th) x.join();
+    }
+    for (int s = 0; s < n; ++s) {
+        if (rcs[s]) {  // again on this thread, which keeps the error message
+            std::vector<float> st(Nz);
+            return sm2d_fe_precompute(sp[s].E0, sp[s].sigma_x, Nz, c->g.dz, A.data(), B.data(), st.data());
+        }
+        sx[s] = sp[s].sigma_x;
+    }
+    HIPCHK(hipMemcpyAsync(c->feA.p, A.data(), sizeof(float) * A.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->feB.p, B.data(), sizeof(float) * B.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->sx0.p, sx.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return SM2D_OK;
 }
 
@@ -1297,7 +1381,7 @@ int sm2d_psi_load(sm2d_ctx* c, int n, const int32_t* cell, const int32_t* slot, 
         HIPCHK(hipMemcpyAsync(c->occ[0].p, occ.data(), sizeof(uint32_t) * occ.size(), hipMemcpyHostToDevice, c->stream));
     uint32_t no = (uint32_t)occ.size();
     HIPCHK(hipMemcpyAsync(c->nocc(0), &no, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
-    c->hint_occ = std::max<uint32_t>(64, 2 * no);
+    seed_hints(c, no);
     HIPCHK(hipStreamSynchronize(c->stream));
     return SM2D_OK;
 }
@@ -1483,14 +1567,12 @@ static int finish_audit(sm2d_ctx* c, sm2d_audit* au, const sm2d_audit& it, int n
 
 // Totals of the per-cell tallies of spots [0, n) (FP64, on the host from a device reduction).
 static int tally_totals(sm2d_ctx* c, int n, double* s7) {
-    for (int k = 0; k < 7; ++k) s7[k] = 0.0;
-    std::vector<double> t(7 * (size_t)c->Ns);
-    for (int s = 0; s < n; ++s) {
-        int rc = download_tallies(c, t.data(), s);
-        if (rc) return rc;
-        for (int k = 0; k < 7; ++k)
-            for (int i = 0; i < c->Ns; ++i) s7[k] += t[(size_t)k * c->Ns + i];
-    }
+    k_tally_part<<<dim3(TSUM_PARTS, 7), 256, 0, c->stream>>>(c->tally(), (size_t)n * c->Ns, c->tsum.p);
+    k_tally_final<<<1, 64, 0, c->stream>>>(c->tsum.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(s7, c->tsum.p + (size_t)7 * TSUM_PARTS, sizeof(double) * 7, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return SM2D_OK;
 }
 
@@ -1541,10 +1623,8 @@ int sm2d_run_batch(sm2d_ctx* c, const sm2d_spot* spots, int n, const sm2d_transp
     HIPCHK(hipSetDevice(c->device));
     rc = zero_state(c);
     if (rc) return rc;
-    for (int s = 0; s < n; ++s) {
-        rc = upload_fe_for(c, s, spots[s], nullptr);
-        if (rc) return rc;
-    }
+    rc = upload_fe_batch(c, spots, n);
+    if (rc) return rc;
     rc = dispatch(c, [&](auto ly) { return inject_all<decltype(ly)>(c, spots, n); });
     if (rc) return rc;
     sm2d_audit it;
