@@ -311,7 +311,13 @@ int sm2d_swap(sm2d_ctx* ctx);
  * that class and then uses the fastest (no counterpart in the reference; every shape gives the
  * same results).  The environment variable SM2D_TRANSPORT_WAVES sets the initial value. */
 int sm2d_set_transport_waves(sm2d_ctx* ctx, int waves);
-/* JSON summary of the tuner's measurements (per phase and size class: mean launch time per
+/* K4 gather launch shape: LDS payload rows per destination (16, 8 or 4 KiB; fewer rows let more
+ * wavefronts share a compute unit, a destination needing more adds into global payloads), or 0
+ * for the tuner, which times the three with the transport shapes (replaces the batch sizing role
+ * of the reference's FineBatchPlanner, src/perf/fine_batch_planner.cpp:7-36; every shape gives
+ * the same results). */
+int sm2d_set_gather_rows(sm2d_ctx* ctx, int kib);
+/* JSON summary of the tuner's measurements (per tuned launch and size class: mean launch time per
  * shape and the chosen shape), truncated to `capacity` bytes. */
 int sm2d_tuner_report(sm2d_ctx* ctx, char* buf, int capacity);
 /* Chunks of iterations are captured as HIP graphs and replayed when nothing they depend on has

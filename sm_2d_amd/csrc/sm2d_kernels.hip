@@ -2654,7 +2654,10 @@ __global__ void k_dest(const uint32_t* list, const uint32_t* nlist, int Nx, int 
 // new payloads get their pool slots with one reservation when the destination is done.  The
 // next bucket payload is fetched while the current one is added.  COUNT (profiling launches
 // only) also adds the destination's non-zero bucket values to *moved.
-template <class LY, bool COUNT>
+// KIB: LDS of payload rows per destination (16, 8 or 4 KiB; the tuner's K4 shape).  Fewer rows let
+// more wavefronts share a CU; a destination that needs more rows adds into global payloads, in
+// the same order, with the same results.
+template <class LY, bool COUNT, int KIB>
 __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const uint32_t* __restrict__ dlist,
                                                const uint32_t* __restrict__ ndest,
                                                const uint32_t* __restrict__ ltag,
@@ -2662,7 +2665,7 @@ __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const ui
                                                uint32_t dcap, double* k4drop_w, double* k4drop_E, uint32_t* flags,
                                                uint32_t* moved) {
     if (ctl[C_DONE]) return;
-    constexpr int NLG = 4096 / LY::LB;  // 16 KiB of payload rows
+    constexpr int NLG = KIB * 256 / LY::LB;
     __shared__ float s_rows[NLG * LY::LB];
     const uint32_t tag = 2 * ctl[C_EP] + (uint32_t)fine;
     const int l = lane_id();
@@ -2914,8 +2917,53 @@ __global__ __launch_bounds__(256) void k_audit_tiles(const double* __restrict__ 
     }
 }
 
+// Stage 1c (long lists): per tile of 256 listed cells of a phase, a fixed tree over the cells'
+// accumulators: their NACC sums, then the largest W closure, E closure and learned-width
+// deviation, and the cell-bin updates (in the fine phase; an integer, exact in any order).
+constexpr int NACCP = NACC + 4;
+
+__global__ __launch_bounds__(256) void k_acc_tiles(const double* __restrict__ acc_c, const uint32_t* nc,
+                                                   const double* __restrict__ acc_f, const uint32_t* nf,
+                                                   double* ap, uint32_t tcap, const uint32_t* ctl) {
+    if (ctl[C_DONE]) return;
+    __shared__ double sh[NACCP][256];
+    for (int ph = 0; ph < 2; ++ph) {
+        const double* acc = ph ? acc_f : acc_c;
+        const uint32_t n = ph ? *nf : *nc;
+        const uint32_t ntile = (n + 255) / 256;
+        for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+            const uint32_t i = tile * 256 + threadIdx.x;
+            double v[NACCP];
+            for (int k = 0; k < NACCP; ++k) v[k] = 0.0;
+            if (i < n) {
+                const double* o = acc + (size_t)i * NACC;
+                for (int k = 0; k < NACC; ++k) v[k] = o[k];
+                if (o[A_CLWIN] > 0) v[NACC] = fabs(o[A_CLWIN] - o[A_CLWOUT]) / o[A_CLWIN];
+                if (ph && o[A_CLEIN] > 0) v[NACC + 1] = fabs(o[A_CLEIN] - o[A_CLEOUT]) / o[A_CLEIN];
+                if (ph) {
+                    v[NACC + 2] = o[A_SDEV];
+                    v[NACC + 3] = o[A_UPD];
+                }
+            }
+            for (int k = 0; k < NACCP; ++k) sh[k][threadIdx.x] = v[k];
+            __syncthreads();
+            for (int o = 128; o > 0; o >>= 1) {
+                if ((int)threadIdx.x < o) {
+                    for (int k = 0; k < NACC; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
+                    for (int k = NACC; k < NACC + 3; ++k)
+                        sh[k][threadIdx.x] = fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + o]);
+                    sh[NACC + 3][threadIdx.x] += sh[NACC + 3][threadIdx.x + o];
+                }
+                __syncthreads();
+            }
+            if (threadIdx.x < NACCP) ap[((size_t)ph * tcap + tile) * NACCP + threadIdx.x] = sh[threadIdx.x][0];
+            __syncthreads();
+        }
+    }
+}
+
 // Stage 2 (one workgroup): the tiles in a fixed order, then the per-listed-cell accumulators of
-// both phases in list order.  Output v: [0,NR1) cell sums, [NR1,NR1+NACC) phase sums,
+// both phases in list order (or, for long lists, their tile partials from k_acc_tiles, ap).  Output v: [0,NR1) cell sums, [NR1,NR1+NACC) phase sums,
 // [NR1+NACC] max cell W closure, [NR1+NACC+1] max fine-cell E closure.  With hist != null the
 // vector goes to row (iteration-1) % hist_rows of hist, else to outv.
 __global__ __launch_bounds__(256) void k_audit_final(const double* part, const uint32_t* nin, const uint32_t* nout,
@@ -2924,7 +2972,8 @@ __global__ __launch_bounds__(256) void k_audit_final(const double* part, const u
                                                      uint32_t hist_rows, uint32_t* ctl, int force,
                                                      const double* __restrict__ pe, Psi nxt,
                                                      const uint32_t* __restrict__ nxt_occ, const uint32_t* nxt_n,
-                                                     ListOut L, int drift_check) {
+                                                     ListOut L, int drift_check, const double* __restrict__ ap,
+                                                     uint32_t tcap) {
     if (ctl[C_DONE] && !force) return;
     if (blockIdx.x > 0) {  // clear the next iteration's output buffer (this iteration's input)
         clear_cells(nxt, nxt_occ, *nxt_n, (blockIdx.x - 1) * blockDim.x + threadIdx.x, (gridDim.x - 1) * blockDim.x);
@@ -2944,24 +2993,43 @@ __global__ __launch_bounds__(256) void k_audit_final(const double* part, const u
         for (uint32_t b = threadIdx.x; b < ntile; b += 256)
             for (int k = 0; k < NR1; ++k) r[k] += part[b * NR1 + k];
     }
+    double fupd = 0.0;  // fine-phase cell-bin updates: integers, exact in FP64 in any order
     for (int ph = 0; ph < 2; ++ph) {
         const double* acc = ph ? acc_f : acc_c;
         const uint32_t n = acc ? (ph ? *nf : *nc) : 0u;
+        if (ap && acc) {
+            const uint32_t ntile = (n + 255) / 256;
+            for (uint32_t b = threadIdx.x; b < ntile; b += 256) {
+                const double* o = ap + ((size_t)ph * tcap + b) * NACCP;
+                for (int k = 0; k < NACC; ++k) r[NR1 + k] += o[k];
+                r[NR1 + NACC] = fmax(r[NR1 + NACC], o[NACC]);
+                r[NR1 + NACC + 1] = fmax(r[NR1 + NACC + 1], o[NACC + 1]);
+                r[IDX_SDEV] = fmax(r[IDX_SDEV], o[NACC + 2]);
+                fupd += o[NACC + 3];
+            }
+            continue;
+        }
         for (uint32_t i = threadIdx.x; i < n; i += 256) {
             const double* o = acc + (size_t)i * NACC;
             for (int k = 0; k < NACC; ++k) r[NR1 + k] += o[k];
             if (o[A_CLWIN] > 0) r[NR1 + NACC] = fmax(r[NR1 + NACC], fabs(o[A_CLWIN] - o[A_CLWOUT]) / o[A_CLWIN]);
             if (ph && o[A_CLEIN] > 0)
                 r[NR1 + NACC + 1] = fmax(r[NR1 + NACC + 1], fabs(o[A_CLEIN] - o[A_CLEOUT]) / o[A_CLEIN]);
-            if (ph) r[IDX_SDEV] = fmax(r[IDX_SDEV], o[A_SDEV]);
+            if (ph) {
+                r[IDX_SDEV] = fmax(r[IDX_SDEV], o[A_SDEV]);
+                fupd += o[A_UPD];
+            }
         }
     }
+    __shared__ double sh_fupd[256];
     for (int k = 0; k < NV; ++k) sh[k][threadIdx.x] = r[k];
+    sh_fupd[threadIdx.x] = fupd;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) {
             for (int k = 0; k < NR1 + NACC; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
             for (int k = NR1 + NACC; k < NV; ++k) sh[k][threadIdx.x] = fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + o]);
+            sh_fupd[threadIdx.x] += sh_fupd[threadIdx.x + o];
         }
         __syncthreads();
     }
@@ -2981,12 +3049,7 @@ __global__ __launch_bounds__(256) void k_audit_final(const double* part, const u
             ctl[C_DRIFT] = 1u;
     }
     if (threadIdx.x == NV) dst[NV] = (double)((acc_c ? *nc : 0u) + (acc_f ? *nf : 0u));
-    if (threadIdx.x == 0) {  // fine-phase updates (integers, exact in FP64 for any realistic count)
-        double u = 0.0;
-        const uint32_t n = acc_f ? *nf : 0u;
-        for (uint32_t i = 0; i < n; ++i) u += acc_f[(size_t)i * NACC + A_UPD];
-        dst[NV + 1] = u;
-    }
+    if (threadIdx.x == 0) dst[NV + 1] = sh_fupd[0];  // fine-phase updates
     if (gridDim.x > 1) {  // open the next iteration once every thread has read the counters
         __syncthreads();
         if (L.occ) {  // the next iteration's lists from the K1 flags of psi_out (k_audit_cells)

@@ -67,34 +67,42 @@ constexpr int CHUNK = 32;           // iterations queued per host read-back
 
 }  // namespace
 
-// Launch-shape tuner of the transport: waves per listed cell (1, 2 or 4; every choice gives the
-// same results).  The choice is made per phase and per size class of the launch (floor(log2) of
-// the listed cells of the previous chunk): the first chunk of a class runs the three shapes on
-// alternating iterations with events around each launch; later chunks use the fastest average.
+// Launch-shape tuner (every choice gives the same results).  Four tuned launches: the coarse and
+// fine transports (kinds 0, 1: waves per listed cell, 1, 2 or 4) and the coarse and fine K4
+// gathers (kinds 2, 3: LDS payload rows per destination, 16, 8 or 4 KiB).  The choice is made per
+// kind and per size class of the launch (floor(log2) of the phase's listed cells in the previous
+// chunk): the first chunk of a class runs the three shapes on alternating iterations with events
+// on each launch; later chunks use the fastest average.
 struct Tuner {
+    static constexpr int NK = 4;
     static constexpr int NC = 3;
     static constexpr int NB = 24;
-    double ms[2][NB][NC] = {};
-    int cnt[2][NB][NC] = {};
-    bool done[2][NB] = {};
+    double ms[NK][NB][NC] = {};
+    int cnt[NK][NB][NC] = {};
+    bool done[2][NB] = {};    // per phase: both of its kinds explored
     int forced = 0;           // 0 = tuned, else the fixed waves per cell
+    int forced_kib = 0;       // 0 = tuned, else the fixed gather rows (KiB)
     static int tw(int c) { return c == 0 ? 1 : c == 1 ? 2 : 4; }
+    static int kib(int c) { return c == 0 ? 16 : c == 1 ? 8 : 4; }
+    // the shape kept on near-ties: four waves per cell, 16 KiB of gather rows
+    static int dflt(int kind) { return kind < 2 ? 2 : 0; }
     static int bucket(uint32_t listed) {
         int b = 0;
         while (b + 1 < NB && (2u << b) <= listed) ++b;
         return b;
     }
-    // The fastest shape; four waves (the default) when it is within 5 % of the fastest, so that
-    // near-ties do not flip with timing noise from one process to the next.
-    int best(int ph, int b) const {
-        int bc = 2;
+    // The fastest shape; the default when it is within 5 % of the fastest, so that near-ties do
+    // not flip with timing noise from one process to the next.
+    int best(int kind, int b) const {
+        const int d = dflt(kind);
+        int bc = d;
         double bt = 1e300;
         for (int c = 0; c < NC; ++c)
-            if (cnt[ph][b][c] && ms[ph][b][c] / cnt[ph][b][c] < bt) {
-                bt = ms[ph][b][c] / cnt[ph][b][c];
+            if (cnt[kind][b][c] && ms[kind][b][c] / cnt[kind][b][c] < bt) {
+                bt = ms[kind][b][c] / cnt[kind][b][c];
                 bc = c;
             }
-        if (cnt[ph][b][2] && ms[ph][b][2] / cnt[ph][b][2] <= 1.05 * bt) bc = 2;
+        if (cnt[kind][b][d] && ms[kind][b][d] / cnt[kind][b][d] <= 1.05 * bt) bc = d;
         return bc;
     }
 };
@@ -123,6 +131,7 @@ struct sm2d_ctx {
     DBuf<uint8_t> flag;
     DBuf<double> edep, E_cut, E_nuc, E_bnd, win, ein, k4w, k4e, acc_c, acc_f, pe, part, k5v, hist, edep_sum;
     DBuf<double> tsum;  // tally totals: per-workgroup partial sums, then the seven totals
+    DBuf<double> accpart;  // tile partials of the listed cells' accumulators, per phase
     DBuf<float> w_cut, w_nuc, w_bnd, edep_f;
     DBuf<sm2d_spot> spots;
     DBuf<InjSample> samples;
@@ -137,8 +146,8 @@ struct sm2d_ctx {
     uint32_t hint = 64;       // expected listed cells per phase (launch sizing only)
     uint32_t hint_ph[2] = {64, 64};  // listed cells of the last chunk, coarse / fine
     Tuner tune;
-    int cand[2] = {2, 2};             // transport shape of the iteration being queued
-    std::vector<int> cand_log;        // shapes of the chunk's iterations (2 per iteration)
+    int cand[4] = {2, 2, 0, 0};       // shapes of the iteration being queued (Tuner kinds)
+    std::vector<int> cand_log;        // shapes of the chunk's iterations (4 per iteration)
     uint32_t hint_occ = 64;   // expected occupied cells per buffer (launch sizing only)
     // stage timing (eager launches with events on `stream`)
     // Events are recorded for every queued iteration of a chunk (12 per iteration) and read
@@ -266,6 +275,9 @@ __global__ void k_undo_open(uint32_t* ctl) {
 
 // Lists up to these lengths are built by one workgroup (one launch instead of three).
 constexpr uint32_t SMALL_LIST = 8192, SMALL_WORDS = 16384, SMALL_AUDIT = 8192;
+
+// Tiles of 256 listed cells per phase (k_acc_tiles).
+inline uint32_t acc_tcap(const sm2d_ctx* c) { return c->lcap / 256 + 2; }
 
 // Occupancy bitmap -> ascending occupied-cell list of buffer k.
 void build_occ(sm2d_ctx* c, int k) {
@@ -408,18 +420,26 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
                                                            c->ctl.p, c->flags.p);
         uint32_t* moved = c->prof ? c->gmoved.p + 2 * c->ev_it + fine : nullptr;
         const unsigned gg = blocks_for(4ull * h, 1, 65536);
-        if (c->rec)
-            hipExtLaunchKernelGGL(c->prof ? k_gather<LY, true> : k_gather<LY, false>, dim3(gg), dim3(64), 0u,
-                                  c->stream,
-                                  c->ev[(size_t)c->ev_it * sm2d_ctx::EPI + e0 + 2],
-                                  c->ev[(size_t)c->ev_it * sm2d_ctx::EPI + e0 + 3], 0u, T, c->psi(out),
-                                  c->bkt(fine), (const uint32_t*)c->dlist.p, (const uint32_t*)ndest,
-                                  (const uint32_t*)c->ltag.p, (const uint32_t*)c->lpos.p,
-                                  (const uint32_t*)c->ctl.p, fine, c->dcap, c->k4w.p, c->k4e.p, c->flags.p, moved);
-        else
-            k_gather<LY, false><<<gg, 64, 0, c->stream>>>(T, c->psi(out), c->bkt(fine), c->dlist.p, ndest, c->ltag.p,
-                                                   c->lpos.p, c->ctl.p, fine, c->dcap, c->k4w.p, c->k4e.p,
-                                                   c->flags.p, moved);
+        const int kib = Tuner::kib(c->cand[2 + fine]);
+#define SM2D_GATHER(K)                                                                                        \
+    do {                                                                                                      \
+        if (c->rec)                                                                                           \
+            hipExtLaunchKernelGGL(c->prof ? k_gather<LY, true, K> : k_gather<LY, false, K>, dim3(gg), dim3(64), \
+                                  0u, c->stream, c->ev[(size_t)c->ev_it * sm2d_ctx::EPI + e0 + 2],            \
+                                  c->ev[(size_t)c->ev_it * sm2d_ctx::EPI + e0 + 3], 0u, T, c->psi(out),      \
+                                  c->bkt(fine), (const uint32_t*)c->dlist.p, (const uint32_t*)ndest,          \
+                                  (const uint32_t*)c->ltag.p, (const uint32_t*)c->lpos.p,                     \
+                                  (const uint32_t*)c->ctl.p, fine, c->dcap, c->k4w.p, c->k4e.p, c->flags.p,   \
+                                  moved);                                                                     \
+        else                                                                                                  \
+            k_gather<LY, false, K><<<gg, 64, 0, c->stream>>>(T, c->psi(out), c->bkt(fine), c->dlist.p, ndest,  \
+                                                             c->ltag.p, c->lpos.p, c->ctl.p, fine, c->dcap,     \
+                                                             c->k4w.p, c->k4e.p, c->flags.p, moved);            \
+    } while (0)
+        if (kib == 16) SM2D_GATHER(16);
+        else if (kib == 8) SM2D_GATHER(8);
+        else SM2D_GATHER(4);
+#undef SM2D_GATHER
     }
     build_occ(c, out);
     c->mark(10);
@@ -427,9 +447,17 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
     ListOut L = list_out(c, out);
     L.skip = skip;
     if (!(open_next && compact_next)) L.occ = nullptr;
+    // long lists: the listed cells' accumulators in tiles first, so the closing workgroup reads
+    // tile partials instead of every cell's
+    const bool acc_tiles = h > SMALL_AUDIT;
+    if (acc_tiles)
+        k_acc_tiles<<<blocks_for(h, 256, 4096), 256, 0, c->stream>>>(c->acc_c.p, c->ctl.p + C_NCRS, c->acc_f.p,
+                                                                     c->ctl.p + C_NACT, c->accpart.p, acc_tcap(c),
+                                                                     c->ctl.p);
     k_audit_final<<<open_next ? 1 + gm : 1, 256, 0, c->stream>>>(
         c->part.p, c->nocc(in), c->nocc(out), c->acc_c.p, c->ctl.p + C_NCRS, c->acc_f.p, c->ctl.p + C_NACT, nullptr,
-        c->hist.p, HROWS, c->ctl.p, 0, pe, c->psi(in), c->occ[in].p, c->nocc(in), L, P.surr);
+        c->hist.p, HROWS, c->ctl.p, 0, pe, c->psi(in), c->occ[in].p, c->nocc(in), L, P.surr,
+        acc_tiles ? c->accpart.p : nullptr, acc_tcap(c));
     c->mark(11);
     c->in = out;  // K6
 }
@@ -441,7 +469,7 @@ int buffer_totals(sm2d_ctx* c, int k, double& W, double& E) {
     ListOut none{};
     k_audit_final<<<1, 256, 0, c->stream>>>(c->part.p, nullptr, c->nocc(k), nullptr, nullptr, nullptr, nullptr,
                                             c->k5v.p, nullptr, 1, c->ctl.p, 1, pe, Psi{}, nullptr, nullptr, none,
-                                            0);
+                                            0, nullptr, 0u);
     HIPCHK(hipGetLastError());
     double v[NV];
     HIPCHK(hipMemcpyAsync(v, c->k5v.p, sizeof(v), hipMemcpyDeviceToHost, c->stream));
@@ -543,8 +571,7 @@ int launch_chunk_graph(sm2d_ctx* c, const Tp& P, int R, bool first_opens, bool f
     key_put(key, c->in);
     key_put(key, (int)first_opens);
     key_put(key, (int)fused_open);
-    key_put(key, c->cand[0]);
-    key_put(key, c->cand[1]);
+    for (int kind = 0; kind < Tuner::NK; ++kind) key_put(key, c->cand[kind]);
     key_put(key, pow2_at_least(std::max<uint32_t>(c->hint, 64)));
     key_put(key, pow2_at_least(c->hint_occ));
     key_put(key, pow2_at_least(std::max<uint32_t>(c->hint_ph[0], 64)));
@@ -635,16 +662,23 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
         }
         c->rec = c->prof || explore[0] || explore[1];
         if (c->prof) HIPCHK(hipMemsetAsync(c->gmoved.p, 0, sizeof(uint32_t) * 2 * CHUNK, c->stream));
-        c->cand_log.assign((size_t)2 * R, 0);
+        c->cand_log.assign((size_t)Tuner::NK * R, 0);
+        // shape of tuned launch `kind` (phase kind & 1) in chunk iteration r
+        auto shape = [&](int kind, int r) {
+            const int ph = kind & 1;
+            if (kind >= 2 && c->tune.forced_kib)
+                return c->tune.forced_kib == 16 ? 0 : c->tune.forced_kib == 8 ? 1 : 2;
+            if (c->tune.forced) return kind < 2 ? (c->tune.forced == 1 ? 0 : c->tune.forced == 2 ? 1 : 2) : 0;
+            if (bkt[ph] == 0) return kind < 2 ? 0 : Tuner::dflt(kind);
+            if (explore[ph]) return r % Tuner::NC;
+            return c->tune.best(kind, bkt[ph]);
+        };
         if (!c->rec && c->graphs_on) {
             // the same shapes for every iteration of the chunk (no exploration)
-            for (int ph = 0; ph < 2; ++ph) {
-                int cd;
-                if (c->tune.forced) cd = c->tune.forced == 1 ? 0 : c->tune.forced == 2 ? 1 : 2;
-                else if (bkt[ph] == 0) cd = 0;
-                else cd = c->tune.best(ph, bkt[ph]);
-                c->cand[ph] = cd;
-                for (int r = 0; r < R; ++r) c->cand_log[(size_t)2 * r + ph] = cd;
+            for (int kind = 0; kind < Tuner::NK; ++kind) {
+                const int cd = shape(kind, 0);
+                c->cand[kind] = cd;
+                for (int r = 0; r < R; ++r) c->cand_log[(size_t)Tuner::NK * r + kind] = cd;
             }
             uint32_t skip = 0;
             if (fused_open && queued > 0 && !paused_once) {
@@ -656,14 +690,10 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
             const int rc = launch_chunk_graph<LY>(c, P, R, fused_open ? queued == 0 : true, fused_open, skip);
             if (rc) return rc;
         } else for (int r = 0; r < R; ++r) {
-            for (int ph = 0; ph < 2; ++ph) {
-                int cd;
-                if (c->tune.forced) cd = c->tune.forced == 1 ? 0 : c->tune.forced == 2 ? 1 : 2;
-                else if (bkt[ph] == 0) cd = 0;
-                else if (explore[ph]) cd = r % Tuner::NC;
-                else cd = c->tune.best(ph, bkt[ph]);
-                c->cand[ph] = cd;
-                c->cand_log[(size_t)2 * r + ph] = cd;
+            for (int kind = 0; kind < Tuner::NK; ++kind) {
+                const int cd = shape(kind, r);
+                c->cand[kind] = cd;
+                c->cand_log[(size_t)Tuner::NK * r + kind] = cd;
             }
             c->ev_it = r;
             // fail-fast keeps psi_in of a failing iteration, so there every iteration opens itself
@@ -724,18 +754,21 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
         for (int ph = 0; ph < 2; ++ph) {
             if (!explore[ph]) continue;
             const int e0 = ph ? 6 : 2;
-            for (int it = std::max(audited, first_it); it < upto; ++it) {
-                const int r = it - first_it;
-                float ms = 0.0f;
-                if (hipEventElapsedTime(&ms, c->ev[(size_t)r * sm2d_ctx::EPI + e0],
-                                        c->ev[(size_t)r * sm2d_ctx::EPI + e0 + 1]) != hipSuccess)
-                    continue;
-                const int cd = c->cand_log[(size_t)2 * r + ph];
-                c->tune.ms[ph][bkt[ph]][cd] += ms;
-                c->tune.cnt[ph][bkt[ph]][cd] += 1;
-            }
             bool all = true;
-            for (int cd = 0; cd < Tuner::NC; ++cd) all = all && c->tune.cnt[ph][bkt[ph]][cd] >= 2;
+            for (int kind = ph; kind < Tuner::NK; kind += 2) {  // the phase's transport, then its gather
+                const int e = e0 + (kind < 2 ? 0 : 2);
+                for (int it = std::max(audited, first_it); it < upto; ++it) {
+                    const int r = it - first_it;
+                    float ms = 0.0f;
+                    if (hipEventElapsedTime(&ms, c->ev[(size_t)r * sm2d_ctx::EPI + e],
+                                            c->ev[(size_t)r * sm2d_ctx::EPI + e + 1]) != hipSuccess)
+                        continue;
+                    const int cd = c->cand_log[(size_t)Tuner::NK * r + kind];
+                    c->tune.ms[kind][bkt[ph]][cd] += ms;
+                    c->tune.cnt[kind][bkt[ph]][cd] += 1;
+                }
+                for (int cd = 0; cd < Tuner::NC; ++cd) all = all && c->tune.cnt[kind][bkt[ph]][cd] >= 2;
+            }
             if (all) c->tune.done[ph][bkt[ph]] = true;
         }
         audited = upto;
@@ -994,6 +1027,7 @@ int each_buffer(sm2d_ctx* c, A&& a) {
     HIPCHK(a(c->hist, (size_t)HROWS * NV, BK_OTHER));
     HIPCHK(a(c->edep_sum, Ns, BK_OTHER));
     HIPCHK(a(c->tsum, 7 * (size_t)(TSUM_PARTS + 1), BK_OTHER));
+    HIPCHK(a(c->accpart, 2 * (size_t)acc_tcap(c) * NACCP, BK_OTHER));
     for (DBuf<float>* b : {&c->w_cut, &c->w_nuc, &c->w_bnd}) HIPCHK(a(*b, N, BK_CELL));
     HIPCHK(a(c->spots, S, BK_OTHER));
     HIPCHK(a(c->injacc, S * 6, BK_OTHER));
@@ -1678,6 +1712,13 @@ int sm2d_set_transport_waves(sm2d_ctx* c, int waves) {
     return SM2D_OK;
 }
 
+int sm2d_set_gather_rows(sm2d_ctx* c, int kib) {
+    if (!c) return err("null context");
+    if (kib != 0 && kib != 4 && kib != 8 && kib != 16) return err("gather rows must be 0, 4, 8 or 16 KiB");
+    c->tune.forced_kib = kib;
+    return SM2D_OK;
+}
+
 int sm2d_set_graphs(sm2d_ctx* c, int on) {
     if (!c) return err("null context");
     HIPCHK(hipSetDevice(c->device));
@@ -1690,22 +1731,31 @@ int sm2d_set_graphs(sm2d_ctx* c, int on) {
 int sm2d_tuner_report(sm2d_ctx* c, char* buf, int cap) {
     if (!c || !buf || cap <= 0) return err("sm2d_tuner_report: null argument");
     std::string o = "{";
-    const char* names[2] = {"coarse", "fine"};
-    for (int ph = 0; ph < 2; ++ph) {
-        o += std::string(ph ? ", " : "") + "\"" + names[ph] + "\": {";
+    const char* names[Tuner::NK] = {"coarse", "fine", "coarse_gather", "fine_gather"};
+    for (int kind = 0; kind < Tuner::NK; ++kind) {
+        o += std::string(kind ? ", " : "") + "\"" + names[kind] + "\": {";
         bool first = true;
         for (int b = 0; b < Tuner::NB; ++b) {
             bool any = false;
-            for (int cd = 0; cd < Tuner::NC; ++cd) any = any || c->tune.cnt[ph][b][cd];
+            for (int cd = 0; cd < Tuner::NC; ++cd) any = any || c->tune.cnt[kind][b][cd];
             if (!any) continue;
             char t[256];
-            std::snprintf(t, sizeof(t), "%s\"%u-%u\": {\"best_waves\": %d", first ? "" : ", ", 1u << b,
-                          (2u << b) - 1u, Tuner::tw(c->tune.best(ph, b)));
+            const int bc = c->tune.best(kind, b);
+            if (kind < 2)
+                std::snprintf(t, sizeof(t), "%s\"%u-%u\": {\"best_waves\": %d", first ? "" : ", ", 1u << b,
+                              (2u << b) - 1u, Tuner::tw(bc));
+            else
+                std::snprintf(t, sizeof(t), "%s\"%u-%u\": {\"best_kib\": %d", first ? "" : ", ", 1u << b,
+                              (2u << b) - 1u, Tuner::kib(bc));
             o += t;
             for (int cd = 0; cd < Tuner::NC; ++cd)
-                if (c->tune.cnt[ph][b][cd]) {
-                    std::snprintf(t, sizeof(t), ", \"us_%dw\": %.2f", Tuner::tw(cd),
-                                  1e3 * c->tune.ms[ph][b][cd] / c->tune.cnt[ph][b][cd]);
+                if (c->tune.cnt[kind][b][cd]) {
+                    if (kind < 2)
+                        std::snprintf(t, sizeof(t), ", \"us_%dw\": %.2f", Tuner::tw(cd),
+                                      1e3 * c->tune.ms[kind][b][cd] / c->tune.cnt[kind][b][cd]);
+                    else
+                        std::snprintf(t, sizeof(t), ", \"us_%dkib\": %.2f", Tuner::kib(cd),
+                                      1e3 * c->tune.ms[kind][b][cd] / c->tune.cnt[kind][b][cd]);
                     o += t;
                 }
             o += "}";

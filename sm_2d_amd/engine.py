@@ -271,6 +271,10 @@ class Engine:
         """Fixes the transport launch shape (1, 2 or 4 waves per listed cell; 0 = tuned)."""
         self.L.call("set_transport_waves", self.h, waves)
 
+    def set_gather_rows(self, kib: int):
+        """Fixes the K4 gather launch shape (16, 8 or 4 KiB of LDS rows per destination; 0 = tuned)."""
+        self.L.call("set_gather_rows", self.h, kib)
+
     def set_graphs(self, on: bool):
         """Captured-and-replayed chunks of iterations (True, the default) or eager launches."""
         self.L.call("set_graphs", self.h, 1 if on else 0)

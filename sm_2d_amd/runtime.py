@@ -20,6 +20,7 @@ HIP_PROTOS = {
                   abi.dp, abi.fp, C.POINTER(abi.Audit)],
     "batch_spot_edep": [C.c_void_p, C.c_int, abi.dp],
     "set_transport_waves": [C.c_void_p, C.c_int],
+    "set_gather_rows": [C.c_void_p, C.c_int],
     "tuner_report": [C.c_void_p, C.c_char_p, C.c_int],
     "set_graphs": [C.c_void_p, C.c_int],
     "mlp_create": [C.c_int, C.c_uint32, C.POINTER(C.c_void_p)],

@@ -379,10 +379,30 @@ def _saturating_psi(rng, g, LB):
     return {k: np.asarray(v) for k, v in rows.items()}
 
 
+@pytest.mark.parametrize("kib", [8, 4])
+@pytest.mark.parametrize("prob", [
+    problems.reference_grid_case(Nx=40, Nz=80, E0=60.0, sigma_x=2.5, n_samples=400, max_iterations=300),
+    problems.config1(n_samples=400),
+], ids=["refgrid60_LB256", "config1_LB128"])
+def test_gather_shapes_bit_exact(hip, orc, prob, kib):
+    """Every launch shape of the K4 gather (LDS rows per destination; destinations with more
+    blocks add into global payloads) reproduces the oracle."""
+    g, o = _pair(hip, orc, prob)
+    g.set_gather_rows(kib)
+    sp, tp = prob.make_spot(), prob.make_transport()
+    eg, ag = g.run_spot(sp, tp)
+    eo, ao = o.run_spot(sp, tp)
+    assert ag.iterations == ao.iterations
+    assert np.array_equal(eg, eo)
+    assert ag.cell_bin_updates == ao.cell_bin_updates
+
+
+@pytest.mark.parametrize("kib", [16, 4])
 @pytest.mark.parametrize("layout", [problems.LAYOUT_REF, problems.LAYOUT_CFG])
-def test_saturated_slot_tables_bit_exact(hip, orc, layout):
+def test_saturated_slot_tables_bit_exact(hip, orc, layout, kib):
     prob = problems.reference_grid_case(Nx=42, Nz=12, layout=layout)
     g, o = _pair(hip, orc, prob)
+    g.set_gather_rows(kib)
     rows = _saturating_psi(np.random.default_rng(5), g.grid, g.LB)
     tp = prob.make_transport(sigma_x_initial=2.0)
     for e in (g, o):
