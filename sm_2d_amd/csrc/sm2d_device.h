@@ -69,7 +69,10 @@ struct Tabs {
 // uses the plain physics.
 enum Ctl {
     C_NACT = 0, C_NCRS = 1, C_NDEST = 2, C_DONE = 3, C_EP = 4, C_IT = 5, C_DONE_IT = 6, C_DRIFT = 7,
-    C_NOCC0 = 8, C_NOCC1 = 9, C_NDEST1 = 10, C_BPC0 = 11, C_BPC1 = 12, C_FNT = 13, NCTL = 16
+    C_NOCC0 = 8, C_NOCC1 = 9, C_NDEST1 = 10, C_FNT = 13, NCTL = 16,
+    // the bucket pools' shard counters (coarse, then fine phase) follow the control words in the
+    // same allocation, from word C_BPC on, then the destination lists' segment counters (C_DSEG)
+    C_BPC = 32, C_DSEG = C_BPC + 2 * 64 * 32, C_CTL_WORDS = C_DSEG + 2 * 64 * 32
 };
 
 // Run-time monitor of the learned physics: in every eighth cell (cell index within its spot
@@ -86,12 +89,33 @@ struct Tp {
     int surr;  // fine transport takes the straggling, scattering and attenuation from the table
 };
 
+// Payload pools are bump allocators split into NSH shards of pcap / NSH rows, each counted on a
+// 128-byte line of its own (pcount[s * SH_STRIDE]).  Atomics on one address serialize (about
+// 11 ns apiece on MI355X, measured), and a batched launch reserves rows for tens of thousands of
+// cells, so a workgroup takes its rows from shard (hint mod NSH) and passes to the next shard
+// when that one is full.  Row positions never enter a result.
+constexpr int NSH = 64, SH_STRIDE = 32;
+static_assert(C_DSEG == C_BPC + 2 * NSH * SH_STRIDE && C_CTL_WORDS == C_DSEG + 2 * NSH * SH_STRIDE,
+              "control-block layout of the shard counters");
+constexpr uint32_t POOL_FULL = 0xFFFFFFFFu;
+
+// First row of n consecutive rows of the pool, or POOL_FULL when no shard has n rows left.
+__device__ inline uint32_t pool_take(uint32_t* pcount, uint32_t pcap, uint32_t n, uint32_t hint) {
+    const uint32_t span = pcap / NSH;
+    for (uint32_t k = 0; k < (uint32_t)NSH; ++k) {
+        const uint32_t sh = (hint + k) % NSH;
+        const uint32_t b = atomicAdd(pcount + sh * SH_STRIDE, n);
+        if (b + n <= span) return sh * span + b;
+    }
+    return POOL_FULL;
+}
+
 struct Psi {
     uint32_t* bid;
     uint32_t* ns;
     uint32_t* sref;
     float* pool;
-    uint32_t* pcount;
+    uint32_t* pcount;  // NSH shard counters (stride SH_STRIDE)
     uint32_t pcap;
 };
 
@@ -100,7 +124,7 @@ struct Bkt {
     uint32_t* n;     // [cap*4]
     uint32_t* sref;  // [cap*4*32]
     float* pool;
-    uint32_t* pcount;
+    uint32_t* pcount;  // NSH shard counters (stride SH_STRIDE)
     uint32_t pcap;
     uint32_t* ftag;   // [cap] phase tag of a listed cell whose +z bucket takes other cells' lateral emissions
     uint32_t* flist;  // [cap] list positions of those cells (count in ctl[C_FNT])

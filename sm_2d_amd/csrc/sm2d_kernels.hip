@@ -132,20 +132,15 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
 
 // Opens an iteration: epoch and iteration counters live on the device so that a chunk of
 // iterations can be queued (or replayed from a graph) without host round trips.
-__global__ void k_begin(uint32_t* ctl) {
-    if (threadIdx.x == 0 && blockIdx.x == 0 && !ctl[C_DONE]) {
-        ctl[C_EP] += 1;
-        ctl[C_IT] += 1;
-        ctl[C_NDEST] = ctl[C_NDEST1] = 0;
-        ctl[C_BPC0] = ctl[C_BPC1] = 0;
-    }
-}
-
 __device__ inline void open_iteration(uint32_t* ctl) {
     ctl[C_EP] += 1;
     ctl[C_IT] += 1;
     ctl[C_NDEST] = ctl[C_NDEST1] = 0;
-    ctl[C_BPC0] = ctl[C_BPC1] = 0;
+    for (int s = 0; s < 4 * NSH; ++s) ctl[C_BPC + s * SH_STRIDE] = 0;  // bucket pools, destination lists
+}
+
+__global__ void k_begin(uint32_t* ctl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !ctl[C_DONE]) open_iteration(ctl);
 }
 
 // Resets the slot tables of the cells on an occupied-cell list and the buffer's payload pool
@@ -157,7 +152,7 @@ __device__ inline void clear_cells(Psi p, const uint32_t* __restrict__ occ, uint
         for (uint32_t s = 0; s < k; ++s) p.bid[(size_t)c * KB + s] = EMPTY;
         p.ns[c] = 0;
     }
-    if (t == 0) *p.pcount = 0;
+    for (uint32_t s = t; s < (uint32_t)NSH; s += nthr) p.pcount[s * SH_STRIDE] = 0;
 }
 
 // Opens an iteration (as k_begin, from the first thread) and clears the output buffer over its
@@ -659,9 +654,9 @@ __device__ inline int table_find_or_claim(uint32_t& mybid, uint32_t& myref, uint
     if (n < (uint32_t)KB) {
         int s = (int)n;
         uint32_t ref = 0;
-        if (l == 0) ref = atomicAdd(pcount, 1u);
+        if (l == 0) ref = pool_take(pcount, pcap, 1u, blockIdx.x);
         ref = rlu(ref, 0);
-        if (ref >= pcap) {
+        if (ref == POOL_FULL) {
             if (l == 0) atomicOr(&flags[flag_full], 1u);
             return -1;
         }
@@ -820,9 +815,9 @@ struct WTabs {
                 used = used + 1;
             } else {
                 ref = 0;
-                if (l == 0) ref = atomicAdd(gpcount, 1u);
+                if (l == 0) ref = pool_take(gpcount, gpcap, 1u, blockIdx.x);
                 ref = rlu(ref, 0);
-                if (ref >= gpcap) {
+                if (ref == POOL_FULL) {
                     if (l == 0) atomicOr(&flags[flag_full], 1u);
                     return -1;
                 }
@@ -980,13 +975,13 @@ __device__ inline void flush_tables(WTabs<LY>& W, Psi& out, Bkt& bk, int cell, u
         mf[f] = __ballot(l < 32 && (uint32_t)l < W.gn(1 + f) && (W.gr(1 + f) & CACHED));
         cb += (uint32_t)__popcll(mf[f]);
     }
-    // both pool reservations are issued before either result is used
-    uint32_t base = 0, bbase = 0;
-    if (l == 0 && co) base = atomicAdd(out.pcount, co);
-    if (l == 1 && cb) bbase = atomicAdd(bk.pcount, cb);
-    base = rlu(base, 0);
-    bbase = rlu(bbase, 1);
-    bool ok = !co || base + co <= out.pcap;
+    // both pool reservations in one pass: lane 0 psi_out's, lane 1 the buckets'
+    uint32_t take = 0;
+    if ((l == 0 && co) || (l == 1 && cb))
+        take = pool_take(l == 0 ? out.pcount : bk.pcount, l == 0 ? out.pcap : bk.pcap, l == 0 ? co : cb,
+                         blockIdx.x);
+    const uint32_t base = rlu(take, 0), bbase = rlu(take, 1);
+    bool ok = !co || base != POOL_FULL;
     if (!ok && l == 0) atomicOr(&flags[F_PSI_FULL], 1u);
     if (ok) {
         uint32_t ref = W.r0;
@@ -1005,8 +1000,7 @@ __device__ inline void flush_tables(WTabs<LY>& W, Psi& out, Bkt& bk, int cell, u
         if (l == 0) out.ns[cell] = W.n0;
     }
     // buckets of faces 0..3
-    base = bbase;
-    ok = !cb || base + cb <= bk.pcap;
+    ok = !cb || bbase != POOL_FULL;
     if (!ok && l == 0) atomicOr(&flags[F_BKT_FULL], 1u);
     uint32_t k0 = 0;
 #pragma unroll
@@ -1014,7 +1008,7 @@ __device__ inline void flush_tables(WTabs<LY>& W, Psi& out, Bkt& bk, int cell, u
         const uint32_t nf = ok ? W.gn(1 + f) : 0u;
         if (ok) {
             uint32_t ref = W.gr(1 + f);
-            if ((mf[f] >> l) & 1ull) ref = base + k0 + (uint32_t)__popcll(mf[f] & below);
+            if ((mf[f] >> l) & 1ull) ref = bbase + k0 + (uint32_t)__popcll(mf[f] & below);
             for (uint64_t m = mf[f]; m; m &= m - 1) {
                 const int s = __ffsll((unsigned long long)m) - 1;
                 const float* src = W.rows + (size_t)(rlu(W.gr(1 + f), s) & ~CACHED) * LY::LB;
@@ -2191,33 +2185,13 @@ __device__ void apply_foreign(const Tabs& T, const Tp& P, CellGeo G, const Psi& 
 // choices give identical results.
 constexpr int QCAP = 512;  // non-zero components queued per round (at least one slot's worth)
 
-// Destination-set output of a transport phase.
-struct Dest {
-    uint32_t* dtag;    // per-cell tag of the phase that listed it as a destination
-    uint32_t* dlist;   // destinations, in first-marked order
-    uint32_t* ndest;   // their count
-    uint32_t cap;      // capacity of dlist
-    uint32_t* bitmap;  // occupancy bitmap of psi_out
-};
-
-__device__ inline void dest_mark(const Dest& D, uint32_t d, uint32_t tag, uint32_t* flags) {
-    const uint32_t old = D.dtag[d];
-    if (old == tag) return;
-    if (atomicCAS(&D.dtag[d], old, tag) == old) {
-        const uint32_t p = atomicAdd(D.ndest, 1u);
-        if (p < D.cap) D.dlist[p] = d;
-        else atomicOr(&flags[F_DEST_FULL], 1u);
-        atomicOr(&D.bitmap[d >> 5], 1u << (d & 31));
-    }
-}
-
 template <class LY, bool FINE, int TW>
 __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_transport(Tabs Tg, Tp P, Psi in, Psi out, Bkt bk,
                                                    const uint32_t* __restrict__ list,
                                                    const uint32_t* __restrict__ nlist,
                                                    const uint32_t* __restrict__ ltag,
                                                    const uint32_t* __restrict__ lpos, uint32_t* ctl,
-                                                   Tally tal, double* acc, uint32_t* flags, Dest D) {
+                                                   Tally tal, double* acc, uint32_t* flags) {
     if (ctl[C_DONE] || blockIdx.x >= *nlist) return;  // no cell for this workgroup
     // LDS payload rows: 4-wave workgroups (few listed cells, one workgroup per CU) keep the full
     // set (24 KiB); 2-wave ones 12 KiB and 1-wave ones 14 KiB, so that three (four) of them fit
@@ -2314,13 +2288,6 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
         if (wv == 0) {
             if (l < 4) tal_d = (l == 0 ? tal.edep : l == 1 ? tal.E_cut : l == 2 ? tal.E_nuc : tal.E_bnd)[cell];
             else if (l < 7) tal_f = (l == 4 ? tal.w_cut : l == 5 ? tal.w_nuc : tal.w_bnd)[cell];
-        }
-        if (wv == 0 && l < 4) {  // the cell's neighbours join the phase's destination set
-            const int d = l == 0   ? (G.iz + 1 < T.Nz ? cell + G.Nx : -1)
-                          : l == 1 ? (G.iz > 0 ? cell - G.Nx : -1)
-                          : l == 2 ? (G.ix + 1 < G.Nx ? cell + 1 : -1)
-                                   : (G.ix > 0 ? cell - 1 : -1);
-            if (d >= 0) dest_mark(D, (uint32_t)d, tag, flags);
         }
         if (threadIdx.x == 0) {
             s_surr = 0;
@@ -2619,50 +2586,59 @@ __global__ __launch_bounds__(64) void k_foreign(Tabs T, Tp P, Psi in, Bkt bk, co
 }
 
 // Destination set of a phase: the four neighbours of every listed cell, deduplicated with a
-// per-cell tag (the order of the resulting list does not affect any result).  The transport
-// kernel marks them as it takes each cell (dest_mark); k_dest is the stand-alone form.
-__global__ void k_dest(const uint32_t* list, const uint32_t* nlist, int Nx, int Nz, int Ns, uint32_t* dtag,
-                       const uint32_t* ctl, int fine, uint32_t* dlist, uint32_t* ndest, uint32_t cap,
-                       uint32_t* bitmap, uint32_t* flags) {
+// per-cell tag and listed in NSH segments of seg_cap entries, each with a counter on a line of
+// its own (seg[s * SH_STRIDE]); destination d goes to the segment of a multiplicative hash of d
+// (columns and rows of cells spread over all segments), so the segments fill evenly at every
+// size and the reservations of a wavefront hit different lines.  The occupancy
+// bit of psi_out is set here.  The order of the lists affects no result.
+static_assert(NSH == 64, "k_dest hashes cells to 64 segments");
+__global__ __launch_bounds__(256) void k_dest(const uint32_t* __restrict__ list, const uint32_t* nlist, int Nx,
+                                              int Nz, int Ns, uint32_t* dtag, const uint32_t* ctl, int fine,
+                                              uint32_t* dlist, uint32_t* seg, uint32_t seg_cap, uint32_t* bitmap,
+                                              uint32_t* flags) {
     if (ctl[C_DONE]) return;
     const uint32_t n = *nlist;
     const uint32_t tag = 2 * ctl[C_EP] + (uint32_t)fine;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int c = (int)list[i];
-        const int lc = c % Ns;
-        const int ix = lc % Nx, iz = lc / Nx;
-        int nb[4] = {iz + 1 < Nz ? c + Nx : -1, iz > 0 ? c - Nx : -1, ix + 1 < Nx ? c + 1 : -1,
-                     ix > 0 ? c - 1 : -1};
-        for (int f = 0; f < 4; ++f) {
-            int d = nb[f];
-            if (d < 0) continue;
-            uint32_t old = dtag[d];
-            if (old == tag) continue;
-            if (atomicCAS(&dtag[d], old, tag) == old) {
-                uint32_t p = atomicAdd(ndest, 1u);
-                if (p < cap) dlist[p] = (uint32_t)d;
-                else atomicOr(&flags[F_DEST_FULL], 1u);
-                atomicOr(&bitmap[(uint32_t)d >> 5], 1u << ((uint32_t)d & 31));
+    const uint32_t nthr = gridDim.x * blockDim.x;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < 4u * n; e += nthr) {
+        int d = -1;
+        {
+            const int c = (int)list[e >> 2], lc = c % Ns;
+            const int ix = lc % Nx, iz = lc / Nx;
+            switch (e & 3u) {
+                case 0: d = iz + 1 < Nz ? c + Nx : -1; break;
+                case 1: d = iz > 0 ? c - Nx : -1; break;
+                case 2: d = ix + 1 < Nx ? c + 1 : -1; break;
+                default: d = ix > 0 ? c - 1 : -1; break;
             }
         }
+        if (d < 0) continue;
+        const uint32_t old = dtag[d];
+        if (old == tag || atomicCAS(&dtag[d], old, tag) != old) continue;
+        const uint32_t s = ((uint32_t)d * 2654435761u) >> 26;  // multiplicative hash, NSH = 64
+        const uint32_t p = atomicAdd(seg + s * SH_STRIDE, 1u);
+        if (p < seg_cap) dlist[(size_t)s * seg_cap + p] = (uint32_t)d;
+        else atomicOr(&flags[F_DEST_FULL], 1u);
+        atomicOr(&bitmap[(uint32_t)d >> 5], 1u << ((uint32_t)d & 31));
     }
 }
 
 // K4: gather the neighbours' buckets into psi_out of one destination (faces 0..3, bucket slots
-// in order), one wavefront per destination.  The destination's payloads are staged in LDS rows
+// in order), one wavefront per destination.  Workgroup b reads segment b mod NSH of the
+// destination list (gridDim is a multiple of NSH).  The destination's payloads are staged in LDS rows
 // (existing ones loaded, new ones zeroed), the adds happen in LDS in the serial order, and the
 // new payloads get their pool slots with one reservation when the destination is done.  The
 // next bucket payload is fetched while the current one is added.  COUNT (profiling launches
-// only) also adds the destination's non-zero bucket values to *moved.
+// only) also adds the destination's non-zero bucket values to moved (NSH shard counters).
 // KIB: LDS of payload rows per destination (16, 8 or 4 KiB; the tuner's K4 shape).  Fewer rows let
 // more wavefronts share a CU; a destination that needs more rows adds into global payloads, in
 // the same order, with the same results.
 template <class LY, bool COUNT, int KIB>
 __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const uint32_t* __restrict__ dlist,
-                                               const uint32_t* __restrict__ ndest,
+                                               const uint32_t* __restrict__ seg, uint32_t seg_cap,
                                                const uint32_t* __restrict__ ltag,
                                                const uint32_t* __restrict__ lpos, const uint32_t* ctl, int fine,
-                                               uint32_t dcap, double* k4drop_w, double* k4drop_E, uint32_t* flags,
+                                               double* k4drop_w, double* k4drop_E, uint32_t* flags,
                                                uint32_t* moved) {
     if (ctl[C_DONE]) return;
     constexpr int NLG = KIB * 256 / LY::LB;
@@ -2670,12 +2646,27 @@ __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const ui
     const uint32_t tag = 2 * ctl[C_EP] + (uint32_t)fine;
     const int l = lane_id();
     const uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
-    const uint32_t n = min(*ndest, dcap);  // k_dest flags an overflow; the host reports it
     const int Nx = T.Nx, Nz = T.Nz;
-    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
-        const int d = (int)dlist[i];
+    const uint32_t sgi = blockIdx.x % NSH, sstride = gridDim.x / NSH;
+    const uint32_t n = min(seg[sgi * SH_STRIDE], seg_cap);  // k_dest flags an overflow
+    for (uint32_t i = blockIdx.x / NSH; i < n; i += sstride) {
+        const int d = (int)dlist[(size_t)sgi * seg_cap + i];
         const int lc = d % T.Ns;
         const int ix = lc % Nx, iz = lc / Nx;
+        // the destination's listed neighbours (face f: its bucket f lands here) and their positions
+        int srcs[4];
+        uint32_t pos[4];
+#pragma unroll
+        for (int face = 0; face < 4; ++face) {
+            int src = -1;
+            if (face == 0 && iz > 0) src = d - Nx;
+            if (face == 1 && iz + 1 < Nz) src = d + Nx;
+            if (face == 2 && ix > 0) src = d - 1;
+            if (face == 3 && ix + 1 < Nx) src = d + 1;
+            if (src >= 0 && ltag[src] != tag) src = -1;
+            srcs[face] = src;
+            pos[face] = src >= 0 ? lpos[src] : 0u;
+        }
         WTabs<LY> W;
         W.n0 = out.ns[d];
         W.own0 = W.n0;
@@ -2693,18 +2684,12 @@ __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const ui
         uint32_t fb[4], fr[4], nf[4];
 #pragma unroll
         for (int face = 0; face < 4; ++face) {
-            int src = -1;
-            if (face == 0 && iz > 0) src = d - Nx;
-            if (face == 1 && iz + 1 < Nz) src = d + Nx;
-            if (face == 2 && ix > 0) src = d - 1;
-            if (face == 3 && ix + 1 < Nx) src = d + 1;
             nf[face] = 0;
             fb[face] = EMPTY;
             fr[face] = 0;
-            if (src >= 0 && ltag[src] == tag) {
-                const uint32_t pos = lpos[src];
-                const uint32_t base = (pos * 4 + face) * KB;  // source face == receiving face index
-                nf[face] = bk.n[pos * 4 + face];
+            if (srcs[face] >= 0) {
+                const uint32_t base = (pos[face] * 4 + face) * KB;  // source face == receiving face index
+                nf[face] = bk.n[pos[face] * 4 + face];
                 if (l < 32 && (uint32_t)l < nf[face]) {
                     fb[face] = bk.bid[base + l];
                     fr[face] = bk.sref[base + l];
@@ -2795,10 +2780,10 @@ __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const ui
         const uint32_t co = (uint32_t)__popcll(mo);
         uint32_t base = 0;
         if (co) {
-            if (l == 0) base = atomicAdd(out.pcount, co);
+            if (l == 0) base = pool_take(out.pcount, out.pcap, co, blockIdx.x);
             base = rlu(base, 0);
         }
-        const bool ok = !co || base + co <= out.pcap;
+        const bool ok = !co || base != POOL_FULL;
         if (!ok && l == 0) atomicOr(&flags[F_PSI_FULL], 1u);
         if (ok) {
             uint32_t ref = W.r0;
@@ -2819,7 +2804,7 @@ __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const ui
         if (l == 0) {
             k4drop_w[d] += dw;
             k4drop_E[d] += de;
-            if (COUNT && nmoved) atomicAdd(moved, nmoved);
+            if (COUNT && nmoved) atomicAdd(moved + (blockIdx.x % NSH) * SH_STRIDE, nmoved);
         }
     }
 }

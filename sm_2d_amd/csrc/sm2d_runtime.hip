@@ -127,7 +127,8 @@ struct sm2d_ctx {
     DBuf<uint32_t> bbid, bn, bsref, bftag, bflist;
     DBuf<float> bpool;
     DBuf<uint32_t> maskv, ltag, lpos, dtag, act, crs, dlist, ctl, tcount, tcount2, flags;
-    DBuf<uint32_t> gmoved;  // profiling: bucket values moved per (iteration slot, phase) of a chunk
+    DBuf<uint32_t> gmoved;  // profiling: bucket values moved per (iteration slot, phase) of a chunk,
+                            // in NSH shard counters each
     DBuf<uint8_t> flag;
     DBuf<double> edep, E_cut, E_nuc, E_bnd, win, ein, k4w, k4e, acc_c, acc_f, pe, part, k5v, hist, edep_sum;
     DBuf<double> tsum;  // tally totals: per-workgroup partial sums, then the seven totals
@@ -197,7 +198,7 @@ struct sm2d_ctx {
     Bkt bkt(int fine) const {
         Bkt b;
         b.bid = bbid.p; b.n = bn.p; b.sref = bsref.p; b.pool = bpool.p;
-        b.pcount = ctl.p + (fine ? C_BPC1 : C_BPC0);
+        b.pcount = ctl.p + C_BPC + (fine ? NSH * SH_STRIDE : 0);
         b.pcap = pcap;
         b.ftag = bftag.p;
         b.flist = bflist.p;
@@ -376,13 +377,6 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
         const uint32_t* nl = c->ctl.p + (fine ? C_NACT : C_NCRS);
         double* acc = fine ? c->acc_f.p : c->acc_c.p;
         const int e0 = fine ? 6 : 2;
-        uint32_t* ndest = c->ctl.p + (fine ? C_NDEST1 : C_NDEST);
-        Dest D;
-        D.dtag = c->dtag.p;
-        D.dlist = c->dlist.p;
-        D.ndest = ndest;
-        D.cap = c->dcap;
-        D.bitmap = c->bitmap.p;
         const size_t dyn = transport_dyn_lds(c->g.N_E);
         const int tw = Tuner::tw(c->cand[fine]);
         const int gb = blocks_for(pow2_at_least(std::max<uint32_t>(c->hint_ph[fine], 64)), 1, 65536);
@@ -397,12 +391,12 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
                                   e_start, e_stop, 0u, T, P, c->psi(in), c->psi(out), c->bkt(fine), list, nl, \
                                   (const uint32_t*)c->ltag.p, (const uint32_t*)c->lpos.p, c->ctl.p,           \
                                   c->tally(), acc,                                                            \
-                                  c->flags.p, D);                                                             \
+                                  c->flags.p);                                                                \
         else                                                                                                  \
             k_transport<LY, F, W><<<gb, 64 * W, dyn, c->stream>>>(T, P, c->psi(in), c->psi(out), c->bkt(fine), \
                                                                  list, nl, c->ltag.p, c->lpos.p, c->ctl.p,    \
                                                                  c->tally(),                                  \
-                                                                 acc, c->flags.p, D);                         \
+                                                                 acc, c->flags.p);                            \
     } while (0)
         if (fine) {
             if (tw == 1) SM2D_TRANSPORT(true, 1);
@@ -418,8 +412,13 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
                                                              c->ctl.p, c->flags.p);
         else k_foreign<LY, false><<<1, 64, 0, c->stream>>>(T, P, c->psi(in), c->bkt(fine), list, c->ltag.p,
                                                            c->ctl.p, c->flags.p);
-        uint32_t* moved = c->prof ? c->gmoved.p + 2 * c->ev_it + fine : nullptr;
-        const unsigned gg = blocks_for(4ull * h, 1, 65536);
+        uint32_t* seg = c->ctl.p + C_DSEG + (fine ? NSH * SH_STRIDE : 0);
+        const uint32_t seg_cap = c->dcap / NSH;
+        k_dest<<<blocks_for(4ull * h, 256, 4096), 256, 0, c->stream>>>(list, nl, c->g.Nx, c->g.Nz, c->Ns, c->dtag.p,
+                                                                       c->ctl.p, fine, c->dlist.p, seg, seg_cap,
+                                                                       c->bitmap.p, c->flags.p);
+        uint32_t* moved = c->prof ? c->gmoved.p + (size_t)(2 * c->ev_it + fine) * NSH * SH_STRIDE : nullptr;
+        const unsigned gg = (blocks_for(4ull * h, 1, 65536) + NSH - 1) / NSH * NSH;
         const int kib = Tuner::kib(c->cand[2 + fine]);
 #define SM2D_GATHER(K)                                                                                        \
     do {                                                                                                      \
@@ -427,13 +426,12 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
             hipExtLaunchKernelGGL(c->prof ? k_gather<LY, true, K> : k_gather<LY, false, K>, dim3(gg), dim3(64), \
                                   0u, c->stream, c->ev[(size_t)c->ev_it * sm2d_ctx::EPI + e0 + 2],            \
                                   c->ev[(size_t)c->ev_it * sm2d_ctx::EPI + e0 + 3], 0u, T, c->psi(out),      \
-                                  c->bkt(fine), (const uint32_t*)c->dlist.p, (const uint32_t*)ndest,          \
+                                  c->bkt(fine), (const uint32_t*)c->dlist.p, (const uint32_t*)seg, seg_cap,   \
                                   (const uint32_t*)c->ltag.p, (const uint32_t*)c->lpos.p,                     \
-                                  (const uint32_t*)c->ctl.p, fine, c->dcap, c->k4w.p, c->k4e.p, c->flags.p,   \
-                                  moved);                                                                     \
+                                  (const uint32_t*)c->ctl.p, fine, c->k4w.p, c->k4e.p, c->flags.p, moved);    \
         else                                                                                                  \
-            k_gather<LY, false, K><<<gg, 64, 0, c->stream>>>(T, c->psi(out), c->bkt(fine), c->dlist.p, ndest,  \
-                                                             c->ltag.p, c->lpos.p, c->ctl.p, fine, c->dcap,     \
+            k_gather<LY, false, K><<<gg, 64, 0, c->stream>>>(T, c->psi(out), c->bkt(fine), c->dlist.p, seg,      \
+                                                             seg_cap, c->ltag.p, c->lpos.p, c->ctl.p, fine,     \
                                                              c->k4w.p, c->k4e.p, c->flags.p, moved);            \
     } while (0)
         if (kib == 16) SM2D_GATHER(16);
@@ -484,10 +482,14 @@ int buffer_totals(sm2d_ctx* c, int k, double& W, double& E) {
 void collect_timing(sm2d_ctx* c, int n_valid) {
     if (!c->prof) return;
     uint32_t moved[2 * CHUNK] = {};
-    if (n_valid > 0 &&
-        hipMemcpy(moved, c->gmoved.p, sizeof(uint32_t) * 2 * std::min(n_valid, CHUNK), hipMemcpyDeviceToHost) !=
-            hipSuccess)
-        return;
+    if (n_valid > 0) {
+        const size_t per = (size_t)NSH * SH_STRIDE;
+        std::vector<uint32_t> sh((size_t)2 * std::min(n_valid, CHUNK) * per);
+        if (hipMemcpy(sh.data(), c->gmoved.p, sizeof(uint32_t) * sh.size(), hipMemcpyDeviceToHost) != hipSuccess)
+            return;
+        for (size_t k = 0; k < sh.size() / per; ++k)
+            for (int s = 0; s < NSH; ++s) moved[k] += sh[k * per + (size_t)s * SH_STRIDE];
+    }
     for (int r = 0; r < n_valid; ++r) {
         hipEvent_t* e = c->ev.data() + (size_t)r * sm2d_ctx::EPI;
         auto span = [&](int a, int b) -> double {
@@ -661,7 +663,8 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
             explore[ph] = !c->tune.forced && bkt[ph] > 0 && !c->tune.done[ph][bkt[ph]];
         }
         c->rec = c->prof || explore[0] || explore[1];
-        if (c->prof) HIPCHK(hipMemsetAsync(c->gmoved.p, 0, sizeof(uint32_t) * 2 * CHUNK, c->stream));
+        if (c->prof)
+            HIPCHK(hipMemsetAsync(c->gmoved.p, 0, sizeof(uint32_t) * 2 * CHUNK * NSH * SH_STRIDE, c->stream));
         c->cand_log.assign((size_t)Tuner::NK * R, 0);
         // shape of tuned launch `kind` (phase kind & 1) in chunk iteration r
         auto shape = [&](int kind, int r) {
@@ -861,7 +864,7 @@ int zero_state(sm2d_ctx* c) {
     for (int k = 0; k < 2; ++k) {
         HIPCHK(hipMemsetAsync(c->bid[k].p, 0xFF, sizeof(uint32_t) * c->bid[k].n, c->stream));
         HIPCHK(hipMemsetAsync(c->ns[k].p, 0, sizeof(uint32_t) * c->ns[k].n, c->stream));
-        HIPCHK(hipMemsetAsync(c->pcount[k].p, 0, sizeof(uint32_t), c->stream));
+        HIPCHK(hipMemsetAsync(c->pcount[k].p, 0, sizeof(uint32_t) * NSH * SH_STRIDE, c->stream));
     }
     const size_t N = c->N;
     for (DBuf<double>* b : {&c->edep, &c->E_cut, &c->E_nuc, &c->E_bnd, &c->k4w, &c->k4e})
@@ -869,6 +872,7 @@ int zero_state(sm2d_ctx* c) {
     for (DBuf<float>* b : {&c->w_cut, &c->w_nuc, &c->w_bnd}) HIPCHK(hipMemsetAsync(b->p, 0, sizeof(float) * N, c->stream));
     HIPCHK(hipMemsetAsync(c->flags.p, 0, sizeof(uint32_t) * NFLAG, c->stream));
     HIPCHK(hipMemsetAsync(c->bitmap.p, 0, sizeof(uint32_t) * c->nwords, c->stream));
+    HIPCHK(hipMemsetAsync(c->ctl.p + C_BPC, 0, sizeof(uint32_t) * (C_CTL_WORDS - C_BPC), c->stream));
     // control block: counts zero, epoch advanced past every stored tag, word 15 = bitmap words
     uint32_t ctl[NCTL];
     HIPCHK(hipMemcpyAsync(ctl, c->ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, c->stream));
@@ -968,7 +972,8 @@ __global__ __launch_bounds__(64) void k_tally_final(double* part) {
 }
 
 // Capacities: listed cells per phase <= 48 rows of every spot (24 in a batch, whose spots are
-// not all at their widest at once); occupied cells and destinations <= 5x that; payload slots
+// not all at their widest at once); occupied cells and destinations <= 5x that (the destination
+// list in NSH hashed segments, each with room for twice its share and 64 more); payload slots
 // <= 16 (8 in a batch) per listed cell, at least 64 Ki.  An overflow stops the transport and
 // is reported as SM2D_ERR_CAPACITY.
 void plan_caps(sm2d_ctx* c) {
@@ -976,7 +981,7 @@ void plan_caps(sm2d_ctx* c) {
     const size_t rows = S > 1 ? 24 : 48, per_listed = S > 1 ? 8 : 16;
     c->lcap = (uint32_t)std::min<size_t>(N, S * rows * c->g.Nx + 1024);
     c->ocap = (uint32_t)std::min<size_t>(N, 5ull * c->lcap);
-    c->dcap = c->ocap;
+    c->dcap = (uint32_t)((2 * ((size_t)c->ocap / NSH) + 64) * NSH);
     c->pcap = (uint32_t)std::min<size_t>(N * KB, std::max<size_t>(65536, per_listed * c->lcap));
     c->nwords = (uint32_t)((N + 31) / 32);
 }
@@ -994,7 +999,7 @@ int each_buffer(sm2d_ctx* c, A&& a) {
         HIPCHK(a(c->bid[k], N * KB, BK_PSI));
         HIPCHK(a(c->ns[k], N, BK_PSI));
         HIPCHK(a(c->sref[k], N * KB, BK_PSI));
-        HIPCHK(a(c->pcount[k], 1, BK_PSI));
+        HIPCHK(a(c->pcount[k], NSH * SH_STRIDE, BK_PSI));
         HIPCHK(a(c->pool[k], (size_t)c->pcap * c->LB, BK_PSI));
         HIPCHK(a(c->occ[k], c->ocap, BK_PSI));
     }
@@ -1006,11 +1011,11 @@ int each_buffer(sm2d_ctx* c, A&& a) {
     HIPCHK(a(c->bftag, c->lcap, BK_BUCKET));
     HIPCHK(a(c->bflist, c->lcap, BK_BUCKET));
     for (DBuf<uint32_t>* b : {&c->maskv, &c->ltag, &c->lpos, &c->dtag}) HIPCHK(a(*b, N, BK_CELL));
+    HIPCHK(a(c->dlist, (size_t)c->dcap, BK_OTHER));
     HIPCHK(a(c->act, c->ocap, BK_OTHER));
     HIPCHK(a(c->crs, c->ocap, BK_OTHER));
-    HIPCHK(a(c->dlist, c->dcap, BK_OTHER));
-    HIPCHK(a(c->ctl, NCTL, BK_OTHER));
-    HIPCHK(a(c->gmoved, (size_t)2 * CHUNK, BK_OTHER));
+    HIPCHK(a(c->ctl, C_CTL_WORDS, BK_OTHER));
+    HIPCHK(a(c->gmoved, (size_t)2 * CHUNK * NSH * SH_STRIDE, BK_OTHER));
     HIPCHK(a(c->tcount, 2 * ((size_t)c->ocap / 256 + 2), BK_OTHER));
     HIPCHK(a(c->tcount2, (size_t)c->nwords / 256 + 2, BK_OTHER));
     HIPCHK(a(c->flags, NFLAG, BK_OTHER));
@@ -1096,8 +1101,8 @@ int create(int device, const sm2d_grid* g, int n_spots, sm2d_ctx** out) {
     HIPCHK(hipMemsetAsync(c->maskv.p, 0, sizeof(uint32_t) * N, c->stream));
     HIPCHK(hipMemsetAsync(c->bftag.p, 0, sizeof(uint32_t) * c->lcap, c->stream));
     HIPCHK(hipMemsetAsync(c->ltag.p, 0, sizeof(uint32_t) * N, c->stream));
+    HIPCHK(hipMemsetAsync(c->ctl.p, 0, sizeof(uint32_t) * C_CTL_WORDS, c->stream));
     HIPCHK(hipMemsetAsync(c->dtag.p, 0, sizeof(uint32_t) * N, c->stream));
-    HIPCHK(hipMemsetAsync(c->ctl.p, 0, sizeof(uint32_t) * NCTL, c->stream));
     HIPCHK(hipMemsetAsync(c->feA.p, 0, sizeof(float) * n_spots * g->Nz, c->stream));
     HIPCHK(hipMemsetAsync(c->feB.p, 0, sizeof(float) * n_spots * g->Nz, c->stream));
     HIPCHK(hipMemsetAsync(c->sx0.p, 0, sizeof(float) * n_spots, c->stream));
@@ -1400,8 +1405,12 @@ int sm2d_psi_load(sm2d_ctx* c, int n, const int32_t* cell, const int32_t* slot, 
         }
         if (hn[cl]) occ.push_back((uint32_t)cl);
     }
-    uint32_t used = (uint32_t)(pool.size() / LB);
-    if (used > c->pcap) return err("psi_load: pool capacity", SM2D_ERR_CAPACITY);
+    // the loaded rows fill the pool's shards in order (row r lies in shard r / span)
+    const uint32_t used = (uint32_t)(pool.size() / LB), span = c->pcap / NSH;
+    if (used > span * NSH) return err("psi_load: pool capacity", SM2D_ERR_CAPACITY);
+    std::vector<uint32_t> shard(NSH * SH_STRIDE, 0);
+    for (int k = 0; k < NSH; ++k)
+        shard[(size_t)k * SH_STRIDE] = std::min(span, used - std::min(used, (uint32_t)k * span));
     if (occ.size() > c->ocap) return err("psi_load: occupied-cell capacity", SM2D_ERR_CAPACITY);
     rc = zero_state(c);
     if (rc) return rc;
@@ -1410,7 +1419,8 @@ int sm2d_psi_load(sm2d_ctx* c, int n, const int32_t* cell, const int32_t* slot, 
     HIPCHK(hipMemcpyAsync(c->sref[0].p, hs.data(), sizeof(uint32_t) * hs.size(), hipMemcpyHostToDevice, c->stream));
     if (!pool.empty())
         HIPCHK(hipMemcpyAsync(c->pool[0].p, pool.data(), sizeof(float) * pool.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->pcount[0].p, &used, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->pcount[0].p, shard.data(), sizeof(uint32_t) * shard.size(), hipMemcpyHostToDevice,
+                          c->stream));
     if (!occ.empty())
         HIPCHK(hipMemcpyAsync(c->occ[0].p, occ.data(), sizeof(uint32_t) * occ.size(), hipMemcpyHostToDevice, c->stream));
     uint32_t no = (uint32_t)occ.size();
@@ -1427,22 +1437,31 @@ int sm2d_psi_dump(sm2d_ctx* c, uint32_t* bids, int cap, int32_t* cell, int32_t* 
     HIPCHK(hipSetDevice(c->device));
     const size_t N = c->N;
     const int LB = c->LB, k = c->in;
-    std::vector<uint32_t> hb(N * KB), hn(N), hs(N * KB), cnt(1);
+    std::vector<uint32_t> hb(N * KB), hn(N), hs(N * KB), cnt(NSH * SH_STRIDE);
     HIPCHK(hipMemcpyAsync(hb.data(), c->bid[k].p, sizeof(uint32_t) * hb.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(hn.data(), c->ns[k].p, sizeof(uint32_t) * hn.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(hs.data(), c->sref[k].p, sizeof(uint32_t) * hs.size(), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(cnt.data(), c->pcount[k].p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(cnt.data(), c->pcount[k].p, sizeof(uint32_t) * cnt.size(), hipMemcpyDeviceToHost,
+                          c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    uint32_t used = std::min(cnt[0], c->pcap);
-    std::vector<float> pool((size_t)used * LB);
-    if (used)
-        HIPCHK(hipMemcpyAsync(pool.data(), c->pool[k].p, sizeof(float) * pool.size(), hipMemcpyDeviceToHost, c->stream));
+    // the used rows of every shard, packed: row r of shard r / span sits at off[r / span] + r % span
+    const uint32_t span = c->pcap / NSH;
+    std::vector<size_t> off(NSH + 1, 0);
+    for (int s = 0; s < NSH; ++s) off[s + 1] = off[s] + std::min(cnt[(size_t)s * SH_STRIDE], span);
+    std::vector<float> pool(off[NSH] * LB);
+    for (int s = 0; s < NSH; ++s)
+        if (off[s + 1] > off[s])
+            HIPCHK(hipMemcpyAsync(pool.data() + off[s] * LB, c->pool[k].p + (size_t)s * span * LB,
+                                  sizeof(float) * (off[s + 1] - off[s]) * LB, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (bids) std::memcpy(bids, hb.data(), sizeof(uint32_t) * hb.size());
     int m = 0;
     for (size_t cl = 0; cl < N; ++cl)
         for (uint32_t s = 0; s < hn[cl]; ++s) {
-            const float* v = pool.data() + (size_t)hs[cl * KB + s] * LB;
+            const uint32_t r = hs[cl * KB + s];
+            if (r / span >= (uint32_t)NSH || r % span >= off[r / span + 1] - off[r / span])
+                return err("psi_dump: slot payload outside the used pool");
+            const float* v = pool.data() + (off[r / span] + r % span) * LB;
             for (int l = 0; l < LB; ++l) {
                 if (v[l] == 0.0f) continue;
                 if (m < cap) {
