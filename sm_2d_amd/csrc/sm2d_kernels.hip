@@ -1424,9 +1424,10 @@ __device__ inline bool fast_claims(FastLds<LY>& F, WTabs<LY>& W, uint32_t* kc) {
 
 // Size of the per-workgroup owner table of the record rounds (hashed payload addresses), 2 KiB:
 // addresses that share a slot only wait a round.
-constexpr int NOWN = 512;
+constexpr int NOWN_BITS = 8;  // 256 owner slots (1 KiB of LDS)
+constexpr int NOWN = 1 << NOWN_BITS;
 
-__device__ inline uint32_t own_slot(uint32_t addr) { return (addr * 2654435761u) >> 23; }
+__device__ inline uint32_t own_slot(uint32_t addr) { return (addr * 2654435761u) >> (32 - NOWN_BITS); }
 
 // The ordered adds, record-parallel: lane l holds component l's records (distinct
 // addresses within a component).  Each round every pending record bids its index (component-major)
@@ -1486,7 +1487,6 @@ __device__ inline void lanes_to_accum(CellAccum& a, double accd, float accf) {
 
 // Dynamic LDS of the transport: the energy edges and the range tables (4 * N_E + 1 floats), so
 // that the binary searches and interpolations of every component read LDS.
-inline size_t transport_dyn_lds(int NE) { return sizeof(float) * (4 * (size_t)NE + 1); }
 
 // Deviation of a substep's learned widths from the plain physics of the same substep (E, ds in
 // density rh and radiation length X0, mean energy loss mdE): the largest of |ln sigma_E|,
@@ -2186,7 +2186,7 @@ __device__ void apply_foreign(const Tabs& T, const Tp& P, CellGeo G, const Psi& 
 constexpr int QCAP = 512;  // non-zero components queued per round (at least one slot's worth)
 
 template <class LY, bool FINE, int TW>
-__global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_transport(Tabs Tg, Tp P, Psi in, Psi out, Bkt bk,
+__global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_transport(Tabs T, Tp P, Psi in, Psi out, Bkt bk,
                                                    const uint32_t* __restrict__ list,
                                                    const uint32_t* __restrict__ nlist,
                                                    const uint32_t* __restrict__ ltag,
@@ -2194,9 +2194,9 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
                                                    Tally tal, double* acc, uint32_t* flags) {
     if (ctl[C_DONE] || blockIdx.x >= *nlist) return;  // no cell for this workgroup
     // LDS payload rows: 4-wave workgroups (few listed cells, one workgroup per CU) keep the full
-    // set (24 KiB); 2-wave ones 12 KiB and 1-wave ones 14 KiB, so that three (four) of them fit
-    // a CU with the 256-bin energy tables.  (Five 1-wave workgroups with 19 rows each were
-    // measured slower: more cells overflow the rows and take the serial apply.)
+    // set (24 KiB); 2-wave ones 12 KiB and 1-wave ones 14 KiB, so that six 1-wave workgroups
+    // (26 KiB of LDS each) fit a CU.  Fewer rows were measured slower: with 10 KiB of rows seven
+    // workgroups fit, but more cells overflow the rows and take the serial apply.
     constexpr int NLT = TW >= 4 ? LY::NL : TW == 2 ? LY::NL / 2 : (LY::NL * 7) / 12;
     static_assert(QCAP >= LY::LB, "a slot's non-zero components must fit one round's queue");
     __shared__ float s_rows[NLT * LY::LB];
@@ -2211,27 +2211,9 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     __shared__ uint32_t s_sdev;  // bits of the largest learned-physics deviation of the cell (>= 0)
     __shared__ uint32_t s_own[NOWN];
     __shared__ uint32_t s_kc[2 * NKC];  // claimed-key cache of the cell (fast_claims)
-    extern __shared__ float s_tab[];
     for (int i = threadIdx.x; i < NOWN; i += blockDim.x) s_own[i] = ~0u;
-    Tabs T = Tg;
-    {
-        const int NE = Tg.NE;
-        float* ee = s_tab;
-        float* R = ee + NE + 1;
-        float* lR = R + NE;
-        float* lE = lR + NE;
-        for (int i = threadIdx.x; i <= NE; i += blockDim.x) ee[i] = Tg.ee[i];
-        for (int i = threadIdx.x; i < NE; i += blockDim.x) {
-            R[i] = Tg.R[i];
-            lR[i] = Tg.lR[i];
-            lE[i] = Tg.lE[i];
-        }
-        __syncthreads();
-        T.ee = ee;
-        T.R = R;
-        T.lR = lR;
-        T.lE = lE;
-    }
+    // The energy tables (edges, ranges, logs) are read through L1/L2.  An LDS copy per workgroup
+    // cost 4 KiB; without it six one-wave workgroups fit a CU instead of five.
     const uint32_t tag = 2 * ctl[C_EP] + (FINE ? 1u : 0u);
     if (ctl[C_DRIFT]) P.surr = 0;  // K5 flagged the learned physics in an earlier iteration
     const int l = lane_id(), wv = (int)(threadIdx.x >> 6);

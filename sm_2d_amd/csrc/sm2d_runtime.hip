@@ -377,7 +377,6 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
         const uint32_t* nl = c->ctl.p + (fine ? C_NACT : C_NCRS);
         double* acc = fine ? c->acc_f.p : c->acc_c.p;
         const int e0 = fine ? 6 : 2;
-        const size_t dyn = transport_dyn_lds(c->g.N_E);
         const int tw = Tuner::tw(c->cand[fine]);
         const int gb = blocks_for(pow2_at_least(std::max<uint32_t>(c->hint_ph[fine], 64)), 1, 65536);
         // timed launches take their events from the dispatch itself (the kernel's own start and
@@ -387,13 +386,13 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
 #define SM2D_TRANSPORT(F, W)                                                                                  \
     do {                                                                                                      \
         if (c->rec)                                                                                           \
-            hipExtLaunchKernelGGL(k_transport<LY, F, W>, dim3(gb), dim3(64 * W), (uint32_t)dyn, c->stream,    \
+            hipExtLaunchKernelGGL(k_transport<LY, F, W>, dim3(gb), dim3(64 * W), 0u, c->stream,    \
                                   e_start, e_stop, 0u, T, P, c->psi(in), c->psi(out), c->bkt(fine), list, nl, \
                                   (const uint32_t*)c->ltag.p, (const uint32_t*)c->lpos.p, c->ctl.p,           \
                                   c->tally(), acc,                                                            \
                                   c->flags.p);                                                                \
         else                                                                                                  \
-            k_transport<LY, F, W><<<gb, 64 * W, dyn, c->stream>>>(T, P, c->psi(in), c->psi(out), c->bkt(fine), \
+            k_transport<LY, F, W><<<gb, 64 * W, 0, c->stream>>>(T, P, c->psi(in), c->psi(out), c->bkt(fine), \
                                                                  list, nl, c->ltag.p, c->lpos.p, c->ctl.p,    \
                                                                  c->tally(),                                  \
                                                                  acc, c->flags.p);                            \
@@ -1079,18 +1078,6 @@ int create(int device, const sm2d_grid* g, int n_spots, sm2d_ctx** out) {
     c->N = (int)N;
     plan_caps(c.get());
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    {   // the transport's dynamic LDS (energy tables) comes on top of its static LDS
-        const int dyn = (int)transport_dyn_lds(g->N_E);
-        const void* fns[12] = {
-            (const void*)&k_transport<LayoutRef, true, 1>, (const void*)&k_transport<LayoutRef, true, 2>,
-            (const void*)&k_transport<LayoutRef, true, 4>, (const void*)&k_transport<LayoutRef, false, 1>,
-            (const void*)&k_transport<LayoutRef, false, 2>, (const void*)&k_transport<LayoutRef, false, 4>,
-            (const void*)&k_transport<LayoutCfg, true, 1>, (const void*)&k_transport<LayoutCfg, true, 2>,
-            (const void*)&k_transport<LayoutCfg, true, 4>, (const void*)&k_transport<LayoutCfg, false, 1>,
-            (const void*)&k_transport<LayoutCfg, false, 2>, (const void*)&k_transport<LayoutCfg, false, 4>};
-        for (int k = (lid == L_REF ? 0 : 6); k < (lid == L_REF ? 6 : 12); ++k)
-            HIPCHK(hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
-    }
     c->bytes = 0;
     rc_alloc = each_buffer(c.get(), [&](auto& b, size_t n, int) {
         c->bytes += b.bytes_for(n);
