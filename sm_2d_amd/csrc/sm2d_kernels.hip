@@ -2185,8 +2185,11 @@ __device__ void apply_foreign(const Tabs& T, const Tp& P, CellGeo G, const Psi& 
 // choices give identical results.
 constexpr int QCAP = 512;  // non-zero components queued per round (at least one slot's worth)
 
+// Dynamic LDS of a transport launch: the energy tables, for the 4-wave shape only.
+inline size_t transport_dyn_lds(int NE, int TW) { return TW == 4 ? sizeof(float) * (4 * (size_t)NE + 1) : 0; }
+
 template <class LY, bool FINE, int TW>
-__global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_transport(Tabs T, Tp P, Psi in, Psi out, Bkt bk,
+__global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_transport(Tabs Tg, Tp P, Psi in, Psi out, Bkt bk,
                                                    const uint32_t* __restrict__ list,
                                                    const uint32_t* __restrict__ nlist,
                                                    const uint32_t* __restrict__ ltag,
@@ -2212,8 +2215,29 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     __shared__ uint32_t s_own[NOWN];
     __shared__ uint32_t s_kc[2 * NKC];  // claimed-key cache of the cell (fast_claims)
     for (int i = threadIdx.x; i < NOWN; i += blockDim.x) s_own[i] = ~0u;
-    // The energy tables (edges, ranges, logs) are read through L1/L2.  An LDS copy per workgroup
-    // cost 4 KiB; without it six one-wave workgroups fit a CU instead of five.
+    // The energy tables (edges, ranges, logs): 4-wave workgroups (one per CU) copy them into LDS
+    // (dynamic LDS, transport_dyn_lds); the 1- and 2-wave shapes read them through L1/L2, since
+    // 4 KiB more per workgroup would fit five one-wave workgroups per CU instead of six.
+    Tabs T = Tg;
+    if constexpr (TW == 4) {
+        extern __shared__ float s_tab[];
+        const int NE = Tg.NE;
+        float* ee = s_tab;
+        float* R = ee + NE + 1;
+        float* lR = R + NE;
+        float* lE = lR + NE;
+        for (int i = threadIdx.x; i <= NE; i += blockDim.x) ee[i] = Tg.ee[i];
+        for (int i = threadIdx.x; i < NE; i += blockDim.x) {
+            R[i] = Tg.R[i];
+            lR[i] = Tg.lR[i];
+            lE[i] = Tg.lE[i];
+        }
+        __syncthreads();
+        T.ee = ee;
+        T.R = R;
+        T.lR = lR;
+        T.lE = lE;
+    }
     const uint32_t tag = 2 * ctl[C_EP] + (FINE ? 1u : 0u);
     if (ctl[C_DRIFT]) P.surr = 0;  // K5 flagged the learned physics in an earlier iteration
     const int l = lane_id(), wv = (int)(threadIdx.x >> 6);

@@ -386,13 +386,15 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
 #define SM2D_TRANSPORT(F, W)                                                                                  \
     do {                                                                                                      \
         if (c->rec)                                                                                           \
-            hipExtLaunchKernelGGL(k_transport<LY, F, W>, dim3(gb), dim3(64 * W), 0u, c->stream,    \
+            hipExtLaunchKernelGGL(k_transport<LY, F, W>, dim3(gb), dim3(64 * W),                              \
+                                  (uint32_t)transport_dyn_lds(c->g.N_E, W), c->stream,                        \
                                   e_start, e_stop, 0u, T, P, c->psi(in), c->psi(out), c->bkt(fine), list, nl, \
                                   (const uint32_t*)c->ltag.p, (const uint32_t*)c->lpos.p, c->ctl.p,           \
                                   c->tally(), acc,                                                            \
                                   c->flags.p);                                                                \
         else                                                                                                  \
-            k_transport<LY, F, W><<<gb, 64 * W, 0, c->stream>>>(T, P, c->psi(in), c->psi(out), c->bkt(fine), \
+            k_transport<LY, F, W><<<gb, 64 * W, transport_dyn_lds(c->g.N_E, W), c->stream>>>(                    \
+                T, P, c->psi(in), c->psi(out), c->bkt(fine),                                                      \
                                                                  list, nl, c->ltag.p, c->lpos.p, c->ctl.p,    \
                                                                  c->tally(),                                  \
                                                                  acc, c->flags.p);                            \
@@ -1078,6 +1080,15 @@ int create(int device, const sm2d_grid* g, int n_spots, sm2d_ctx** out) {
     c->N = (int)N;
     plan_caps(c.get());
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    {   // the 4-wave transport's dynamic LDS (energy tables) comes on top of its static LDS
+        const int dyn = (int)transport_dyn_lds(g->N_E, 4);
+        const void* fns[4] = {(const void*)&k_transport<LayoutRef, true, 4>,
+                              (const void*)&k_transport<LayoutRef, false, 4>,
+                              (const void*)&k_transport<LayoutCfg, true, 4>,
+                              (const void*)&k_transport<LayoutCfg, false, 4>};
+        for (int k = (lid == L_REF ? 0 : 2); k < (lid == L_REF ? 2 : 4); ++k)
+            HIPCHK(hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+    }
     c->bytes = 0;
     rc_alloc = each_buffer(c.get(), [&](auto& b, size_t n, int) {
         c->bytes += b.bytes_for(n);
