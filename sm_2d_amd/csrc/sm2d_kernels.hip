@@ -2591,7 +2591,9 @@ __global__ __launch_bounds__(64) void k_foreign(Tabs T, Tp P, Psi in, Bkt bk, co
     if (threadIdx.x == 0) ctl[C_FNT] = 0u;
 }
 
-// Destination set of a phase: the four neighbours of every listed cell, deduplicated with a
+// Destination set of a phase: the neighbours of the listed cells whose bucket toward them is not
+// empty (the gather would add nothing to the others, and a cell that receives nothing and is not
+// listed keeps an empty psi_out), deduplicated with a
 // per-cell tag and listed in NSH segments of seg_cap entries, each with a counter on a line of
 // its own (seg[s * SH_STRIDE]); destination d goes to the segment of a multiplicative hash of d
 // (columns and rows of cells spread over all segments), so the segments fill evenly at every
@@ -2599,9 +2601,9 @@ __global__ __launch_bounds__(64) void k_foreign(Tabs T, Tp P, Psi in, Bkt bk, co
 // bit of psi_out is set here.  The order of the lists affects no result.
 static_assert(NSH == 64, "k_dest hashes cells to 64 segments");
 __global__ __launch_bounds__(256) void k_dest(const uint32_t* __restrict__ list, const uint32_t* nlist, int Nx,
-                                              int Nz, int Ns, uint32_t* dtag, const uint32_t* ctl, int fine,
-                                              uint32_t* dlist, uint32_t* seg, uint32_t seg_cap, uint32_t* bitmap,
-                                              uint32_t* flags) {
+                                              int Nz, int Ns, const uint32_t* __restrict__ bkn, uint32_t* dtag,
+                                              const uint32_t* ctl, int fine, uint32_t* dlist, uint32_t* seg,
+                                              uint32_t seg_cap, uint32_t* bitmap, uint32_t* flags) {
     if (ctl[C_DONE]) return;
     const uint32_t n = *nlist;
     const uint32_t tag = 2 * ctl[C_EP] + (uint32_t)fine;
@@ -2618,7 +2620,7 @@ __global__ __launch_bounds__(256) void k_dest(const uint32_t* __restrict__ list,
                 default: d = ix > 0 ? c - 1 : -1; break;
             }
         }
-        if (d < 0) continue;
+        if (d < 0 || bkn[e] == 0u) continue;  // bucket e & 3 of listed cell e >> 2 lands in d
         const uint32_t old = dtag[d];
         if (old == tag || atomicCAS(&dtag[d], old, tag) != old) continue;
         const uint32_t s = ((uint32_t)d * 2654435761u) >> 26;  // multiplicative hash, NSH = 64

@@ -415,9 +415,9 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
                                                            c->ctl.p, c->flags.p);
         uint32_t* seg = c->ctl.p + C_DSEG + (fine ? NSH * SH_STRIDE : 0);
         const uint32_t seg_cap = c->dcap / NSH;
-        k_dest<<<blocks_for(4ull * h, 256, 4096), 256, 0, c->stream>>>(list, nl, c->g.Nx, c->g.Nz, c->Ns, c->dtag.p,
-                                                                       c->ctl.p, fine, c->dlist.p, seg, seg_cap,
-                                                                       c->bitmap.p, c->flags.p);
+        k_dest<<<blocks_for(4ull * h, 256, 4096), 256, 0, c->stream>>>(list, nl, c->g.Nx, c->g.Nz, c->Ns, c->bn.p,
+                                                                       c->dtag.p, c->ctl.p, fine, c->dlist.p, seg,
+                                                                       seg_cap, c->bitmap.p, c->flags.p);
         uint32_t* moved = c->prof ? c->gmoved.p + (size_t)(2 * c->ev_it + fine) * NSH * SH_STRIDE : nullptr;
         const unsigned gg = (blocks_for(4ull * h, 1, 65536) + NSH - 1) / NSH * NSH;
         const int kib = Tuner::kib(c->cand[2 + fine]);
