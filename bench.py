@@ -203,6 +203,17 @@ def fine_grid_variant(hip, mlp, name: str, n: int, plain_sample: int, slabs: boo
                     "learned_audit": _audit(a_l), "learned_vs_plain": dose_diff(e_l, e_plain),
                     "learned_table": {"nodes_checked": st[0], "nodes_rejected": st[1]},
                     "learned_over_plain_batch_speed": dt_plain / dt_l})
+        # the same network evaluated inside the fine transport for every substep (no table)
+        eng.surrogate_enable(2)
+        eng.run_batch(spots, tp)
+        _sync()
+        t0 = time.perf_counter()
+        e_i, _, a_i = eng.run_batch(spots, tp)
+        _sync()
+        dt_i = time.perf_counter() - t0
+        out["inferred"] = {"batch_s": dt_i, "spots_per_s": n / dt_i, "audit": _audit(a_i),
+                           "vs_plain": dose_diff(e_i, e_plain), "vs_table": dose_diff(e_i, e_l),
+                           "table_over_inferred_speed": dt_i / dt_l}
     eng.close()
     if plain_sample:
         _, dts, upds = plain_spots(hip, prob, spots[:plain_sample], tp)

@@ -361,6 +361,10 @@ int sm2d_mlp_get_params(sm2d_mlp* mlp, float* params, float* dict4);
 int sm2d_mlp_set_params(sm2d_mlp* mlp, const float* params, const float* dict4);
 /* Raw outputs z[B][33] of normalised inputs x[B][4]. */
 int sm2d_mlp_forward(sm2d_mlp* mlp, const float* x, int B, float* z);
+/* Decoded outputs out3[B][3] = (ln sigma_E, ln sigma_theta, a) of normalised inputs x[B][4],
+ * evaluated by the same in-kernel inference the fine transport runs in mode 2 of
+ * sm2d_surrogate_enable (64 samples per wavefront on the matrix cores). */
+int sm2d_mlp_infer(sm2d_mlp* mlp, const float* x, int B, float* out3);
 /* Mean loss and its gradient (parameter layout above) of B samples (a multiple of 64):
  * labels[B][3] = (ln sigma_E, ln sigma_theta, ln(a + 1e-9)); loss = soft-target cross-entropy
  * + w_mom * (squared errors of the decoded ln widths) + w_att * (attenuation output error)^2. */
@@ -387,6 +391,11 @@ int sm2d_surrogate_attach(sm2d_ctx* ctx, sm2d_mlp* mlp, int n_nodes);
 int sm2d_surrogate_check(sm2d_ctx* ctx, float tol, float* stats4);
 /* Statistics of the last validation (attach or check). */
 int sm2d_surrogate_stats(sm2d_ctx* ctx, float* stats4);
+/* Learned-physics mode of the fine transport: 0 the plain physics, 1 the attached table (the
+ * default after sm2d_surrogate_attach), 2 the attached network evaluated inside the transport
+ * for every substep (BF16 operands on the matrix cores, 64 components per wavefront; the network
+ * must outlive that use).  Mode 2 applies the same range fallback and K5 drift monitor; its
+ * results depend on the matrix cores' FP32 accumulation and are not compared bit for bit. */
 int sm2d_surrogate_enable(sm2d_ctx* ctx, int on);
 /* The attached table [n_mat][N_E][n_nodes][3] (ln sigma_E, ln sigma_theta, a; NaN outside the
  * training range) and the material index of every depth row (-1: plain physics). */

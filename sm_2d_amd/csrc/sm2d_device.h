@@ -59,6 +59,11 @@ struct Tabs {
     const float* surr;    // learned-physics table [materials][NE][surr_nodes][3] or null
     const int* rowmat;    // material index of every row in that table (-1: plain physics)
     int surr_nodes;
+    // the network itself, for inference inside the fine transport (null when not attached):
+    // padded BF16 weights, FP32 parameters and the width dictionaries (lnE lo/hi, lnT lo/hi)
+    const uint16_t* mlp_wb;
+    const float* mlp_p;
+    float mlp_dict[4];
 };
 
 // Device control block (u32 words).  C_DONE: 0 running, 1 ended (empty lists or a capacity

@@ -1515,9 +1515,70 @@ struct CellGeo {
     int monitor;  // the learned physics of this cell's substeps is compared with the plain physics
 };
 
+// Learned widths of a component evaluated by the network itself (learned-physics mode 2).
+struct Learned {
+    int use;  // the substep takes these widths (inputs inside the training range)
+    float sE, sth, a;
+};
+
+// Mode 2: the substep inputs of every lane's fine-transport component (the energy at its bin
+// centre and the step ds as transport_component computes them, dR = rho ds, rho, X0 ratio) go
+// through the network on the matrix cores, 64 components per wavefront.  Every lane of the
+// wavefront must call it.
+template <class LY>
+__device__ Learned learned_inline(const Tabs& T, const Tp& P, const CellGeo& G, bool live, float w_in, int lidx,
+                                  uint32_t bT, uint32_t bEb) {
+    float x[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    bool use = false;
+    if (live && !(w_in < 1e-15f)) {
+        const int tbin = max(0, min((int)bT * LY::TL + LY::lt(lidx), G.NT - 1));
+        const int ebin = max(0, min((int)bEb * LY::EL + LY::le(lidx), G.NE - 1));
+        const float E = ecenter(T.ee, ebin);
+        if (!(E <= 0.1f)) {
+            const float theta = T.te[tbin] + 0.5f * G.dth;
+            const float xc = G.dx * ((float)(2 * LY::lx(lidx) + 1 - LY::XS) / (float)(2 * LY::XS));
+            const float zc = G.dz * ((float)(2 * LY::lz(lidx) + 1 - LY::ZS) / (float)(2 * LY::ZS));
+            float mu, eta;
+            sincosf_(theta, eta, mu);
+            const float s1 = (mu > 0) ? (G.hdz - zc) / mu : 1e30f;
+            const float s2 = (mu < 0) ? (-G.hdz - zc) / mu : 1e30f;
+            const float s3 = (eta > 0) ? (G.hdx - xc) / eta : 1e30f;
+            const float s4 = (eta < 0) ? (-G.hdx - xc) / eta : 1e30f;
+            const float sb = fmaxf(fminf(fminf(s1, s2), fminf(s3, s4)), 0.0f);
+            const float ds = fminf(max_step(T, E, G.rh, P.rule, ebin), sb * 1.001f);
+            use = T.rowmat && T.rowmat[G.iz] >= 0 && ds > 0.0f && ds / G.X0 >= 1e-6f &&
+                  mlp::encode_input(E, G.rh * ds, G.rh, T.x0r[G.iz], x);
+        }
+    }
+    float lsE, lsT, a;
+    mlp::mlp_wave_forward(T.mlp_wb, T.mlp_p, T.mlp_dict, x, lsE, lsT, a);
+    Learned o;
+    o.use = use ? 1 : 0;
+    o.sE = expf_(lsE);
+    o.sth = expf_(lsT);
+    o.a = fmaxf(a, 0.0f);
+    return o;
+}
+
+// Mode-2 widths of the wavefront's components, or none (the table or the plain physics).  INFER
+// selects the kernel instances that carry the network: its inference needs most of the register
+// file, so the other instances do not contain it.
+template <class LY, bool FINE, bool INFER>
+__device__ inline Learned learned_for(const Tabs& T, const Tp& P, const CellGeo& G, bool live, float w_in, int lidx,
+                                      uint32_t bT, uint32_t bEb) {
+    if constexpr (FINE && INFER) {
+        if (P.surr == 2) return learned_inline<LY>(T, P, G, live, w_in, lidx, bT, bEb);
+    }
+    Learned o;
+    o.use = 0;
+    o.sE = o.sth = o.a = 0.0f;
+    return o;
+}
+
 template <class LY, bool FINE>
 __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const CellGeo& G, bool live, float w_in,
-                                           int lidx_in, uint32_t slot, uint32_t bT, uint32_t bEb SM2D_SUBP) {
+                                           int lidx_in, uint32_t slot, uint32_t bT, uint32_t bEb,
+                                           const Learned& Lx SM2D_SUBP) {
     SM2D_SUB_START();
     const int lc = G.lc, iz = G.iz, ix = G.ix, Nx = G.Nx, NT = G.NT, NE = G.NE;
     const float dx = G.dx, dz = G.dz, hdx = G.hdx, hdz = G.hdz, tmin = G.tmin, dth = G.dth;
@@ -1572,8 +1633,16 @@ __device__ inline Comp transport_component(const Tabs& T, const Tp& P, const Cel
                 ds = fminf(max_step(T, E, rh, P.rule, ebin), sb * 1.001f);
                 float mdE = deposition(T, E, ds, rh, ebin);
                 float s_sE = 0.0f, s_sth = 0.0f, s_a = 0.0f;
-                const bool use_s = P.surr && ds > 0.0f && ds / X0 >= 1e-6f &&
-                                   surrogate_lookup(T, ebin, iz, rh * ds, s_sE, s_sth, s_a);
+                bool use_s;
+                if (P.surr == 2) {  // the network, evaluated for the wavefront (learned_inline)
+                    use_s = Lx.use != 0;
+                    s_sE = Lx.sE;
+                    s_sth = Lx.sth;
+                    s_a = Lx.a;
+                } else {
+                    use_s = P.surr && ds > 0.0f && ds / X0 >= 1e-6f &&
+                            surrogate_lookup(T, ebin, iz, rh * ds, s_sE, s_sth, s_a);
+                }
                 r.surr = use_s ? 1 : 0;
                 if (P.strag) {
                     unsigned sd = (unsigned)((lc * 7 + (int)slot * 13 + lidx * 17) ^ 0x5DEECE66DL);
@@ -2108,7 +2177,7 @@ __device__ inline void set_cell_geo(CellGeo& G, const Tabs& T, int cell, const u
 // order, theta part.  This is one of the orders the reference's device atomics can produce: each
 // bucket location orders its relaxed atomics independently, so the bucket's own cell may come
 // first.  The weights were booked to their sources when emitted.
-template <class LY, bool FINE>
+template <class LY, bool FINE, bool INFER>
 __device__ void apply_foreign(const Tabs& T, const Tp& P, CellGeo G, const Psi& in, Bkt& bk,
                               const uint32_t* __restrict__ list, const uint32_t* __restrict__ ltag, uint32_t tag,
                               uint32_t nf, uint32_t* flags) {
@@ -2140,8 +2209,11 @@ __device__ void apply_foreign(const Tabs& T, const Tp& P, CellGeo G, const Psi& 
                     const float w = vin[l + 64 * kk];
                     const bool live = w > 0.0f;
                     if (!__ballot(live)) continue;
+                    const Learned Lx = learned_for<LY, FINE, INFER>(T, P, G, live, live ? w : 0.0f, l + 64 * kk,
+                                                             bid & 0xFFF, (bid >> 12) & 0xFFF);
                     const Comp r = transport_component<LY, FINE>(T, P, G, live, live ? w : 0.0f, l + 64 * kk, sl,
-                                                                 bid & 0xFFF, (bid >> 12) & 0xFFF SM2D_SUBA(nullptr));
+                                                                 bid & 0xFFF, (bid >> 12) & 0xFFF,
+                                                                 Lx SM2D_SUBA(nullptr));
                     float wq = 0.0f;
 #pragma unroll
                     for (int u = 0; u < 11; ++u)
@@ -2188,7 +2260,7 @@ constexpr int QCAP = 512;  // non-zero components queued per round (at least one
 // Dynamic LDS of a transport launch: the energy tables, for the 4-wave shape only.
 inline size_t transport_dyn_lds(int NE, int TW) { return TW == 4 ? sizeof(float) * (4 * (size_t)NE + 1) : 0; }
 
-template <class LY, bool FINE, int TW>
+template <class LY, bool FINE, int TW, bool INFER = false>
 __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) void k_transport(Tabs Tg, Tp P, Psi in, Psi out, Bkt bk,
                                                    const uint32_t* __restrict__ list,
                                                    const uint32_t* __restrict__ nlist,
@@ -2452,9 +2524,11 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
                     const bool live = i < total;
                     const uint32_t it = live ? s_q[i] : 0u;
                     const uint32_t slot = it >> 16, bid = s_bid[slot];
+                    const Learned Lx = learned_for<LY, FINE, INFER>(T, P, G, live, live ? s_qw[i] : 0.0f,
+                                                             (int)(it & 0xFFFFu), bid & 0xFFF, (bid >> 12) & 0xFFF);
                     const Comp r = transport_component<LY, FINE>(T, P, G, live, live ? s_qw[i] : 0.0f,
                                                                  (int)(it & 0xFFFFu), slot, bid & 0xFFF,
-                                                                 (bid >> 12) & 0xFFF SM2D_SUBA(sc_));
+                                                                 (bid >> 12) & 0xFFF, Lx SM2D_SUBA(sc_));
                     const LaneOut o = fast_prepare<LY, FINE>(s_fast[wv], r, live, G.ix, G.iz, G.Nx, T.Nz);
                     s_fast[wv].nkey[l] = o.nkey;
                     s_fast[wv].nrec[l] = o.nrec;
@@ -2508,9 +2582,12 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
                             const bool live = i < total;
                             const uint32_t it = live ? s_q[i] : 0u;
                             const uint32_t slot = it >> 16, bid = s_bid[slot];
+                            const Learned Lx = learned_for<LY, FINE, INFER>(T, P, G, live, live ? s_qw[i] : 0.0f,
+                                                                     (int)(it & 0xFFFFu), bid & 0xFFF,
+                                                                     (bid >> 12) & 0xFFF);
                             const Comp r = transport_component<LY, FINE>(T, P, G, live, live ? s_qw[i] : 0.0f,
                                                                          (int)(it & 0xFFFFu), slot, bid & 0xFFF,
-                                                                         (bid >> 12) & 0xFFF SM2D_SUBA(nullptr));
+                                                                         (bid >> 12) & 0xFFF, Lx SM2D_SUBA(nullptr));
                             CellAccum mine = {};
                             serial_apply<LY, FINE>(r, m, W, mine, comps, T, out, bk, flags, G);
                             add_batch_sum(batch_sum_lane(mine), accd, accf);
@@ -2569,7 +2646,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
 
 // The phase's lateral emissions into other listed cells' +z buckets (apply_foreign), after the
 // transport launch that marked them: one wavefront; returns at once when nothing was marked.
-template <class LY, bool FINE>
+template <class LY, bool FINE, bool INFER>
 __global__ __launch_bounds__(64) void k_foreign(Tabs T, Tp P, Psi in, Bkt bk, const uint32_t* __restrict__ list,
                                                 const uint32_t* __restrict__ ltag, uint32_t* ctl, uint32_t* flags) {
     const uint32_t nf = ctl[C_FNT];
@@ -2587,7 +2664,7 @@ __global__ __launch_bounds__(64) void k_foreign(Tabs T, Tp P, Psi in, Bkt bk, co
     G.tmin = T.te[0];
     G.dth = (T.te[T.NT] - G.tmin) / T.NT;
     G.R_on = lookup_R(T, fmaxf(P.E_on, T.ee[0]));
-    apply_foreign<LY, FINE>(T, P, G, in, bk, list, ltag, tag, nf, flags);
+    apply_foreign<LY, FINE, INFER>(T, P, G, in, bk, list, ltag, tag, nf, flags);
     if (threadIdx.x == 0) ctl[C_FNT] = 0u;
 }
 

@@ -411,5 +411,133 @@ __global__ void k_table_plain(Tabs T, const float* __restrict__ mrho, const floa
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// The network inside a kernel (inference in the fine transport): the 64 lanes of a wavefront
+// are 64 samples, which go through in four tiles of 16.  Transposed form Z^T = W A^T: the
+// weights are the A operand (lane l: W[16t + l%16][16kk + 4(l/16) + i], four consecutive BF16 of
+// a row of W[out][in]) and the activations the B operand (lane l: unit 16kk + 4(l/16) + i of
+// sample 16s + l%16).  Output tile t of a layer (lane l: unit 16t + 4(l/16) + i of sample
+// 16s + l%16) is then, after bias, GELU and rounding to BF16, the B operand of K step t of the
+// next layer, with no data movement between layers.  Every lane of the wavefront must call it.
+// ------------------------------------------------------------------------------------------
+__device__ inline bf16x4 ld_bf16x4(const uint16_t* q) {  // 4 BF16 at an 8-byte aligned address
+    const uint2 v = *reinterpret_cast<const uint2*>(q);
+    bf16x4 r;
+    r[0] = (short)(v.x & 0xFFFFu);
+    r[1] = (short)(v.x >> 16);
+    r[2] = (short)(v.y & 0xFFFFu);
+    r[3] = (short)(v.y >> 16);
+    return r;
+}
+
+// One layer of 128 outputs over K steps of 16 inputs: out[t] = BF16(GELU(W in + b)).
+template <int KSTEPS, int LD>
+__device__ inline void mlp_tile_layer(const uint16_t* __restrict__ W, const float* __restrict__ b, const bf16x4* in,
+                                      bf16x4* out) {
+    const int l = (int)(threadIdx.x & 63), g = l >> 4, c = l & 15;
+#pragma unroll
+    for (int t = 0; t < NH / 16; ++t) {
+        f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        const uint16_t* row = W + (size_t)(16 * t + c) * LD + 4 * g;
+#pragma unroll
+        for (int kk = 0; kk < KSTEPS; ++kk)
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_bf16x4(row + 16 * kk), in[kk], acc, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) out[t][i] = (short)f2bf(gelu(acc[i] + b[16 * t + 4 * g + i]));
+        // one tile's weight loads at a time: hoisting the next tiles' loads would hold up to 128
+        // more registers
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// Decoded outputs (ln sigma_E, ln sigma_theta, a) of this lane's sample, input x[4] (normalised).
+__device__ inline void mlp_wave_forward(const uint16_t* __restrict__ wb, const float* __restrict__ p,
+                                        const float* dict4, const float x[4], float& lsE_out, float& lsT_out,
+                                        float& a_out) {
+    const int l = (int)(threadIdx.x & 63), g = l >> 4, c = l & 15;
+    const float eLo = dict4[0], eHi = dict4[1], tLo = dict4[2], tHi = dict4[3];
+    lsE_out = lsT_out = a_out = 0.0f;
+#pragma unroll 1
+    for (int s = 0; s < 4; ++s) {
+        // layer 0 input: features 0..3 of sample 16s + c in the lanes of group 0, zeros above
+        bf16x4 x0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float v = __shfl(x[i], 16 * s + c);
+            x0[i] = (short)(g == 0 ? f2bf(v) : 0);
+        }
+        bf16x4 h0[8], h1[8];
+        mlp_tile_layer<1, KP0>(wb + BOFF_W0, p + OFF_B0, &x0, h0);
+        mlp_tile_layer<NH / 16, NH>(wb + BOFF_W1, p + OFF_B1, h0, h1);
+        mlp_tile_layer<NH / 16, NH>(wb + BOFF_W2, p + OFF_B2, h1, h0);
+        // output layer: tiles 0, 1 are the 32 class logits, unit 32 of tile 2 is ln(a + 1e-9)
+        f32x4 z[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            const uint16_t* row = wb + BOFF_W3 + (size_t)(16 * t + c) * NH + 4 * g;
+#pragma unroll
+            for (int kk = 0; kk < NH / 16; ++kk)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_bf16x4(row + 16 * kk), h0[kk], acc, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int u = 16 * t + 4 * g + i;
+                z[t][i] = acc[i] + (u < NOUT ? p[OFF_B3 + u] : 0.0f);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // class distribution of sample 16s + c: the 32 logits lie in its four lanes c + 16 g
+        float mx = z[0][0];
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) mx = fmaxf(mx, z[t][i]);
+        mx = fmaxf(mx, __shfl_xor(mx, 16));
+        mx = fmaxf(mx, __shfl_xor(mx, 32));
+        float sp = 0.0f, sE = 0.0f, sT = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int u = 16 * t + 4 * g + i, ce = u / NCLS_T, ct = u % NCLS_T;
+                const float q = expf_(z[t][i] - mx);
+                sp += q;
+                sE += q * (eLo + (eHi - eLo) * (float)ce / (float)(NCLS_E - 1));
+                sT += q * (tLo + (tHi - tLo) * (float)ct / (float)(NCLS_T - 1));
+            }
+        sp += __shfl_xor(sp, 16);
+        sE += __shfl_xor(sE, 16);
+        sT += __shfl_xor(sT, 16);
+        sp += __shfl_xor(sp, 32);
+        sE += __shfl_xor(sE, 32);
+        sT += __shfl_xor(sT, 32);
+        // sample 16s + c's results sit in lane c (group 0 holds unit 32 in z[2][0])
+        const float lsE = __shfl(sE / sp, l & 15), lsT = __shfl(sT / sp, l & 15), za = __shfl(z[2][0], l & 15);
+        if ((l >> 4) == s) {
+            lsE_out = lsE;
+            lsT_out = lsT;
+            a_out = fmaxf(expf_(za) - 1e-9f, 0.0f);
+        }
+    }
+}
+
+// Inference probe: decoded outputs of n normalised inputs x[n][4] through mlp_wave_forward (for
+// the tests against the numpy restatement and the GEMM path).
+__global__ __launch_bounds__(64) void k_mlp_wave_probe(const uint16_t* __restrict__ wb, const float* __restrict__ p,
+                                                       Dict D, const float* __restrict__ x, int n, float* out3) {
+    const int i = (int)(blockIdx.x * 64 + threadIdx.x);
+    float xi[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (i < n)
+        for (int k = 0; k < 4; ++k) xi[k] = x[4 * (size_t)i + k];
+    const float d4[4] = {D.lsE_lo, D.lsE_hi, D.lsT_lo, D.lsT_hi};
+    float a0, a1, a2;
+    mlp_wave_forward(wb, p, d4, xi, a0, a1, a2);
+    if (i < n) {
+        out3[3 * (size_t)i + 0] = a0;
+        out3[3 * (size_t)i + 1] = a1;
+        out3[3 * (size_t)i + 2] = a2;
+    }
+}
+
 }  // namespace mlp
 }  // namespace sm2d

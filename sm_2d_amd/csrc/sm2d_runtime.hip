@@ -140,7 +140,10 @@ struct sm2d_ctx {
     // learned physics of the fine transport (sm2d_surrogate_attach)
     DBuf<float> surr;
     DBuf<int> rowmat;
-    int surr_nodes = 0, surr_nmat = 0, surr_on = 0;
+    int surr_nodes = 0, surr_nmat = 0, surr_on = 0;  // surr_on: 0 off, 1 table, 2 network inference
+    const uint16_t* mlp_wb = nullptr;  // the attached network (inference in the fine transport)
+    const float* mlp_p = nullptr;
+    float mlp_dict[4] = {};
     float surr_tol = 0.25f;   // validation tolerance of an attached network's table
     float surr_stats[4] = {}; // last validation: nodes checked, flagged, max and RMS deviation
     std::vector<double> src;  // per spot: W in/out/slot, E in/out/slot, E representation loss
@@ -185,6 +188,8 @@ struct sm2d_ctx {
         t.te = te.p; t.ee = ee.p; t.R = R.p; t.lR = lR.p; t.lE = lE.p;
         t.feA = feA.p; t.feB = feB.p; t.sx0 = sx0.p; t.rho = rho.p; t.x0r = x0r.p;
         t.surr = surr.p; t.rowmat = rowmat.p; t.surr_nodes = surr_nodes;
+        t.mlp_wb = mlp_wb; t.mlp_p = mlp_p;
+        for (int k = 0; k < 4; ++k) t.mlp_dict[k] = mlp_dict[k];
         return t;
     }
     Psi psi(int k) const {
@@ -256,7 +261,7 @@ Tp make_tp(const sm2d_ctx* c, const sm2d_transport* t) {
     p.rule = t->step_rule;
     p.strag = t->enable_straggling;
     p.nuc = t->enable_nuclear;
-    p.surr = (c->surr_on && c->surr.p) ? 1 : 0;
+    p.surr = (c->surr_on && c->surr.p) ? (c->surr_on == 2 && c->mlp_wb ? 2 : 1) : 0;
     return p;
 }
 
@@ -383,36 +388,41 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
         // end), so stage times and the tuner see the kernel without marker packets around it
         hipEvent_t e_start = c->rec ? c->ev[(size_t)c->ev_it * sm2d_ctx::EPI + e0] : nullptr;
         hipEvent_t e_stop = c->rec ? c->ev[(size_t)c->ev_it * sm2d_ctx::EPI + e0 + 1] : nullptr;
-#define SM2D_TRANSPORT(F, W)                                                                                  \
+#define SM2D_TRANSPORT(F, W, I)                                                                               \
     do {                                                                                                      \
         if (c->rec)                                                                                           \
-            hipExtLaunchKernelGGL(k_transport<LY, F, W>, dim3(gb), dim3(64 * W),                              \
+            hipExtLaunchKernelGGL(k_transport<LY, F, W, I>, dim3(gb), dim3(64 * W),                           \
                                   (uint32_t)transport_dyn_lds(c->g.N_E, W), c->stream,                        \
                                   e_start, e_stop, 0u, T, P, c->psi(in), c->psi(out), c->bkt(fine), list, nl, \
                                   (const uint32_t*)c->ltag.p, (const uint32_t*)c->lpos.p, c->ctl.p,           \
                                   c->tally(), acc,                                                            \
                                   c->flags.p);                                                                \
         else                                                                                                  \
-            k_transport<LY, F, W><<<gb, 64 * W, transport_dyn_lds(c->g.N_E, W), c->stream>>>(                    \
+            k_transport<LY, F, W, I><<<gb, 64 * W, transport_dyn_lds(c->g.N_E, W), c->stream>>>(                 \
                 T, P, c->psi(in), c->psi(out), c->bkt(fine),                                                      \
                                                                  list, nl, c->ltag.p, c->lpos.p, c->ctl.p,    \
                                                                  c->tally(),                                  \
                                                                  acc, c->flags.p);                            \
     } while (0)
-        if (fine) {
-            if (tw == 1) SM2D_TRANSPORT(true, 1);
-            else if (tw == 2) SM2D_TRANSPORT(true, 2);
-            else SM2D_TRANSPORT(true, 4);
+        // learned-physics mode 2 (the network inside the fine transport): its own one-wave instances
+        const bool infer = fine && P.surr == 2;
+        if (infer) SM2D_TRANSPORT(true, 1, true);
+        else if (fine) {
+            if (tw == 1) SM2D_TRANSPORT(true, 1, false);
+            else if (tw == 2) SM2D_TRANSPORT(true, 2, false);
+            else SM2D_TRANSPORT(true, 4, false);
         } else {
-            if (tw == 1) SM2D_TRANSPORT(false, 1);
-            else if (tw == 2) SM2D_TRANSPORT(false, 2);
-            else SM2D_TRANSPORT(false, 4);
+            if (tw == 1) SM2D_TRANSPORT(false, 1, false);
+            else if (tw == 2) SM2D_TRANSPORT(false, 2, false);
+            else SM2D_TRANSPORT(false, 4, false);
         }
 #undef SM2D_TRANSPORT
-        if (fine) k_foreign<LY, true><<<1, 64, 0, c->stream>>>(T, P, c->psi(in), c->bkt(fine), list, c->ltag.p,
-                                                             c->ctl.p, c->flags.p);
-        else k_foreign<LY, false><<<1, 64, 0, c->stream>>>(T, P, c->psi(in), c->bkt(fine), list, c->ltag.p,
-                                                           c->ctl.p, c->flags.p);
+        if (infer) k_foreign<LY, true, true><<<1, 64, 0, c->stream>>>(T, P, c->psi(in), c->bkt(fine), list,
+                                                                    c->ltag.p, c->ctl.p, c->flags.p);
+        else if (fine) k_foreign<LY, true, false><<<1, 64, 0, c->stream>>>(T, P, c->psi(in), c->bkt(fine), list,
+                                                                         c->ltag.p, c->ctl.p, c->flags.p);
+        else k_foreign<LY, false, false><<<1, 64, 0, c->stream>>>(T, P, c->psi(in), c->bkt(fine), list, c->ltag.p,
+                                                                  c->ctl.p, c->flags.p);
         uint32_t* seg = c->ctl.p + C_DSEG + (fine ? NSH * SH_STRIDE : 0);
         const uint32_t seg_cap = c->dcap / NSH;
         k_dest<<<blocks_for(4ull * h, 256, 4096), 256, 0, c->stream>>>(list, nl, c->g.Nx, c->g.Nz, c->Ns, c->bn.p,

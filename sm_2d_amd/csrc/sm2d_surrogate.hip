@@ -310,6 +310,23 @@ int sm2d_mlp_forward(sm2d_mlp* M, const float* x, int B, float* z) {
     return SM2D_OK;
 }
 
+// Decoded outputs (ln sigma_E, ln sigma_theta, a) of normalised inputs x[B][4], evaluated by the
+// in-kernel inference of the fine transport (mlp_wave_forward).
+int sm2d_mlp_infer(sm2d_mlp* M, const float* x, int B, float* out3) {
+    if (!M || !x || !out3 || B <= 0) return err("sm2d_mlp_infer: bad argument");
+    if (!M->has_dict) return err("sm2d_mlp_infer: the network has no width dictionaries");
+    HIPCHK(hipSetDevice(M->device));
+    DBuf<float> dx, dout;
+    HIPCHK(dx.alloc((size_t)B * 4));
+    HIPCHK(dout.alloc((size_t)B * 3));
+    HIPCHK(hipMemcpyAsync(dx.p, x, sizeof(float) * 4 * (size_t)B, hipMemcpyHostToDevice, M->stream));
+    k_mlp_wave_probe<<<(B + 63) / 64, 64, 0, M->stream>>>(M->wb.p, M->p.p, M->dict, dx.p, B, dout.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out3, dout.p, sizeof(float) * 3 * (size_t)B, hipMemcpyDeviceToHost, M->stream));
+    HIPCHK(hipStreamSynchronize(M->stream));
+    return SM2D_OK;
+}
+
 // Loss and its gradient (FP32, layout of sm2d_mlp_get_params) for B samples (B multiple of 64):
 // x[B][4] normalised inputs, labels[B][3] = (ln sigma_E, ln sigma_theta, ln(a + 1e-9)).
 int sm2d_mlp_grad(sm2d_mlp* M, const float* x, const float* labels, int B, float w_att, float w_mom, float* loss,
@@ -500,6 +517,13 @@ int sm2d_surrogate_attach(sm2d_ctx* c, sm2d_mlp* M, int n_nodes) {
     c->surr_nodes = n_nodes;
     c->surr_nmat = nmat;
     c->surr_on = 1;
+    // the network itself, for the inline mode of sm2d_surrogate_enable (M must outlive that use)
+    c->mlp_wb = M->wb.p;
+    c->mlp_p = M->p.p;
+    c->mlp_dict[0] = M->dict.lsE_lo;
+    c->mlp_dict[1] = M->dict.lsE_hi;
+    c->mlp_dict[2] = M->dict.lsT_lo;
+    c->mlp_dict[3] = M->dict.lsT_hi;
     return surrogate_validate(c, c->surr_tol, nullptr);
 }
 
@@ -519,8 +543,11 @@ int sm2d_surrogate_stats(sm2d_ctx* c, float* stats4) {
 
 int sm2d_surrogate_enable(sm2d_ctx* c, int on) {
     if (!c) return err("null context");
+    if (on < 0 || on > 2) return err("sm2d_surrogate_enable: mode must be 0, 1 or 2");
     if (on && !c->surr.p) return err("sm2d_surrogate_enable: no surrogate table attached");
-    c->surr_on = on ? 1 : 0;
+    if (on == 2 && !c->mlp_wb) return err("sm2d_surrogate_enable: no network attached (sm2d_surrogate_attach)");
+    if (on != c->surr_on) c->graphs_clear();
+    c->surr_on = on;
     return SM2D_OK;
 }
 

@@ -302,8 +302,10 @@ class Engine:
         self.L.call("surrogate_stats", self.h, _f(st))
         return tuple(float(v) for v in st)
 
-    def surrogate_enable(self, on: bool):
-        self.L.call("surrogate_enable", self.h, 1 if on else 0)
+    def surrogate_enable(self, on):
+        """0 / False: plain physics; 1 / True: the attached table; 2: the attached network
+        evaluated inside the fine transport."""
+        self.L.call("surrogate_enable", self.h, int(on))
 
     def surrogate_export(self):
         """(table [n_mat][N_E][n_nodes][3], n_mat, n_nodes, rowmat [Nz])."""
@@ -379,6 +381,13 @@ class Mlp:
         z = np.zeros((len(x), 33), np.float32)
         self.L.call("mlp_forward", self.h, _f(x), len(x), _f(z))
         return z
+
+    def infer(self, x):
+        """Decoded (ln sigma_E, ln sigma_theta, a) of normalised inputs by the in-kernel inference."""
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.zeros((len(x), 3), np.float32)
+        self.L.call("mlp_infer", self.h, _f(x), len(x), _f(out))
+        return out
 
     def grad(self, x, labels, w_att: float = 1.0, w_mom: float = 1.0):
         x = np.ascontiguousarray(x, np.float32)

@@ -28,6 +28,7 @@ HIP_PROTOS = {
     "mlp_get_params": [C.c_void_p, abi.fp, abi.fp],
     "mlp_set_params": [C.c_void_p, abi.fp, abi.fp],
     "mlp_forward": [C.c_void_p, abi.fp, C.c_int, abi.fp],
+    "mlp_infer": [C.c_void_p, abi.fp, C.c_int, abi.fp],
     "mlp_grad": [C.c_void_p, abi.fp, abi.fp, C.c_int, C.c_float, C.c_float, abi.fp, abi.fp],
     "mlp_train": [C.c_void_p, C.c_void_p, C.POINTER(abi.MlpTrainCfg), abi.fp, C.c_int],
     "mlp_evaluate": [C.c_void_p, C.c_void_p, C.c_int, C.c_uint32, abi.fp],

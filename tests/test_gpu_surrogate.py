@@ -73,6 +73,27 @@ def test_gradients_match_numpy(hip):
     m.close()
 
 
+def test_inference_matches_numpy(hip):
+    """The in-transport inference (one wavefront per 64 inputs, decoded in registers) against the
+    numpy forward pass and decoder, and against the GEMM forward path; a ragged count exercises
+    the idle lanes of the last wavefront."""
+    from oracle import mlp_ref as R
+    rng = np.random.default_rng(2)
+    m = Mlp(hip, seed=9)
+    dict4 = np.array([-7.0, 0.5, -10.0, -0.5], np.float32)
+    m.set_params(dict4=dict4)
+    p, _ = m.params()
+    x = rng.uniform(-1, 1, (200, 4)).astype(np.float32)
+    out = m.infer(x)
+    ref = R.decode(R.forward(p, x, emulate=True)[0], dict4)
+    gemm = R.decode(m.forward(x), dict4)
+    for other in (ref, gemm):
+        # same BF16 operands; the FP32 accumulation order and the exponential differ
+        assert np.abs(out[:, :2] - other[:, :2]).max() <= 1e-3 * np.abs(dict4).max()
+        assert np.allclose(out[:, 2], other[:, 2], rtol=5e-3, atol=1e-9)
+    m.close()
+
+
 def test_training_learns_the_plain_physics(trained):
     _, eng, m, hist = trained
     assert np.all(np.isfinite(hist))
@@ -359,3 +380,68 @@ def test_learned_dose_against_plain(hip, trained, orc):
           flush=True)
     assert bl < 2.0 * bf and pl < 2.0 * pf
     assert as_.max_cell_W_closure < 1e-6 and af.max_cell_W_closure < 1e-6
+
+
+def test_inference_in_the_transport(hip, trained, orc):
+    """Mode 2 evaluates the network inside the fine transport for every substep instead of
+    reading the node table.  It is deterministic (a spot twice, and spot by spot in a batch, give
+    the same EdepC) and conserves weight like the other modes.  Its dose meets the table mode's
+    bar against the plain physics (integral within 1 %, block and depth-dose differences within
+    twice the response of the plain physics to random 1 % width changes): the table interpolates
+    the same network between nodes, and any width change moves the hashed per-substep draws."""
+    prob, _, m, _ = trained
+    g = problems.setup_engine(hip, prob, Engine)
+    sp, tp = prob.make_spot(), prob.make_transport()
+    ep, _ = g.run_spot(sp, tp)
+    g.surrogate_attach(m, 64)
+    _, nm, nn, rows = g.surrogate_export()
+    et, _ = g.run_spot(sp, tp)
+    g.surrogate_enable(2)
+    ei, ai = g.run_spot(sp, tp)
+    ei2, _ = g.run_spot(sp, tp)
+    assert np.array_equal(ei, ei2)
+    # surrogate_active reports the mode in use
+    assert ai.surrogate_active == 2 and ai.surrogate_fallback == 0 and ai.surrogate_substeps > 0
+    assert ai.fine_cell_bin_updates > 0 and ai.W_fail_iters == 0
+    assert ai.max_cell_W_closure < 1e-6
+    assert abs(ei.sum() / ep.sum() - 1.0) < 0.01
+    g.surrogate_enable(1)
+    eb, _ = g.run_spot(sp, tp)
+    assert np.array_equal(eb, et)
+    rng = np.random.default_rng(7)
+    pert = _plain_table(orc, prob, nn)
+    pert[:, :2] += rng.normal(0.0, 0.01, (len(pert), 2)).astype(np.float32)
+    g.surrogate_set_table(pert.reshape(-1), nm, nn, rows)
+    ef, _ = g.run_spot(sp, tp)
+    sp2 = prob.make_spot()
+    sp2.E0, sp2.x0, sp2.random_seed = 45.0, -3.0, 78
+    g.surrogate_attach(m, 64)
+    g.surrogate_enable(2)
+    e2, _ = g.run_spot(sp2, tp)
+    g.close()
+
+    def blk(d, b=4):
+        nz, nx = d.shape
+        return d[: nz // b * b, : nx // b * b].reshape(nz // b, b, nx // b, b).sum((1, 3))
+
+    bp, pdd0 = blk(ep), ep.sum(1)
+    mask, pm = bp > 0.1 * bp.max(), pdd0 > 0.1 * pdd0.max()
+
+    def diffs(e):
+        b = blk(e)
+        return (np.abs(b - bp)[mask] / bp[mask]).mean(), (np.abs(e.sum(1) - pdd0)[pm] / pdd0[pm]).mean()
+
+    bi, pi = diffs(ei)
+    bt, pt = diffs(et)
+    bf, pf = diffs(ef)
+    print(f"vs plain: inferred block {bi:.4f} PDD {pi:.4f}; table block {bt:.4f} PDD {pt:.4f}; "
+          f"plain 1% perturbed block {bf:.4f} PDD {pf:.4f}", flush=True)
+    assert bi < 2.0 * bf and pi < 2.0 * pf
+    batch = problems.setup_engine(hip, prob, Engine, n_spots=2)
+    batch.surrogate_attach(m, 64)
+    batch.surrogate_enable(2)
+    _, per, abt = batch.run_batch([sp, sp2], tp, per_spot=True)
+    batch.close()
+    assert abt.surrogate_active == 2 and abt.surrogate_substeps > 0
+    assert np.array_equal(per[0], ei.astype(np.float32))
+    assert np.array_equal(per[1], e2.astype(np.float32))
