@@ -953,19 +953,43 @@ __device__ inline float emit(WTabs<LY>& W, CellAccum& a, const Tabs& T, Bkt& bk,
 // End of a cell: one pool reservation per pool for the cell's LDS-cached slots, payload copies
 // LDS -> global (coalesced) and the table writes.  A pool overflow raises its flag and leaves
 // the affected tables without the new slots, so no later kernel reads past a pool.
+// Copies whole LDS payload rows to pool rows, 16 bytes per lane, so that one wavefront
+// instruction moves 64 / (LB / 4) rows (two 512 B rows, or one 1 KiB row): the table slots in
+// mask m, where lane s holds slot s's LDS row in xr (CACHED | row) and its pool row in yr.
+// Fewer, wider stores keep a cell's flush well inside the wave's 63 outstanding memory
+// operations.
+template <class LY>
+__device__ inline void rows_to_pool(uint64_t m, uint32_t xr, uint32_t yr, const float* rows, float* pool) {
+    constexpr int LPR = LY::LB / 4;  // lanes per row
+    constexpr int RPI = 64 / LPR;    // rows per instruction
+    static_assert(LY::LB % 4 == 0 && 64 % LPR == 0, "rows must split into 16-byte lane parts");
+    const int l = lane_id(), g = l / LPR, e = l - g * LPR;
+    while (m) {
+        int s = -1;
+#pragma unroll
+        for (int k = 0; k < RPI; ++k) {
+            if (m) {
+                const int sk = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                if (g == k) s = sk;
+            }
+        }
+        const uint32_t x = (uint32_t)__shfl((int)xr, s < 0 ? 0 : s);
+        const uint32_t y = (uint32_t)__shfl((int)yr, s < 0 ? 0 : s);
+        if (s >= 0) {
+            const float4 v = *reinterpret_cast<const float4*>(rows + (size_t)(x & ~CACHED) * LY::LB + 4 * e);
+            *reinterpret_cast<float4*>(pool + (size_t)y * LY::LB + 4 * e) = v;
+        }
+    }
+}
+
 template <class LY>
 __device__ inline void flush_tables(WTabs<LY>& W, Psi& out, Bkt& bk, int cell, uint32_t li, uint32_t* flags) {
     const int l = lane_id();
     const uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
     // own psi_out: preloaded slots go back to their payloads, new cached slots get new ones
     const uint64_t mp = __ballot(l < 32 && (uint32_t)l < W.own0 && (W.r0 & CACHED));
-    for (uint64_t m = mp; m; m &= m - 1) {
-        const int s = __ffsll((unsigned long long)m) - 1;
-        const float* src = W.rows + (size_t)(rlu(W.r0, s) & ~CACHED) * LY::LB;
-        float* dst = out.pool + (size_t)rlu(W.orig, s) * LY::LB;
-#pragma unroll
-        for (int k = 0; k < LY::PL; ++k) dst[l + 64 * k] = src[l + 64 * k];
-    }
+    rows_to_pool<LY>(mp, W.r0, W.orig, W.rows, out.pool);
     const uint64_t mo = __ballot(l < 32 && (uint32_t)l >= W.own0 && (uint32_t)l < W.n0 && (W.r0 & CACHED));
     const uint32_t co = (uint32_t)__popcll(mo);
     uint64_t mf[4];
@@ -986,13 +1010,7 @@ __device__ inline void flush_tables(WTabs<LY>& W, Psi& out, Bkt& bk, int cell, u
     if (ok) {
         uint32_t ref = W.r0;
         if ((mo >> l) & 1ull) ref = base + (uint32_t)__popcll(mo & below);
-        for (uint64_t m = mo; m; m &= m - 1) {
-            const int s = __ffsll((unsigned long long)m) - 1;
-            const float* src = W.rows + (size_t)(rlu(W.r0, s) & ~CACHED) * LY::LB;
-            float* dst = out.pool + (size_t)rlu(ref, s) * LY::LB;
-#pragma unroll
-            for (int k = 0; k < LY::PL; ++k) dst[l + 64 * k] = src[l + 64 * k];
-        }
+        rows_to_pool<LY>(mo, W.r0, ref, W.rows, out.pool);
         if (l < 32 && (uint32_t)l >= W.own0 && (uint32_t)l < W.n0) {
             out.bid[(size_t)cell * KB + l] = W.b0;
             out.sref[(size_t)cell * KB + l] = ref;
@@ -1009,13 +1027,7 @@ __device__ inline void flush_tables(WTabs<LY>& W, Psi& out, Bkt& bk, int cell, u
         if (ok) {
             uint32_t ref = W.gr(1 + f);
             if ((mf[f] >> l) & 1ull) ref = bbase + k0 + (uint32_t)__popcll(mf[f] & below);
-            for (uint64_t m = mf[f]; m; m &= m - 1) {
-                const int s = __ffsll((unsigned long long)m) - 1;
-                const float* src = W.rows + (size_t)(rlu(W.gr(1 + f), s) & ~CACHED) * LY::LB;
-                float* dst = bk.pool + (size_t)rlu(ref, s) * LY::LB;
-#pragma unroll
-                for (int k = 0; k < LY::PL; ++k) dst[l + 64 * k] = src[l + 64 * k];
-            }
+            rows_to_pool<LY>(mf[f], W.gr(1 + f), ref, W.rows, bk.pool);
             const uint32_t base_t = (li * 4 + f) * KB;
             if (l < 32 && (uint32_t)l < nf) {
                 bk.bid[base_t + l] = W.gb(1 + f);
@@ -2274,7 +2286,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     // workgroups fit, but more cells overflow the rows and take the serial apply.
     constexpr int NLT = TW >= 4 ? LY::NL : TW == 2 ? LY::NL / 2 : (LY::NL * 7) / 12;
     static_assert(QCAP >= LY::LB, "a slot's non-zero components must fit one round's queue");
-    __shared__ float s_rows[NLT * LY::LB];
+    __shared__ __align__(16) float s_rows[NLT * LY::LB];
     __shared__ FastLds<LY> s_fast[TW];
     __shared__ uint32_t s_q[QCAP];
     __shared__ float s_qw[QCAP];
@@ -2727,7 +2739,7 @@ __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const ui
                                                uint32_t* moved) {
     if (ctl[C_DONE]) return;
     constexpr int NLG = KIB * 256 / LY::LB;
-    __shared__ float s_rows[NLG * LY::LB];
+    __shared__ __align__(16) float s_rows[NLG * LY::LB];
     const uint32_t tag = 2 * ctl[C_EP] + (uint32_t)fine;
     const int l = lane_id();
     const uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
@@ -2854,13 +2866,7 @@ __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const ui
         }
         // staged payloads back to the pool: existing rows in place, new rows with one reservation
         const uint64_t mp = __ballot(l < 32 && (uint32_t)l < W.own0 && (W.r0 & CACHED));
-        for (uint64_t m = mp; m; m &= m - 1) {
-            const int s = __ffsll((unsigned long long)m) - 1;
-            const float* src = W.rows + (size_t)(rlu(W.r0, s) & ~CACHED) * LY::LB;
-            float* dst = out.pool + (size_t)rlu(W.orig, s) * LY::LB;
-#pragma unroll
-            for (int k = 0; k < LY::PL; ++k) dst[l + 64 * k] = src[l + 64 * k];
-        }
+        rows_to_pool<LY>(mp, W.r0, W.orig, W.rows, out.pool);
         const uint64_t mo = __ballot(l < 32 && (uint32_t)l >= W.own0 && (uint32_t)l < W.n0 && (W.r0 & CACHED));
         const uint32_t co = (uint32_t)__popcll(mo);
         uint32_t base = 0;
@@ -2873,13 +2879,7 @@ __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const ui
         if (ok) {
             uint32_t ref = W.r0;
             if ((mo >> l) & 1ull) ref = base + (uint32_t)__popcll(mo & below);
-            for (uint64_t m = mo; m; m &= m - 1) {
-                const int s = __ffsll((unsigned long long)m) - 1;
-                const float* src = W.rows + (size_t)(rlu(W.r0, s) & ~CACHED) * LY::LB;
-                float* dst = out.pool + (size_t)rlu(ref, s) * LY::LB;
-#pragma unroll
-                for (int k = 0; k < LY::PL; ++k) dst[l + 64 * k] = src[l + 64 * k];
-            }
+            rows_to_pool<LY>(mo, W.r0, ref, W.rows, out.pool);
             if (l < 32 && (uint32_t)l >= W.own0 && (uint32_t)l < W.n0) {
                 out.bid[(size_t)d * KB + l] = W.b0;
                 out.sref[(size_t)d * KB + l] = ref;

@@ -420,8 +420,14 @@ __global__ void k_table_plain(Tabs T, const float* __restrict__ mrho, const floa
 // 16s + l%16) is then, after bias, GELU and rounding to BF16, the B operand of K step t of the
 // next layer, with no data movement between layers.  Every lane of the wavefront must call it.
 // ------------------------------------------------------------------------------------------
-__device__ inline bf16x4 ld_bf16x4(const uint16_t* q) {  // 4 BF16 at an 8-byte aligned address
-    const uint2 v = *reinterpret_cast<const uint2*>(q);
+// Global-memory views of the weights (generic pointers would compile to flat loads, which also
+// hold the LDS counter of the calling transport).
+typedef const __attribute__((address_space(1))) uint16_t* gbf16p;
+typedef const __attribute__((address_space(1))) float* gcf32p;
+
+__device__ inline bf16x4 ld_bf16x4(gbf16p q) {  // 4 BF16 at an 8-byte aligned address
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    const u32x2 v = *reinterpret_cast<const __attribute__((address_space(1))) u32x2*>(q);
     bf16x4 r;
     r[0] = (short)(v.x & 0xFFFFu);
     r[1] = (short)(v.x >> 16);
@@ -432,13 +438,13 @@ __device__ inline bf16x4 ld_bf16x4(const uint16_t* q) {  // 4 BF16 at an 8-byte 
 
 // One layer of 128 outputs over K steps of 16 inputs: out[t] = BF16(GELU(W in + b)).
 template <int KSTEPS, int LD>
-__device__ inline void mlp_tile_layer(const uint16_t* __restrict__ W, const float* __restrict__ b, const bf16x4* in,
+__device__ inline void mlp_tile_layer(gbf16p __restrict__ W, gcf32p __restrict__ b, const bf16x4* in,
                                       bf16x4* out) {
     const int l = (int)(threadIdx.x & 63), g = l >> 4, c = l & 15;
 #pragma unroll
     for (int t = 0; t < NH / 16; ++t) {
         f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        const uint16_t* row = W + (size_t)(16 * t + c) * LD + 4 * g;
+        const gbf16p row = W + (size_t)(16 * t + c) * LD + 4 * g;
 #pragma unroll
         for (int kk = 0; kk < KSTEPS; ++kk)
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_bf16x4(row + 16 * kk), in[kk], acc, 0, 0, 0);
@@ -451,9 +457,11 @@ __device__ inline void mlp_tile_layer(const uint16_t* __restrict__ W, const floa
 }
 
 // Decoded outputs (ln sigma_E, ln sigma_theta, a) of this lane's sample, input x[4] (normalised).
-__device__ inline void mlp_wave_forward(const uint16_t* __restrict__ wb, const float* __restrict__ p,
+__device__ __attribute__((always_inline)) inline void mlp_wave_forward(const uint16_t* __restrict__ wb_, const float* __restrict__ p_,
                                         const float* dict4, const float x[4], float& lsE_out, float& lsT_out,
                                         float& a_out) {
+    const gbf16p wb = (gbf16p)wb_;
+    const gcf32p p = (gcf32p)p_;
     const int l = (int)(threadIdx.x & 63), g = l >> 4, c = l & 15;
     const float eLo = dict4[0], eHi = dict4[1], tLo = dict4[2], tHi = dict4[3];
     lsE_out = lsT_out = a_out = 0.0f;
@@ -475,7 +483,7 @@ __device__ inline void mlp_wave_forward(const uint16_t* __restrict__ wb, const f
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
             f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-            const uint16_t* row = wb + BOFF_W3 + (size_t)(16 * t + c) * NH + 4 * g;
+            const gbf16p row = wb + BOFF_W3 + (size_t)(16 * t + c) * NH + 4 * g;
 #pragma unroll
             for (int kk = 0; kk < NH / 16; ++kk)
                 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_bf16x4(row + 16 * kk), h0[kk], acc, 0, 0, 0);
