@@ -319,7 +319,8 @@ __global__ __launch_bounds__(256) void k_flag_count(const uint8_t* __restrict__ 
 // ends the transport (C_DONE, C_DONE_IT); with K = 1 the total goes to *total1.
 template <int K>
 __global__ __launch_bounds__(1024) void k_scan(uint32_t* tcount, const uint32_t* n_items, uint32_t per_tile,
-                                               uint32_t* ctl, uint32_t* total1, uint32_t cap, uint32_t* flags) {
+                                               uint32_t* ctl, uint32_t* total1, uint32_t cap, uint32_t* flags,
+                                               uint32_t skip) {
     if (ctl[C_DONE]) {
         if (K == 2 && threadIdx.x == 0) {
             ctl[C_NACT] = 0;
@@ -365,6 +366,8 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t* tcount, const uint32_t*
             if (over || (carry[0] == 0 && carry[1 % K] == 0)) {
                 ctl[C_DONE] = 1;
                 ctl[C_DONE_IT] = ctl[C_IT];
+            } else if (((skip & 2u) && carry[0]) || ((skip & 1u) && carry[1 % K])) {
+                ctl[C_DONE] = 2;  // a list needs a phase this chunk left out: pause before it runs
             }
         } else {
             *total1 = over ? 0u : carry[0];

@@ -180,7 +180,7 @@ extern "C" int sm2d_k1k2_mask_compact_noalloc(int Nx, int Nz, int local_bins, co
                                           b_E_fine_off, weight_active_min);
     const int gt = blocks_for(n, 256, 4096);
     k_flag_count<<<gt, 256, 0, s>>>(flag, sw + SW_N, tiles, sw);
-    k_scan<2><<<1, 1024, 0, s>>>(tiles, sw + SW_N, 256, sw, nullptr, n, sw + SW_FLAGS);
+    k_scan<2><<<1, 1024, 0, s>>>(tiles, sw + SW_N, 256, sw, nullptr, n, sw + SW_FLAGS, 0u);
     k_dense_emit<<<gt, 256, 0, s>>>(flag, sw, tiles, d_active, d_coarse, d_counts);
     HIPCHK(hipGetLastError());
     return SM2D_OK;

@@ -296,7 +296,7 @@ void build_occ(sm2d_ctx* c, int k) {
     k_occ_count<<<nt, 256, 0, c->stream>>>(c->bitmap.p, c->nwords, c->tcount2.p, c->ctl.p);
     // n_items = nwords is held in ctl[15]
     k_scan<1><<<1, 1024, 0, c->stream>>>(c->tcount2.p, c->ctl.p + 15, 256, c->ctl.p, c->nocc(k), c->ocap,
-                                         c->flags.p);
+                                         c->flags.p, 0u);
     k_occ_emit<<<nt, 256, 0, c->stream>>>(c->bitmap.p, c->nwords, c->tcount2.p, c->occ[k].p, c->ocap, c->ctl.p);
 }
 
@@ -335,7 +335,8 @@ void launch_mask_compact(sm2d_ctx* c, const Tp& P, int in, int grid_entries, boo
         return;
     }
     k_flag_count<<<gt, 256, 0, c->stream>>>(c->flag.p, c->nocc(in), c->tcount.p, c->ctl.p);
-    k_scan<2><<<1, 1024, 0, c->stream>>>(c->tcount.p, c->nocc(in), 256, c->ctl.p, nullptr, c->lcap, c->flags.p);
+    k_scan<2><<<1, 1024, 0, c->stream>>>(c->tcount.p, c->nocc(in), 256, c->ctl.p, nullptr, c->lcap, c->flags.p,
+                                          skip);
     k_emit<<<gt, 256, 0, c->stream>>>(c->occ[in].p, c->nocc(in), c->flag.p, c->tcount.p, c->act.p, c->crs.p,
                                       c->ltag.p, c->lpos.p, c->bitmap.p, c->ctl.p);
 }
@@ -383,7 +384,11 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
         double* acc = fine ? c->acc_f.p : c->acc_c.p;
         const int e0 = fine ? 6 : 2;
         const int tw = Tuner::tw(c->cand[fine]);
-        const int gb = blocks_for(pow2_at_least(std::max<uint32_t>(c->hint_ph[fine], 64)), 1, 65536);
+        // launch sizes of the phase from its own list hint (a phase with a short or empty list,
+        // such as the fine phase while no component has slowed down, launches small grids; every
+        // kernel strides over its actual list)
+        const uint32_t hp = pow2_at_least(std::max<uint32_t>(c->hint_ph[fine], 64));
+        const int gb = blocks_for(hp, 1, 65536);
         // timed launches take their events from the dispatch itself (the kernel's own start and
         // end), so stage times and the tuner see the kernel without marker packets around it
         hipEvent_t e_start = c->rec ? c->ev[(size_t)c->ev_it * sm2d_ctx::EPI + e0] : nullptr;
@@ -425,11 +430,11 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
                                                                   c->ctl.p, c->flags.p);
         uint32_t* seg = c->ctl.p + C_DSEG + (fine ? NSH * SH_STRIDE : 0);
         const uint32_t seg_cap = c->dcap / NSH;
-        k_dest<<<blocks_for(4ull * h, 256, 4096), 256, 0, c->stream>>>(list, nl, c->g.Nx, c->g.Nz, c->Ns, c->bn.p,
+        k_dest<<<blocks_for(4ull * hp, 256, 4096), 256, 0, c->stream>>>(list, nl, c->g.Nx, c->g.Nz, c->Ns, c->bn.p,
                                                                        c->dtag.p, c->ctl.p, fine, c->dlist.p, seg,
                                                                        seg_cap, c->bitmap.p, c->flags.p);
         uint32_t* moved = c->prof ? c->gmoved.p + (size_t)(2 * c->ev_it + fine) * NSH * SH_STRIDE : nullptr;
-        const unsigned gg = (blocks_for(4ull * h, 1, 65536) + NSH - 1) / NSH * NSH;
+        const unsigned gg = (blocks_for(4ull * hp, 1, 65536) + NSH - 1) / NSH * NSH;
         const int kib = Tuner::kib(c->cand[2 + fine]);
 #define SM2D_GATHER(K)                                                                                        \
     do {                                                                                                      \
@@ -607,8 +612,7 @@ int launch_chunk_graph(sm2d_ctx* c, const Tp& P, int R, bool first_opens, bool f
     // inside a chunk, each iteration's closing kernel builds the next one's short lists
     const bool fold = fused_open && pow2_at_least(c->hint_occ) <= SMALL_LIST;
     for (int r = 0; r < R; ++r)
-        launch_iteration<LY>(c, P, first_opens && r == 0, fused_open, fold && r > 0, fold && r + 1 < R,
-                             fold ? skip : 0u);
+        launch_iteration<LY>(c, P, first_opens && r == 0, fused_open, fold && r > 0, fold && r + 1 < R, skip);
     hipGraph_t graph = nullptr;
     const hipError_t le = hipGetLastError();
     const hipError_t ce = hipStreamEndCapture(c->stream, &graph);

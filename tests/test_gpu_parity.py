@@ -357,6 +357,30 @@ def test_phase_skipping_chunks_bit_exact(hip, orc, fine_on):
     assert rep["pauses"] == 1 if fine_on == 10.0 else rep["pauses"] <= 1
 
 
+def test_phase_skipping_long_lists(hip):
+    """Phase skipping with lists too long for the one-workgroup compaction: 128 spots on the
+    1029-bin grid start coarse-only (the fine kernels stay out of the graph chunks) until the
+    first fine list, which k_scan<2> turns into a pause.  The graph run equals the eager run
+    (no chunk leaves a phase out there) spot by spot."""
+    prob = problems.config4_fine_grid(n_spots=128, n_samples=300)
+    spots = [prob.make_spot(i) for i in range(len(prob.spots))]
+    tp = prob.make_transport()
+    g = problems.setup_engine(hip, prob, Engine, n_spots=len(spots))
+    g.set_transport_waves(1)  # no tuner exploration: every chunk is a graph chunk
+    g.set_graphs(False)
+    ee, pe, ae = g.run_batch(spots, tp, per_spot=True)
+    g.set_graphs(True)
+    eg, pg, ag = g.run_batch(spots, tp, per_spot=True)
+    rep = g.tuner_report()["chunk_graphs"]
+    g.close()
+    assert ae.fine_cell_bin_updates > 0 and ae.iterations == ag.iterations
+    assert rep["phase_skipping_chunks"] > 0 and rep["pauses"] == 1, rep
+    assert ag.cell_bin_updates == ae.cell_bin_updates
+    assert np.array_equal(eg, ee)
+    for k in range(len(spots)):
+        assert np.array_equal(pg[k], pe[k]), k
+
+
 def _saturating_psi(rng, g, LB):
     """Cells spaced 7 apart in x and 3 in z (no lateral emission into another listed cell), each
     with 32 occupied blocks over the upper half of the energy range and 8-39 weighted bins per
