@@ -1068,10 +1068,10 @@ __device__ inline bool kacc_is_float(int a) { return a == K_WCUT || a == K_WNUC 
 template <class LY>
 struct FastLds {
     static constexpr int MR = LY::XS + 4;  // records per lane: x_sub spread + two two-bin side emissions
-    uint32_t rec[64 * MR];   // (key index << 16 | lidx), then the LDS payload address
+    uint16_t rec[64 * MR];   // (key index << 12 | lidx), then the LDS payload address (< 6144)
     float recw[64 * MR];
     uint32_t key[64 * MAXKEY];  // claim keys; the claims reuse it for the new keys in order
-    int nkey[64], nrec[64];
+    uint8_t nkey[64], nrec[64];
 };
 
 struct LaneOut {
@@ -1177,7 +1177,7 @@ __device__ inline void rec_emit(FastLds<LY>& F, LaneOut& o, int t, const Bins& b
         const int k = o.nkey++;
         F.key[l * MAXKEY + k] = ((uint32_t)t << 24) | bid;
         const int r = o.nrec++;
-        F.rec[l * FastLds<LY>::MR + r] = ((uint32_t)k << 16) | (uint32_t)lidx;
+        F.rec[l * FastLds<LY>::MR + r] = (uint16_t)(((uint32_t)k << 12) | (uint32_t)lidx);
         F.recw[l * FastLds<LY>::MR + r] = we;
         o.clwout += we;
         o.cleout += (double)we * (double)b.Ecen;
@@ -1287,7 +1287,7 @@ __device__ inline LaneOut fast_prepare(FastLds<LY>& F, const Comp& r, bool live,
                     o.clwout += (double)wsp;
                     o.cleout += (double)wsp * (double)E_emit;
                     const int rr = o.nrec++;
-                    F.rec[l * FastLds<LY>::MR + rr] = (uint32_t)LY::enc(tbn % LY::TL, ebn % LY::EL, q, zst);
+                    F.rec[l * FastLds<LY>::MR + rr] = (uint16_t)LY::enc(tbn % LY::TL, ebn % LY::EL, q, zst);
                     F.recw[l * FastLds<LY>::MR + rr] = wsp;
                 }
             }
@@ -1332,8 +1332,8 @@ __device__ inline LaneOut fast_prepare(FastLds<LY>& F, const Comp& r, bool live,
 // LDS: keys found in it skip the serial loop over distinct keys.  Only slots whose payload is
 // an LDS row enter it, and a cell's slots keep their rows until the cell ends, so a hit gives
 // the loop's answer.
-constexpr int NKC = 64;
-__device__ inline uint32_t kc_slot(uint32_t K) { return (K * 2654435761u) >> 26; }
+constexpr int NKC = 32;
+__device__ inline uint32_t kc_slot(uint32_t K) { return (K * 2654435761u) >> 27; }
 
 template <class LY>
 __device__ inline bool fast_claims(FastLds<LY>& F, WTabs<LY>& W, uint32_t* kc) {
@@ -1430,16 +1430,16 @@ __device__ inline bool fast_claims(FastLds<LY>& F, WTabs<LY>& W, uint32_t* kc) {
     }
     for (int rr = 0; rr < nrec; ++rr) {
         const uint32_t p = F.rec[l * FastLds<LY>::MR + rr];
-        const uint32_t kidx = p >> 16;
+        const uint32_t kidx = p >> 12;
         const uint32_t row = kidx == 0 ? r0 : kidx == 1 ? r1 : kidx == 2 ? r2 : kidx == 3 ? r3 : r4;
-        F.rec[l * FastLds<LY>::MR + rr] = row * (uint32_t)LY::LB + (p & 0xFFFFu);
+        F.rec[l * FastLds<LY>::MR + rr] = (uint16_t)(row * (uint32_t)LY::LB + (p & 0xFFFu));
     }
     return true;
 }
 
 // Size of the per-workgroup owner table of the record rounds (hashed payload addresses), 2 KiB:
 // addresses that share a slot only wait a round.
-constexpr int NOWN_BITS = 8;  // 256 owner slots (1 KiB of LDS)
+constexpr int NOWN_BITS = 7;  // 128 owner slots (512 B of LDS)
 constexpr int NOWN = 1 << NOWN_BITS;
 
 __device__ inline uint32_t own_slot(uint32_t addr) { return (addr * 2654435761u) >> (32 - NOWN_BITS); }
@@ -2284,14 +2284,17 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
                                                    Tally tal, double* acc, uint32_t* flags) {
     if (ctl[C_DONE] || blockIdx.x >= *nlist) return;  // no cell for this workgroup
     // LDS payload rows: 4-wave workgroups (few listed cells, one workgroup per CU) keep the full
-    // set (24 KiB); 2-wave ones 12 KiB and 1-wave ones 14 KiB, so that six 1-wave workgroups
-    // (26 KiB of LDS each) fit a CU.  Fewer rows were measured slower: with 10 KiB of rows seven
-    // workgroups fit, but more cells overflow the rows and take the serial apply.
-    constexpr int NLT = TW >= 4 ? LY::NL : TW == 2 ? LY::NL / 2 : (LY::NL * 7) / 12;
+    // set (24 KiB); 2-wave ones 12 KiB and 1-wave ones 13 KiB, so that seven 1-wave workgroups
+    // (21.75 KiB of LDS each, with 16-bit queue entries and records) fit a CU.  Fewer rows were
+    // measured slower: with 11 KiB of rows eight workgroups fit, but more cells overflow the
+    // rows and take the serial apply.
+    constexpr int NLT = TW >= 4 ? LY::NL : TW == 2 ? LY::NL / 2 : (LY::NL * 13) / 24;
     static_assert(QCAP >= LY::LB, "a slot's non-zero components must fit one round's queue");
+    static_assert(LY::LB <= 256 && KB <= 256, "queue entries pack the slot and the local bin in 16 bits");
+    static_assert(LY::NL * LY::LB <= 65536, "record addresses are 16-bit LDS payload offsets");
     __shared__ __align__(16) float s_rows[NLT * LY::LB];
     __shared__ FastLds<LY> s_fast[TW];
-    __shared__ uint32_t s_q[QCAP];
+    __shared__ uint16_t s_q[QCAP];  // (slot << 8 | lidx) of the queued non-zero components
     __shared__ float s_qw[QCAP];
     __shared__ uint32_t s_bid[KB], s_ref[KB], s_cnt[KB], s_off[KB];
     __shared__ uint64_t s_mask[TW];
@@ -2304,7 +2307,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     for (int i = threadIdx.x; i < NOWN; i += blockDim.x) s_own[i] = ~0u;
     // The energy tables (edges, ranges, logs): 4-wave workgroups (one per CU) copy them into LDS
     // (dynamic LDS, transport_dyn_lds); the 1- and 2-wave shapes read them through L1/L2, since
-    // 4 KiB more per workgroup would fit five one-wave workgroups per CU instead of six.
+    // 4 KiB more per workgroup would fit one one-wave workgroup less per CU.
     Tabs T = Tg;
     if constexpr (TW == 4) {
         extern __shared__ float s_tab[];
@@ -2490,7 +2493,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
                             const uint64_t mk = __ballot(w > 0.0f);
                             if (w > 0.0f) {
                                 const uint32_t p = pos + (uint32_t)__popcll(mk & below);
-                                s_q[p] = (su << 16) | (uint32_t)(l + 64 * k);
+                                s_q[p] = (uint16_t)((su << 8) | (uint32_t)(l + 64 * k));
                                 s_qw[p] = w;
                             }
                             pos += (uint32_t)__popcll(mk);
@@ -2522,7 +2525,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
                         const uint64_t mk = __ballot(w > 0.0f);
                         if (w > 0.0f) {
                             const uint32_t p = pos + (uint32_t)__popcll(mk & below);
-                            s_q[p] = (su << 16) | (uint32_t)(l + 64 * k);
+                            s_q[p] = (uint16_t)((su << 8) | (uint32_t)(l + 64 * k));
                             s_qw[p] = w;
                         }
                         pos += (uint32_t)__popcll(mk);
@@ -2538,11 +2541,11 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
                     const uint32_t i = b * 64 + (uint32_t)l;
                     const bool live = i < total;
                     const uint32_t it = live ? s_q[i] : 0u;
-                    const uint32_t slot = it >> 16, bid = s_bid[slot];
+                    const uint32_t slot = it >> 8, bid = s_bid[slot];
                     const Learned Lx = learned_for<LY, FINE, INFER>(T, P, G, live, live ? s_qw[i] : 0.0f,
-                                                             (int)(it & 0xFFFFu), bid & 0xFFF, (bid >> 12) & 0xFFF);
+                                                             (int)(it & 0xFFu), bid & 0xFFF, (bid >> 12) & 0xFFF);
                     const Comp r = transport_component<LY, FINE>(T, P, G, live, live ? s_qw[i] : 0.0f,
-                                                                 (int)(it & 0xFFFFu), slot, bid & 0xFFF,
+                                                                 (int)(it & 0xFFu), slot, bid & 0xFFF,
                                                                  (bid >> 12) & 0xFFF, Lx SM2D_SUBA(sc_));
                     const LaneOut o = fast_prepare<LY, FINE>(s_fast[wv], r, live, G.ix, G.iz, G.Nx, T.Nz);
                     s_fast[wv].nkey[l] = o.nkey;
@@ -2596,12 +2599,12 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
                             const uint32_t i = bb * 64 + (uint32_t)l;
                             const bool live = i < total;
                             const uint32_t it = live ? s_q[i] : 0u;
-                            const uint32_t slot = it >> 16, bid = s_bid[slot];
+                            const uint32_t slot = it >> 8, bid = s_bid[slot];
                             const Learned Lx = learned_for<LY, FINE, INFER>(T, P, G, live, live ? s_qw[i] : 0.0f,
-                                                                     (int)(it & 0xFFFFu), bid & 0xFFF,
+                                                                     (int)(it & 0xFFu), bid & 0xFFF,
                                                                      (bid >> 12) & 0xFFF);
                             const Comp r = transport_component<LY, FINE>(T, P, G, live, live ? s_qw[i] : 0.0f,
-                                                                         (int)(it & 0xFFFFu), slot, bid & 0xFFF,
+                                                                         (int)(it & 0xFFu), slot, bid & 0xFFF,
                                                                          (bid >> 12) & 0xFFF, Lx SM2D_SUBA(nullptr));
                             CellAccum mine = {};
                             serial_apply<LY, FINE>(r, m, W, mine, comps, T, out, bk, flags, G);
