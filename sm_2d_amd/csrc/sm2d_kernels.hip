@@ -986,21 +986,33 @@ __device__ inline void rows_to_pool(uint64_t m, uint32_t xr, uint32_t yr, const 
     }
 }
 
+// Size of the claimed-key cache of fast_claims (entries) and its hash.
+constexpr int NKC = 32;
+__device__ inline uint32_t kc_slot(uint32_t K) { return (K * 2654435761u) >> 27; }
+
+// Home pool rows of a transport cell's slots (LDS, table t slot s at home[t * KB + s]): the pool
+// row a slot's payload was moved to by evict_rows, or read from by a reload, or (psi_out slots
+// that held weight when the cell started) its original payload; NOHOME otherwise.
+constexpr uint32_t NOHOME = 0xFFFFFFFFu;
+
+// End of a transport cell: LDS rows go to their home rows, rows without one to rows reserved now
+// (one sharded reservation per pool); then the psi_out table and the four bucket tables.
 template <class LY>
-__device__ inline void flush_tables(WTabs<LY>& W, Psi& out, Bkt& bk, int cell, uint32_t li, uint32_t* flags) {
+__device__ inline void flush_tables(WTabs<LY>& W, Psi& out, Bkt& bk, int cell, uint32_t li, const uint32_t* home,
+                                    uint32_t* flags) {
     const int l = lane_id();
     const uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
-    // own psi_out: preloaded slots go back to their payloads, new cached slots get new ones
-    const uint64_t mp = __ballot(l < 32 && (uint32_t)l < W.own0 && (W.r0 & CACHED));
-    rows_to_pool<LY>(mp, W.r0, W.orig, W.rows, out.pool);
-    const uint64_t mo = __ballot(l < 32 && (uint32_t)l >= W.own0 && (uint32_t)l < W.n0 && (W.r0 & CACHED));
-    const uint32_t co = (uint32_t)__popcll(mo);
-    uint64_t mf[4];
-    uint32_t cb = 0;
+    uint64_t mc[5], mn[5];
+    uint32_t hh[5];
+    uint32_t co = 0, cb = 0;
 #pragma unroll
-    for (int f = 0; f < 4; ++f) {
-        mf[f] = __ballot(l < 32 && (uint32_t)l < W.gn(1 + f) && (W.gr(1 + f) & CACHED));
-        cb += (uint32_t)__popcll(mf[f]);
+    for (int t = 0; t < 5; ++t) {
+        hh[t] = l < 32 ? home[t * KB + l] : NOHOME;
+        const bool c = l < 32 && (uint32_t)l < W.gn(t) && (W.gr(t) & CACHED);
+        mc[t] = __ballot(c);
+        mn[t] = __ballot(c && hh[t] == NOHOME);
+        if (t == 0) co = (uint32_t)__popcll(mn[0]);
+        else cb += (uint32_t)__popcll(mn[t]);
     }
     // both pool reservations in one pass: lane 0 psi_out's, lane 1 the buckets'
     uint32_t take = 0;
@@ -1011,9 +1023,11 @@ __device__ inline void flush_tables(WTabs<LY>& W, Psi& out, Bkt& bk, int cell, u
     bool ok = !co || base != POOL_FULL;
     if (!ok && l == 0) atomicOr(&flags[F_PSI_FULL], 1u);
     if (ok) {
-        uint32_t ref = W.r0;
-        if ((mo >> l) & 1ull) ref = base + (uint32_t)__popcll(mo & below);
-        rows_to_pool<LY>(mo, W.r0, ref, W.rows, out.pool);
+        uint32_t dst = hh[0];
+        if ((mn[0] >> l) & 1ull) dst = base + (uint32_t)__popcll(mn[0] & below);
+        rows_to_pool<LY>(mc[0], W.r0, dst, W.rows, out.pool);
+        const uint32_t ref = ((mc[0] >> l) & 1ull) ? dst : W.r0;
+        // slots that held weight keep their payload reference
         if (l < 32 && (uint32_t)l >= W.own0 && (uint32_t)l < W.n0) {
             out.bid[(size_t)cell * KB + l] = W.b0;
             out.sref[(size_t)cell * KB + l] = ref;
@@ -1028,18 +1042,66 @@ __device__ inline void flush_tables(WTabs<LY>& W, Psi& out, Bkt& bk, int cell, u
     for (int f = 0; f < 4; ++f) {
         const uint32_t nf = ok ? W.gn(1 + f) : 0u;
         if (ok) {
-            uint32_t ref = W.gr(1 + f);
-            if ((mf[f] >> l) & 1ull) ref = bbase + k0 + (uint32_t)__popcll(mf[f] & below);
-            rows_to_pool<LY>(mf[f], W.gr(1 + f), ref, W.rows, bk.pool);
+            uint32_t dst = hh[1 + f];
+            if ((mn[1 + f] >> l) & 1ull) dst = bbase + k0 + (uint32_t)__popcll(mn[1 + f] & below);
+            rows_to_pool<LY>(mc[1 + f], W.gr(1 + f), dst, W.rows, bk.pool);
+            const uint32_t ref = ((mc[1 + f] >> l) & 1ull) ? dst : W.gr(1 + f);
             const uint32_t base_t = (li * 4 + f) * KB;
             if (l < 32 && (uint32_t)l < nf) {
                 bk.bid[base_t + l] = W.gb(1 + f);
                 bk.sref[base_t + l] = ref;
             }
-            k0 += (uint32_t)__popcll(mf[f]);
+            k0 += (uint32_t)__popcll(mn[1 + f]);
         }
         if (l == 0) bk.n[li * 4 + f] = nf;
     }
+}
+
+// Mid-cell eviction, when a batch's new keys would need more LDS rows than remain: every LDS row
+// goes to its slot's home row (reserved now where there is none), the slots then refer to those
+// pool rows, and the LDS rows are free again; fast_claims copies a slot back into an LDS row when
+// a later batch adds to it.  The payload values move unchanged, so the adds keep their order and
+// their results.  Returns false (with the pool's full flag) when a reservation fails.
+template <class LY>
+__device__ inline bool evict_rows(WTabs<LY>& W, Psi& out, Bkt& bk, uint32_t* home, uint32_t* kc, uint32_t* flags) {
+    const int l = lane_id();
+    const uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
+    uint64_t mc[5], mn[5];
+    uint32_t hh[5];
+    uint32_t co = 0, cb = 0;
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+        hh[t] = l < 32 ? home[t * KB + l] : NOHOME;
+        const bool c = l < 32 && (uint32_t)l < W.gn(t) && (W.gr(t) & CACHED);
+        mc[t] = __ballot(c);
+        mn[t] = __ballot(c && hh[t] == NOHOME);
+        if (t == 0) co = (uint32_t)__popcll(mn[0]);
+        else cb += (uint32_t)__popcll(mn[t]);
+    }
+    uint32_t take = 0;
+    if ((l == 0 && co) || (l == 1 && cb))
+        take = pool_take(l == 0 ? out.pcount : bk.pcount, l == 0 ? out.pcap : bk.pcap, l == 0 ? co : cb,
+                         blockIdx.x);
+    const uint32_t base = rlu(take, 0), bbase = rlu(take, 1);
+    if ((co && base == POOL_FULL) || (cb && bbase == POOL_FULL)) {
+        if (l == 0) atomicOr(&flags[co && base == POOL_FULL ? F_PSI_FULL : F_BKT_FULL], 1u);
+        return false;
+    }
+    uint32_t k0 = 0;
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+        uint32_t dst = hh[t];
+        if ((mn[t] >> l) & 1ull) dst = (t == 0 ? base : bbase + k0) + (uint32_t)__popcll(mn[t] & below);
+        rows_to_pool<LY>(mc[t], W.gr(t), dst, W.rows, t == 0 ? out.pool : bk.pool);
+        const bool mine = (mc[t] >> l) & 1ull;
+        if (mine) home[t * KB + l] = dst;
+        W.sr(t, dst, mine);
+        if (t > 0) k0 += (uint32_t)__popcll(mn[t]);
+    }
+    W.used = 0;
+    if (l < NKC) kc[l] = EMPTY;  // the key cache maps keys to LDS rows
+    __threadfence_block();        // the pool rows are complete before a reload reads one back
+    return true;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1324,27 +1386,25 @@ __device__ inline LaneOut fast_prepare(FastLds<LY>& F, const Comp& r, bool live,
 
 // Claims of the batch's distinct keys in order of first appearance, with the serial claims'
 // outcome: a key found in its table takes that slot, any other key takes its table's next free
-// slot and the next LDS row.  The loop over distinct keys only decides; the tables and rows
-// change once every key is decided and the capacities are checked, so a false return (a table
-// would overflow into the closest-block fallback, a slot's payload is not in LDS, or the LDS
-// rows run out) leaves them untouched for the serial path.  Resolves record addresses.
-// Direct-mapped cache of the current cell's claimed keys (key -> LDS row) in 2 x 64 words of
+// slot and the next LDS row.  A found slot whose payload is a pool row (moved out by
+// evict_rows, or claimed by the serial path) is copied back into the next LDS row, the pool row
+// becoming its home.  The loop over distinct keys only decides; the tables and rows change once
+// every key is decided and the capacities are checked, so a return other than FC_OK leaves them
+// untouched: FC_ROWS (the LDS rows would run out: evict and try again) or FC_SERIAL (a table
+// would overflow into the closest-block fallback: the serial path).  Resolves record addresses.
+// Direct-mapped cache of the current cell's claimed keys (key -> LDS row) in 2 x NKC words of
 // LDS: keys found in it skip the serial loop over distinct keys.  Only slots whose payload is
-// an LDS row enter it, and a cell's slots keep their rows until the cell ends, so a hit gives
-// the loop's answer.
-constexpr int NKC = 32;
-__device__ inline uint32_t kc_slot(uint32_t K) { return (K * 2654435761u) >> 27; }
+// an LDS row enter it, and evict_rows clears it, so a hit gives the loop's answer.
+enum { FC_SERIAL = 0, FC_OK = 1, FC_ROWS = 2 };
+constexpr uint32_t RELOAD = 0x80000000u;  // pending-key flag: a found slot copied back into LDS
 
+// One attempt with the lane's keys k0..k4 in registers: the attempt reuses F.key for its list
+// of pending keys, so the lane's keys are read from F.key once, before the first attempt.
 template <class LY>
-__device__ inline bool fast_claims(FastLds<LY>& F, WTabs<LY>& W, uint32_t* kc) {
+__device__ inline int fast_claims_try(FastLds<LY>& F, WTabs<LY>& W, uint32_t* kc, uint32_t* home,
+                                      const float* psi_pool, const float* bk_pool, int nkey, int nrec, uint32_t k0,
+                                      uint32_t k1, uint32_t k2, uint32_t k3, uint32_t k4) {
     const int l = lane_id();
-    const int nkey = F.nkey[l], nrec = F.nrec[l];
-    uint32_t k0 = EMPTY, k1 = EMPTY, k2 = EMPTY, k3 = EMPTY, k4 = EMPTY;
-    if (nkey > 0) k0 = F.key[l * MAXKEY + 0];
-    if (nkey > 1) k1 = F.key[l * MAXKEY + 1];
-    if (nkey > 2) k2 = F.key[l * MAXKEY + 2];
-    if (nkey > 3) k3 = F.key[l * MAXKEY + 3];
-    if (nkey > 4) k4 = F.key[l * MAXKEY + 4];
     uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;
     uint32_t pend = (nkey > 0) ? ((1u << nkey) - 1u) : 0u;
     {
@@ -1381,12 +1441,17 @@ __device__ inline bool fast_claims(FastLds<LY>& F, WTabs<LY>& W, uint32_t* kc) {
         uint32_t row;
         if (found) {
             const uint32_t ref = rlu(W.gr(t), __ffsll((unsigned long long)found) - 1);
-            if (!(ref & CACHED)) return false;
-            row = ref & ~CACHED;
-            if (l == 0) {
-                const uint32_t h = kc_slot(K);
-                kc[h] = K;
-                kc[NKC + h] = row;
+            if (ref & CACHED) {
+                row = ref & ~CACHED;
+                if (l == 0) {
+                    const uint32_t h = kc_slot(K);
+                    kc[h] = K;
+                    kc[NKC + h] = row;
+                }
+            } else {  // copied back into the next LDS row at the commit
+                if (l == 0 && nnew < 64 * MAXKEY) F.key[nnew] = K | RELOAD;
+                row = used0 + nnew;
+                ++nnew;
             }
         } else {
             if (l == 0 && nnew < 64 * MAXKEY) F.key[nnew] = K;
@@ -1401,28 +1466,43 @@ __device__ inline bool fast_claims(FastLds<LY>& F, WTabs<LY>& W, uint32_t* kc) {
         if (hit & 16u) r4 = row;
     }
     if (nnew) {
-        if (used0 + nnew > W.cap) return false;  // (cap < 64, so no 6-bit count overflowed)
 #pragma unroll
         for (int t = 0; t < 5; ++t)
-            if (W.gn(t) + ((cnt >> (6 * t)) & 63u) > (uint32_t)KB) return false;
-        // new slot i (first-appearance order) of table t takes the table's next slot and row used0 + i
+            if (W.gn(t) + ((cnt >> (6 * t)) & 63u) > (uint32_t)KB) return FC_SERIAL;
+        if (used0 + nnew > W.cap) return FC_ROWS;  // (cap < 64, so no 6-bit count overflowed)
+        // new slot i (first-appearance order) of table t takes the table's next slot and row
+        // used0 + i; a reloaded slot keeps its slot and takes row used0 + i
         const uint32_t kv = ((uint32_t)l < nnew) ? F.key[l] : EMPTY;
         uint32_t seen = 0;
         for (uint32_t i = 0; i < nnew; ++i) {
-            const uint32_t K = rlu(kv, (int)i);
+            const uint32_t KF = rlu(kv, (int)i), K = KF & ~RELOAD;
             const int t = (int)(K >> 24);
-            const uint32_t slot = W.gn(t) + ((seen >> (6 * t)) & 63u);
-            seen += 1u << (6 * t);
-            W.sb(t, K & 0xFFFFFFu, (uint32_t)l == slot);
-            W.sr(t, CACHED | (used0 + i), (uint32_t)l == slot);
+            float* row = W.rows + (size_t)(used0 + i) * LY::LB;
+            if (KF & RELOAD) {
+                // the slot the decision loop found (the tables have not changed since)
+                const uint64_t found = __ballot(l < 32 && W.gb(t) == (K & 0xFFFFFFu));
+                const int slot = found ? __ffsll((unsigned long long)found) - 1 : 0;
+                const uint32_t g = rlu(W.gr(t), slot);
+                if (found && !(g & CACHED)) {
+                    const float* src = (t == 0 ? psi_pool : bk_pool) + (size_t)g * LY::LB;
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k) row[l + 64 * k] = src[l + 64 * k];
+                    W.sr(t, CACHED | (used0 + i), l == slot);
+                    if (l == 0) home[t * KB + slot] = g;
+                }
+            } else {
+                const uint32_t slot = W.gn(t) + ((seen >> (6 * t)) & 63u);
+                seen += 1u << (6 * t);
+                W.sb(t, K & 0xFFFFFFu, (uint32_t)l == slot);
+                W.sr(t, CACHED | (used0 + i), (uint32_t)l == slot);
+#pragma unroll
+                for (int k = 0; k < LY::PL; ++k) row[l + 64 * k] = 0.0f;
+            }
             if (l == 0) {
                 const uint32_t h = kc_slot(K);
                 kc[h] = K;
                 kc[NKC + h] = used0 + i;
             }
-            float* row = W.rows + (size_t)(used0 + i) * LY::LB;
-#pragma unroll
-            for (int k = 0; k < LY::PL; ++k) row[l + 64 * k] = 0.0f;
         }
 #pragma unroll
         for (int t = 0; t < 5; ++t) W.sn(t, W.gn(t) + ((cnt >> (6 * t)) & 63u));
@@ -1434,7 +1514,27 @@ __device__ inline bool fast_claims(FastLds<LY>& F, WTabs<LY>& W, uint32_t* kc) {
         const uint32_t row = kidx == 0 ? r0 : kidx == 1 ? r1 : kidx == 2 ? r2 : kidx == 3 ? r3 : r4;
         F.rec[l * FastLds<LY>::MR + rr] = (uint16_t)(row * (uint32_t)LY::LB + (p & 0xFFFu));
     }
-    return true;
+    return FC_OK;
+}
+
+// The claims of a batch (fast_claims_try), with one eviction of the cell's LDS rows when they
+// run out: FC_OK, or FC_SERIAL for the serial path (a table would overflow, the batch alone needs
+// more rows than the cell has, or a pool is full).
+template <class LY>
+__device__ inline int fast_claims(FastLds<LY>& F, WTabs<LY>& W, uint32_t* kc, uint32_t* home, Psi& out, Bkt& bk,
+                                  uint32_t* flags) {
+    const int l = lane_id();
+    const int nkey = F.nkey[l], nrec = F.nrec[l];
+    uint32_t k0 = EMPTY, k1 = EMPTY, k2 = EMPTY, k3 = EMPTY, k4 = EMPTY;
+    if (nkey > 0) k0 = F.key[l * MAXKEY + 0];
+    if (nkey > 1) k1 = F.key[l * MAXKEY + 1];
+    if (nkey > 2) k2 = F.key[l * MAXKEY + 2];
+    if (nkey > 3) k3 = F.key[l * MAXKEY + 3];
+    if (nkey > 4) k4 = F.key[l * MAXKEY + 4];
+    int fc = fast_claims_try<LY>(F, W, kc, home, out.pool, bk.pool, nkey, nrec, k0, k1, k2, k3, k4);
+    if (fc == FC_ROWS && W.used > 0 && evict_rows<LY>(W, out, bk, home, kc, flags))
+        fc = fast_claims_try<LY>(F, W, kc, home, out.pool, bk.pool, nkey, nrec, k0, k1, k2, k3, k4);
+    return fc == FC_OK ? FC_OK : FC_SERIAL;
 }
 
 // Size of the per-workgroup owner table of the record rounds (hashed payload addresses), 2 KiB:
@@ -2304,6 +2404,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     __shared__ uint32_t s_sdev;  // bits of the largest learned-physics deviation of the cell (>= 0)
     __shared__ uint32_t s_own[NOWN];
     __shared__ uint32_t s_kc[2 * NKC];  // claimed-key cache of the cell (fast_claims)
+    __shared__ uint32_t s_home[5 * KB];  // home pool rows of the cell's slots (flush_tables)
     for (int i = threadIdx.x; i < NOWN; i += blockDim.x) s_own[i] = ~0u;
     // The energy tables (edges, ranges, logs): 4-wave workgroups (one per CU) copy them into LDS
     // (dynamic LDS, transport_dyn_lds); the 1- and 2-wave shapes read them through L1/L2, since
@@ -2413,6 +2514,8 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
             W.rows = s_rows;
             W.orig = 0u;
             W.stage_own(out);  // e.g. the coarse gather into a fine cell
+            for (int i = l; i < 5 * KB; i += 64) s_home[i] = NOHOME;
+            if ((uint32_t)l < W.used) s_home[l] = W.orig;  // psi_out slots staged from their payloads
         }
         __syncthreads();
         SM2D_STAMP(1);
@@ -2586,7 +2689,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
                     for (uint32_t bb = b0; bb < nb && bb < b0 + TW; ++bb) {
                         FastLds<LY>& F = s_fast[bb - b0];
                         const uint64_t m = s_mask[bb - b0];
-                        const bool fast = fast_claims<LY>(F, W, s_kc);
+                        const bool fast = fast_claims<LY>(F, W, s_kc, s_home, out, bk, flags) == FC_OK;
                         SM2D_STAMP(5);
                         if (fast) {
                             fast_rounds<LY>(F, W, m, s_own);
@@ -2623,7 +2726,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
             lanes_to_accum(a, accd, accf);
             a.comps = comps;
             SM2D_STAMP(8);
-            flush_tables<LY>(W, out, bk, cell, li, flags);
+            flush_tables<LY>(W, out, bk, cell, li, s_home, flags);
             SM2D_STAMP(9);
             if (l < 4) {
                 double* t = l == 0 ? tal.edep : l == 1 ? tal.E_cut : l == 2 ? tal.E_nuc : tal.E_bnd;
