@@ -2384,11 +2384,10 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
                                                    Tally tal, double* acc, uint32_t* flags) {
     if (ctl[C_DONE] || blockIdx.x >= *nlist) return;  // no cell for this workgroup
     // LDS payload rows: 4-wave workgroups (few listed cells, one workgroup per CU) keep the full
-    // set (24 KiB); 2-wave ones 12 KiB and 1-wave ones 13 KiB, so that seven 1-wave workgroups
-    // (21.75 KiB of LDS each, with 16-bit queue entries and records) fit a CU.  Fewer rows were
-    // measured slower: with 11 KiB of rows eight workgroups fit, but more cells overflow the
-    // rows and take the serial apply.
-    constexpr int NLT = TW >= 4 ? LY::NL : TW == 2 ? LY::NL / 2 : (LY::NL * 13) / 24;
+    // set (24 KiB); 2-wave ones 12 KiB and 1-wave ones 10.5 KiB, so that eight 1-wave workgroups
+    // (19.9 KiB of LDS each, with 16-bit queue entries and records) fit a CU, as many as the
+    // registers allow.  A cell whose rows run out moves them to the pool (evict_rows).
+    constexpr int NLT = TW >= 4 ? LY::NL : TW == 2 ? LY::NL / 2 : (LY::NL * 7) / 16;
     static_assert(QCAP >= LY::LB, "a slot's non-zero components must fit one round's queue");
     static_assert(LY::LB <= 256 && KB <= 256, "queue entries pack the slot and the local bin in 16 bits");
     static_assert(LY::NL * LY::LB <= 65536, "record addresses are 16-bit LDS payload offsets");
