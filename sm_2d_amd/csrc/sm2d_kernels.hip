@@ -235,7 +235,7 @@ __device__ inline void k1_sums(const Tabs& T, const Psi& buf, uint32_t ns, uint3
 // active, 2 coarse, 0 nothing).
 template <class LY>
 __device__ inline uint8_t k1_decide(const Tp& P, const Psi& buf, uint32_t c, uint32_t ns, uint32_t mb, double W64,
-                                    bool bad, uint32_t ep, uint32_t* maskv) {
+                                    bool bad, uint32_t ep, uint32_t* maskv, uint32_t tag) {
     const int l = lane_id();
     const uint32_t bE = (mb >> 12) & 0xFFF;
     const bool on = __ballot(mb != EMPTY && bE <= (uint32_t)P.b_on) != 0;
@@ -257,7 +257,7 @@ __device__ inline uint8_t k1_decide(const Tp& P, const Psi& buf, uint32_t c, uin
     }
     uint8_t f = 0;
     if (l == 0 && ns) {
-        const uint32_t tag = maskv[c], te = tag >> 2;
+        const uint32_t te = tag >> 2;  // tag: maskv[c], read by the caller before the sums
         const bool prev = (te == ep) ? ((tag & 2u) != 0) : (te == ep - 1u && (tag & 1u));
         const bool m = (on || (prev && off)) && pass;
         maskv[c] = (ep << 2) | (prev ? 2u : 0u) | (m ? 1u : 0u);
@@ -276,12 +276,13 @@ __global__ __launch_bounds__(64) void k_mask(Tabs T, Tp P, Psi in, const uint32_
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
         const uint32_t c = occ[i];
         const uint32_t ns = in.ns[c];
+        const uint32_t tag = maskv[c];
         const uint32_t mb = (l < (int)ns) ? in.bid[(size_t)c * KB + l] : EMPTY;
         const uint32_t mr = (l < (int)ns) ? in.sref[(size_t)c * KB + l] : 0u;
         double W64, E64;
         bool bad, nan;
         k1_sums<LY>(T, in, ns, mb, mr, W64, E64, bad, nan);
-        const uint8_t f = k1_decide<LY>(P, in, c, ns, mb, W64, bad, ep, maskv);
+        const uint8_t f = k1_decide<LY>(P, in, c, ns, mb, W64, bad, ep, maskv, tag);
         if (l == 0) {
             flag[i] = f;
             win[i] = W64;
@@ -2830,14 +2831,16 @@ __global__ __launch_bounds__(256) void k_dest(const uint32_t* __restrict__ list,
 
 // K4: gather the neighbours' buckets into psi_out of one destination (faces 0..3, bucket slots
 // in order), one wavefront per destination.  Workgroup b reads segment b mod NSH of the
-// destination list (gridDim is a multiple of NSH).  The destination's payloads are staged in LDS rows
+// destination list (gridDim is a multiple of NSH).  The direct form (below) adds each block's
+// entries in registers.  The staged form, for a destination whose own table would overflow or
+// whose new rows the pool cannot give: the destination's payloads are staged in LDS rows
 // (existing ones loaded, new ones zeroed), the adds happen in LDS in the serial order, and the
-// new payloads get their pool slots with one reservation when the destination is done.  The
-// next bucket payload is fetched while the current one is added.  COUNT (profiling launches
-// only) also adds the destination's non-zero bucket values to moved (NSH shard counters).
-// KIB: LDS of payload rows per destination (16, 8 or 4 KiB; the tuner's K4 shape).  Fewer rows let
-// more wavefronts share a CU; a destination that needs more rows adds into global payloads, in
-// the same order, with the same results.
+// new payloads get their pool slots with one reservation when the destination is done; the next
+// bucket payload is fetched while the current one is added.  COUNT (profiling launches only)
+// also adds the destination's non-zero bucket values to moved (NSH shard counters).  KIB: LDS of
+// payload rows per destination for the staged form (16, 8 or 4 KiB; the tuner's K4 shape).
+// Fewer rows let more wavefronts share a CU; a destination that needs more rows adds into
+// global payloads, in the same order, with the same results.
 template <class LY, bool COUNT, int KIB>
 __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const uint32_t* __restrict__ dlist,
                                                const uint32_t* __restrict__ seg, uint32_t seg_cap,
@@ -2872,19 +2875,10 @@ __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const ui
             srcs[face] = src;
             pos[face] = src >= 0 ? lpos[src] : 0u;
         }
-        WTabs<LY> W;
-        W.n0 = out.ns[d];
-        W.own0 = W.n0;
-        W.b0 = (l < 32 && (uint32_t)l < W.n0) ? out.bid[(size_t)d * KB + l] : EMPTY;
-        W.r0 = (l < 32 && (uint32_t)l < W.n0) ? out.sref[(size_t)d * KB + l] : 0u;
-        W.b1 = W.b2 = W.b3 = W.b4 = EMPTY;
-        W.r1 = W.r2 = W.r3 = W.r4 = 0u;
-        W.n1 = W.n2 = W.n3 = W.n4 = 0u;
-        W.used = 0;
-        W.cap = (uint32_t)NLG;
-        W.rows = s_rows;
-        W.orig = 0u;
-        W.stage_own(out);
+        // the destination's own psi_out table (lane s < 32: slot s)
+        const uint32_t n0 = out.ns[d];
+        const uint32_t ob = (l < 32 && (uint32_t)l < n0) ? out.bid[(size_t)d * KB + l] : EMPTY;
+        const uint32_t orf = (l < 32 && (uint32_t)l < n0) ? out.sref[(size_t)d * KB + l] : 0u;
         // the four faces' bucket tables (lane s < 32: slot s)
         uint32_t fb[4], fr[4], nf[4];
 #pragma unroll
@@ -2901,98 +2895,197 @@ __global__ __launch_bounds__(64) void k_gather(Tabs T, Psi out, Bkt bk, const ui
                 }
             }
         }
-        const uint32_t ne = nf[0] + nf[1] + nf[2] + nf[3];
-        auto entry = [&](uint32_t e, uint32_t& bid, uint32_t& ref) {
-            int f = 0;
-            while (e >= nf[f]) {
-                e -= nf[f];
-                ++f;
-            }
-            const uint32_t bb = f == 0 ? fb[0] : f == 1 ? fb[1] : f == 2 ? fb[2] : fb[3];
-            const uint32_t rr = f == 0 ? fr[0] : f == 1 ? fr[1] : f == 2 ? fr[2] : fr[3];
-            bid = rlu(bb, (int)e);
-            ref = rlu(rr, (int)e);
-        };
         double dw = 0.0, de = 0.0;
         uint32_t nmoved = 0;  // non-zero bucket values of this destination (profiling only)
-        float v[LY::PL], vn[LY::PL];
-        uint32_t bid = EMPTY, bref = 0;
-        if (ne) {
-            entry(0, bid, bref);
+        // Direct form: the entries grouped by block, groups in order of first appearance.  A block
+        // occurs at most once per bucket table, so a group holds at most four entries (one per
+        // face, faces in order); the group's payload (existing or new) and its entries are read
+        // together and added in registers in the serial order, then written once.  Taken when
+        // the own table has room for every new block (no closest-block fallback) and the pool
+        // has the new rows; otherwise the staged form below.
+        uint64_t rem[4];
 #pragma unroll
-            for (int k = 0; k < LY::PL; ++k) v[k] = bid != EMPTY ? bk.pool[(size_t)bref * LY::LB + l + 64 * k] : 0.0f;
-        }
-        for (uint32_t e = 0; e < ne; ++e) {
-            uint32_t bidn = EMPTY, brefn = 0;
-            if (e + 1 < ne) {
-                entry(e + 1, bidn, brefn);
+        for (int f = 0; f < 4; ++f) rem[f] = __ballot(l < 32 && (uint32_t)l < nf[f] && fb[f] != EMPTY);
+        uint32_t nnew = 0;
+        {
+            uint64_t r2[4] = {rem[0], rem[1], rem[2], rem[3]};
+            while (r2[0] | r2[1] | r2[2] | r2[3]) {
+                const int f0 = r2[0] ? 0 : r2[1] ? 1 : r2[2] ? 2 : 3;
+                const uint32_t fbf = f0 == 0 ? fb[0] : f0 == 1 ? fb[1] : f0 == 2 ? fb[2] : fb[3];
+                const uint64_t rf = f0 == 0 ? r2[0] : f0 == 1 ? r2[1] : f0 == 2 ? r2[2] : r2[3];
+                const uint32_t b = rlu(fbf, __ffsll((unsigned long long)rf) - 1);
 #pragma unroll
-                for (int k = 0; k < LY::PL; ++k)
-                    vn[k] = bidn != EMPTY ? bk.pool[(size_t)brefn * LY::LB + l + 64 * k] : 0.0f;
+                for (int f = 0; f < 4; ++f) r2[f] &= ~__ballot(fb[f] == b);
+                if (!__ballot(l < 32 && (uint32_t)l < n0 && ob == b)) ++nnew;
             }
-            if (bid != EMPTY) {
-                if constexpr (COUNT) {
-#pragma unroll
-                    for (int k = 0; k < LY::PL; ++k) nmoved += (uint32_t)__popcll(__ballot(v[k] > 0.0f));
+        }
+        bool direct = n0 + nnew <= (uint32_t)KB;
+        uint32_t nbase = 0;
+        if (direct && nnew) {
+            if (l == 0) nbase = pool_take(out.pcount, out.pcap, nnew, blockIdx.x);
+            nbase = rlu(nbase, 0);
+            direct = nbase != POOL_FULL;  // the staged form flags the full pool and books the drops
+        }
+        if (direct) {
+            uint32_t nb = n0, own_b = ob, own_r = orf;
+            while (rem[0] | rem[1] | rem[2] | rem[3]) {
+                const int f0 = rem[0] ? 0 : rem[1] ? 1 : rem[2] ? 2 : 3;
+                const uint32_t fbf = f0 == 0 ? fb[0] : f0 == 1 ? fb[1] : f0 == 2 ? fb[2] : fb[3];
+                const uint64_t rf = f0 == 0 ? rem[0] : f0 == 1 ? rem[1] : f0 == 2 ? rem[2] : rem[3];
+                const uint32_t b = rlu(fbf, __ffsll((unsigned long long)rf) - 1);
+                const uint64_t fo = __ballot(l < 32 && (uint32_t)l < n0 && ob == b);
+                uint32_t ref;
+                if (fo) {
+                    ref = rlu(orf, __ffsll((unsigned long long)fo) - 1);
+                } else {
+                    ref = nbase + (nb - n0);
+                    own_b = (uint32_t)l == nb ? b : own_b;
+                    own_r = (uint32_t)l == nb ? ref : own_r;
+                    ++nb;
                 }
-                uint32_t sel;
-                const int os = W.claim(0, bid, sel, out.pool, out.pcount, out.pcap, flags, F_PSI_FULL);
-                if (os < 0) {
-                    double pw = 0.0, pe = 0.0;
+                float acc[LY::PL], v[4][LY::PL];
+#pragma unroll
+                for (int k = 0; k < LY::PL; ++k) acc[k] = 0.0f;
+                if (fo) {
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k) acc[k] = out.pool[(size_t)ref * LY::LB + l + 64 * k];
+                }
+                uint32_t has = 0;
+#pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    const uint64_t gm = rem[f] & __ballot(fb[f] == b);
+                    rem[f] &= ~gm;
+#pragma unroll
+                    for (int k = 0; k < LY::PL; ++k) v[f][k] = 0.0f;
+                    if (gm) {
+                        has |= 1u << f;
+                        const uint32_t br = rlu(fr[f], __ffsll((unsigned long long)gm) - 1);
+#pragma unroll
+                        for (int k = 0; k < LY::PL; ++k) v[f][k] = bk.pool[(size_t)br * LY::LB + l + 64 * k];
+                    }
+                }
+#pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    if (!((has >> f) & 1u)) continue;
 #pragma unroll
                     for (int k = 0; k < LY::PL; ++k) {
-                        if (v[k] > 0.0f) {
-                            int lidx = l + 64 * k;
-                            int eb = min(max((int)((bid >> 12) & 0xFFF) * LY::EL + LY::le(lidx), 0), T.NE - 1);
-                            pw += (double)v[k];
-                            pe += (double)ecenter(T.ee, eb) * (double)v[k];
-                        }
+                        if constexpr (COUNT) nmoved += (uint32_t)__popcll(__ballot(v[f][k] > 0.0f));
+                        if (v[f][k] > 0.0f) acc[k] += v[f][k];
                     }
-                    dw += wave_sum(pw);
-                    de += wave_sum(pe);
-                } else {
-                    const uint32_t ref = rlu(W.r0, os);
-                    if (sel == bid) {
+                }
 #pragma unroll
-                        for (int k = 0; k < LY::PL; ++k)
-                            if (v[k] > 0.0f) W.add(ref, l + 64 * k, v[k], out.pool);
-                    } else {
-                        // closest-block fallback: remapped bins may coincide, add in lidx order
+                for (int k = 0; k < LY::PL; ++k) out.pool[(size_t)ref * LY::LB + l + 64 * k] = acc[k];
+            }
+            if (l < 32 && (uint32_t)l >= n0 && (uint32_t)l < nb) {
+                out.bid[(size_t)d * KB + l] = own_b;
+                out.sref[(size_t)d * KB + l] = own_r;
+            }
+            if (l == 0 && nb != n0) out.ns[d] = nb;
+        } else {
+            WTabs<LY> W;
+            W.n0 = n0;
+            W.own0 = n0;
+            W.b0 = ob;
+            W.r0 = orf;
+            W.b1 = W.b2 = W.b3 = W.b4 = EMPTY;
+            W.r1 = W.r2 = W.r3 = W.r4 = 0u;
+            W.n1 = W.n2 = W.n3 = W.n4 = 0u;
+            W.used = 0;
+            W.cap = (uint32_t)NLG;
+            W.rows = s_rows;
+            W.orig = 0u;
+            W.stage_own(out);
+            const uint32_t ne = nf[0] + nf[1] + nf[2] + nf[3];
+            auto entry = [&](uint32_t e, uint32_t& bid, uint32_t& ref) {
+                int f = 0;
+                while (e >= nf[f]) {
+                    e -= nf[f];
+                    ++f;
+                }
+                const uint32_t bb = f == 0 ? fb[0] : f == 1 ? fb[1] : f == 2 ? fb[2] : fb[3];
+                const uint32_t rr = f == 0 ? fr[0] : f == 1 ? fr[1] : f == 2 ? fr[2] : fr[3];
+                bid = rlu(bb, (int)e);
+                ref = rlu(rr, (int)e);
+            };
+            float v[LY::PL], vn[LY::PL];
+            uint32_t bid = EMPTY, bref = 0;
+            if (ne) {
+                entry(0, bid, bref);
+    #pragma unroll
+                for (int k = 0; k < LY::PL; ++k) v[k] = bid != EMPTY ? bk.pool[(size_t)bref * LY::LB + l + 64 * k] : 0.0f;
+            }
+            for (uint32_t e = 0; e < ne; ++e) {
+                uint32_t bidn = EMPTY, brefn = 0;
+                if (e + 1 < ne) {
+                    entry(e + 1, bidn, brefn);
+    #pragma unroll
+                    for (int k = 0; k < LY::PL; ++k)
+                        vn[k] = bidn != EMPTY ? bk.pool[(size_t)brefn * LY::LB + l + 64 * k] : 0.0f;
+                }
+                if (bid != EMPTY) {
+                    if constexpr (COUNT) {
+    #pragma unroll
+                        for (int k = 0; k < LY::PL; ++k) nmoved += (uint32_t)__popcll(__ballot(v[k] > 0.0f));
+                    }
+                    uint32_t sel;
+                    const int os = W.claim(0, bid, sel, out.pool, out.pcount, out.pcap, flags, F_PSI_FULL);
+                    if (os < 0) {
+                        double pw = 0.0, pe = 0.0;
+    #pragma unroll
                         for (int k = 0; k < LY::PL; ++k) {
-                            for (int j = 0; j < 64; ++j) {
-                                float w = rlf(v[k], j);
-                                if (w > 0.0f && l == 0) W.add(ref, remap<LY>(j + 64 * k, bid, sel), w, out.pool);
+                            if (v[k] > 0.0f) {
+                                int lidx = l + 64 * k;
+                                int eb = min(max((int)((bid >> 12) & 0xFFF) * LY::EL + LY::le(lidx), 0), T.NE - 1);
+                                pw += (double)v[k];
+                                pe += (double)ecenter(T.ee, eb) * (double)v[k];
+                            }
+                        }
+                        dw += wave_sum(pw);
+                        de += wave_sum(pe);
+                    } else {
+                        const uint32_t ref = rlu(W.r0, os);
+                        if (sel == bid) {
+    #pragma unroll
+                            for (int k = 0; k < LY::PL; ++k)
+                                if (v[k] > 0.0f) W.add(ref, l + 64 * k, v[k], out.pool);
+                        } else {
+                            // closest-block fallback: remapped bins may coincide, add in lidx order
+                            for (int k = 0; k < LY::PL; ++k) {
+                                for (int j = 0; j < 64; ++j) {
+                                    float w = rlf(v[k], j);
+                                    if (w > 0.0f && l == 0) W.add(ref, remap<LY>(j + 64 * k, bid, sel), w, out.pool);
+                                }
                             }
                         }
                     }
                 }
+                bid = bidn;
+                bref = brefn;
+    #pragma unroll
+                for (int k = 0; k < LY::PL; ++k) v[k] = vn[k];
             }
-            bid = bidn;
-            bref = brefn;
-#pragma unroll
-            for (int k = 0; k < LY::PL; ++k) v[k] = vn[k];
-        }
-        // staged payloads back to the pool: existing rows in place, new rows with one reservation
-        const uint64_t mp = __ballot(l < 32 && (uint32_t)l < W.own0 && (W.r0 & CACHED));
-        rows_to_pool<LY>(mp, W.r0, W.orig, W.rows, out.pool);
-        const uint64_t mo = __ballot(l < 32 && (uint32_t)l >= W.own0 && (uint32_t)l < W.n0 && (W.r0 & CACHED));
-        const uint32_t co = (uint32_t)__popcll(mo);
-        uint32_t base = 0;
-        if (co) {
-            if (l == 0) base = pool_take(out.pcount, out.pcap, co, blockIdx.x);
-            base = rlu(base, 0);
-        }
-        const bool ok = !co || base != POOL_FULL;
-        if (!ok && l == 0) atomicOr(&flags[F_PSI_FULL], 1u);
-        if (ok) {
-            uint32_t ref = W.r0;
-            if ((mo >> l) & 1ull) ref = base + (uint32_t)__popcll(mo & below);
-            rows_to_pool<LY>(mo, W.r0, ref, W.rows, out.pool);
-            if (l < 32 && (uint32_t)l >= W.own0 && (uint32_t)l < W.n0) {
-                out.bid[(size_t)d * KB + l] = W.b0;
-                out.sref[(size_t)d * KB + l] = ref;
+            // staged payloads back to the pool: existing rows in place, new rows with one reservation
+            const uint64_t mp = __ballot(l < 32 && (uint32_t)l < W.own0 && (W.r0 & CACHED));
+            rows_to_pool<LY>(mp, W.r0, W.orig, W.rows, out.pool);
+            const uint64_t mo = __ballot(l < 32 && (uint32_t)l >= W.own0 && (uint32_t)l < W.n0 && (W.r0 & CACHED));
+            const uint32_t co = (uint32_t)__popcll(mo);
+            uint32_t base = 0;
+            if (co) {
+                if (l == 0) base = pool_take(out.pcount, out.pcap, co, blockIdx.x);
+                base = rlu(base, 0);
             }
-            if (l == 0) out.ns[d] = W.n0;
+            const bool ok = !co || base != POOL_FULL;
+            if (!ok && l == 0) atomicOr(&flags[F_PSI_FULL], 1u);
+            if (ok) {
+                uint32_t ref = W.r0;
+                if ((mo >> l) & 1ull) ref = base + (uint32_t)__popcll(mo & below);
+                rows_to_pool<LY>(mo, W.r0, ref, W.rows, out.pool);
+                if (l < 32 && (uint32_t)l >= W.own0 && (uint32_t)l < W.n0) {
+                    out.bid[(size_t)d * KB + l] = W.b0;
+                    out.sref[(size_t)d * KB + l] = ref;
+                }
+                if (l == 0) out.ns[d] = W.n0;
+            }
         }
         if (l == 0) {
             k4drop_w[d] += dw;
@@ -3039,6 +3132,9 @@ __global__ __launch_bounds__(64) void k_audit_cells(Tabs T, Psi out, const uint3
         if (i < n_out) {
             const uint32_t c = occ_out[i];
             const uint32_t ns = out.ns[c];
+            // everything else the cell's entry reads, fetched with its slot headers
+            const uint32_t tag = maskv ? maskv[c] : 0u;
+            const double dk4w = k4w ? k4w[c] : 0.0, dk4e = k4w ? k4e[c] : 0.0;
             const uint32_t mb = (l < (int)ns) ? out.bid[(size_t)c * KB + l] : EMPTY;
             const uint32_t mr = (l < (int)ns) ? out.sref[(size_t)c * KB + l] : 0u;
             double W, E;
@@ -3048,9 +3144,9 @@ __global__ __launch_bounds__(64) void k_audit_cells(Tabs T, Psi out, const uint3
             r[R_WOUT] = nan ? __builtin_nan("") : W;
             r[R_EOUT] = nan ? __builtin_nan("") : E;
             if (k4w && l == 0) {
-                r[R_K4W] = k4w[c];
-                r[R_K4E] = k4e[c];
-                if (r[R_K4W] != 0.0 || r[R_K4E] != 0.0) {
+                r[R_K4W] = dk4w;
+                r[R_K4E] = dk4e;
+                if (dk4w != 0.0 || dk4e != 0.0) {
                     k4w[c] = 0.0;
                     k4e[c] = 0.0;
                 }
@@ -3059,7 +3155,7 @@ __global__ __launch_bounds__(64) void k_audit_cells(Tabs T, Psi out, const uint3
             // list flag and input sums are those the next iteration's K1 would compute (entry i is
             // read above before it is replaced here, by the same wavefront)
             if (maskv) {
-                const uint8_t f = k1_decide<LY>(P, out, c, ns, mb, W, bad, ep1, maskv);
+                const uint8_t f = k1_decide<LY>(P, out, c, ns, mb, W, bad, ep1, maskv, tag);
                 if (l == 0) {
                     flag[i] = f;
                     win[i] = W;
