@@ -323,7 +323,9 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t* tcount, const uint32_t*
                                                uint32_t* ctl, uint32_t* total1, uint32_t cap, uint32_t* flags,
                                                uint32_t skip) {
     if (ctl[C_DONE]) {
-        if (K == 2 && threadIdx.x == 0) {
+        // an ended transport has empty lists; a paused one keeps the counts of the iteration that
+        // paused it, which size the launches of the chunk queued next
+        if (K == 2 && threadIdx.x == 0 && ctl[C_DONE] == 1) {
             ctl[C_NACT] = 0;
             ctl[C_NCRS] = 0;
         }
@@ -521,7 +523,7 @@ __device__ inline void compact_lists(const ListOut& L, uint32_t n, uint32_t ep, 
 // chunk, and lists too long for the closing audit kernel's workgroup.
 __global__ __launch_bounds__(1024) void k_compact_small(ListOut L, const uint32_t* nocc, uint32_t* ctl) {
     if (ctl[C_DONE]) {
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0 && ctl[C_DONE] == 1) {  // (a pause keeps the counts, as k_scan<2>)
             ctl[C_NACT] = 0;
             ctl[C_NCRS] = 0;
         }

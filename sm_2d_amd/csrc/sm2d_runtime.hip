@@ -384,11 +384,12 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
         double* acc = fine ? c->acc_f.p : c->acc_c.p;
         const int e0 = fine ? 6 : 2;
         const int tw = Tuner::tw(c->cand[fine]);
-        // launch sizes of the phase from its own list hint (a phase with a short or empty list,
-        // such as the fine phase while no component has slowed down, launches small grids; every
-        // kernel strides over its actual list)
-        const uint32_t hp = pow2_at_least(std::max<uint32_t>(c->hint_ph[fine], 64));
-        const int gb = blocks_for(hp, 1, 65536);
+        // launch sizes from the longer list's hint, for both phases: a list can grow within a
+        // chunk (components slowing into the fine phase), and a grid sized from the last chunk's
+        // short fine list would stride over the grown list with too few workgroups; surplus
+        // workgroups return at once.  A phase whose list was empty is left out of the chunk instead
+        // (skip).
+        const int gb = blocks_for(h, 1, 65536);
         // timed launches take their events from the dispatch itself (the kernel's own start and
         // end), so stage times and the tuner see the kernel without marker packets around it
         hipEvent_t e_start = c->rec ? c->ev[(size_t)c->ev_it * sm2d_ctx::EPI + e0] : nullptr;
@@ -430,11 +431,11 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
                                                                   c->ctl.p, c->flags.p);
         uint32_t* seg = c->ctl.p + C_DSEG + (fine ? NSH * SH_STRIDE : 0);
         const uint32_t seg_cap = c->dcap / NSH;
-        k_dest<<<blocks_for(4ull * hp, 256, 4096), 256, 0, c->stream>>>(list, nl, c->g.Nx, c->g.Nz, c->Ns, c->bn.p,
+        k_dest<<<blocks_for(4ull * h, 256, 4096), 256, 0, c->stream>>>(list, nl, c->g.Nx, c->g.Nz, c->Ns, c->bn.p,
                                                                        c->dtag.p, c->ctl.p, fine, c->dlist.p, seg,
                                                                        seg_cap, c->bitmap.p, c->flags.p);
         uint32_t* moved = c->prof ? c->gmoved.p + (size_t)(2 * c->ev_it + fine) * NSH * SH_STRIDE : nullptr;
-        const unsigned gg = (blocks_for(4ull * hp, 1, 65536) + NSH - 1) / NSH * NSH;
+        const unsigned gg = (blocks_for(4ull * h, 1, 65536) + NSH - 1) / NSH * NSH;
         const int kib = Tuner::kib(c->cand[2 + fine]);
 #define SM2D_GATHER(K)                                                                                        \
     do {                                                                                                      \
