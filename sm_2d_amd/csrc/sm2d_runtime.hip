@@ -92,8 +92,13 @@ struct Tuner {
         return b;
     }
     // The fastest shape; the default when it is within 5 % of the fastest, so that near-ties do
-    // not flip with timing noise from one process to the next.
+    // not flip with timing noise from one process to the next.  Transport lists of 2048 cells or
+    // more take one wave per cell: eight cells in flight per CU against one for four waves, and
+    // a class's first chunk can mislead (a fine list that has just appeared is cheap per cell in
+    // the iterations the tuner times, then grows and becomes the costliest phase; measured on the
+    // 1029-bin batch, four waves there took 2.7 ms per launch against 1.9 ms for one).
     int best(int kind, int b) const {
+        if (kind < 2 && b >= 11) return 0;
         const int d = dflt(kind);
         int bc = d;
         double bt = 1e300;
