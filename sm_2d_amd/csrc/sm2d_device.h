@@ -91,7 +91,8 @@ constexpr float SURR_DRIFT_TOL = 0.5f;
 struct Tp {
     float E_on, E_off, wmin, step_coarse;
     int b_on, b_off, rule, strag, nuc;
-    int surr;  // fine transport takes the straggling, scattering and attenuation from the table
+    int surr;    // fine transport takes the straggling, scattering and attenuation from the table
+    float R_on;  // water range at max(E_on, lowest edge) (lookup_R_tab, computed on the host)
 };
 
 // Payload pools are bump allocators split into NSH shards of pcap / NSH rows, each counted on a
@@ -175,7 +176,7 @@ __device__ inline double wave_sum(double v) {  // fixed butterfly: deterministic
 // device_find_bin_edges (src/cuda/device/device_lut.cuh:34-49).  With a guess g >= 0 the bins
 // g, g-1, g-2 are tried first (strictly increasing edges: a bin that brackets E is the binary
 // search's answer), so lookups near a known bin need no search; the result never depends on g.
-__device__ inline int find_bin(const float* __restrict__ e, int NE, float E, int g = -1) {
+SM2D_HD inline int find_bin(const float* __restrict__ e, int NE, float E, int g = -1) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const int b = g - k;
@@ -194,14 +195,19 @@ __device__ inline float ecenter(const float* __restrict__ e, int b) {
     float lo = e[b], hi = e[b + 1];
     return lo + 1.00f * ((hi - lo) * 0.5f);
 }
-// device_lookup_R (device_lut.cuh:66-85); g: bin guess (see find_bin)
-__device__ inline float lookup_R(const Tabs& T, float E, int g = -1) {
-    const float Emin = T.ee[0], Emax = T.ee[T.NE];
+// device_lookup_R (device_lut.cuh:66-85) over the tables as arrays; g: bin guess (see find_bin).
+// Host and device: the transport's launch constant R_on is computed on the host with it.
+SM2D_HD inline float lookup_R_tab(const float* ee, const float* R, const float* lR, const float* lE, int NE, float E,
+                                  int g = -1) {
+    const float Emin = ee[0], Emax = ee[NE];
     float Ec = fmaxf(Emin, fminf(E, Emax));
-    int b = find_bin(T.ee, T.NE, Ec, g);
-    if (b == T.NE - 1 || Ec >= Emax) return T.R[T.NE - 1];
+    int b = find_bin(ee, NE, Ec, g);
+    if (b == NE - 1 || Ec >= Emax) return R[NE - 1];
     float lv = logf_(Ec);
-    return expf_(T.lR[b] + (T.lR[b + 1] - T.lR[b]) * (lv - T.lE[b]) / (T.lE[b + 1] - T.lE[b]));
+    return expf_(lR[b] + (lR[b + 1] - lR[b]) * (lv - lE[b]) / (lE[b + 1] - lE[b]));
+}
+__device__ inline float lookup_R(const Tabs& T, float E, int g = -1) {
+    return lookup_R_tab(T.ee, T.R, T.lR, T.lE, T.NE, E, g);
 }
 // device_lookup_E_inverse (device_lut.cuh:110-138): lower_bound bin, no upper clamp.  With a
 // guess g whose R[g] >= Rin, the lower bound is found walking down from g (at most 3 steps, R

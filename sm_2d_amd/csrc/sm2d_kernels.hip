@@ -2446,7 +2446,7 @@ __global__ __launch_bounds__(64 * TW) __attribute__((amdgpu_waves_per_eu(2))) vo
     G.hdz = T.dz * 0.5f;
     G.tmin = T.te[0];
     G.dth = (T.te[T.NT] - G.tmin) / T.NT;
-    G.R_on = lookup_R(T, fmaxf(P.E_on, T.ee[0]));
+    G.R_on = P.R_on;
     // The slot headers of a workgroup's next cell (psi_in and psi_out block ids, payload
     // references and slot counts; lanes < 32 of each wave) are fetched while the current cell is
     // transported.  Only this cell's workgroup writes its psi_out during the launch.
@@ -2786,7 +2786,7 @@ __global__ __launch_bounds__(64) void k_foreign(Tabs T, Tp P, Psi in, Bkt bk, co
     G.hdz = T.dz * 0.5f;
     G.tmin = T.te[0];
     G.dth = (T.te[T.NT] - G.tmin) / T.NT;
-    G.R_on = lookup_R(T, fmaxf(P.E_on, T.ee[0]));
+    G.R_on = P.R_on;
     apply_foreign<LY, FINE, INFER>(T, P, G, in, bk, list, ltag, tag, nf, flags);
     if (threadIdx.x == 0) ctl[C_FNT] = 0u;
 }

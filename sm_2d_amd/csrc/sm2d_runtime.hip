@@ -124,7 +124,7 @@ struct sm2d_ctx {
     uint64_t skip_chunks = 0, pauses = 0;  // graph chunks without a phase, and their pauses
     bool tables = false;
     DBuf<float> te, ee, R, Sp, lE, lR, lS, feA, feB, sx0, rho, x0r;
-    std::vector<float> h_ee;
+    std::vector<float> h_ee, h_R, h_lR, h_lE;  // host copies of the energy tables (launch constants)
     DBuf<uint32_t> bid[2], ns[2], sref[2], pcount[2], occ[2];
     DBuf<float> pool[2];
     int in = 0;
@@ -267,6 +267,9 @@ Tp make_tp(const sm2d_ctx* c, const sm2d_transport* t) {
     p.strag = t->enable_straggling;
     p.nuc = t->enable_nuclear;
     p.surr = (c->surr_on && c->surr.p) ? (c->surr_on == 2 && c->mlp_wb ? 2 : 1) : 0;
+    if (!c->h_R.empty())
+        p.R_on = lookup_R_tab(c->h_ee.data(), c->h_R.data(), c->h_lR.data(), c->h_lE.data(), c->g.N_E,
+                              std::max(p.E_on, c->h_ee[0]));
     return p;
 }
 
@@ -1326,6 +1329,9 @@ int sm2d_lut_upload(sm2d_ctx* c, const float* te, const float* ee, const float* 
     HIPCHK(hipMemcpyAsync(c->lR.p, lR, sizeof(float) * NE, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->lS.p, lS, sizeof(float) * NE, hipMemcpyHostToDevice, c->stream));
     c->h_ee.assign(ee, ee + NE + 1);
+    c->h_R.assign(R, R + NE);
+    c->h_lR.assign(lR, lR + NE);
+    c->h_lE.assign(lE, lE + NE);
     HIPCHK(hipStreamSynchronize(c->stream));
     c->tables = true;
     return SM2D_OK;
