@@ -3170,16 +3170,13 @@ __global__ __launch_bounds__(64) void k_audit_cells(Tabs T, Psi out, const uint3
     }
 }
 
-// Stage 1b: per tile of 256 entries, a fixed tree over the entries' values.
-__global__ __launch_bounds__(256) void k_audit_tiles(const double* __restrict__ pe, const uint32_t* nin,
-                                                     const uint32_t* nout, double* part, const uint32_t* ctl,
-                                                     int force) {
-    if (ctl[C_DONE] && !force) return;
-    __shared__ double sh[NR1][256];
+// Stage 1b: per tile of 256 entries, a fixed tree over the entries' values (workgroup b of g).
+__device__ inline void audit_tiles_body(double (*sh)[256], const double* __restrict__ pe, const uint32_t* nin,
+                                        const uint32_t* nout, double* part, uint32_t b, uint32_t g) {
     const uint32_t n_in = nin ? *nin : 0u, n_out = *nout;
     const uint32_t n = max(n_in, n_out);
     const uint32_t ntile = (n + 255) / 256;
-    for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+    for (uint32_t tile = b; tile < ntile; tile += g) {
         const uint32_t i = tile * 256 + threadIdx.x;
         for (int k = 0; k < NR1; ++k) sh[k][threadIdx.x] = i < n ? pe[(size_t)i * NR1 + k] : 0.0;
         __syncthreads();
@@ -3193,21 +3190,27 @@ __global__ __launch_bounds__(256) void k_audit_tiles(const double* __restrict__ 
     }
 }
 
+__global__ __launch_bounds__(256) void k_audit_tiles(const double* __restrict__ pe, const uint32_t* nin,
+                                                     const uint32_t* nout, double* part, const uint32_t* ctl,
+                                                     int force) {
+    if (ctl[C_DONE] && !force) return;
+    __shared__ double sh[NR1][256];
+    audit_tiles_body(sh, pe, nin, nout, part, blockIdx.x, gridDim.x);
+}
+
 // Stage 1c (long lists): per tile of 256 listed cells of a phase, a fixed tree over the cells'
 // accumulators: their NACC sums, then the largest W closure, E closure and learned-width
 // deviation, and the cell-bin updates (in the fine phase; an integer, exact in any order).
 constexpr int NACCP = NACC + 4;
 
-__global__ __launch_bounds__(256) void k_acc_tiles(const double* __restrict__ acc_c, const uint32_t* nc,
-                                                   const double* __restrict__ acc_f, const uint32_t* nf,
-                                                   double* ap, uint32_t tcap, const uint32_t* ctl) {
-    if (ctl[C_DONE]) return;
-    __shared__ double sh[NACCP][256];
+__device__ inline void acc_tiles_body(double (*sh)[256], const double* __restrict__ acc_c, const uint32_t* nc,
+                                      const double* __restrict__ acc_f, const uint32_t* nf, double* ap,
+                                      uint32_t tcap, uint32_t b, uint32_t g) {
     for (int ph = 0; ph < 2; ++ph) {
         const double* acc = ph ? acc_f : acc_c;
         const uint32_t n = ph ? *nf : *nc;
         const uint32_t ntile = (n + 255) / 256;
-        for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        for (uint32_t tile = b; tile < ntile; tile += g) {
             const uint32_t i = tile * 256 + threadIdx.x;
             double v[NACCP];
             for (int k = 0; k < NACCP; ++k) v[k] = 0.0;
@@ -3236,6 +3239,19 @@ __global__ __launch_bounds__(256) void k_acc_tiles(const double* __restrict__ ac
             __syncthreads();
         }
     }
+}
+
+// The audit tiles (workgroups [0, ga)) and the accumulator tiles (the rest) of an iteration with
+// long lists in one launch: the two reductions read different arrays and write different partials.
+__global__ __launch_bounds__(256) void k_tiles(const double* __restrict__ pe, const uint32_t* nin,
+                                               const uint32_t* nout, double* part, uint32_t ga,
+                                               const double* __restrict__ acc_c, const uint32_t* nc,
+                                               const double* __restrict__ acc_f, const uint32_t* nf, double* ap,
+                                               uint32_t tcap, const uint32_t* ctl) {
+    if (ctl[C_DONE]) return;
+    __shared__ double sh[NACCP][256];
+    if (blockIdx.x < ga) audit_tiles_body(sh, pe, nin, nout, part, blockIdx.x, ga);
+    else acc_tiles_body(sh, acc_c, nc, acc_f, nf, ap, tcap, blockIdx.x - ga, gridDim.x - ga);
 }
 
 // Stage 2 (one workgroup): the tiles in a fixed order, then the per-listed-cell accumulators of

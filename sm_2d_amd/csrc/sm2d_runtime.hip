@@ -353,8 +353,9 @@ void launch_mask_compact(sm2d_ctx* c, const Tp& P, int in, int grid_entries, boo
 // per-cell sums for k_audit_final to tile itself (short lists), else null after the tile pass.
 // With P, buffer k's K1 decision for the next iteration is made in the same pass.
 template <class LY>
+// tiles = false: the caller launches the entry tiles itself (with the accumulator tiles, k_tiles).
 const double* launch_audit_cells(sm2d_ctx* c, int k, const uint32_t* nin, bool k4, int force, int grid_entries,
-                                 const Tp* P = nullptr) {
+                                 const Tp* P = nullptr, bool tiles = true) {
     const int ge = blocks_for(grid_entries, 1, 1 << 16);
     const int gt = blocks_for(grid_entries, 256, 4096);
     k_audit_cells<LY><<<ge, 64, 0, c->stream>>>(c->tabs(), c->psi(k), nin, c->win.p, c->ein.p, c->occ[k].p,
@@ -362,7 +363,7 @@ const double* launch_audit_cells(sm2d_ctx* c, int k, const uint32_t* nin, bool k
                                                 c->pe.p, c->ctl.p, force, P ? *P : Tp{}, P ? c->maskv.p : nullptr,
                                                 c->flag.p);
     if ((uint32_t)grid_entries <= SMALL_AUDIT) return c->pe.p;
-    k_audit_tiles<<<gt, 256, 0, c->stream>>>(c->pe.p, nin, c->nocc(k), c->part.p, c->ctl.p, force);
+    if (tiles) k_audit_tiles<<<gt, 256, 0, c->stream>>>(c->pe.p, nin, c->nocc(k), c->part.p, c->ctl.p, force);
     return nullptr;
 }
 
@@ -466,17 +467,21 @@ void launch_iteration(sm2d_ctx* c, const Tp& P, bool opener, bool open_next, boo
     }
     build_occ(c, out);
     c->mark(10);
-    const double* pe = launch_audit_cells<LY>(c, out, c->nocc(in), true, 0, (int)hocc, open_next ? &P : nullptr);
+    // long lists: the listed cells' accumulators in tiles too, so the closing workgroup reads tile
+    // partials instead of every cell's; with a long occupied list as well, one launch does the
+    // entry tiles and the accumulator tiles (k_tiles)
+    const bool acc_tiles = h > SMALL_AUDIT, audit_tiles = hocc > SMALL_AUDIT;
+    const double* pe = launch_audit_cells<LY>(c, out, c->nocc(in), true, 0, (int)hocc, open_next ? &P : nullptr,
+                                              !acc_tiles);
     ListOut L = list_out(c, out);
     L.skip = skip;
     if (!(open_next && compact_next)) L.occ = nullptr;
-    // long lists: the listed cells' accumulators in tiles first, so the closing workgroup reads
-    // tile partials instead of every cell's
-    const bool acc_tiles = h > SMALL_AUDIT;
-    if (acc_tiles)
-        k_acc_tiles<<<blocks_for(h, 256, 4096), 256, 0, c->stream>>>(c->acc_c.p, c->ctl.p + C_NCRS, c->acc_f.p,
-                                                                     c->ctl.p + C_NACT, c->accpart.p, acc_tcap(c),
-                                                                     c->ctl.p);
+    if (acc_tiles) {
+        const uint32_t ga = audit_tiles ? (uint32_t)blocks_for(hocc, 256, 4096) : 0u;
+        k_tiles<<<ga + blocks_for(h, 256, 4096), 256, 0, c->stream>>>(
+            c->pe.p, c->nocc(in), c->nocc(out), c->part.p, ga, c->acc_c.p, c->ctl.p + C_NCRS, c->acc_f.p,
+            c->ctl.p + C_NACT, c->accpart.p, acc_tcap(c), c->ctl.p);
+    }
     k_audit_final<<<open_next ? 1 + gm : 1, 256, 0, c->stream>>>(
         c->part.p, c->nocc(in), c->nocc(out), c->acc_c.p, c->ctl.p + C_NCRS, c->acc_f.p, c->ctl.p + C_NACT, nullptr,
         c->hist.p, HROWS, c->ctl.p, 0, pe, c->psi(in), c->occ[in].p, c->nocc(in), L, P.surr,
