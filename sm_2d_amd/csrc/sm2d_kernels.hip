@@ -331,30 +331,43 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t* tcount, const uint32_t*
         }
         return;
     }
-    __shared__ uint32_t s[K][1024];
+    // exclusive scan of 1024 tile counts per pass: wave scans by shuffles, then the 16 wave totals
+    __shared__ uint32_t wsum[K][16];
     __shared__ uint32_t carry[K];
     const int ntile = (int)((*n_items + per_tile - 1) / per_tile);
+    const int w = (int)(threadIdx.x >> 6), l = lane_id();
     if (threadIdx.x < K) carry[threadIdx.x] = 0;
     __syncthreads();
     for (int base = 0; base < ntile; base += 1024) {
         const int i = base + threadIdx.x;
-        uint32_t own[K];
+        uint32_t own[K], inc[K];
         for (int k = 0; k < K; ++k) {
             own[k] = (i < ntile) ? tcount[i * K + k] : 0;
-            s[k][threadIdx.x] = own[k];
+            inc[k] = own[k];
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = (uint32_t)__shfl_up((int)inc[k], o);
+                if (l >= o) inc[k] += y;
+            }
+            if (l == 63) wsum[k][w] = inc[k];
         }
         __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            uint32_t a[K];
-            for (int k = 0; k < K; ++k) a[k] = threadIdx.x >= o ? s[k][threadIdx.x - o] : 0;
-            __syncthreads();
-            for (int k = 0; k < K; ++k) s[k][threadIdx.x] += a[k];
-            __syncthreads();
+        if (w == 0) {
+            for (int k = 0; k < K; ++k) {
+                uint32_t v = l < 16 ? wsum[k][l] : 0u;  // inclusive scan of the wave totals
+                for (int o = 1; o < 16; o <<= 1) {
+                    const uint32_t y = (uint32_t)__shfl_up((int)v, o);
+                    if (l >= o) v += y;
+                }
+                if (l < 16) wsum[k][l] = v;
+            }
         }
-        for (int k = 0; k < K; ++k)
-            if (i < ntile) tcount[i * K + k] = carry[k] + s[k][threadIdx.x] - own[k];
         __syncthreads();
-        if (threadIdx.x < K) carry[threadIdx.x] += s[threadIdx.x][1023];
+        for (int k = 0; k < K; ++k) {
+            const uint32_t before = (w > 0 ? wsum[k][w - 1] : 0u) + inc[k] - own[k];
+            if (i < ntile) tcount[i * K + k] = carry[k] + before;
+        }
+        __syncthreads();
+        if (threadIdx.x < K) carry[threadIdx.x] += wsum[threadIdx.x][15];
         __syncthreads();
     }
     if (threadIdx.x == 0) {
