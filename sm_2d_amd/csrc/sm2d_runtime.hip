@@ -97,8 +97,9 @@ struct Tuner {
     // a class's first chunk can mislead (a fine list that has just appeared is cheap per cell in
     // the iterations the tuner times, then grows and becomes the costliest phase; measured on the
     // 1029-bin batch, four waves there took 2.7 ms per launch against 1.9 ms for one).
+    static constexpr int ONE_WAVE = 11;  // size class of 2048 listed cells
     int best(int kind, int b) const {
-        if (kind < 2 && b >= 11) return 0;
+        if (kind < 2 && b >= ONE_WAVE) return 0;
         const int d = dflt(kind);
         int bc = d;
         double bt = 1e300;
@@ -702,6 +703,7 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
                 return c->tune.forced_kib == 16 ? 0 : c->tune.forced_kib == 8 ? 1 : 2;
             if (c->tune.forced) return kind < 2 ? (c->tune.forced == 1 ? 0 : c->tune.forced == 2 ? 1 : 2) : 0;
             if (bkt[ph] == 0) return kind < 2 ? 0 : Tuner::dflt(kind);
+            if (kind < 2 && bkt[ph] >= Tuner::ONE_WAVE) return 0;  // long lists: one wave per cell
             if (explore[ph]) return r % Tuner::NC;
             return c->tune.best(kind, bkt[ph]);
         };
@@ -799,6 +801,7 @@ int run_iters(sm2d_ctx* c, const sm2d_transport* tp, int n_iter, int src_terms, 
                     c->tune.ms[kind][bkt[ph]][cd] += ms;
                     c->tune.cnt[kind][bkt[ph]][cd] += 1;
                 }
+                if (kind < 2 && bkt[ph] >= Tuner::ONE_WAVE) continue;  // not explored (best() is one wave)
                 for (int cd = 0; cd < Tuner::NC; ++cd) all = all && c->tune.cnt[kind][bkt[ph]][cd] >= 2;
             }
             if (all) c->tune.done[ph][bkt[ph]] = true;
