@@ -611,38 +611,42 @@ __global__ __launch_bounds__(1024) void k_occ_small(uint32_t* bitmap, uint32_t n
 __global__ __launch_bounds__(256) void k_occ_count(const uint32_t* __restrict__ bitmap, uint32_t nwords,
                                                    uint32_t* tcount, const uint32_t* ctl) {
     if (ctl[C_DONE]) return;
-    __shared__ uint32_t s;
+    __shared__ uint32_t wsum[4];
     const uint32_t ntile = (nwords + 255) / 256;
+    const int w = (int)(threadIdx.x >> 6), l = lane_id();
     for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
-        if (threadIdx.x == 0) s = 0;
-        __syncthreads();
         const uint32_t i = tile * 256 + threadIdx.x;
-        const uint32_t b = i < nwords ? bitmap[i] : 0u;
-        if (b) atomicAdd(&s, (uint32_t)__popc(b));
+        uint32_t c = i < nwords ? (uint32_t)__popc(bitmap[i]) : 0u;
+        for (int o = 32; o > 0; o >>= 1) c += (uint32_t)__shfl_xor((int)c, o);
+        if (l == 0) wsum[w] = c;
         __syncthreads();
-        if (threadIdx.x == 0) tcount[tile] = s;
+        if (threadIdx.x == 0) tcount[tile] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
         __syncthreads();
     }
 }
 
-// ... and the ordered emission (ascending cells), clearing the bitmap as it goes.
+// ... and the ordered emission (ascending cells), clearing the bitmap as it goes: positions from
+// a wave scan of the words' popcounts and the waves' totals.
 __global__ __launch_bounds__(256) void k_occ_emit(uint32_t* bitmap, uint32_t nwords, const uint32_t* toff,
                                                   uint32_t* occ, uint32_t cap, const uint32_t* ctl) {
     if (ctl[C_DONE]) return;
-    __shared__ uint32_t sc[256];
+    __shared__ uint32_t wsum[4];
     const uint32_t ntile = (nwords + 255) / 256;
+    const int w = (int)(threadIdx.x >> 6), l = lane_id();
     for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
         const uint32_t i = tile * 256 + threadIdx.x;
         uint32_t b = i < nwords ? bitmap[i] : 0u;
-        sc[threadIdx.x] = __popc(b);
-        __syncthreads();
-        for (int o = 1; o < 256; o <<= 1) {
-            uint32_t a = threadIdx.x >= (uint32_t)o ? sc[threadIdx.x - o] : 0;
-            __syncthreads();
-            sc[threadIdx.x] += a;
-            __syncthreads();
+        const uint32_t c = (uint32_t)__popc(b);
+        uint32_t inc = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)inc, o);
+            if (l >= o) inc += y;
         }
-        uint32_t pos = toff[tile] + sc[threadIdx.x] - __popc(b);
+        if (l == 63) wsum[w] = inc;
+        __syncthreads();
+        uint32_t before = 0;
+        for (int k = 0; k < w; ++k) before += wsum[k];
+        uint32_t pos = toff[tile] + before + inc - c;
         if (b) {
             bitmap[i] = 0u;
             while (b) {
