@@ -308,10 +308,12 @@ int sm2d_swap(sm2d_ctx* ctx);
 
 /* Transport launch shape: waves per listed cell (1, 2 or 4), or 0 for the built-in tuner, which
  * times the three shapes per phase and launch size class on the first chunk of iterations of
- * that class and then uses the fastest (no counterpart in the reference; every shape gives the
- * same results).  The environment variable SM2D_TRANSPORT_WAVES sets the initial value. */
+ * that class and then uses the fastest; lists of 2048 cells or more take one wave per cell
+ * untimed (no counterpart in the reference; every shape gives the same results).  The
+ * environment variable SM2D_TRANSPORT_WAVES sets the initial value. */
 int sm2d_set_transport_waves(sm2d_ctx* ctx, int waves);
-/* K4 gather launch shape: LDS payload rows per destination (16, 8 or 4 KiB; fewer rows let more
+/* K4 gather launch shape: LDS payload rows per destination for its staged form (16, 8 or 4 KiB;
+ * the staged form serves destinations whose own slot table would overflow; fewer rows let more
  * wavefronts share a compute unit, a destination needing more adds into global payloads), or 0
  * for the tuner, which times the three with the transport shapes (replaces the batch sizing role
  * of the reference's FineBatchPlanner, src/perf/fine_batch_planner.cpp:7-36; every shape gives
